@@ -1,0 +1,353 @@
+"""Synthetic scenes for the frame step, as host arrays of body parameters.
+
+A ``Scene`` holds what ``BodyStore.add`` / ``ShapeStore.add`` receive in the
+reference (position, rotation wxyz, velocities, mass, 3x3 inertia, shape
+handle, flags; shape type and half extents), plus the run parameters the
+driver uses.  The same arrays feed the HIP path and the CPU oracle, so both
+see bit-identical inputs.
+
+Scenes:
+  * ``stress_blocks``: examples/stress_test_blocks/main.cpp:72-166 (the
+    1 024-box stack over a static ground, 1.0 m pitch as shipped).
+  * ``determinism_boxes``: core/tests/sim_determinism_test.cpp:248-349 (the
+    8-box golden-hash scene and its 256-box variant).
+  * ``random_mixed``: 2/3 boxes, 1/3 spheres, seeded uniform positions and
+    Shoemake quaternions, density 1000 kg/m^3 (32 768 bodies by default).
+  * ``batched_piles``: independent scenes with lognormal body counts, each with
+    its own ground, stacked in jittered columns.
+
+Randomness: PCG32 (XSH-RR, O'Neill 2014) seeded through splitmix64, so a seed
+gives the same scene on every machine with this numpy.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F32 = np.float32
+SHAPE_BOX = 0
+SHAPE_SPHERE = 1
+FLAG_STATIC = 1
+
+
+# ---------------------------------------------------------------------------
+# PCG32 seeded by splitmix64
+# ---------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+def _splitmix64(state: int) -> tuple[int, int]:
+    state = (state + 0x9E3779B97F4A7C15) & _M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return state, z ^ (z >> 31)
+
+
+class Pcg32:
+    """PCG32 XSH-RR generator; ``seed`` is expanded with splitmix64."""
+
+    _MULT = 6364136223846793005
+
+    def __init__(self, seed: int):
+        sm, s0 = _splitmix64(seed & _M64)
+        _, s1 = _splitmix64(sm)
+        self.inc = ((s1 << 1) | 1) & _M64
+        self.state = 0
+        self.next_u32()
+        self.state = (self.state + s0) & _M64
+        self.next_u32()
+
+    def next_u32(self) -> int:
+        old = self.state
+        self.state = (old * self._MULT + self.inc) & _M64
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self) -> float:
+        """Uniform in [0, 1) with 32 random bits (float64)."""
+        return self.next_u32() * (1.0 / 4294967296.0)
+
+    def uniform_range(self, lo: float, hi: float) -> float:
+        return lo + (hi - lo) * self.uniform()
+
+    def normal(self) -> float:
+        """Standard normal by Box-Muller (one draw per call, float64)."""
+        u1 = 1.0 - self.uniform()
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+# ---------------------------------------------------------------------------
+# Scene container
+# ---------------------------------------------------------------------------
+@dataclass
+class Scene:
+    name: str
+    # shapes
+    shape_type: np.ndarray
+    shape_half: np.ndarray          # (S, 3) float32
+    # bodies
+    pos: np.ndarray                 # (N, 3) float32
+    rot: np.ndarray                 # (N, 4) float32, (w, x, y, z)
+    vel: np.ndarray                 # (N, 3)
+    ang: np.ndarray                 # (N, 3)
+    mass: np.ndarray                # (N,)
+    inertia: np.ndarray             # (N, 9) row-major body-frame inertia
+    shape: np.ndarray               # (N,) uint32
+    flags: np.ndarray               # (N,) uint32
+    bounds: np.ndarray              # (6,) float32 min xyz, max xyz
+    max_pairs: int
+    max_contacts: int
+    dt: float = 1.0 / 60.0
+    n_substeps: int = 1
+    n_iters: int = 10
+    mu: float = 0.5
+    gravity: tuple = (0.0, -9.81, 0.0)
+    frames: int = 300
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def n_shapes(self) -> int:
+        return int(self.shape_type.shape[0])
+
+
+def _alloc(n: int):
+    rot = np.zeros((n, 4), F32)
+    rot[:, 0] = 1.0
+    eye = np.zeros((n, 9), F32)
+    eye[:, [0, 4, 8]] = 1.0
+    return dict(pos=np.zeros((n, 3), F32), rot=rot, vel=np.zeros((n, 3), F32),
+                ang=np.zeros((n, 3), F32), mass=np.ones(n, F32), inertia=eye,
+                shape=np.zeros(n, np.uint32), flags=np.zeros(n, np.uint32))
+
+
+def _diag(v: float) -> np.ndarray:
+    return np.array([v, 0, 0, 0, v, 0, 0, 0, v], F32)
+
+
+# ---------------------------------------------------------------------------
+# examples/stress_test_blocks/main.cpp:72-166
+# ---------------------------------------------------------------------------
+def stress_blocks(nx: int = 8, ny: int = 16, nz: int = 8, pitch: float = 1.0) -> Scene:
+    n_dyn = nx * ny * nz
+    n = n_dyn + 1
+    box_h = F32(0.4)
+    gnd_hy = F32(0.5)
+    gnd_hx = F32(max(20.0, 0.5 * max(nx, nz) + 2.0))
+    b = _alloc(n)
+    m = F32(1.0)
+    i_diag = m / F32(3.0) * (box_h * box_h + box_h * box_h)
+    k = 0
+    p = F32(pitch)
+    for iy in range(ny):
+        for ix in range(nx):
+            for iz in range(nz):
+                b["pos"][k] = (F32(ix - nx // 2) + F32(0.5)) * p, gnd_hy + F32(iy + 1) * p, \
+                              (F32(iz - nz // 2) + F32(0.5)) * p
+                b["inertia"][k] = _diag(i_diag)
+                k += 1
+    # static ground plate (mass 0, identity inertia, Static flag)
+    b["mass"][n_dyn] = 0.0
+    b["shape"][n_dyn] = 1
+    b["flags"][n_dyn] = FLAG_STATIC
+    bound_xz = float(gnd_hx) + 5.0
+    bound_y = max(22.0, float(gnd_hy) + ny + 5.0)
+    return Scene(
+        name=f"stress_blocks_{nx}x{ny}x{nz}" + ("" if pitch == 1.0 else f"_pitch{pitch}"),
+        shape_type=np.array([SHAPE_BOX, SHAPE_BOX], np.uint32),
+        shape_half=np.array([[box_h] * 3, [gnd_hx, gnd_hy, gnd_hx]], F32),
+        bounds=np.array([-bound_xz, -2.0, -bound_xz, bound_xz, bound_y, bound_xz], F32),
+        max_pairs=max(65536, n_dyn * 48), max_contacts=max(16384, n_dyn * 16),
+        frames=300, **b)
+
+
+# ---------------------------------------------------------------------------
+# core/tests/sim_determinism_test.cpp:248-349
+# ---------------------------------------------------------------------------
+def determinism_boxes(nx: int = 2, ny: int = 2, nz: int = 2) -> Scene:
+    n_dyn = nx * ny * nz
+    n = n_dyn + 1
+    box_h = F32(0.4)
+    gnd_hx, gnd_hy = F32(5.0), F32(0.5)
+    b = _alloc(n)
+    m = F32(1.0)
+    i_diag = m / F32(3.0) * F32(2.0) * box_h * box_h
+    k = 0
+    for iy in range(ny):
+        for ix in range(nx):
+            for iz in range(nz):
+                b["pos"][k] = (F32(ix - nx // 2) + F32(0.5)) * F32(1.0), \
+                              gnd_hy + F32(iy + 1) * F32(1.0), \
+                              (F32(iz - nz // 2) + F32(0.5)) * F32(1.0)
+                b["inertia"][k] = _diag(i_diag)
+                k += 1
+    b["mass"][n_dyn] = 0.0
+    b["shape"][n_dyn] = 1
+    b["flags"][n_dyn] = FLAG_STATIC
+    cap = n * 32 + 256
+    return Scene(
+        name=f"determinism_{nx}x{ny}x{nz}",
+        shape_type=np.array([SHAPE_BOX, SHAPE_BOX], np.uint32),
+        shape_half=np.array([[box_h] * 3, [gnd_hx, gnd_hy, gnd_hx]], F32),
+        bounds=np.array([-20, -2, -20, 20, 30, 20], F32),
+        max_pairs=cap, max_contacts=16384, frames=20, **b)
+
+
+# ---------------------------------------------------------------------------
+# Random mixed boxes/spheres in a volume above a ground plate
+# ---------------------------------------------------------------------------
+def _shoemake(rng: Pcg32) -> tuple[float, float, float, float]:
+    u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
+    a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+    t2, t3 = 2.0 * math.pi * u2, 2.0 * math.pi * u3
+    # (w, x, y, z)
+    return b * math.cos(t3), a * math.sin(t2), a * math.cos(t2), b * math.sin(t3)
+
+
+def random_mixed(n_dyn: int = 32768, seed: int = 20260002, scale: float = 1.0,
+                 extent=(30.0, 40.0, 30.0), y0: float = 1.0, frames: int = 600) -> Scene:
+    """Config: 2/3 boxes (half extents U[0.1,0.5]^3), 1/3 spheres (r U[0.1,0.4]),
+    uniform positions in x,z in [-ex/2, ex/2], y in [y0, y0+ey], Shoemake
+    rotations, density 1000 kg/m^3; boxes then spheres, then shuffled; one
+    static ground (half 20 x 0.5 x 20) as the last body.  Every body has its
+    own shape record."""
+    rng = Pcg32(seed)
+    n_box = (2 * n_dyn) // 3
+    kinds = np.array([SHAPE_BOX] * n_box + [SHAPE_SPHERE] * (n_dyn - n_box), np.uint32)
+    # Fisher-Yates shuffle with the same generator.
+    for i in range(n_dyn - 1, 0, -1):
+        j = rng.next_u32() % (i + 1)
+        kinds[i], kinds[j] = kinds[j], kinds[i]
+    n = n_dyn + 1
+    b = _alloc(n)
+    shape_type = np.zeros(n, np.uint32)
+    shape_half = np.zeros((n, 3), F32)
+    ex, ey, ez = extent
+    for k in range(n_dyn):
+        if kinds[k] == SHAPE_BOX:
+            h = [rng.uniform_range(0.1, 0.5) * scale for _ in range(3)]
+            hf = [F32(v) for v in h]
+            m = 8000.0 * float(hf[0]) * float(hf[1]) * float(hf[2])
+            ixx = m / 3.0 * (float(hf[1]) ** 2 + float(hf[2]) ** 2)
+            iyy = m / 3.0 * (float(hf[0]) ** 2 + float(hf[2]) ** 2)
+            izz = m / 3.0 * (float(hf[0]) ** 2 + float(hf[1]) ** 2)
+            shape_type[k] = SHAPE_BOX
+            shape_half[k] = hf
+            b["inertia"][k] = np.array([ixx, 0, 0, 0, iyy, 0, 0, 0, izz], F32)
+        else:
+            r = F32(rng.uniform_range(0.1, 0.4) * scale)
+            m = 4000.0 * math.pi * float(r) ** 3 / 3.0
+            i = 0.4 * m * float(r) ** 2
+            shape_type[k] = SHAPE_SPHERE
+            shape_half[k] = (r, r, r)
+            b["inertia"][k] = _diag(i)
+        b["mass"][k] = m
+        b["pos"][k] = (rng.uniform_range(-ex / 2, ex / 2) * scale,
+                       (y0 + rng.uniform() * ey) * scale,
+                       rng.uniform_range(-ez / 2, ez / 2) * scale)
+        b["rot"][k] = _shoemake(rng)
+        b["shape"][k] = k
+    g = n_dyn
+    shape_type[g] = SHAPE_BOX
+    shape_half[g] = (20.0 * scale, 0.5, 20.0 * scale)
+    b["mass"][g] = 0.0
+    b["shape"][g] = g
+    b["flags"][g] = FLAG_STATIC
+    top = (y0 + ey) * scale + 5.0
+    half_xz = 20.0 * scale
+    return Scene(
+        name=f"random_mixed_{n_dyn}",
+        shape_type=shape_type, shape_half=shape_half,
+        bounds=np.array([-half_xz, -2.0, -half_xz, half_xz, top, half_xz], F32),
+        max_pairs=48 * n, max_contacts=16 * n, frames=frames,
+        meta=dict(seed=seed, n_box=int(n_box), n_sphere=int(n_dyn - n_box)), **b)
+
+
+# ---------------------------------------------------------------------------
+# Batched: many independent scenes
+# ---------------------------------------------------------------------------
+def lognormal_counts(n_scenes: int, median: float, sigma: float, lo: int, hi: int,
+                     rng: Pcg32) -> np.ndarray:
+    out = np.zeros(n_scenes, np.int64)
+    for s in range(n_scenes):
+        v = int(round(math.exp(math.log(median) + sigma * rng.normal())))
+        out[s] = min(max(v, lo), hi)
+    return out
+
+
+def pile_scene(n_dyn: int, rng: Pcg32, h_lo: float = 0.1, h_hi: float = 0.4,
+               v_sigma: float = 0.5, gap: float = 0.05, per_column: int = 8,
+               ground_half: float = 10.0) -> Scene:
+    """One pile scene: boxes in a jittered column grid (<= per_column per
+    column, 5 cm vertical gaps), N(0, v_sigma) initial linear velocity per
+    axis, own static ground (half ground_half x 0.5 x ground_half) last."""
+    n = n_dyn + 1
+    b = _alloc(n)
+    shape_type = np.zeros(n, np.uint32)
+    shape_half = np.zeros((n, 3), F32)
+    n_cols = (n_dyn + per_column - 1) // per_column
+    side = max(1, int(math.ceil(math.sqrt(n_cols))))
+    pitch = 2.0 * h_hi + 0.3
+    col_top = [0.5] * n_cols
+    for k in range(n_dyn):
+        c = k % n_cols
+        h = [F32(rng.uniform_range(h_lo, h_hi)) for _ in range(3)]
+        m = 1000.0 * 8.0 * float(h[0]) * float(h[1]) * float(h[2])
+        cx = (c % side - (side - 1) / 2.0) * pitch + rng.uniform_range(-0.05, 0.05)
+        cz = (c // side - (side - 1) / 2.0) * pitch + rng.uniform_range(-0.05, 0.05)
+        y = col_top[c] + gap + float(h[1])
+        col_top[c] = y + float(h[1])
+        shape_type[k] = SHAPE_BOX
+        shape_half[k] = h
+        b["pos"][k] = (cx, y, cz)
+        b["mass"][k] = m
+        b["inertia"][k] = np.array([m / 3.0 * (float(h[1]) ** 2 + float(h[2]) ** 2), 0, 0,
+                                    0, m / 3.0 * (float(h[0]) ** 2 + float(h[2]) ** 2), 0,
+                                    0, 0, m / 3.0 * (float(h[0]) ** 2 + float(h[1]) ** 2)], F32)
+        b["vel"][k] = (v_sigma * rng.normal(), v_sigma * rng.normal(), v_sigma * rng.normal())
+        b["shape"][k] = k
+    g = n_dyn
+    shape_type[g] = SHAPE_BOX
+    shape_half[g] = (ground_half, 0.5, ground_half)
+    b["mass"][g] = 0.0
+    b["shape"][g] = g
+    b["flags"][g] = FLAG_STATIC
+    top = max(col_top) + 10.0
+    return Scene(name=f"pile_{n_dyn}", shape_type=shape_type, shape_half=shape_half,
+                 bounds=np.array([-ground_half - 5, -2, -ground_half - 5,
+                                  ground_half + 5, top, ground_half + 5], F32),
+                 max_pairs=max(256, 48 * n), max_contacts=max(64, 16 * n), frames=300, **b)
+
+
+def batched_piles(n_scenes: int = 4096, seed: int = 20260003, median: float = 64.0,
+                  sigma: float = 0.75, lo: int = 8, hi: int = 512) -> list[Scene]:
+    rng = Pcg32(seed)
+    counts = lognormal_counts(n_scenes, median, sigma, lo, hi, rng)
+    return [pile_scene(int(c), rng) for c in counts]
+
+
+def two_body(shape_a: int, shape_b: int, half_a, half_b, pos_a, pos_b, mass=(1.0, 1.0),
+             flags=(0, 0), rot_a=(1, 0, 0, 0), rot_b=(1, 0, 0, 0), vel_a=(0, 0, 0),
+             vel_b=(0, 0, 0), inertia=(1.0, 1.0)) -> Scene:
+    """Two bodies with their own shapes, as the reference's single-pair KATs build."""
+    b = _alloc(2)
+    b["pos"][:] = [pos_a, pos_b]
+    b["rot"][:] = [rot_a, rot_b]
+    b["vel"][:] = [vel_a, vel_b]
+    b["mass"][:] = mass
+    b["flags"][:] = flags
+    b["shape"][:] = [0, 1]
+    b["inertia"][0] = _diag(inertia[0])
+    b["inertia"][1] = _diag(inertia[1])
+    return Scene(name="two_body", shape_type=np.array([shape_a, shape_b], np.uint32),
+                 shape_half=np.array([half_a, half_b], F32),
+                 bounds=np.array([-4, -4, -4, 4, 4, 4], F32), max_pairs=32, max_contacts=16,
+                 frames=1, **b)

@@ -1,0 +1,144 @@
+/*
+ * ref_cpu — CPU restatement of the dyphur frame step, used ONLY as test
+ * infrastructure: the parity oracle for the gfx950 kernels and the host-CPU
+ * baseline timed beside them.  Nothing in the product links, loads or calls
+ * this library; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg do.
+ *
+ * It restates the reference algorithms stage by stage (same loop structure,
+ * same floating-point operation order, one OpenMP parallel-for per reference
+ * parallel_for, including the multi-launch bitonic sorts), compiled with
+ * -O2 -ffp-contract=off and no fast-math.  Reference files followed:
+ *   core/src/integrator.cpp:7-71          (rc_integrate)
+ *   core/src/broadphase.cpp:14-365        (rc_bp_*)
+ *   core/src/narrowphase.cpp:12-306,912-1127 (rc_np_run)
+ *   core/src/xpbd_solver.cpp:1-134,409-521,558-733 (rc_solve)
+ *   core/src/body_store.cpp:28-69, core/include/core/math/mat3.hpp:61-79
+ *                                          (rc_body_setup)
+ *   compute/include/compute/sort.hpp:68-85 (rc_bitonic_sort_*)
+ *   examples/stress_test_blocks/main.cpp:34-41,301-311 (rc_fnv1a64_state)
+ *
+ * Semantic choices at the SYCL boundary (AdaptiveCpp is not available here):
+ *   sycl::half(float) = IEEE binary16 round-to-nearest-even;
+ *   sycl::rsqrt(x)    = 1.f / sqrtf(x) (correctly rounded sqrt and divide);
+ *   sycl::clz(0ull)   = 64.
+ */
+#ifndef DYPHUR_REF_CPU_H
+#define DYPHUR_REF_CPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Host SoA body arrays; same field order as the reference BodyView
+ * (core/include/core/body.hpp:30-46). */
+typedef struct rc_bodies {
+    float *pos_x, *pos_y, *pos_z;
+    float *rot_w, *rot_x, *rot_y, *rot_z;
+    float *vel_x, *vel_y, *vel_z;
+    float *ang_x, *ang_y, *ang_z;
+    float *inv_mass;
+    float *iI_xx, *iI_yy, *iI_zz;
+    float *iI_xy, *iI_xz, *iI_yz;
+    uint32_t *shape, *flags;
+    uint32_t n;
+} rc_bodies;
+
+/* Shape catalog (core/include/core/shapes.hpp:22-28). */
+typedef struct rc_shapes {
+    const uint32_t *type;
+    const float *half_x, *half_y, *half_z;
+    const uint32_t *ext_id;
+    uint32_t n;
+} rc_shapes;
+
+typedef struct rc_aabb { float min_x, min_y, min_z, max_x, max_y, max_z; } rc_aabb;
+typedef struct rc_pair { uint32_t a, b; } rc_pair;
+
+/* Host views of the oracle's contact store (core/include/core/contact.hpp). */
+typedef struct rc_contacts {
+    const uint32_t *body_a, *body_b;
+    const float *pos_x, *pos_y, *pos_z;
+    const float *norm_x, *norm_y, *norm_z;
+    const float *depth;
+    uint32_t n;          /* stored (clamped) count */
+    uint32_t last_count; /* true count before clamping */
+} rc_contacts;
+
+typedef struct rc_solve_stats {
+    uint32_t n_constraints;
+    uint32_t used_serial;  /* 1 = run_serial path taken */
+    uint32_t n_colors;     /* colours used by the coloured path */
+    uint32_t rounds;       /* JP rounds executed */
+    uint32_t overflow;     /* >64 colours on a body */
+} rc_solve_stats;
+
+/* ---- numeric building blocks ------------------------------------------ */
+uint16_t rc_half_from_float(float f);
+float    rc_half_to_float(uint16_t h);
+void     rc_half_from_float_array(const float *in, uint16_t *out, size_t n);
+uint32_t rc_hash_u32(uint32_t x);
+uint32_t rc_morton3(uint32_t x, uint32_t y, uint32_t z);
+uint64_t rc_fnv1a64(uint64_t h, const void *data, size_t n);
+uint64_t rc_fnv1a64_state(const rc_bodies *b);
+
+/* BodyStore::add for n bodies: mass[n], inertia[9n] row-major, flags_in[n]
+ * -> inv_mass, iI_* (inverse inertia, zero for statics), flags (Static OR-ed). */
+void rc_body_setup(uint32_t n, const float *mass, const float *inertia,
+                   const uint32_t *flags_in, rc_bodies *out);
+
+/* ---- stages -------------------------------------------------------------*/
+void rc_integrate(rc_bodies *b, float gx, float gy, float gz, float dt, int n_substeps);
+
+void rc_bitonic_sort_u64_u32(uint64_t *keys, uint32_t *vals, size_t n);
+void rc_bitonic_sort_u32_u32(uint32_t *keys, uint32_t *vals, size_t n);
+void rc_bitonic_sort_u32(uint32_t *keys, size_t n);
+
+typedef struct rc_world rc_world;
+rc_world *rc_world_create(uint32_t max_bodies, uint32_t max_pairs,
+                          uint32_t max_contacts, int n_iters, float mu);
+void rc_world_destroy(rc_world *w);
+
+/* Broadphase::build_and_query + download_count. Returns the true pair count. */
+uint32_t rc_bp_build_and_query(rc_world *w, const rc_bodies *b, const rc_shapes *s,
+                               const rc_aabb *bounds);
+/* Copies BVH internals out (any pointer may be NULL).  aabb holds 8 u16 per
+ * node (mn xyz, mx xyz, pad, pad); node count 2n-1; internal n-1. */
+void rc_bp_copy_out(const rc_world *w, uint32_t n, uint64_t *morton, uint32_t *sorted_idx,
+                    int32_t *left, int32_t *right, int32_t *parent, int32_t *root,
+                    uint16_t *aabb, uint32_t *refit_rounds);
+void rc_bp_sort_pairs(rc_world *w, uint32_t n_pairs);
+void rc_bp_copy_pairs(const rc_world *w, rc_pair *out, uint32_t n);
+const rc_pair *rc_bp_pairs(const rc_world *w);
+
+/* Narrowphase::run (CPU pair count). Returns stored contact count. */
+uint32_t rc_np_run(rc_world *w, const rc_pair *pairs, uint32_t n_pairs,
+                   const rc_bodies *b, const rc_shapes *s);
+void rc_np_contacts(const rc_world *w, rc_contacts *out);
+/* Overwrite the contact store (for solver-only tests). */
+void rc_np_set_contacts(rc_world *w, uint32_t n, const uint32_t *body_a, const uint32_t *body_b,
+                        const float *px, const float *py, const float *pz,
+                        const float *nx, const float *ny, const float *nz, const float *depth);
+
+/* XpbdSolver::solve with JointView{} (n = 0). */
+void rc_solve(rc_world *w, rc_bodies *b, float dt, int force_serial, rc_solve_stats *stats);
+/* Colour array of the last coloured solve (n_constraints entries). */
+void rc_solve_colors(const rc_world *w, uint32_t *out, uint32_t n);
+
+/* One frame of the stress driver loop (examples/stress_test_blocks/main.cpp:265-272):
+ * integrate -> build_and_query -> download_count -> sort_pairs -> narrowphase -> solve. */
+typedef struct rc_frame_stats { uint32_t n_pairs, n_contacts, n_colors, used_serial; } rc_frame_stats;
+void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *bounds,
+              float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
+              rc_frame_stats *st);
+
+int rc_max_threads(void);
+void rc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
