@@ -1,0 +1,544 @@
+"""ctypes binding of the C ABI in include/dyphur_hip.h (libdyphur_hip.so).
+
+This is the binding a Python caller of the reference's path would add on top of
+the drop-in library; the GPU parity tests and bench.py call the frame step
+through it.  ``load()`` raises when the library is missing: there is no CPU
+fallback for the product path.
+
+Device memory is allocated through the ABI (``dy_malloc``); ``DeviceBodies`` and
+``DeviceShapes`` hold SoA arrays laid out exactly like the reference's
+BodyView / ShapeView.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libdyphur_hip.so")
+_lib = None
+
+DY_OK = 0
+DY_ERR_INVALID_ARG = -1
+DY_ERR_UNSUPPORTED = -2
+DY_ERR_NO_DEVICE = -3
+DY_ERR_INTERNAL = -4
+
+vp = C.c_void_p
+u32 = C.c_uint32
+f32 = C.c_float
+
+BODY_FLOAT_FIELDS = ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
+                     "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z", "inv_mass",
+                     "iI_xx", "iI_yy", "iI_zz", "iI_xy", "iI_xz", "iI_yz")
+CONTACT_FIELDS = (("body_a", np.uint32), ("body_b", np.uint32), ("pos_x", np.float32),
+                  ("pos_y", np.float32), ("pos_z", np.float32), ("norm_x", np.float32),
+                  ("norm_y", np.float32), ("norm_z", np.float32), ("depth", np.float32))
+
+
+class BodyView(C.Structure):
+    _fields_ = [(f, vp) for f in BODY_FLOAT_FIELDS] + [("shape", vp), ("flags", vp), ("n", u32)]
+
+
+class ShapeView(C.Structure):
+    _fields_ = [("type", vp), ("half_x", vp), ("half_y", vp), ("half_z", vp), ("ext_id", vp), ("n", u32)]
+
+
+class ContactView(C.Structure):
+    _fields_ = [(f, vp) for f, _ in CONTACT_FIELDS] + [("n", vp), ("capacity", u32)]
+
+
+class Aabb(C.Structure):
+    _fields_ = [(n, f32) for n in ("min_x", "min_y", "min_z", "max_x", "max_y", "max_z")]
+
+
+class BvhView(C.Structure):
+    _fields_ = [("left", vp), ("right", vp), ("parent", vp), ("root", vp), ("sorted_idx", vp),
+                ("aabb", vp), ("n", u32)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("is_gpu", C.c_int), ("arch_major", C.c_int),
+                ("arch_minor", C.c_int), ("compute_units", C.c_size_t), ("global_mem_bytes", C.c_size_t)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [(n, u32) for n in ("n_constraints", "used_serial", "n_classes", "rounds", "overflow")]
+
+
+class DyError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        msg = lib().dy_status_string(status).decode() if _lib else str(status)
+        super().__init__(f"{what} failed: {msg} (status {status})")
+
+
+_SIGS = {
+    "dy_status_string": (C.c_char_p, [C.c_int]),
+    "dy_version": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "dy_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "dy_device_get_info": (C.c_int, [C.c_int, C.POINTER(DeviceInfo)]),
+    "dy_set_device": (C.c_int, [C.c_int]),
+    "dy_stream_create": (C.c_int, [C.POINTER(vp)]),
+    "dy_stream_destroy": (C.c_int, [vp]),
+    "dy_stream_synchronize": (C.c_int, [vp]),
+    "dy_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+    "dy_free": (C.c_int, [vp]),
+    "dy_memcpy_h2d_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "dy_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "dy_memcpy_d2d_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "dy_memset_async": (C.c_int, [vp, C.c_int, C.c_size_t, vp]),
+    "dy_event_create": (C.c_int, [C.POINTER(vp)]),
+    "dy_event_destroy": (C.c_int, [vp]),
+    "dy_event_record": (C.c_int, [vp, vp]),
+    "dy_event_synchronize": (C.c_int, [vp]),
+    "dy_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(f32)]),
+    "dy_sort_by_key_u64_u32": (C.c_int, [vp, vp, vp, C.c_size_t]),
+    "dy_sort_by_key_u32_u32": (C.c_int, [vp, vp, vp, C.c_size_t]),
+    "dy_sort_u32": (C.c_int, [vp, vp, C.c_size_t]),
+    "dy_sort_u64": (C.c_int, [vp, vp, C.c_size_t]),
+    "dy_reduce_sum_f32": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(f32)]),
+    "dy_scan_exclusive_u32": (C.c_int, [vp, vp, vp, C.c_size_t, vp]),
+    "dy_body_dynamics_host": (C.c_int, [u32, vp, vp, vp, vp, vp, vp]),
+    "dy_integrate": (C.c_int, [vp, C.POINTER(BodyView), f32, f32, f32, f32, C.c_int]),
+    "dy_broadphase_create": (C.c_int, [vp, u32, u32, C.POINTER(vp)]),
+    "dy_broadphase_destroy": (C.c_int, [vp]),
+    "dy_broadphase_build_and_query": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView),
+                                                 C.POINTER(Aabb)]),
+    "dy_broadphase_pairs": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(u32)]),
+    "dy_broadphase_sort_pairs": (C.c_int, [vp, vp, u32]),
+    "dy_broadphase_download_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
+    "dy_broadphase_bvh_view": (C.c_int, [vp, u32, C.POINTER(BvhView)]),
+    "dy_broadphase_morton_keys": (C.c_int, [vp, C.POINTER(vp)]),
+    "dy_narrowphase_create": (C.c_int, [vp, u32, C.POINTER(vp)]),
+    "dy_narrowphase_destroy": (C.c_int, [vp]),
+    "dy_narrowphase_run": (C.c_int, [vp, vp, vp, u32, C.POINTER(BodyView), C.POINTER(ShapeView)]),
+    "dy_narrowphase_run_device_count": (C.c_int, [vp, vp, vp, vp, u32, C.POINTER(BodyView),
+                                                   C.POINTER(ShapeView)]),
+    "dy_narrowphase_contacts": (C.c_int, [vp, C.POINTER(ContactView)]),
+    "dy_narrowphase_download_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
+    "dy_narrowphase_last_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
+    "dy_xpbd_create": (C.c_int, [vp, C.c_int, u32, f32, C.POINTER(vp)]),
+    "dy_xpbd_destroy": (C.c_int, [vp]),
+    "dy_xpbd_set_force_serial": (C.c_int, [vp, C.c_int]),
+    "dy_xpbd_solve": (C.c_int, [vp, vp, C.POINTER(ContactView), u32, C.POINTER(BodyView), f32]),
+    "dy_xpbd_last_stats": (C.c_int, [vp, C.POINTER(SolveStats)]),
+    "dy_xpbd_download_classes": (C.c_int, [vp, vp, vp, u32]),
+    "dy_state_hash_fnv1a64": (C.c_int, [vp, C.POINTER(BodyView), C.POINTER(C.c_uint64)]),
+}
+
+# Symbols declared in include/dyphur_hip.h (checked by the CPU test suite).
+DECLARED = tuple(k for k in _SIGS if k != "dy_broadphase_morton_keys")
+
+
+def load(path: str = LIB_PATH):
+    """Load libdyphur_hip.so; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libdyphur_hip.so not built ({path}); run python -m dyphur_amd.build")
+    L = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def lib():
+    return load()
+
+
+def check(status: int, what: str):
+    if status != DY_OK:
+        raise DyError(status, what)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().dy_device_count(C.byref(n)), "dy_device_count")
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# Streams, events, device memory
+# ---------------------------------------------------------------------------
+class Stream:
+    def __init__(self):
+        h = vp()
+        check(lib().dy_stream_create(C.byref(h)), "dy_stream_create")
+        self.h = h
+
+    def wait(self):
+        check(lib().dy_stream_synchronize(self.h), "dy_stream_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = vp()
+        check(lib().dy_event_create(C.byref(h)), "dy_event_create")
+        self.h = h
+
+    def record(self, s: Stream):
+        check(lib().dy_event_record(self.h, s.h), "dy_event_record")
+
+    def sync(self):
+        check(lib().dy_event_synchronize(self.h), "dy_event_synchronize")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = f32(0)
+        check(lib().dy_event_elapsed_ms(self.h, end.h, C.byref(ms)), "dy_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().dy_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation with a dtype and length."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = vp()
+        check(lib().dy_malloc(C.byref(p), max(self.nbytes, 1)), "dy_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, s: Stream) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a, s)
+        return d
+
+    def upload(self, a: np.ndarray, s: Stream, count: int | None = None):
+        a = np.ascontiguousarray(a, self.dtype)
+        nb = (a.size if count is None else count) * self.dtype.itemsize
+        check(lib().dy_memcpy_h2d_async(self.ptr, a.ctypes.data, nb, s.h), "dy_memcpy_h2d_async")
+        s.wait()  # the numpy source may be freed after return
+
+    def download(self, s: Stream, count: int | None = None) -> np.ndarray:
+        n = self.n if count is None else count
+        out = np.empty(n, self.dtype)
+        if n:
+            check(lib().dy_memcpy_d2h(out.ctypes.data, self.ptr, n * self.dtype.itemsize, s.h), "dy_memcpy_d2h")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().dy_free(vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_ptr(ptr: int, n: int, dtype, s: Stream) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    out = np.empty(n, dtype)
+    if n:
+        check(lib().dy_memcpy_d2h(out.ctypes.data, vp(ptr), n * dtype.itemsize, s.h), "dy_memcpy_d2h")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Body / shape stores
+# ---------------------------------------------------------------------------
+def body_dynamics(mass: np.ndarray, inertia: np.ndarray, flags: np.ndarray):
+    """BodyStore::add dynamics on the host: inv_mass, iI (n, 6), flags."""
+    n = len(mass)
+    mass = np.ascontiguousarray(mass, np.float32)
+    inertia = np.ascontiguousarray(inertia, np.float32).reshape(n, 9)
+    flags = np.ascontiguousarray(flags, np.uint32)
+    inv = np.zeros(n, np.float32)
+    iI = np.zeros((n, 6), np.float32)
+    fo = np.zeros(n, np.uint32)
+    check(lib().dy_body_dynamics_host(n, mass.ctypes.data, inertia.ctypes.data, flags.ctypes.data,
+                                      inv.ctypes.data, iI.ctypes.data, fo.ctypes.data), "dy_body_dynamics_host")
+    return inv, iI, fo
+
+
+class DeviceBodies:
+    """Device SoA body arrays (BodyView layout) for a Scene."""
+
+    def __init__(self, scene, s: Stream, capacity: int | None = None):
+        self.n = scene.n
+        self.capacity = capacity or scene.n
+        self.f = {k: DeviceArray(self.capacity, np.float32) for k in BODY_FLOAT_FIELDS}
+        self.shape = DeviceArray(self.capacity, np.uint32)
+        self.flags = DeviceArray(self.capacity, np.uint32)
+        self.load(scene, s)
+
+    def host_arrays(self, scene) -> dict:
+        inv, iI, fo = body_dynamics(scene.mass, scene.inertia, scene.flags)
+        h = {}
+        for i, k in enumerate(("pos_x", "pos_y", "pos_z")):
+            h[k] = scene.pos[:, i]
+        for i, k in enumerate(("rot_w", "rot_x", "rot_y", "rot_z")):
+            h[k] = scene.rot[:, i]
+        for i, k in enumerate(("vel_x", "vel_y", "vel_z")):
+            h[k] = scene.vel[:, i]
+        for i, k in enumerate(("ang_x", "ang_y", "ang_z")):
+            h[k] = scene.ang[:, i]
+        h["inv_mass"] = inv
+        for i, k in enumerate(("iI_xx", "iI_yy", "iI_zz", "iI_xy", "iI_xz", "iI_yz")):
+            h[k] = iI[:, i]
+        h["shape"] = scene.shape
+        h["flags"] = fo
+        return h
+
+    def load(self, scene, s: Stream):
+        h = self.host_arrays(scene)
+        for k in BODY_FLOAT_FIELDS:
+            self.f[k].upload(h[k], s)
+        self.shape.upload(h["shape"], s)
+        self.flags.upload(h["flags"], s)
+        self.n = scene.n
+
+    def view(self) -> BodyView:
+        v = BodyView()
+        for k in BODY_FLOAT_FIELDS:
+            setattr(v, k, self.f[k].ptr)
+        v.shape = self.shape.ptr
+        v.flags = self.flags.ptr
+        v.n = self.n
+        return v
+
+    def download(self, s: Stream) -> dict:
+        d = {k: self.f[k].download(s, self.n) for k in BODY_FLOAT_FIELDS}
+        d["shape"] = self.shape.download(s, self.n)
+        d["flags"] = self.flags.download(s, self.n)
+        return d
+
+    def upload_state(self, state: dict, s: Stream):
+        for k in BODY_FLOAT_FIELDS:
+            if k in state:
+                self.f[k].upload(state[k], s)
+
+    def fnv(self, s: Stream) -> int:
+        out = C.c_uint64(0)
+        v = self.view()
+        check(lib().dy_state_hash_fnv1a64(s.h, C.byref(v), C.byref(out)), "dy_state_hash_fnv1a64")
+        return out.value
+
+
+class DeviceShapes:
+    def __init__(self, scene, s: Stream):
+        self.n = scene.n_shapes
+        self.type = DeviceArray.from_numpy(np.asarray(scene.shape_type, np.uint32), s)
+        self.hx = DeviceArray.from_numpy(np.asarray(scene.shape_half[:, 0], np.float32), s)
+        self.hy = DeviceArray.from_numpy(np.asarray(scene.shape_half[:, 1], np.float32), s)
+        self.hz = DeviceArray.from_numpy(np.asarray(scene.shape_half[:, 2], np.float32), s)
+        self.ext = DeviceArray.from_numpy(np.zeros(self.n, np.uint32), s)
+
+    def view(self) -> ShapeView:
+        return ShapeView(self.type.ptr, self.hx.ptr, self.hy.ptr, self.hz.ptr, self.ext.ptr, self.n)
+
+
+def aabb(bounds) -> Aabb:
+    return Aabb(*[float(x) for x in bounds])
+
+
+# ---------------------------------------------------------------------------
+# Stages
+# ---------------------------------------------------------------------------
+def integrate(s: Stream, bodies: DeviceBodies, gravity, dt: float, n_substeps: int):
+    v = bodies.view()
+    check(lib().dy_integrate(s.h, C.byref(v), gravity[0], gravity[1], gravity[2], dt, n_substeps), "dy_integrate")
+
+
+class Broadphase:
+    def __init__(self, s: Stream, max_bodies: int, max_pairs: int):
+        h = vp()
+        check(lib().dy_broadphase_create(s.h, max_bodies, max_pairs, C.byref(h)), "dy_broadphase_create")
+        self.h = h
+        self.max_bodies, self.max_pairs = max_bodies, max_pairs
+
+    def build_and_query(self, s: Stream, bodies: DeviceBodies, shapes: DeviceShapes, bounds):
+        bv, sv, bd = bodies.view(), shapes.view(), aabb(bounds)
+        check(lib().dy_broadphase_build_and_query(self.h, s.h, C.byref(bv), C.byref(sv), C.byref(bd)),
+              "dy_broadphase_build_and_query")
+
+    def pairs_ptr(self):
+        p, c, m = vp(), vp(), u32()
+        check(lib().dy_broadphase_pairs(self.h, C.byref(p), C.byref(c), C.byref(m)), "dy_broadphase_pairs")
+        return p.value, c.value
+
+    def download_count(self, s: Stream) -> int:
+        out = u32(0)
+        check(lib().dy_broadphase_download_count(self.h, s.h, C.byref(out)), "dy_broadphase_download_count")
+        return out.value
+
+    def sort_pairs(self, s: Stream, n_pairs: int):
+        check(lib().dy_broadphase_sort_pairs(self.h, s.h, n_pairs), "dy_broadphase_sort_pairs")
+
+    def pairs(self, s: Stream, n: int) -> np.ndarray:
+        n = min(n, self.max_pairs)
+        p, _ = self.pairs_ptr()
+        return download_ptr(p, 2 * n, np.uint32, s).reshape(n, 2)
+
+    def bvh(self, s: Stream, n: int) -> dict:
+        v = BvhView()
+        check(lib().dy_broadphase_bvh_view(self.h, n, C.byref(v)), "dy_broadphase_bvh_view")
+        k = vp()
+        check(lib().dy_broadphase_morton_keys(self.h, C.byref(k)), "dy_broadphase_morton_keys")
+        out = dict(morton=download_ptr(k.value, n, np.uint64, s),
+                   sorted_idx=download_ptr(v.sorted_idx, n, np.uint32, s),
+                   root=download_ptr(v.root, 1, np.int32, s))
+        if n >= 2:
+            out.update(left=download_ptr(v.left, n - 1, np.int32, s),
+                       right=download_ptr(v.right, n - 1, np.int32, s),
+                       parent=download_ptr(v.parent, 2 * n - 1, np.int32, s),
+                       aabb=download_ptr(v.aabb, 8 * (2 * n - 1), np.uint16, s).reshape(2 * n - 1, 8))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().dy_broadphase_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Narrowphase:
+    def __init__(self, s: Stream, max_contacts: int):
+        h = vp()
+        check(lib().dy_narrowphase_create(s.h, max_contacts, C.byref(h)), "dy_narrowphase_create")
+        self.h = h
+        self.max_contacts = max_contacts
+
+    def run(self, s: Stream, pairs_ptr: int, n_pairs: int, bodies: DeviceBodies, shapes: DeviceShapes):
+        bv, sv = bodies.view(), shapes.view()
+        check(lib().dy_narrowphase_run(self.h, s.h, vp(pairs_ptr), n_pairs, C.byref(bv), C.byref(sv)),
+              "dy_narrowphase_run")
+
+    def run_device_count(self, s: Stream, pairs_ptr: int, count_ptr: int, max_pairs: int,
+                         bodies: DeviceBodies, shapes: DeviceShapes):
+        bv, sv = bodies.view(), shapes.view()
+        check(lib().dy_narrowphase_run_device_count(self.h, s.h, vp(pairs_ptr), vp(count_ptr), max_pairs,
+                                                    C.byref(bv), C.byref(sv)), "dy_narrowphase_run_device_count")
+
+    def contacts_view(self) -> ContactView:
+        v = ContactView()
+        check(lib().dy_narrowphase_contacts(self.h, C.byref(v)), "dy_narrowphase_contacts")
+        return v
+
+    def download_count(self, s: Stream) -> int:
+        out = u32(0)
+        check(lib().dy_narrowphase_download_count(self.h, s.h, C.byref(out)), "dy_narrowphase_download_count")
+        return out.value
+
+    def last_count(self, s: Stream) -> int:
+        out = u32(0)
+        check(lib().dy_narrowphase_last_count(self.h, s.h, C.byref(out)), "dy_narrowphase_last_count")
+        return out.value
+
+    def contacts(self, s: Stream) -> dict:
+        n = self.download_count(s)
+        v = self.contacts_view()
+        out = {"n": n}
+        for f, dt in CONTACT_FIELDS:
+            out[f] = download_ptr(getattr(v, f), n, dt, s)
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().dy_narrowphase_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class XpbdSolver:
+    def __init__(self, s: Stream, n_iters: int = 10, max_contacts: int = 16384, friction: float = 0.5):
+        h = vp()
+        check(lib().dy_xpbd_create(s.h, n_iters, max_contacts, friction, C.byref(h)), "dy_xpbd_create")
+        self.h = h
+        self.max_contacts = max_contacts
+
+    def set_force_serial(self, v: bool):
+        check(lib().dy_xpbd_set_force_serial(self.h, int(bool(v))), "dy_xpbd_set_force_serial")
+
+    def solve(self, s: Stream, contacts: ContactView, bodies: DeviceBodies, dt: float, n_joints: int = 0):
+        bv = bodies.view()
+        check(lib().dy_xpbd_solve(self.h, s.h, C.byref(contacts), n_joints, C.byref(bv), dt), "dy_xpbd_solve")
+
+    def stats(self) -> dict:
+        st = SolveStats()
+        check(lib().dy_xpbd_last_stats(self.h, C.byref(st)), "dy_xpbd_last_stats")
+        return {k: getattr(st, k) for k, _ in SolveStats._fields_}
+
+    def classes(self, s: Stream, n: int) -> np.ndarray:
+        out = np.zeros(n, np.uint32)
+        check(lib().dy_xpbd_download_classes(self.h, s.h, out.ctypes.data, n), "dy_xpbd_download_classes")
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().dy_xpbd_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class World:
+    """The stress driver's per-frame sequence (examples/stress_test_blocks/main.cpp:264-273)
+    over the C ABI: integrate -> build_and_query -> download_count -> sort_pairs ->
+    narrowphase -> solve."""
+
+    def __init__(self, scene, s: Stream | None = None):
+        self.scene = scene
+        self.s = s or Stream()
+        self.bodies = DeviceBodies(scene, self.s)
+        self.shapes = DeviceShapes(scene, self.s)
+        self.bp = Broadphase(self.s, scene.n, scene.max_pairs)
+        self.np = Narrowphase(self.s, scene.max_contacts)
+        self.solver = XpbdSolver(self.s, scene.n_iters, scene.max_contacts, scene.mu)
+        self.last = {}
+
+    def step(self, force_serial: bool = False) -> dict:
+        sc, s = self.scene, self.s
+        integrate(s, self.bodies, sc.gravity, sc.dt, sc.n_substeps)
+        self.bp.build_and_query(s, self.bodies, self.shapes, sc.bounds)
+        n_pairs = self.bp.download_count(s)
+        self.bp.sort_pairs(s, n_pairs)
+        pairs, _ = self.bp.pairs_ptr()
+        self.np.run(s, pairs, min(n_pairs, self.bp.max_pairs), self.bodies, self.shapes)
+        self.solver.set_force_serial(force_serial)
+        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt)
+        st = self.solver.stats()
+        self.last = dict(n_pairs=n_pairs, n_contacts=st["n_constraints"], n_colors=st["n_classes"],
+                         used_serial=st["used_serial"])
+        return self.last
+
+    def state(self) -> dict:
+        return self.bodies.download(self.s)
+
+    def fnv(self) -> int:
+        return self.bodies.fnv(self.s)

@@ -1,0 +1,562 @@
+// broadphase.hip — LBVH broadphase for gfx950 (core/src/broadphase.cpp:101-365).
+//
+// Pipeline (all on one stream, no host synchronisation):
+//   1. morton_sort     one workgroup: 30-bit Morton codes of body centres, then a
+//                      stable 4-pass LSD radix sort of (code, body) — the same
+//                      order as the reference's sort of (code << 32 | body).
+//   2. karras_build    one thread per internal node: children, parents, leaf
+//                      ranges and the depth-first "escape" links that make the
+//                      traversal stackless.
+//   3. leaf_refit      one workgroup per 1024 leaves: fp16 leaf boxes, then a
+//                      counter-gated bottom-up merge inside the chunk (LDS
+//                      counters, workgroup scope); the last workgroup to finish
+//                      (agent-scope release/acquire around a ticket) merges the
+//                      few nodes whose ranges cross chunks.
+//   4. traverse_count  one thread per leaf in Morton order: stackless walk,
+//                      counting partners b > a and keeping the 16 smallest in
+//                      registers (sorted).
+//   5. scan of counts  row offsets per body a; the total is the true pair count.
+//   6. emit_pairs      one thread per body: writes its row (a, b0 < b1 < ...);
+//                      rows longer than 16 re-walk the tree for the next 16.
+// The output is the reference's pair set in ascending (a, b) order, so
+// sort_pairs has nothing left to do.
+#include "dy_internal.hpp"
+
+namespace dy {
+
+constexpr uint32_t kRow = 16;            // partners kept in registers per walk
+constexpr uint32_t kChunk = 1024;        // leaves per refit workgroup
+constexpr uint32_t kMortonMax = 1u << 30;
+// A Karras tree over distinct 64-bit keys is at most 65 levels deep; walks
+// that exceed this bound flag an internal error instead of looping.
+constexpr int kMaxHeight = 96;
+
+// Traversal record, 32 B: fp16 box, subtree max body index, first child
+// (internal) or body (leaf), escape link.
+struct alignas(16) TNode {
+    uint32_t b01, b23, b45;  // (mn_x | mn_y << 16), (mn_z | mx_x << 16), (mx_y | mx_z << 16)
+    uint32_t maxbody;
+    int32_t child;
+    int32_t escape;
+    uint32_t pad0, pad1;
+};
+
+struct Box6 { float mn[3], mx[3]; };
+
+__device__ __forceinline__ Box6 unpack_box(uint32_t b01, uint32_t b23, uint32_t b45) {
+    Box6 b;
+    b.mn[0] = f16_to_f32(static_cast<uint16_t>(b01 & 0xffffu));
+    b.mn[1] = f16_to_f32(static_cast<uint16_t>(b01 >> 16));
+    b.mn[2] = f16_to_f32(static_cast<uint16_t>(b23 & 0xffffu));
+    b.mx[0] = f16_to_f32(static_cast<uint16_t>(b23 >> 16));
+    b.mx[1] = f16_to_f32(static_cast<uint16_t>(b45 & 0xffffu));
+    b.mx[2] = f16_to_f32(static_cast<uint16_t>(b45 >> 16));
+    return b;
+}
+
+// Reference overlap test (broadphase.cpp:322-325): touching counts.
+__device__ __forceinline__ bool overlaps(const Box6 &n, const Box6 &q) {
+    return !(n.mn[0] > q.mx[0] || n.mx[0] < q.mn[0] || n.mn[1] > q.mx[1] || n.mx[1] < q.mn[1] ||
+             n.mn[2] > q.mx[2] || n.mx[2] < q.mn[2]);
+}
+
+// Union of two packed fp16 boxes (exact: fmin/fmax of representable values).
+__device__ __forceinline__ void merge_packed(const uint32_t a[3], const uint32_t b[3], uint32_t o[3]) {
+    uint16_t ha[6] = {static_cast<uint16_t>(a[0] & 0xffffu), static_cast<uint16_t>(a[0] >> 16),
+                      static_cast<uint16_t>(a[1] & 0xffffu), static_cast<uint16_t>(a[1] >> 16),
+                      static_cast<uint16_t>(a[2] & 0xffffu), static_cast<uint16_t>(a[2] >> 16)};
+    uint16_t hb[6] = {static_cast<uint16_t>(b[0] & 0xffffu), static_cast<uint16_t>(b[0] >> 16),
+                      static_cast<uint16_t>(b[1] & 0xffffu), static_cast<uint16_t>(b[1] >> 16),
+                      static_cast<uint16_t>(b[2] & 0xffffu), static_cast<uint16_t>(b[2] >> 16)};
+    uint16_t r[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = f32_to_f16(fminf(f16_to_f32(ha[c]), f16_to_f32(hb[c])));
+#pragma unroll
+    for (int c = 3; c < 6; ++c) r[c] = f32_to_f16(fmaxf(f16_to_f32(ha[c]), f16_to_f32(hb[c])));
+    o[0] = r[0] | (static_cast<uint32_t>(r[1]) << 16);
+    o[1] = r[2] | (static_cast<uint32_t>(r[3]) << 16);
+    o[2] = r[4] | (static_cast<uint32_t>(r[5]) << 16);
+}
+
+__device__ __forceinline__ void store_node(dy_node_box *nb, TNode *tn, int node, const uint32_t p[3],
+                                           uint32_t maxbody) {
+    // Reference NodeBox layout (pads are zero) and the traversal record.
+    uint4 v = make_uint4(p[0], p[1], p[2], 0u);
+    *reinterpret_cast<uint4 *>(&nb[node]) = v;
+    tn[node].b01 = p[0];
+    tn[node].b23 = p[1];
+    tn[node].b45 = p[2];
+    tn[node].maxbody = maxbody;
+}
+
+// ---------------------------------------------------------------------------
+// 1. Morton codes + stable radix sort, one workgroup
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMsThreads = 1024;
+constexpr uint32_t kMsWaves = kMsThreads / 64;
+constexpr uint32_t kMsItems = 4;
+constexpr uint32_t kMsTile = kMsThreads * kMsItems;
+
+__global__ __launch_bounds__(kMsThreads) void morton_sort(dy_body_view bv, dy_aabb bd, uint32_t n,
+                                                          uint32_t *code0, uint32_t *idx0, uint32_t *code1,
+                                                          uint32_t *idx1, uint64_t *morton64) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[kMsWaves][256];
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t i = t; i < n; i += kMsThreads) {
+        const uint32_t qx = quantize10(bv.pos_x[i], bd.min_x, bd.max_x);
+        const uint32_t qy = quantize10(bv.pos_y[i], bd.min_y, bd.max_y);
+        const uint32_t qz = quantize10(bv.pos_z[i], bd.min_z, bd.max_z);
+        code0[i] = morton3(qx, qy, qz);
+        idx0[i] = i;
+    }
+    __syncthreads();
+    uint32_t *sk = code0, *sv = idx0, *dk = code1, *dv = idx1;
+    for (int bit = 0; bit < 30; bit += 8) {
+        const uint32_t mask = (bit == 24) ? 0x3fu : 0xffu;
+        if (t < 256) hist[t] = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += kMsThreads) atomicAdd(&hist[(sk[i] >> bit) & mask], 1u);
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t h[4], s = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { h[j] = hist[lane * 4 + j]; s += h[j]; }
+            uint32_t run = wave_inclusive_scan(s) - s;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { base[lane * 4 + j] = run; run += h[j]; }
+        }
+        __syncthreads();
+        for (uint32_t tile = 0; tile < n; tile += kMsTile) {
+            for (uint32_t i = t; i < kMsWaves * 256; i += kMsThreads) (&wcnt[0][0])[i] = 0;
+            __syncthreads();
+            uint32_t kk[kMsItems], vv[kMsItems], dd[kMsItems], rr[kMsItems];
+#pragma unroll
+            for (uint32_t j = 0; j < kMsItems; ++j) {
+                const uint32_t idx = tile + wave * (64 * kMsItems) + j * 64 + lane;
+                const bool valid = idx < n;
+                kk[j] = valid ? sk[idx] : 0u;
+                vv[j] = valid ? sv[idx] : 0u;
+                const uint32_t d = valid ? ((kk[j] >> bit) & mask) : 256u;
+                uint64_t peers = ~0ull;
+#pragma unroll
+                for (int b = 0; b < 9; ++b) {
+                    const bool set = (d >> b) & 1u;
+                    const uint64_t bal = __ballot(set);
+                    peers &= set ? bal : ~bal;
+                }
+                const uint32_t pre = static_cast<uint32_t>(__popcll(peers & lt_mask));
+                const uint32_t c = valid ? wcnt[wave][valid ? d : 0u] : 0u;
+                rr[j] = c + pre;
+                dd[j] = d;
+                if (valid && pre == 0) wcnt[wave][d] = c + static_cast<uint32_t>(__popcll(peers));
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+            if (t < 256) {
+                uint32_t run = base[t];
+                for (uint32_t w = 0; w < kMsWaves; ++w) {
+                    const uint32_t c = wcnt[w][t];
+                    wcnt[w][t] = run;
+                    run += c;
+                }
+                base[t] = run;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < kMsItems; ++j) {
+                if (dd[j] < 256u) {
+                    const uint32_t pos = wcnt[wave][dd[j]] + rr[j];
+                    dk[pos] = kk[j];
+                    dv[pos] = vv[j];
+                }
+            }
+            __syncthreads();
+        }
+        uint32_t *tk = sk; sk = dk; dk = tk;
+        uint32_t *tv = sv; sv = dv; dv = tv;
+        __syncthreads();
+    }
+    // Four passes: the sorted data is back in (code0, idx0).
+    for (uint32_t i = t; i < n; i += kMsThreads)
+        morton64[i] = (static_cast<uint64_t>(code0[i]) << 32) | idx0[i];
+}
+
+// ---------------------------------------------------------------------------
+// 2. Karras build + escape links
+// ---------------------------------------------------------------------------
+__global__ void karras_build(const uint64_t *__restrict__ codes, int n, int32_t *__restrict__ left,
+                             int32_t *__restrict__ right, int32_t *__restrict__ parent, int2 *__restrict__ range,
+                             TNode *__restrict__ tn, uint32_t *__restrict__ cnt_top, int32_t *__restrict__ root) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n - 1) return;
+    int lo, hi, g;
+    karras_node(codes, n, i, lo, hi, g);
+    const int l = (lo == g) ? (n - 1 + lo) : g;
+    const int r = (g + 1 == hi) ? (n - 1 + g + 1) : (g + 1);
+    left[i] = l;
+    right[i] = r;
+    parent[l] = i;
+    parent[r] = i;
+    range[i] = make_int2(lo, hi);
+    cnt_top[i] = 0u;
+    tn[i].child = l;
+    tn[l].escape = r;
+    tn[r].escape = karras_escape(codes, n, hi);
+    if (i == 0) {
+        parent[0] = -1;
+        tn[0].escape = kEnd;
+        root[0] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Leaf boxes + refit
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_view sv, int n,
+                                                     const uint32_t *__restrict__ sorted_idx,
+                                                     const int32_t *__restrict__ parent,
+                                                     const int32_t *__restrict__ left,
+                                                     const int32_t *__restrict__ right,
+                                                     const int2 *__restrict__ range, dy_node_box *nodebox,
+                                                     TNode *tn, uint32_t *cnt_top, uint32_t *entries,
+                                                     uint32_t *counters /* [0] entries, [1] done */) {
+    __shared__ uint32_t lbox[2][kChunk][3];  // [0] leaves, [1] internal nodes
+    __shared__ uint32_t lmax[2][kChunk];
+    __shared__ uint32_t lcnt[kChunk];
+    __shared__ uint32_t is_last;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int base = static_cast<int>(blockIdx.x * kChunk);
+    const int k = base + tid;
+    lcnt[tid] = 0u;
+    __syncthreads();
+    if (k < n) {
+        const uint32_t body = sorted_idx[k];
+        const uint32_t sh = bv.shape[body];
+        float ex, ey, ez;
+        leaf_extent(sv.type[sh], sv.half_x[sh], sv.half_y[sh], sv.half_z[sh], bv.rot_w[body],
+                    bv.rot_x[body], bv.rot_y[body], bv.rot_z[body], ex, ey, ez);
+        const float px = bv.pos_x[body], py = bv.pos_y[body], pz = bv.pos_z[body];
+        uint32_t p[3];
+        p[0] = f32_to_f16(px - ex) | (static_cast<uint32_t>(f32_to_f16(py - ey)) << 16);
+        p[1] = f32_to_f16(pz - ez) | (static_cast<uint32_t>(f32_to_f16(px + ex)) << 16);
+        p[2] = f32_to_f16(py + ey) | (static_cast<uint32_t>(f32_to_f16(pz + ez)) << 16);
+        const int leaf = n - 1 + k;
+        store_node(nodebox, tn, leaf, p, body);
+        tn[leaf].child = static_cast<int32_t>(body);
+        lbox[0][tid][0] = p[0]; lbox[0][tid][1] = p[1]; lbox[0][tid][2] = p[2];
+        lmax[0][tid] = body;
+        int node = leaf;
+        for (int guard = 0;; ++guard) {
+            if (guard >= kMaxHeight) { atomicOr(&counters[2], 1u); break; }
+            const int par = parent[node];
+            if (par < 0) break;
+            const int2 rg = range[par];
+            if (rg.x < base || rg.y >= base + static_cast<int>(kChunk)) {
+                const uint32_t e = atomicAdd(&counters[0], 1u);
+                if (e < static_cast<uint32_t>(n)) entries[e] = static_cast<uint32_t>(node);
+                else atomicOr(&counters[2], 2u);
+                break;
+            }
+            const uint32_t old = __hip_atomic_fetch_add(&lcnt[par - base], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            const int c[2] = {left[par], right[par]};
+            uint32_t cb[2][3], cm[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const bool is_leaf = c[s] >= n - 1;
+                const int slot = is_leaf ? (c[s] - (n - 1) - base) : (c[s] - base);
+                const int kind = is_leaf ? 0 : 1;
+                cb[s][0] = lbox[kind][slot][0];
+                cb[s][1] = lbox[kind][slot][1];
+                cb[s][2] = lbox[kind][slot][2];
+                cm[s] = lmax[kind][slot];
+            }
+            uint32_t m[3];
+            merge_packed(cb[0], cb[1], m);
+            const uint32_t mb = cm[0] > cm[1] ? cm[0] : cm[1];
+            store_node(nodebox, tn, par, m, mb);
+            lbox[1][par - base][0] = m[0]; lbox[1][par - base][1] = m[1]; lbox[1][par - base][2] = m[2];
+            lmax[1][par - base] = mb;
+            node = par;
+        }
+    }
+    // Publish this chunk's nodes, then elect the last workgroup.
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t ticket = atomicAdd(&counters[1], 1u);
+        is_last = (ticket == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    uint32_t n_entries = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    n_entries = n_entries < static_cast<uint32_t>(n) ? n_entries : static_cast<uint32_t>(n);
+    for (uint32_t e = tid; e < n_entries; e += kChunk) {
+        int node = static_cast<int>(entries[e]);
+        for (int guard = 0;; ++guard) {
+            if (guard >= kMaxHeight) { atomicOr(&counters[2], 1u); break; }
+            const int par = parent[node];
+            if (par < 0) break;
+            const uint32_t old = __hip_atomic_fetch_add(&cnt_top[par], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            const int l = left[par], r = right[par];
+            const uint32_t a[3] = {tn[l].b01, tn[l].b23, tn[l].b45};
+            const uint32_t b[3] = {tn[r].b01, tn[r].b23, tn[r].b45};
+            uint32_t m[3];
+            merge_packed(a, b, m);
+            const uint32_t ml = tn[l].maxbody, mr = tn[r].maxbody;
+            store_node(nodebox, tn, par, m, ml > mr ? ml : mr);
+            node = par;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        counters[0] = 0u;
+        counters[1] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4-6. Traversal and ordered pair emission
+// ---------------------------------------------------------------------------
+// Insert x into the ascending register list s (keeps the kRow smallest).
+__device__ __forceinline__ void row_insert(uint32_t (&s)[kRow], uint32_t x) {
+#pragma unroll
+    for (uint32_t j = 0; j < kRow; ++j) {
+        const uint32_t lo = s[j] < x ? s[j] : x;
+        x = s[j] < x ? x : s[j];
+        s[j] = lo;
+    }
+}
+
+// Walk the tree for query box q; count partners with body index > thr and
+// keep the kRow smallest of them in s.  Returns the count.
+__device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, const Box6 &q, uint32_t thr,
+                                         uint32_t (&s)[kRow], uint32_t *err) {
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRow; ++j) s[j] = 0xffffffffu;
+    int node = 0;
+    // A depth-first walk visits each of the 2n-1 nodes at most once.
+    const int max_steps = 2 * n;
+    for (int steps = 0; node != kEnd; ++steps) {
+        if (steps >= max_steps || node < 0 || node >= 2 * n - 1) { atomicOr(err, 4u); break; }
+        const uint4 w0 = *reinterpret_cast<const uint4 *>(&tn[node]);
+        const int2 w1 = *reinterpret_cast<const int2 *>(&tn[node].child);
+        const Box6 b = unpack_box(w0.x, w0.y, w0.z);
+        if (w0.w > thr && overlaps(b, q)) {
+            if (node >= n - 1) {
+                ++count;
+                row_insert(s, static_cast<uint32_t>(w1.x));
+                node = w1.y;
+            } else {
+                node = w1.x;
+            }
+        } else {
+            node = w1.y;
+        }
+    }
+    return count;
+}
+
+__global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ tn, int n,
+                                                      uint32_t *__restrict__ counts, uint32_t *__restrict__ row,
+                                                      uint32_t *__restrict__ leaf_of, uint32_t *err) {
+    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    const TNode leaf = tn[n - 1 + k];
+    const uint32_t a = static_cast<uint32_t>(leaf.child);
+    const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
+    uint32_t s[kRow];
+    const uint32_t c = walk(tn, n, q, a, s, err);
+    counts[a] = c;
+    leaf_of[a] = static_cast<uint32_t>(k);
+    const uint32_t m = c < kRow ? c : kRow;
+    for (uint32_t j = 0; j < kRow; ++j)
+        if (j < m) row[static_cast<size_t>(a) * kRow + j] = s[j];
+}
+
+__global__ __launch_bounds__(256) void emit_pairs(const TNode *__restrict__ tn, int n,
+                                                  const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ offs, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ leaf_of,
+                                                  dy_contact_pair *__restrict__ pairs, uint32_t cap,
+                                                  uint32_t *err) {
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= static_cast<uint32_t>(n)) return;
+    const uint32_t L = counts[a];
+    if (L == 0) return;
+    const uint32_t off = offs[a];
+    if (off >= cap) return;
+    const uint32_t m = L < kRow ? L : kRow;
+    for (uint32_t j = 0; j < m; ++j) {
+        if (off + j < cap) pairs[off + j] = dy_contact_pair{a, row[static_cast<size_t>(a) * kRow + j]};
+    }
+    if (L <= kRow) return;
+    // Long row: fetch the next kRow partners above the last one written.
+    const TNode leaf = tn[n - 1 + leaf_of[a]];
+    const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
+    uint32_t thr = row[static_cast<size_t>(a) * kRow + kRow - 1];
+    uint32_t done = kRow;
+    uint32_t s[kRow];
+    while (done < L && off + done < cap) {
+        walk(tn, n, q, thr, s, err);
+        const uint32_t c = (L - done) < kRow ? (L - done) : kRow;
+        for (uint32_t j = 0; j < kRow; ++j)
+            if (j < c && off + done + j < cap) pairs[off + done + j] = dy_contact_pair{a, s[j]};
+        done += c;
+        thr = s[kRow - 1];
+    }
+}
+
+}  // namespace dy
+
+using namespace dy;
+
+struct dy_broadphase_s {
+    uint32_t max_bodies = 0, max_pairs = 0;
+    uint32_t *code0 = nullptr, *idx0 = nullptr, *code1 = nullptr, *idx1 = nullptr;
+    uint64_t *morton64 = nullptr;
+    int32_t *left = nullptr, *right = nullptr, *parent = nullptr, *root = nullptr;
+    int2 *range = nullptr;
+    dy_node_box *nodebox = nullptr;
+    TNode *tn = nullptr;
+    uint32_t *cnt_top = nullptr, *entries = nullptr, *counters = nullptr;
+    uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *leaf_of = nullptr;
+    uint32_t *scan_tmp = nullptr;
+    dy_contact_pair *pairs = nullptr;
+    uint32_t *count = nullptr;  // = counters + 3
+};
+
+namespace {
+void bp_free(dy_broadphase_s *b) {
+    void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
+                    b->root, b->range, b->nodebox, b->tn, b->cnt_top, b->entries, b->counters,
+                    b->counts, b->offs, b->row, b->leaf_of, b->scan_tmp, b->pairs};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+}
+template <typename T>
+hipError_t dalloc(T *&p, size_t n) {
+    return hipMalloc(reinterpret_cast<void **>(&p), (n ? n : 1) * sizeof(T));
+}
+}  // namespace
+
+extern "C" {
+
+int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs, dy_broadphase_t *out) {
+    if (!out) return DY_ERR_INVALID_ARG;
+    *out = nullptr;
+    dy_broadphase_s *b = new dy_broadphase_s;
+    b->max_bodies = max_bodies;
+    b->max_pairs = max_pairs;
+    const size_t nb = max_bodies > 1 ? max_bodies : 1;
+    const size_t nodes = 2 * nb - 1;
+    hipError_t e = hipSuccess;
+#define DY_A(ptr, cnt) \
+    if (e == hipSuccess) e = dalloc(ptr, cnt)
+    DY_A(b->code0, nb); DY_A(b->idx0, nb); DY_A(b->code1, nb); DY_A(b->idx1, nb);
+    DY_A(b->morton64, nb);
+    DY_A(b->left, nb); DY_A(b->right, nb); DY_A(b->parent, nodes); DY_A(b->root, 1);
+    DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes);
+    DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 4);
+    DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->leaf_of, nb);
+    DY_A(b->scan_tmp, scan_tmp_words(nb)); DY_A(b->pairs, max_pairs);
+#undef DY_A
+    if (e == hipSuccess) b->count = b->counters + 3;
+    if (e == hipSuccess) e = hipMemsetAsync(b->counters, 0, 4 * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) e = hipMemsetAsync(b->root, 0, sizeof(int32_t), hs(s));
+    if (e != hipSuccess) {
+        bp_free(b);
+        delete b;
+        return static_cast<int>(e);
+    }
+    *out = b;
+    return DY_OK;
+}
+
+int dy_broadphase_destroy(dy_broadphase_t bp) {
+    if (!bp) return DY_OK;
+    (void)hipDeviceSynchronize();
+    bp_free(bp);
+    delete bp;
+    return DY_OK;
+}
+
+int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_body_view *bodies,
+                                  const dy_shape_view *shapes, const dy_aabb *bounds) {
+    if (!bp || !bodies || !shapes || !bounds) return DY_ERR_INVALID_ARG;
+    const uint32_t n = bodies->n;
+    if (n > bp->max_bodies || n >= kMortonMax) return DY_ERR_INVALID_ARG;
+    hipStream_t st = hs(s);
+    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 2 * sizeof(uint32_t), st));
+    if (n < 2) return DY_OK;
+    const int ni = static_cast<int>(n);
+    morton_sort<<<1, kMsThreads, 0, st>>>(*bodies, *bounds, n, bp->code0, bp->idx0, bp->code1, bp->idx1,
+                                          bp->morton64);
+    DY_LAUNCH_CHECK();
+    karras_build<<<ceil_div(n - 1, 256), 256, 0, st>>>(bp->morton64, ni, bp->left, bp->right, bp->parent,
+                                                       bp->range, bp->tn, bp->cnt_top, bp->root);
+    DY_LAUNCH_CHECK();
+    leaf_refit<<<ceil_div(n, kChunk), kChunk, 0, st>>>(*bodies, *shapes, ni, bp->idx0, bp->parent, bp->left,
+                                                       bp->right, bp->range, bp->nodebox, bp->tn, bp->cnt_top,
+                                                       bp->entries, bp->counters);
+    DY_LAUNCH_CHECK();
+    traverse_count<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->row, bp->leaf_of,
+                                                     bp->counters + 2);
+    DY_LAUNCH_CHECK();
+    DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
+    emit_pairs<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->leaf_of,
+                                                 bp->pairs, bp->max_pairs, bp->counters + 2);
+    DY_LAUNCH_CHECK();
+    return DY_OK;
+}
+
+int dy_broadphase_pairs(dy_broadphase_t bp, const dy_contact_pair **pairs, const uint32_t **d_count,
+                        uint32_t *max_pairs) {
+    if (!bp) return DY_ERR_INVALID_ARG;
+    if (pairs) *pairs = bp->pairs;
+    if (d_count) *d_count = bp->count;
+    if (max_pairs) *max_pairs = bp->max_pairs;
+    return DY_OK;
+}
+
+int dy_broadphase_sort_pairs(dy_broadphase_t bp, dy_stream_t, uint32_t) {
+    return bp ? DY_OK : DY_ERR_INVALID_ARG;
+}
+
+int dy_broadphase_download_count(dy_broadphase_t bp, dy_stream_t s, uint32_t *out) {
+    if (!bp || !out) return DY_ERR_INVALID_ARG;
+    uint32_t h[2];
+    DY_TRY(hipMemcpyAsync(h, bp->counters + 2, sizeof(h), hipMemcpyDeviceToHost, hs(s)));
+    DY_TRY(hipStreamSynchronize(hs(s)));
+    *out = h[1];
+    return h[0] ? DY_ERR_INTERNAL : DY_OK;
+}
+
+int dy_broadphase_bvh_view(dy_broadphase_t bp, uint32_t n_active, dy_bvh_view *out) {
+    if (!bp || !out) return DY_ERR_INVALID_ARG;
+    out->left = bp->left;
+    out->right = bp->right;
+    out->parent = bp->parent;
+    out->root = bp->root;
+    out->sorted_idx = bp->idx0;
+    out->aabb = bp->nodebox;
+    out->n = n_active;
+    return DY_OK;
+}
+
+// Morton keys of the last build (leaf order); for parity tests.  Async.
+int dy_broadphase_morton_keys(dy_broadphase_t bp, const uint64_t **keys) {
+    if (!bp || !keys) return DY_ERR_INVALID_ARG;
+    *keys = bp->morton64;
+    return DY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// dy_internal.hpp — helpers shared by the HIP translation units of
+// libdyphur_hip.so (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dyphur_hip.h"
+#include "dy_math.hpp"
+
+#define DY_TRY(expr)                                          \
+    do {                                                      \
+        hipError_t dy_e_ = (expr);                            \
+        if (dy_e_ != hipSuccess) return static_cast<int>(dy_e_); \
+    } while (0)
+
+#define DY_TRY_STATUS(expr)              \
+    do {                                 \
+        int dy_s_ = (expr);              \
+        if (dy_s_ != DY_OK) return dy_s_; \
+    } while (0)
+
+#define DY_LAUNCH_CHECK() DY_TRY(hipGetLastError())
+
+namespace dy {
+
+inline hipStream_t hs(dy_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Device buffer that grows on demand (contents are not preserved).
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n) {
+        if (n <= cap) return DY_OK;
+        if (p) DY_TRY(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        size_t c = n < 64 ? 64 : n;
+        DY_TRY(hipMalloc(reinterpret_cast<void **>(&p), c * sizeof(T)));
+        cap = c;
+        return DY_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Primitives implemented in primitives.hip, used by the stage translation units.
+// Exclusive scan of n u32 values; d_total receives the sum (may be null).
+// `tmp` must hold at least scan_tmp_words(n) u32.
+size_t scan_tmp_words(size_t n);
+int scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total,
+                   uint32_t *tmp);
+
+// Stable LSD radix sort in one workgroup of (key, value) pairs over key bits
+// [begin_bit, end_bit).  Keys/values in (k0, v0); (k1, v1) are scratch of the
+// same size.  The result ends in (k0, v0).  vals may be null for key-only sorts.
+int radix_sort_u32(hipStream_t s, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n,
+                   int begin_bit, int end_bit);
+int radix_sort_u64(hipStream_t s, uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n,
+                   int begin_bit, int end_bit);
+
+// Block-wide helpers ----------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Inclusive scan across the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (static_cast<int>(lane_id()) >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across a workgroup of `nthreads` (multiple of 64, <= 1024).
+// `wsum` is LDS scratch of at least 16 words.  Returns the exclusive prefix;
+// `total` receives the block sum.  Contains __syncthreads().
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *wsum, uint32_t &total) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nw = blockDim.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < nw ? wsum[lane] : 0u;
+        uint32_t wi = wave_inclusive_scan(w);
+        if (lane < nw) wsum[lane] = wi - w;
+        if (lane == nw - 1) wsum[16] = wi;
+    }
+    __syncthreads();
+    total = wsum[16];
+    const uint32_t r = wsum[wave] + inc - v;
+    __syncthreads();
+    return r;
+}
+
+}  // namespace dy

@@ -1,0 +1,240 @@
+/*
+ * dyphur_hip.h — C ABI of the gfx950 frame step (libdyphur_hip.so).
+ *
+ * The reference exposes its path as C++ classes over a SYCL queue; this ABI
+ * sits beneath an unchanged C++ surface (compute::Device/Stream/Buffer,
+ * core::Broadphase/Narrowphase/XpbdSolver/integrate) and beneath the Python
+ * module.  Plain pointers and sizes only.  Every function returns an int
+ * status: 0 on success, a positive hipError_t from the runtime, or one of the
+ * negative DY_ERR_* codes below.
+ *
+ * Memory: all dy_*_view pointers are device pointers obtained from dy_malloc
+ * (or any hipMalloc'd memory).  Stage objects own their scratch and are bound
+ * to one stream; they are not re-entrant.
+ *
+ * Blocking calls (host waits for the stream): dy_memcpy_d2h, dy_stream_synchronize,
+ * dy_event_synchronize, dy_reduce_sum_f32, *_download_count, dy_narrowphase_last_count,
+ * dy_state_hash_fnv1a64, dy_xpbd_solve (it reads the device contact count and the
+ * colour count, as the reference solver does).  Everything else is asynchronous
+ * on the given stream, in stream order.
+ */
+#ifndef DYPHUR_HIP_H
+#define DYPHUR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DY_OK 0
+#define DY_ERR_INVALID_ARG (-1)  /* null/oversized argument, capacity exceeded */
+#define DY_ERR_UNSUPPORTED (-2)  /* joints, convex-hull or triangle-mesh shapes */
+#define DY_ERR_NO_DEVICE   (-3)  /* no HIP device */
+#define DY_ERR_INTERNAL    (-4)  /* a device-side consistency check failed */
+
+/* Body flags: core/include/core/body.hpp:8-12 */
+#define DY_BODY_STATIC    1u
+#define DY_BODY_KINEMATIC 2u
+#define DY_BODY_SLEEPING  4u
+
+/* Shape types: core/include/core/shapes.hpp:6-11 (2 and 3 are rejected) */
+#define DY_SHAPE_BOX    0u
+#define DY_SHAPE_SPHERE 1u
+
+/* SoA body view; layout identical to dyphur::BodyView (core/include/core/body.hpp:30-46). */
+typedef struct dy_body_view {
+    float *pos_x, *pos_y, *pos_z;
+    float *rot_w, *rot_x, *rot_y, *rot_z;
+    float *vel_x, *vel_y, *vel_z;
+    float *ang_x, *ang_y, *ang_z;
+    float *inv_mass;
+    float *iI_xx, *iI_yy, *iI_zz;
+    float *iI_xy, *iI_xz, *iI_yz;
+    uint32_t *shape;
+    uint32_t *flags;
+    uint32_t n;
+} dy_body_view;
+
+/* Shape catalog; layout identical to dyphur::ShapeView (core/include/core/shapes.hpp:22-28). */
+typedef struct dy_shape_view {
+    const uint32_t *type;
+    const float *half_x, *half_y, *half_z;
+    const uint32_t *ext_id;
+    uint32_t n;
+} dy_shape_view;
+
+/* Contact store view; layout identical to dyphur::ContactView (core/include/core/contact.hpp:8-20). */
+typedef struct dy_contact_view {
+    uint32_t *body_a, *body_b;
+    float *pos_x, *pos_y, *pos_z;
+    float *norm_x, *norm_y, *norm_z;
+    float *depth;
+    uint32_t *n;        /* device pointer: live contact count */
+    uint32_t capacity;
+} dy_contact_view;
+
+/* core/include/core/aabb.hpp:7-31 */
+typedef struct dy_aabb { float min_x, min_y, min_z, max_x, max_y, max_z; } dy_aabb;
+
+/* core/include/core/broadphase.hpp:12-14: a < b always. */
+typedef struct dy_contact_pair { uint32_t a, b; } dy_contact_pair;
+
+/* core/include/core/broadphase.hpp:21-25: 6 binary16 bounds + 2 zero pads, 16 B. */
+typedef struct dy_node_box { uint16_t mn_x, mn_y, mn_z, mx_x, mx_y, mx_z, pad0, pad1; } dy_node_box;
+
+/* core/include/core/broadphase.hpp:30-38: BVH internals for ray queries. */
+typedef struct dy_bvh_view {
+    const int32_t *left;        /* internal nodes [0, n-2] */
+    const int32_t *right;       /* internal nodes [0, n-2] */
+    const int32_t *parent;      /* all nodes [0, 2n-2], -1 for the root */
+    const int32_t *root;        /* 1 element */
+    const uint32_t *sorted_idx; /* leaf k -> body index */
+    const dy_node_box *aabb;    /* all nodes [0, 2n-2] */
+    uint32_t n;
+} dy_bvh_view;
+
+typedef struct dy_device_info {
+    char name[256];
+    int is_gpu;
+    int arch_major, arch_minor; /* gcnArch, e.g. 9, 5 for gfx950 */
+    size_t compute_units;
+    size_t global_mem_bytes;
+} dy_device_info;
+
+typedef struct dy_solve_stats {
+    uint32_t n_constraints;  /* contacts solved (joints are rejected) */
+    uint32_t used_serial;    /* 1: reference-order (run_serial) semantics */
+    uint32_t n_classes;      /* colours (coloured path) or dependency levels (serial path) */
+    uint32_t rounds;         /* colouring rounds the reference would run (max dependency depth) */
+    uint32_t overflow;       /* 1: a body would need more than 64 colours -> serial fallback */
+} dy_solve_stats;
+
+typedef struct dy_stream_s *dy_stream_t;        /* a hipStream_t */
+typedef struct dy_event_s *dy_event_t;          /* a hipEvent_t */
+typedef struct dy_broadphase_s *dy_broadphase_t;
+typedef struct dy_narrowphase_s *dy_narrowphase_t;
+typedef struct dy_xpbd_s *dy_xpbd_t;
+
+/* ---- runtime (compute/include/compute/{device,stream,buffer}.hpp) -------- */
+const char *dy_status_string(int status);
+int dy_version(int *major, int *minor);
+/* Device::default_gpu / name / is_gpu / max_compute_units / global_mem_bytes (device.hpp:12-37) */
+int dy_device_count(int *count);
+int dy_device_get_info(int device, dy_device_info *out);
+int dy_set_device(int device);
+/* Stream(const sycl::device&) in-order queue; Stream::wait (stream.hpp:18-30) */
+int dy_stream_create(dy_stream_t *out);
+int dy_stream_destroy(dy_stream_t s);
+int dy_stream_synchronize(dy_stream_t s);
+/* Buffer<T>(Stream&, n) / ~Buffer (buffer.hpp:15-19) */
+int dy_malloc(void **ptr, size_t bytes);
+int dy_free(void *ptr);
+/* Buffer::upload (async), Buffer::download (blocking), queue().memcpy/memset (buffer.hpp:42-49) */
+int dy_memcpy_h2d_async(void *dst, const void *src, size_t bytes, dy_stream_t s);
+int dy_memcpy_d2h(void *dst, const void *src, size_t bytes, dy_stream_t s);
+int dy_memcpy_d2d_async(void *dst, const void *src, size_t bytes, dy_stream_t s);
+int dy_memset_async(void *dst, int value, size_t bytes, dy_stream_t s);
+/* Event (stream.hpp:6-14), plus timing */
+int dy_event_create(dy_event_t *out);
+int dy_event_destroy(dy_event_t e);
+int dy_event_record(dy_event_t e, dy_stream_t s);
+int dy_event_synchronize(dy_event_t e);
+int dy_event_elapsed_ms(dy_event_t start, dy_event_t stop, float *ms);
+
+/* ---- compute primitives ------------------------------------------------- */
+/* sort_by_key / sort (compute/include/compute/sort.hpp:29,89): ascending, stable.
+ * Unlike the reference, n need not be a power of two.  Async. */
+int dy_sort_by_key_u64_u32(dy_stream_t s, uint64_t *keys, uint32_t *vals, size_t n);
+int dy_sort_by_key_u32_u32(dy_stream_t s, uint32_t *keys, uint32_t *vals, size_t n);
+int dy_sort_u32(dy_stream_t s, uint32_t *keys, size_t n);
+int dy_sort_u64(dy_stream_t s, uint64_t *keys, size_t n);
+/* reduce_sum (compute/include/compute/reduction.hpp:22-62): deterministic
+ * fixed-order tree; blocking, result to host. */
+int dy_reduce_sum_f32(dy_stream_t s, const float *d_data, size_t n, float *out);
+/* Exclusive prefix sum of u32 with the total written to d_total (may be NULL). Async. */
+int dy_scan_exclusive_u32(dy_stream_t s, const uint32_t *d_in, uint32_t *d_out, size_t n,
+                          uint32_t *d_total);
+
+/* ---- host set-up --------------------------------------------------------- */
+/* BodyStore::add dynamics (core/src/body_store.cpp:121-139, mat3.hpp:61-79) for n
+ * bodies: mass[n], inertia[9n] row-major, flags_in[n] -> inv_mass[n], iI[6n]
+ * (xx yy zz xy xz yz interleaved per body), flags_out[n].  Host memory, no GPU. */
+int dy_body_dynamics_host(uint32_t n, const float *mass, const float *inertia,
+                          const uint32_t *flags_in, float *inv_mass, float *iI,
+                          uint32_t *flags_out);
+
+/* ---- stages --------------------------------------------------------------- */
+/* integrate (core/include/core/integrator.hpp:25, core/src/integrator.cpp:7-71).
+ * All substeps in one launch (each body's substeps depend only on itself). */
+int dy_integrate(dy_stream_t s, const dy_body_view *bodies, float gx, float gy, float gz,
+                 float dt, int n_substeps);
+
+/* Broadphase(Stream&, max_bodies, max_pairs) (core/include/core/broadphase.hpp:57) */
+int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
+                         dy_broadphase_t *out);
+int dy_broadphase_destroy(dy_broadphase_t bp);
+/* build_and_query (broadphase.hpp:62-65): Morton sort, Karras tree, fp16 refit,
+ * and the candidate pairs, written in ascending (a, b) order.  No host sync.
+ * Pairs beyond max_pairs are dropped (the first max_pairs in (a, b) order are
+ * kept); the device count holds the true number. */
+int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_body_view *bodies,
+                                  const dy_shape_view *shapes, const dy_aabb *scene_bounds);
+/* pairs_ptr / count_ptr / max_pairs (broadphase.hpp:68-70) */
+int dy_broadphase_pairs(dy_broadphase_t bp, const dy_contact_pair **pairs, const uint32_t **d_count,
+                        uint32_t *max_pairs);
+/* sort_pairs (broadphase.hpp:75): the pairs are already in canonical order, so
+ * this only validates its argument. */
+int dy_broadphase_sort_pairs(dy_broadphase_t bp, dy_stream_t s, uint32_t n_pairs);
+/* download_count (broadphase.hpp:78): blocking; the true count. */
+int dy_broadphase_download_count(dy_broadphase_t bp, dy_stream_t s, uint32_t *out);
+/* bvh_view (broadphase.hpp:82) */
+int dy_broadphase_bvh_view(dy_broadphase_t bp, uint32_t n_active, dy_bvh_view *out);
+
+/* Narrowphase(Stream&, max_contacts) (core/include/core/narrowphase.hpp:29) */
+int dy_narrowphase_create(dy_stream_t s, uint32_t max_contacts, dy_narrowphase_t *out);
+int dy_narrowphase_destroy(dy_narrowphase_t np);
+/* run(s, pairs, n_pairs, bodies, shapes) (narrowphase.hpp:33-37): one contact per
+ * colliding pair, stored in pair order; no host sync.  Contacts beyond capacity
+ * are dropped in pair order; the true count is kept. */
+int dy_narrowphase_run(dy_narrowphase_t np, dy_stream_t s, const dy_contact_pair *d_pairs,
+                       uint32_t n_pairs, const dy_body_view *bodies, const dy_shape_view *shapes);
+/* run(s, pairs, d_n_pairs, ...) (narrowphase.hpp:42-46): the pair count stays on
+ * the device; max_pairs bounds the launch (entries beyond *d_n_pairs are ignored). */
+int dy_narrowphase_run_device_count(dy_narrowphase_t np, dy_stream_t s, const dy_contact_pair *d_pairs,
+                                    const uint32_t *d_n_pairs, uint32_t max_pairs,
+                                    const dy_body_view *bodies, const dy_shape_view *shapes);
+/* contacts() (narrowphase.hpp:48) */
+int dy_narrowphase_contacts(dy_narrowphase_t np, dy_contact_view *out);
+/* download_count (narrowphase.hpp:49): blocking; stored count.  Returns
+ * DY_ERR_UNSUPPORTED if the last run met a hull or mesh shape. */
+int dy_narrowphase_download_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out);
+/* last_contact_count (narrowphase.hpp:57): blocking here; the true count. */
+int dy_narrowphase_last_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out);
+
+/* XpbdSolver(Stream&, n_iters, max_contacts, friction) (core/include/core/xpbd_solver.hpp:31-34) */
+int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float friction, dy_xpbd_t *out);
+int dy_xpbd_destroy(dy_xpbd_t x);
+/* set_force_serial (xpbd_solver.hpp:48) */
+int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial);
+/* solve(s, contacts, joints, bodies, dt) (core/include/core/solver.hpp:17-21).
+ * n_joints must be 0 (joints are out of scope: DY_ERR_UNSUPPORTED otherwise).
+ * <= 256 contacts or force_serial: reference contact order (run_serial
+ * semantics), executed in parallel over dependency levels; otherwise the
+ * reference's Jones-Plassmann colours, swept in colour order.  Blocking (reads
+ * the contact count and the class count). */
+int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *contacts, uint32_t n_joints,
+                  const dy_body_view *bodies, float dt);
+int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out);
+/* Class (colour or level) of each contact of the last solve; blocking. */
+int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n);
+
+/* FNV-1a-64 over pos xyz, rot wxyz of all bodies
+ * (examples/stress_test_blocks/main.cpp:34-41,301-311); blocking. */
+int dy_state_hash_fnv1a64(dy_stream_t s, const dy_body_view *bodies, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYPHUR_HIP_H */

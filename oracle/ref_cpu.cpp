@@ -1089,6 +1089,62 @@ void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *boun
     if (st) { st->n_pairs = np; st->n_contacts = nc; st->n_colors = ss.n_colors; st->used_serial = ss.used_serial; }
 }
 
+int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                    const float *pb, const float *qa, const float *qb, float *out) {
+    float px[2] = {pa[0], pb[0]}, py[2] = {pa[1], pb[1]}, pz[2] = {pa[2], pb[2]};
+    float rw[2] = {qa[0], qb[0]}, rx[2] = {qa[1], qb[1]}, ry[2] = {qa[2], qb[2]}, rz[2] = {qa[3], qb[3]};
+    uint32_t shape[2] = {0, 1}, flags[2] = {0, 0};
+    rc_bodies b{};
+    b.pos_x = px; b.pos_y = py; b.pos_z = pz;
+    b.rot_w = rw; b.rot_x = rx; b.rot_y = ry; b.rot_z = rz;
+    b.shape = shape; b.flags = flags; b.n = 2;
+    uint32_t type[2] = {ta, tb}, ext[2] = {0, 0};
+    float hx[2] = {ha[0], hb[0]}, hy[2] = {ha[1], hb[1]}, hz[2] = {ha[2], hb[2]};
+    rc_shapes sh{type, hx, hy, hz, ext, 2};
+    Hit h{};
+    if (!contact_for_pair(b, sh, 0, 1, h)) return 0;
+    out[0] = h.px; out[1] = h.py; out[2] = h.pz;
+    out[3] = h.nx; out[4] = h.ny; out[5] = h.nz;
+    out[6] = h.depth;
+    return 1;
+}
+
+void rc_project_contact(const float *contact, float *a, float *b, float mu, float *lam, float dt) {
+    float f[20][2];
+    for (int k = 0; k < 20; ++k) { f[k][0] = a[k]; f[k][1] = b[k]; }
+    uint32_t shape[2] = {0, 0}, flags[2] = {0, 0};
+    rc_bodies bv{};
+    float **dst[20] = {&bv.pos_x, &bv.pos_y, &bv.pos_z, &bv.rot_w, &bv.rot_x, &bv.rot_y, &bv.rot_z,
+                       &bv.vel_x, &bv.vel_y, &bv.vel_z, &bv.ang_x, &bv.ang_y, &bv.ang_z, &bv.inv_mass,
+                       &bv.iI_xx, &bv.iI_yy, &bv.iI_zz, &bv.iI_xy, &bv.iI_xz, &bv.iI_yz};
+    for (int k = 0; k < 20; ++k) *dst[k] = f[k];
+    bv.shape = shape; bv.flags = flags; bv.n = 2;
+    ContactSoA cs;
+    cs.resize(1);
+    cs.a[0] = 0; cs.b[0] = 1;
+    cs.px[0] = contact[0]; cs.py[0] = contact[1]; cs.pz[0] = contact[2];
+    cs.nx[0] = contact[3]; cs.ny[0] = contact[4]; cs.nz[0] = contact[5];
+    cs.depth[0] = contact[6];
+    project_contact(0, cs, bv, mu, lam, dt);
+    for (int k = 0; k < 20; ++k) { a[k] = f[k][0]; b[k] = f[k][1]; }
+}
+
+void rc_leaf_extent(uint32_t stype, const float *he, const float *q, float *out) {
+    if (stype == kShapeSphere) { out[0] = out[1] = out[2] = he[0]; return; }
+    const Rot3 R = leaf_rotation(q[0], q[1], q[2], q[3]);
+    out[0] = std::fabs(R.m[0][0]) * he[0] + std::fabs(R.m[0][1]) * he[1] + std::fabs(R.m[0][2]) * he[2];
+    out[1] = std::fabs(R.m[1][0]) * he[0] + std::fabs(R.m[1][1]) * he[1] + std::fabs(R.m[1][2]) * he[2];
+    out[2] = std::fabs(R.m[2][0]) * he[0] + std::fabs(R.m[2][1]) * he[1] + std::fabs(R.m[2][2]) * he[2];
+}
+
+void rc_karras_node(const uint64_t *codes, int n, int i, int32_t *out) {
+    int lo, hi, g;
+    karras_node(codes, n, i, lo, hi, g);
+    out[0] = lo; out[1] = hi; out[2] = g;
+    out[3] = (lo == g) ? (n - 1 + lo) : g;
+    out[4] = (g + 1 == hi) ? (n - 1 + g + 1) : (g + 1);
+}
+
 int rc_max_threads(void) { return omp_get_max_threads(); }
 void rc_set_threads(int n) { omp_set_num_threads(n); }
 

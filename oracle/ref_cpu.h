@@ -135,6 +135,19 @@ void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *boun
               float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
               rc_frame_stats *st);
 
+/* ---- per-element entry points (for function-level parity checks) ----- */
+/* Contact of one pair (narrowphase.cpp:920-1000): ha/hb half extents, pa/pb
+ * centres, qa/qb quaternions wxyz.  out[7] = p xyz, n xyz, depth; returns 1 on hit. */
+int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                    const float *pb, const float *qa, const float *qb, float *out);
+/* One solve_contact (xpbd_solver.cpp:413-521) on two bodies given as 20 floats
+ * (pos xyz, rot wxyz, vel xyz, ang xyz, inv_mass, iI xx yy zz xy xz yz). */
+void rc_project_contact(const float *contact, float *a, float *b, float mu, float *lam, float dt);
+/* Leaf AABB half extents of one body (broadphase.cpp:222-244). */
+void rc_leaf_extent(uint32_t stype, const float *half3, const float *q4, float *out3);
+/* Karras node i (broadphase.cpp:164-174): out = lo, hi, split, left, right. */
+void rc_karras_node(const uint64_t *codes, int n, int i, int32_t *out);
+
 int rc_max_threads(void);
 void rc_set_threads(int n);
 

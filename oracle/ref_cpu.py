@@ -107,6 +107,11 @@ def lib():
         L.rc_frame.argtypes = [vp, C.POINTER(RcBodies), C.POINTER(RcShapes), C.POINTER(RcAabb),
                                C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                                C.POINTER(RcFrameStats)]
+        L.rc_pair_contact.restype = C.c_int
+        L.rc_pair_contact.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
+        L.rc_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
+        L.rc_leaf_extent.argtypes = [C.c_uint32, vp, vp, vp]
+        L.rc_karras_node.argtypes = [vp, C.c_int, C.c_int, vp]
         L.rc_max_threads.restype = C.c_int
         L.rc_set_threads.argtypes = [C.c_int]
         _lib = L

@@ -1,0 +1,186 @@
+"""gfx950 kernels vs the CPU oracle, through the C ABI, on identical inputs.
+
+Bar: bit-exact for every integer/index output (Morton keys, sorted order, BVH
+topology, fp16 node boxes, pair list, contact order) and for the float outputs
+too (contacts, solver state, whole-frame state hash): both sides run the same
+IEEE operation sequence with no contraction, so any difference is a bug.
+"""
+import numpy as np
+import pytest
+
+from dyphur_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+U32 = np.uint32
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _settled_state(oracle, sc, frames):
+    """Advance the oracle `frames` frames; returns (oracle Bodies, shapes)."""
+    b, s = oracle.Bodies(sc), oracle.Shapes(sc)
+    w = oracle.World.for_scene(sc)
+    for _ in range(frames):
+        w.frame(b, s, sc)
+    return b, s
+
+
+def _gpu_bodies(gpu, sc, s, state):
+    d = gpu.DeviceBodies(sc, s)
+    d.upload_state(state, s)
+    return d
+
+
+CASES = [
+    ("stress_f0", lambda: scenes.stress_blocks(), 0),
+    ("stress_f90", lambda: scenes.stress_blocks(), 90),
+    ("stress_f140", lambda: scenes.stress_blocks(), 140),
+    ("mixed_3000", lambda: scenes.random_mixed(3000, seed=11, extent=(12, 10, 12)), 0),
+    ("mixed_3000_f30", lambda: scenes.random_mixed(3000, seed=12, extent=(12, 10, 12)), 30),
+]
+
+
+@pytest.fixture(scope="module")
+def stage_inputs(oracle):
+    out = {}
+    for name, mk, frames in CASES:
+        sc = mk()
+        b, sh = _settled_state(oracle, sc, frames)
+        out[name] = (sc, b, sh)
+    return out
+
+
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+def test_broadphase_bitexact(gpu, oracle, stage_inputs, case):
+    sc, ob, osh = stage_inputs[case]
+    print(f"[broadphase] {case} n={sc.n}", flush=True)
+    w = oracle.World.for_scene(sc)
+    n_ref = w.build_and_query(ob, osh, sc.bounds)
+    ref = w.bvh(sc.n)
+    w.sort_pairs(n_ref)
+    ref_pairs = w.pairs(n_ref)
+
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob.state())
+    ds = gpu.DeviceShapes(sc, s)
+    bp = gpu.Broadphase(s, sc.n, sc.max_pairs)
+    bp.build_and_query(s, db, ds, sc.bounds)
+    n = bp.download_count(s)
+    got = bp.bvh(s, sc.n)
+    assert np.array_equal(got["morton"], ref["morton"])
+    assert np.array_equal(got["sorted_idx"], ref["sorted_idx"])
+    assert got["root"][0] == ref["root"][0] == 0
+    assert np.array_equal(got["left"], ref["left"])
+    assert np.array_equal(got["right"], ref["right"])
+    assert np.array_equal(got["parent"], ref["parent"])
+    assert np.array_equal(got["aabb"], ref["aabb"])
+    assert n == n_ref
+    assert np.array_equal(bp.pairs(s, n), ref_pairs)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+def test_narrowphase_bitexact(gpu, oracle, stage_inputs, case):
+    sc, ob, osh = stage_inputs[case]
+    print(f"[narrowphase] {case}", flush=True)
+    w = oracle.World.for_scene(sc)
+    n_pairs = w.build_and_query(ob, osh, sc.bounds)
+    w.sort_pairs(n_pairs)
+    pairs = w.pairs(n_pairs)
+    nc_ref = w.narrowphase(pairs, ob, osh)
+    ref = w.contacts()
+
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob.state())
+    ds = gpu.DeviceShapes(sc, s)
+    dp = gpu.DeviceArray.from_numpy(pairs.ravel(), s)
+    np_ = gpu.Narrowphase(s, sc.max_contacts)
+    np_.run(s, dp.ptr, len(pairs), db, ds)
+    got = np_.contacts(s)
+    assert got["n"] == nc_ref == ref["n"]
+    assert np_.last_count(s) == ref["last_count"]
+    for f, _ in gpu.CONTACT_FIELDS:
+        assert np.array_equal(bits(got[f]), bits(ref[f])), f
+
+
+def _contacts_for(oracle, sc, ob, osh):
+    w = oracle.World.for_scene(sc)
+    n_pairs = w.build_and_query(ob, osh, sc.bounds)
+    w.sort_pairs(n_pairs)
+    w.narrowphase(w.pairs(n_pairs), ob, osh)
+    return w
+
+
+@pytest.mark.parametrize("case", ["stress_f90", "stress_f140", "mixed_3000", "mixed_3000_f30"])
+@pytest.mark.parametrize("force_serial", [False, True])
+def test_solver_bitexact(gpu, oracle, stage_inputs, case, force_serial):
+    sc, ob0, osh = stage_inputs[case]
+    print(f"[solver] {case} serial={force_serial}", flush=True)
+    ob = ob0.copy()
+    w = _contacts_for(oracle, sc, ob, osh)
+    c = w.contacts()
+    st_ref = w.solve(ob, sc.dt, force_serial=force_serial)
+    colors_ref = w.colors(c["n"]) if not st_ref["used_serial"] else None
+
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob0.state())
+    ds = gpu.DeviceShapes(sc, s)
+    np_ = gpu.Narrowphase(s, sc.max_contacts)
+    pairs = np.stack([c["body_a"], c["body_b"]], 1).astype(U32)
+    # Feed the identical contact list (bit-identical narrowphase is tested above).
+    dp = gpu.DeviceArray.from_numpy(pairs.ravel(), s)
+    np_.run(s, dp.ptr, len(pairs), db, ds)
+    assert np_.download_count(s) == c["n"]
+    solver = gpu.XpbdSolver(s, sc.n_iters, sc.max_contacts, sc.mu)
+    solver.set_force_serial(force_serial)
+    solver.solve(s, np_.contacts_view(), db, sc.dt)
+    st = solver.stats()
+    assert st["used_serial"] == st_ref["used_serial"]
+    if colors_ref is not None:
+        assert st["n_classes"] == st_ref["n_colors"]
+        assert np.array_equal(solver.classes(s, c["n"]), colors_ref)
+    got = db.download(s)
+    ref = ob.state()
+    for f in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
+              "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z"):
+        assert np.array_equal(bits(got[f]), bits(ref[f])), f
+
+
+def test_golden_box_hash_on_gpu(gpu):
+    # core/tests/sim_determinism_test.cpp:224 (OMP golden), 20 frames.
+    sc = scenes.determinism_boxes()
+    hs = []
+    for _ in range(2):
+        w = gpu.World(sc)
+        for _ in range(20):
+            w.step()
+        hs.append(w.fnv())
+    assert hs[0] == hs[1] == 0xCBCD209C3665C818
+
+
+def test_stress_frames_match_oracle(gpu, oracle):
+    sc = scenes.stress_blocks()
+    frames = 150
+    b, s = oracle.Bodies(sc), oracle.Shapes(sc)
+    ow = oracle.World.for_scene(sc)
+    gw = gpu.World(sc)
+    for f in range(frames):
+        ost = ow.frame(b, s, sc)
+        gst = gw.step()
+        assert gst["n_pairs"] == ost["n_pairs"], f
+        assert gst["n_contacts"] == ost["n_contacts"], f
+        if f % 10 == 9 or f == frames - 1:
+            assert gw.fnv() == b.fnv(), f"frame {f}"
+
+
+def test_coloured_run_to_run_identical(gpu):
+    sc = scenes.random_mixed(4000, seed=5, extent=(14, 12, 14))
+    hs = []
+    for _ in range(2):
+        w = gpu.World(sc)
+        for _ in range(40):
+            w.step()
+        hs.append(w.fnv())
+    assert hs[0] == hs[1]
