@@ -9,9 +9,9 @@
 //    each constraint the lowest colour unused by its higher-priority
 //    neighbours, i.e. greedy colouring in decreasing priority order; the round
 //    in which the reference colours k is the length of the longest
-//    decreasing-priority chain ending at k.  assign_round computes exactly
-//    that (colour and depth) with one pass per round that may also finish
-//    deeper constraints early — the values do not depend on timing.  The
+//    decreasing-priority chain ending at k.  Both are computed here as a
+//    dataflow over the conflict graph (each contact waits for its
+//    higher-priority neighbours), so the values do not depend on timing.  The
 //    reference's serial fallbacks (a body needing > 64 colours, or more than
 //    256 rounds) are detected from the same data.
 //  * Reference-order path (<= 256 contacts or force_serial): run_serial
@@ -20,12 +20,20 @@
 //    earlier contact on either dynamic body) reproduces the serial result bit
 //    for bit while running each level in parallel.
 //
+// Data layout during a solve: bodies are gathered into 64 B dynamic + 32 B
+// static AoS records (one or two 128-bit loads per field group instead of 20
+// scattered SoA loads), contacts into class-sorted 40 B records.  All I x K
+// phases then run in one cooperative persistent kernel separated by a grid
+// barrier; the minimum number of dependent phases is I x (max contacts on one
+// body), so the per-phase cost is what matters.
+//
 // Quaternion renormalisation after every iteration is applied lazily: a
 // dynamic body is brought to `it` renormalisations when a contact of
 // iteration `it` first touches it, and every body to n_iters at the end.
 // Static bodies are never written by contacts and their orientation only
 // enters products with a zero inverse inertia, so they are renormalised at
 // the end.  Every body ends with the reference's sequence of operations.
+#include <cstdlib>
 #include <vector>
 
 #include "dy_internal.hpp"
@@ -34,29 +42,31 @@ namespace dy {
 
 constexpr uint64_t kUnassigned = ~0ull;
 constexpr uint32_t kSolveThreads = 256;
-constexpr uint32_t kMetaWords = 16;   // [0] remaining [1] overflow [2] max class [3] max depth
-constexpr uint32_t kHistDownload = 256;
+// meta: [0] remaining [1] colour overflow [2] max class [3] max depth [4] error flags
+constexpr uint32_t kMetaWords = 16;
+constexpr int kOwn = 4;                  // contacts per thread in the dataflow colouring
+constexpr uint32_t kLdsClasses = 2048;   // block-aggregated class counting limit
+constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t kErrTimeout = 1u, kErrBarrier = 2u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
 __device__ __forceinline__ uint32_t dep_of(uint64_t s) { return static_cast<uint32_t>(s >> 32); }
+__device__ __forceinline__ uint64_t ld_state(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ void solve_prep(uint32_t nb, uint32_t nc, uint32_t *__restrict__ deg, uint32_t *__restrict__ fill,
-                           uint32_t *__restrict__ renorm, float *__restrict__ lam, uint64_t *__restrict__ state,
-                           uint32_t *__restrict__ meta, uint32_t n_hist, uint32_t *__restrict__ hist) {
+                           float *__restrict__ lam, uint64_t *__restrict__ state, uint32_t *__restrict__ meta) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) { deg[i] = 0; fill[i] = 0; renorm[i] = 0; }
+    if (i < nb) { deg[i] = 0; fill[i] = 0; }
     if (i < nc) { lam[i] = 0.f; state[i] = kUnassigned; }
-    if (i < n_hist) hist[i] = 0;
     if (i < kMetaWords) meta[i] = (i == 0) ? nc : 0u;
 }
 
-// Reset the class assignment only (serial fallback after a failed colouring).
-__global__ void solve_reassign_prep(uint32_t nc, uint64_t *__restrict__ state, uint32_t *__restrict__ meta,
-                                    uint32_t n_hist, uint32_t *__restrict__ hist) {
+__global__ void solve_reassign_prep(uint32_t nc, uint64_t *__restrict__ state, uint32_t *__restrict__ meta) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nc) state[i] = kUnassigned;
-    if (i < n_hist) hist[i] = 0;
-    if (i < kMetaWords) meta[i] = (i == 0) ? nc : 0u;
+    if (i < kMetaWords && i != 4) meta[i] = (i == 0) ? nc : 0u;
 }
 
 __global__ void solve_degree(dy_contact_view cv, uint32_t nc, const float *__restrict__ im, uint32_t *__restrict__ deg) {
@@ -76,15 +86,39 @@ __global__ void solve_fill(dy_contact_view cv, uint32_t nc, const float *__restr
     if (im[b] != 0.f) adj[off[b] + atomicAdd(&fill[b], 1u)] = c;
 }
 
+// Class and depth of contact c once every higher-priority neighbour has one.
+__device__ __forceinline__ void finish_class(uint32_t c, uint64_t used, uint32_t depth, int serial, uint64_t *state,
+                                             uint32_t *meta) {
+    uint32_t cls;
+    if (serial) {
+        cls = depth;
+    } else if (used == ~0ull) {
+        atomicOr(&meta[1], 1u);
+        cls = 64u;
+    } else {
+        cls = static_cast<uint32_t>(__builtin_ctzll(~used));
+    }
+    const uint32_t d = depth + 1u;
+    __hip_atomic_store(&state[c], (static_cast<uint64_t>(d) << 32) | cls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    atomicMax(&meta[2], cls);
+    atomicMax(&meta[3], d);
+    atomicSub(&meta[0], 1u);
+}
+
+__device__ __forceinline__ bool higher_priority(uint32_t j, uint32_t c, uint64_t pc, int serial) {
+    return serial ? (j < c) : (jp_priority(j) > pc);
+}
+
+// Round-based assignment (fallback when a cooperative launch cannot hold the grid).
 __global__ __launch_bounds__(kSolveThreads) void assign_round(dy_contact_view cv, uint32_t nc, const float *__restrict__ im,
                                                               const uint32_t *__restrict__ off,
                                                               const uint32_t *__restrict__ deg,
                                                               const uint32_t *__restrict__ adj, uint64_t *state,
-                                                              uint32_t *meta, uint32_t *hist, int serial) {
+                                                              uint32_t *meta, int serial) {
     if (__hip_atomic_load(&meta[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    if (__hip_atomic_load(&state[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnassigned) return;
+    if (ld_state(&state[c]) != kUnassigned) return;
     const uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
     const uint64_t pc = jp_priority(c);
     uint64_t used = 0;
@@ -95,92 +129,263 @@ __global__ __launch_bounds__(kSolveThreads) void assign_round(dy_contact_view cv
         const uint32_t beg = off[x], cnt = deg[x];
         for (uint32_t q = 0; q < cnt; ++q) {
             const uint32_t j = adj[beg + q];
-            if (j == c) continue;
-            const bool higher = serial ? (j < c) : (jp_priority(j) > pc);
-            if (!higher) continue;
-            const uint64_t sj = __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (sj == kUnassigned) return;  // a higher-priority neighbour is still open
+            if (j == c || !higher_priority(j, c, pc, serial)) continue;
+            const uint64_t sj = ld_state(&state[j]);
+            if (sj == kUnassigned) return;
             used |= 1ull << (cls_of(sj) & 63u);
             const uint32_t dj = dep_of(sj);
             depth = dj > depth ? dj : depth;
         }
     }
-    uint32_t cls;
-    if (serial) {
-        cls = depth;  // level
-    } else if (used == ~0ull) {
-        atomicOr(&meta[1], 1u);
-        cls = 64u;
-    } else {
-        cls = static_cast<uint32_t>(__builtin_ctzll(~used));
+    finish_class(c, used, depth, serial, state, meta);
+}
+
+// Dataflow assignment in one cooperative launch: each thread owns up to kOwn
+// contacts and advances each one's scan of its neighbour lists as far as the
+// neighbours' classes are known, in round robin, so no thread ever blocks on a
+// single item.  The globally highest-priority open contact is always ready, so
+// with every workgroup resident the loop terminates; the spin bound only
+// guards against a logic error (flagged, never expected).
+__global__ __launch_bounds__(kSolveThreads) void assign_dataflow(dy_contact_view cv, uint32_t nc,
+                                                                 const float *__restrict__ im,
+                                                                 const uint32_t *__restrict__ off,
+                                                                 const uint32_t *__restrict__ deg,
+                                                                 const uint32_t *__restrict__ adj, uint64_t *state,
+                                                                 uint32_t *meta, int serial) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+    uint32_t cc[kOwn], cur[kOwn], na[kOwn], tot[kOwn], oa[kOwn], ob[kOwn], dep[kOwn];
+    uint64_t used[kOwn], pc[kOwn];
+    bool open[kOwn];
+#pragma unroll
+    for (int o = 0; o < kOwn; ++o) {
+        cc[o] = gtid + o * gstride;
+        open[o] = cc[o] < nc;
+        cur[o] = na[o] = tot[o] = oa[o] = ob[o] = dep[o] = 0;
+        used[o] = 0;
+        pc[o] = 0;
+        if (open[o]) {
+            const uint32_t a = cv.body_a[cc[o]], b = cv.body_b[cc[o]];
+            const uint32_t da = im[a] != 0.f ? deg[a] : 0u, db = im[b] != 0.f ? deg[b] : 0u;
+            oa[o] = off[a];
+            ob[o] = off[b];
+            na[o] = da;
+            tot[o] = da + db;
+            pc[o] = jp_priority(cc[o]);
+        }
     }
-    const uint32_t d = depth + 1u;
-    __hip_atomic_store(&state[c], (static_cast<uint64_t>(d) << 32) | cls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    atomicAdd(&hist[cls], 1u);
-    atomicMax(&meta[2], cls);
-    atomicMax(&meta[3], d);
-    atomicSub(&meta[0], 1u);
+    for (uint32_t spin = 0; spin < kSpinLimit; ++spin) {
+        bool any = false;
+#pragma unroll
+        for (int o = 0; o < kOwn; ++o) {
+            if (!open[o]) continue;
+            while (cur[o] < tot[o]) {
+                const uint32_t q = cur[o];
+                const uint32_t j = q < na[o] ? adj[oa[o] + q] : adj[ob[o] + (q - na[o])];
+                if (j != cc[o] && higher_priority(j, cc[o], pc[o], serial)) {
+                    const uint64_t sj = ld_state(&state[j]);
+                    if (sj == kUnassigned) break;
+                    used[o] |= 1ull << (cls_of(sj) & 63u);
+                    const uint32_t dj = dep_of(sj);
+                    dep[o] = dj > dep[o] ? dj : dep[o];
+                }
+                cur[o] = q + 1;
+            }
+            if (cur[o] == tot[o]) {
+                finish_class(cc[o], used[o], dep[o], serial, state, meta);
+                open[o] = false;
+            } else {
+                any = true;
+            }
+        }
+        if (!any) return;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    atomicOr(&meta[4], kErrTimeout);
 }
 
-__global__ void bucket_scatter(uint32_t nc, const uint64_t *__restrict__ state, const uint32_t *__restrict__ boff,
-                               uint32_t *__restrict__ bfill, uint32_t *__restrict__ bucket) {
+// Class histogram, aggregated per workgroup in LDS when the class count fits.
+__global__ __launch_bounds__(kSolveThreads) void class_count(uint32_t nc, uint32_t n_cls,
+                                                             const uint64_t *__restrict__ state,
+                                                             uint32_t *__restrict__ hist) {
+    __shared__ uint32_t lh[kLdsClasses];
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t k = cls_of(state[c]);
-    bucket[boff[k] + atomicAdd(&bfill[k], 1u)] = c;
+    const bool local = n_cls <= kLdsClasses;
+    if (local)
+        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    if (c < nc) {
+        const uint32_t k = cls_of(state[c]);
+        if (local) atomicAdd(&lh[k], 1u);
+        else atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    if (local)
+        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x)
+            if (lh[k]) atomicAdd(&hist[k], lh[k]);
 }
 
-__device__ __forceinline__ BodyState load_body(const dy_body_view &bv, uint32_t i) {
+// Contact record for the sweeps, class-sorted: (a, b), (p, n.x), (n.y, n.z, depth, -).
+struct CRec {
+    uint2 *ab;
+    float4 *p0, *p1;
+};
+
+__global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv, uint32_t nc, uint32_t n_cls,
+                                                              const uint64_t *__restrict__ state,
+                                                              const uint32_t *__restrict__ boff,
+                                                              uint32_t *__restrict__ bfill, CRec rec,
+                                                              uint32_t *__restrict__ rec_id) {
+    __shared__ uint32_t lh[kLdsClasses];
+    __shared__ uint32_t lbase[kLdsClasses];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool local = n_cls <= kLdsClasses;
+    if (local)
+        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    uint32_t k = 0, rank = 0;
+    if (c < nc) {
+        k = cls_of(state[c]);
+        rank = local ? atomicAdd(&lh[k], 1u) : atomicAdd(&bfill[k], 1u);
+    }
+    __syncthreads();
+    if (local)
+        for (uint32_t q = threadIdx.x; q < n_cls; q += blockDim.x)
+            lbase[q] = lh[q] ? atomicAdd(&bfill[q], lh[q]) : 0u;
+    __syncthreads();
+    if (c >= nc) return;
+    const uint32_t pos = boff[k] + (local ? lbase[k] + rank : rank);
+    rec.ab[pos] = make_uint2(cv.body_a[c], cv.body_b[c]);
+    rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
+    rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
+    rec_id[pos] = c;
+}
+
+// AoS body records: dyn = (p.xyz, q.w) (q.xyz, v.x) (v.yz, w.xy) (w.z, renorms, -, -);
+// sta = (inv_mass, Ixx, Iyy, Izz) (Ixy, Ixz, Iyz, -).
+__global__ void gather_bodies(dy_body_view bv, float4 *__restrict__ dyn, float4 *__restrict__ sta) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bv.n) return;
+    dyn[4 * i + 0] = make_float4(bv.pos_x[i], bv.pos_y[i], bv.pos_z[i], bv.rot_w[i]);
+    dyn[4 * i + 1] = make_float4(bv.rot_x[i], bv.rot_y[i], bv.rot_z[i], bv.vel_x[i]);
+    dyn[4 * i + 2] = make_float4(bv.vel_y[i], bv.vel_z[i], bv.ang_x[i], bv.ang_y[i]);
+    dyn[4 * i + 3] = make_float4(bv.ang_z[i], __uint_as_float(0u), 0.f, 0.f);
+    sta[2 * i + 0] = make_float4(bv.inv_mass[i], bv.iI_xx[i], bv.iI_yy[i], bv.iI_zz[i]);
+    sta[2 * i + 1] = make_float4(bv.iI_xy[i], bv.iI_xz[i], bv.iI_yz[i], 0.f);
+}
+
+__device__ __forceinline__ BodyState load_aos(const float4 *__restrict__ dyn, const float4 *__restrict__ sta,
+                                              uint32_t i, uint32_t &renorms) {
+    const float4 d0 = dyn[4 * i], d1 = dyn[4 * i + 1], d2 = dyn[4 * i + 2], d3 = dyn[4 * i + 3];
+    const float4 s0 = sta[2 * i], s1 = sta[2 * i + 1];
     BodyState s;
-    s.px = bv.pos_x[i]; s.py = bv.pos_y[i]; s.pz = bv.pos_z[i];
-    s.qw = bv.rot_w[i]; s.qx = bv.rot_x[i]; s.qy = bv.rot_y[i]; s.qz = bv.rot_z[i];
-    s.vx = bv.vel_x[i]; s.vy = bv.vel_y[i]; s.vz = bv.vel_z[i];
-    s.wx = bv.ang_x[i]; s.wy = bv.ang_y[i]; s.wz = bv.ang_z[i];
-    s.im = bv.inv_mass[i];
-    s.ixx = bv.iI_xx[i]; s.iyy = bv.iI_yy[i]; s.izz = bv.iI_zz[i];
-    s.ixy = bv.iI_xy[i]; s.ixz = bv.iI_xz[i]; s.iyz = bv.iI_yz[i];
+    s.px = d0.x; s.py = d0.y; s.pz = d0.z; s.qw = d0.w;
+    s.qx = d1.x; s.qy = d1.y; s.qz = d1.z; s.vx = d1.w;
+    s.vy = d2.x; s.vz = d2.y; s.wx = d2.z; s.wy = d2.w;
+    s.wz = d3.x;
+    renorms = __float_as_uint(d3.y);
+    s.im = s0.x; s.ixx = s0.y; s.iyy = s0.z; s.izz = s0.w;
+    s.ixy = s1.x; s.ixz = s1.y; s.iyz = s1.z;
     return s;
 }
 
-__device__ __forceinline__ void store_body(const dy_body_view &bv, uint32_t i, const BodyState &s) {
-    bv.pos_x[i] = s.px; bv.pos_y[i] = s.py; bv.pos_z[i] = s.pz;
-    bv.rot_w[i] = s.qw; bv.rot_x[i] = s.qx; bv.rot_y[i] = s.qy; bv.rot_z[i] = s.qz;
-    bv.vel_x[i] = s.vx; bv.vel_y[i] = s.vy; bv.vel_z[i] = s.vz;
-    bv.ang_x[i] = s.wx; bv.ang_y[i] = s.wy; bv.ang_z[i] = s.wz;
+__device__ __forceinline__ void store_aos(float4 *__restrict__ dyn, uint32_t i, const BodyState &s, uint32_t renorms) {
+    dyn[4 * i] = make_float4(s.px, s.py, s.pz, s.qw);
+    dyn[4 * i + 1] = make_float4(s.qx, s.qy, s.qz, s.vx);
+    dyn[4 * i + 2] = make_float4(s.vy, s.vz, s.wx, s.wy);
+    dyn[4 * i + 3] = make_float4(s.wz, __uint_as_float(renorms), 0.f, 0.f);
 }
 
 __device__ __forceinline__ void catch_up(BodyState &s, uint32_t done, uint32_t target) {
     for (uint32_t r = done; r < target; ++r) renormalise(s.qw, s.qx, s.qy, s.qz);
 }
 
-__global__ __launch_bounds__(kSolveThreads) void solve_bucket(const uint32_t *__restrict__ bucket, uint32_t count,
-                                                              uint32_t it, dy_contact_view cv, dy_body_view bv,
-                                                              float *__restrict__ lam, uint32_t *__restrict__ renorm,
-                                                              float mu, float dt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t c = bucket[i];
-    const uint32_t a = cv.body_a[c], b = cv.body_b[c];
-    const ContactIn ci{cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c], cv.norm_y[c], cv.norm_z[c], cv.depth[c]};
-    BodyState A = load_body(bv, a), B = load_body(bv, b);
+__device__ __forceinline__ void solve_one(uint32_t i, uint32_t it, const CRec &rec, float *__restrict__ lam,
+                                          float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
+                                          float dt) {
+    const uint2 ab = rec.ab[i];
+    const float4 p0 = rec.p0[i], p1 = rec.p1[i];
+    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+    uint32_t ra, rb;
+    BodyState A = load_aos(dyn, sta, ab.x, ra), B = load_aos(dyn, sta, ab.y, rb);
     const bool da = A.im != 0.f, db = B.im != 0.f;
-    if (da) catch_up(A, renorm[a], it);
-    if (db) catch_up(B, renorm[b], it);
-    float l = lam[c];
+    if (da) catch_up(A, ra, it);
+    if (db) catch_up(B, rb, it);
+    float l = lam[i];
     project_contact(ci, A, B, mu, l, dt);
-    lam[c] = l;
-    if (da) { store_body(bv, a, A); renorm[a] = it; }
-    if (db) { store_body(bv, b, B); renorm[b] = it; }
+    lam[i] = l;
+    if (da) store_aos(dyn, ab.x, A, it);
+    if (db) store_aos(dyn, ab.y, B, it);
 }
 
-__global__ void final_renorm(dy_body_view bv, const uint32_t *__restrict__ renorm, uint32_t n_iters, int use_counts) {
+// One class (fallback path: one launch per class).
+__global__ __launch_bounds__(kSolveThreads) void sweep_class(const uint32_t *__restrict__ boff, uint32_t k, uint32_t it,
+                                                             CRec rec, float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                             const float4 *__restrict__ sta, float mu, float dt) {
+    const uint32_t beg = boff[k], end = boff[k + 1];
+    for (uint32_t i = beg + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x)
+        solve_one(i, it, rec, lam, dyn, sta, mu, dt);
+}
+
+// Grid-wide barrier for a cooperative launch: arrival counter + generation.
+// Wave 0 of each workgroup publishes the workgroup's stores (agent release)
+// before arriving and acquires after the generation moves.
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint32_t *err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = gen;
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, kErrBarrier);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    gen += 1u;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kSolveThreads) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_cls,
+                                                                  uint32_t n_iters, CRec rec, float *__restrict__ lam,
+                                                                  float4 *__restrict__ dyn,
+                                                                  const float4 *__restrict__ sta, float mu, float dt,
+                                                                  uint32_t *bar, uint32_t *err) {
+    uint32_t gen = 0;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+    for (uint32_t it = 0; it < n_iters; ++it) {
+        for (uint32_t k = 0; k < n_cls; ++k) {
+            const uint32_t beg = boff[k], end = boff[k + 1];
+            for (uint32_t i = beg + gtid; i < end; i += gstride) solve_one(i, it, rec, lam, dyn, sta, mu, dt);
+            if (it + 1 < n_iters || k + 1 < n_cls) grid_barrier(bar, gen, err);
+        }
+    }
+}
+
+// Back to the SoA view, bringing every body to n_iters renormalisations.
+__global__ void scatter_bodies(dy_body_view bv, const float4 *__restrict__ dyn, uint32_t n_iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bv.n) return;
-    const uint32_t done = use_counts ? renorm[i] : 0u;
-    if (done >= n_iters) return;
-    float qw = bv.rot_w[i], qx = bv.rot_x[i], qy = bv.rot_y[i], qz = bv.rot_z[i];
-    for (uint32_t r = done; r < n_iters; ++r) renormalise(qw, qx, qy, qz);
+    const float4 d0 = dyn[4 * i], d1 = dyn[4 * i + 1], d2 = dyn[4 * i + 2], d3 = dyn[4 * i + 3];
+    float qw = d0.w, qx = d1.x, qy = d1.y, qz = d1.z;
+    for (uint32_t r = __float_as_uint(d3.y); r < n_iters; ++r) renormalise(qw, qx, qy, qz);
+    bv.pos_x[i] = d0.x; bv.pos_y[i] = d0.y; bv.pos_z[i] = d0.z;
     bv.rot_w[i] = qw; bv.rot_x[i] = qx; bv.rot_y[i] = qy; bv.rot_z[i] = qz;
+    bv.vel_x[i] = d1.w; bv.vel_y[i] = d2.x; bv.vel_z[i] = d2.y;
+    bv.ang_x[i] = d2.z; bv.ang_y[i] = d2.w; bv.ang_z[i] = d3.x;
+}
+
+// Class of each contact id (for dy_xpbd_download_classes).
+__global__ void unpack_classes(uint32_t n, const uint64_t *__restrict__ state, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cls_of(state[i]);
 }
 
 }  // namespace dy
@@ -194,53 +399,88 @@ struct dy_xpbd_s {
     uint32_t cap = 0;
     float *lam = nullptr;
     uint64_t *state = nullptr;
-    uint32_t *adj = nullptr, *bucket = nullptr, *boff = nullptr, *bfill = nullptr;
-    uint32_t *meta_hist = nullptr;  // kMetaWords meta + (cap + 65) histogram
-    DevBuf<uint32_t> deg, off, fill, renorm, scan_tmp;
+    uint32_t *adj = nullptr, *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr;
+    uint32_t *meta = nullptr;  // kMetaWords + barrier words
+    uint2 *rec_ab = nullptr;
+    float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
+    DevBuf<uint32_t> deg, off, fill, scan_tmp;
+    DevBuf<float4> dyn, sta;
     dy_solve_stats stats{};
-    std::vector<uint32_t> h_meta_hist;
-    std::vector<uint32_t> h_boff;
+    uint32_t h_meta[kMetaWords] = {};
+    int coop_assign_blocks = 0, coop_sweep_max = 0, device_cus = 0;
+    int sweep_blocks_override = 0;
+    int path = 0;  // 0 persistent sweep, 1 per-class launches
 };
 
 namespace {
 void xp_free(dy_xpbd_s *x) {
-    void *ptrs[] = {x->lam, x->state, x->adj, x->bucket, x->boff, x->bfill, x->meta_hist};
+    void *ptrs[] = {x->lam, x->state, x->adj, x->boff, x->bfill, x->hist, x->rec_id, x->meta,
+                    x->rec_ab, x->rec_p0, x->rec_p1};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    x->deg.release(); x->off.release(); x->fill.release(); x->renorm.release(); x->scan_tmp.release();
+    x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release();
+    x->dyn.release(); x->sta.release();
 }
 
-// Runs assignment rounds until every contact has a class.  Returns DY_OK with
-// meta/hist downloaded into x->h_meta_hist, or 1 when the coloured path must
-// fall back (too many rounds).
-int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint32_t nc, const float *im, int serial,
-                   uint32_t &k_classes) {
-    uint32_t *meta = x->meta_hist, *hist = x->meta_hist + kMetaWords;
+int query_coop(dy_xpbd_s *x) {
+    int dev = 0, coop = 0;
+    DY_TRY(hipGetDevice(&dev));
+    DY_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    DY_TRY(hipDeviceGetAttribute(&x->device_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (!coop) return DY_OK;
+    int per_cu = 0;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_dataflow),
+                                                        kSolveThreads, 0));
+    x->coop_assign_blocks = per_cu * x->device_cus;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_persistent),
+                                                        kSolveThreads, 0));
+    x->coop_sweep_max = per_cu * x->device_cus;
+    if (const char *e = std::getenv("DYPHUR_SWEEP_BLOCKS")) x->sweep_blocks_override = std::atoi(e);
+    if (const char *e = std::getenv("DYPHUR_SOLVER_PATH")) x->path = std::atoi(e);
+    return DY_OK;
+}
+
+// Assigns a class to every contact; leaves meta downloaded in x->h_meta.
+int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint32_t nc, const float *im, int serial) {
+    const uint32_t threads = static_cast<uint32_t>(x->coop_assign_blocks) * kSolveThreads;
+    if (x->coop_assign_blocks > 0 && static_cast<uint64_t>(threads) * kOwn >= nc) {
+        uint32_t blocks = ceil_div(ceil_div(nc, kOwn), kSolveThreads);
+        if (blocks > static_cast<uint32_t>(x->coop_assign_blocks)) blocks = x->coop_assign_blocks;
+        dy_contact_view cvv = cv;
+        uint32_t ncv = nc;
+        const float *imv = im;
+        const uint32_t *offv = x->off.p, *degv = x->deg.p, *adjv = x->adj;
+        uint64_t *statev = x->state;
+        uint32_t *metav = x->meta;
+        int serialv = serial;
+        void *args[] = {&cvv, &ncv, &imv, &offv, &degv, &adjv, &statev, &metav, &serialv};
+        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_dataflow), dim3(blocks),
+                                          dim3(kSolveThreads), args, 0, st));
+        DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
+        DY_TRY(hipStreamSynchronize(st));
+        if (x->h_meta[4] & kErrTimeout) return DY_ERR_INTERNAL;
+        return x->h_meta[0] == 0u ? DY_OK : DY_ERR_INTERNAL;
+    }
     const uint32_t grid = ceil_div(nc, kSolveThreads);
-    const uint32_t max_rounds = serial ? nc + 1u : 256u;
-    uint32_t rounds = 0, batch = serial ? 16u : 12u;
-    x->h_meta_hist.resize(kMetaWords + kHistDownload);
+    const uint32_t max_rounds = serial ? nc + 1u : 257u;
+    uint32_t rounds = 0, batch = 16;
     while (true) {
         for (uint32_t r = 0; r < batch && rounds < max_rounds; ++r, ++rounds) {
-            assign_round<<<grid, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->adj, x->state, meta, hist,
+            assign_round<<<grid, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->adj, x->state, x->meta,
                                                          serial);
             DY_LAUNCH_CHECK();
         }
-        DY_TRY(hipMemcpyAsync(x->h_meta_hist.data(), meta, (kMetaWords + kHistDownload) * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, st));
+        DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
         DY_TRY(hipStreamSynchronize(st));
-        if (x->h_meta_hist[0] == 0u) break;
-        if (rounds >= max_rounds) return serial ? DY_ERR_INTERNAL : 1;
-        batch = 8;
+        if (x->h_meta[0] == 0u) return DY_OK;
+        // Colouring deeper than the reference's 256 rounds: report via depth.
+        if (rounds >= max_rounds) {
+            if (serial) return DY_ERR_INTERNAL;
+            x->h_meta[3] = 257u;
+            return DY_OK;
+        }
+        batch = 16;
     }
-    k_classes = x->h_meta_hist[2] + 1u;
-    if (k_classes > kHistDownload) {
-        x->h_meta_hist.resize(kMetaWords + k_classes);
-        DY_TRY(hipMemcpyAsync(x->h_meta_hist.data() + kMetaWords + kHistDownload, hist + kHistDownload,
-                              (k_classes - kHistDownload) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        DY_TRY(hipStreamSynchronize(st));
-    }
-    return DY_OK;
 }
 }  // namespace
 
@@ -254,20 +494,29 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     x->mu = friction;
     x->cap = max_contacts;
     const size_t c = max_contacts ? max_contacts : 1;
+    const size_t ncls = c + 66;
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&x->lam), c * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->state), c * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->adj), 2 * c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bucket), c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->boff), (c + 66) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bfill), (c + 66) * sizeof(uint32_t));
-    const size_t hist_words = (c + 66) > kHistDownload ? (c + 66) : kHistDownload;
-    if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&x->meta_hist), (kMetaWords + hist_words) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemsetAsync(x->lam, 0, c * sizeof(float), hs(s));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->boff), ncls * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bfill), ncls * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->hist), ncls * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_id), c * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), (kMetaWords + 4) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_ab), c * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p0), c * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p1), c * sizeof(float4));
+    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, (kMetaWords + 4) * sizeof(uint32_t), hs(s));
     if (e != hipSuccess) {
         xp_free(x);
         delete x;
         return static_cast<int>(e);
+    }
+    const int q = query_coop(x);
+    if (q != DY_OK) {
+        xp_free(x);
+        delete x;
+        return q;
     }
     *out = x;
     return DY_OK;
@@ -293,9 +542,12 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     if (n_joints != 0) return DY_ERR_UNSUPPORTED;
     hipStream_t st = hs(s);
     x->stats = dy_solve_stats{};
-    uint32_t nc = 0;
-    DY_TRY(hipMemcpyAsync(&nc, cv->n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    uint32_t head[2] = {0, 0};  // contact count; sweep error flags of the previous solve
+    DY_TRY(hipMemcpyAsync(&head[0], cv->n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    DY_TRY(hipMemcpyAsync(&head[1], x->meta + kMetaWords + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     DY_TRY(hipStreamSynchronize(st));
+    if (head[1]) return DY_ERR_INTERNAL;
+    const uint32_t nc = head[0];
     x->stats.n_constraints = nc;
     if (nc == 0) return DY_OK;
     if (nc > x->cap) return DY_ERR_INVALID_ARG;
@@ -303,13 +555,11 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_TRY_STATUS(x->deg.ensure(nb));
     DY_TRY_STATUS(x->off.ensure(nb));
     DY_TRY_STATUS(x->fill.ensure(nb));
-    DY_TRY_STATUS(x->renorm.ensure(nb));
+    DY_TRY_STATUS(x->dyn.ensure(4 * static_cast<size_t>(nb)));
+    DY_TRY_STATUS(x->sta.ensure(2 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > nc + 66u ? nb : nc + 66u)));
-    uint32_t *meta = x->meta_hist, *hist = x->meta_hist + kMetaWords;
-    const uint32_t n_hist = nc + 65u;
-    const uint32_t prep_n = (nb > nc ? nb : nc) > n_hist ? (nb > nc ? nb : nc) : n_hist;
-    solve_prep<<<ceil_div(prep_n, 256), 256, 0, st>>>(nb, nc, x->deg.p, x->fill.p, x->renorm.p, x->lam, x->state,
-                                                      meta, n_hist, hist);
+    const uint32_t prep_n = nb > nc ? nb : nc;
+    solve_prep<<<ceil_div(prep_n, 256), 256, 0, st>>>(nb, nc, x->deg.p, x->fill.p, x->lam, x->state, x->meta);
     DY_LAUNCH_CHECK();
     const uint32_t gc = ceil_div(nc, kSolveThreads);
     solve_degree<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->deg.p);
@@ -317,50 +567,69 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, nullptr, x->scan_tmp.p));
     solve_fill<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->off.p, x->fill.p, x->adj);
     DY_LAUNCH_CHECK();
-
-    int serial = (x->force_serial || nc <= 256u) ? 1 : 0;
-    uint32_t k = 0;
-    int r = assign_classes(x, st, *cv, nc, bv->inv_mass, serial, k);
-    if (r < 0) return r;
-    if (!serial) {
-        const uint32_t overflow = x->h_meta_hist[1], depth = x->h_meta_hist[3];
-        x->stats.rounds = r == 1 ? 257u : depth;
-        x->stats.overflow = overflow;
-        if (r == 1 || overflow || depth > 256u) {
-            // The reference runs its serial path in these cases.
-            serial = 1;
-            solve_reassign_prep<<<ceil_div(n_hist, 256), 256, 0, st>>>(nc, x->state, meta, n_hist, hist);
-            DY_LAUNCH_CHECK();
-            r = assign_classes(x, st, *cv, nc, bv->inv_mass, 1, k);
-            if (r != DY_OK) return r < 0 ? r : DY_ERR_INTERNAL;
-        }
-    }
-    x->stats.used_serial = static_cast<uint32_t>(serial);
-    x->stats.n_classes = k;
-    if (serial) x->stats.rounds = x->h_meta_hist[3];
-
-    // Bucket contacts by class: offsets scanned on the device from the same
-    // histogram the host downloaded (the host copy only sizes the launches).
-    const uint32_t *h_hist = x->h_meta_hist.data() + kMetaWords;
-    x->h_boff.resize(k);
-    uint32_t run = 0;
-    for (uint32_t c = 0; c < k; ++c) { x->h_boff[c] = run; run += h_hist[c]; }
-    if (run != nc) return DY_ERR_INTERNAL;
-    DY_TRY_STATUS(scan_exclusive(st, hist, x->boff, k, nullptr, x->scan_tmp.p));
-    DY_TRY(hipMemsetAsync(x->bfill, 0, k * sizeof(uint32_t), st));
-    bucket_scatter<<<gc, kSolveThreads, 0, st>>>(nc, x->state, x->boff, x->bfill, x->bucket);
+    gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
 
-    for (int it = 0; it < x->n_iters; ++it) {
-        for (uint32_t c = 0; c < k; ++c) {
-            const uint32_t cnt = h_hist[c];
-            if (cnt == 0) continue;
-            solve_bucket<<<ceil_div(cnt, kSolveThreads), kSolveThreads, 0, st>>>(
-                x->bucket + x->h_boff[c], cnt, static_cast<uint32_t>(it), *cv, *bv, x->lam, x->renorm.p, x->mu, dt);
+    int serial = (x->force_serial || nc <= 256u) ? 1 : 0;
+    DY_TRY_STATUS(assign_classes(x, st, *cv, nc, bv->inv_mass, serial));
+    if (!serial) {
+        x->stats.overflow = x->h_meta[1];
+        x->stats.rounds = x->h_meta[3];
+        if (x->h_meta[1] || x->h_meta[3] > 256u) {
+            // The reference runs its serial path in these cases.
+            serial = 1;
+            solve_reassign_prep<<<ceil_div(nc, 256), 256, 0, st>>>(nc, x->state, x->meta);
             DY_LAUNCH_CHECK();
+            DY_TRY_STATUS(assign_classes(x, st, *cv, nc, bv->inv_mass, 1));
         }
     }
-    final_renorm<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->renorm.p, static_cast<uint32_t>(x->n_iters), 1);
+    const uint32_t k = x->h_meta[2] + 1u;
+    x->stats.used_serial = static_cast<uint32_t>(serial);
+    x->stats.n_classes = k;
+    if (serial) x->stats.rounds = x->h_meta[3];
+
+    // Class-sorted contact records.
+    DY_TRY(hipMemsetAsync(x->hist, 0, (k + 1) * sizeof(uint32_t), st));
+    DY_TRY(hipMemsetAsync(x->bfill, 0, k * sizeof(uint32_t), st));
+    class_count<<<gc, kSolveThreads, 0, st>>>(nc, k, x->state, x->hist);
+    DY_LAUNCH_CHECK();
+    DY_TRY_STATUS(scan_exclusive(st, x->hist, x->boff, k + 1, nullptr, x->scan_tmp.p));
+    CRec rec{x->rec_ab, x->rec_p0, x->rec_p1};
+    class_gather<<<gc, kSolveThreads, 0, st>>>(*cv, nc, k, x->state, x->boff, x->bfill, rec, x->rec_id);
+    DY_LAUNCH_CHECK();
+
+    const uint32_t iters = static_cast<uint32_t>(x->n_iters);
+    if (iters > 0) {
+        const bool persistent = x->path == 0 && x->coop_sweep_max > 0;
+        if (persistent) {
+            uint32_t blocks = x->sweep_blocks_override > 0 ? static_cast<uint32_t>(x->sweep_blocks_override)
+                                                           : static_cast<uint32_t>(x->device_cus / 8);
+            if (blocks < 1) blocks = 1;
+            if (blocks > static_cast<uint32_t>(x->coop_sweep_max)) blocks = x->coop_sweep_max;
+            uint32_t *bar = x->meta + kMetaWords;
+            DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
+            const uint32_t *boffv = x->boff;
+            uint32_t kv = k, iv = iters;
+            float *lamv = x->lam;
+            float4 *dynv = x->dyn.p;
+            const float4 *stav = x->sta.p;
+            float muv = x->mu, dtv = dt;
+            uint32_t *errv = x->meta + kMetaWords + 2;
+            void *args[] = {&boffv, &kv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &bar, &errv};
+            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_persistent), dim3(blocks),
+                                              dim3(kSolveThreads), args, 0, st));
+        } else {
+            // Per-class launches; each reads its range from the device offsets.
+            const uint32_t blocks = gc < 1024u ? gc : 1024u;
+            for (uint32_t it = 0; it < iters; ++it)
+                for (uint32_t c = 0; c < k; ++c) {
+                    sweep_class<<<blocks, kSolveThreads, 0, st>>>(x->boff, c, it, rec, x->lam, x->dyn.p, x->sta.p,
+                                                                  x->mu, dt);
+                    DY_LAUNCH_CHECK();
+                }
+        }
+    }
+    scatter_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, iters);
     DY_LAUNCH_CHECK();
     return DY_OK;
 }
@@ -374,12 +643,11 @@ int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out) {
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n) {
     if (!x || (!out && n)) return DY_ERR_INVALID_ARG;
     if (n > x->cap) return DY_ERR_INVALID_ARG;
-    std::vector<uint64_t> h(n);
-    if (n) {
-        DY_TRY(hipMemcpyAsync(h.data(), x->state, n * sizeof(uint64_t), hipMemcpyDeviceToHost, hs(s)));
-        DY_TRY(hipStreamSynchronize(hs(s)));
-    }
-    for (uint32_t i = 0; i < n; ++i) out[i] = static_cast<uint32_t>(h[i] & 0xffffffffu);
+    if (n == 0) return DY_OK;
+    unpack_classes<<<ceil_div(n, 256), 256, 0, hs(s)>>>(n, x->state, x->rec_id);
+    DY_LAUNCH_CHECK();
+    DY_TRY(hipMemcpyAsync(out, x->rec_id, n * sizeof(uint32_t), hipMemcpyDeviceToHost, hs(s)));
+    DY_TRY(hipStreamSynchronize(hs(s)));
     return DY_OK;
 }
 
