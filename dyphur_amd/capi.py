@@ -65,7 +65,13 @@ class DeviceInfo(C.Structure):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [(n, u32) for n in ("n_constraints", "used_serial", "n_classes", "rounds", "overflow")]
+    _fields_ = [(n, u32) for n in ("n_constraints", "used_serial", "n_classes", "rounds", "overflow",
+                                   "dyn_endpoints")]
+
+
+class SolverTiming(C.Structure):
+    _fields_ = [("solves", u32), ("assign_ms", C.c_double), ("sweep_ms", C.c_double),
+                ("sweep_bytes", C.c_double)]
 
 
 class DyError(RuntimeError):
@@ -126,6 +132,8 @@ _SIGS = {
     "dy_xpbd_solve": (C.c_int, [vp, vp, C.POINTER(ContactView), u32, C.POINTER(BodyView), f32]),
     "dy_xpbd_last_stats": (C.c_int, [vp, C.POINTER(SolveStats)]),
     "dy_xpbd_download_classes": (C.c_int, [vp, vp, vp, u32]),
+    "dy_xpbd_set_timing": (C.c_int, [vp, C.c_int]),
+    "dy_xpbd_get_timing": (C.c_int, [vp, vp, C.POINTER(SolverTiming)]),
     "dy_state_hash_fnv1a64": (C.c_int, [vp, C.POINTER(BodyView), C.POINTER(C.c_uint64)]),
 }
 
@@ -493,6 +501,14 @@ class XpbdSolver:
         check(lib().dy_xpbd_last_stats(self.h, C.byref(st)), "dy_xpbd_last_stats")
         return {k: getattr(st, k) for k, _ in SolveStats._fields_}
 
+    def set_timing(self, on: bool):
+        check(lib().dy_xpbd_set_timing(self.h, int(bool(on))), "dy_xpbd_set_timing")
+
+    def timing(self, s: Stream) -> dict:
+        t = SolverTiming()
+        check(lib().dy_xpbd_get_timing(self.h, s.h, C.byref(t)), "dy_xpbd_get_timing")
+        return {k: getattr(t, k) for k, _ in SolverTiming._fields_}
+
     def classes(self, s: Stream, n: int) -> np.ndarray:
         out = np.zeros(n, np.uint32)
         check(lib().dy_xpbd_download_classes(self.h, s.h, out.ctypes.data, n), "dy_xpbd_download_classes")
@@ -534,7 +550,7 @@ class World:
         self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt)
         st = self.solver.stats()
         self.last = dict(n_pairs=n_pairs, n_contacts=st["n_constraints"], n_colors=st["n_classes"],
-                         used_serial=st["used_serial"])
+                         used_serial=st["used_serial"], dyn_endpoints=st["dyn_endpoints"])
         return self.last
 
     def state(self) -> dict:

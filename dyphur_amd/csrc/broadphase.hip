@@ -25,6 +25,7 @@
 namespace dy {
 
 constexpr uint32_t kRow = 16;            // partners kept in registers per walk
+constexpr uint32_t kOvf = 128;           // further partners kept per body in an overflow buffer
 constexpr uint32_t kChunk = 1024;        // leaves per refit workgroup
 constexpr uint32_t kMortonMax = 1u << 30;
 // A Karras tree over distinct 64-bit keys is at most 65 levels deep; walks
@@ -325,20 +326,22 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
 // ---------------------------------------------------------------------------
 // 4-6. Traversal and ordered pair emission
 // ---------------------------------------------------------------------------
-// Insert x into the ascending register list s (keeps the kRow smallest).
-__device__ __forceinline__ void row_insert(uint32_t (&s)[kRow], uint32_t x) {
+// Insert x into the ascending register list s (keeps the kRow smallest);
+// returns the element that no longer fits (all ones while the list has room).
+__device__ __forceinline__ uint32_t row_insert(uint32_t (&s)[kRow], uint32_t x) {
 #pragma unroll
     for (uint32_t j = 0; j < kRow; ++j) {
         const uint32_t lo = s[j] < x ? s[j] : x;
         x = s[j] < x ? x : s[j];
         s[j] = lo;
     }
+    return x;
 }
 
 // Walk the tree for query box q; count partners with body index > thr and
 // keep the kRow smallest of them in s.  Returns the count.
 __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, const Box6 &q, uint32_t thr,
-                                         uint32_t (&s)[kRow], uint32_t *err) {
+                                         uint32_t (&s)[kRow], uint32_t *err, uint32_t *ovf = nullptr) {
     uint32_t count = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kRow; ++j) s[j] = 0xffffffffu;
@@ -352,8 +355,9 @@ __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, co
         const Box6 b = unpack_box(w0.x, w0.y, w0.z);
         if (w0.w > thr && overlaps(b, q)) {
             if (node >= n - 1) {
+                const uint32_t ev = row_insert(s, static_cast<uint32_t>(w1.x));
+                if (ovf && ev != 0xffffffffu && count - kRow < kOvf) ovf[count - kRow] = ev;
                 ++count;
-                row_insert(s, static_cast<uint32_t>(w1.x));
                 node = w1.y;
             } else {
                 node = w1.x;
@@ -367,14 +371,15 @@ __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, co
 
 __global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ tn, int n,
                                                       uint32_t *__restrict__ counts, uint32_t *__restrict__ row,
-                                                      uint32_t *__restrict__ leaf_of, uint32_t *err) {
+                                                      uint32_t *__restrict__ ovf, uint32_t *__restrict__ leaf_of,
+                                                      uint32_t *err) {
     const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const TNode leaf = tn[n - 1 + k];
     const uint32_t a = static_cast<uint32_t>(leaf.child);
     const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     uint32_t s[kRow];
-    const uint32_t c = walk(tn, n, q, a, s, err);
+    const uint32_t c = walk(tn, n, q, a, s, err, ovf + static_cast<size_t>(a) * kOvf);
     counts[a] = c;
     leaf_of[a] = static_cast<uint32_t>(k);
     const uint32_t m = c < kRow ? c : kRow;
@@ -385,6 +390,7 @@ __global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ 
 __global__ __launch_bounds__(256) void emit_pairs(const TNode *__restrict__ tn, int n,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ offs, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ ovf,
                                                   const uint32_t *__restrict__ leaf_of,
                                                   dy_contact_pair *__restrict__ pairs, uint32_t cap,
                                                   uint32_t *err) {
@@ -399,12 +405,32 @@ __global__ __launch_bounds__(256) void emit_pairs(const TNode *__restrict__ tn, 
         if (off + j < cap) pairs[off + j] = dy_contact_pair{a, row[static_cast<size_t>(a) * kRow + j]};
     }
     if (L <= kRow) return;
-    // Long row: fetch the next kRow partners above the last one written.
-    const TNode leaf = tn[n - 1 + leaf_of[a]];
-    const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     uint32_t thr = row[static_cast<size_t>(a) * kRow + kRow - 1];
     uint32_t done = kRow;
     uint32_t s[kRow];
+    if (L <= kRow + kOvf) {
+        // Every further partner is in the overflow buffer (unordered, all
+        // above the kept kRow): select them kRow at a time in order.
+        const uint32_t *o = ovf + static_cast<size_t>(a) * kOvf;
+        const uint32_t m2 = L - kRow;
+        while (done < L && off + done < cap) {
+#pragma unroll
+            for (uint32_t j = 0; j < kRow; ++j) s[j] = 0xffffffffu;
+            for (uint32_t j = 0; j < m2; ++j) {
+                const uint32_t v = o[j];
+                if (v > thr) row_insert(s, v);
+            }
+            const uint32_t c = (L - done) < kRow ? (L - done) : kRow;
+            for (uint32_t j = 0; j < kRow; ++j)
+                if (j < c && off + done + j < cap) pairs[off + done + j] = dy_contact_pair{a, s[j]};
+            done += c;
+            thr = s[kRow - 1];
+        }
+        return;
+    }
+    // Longer rows: fetch the next kRow partners above the last one written.
+    const TNode leaf = tn[n - 1 + leaf_of[a]];
+    const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     while (done < L && off + done < cap) {
         walk(tn, n, q, thr, s, err);
         const uint32_t c = (L - done) < kRow ? (L - done) : kRow;
@@ -428,7 +454,7 @@ struct dy_broadphase_s {
     dy_node_box *nodebox = nullptr;
     TNode *tn = nullptr;
     uint32_t *cnt_top = nullptr, *entries = nullptr, *counters = nullptr;
-    uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *leaf_of = nullptr;
+    uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *ovf = nullptr, *leaf_of = nullptr;
     uint32_t *scan_tmp = nullptr;
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
@@ -438,7 +464,7 @@ namespace {
 void bp_free(dy_broadphase_s *b) {
     void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
                     b->root, b->range, b->nodebox, b->tn, b->cnt_top, b->entries, b->counters,
-                    b->counts, b->offs, b->row, b->leaf_of, b->scan_tmp, b->pairs};
+                    b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -466,7 +492,7 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     DY_A(b->left, nb); DY_A(b->right, nb); DY_A(b->parent, nodes); DY_A(b->root, 1);
     DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes);
     DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 4);
-    DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->leaf_of, nb);
+    DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->ovf, nb * kOvf); DY_A(b->leaf_of, nb);
     DY_A(b->scan_tmp, scan_tmp_words(nb)); DY_A(b->pairs, max_pairs);
 #undef DY_A
     if (e == hipSuccess) b->count = b->counters + 3;
@@ -508,11 +534,11 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
                                                        bp->right, bp->range, bp->nodebox, bp->tn, bp->cnt_top,
                                                        bp->entries, bp->counters);
     DY_LAUNCH_CHECK();
-    traverse_count<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->row, bp->leaf_of,
+    traverse_count<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of,
                                                      bp->counters + 2);
     DY_LAUNCH_CHECK();
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
-    emit_pairs<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->leaf_of,
+    emit_pairs<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
                                                  bp->pairs, bp->max_pairs, bp->counters + 2);
     DY_LAUNCH_CHECK();
     return DY_OK;

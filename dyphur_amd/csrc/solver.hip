@@ -42,12 +42,11 @@ namespace dy {
 
 constexpr uint64_t kUnassigned = ~0ull;
 constexpr uint32_t kSolveThreads = 256;
-// meta: [0] remaining [1] colour overflow [2] max class [3] max depth [4] error flags
+// meta: [0] remaining [1] colour overflow [2] max class [3] max depth [4] error flags [5] assigned
 constexpr uint32_t kMetaWords = 16;
-constexpr int kOwn = 4;                  // contacts per thread in the dataflow colouring
 constexpr uint32_t kLdsClasses = 2048;   // block-aggregated class counting limit
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr uint32_t kErrTimeout = 1u, kErrBarrier = 2u;
+constexpr uint32_t kErrTimeout = 1u, kErrBarrier = 2u, kErrHpCapacity = 4u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
 __device__ __forceinline__ uint32_t dep_of(uint64_t s) { return static_cast<uint32_t>(s >> 32); }
@@ -66,7 +65,7 @@ __global__ void solve_prep(uint32_t nb, uint32_t nc, uint32_t *__restrict__ deg,
 __global__ void solve_reassign_prep(uint32_t nc, uint64_t *__restrict__ state, uint32_t *__restrict__ meta) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nc) state[i] = kUnassigned;
-    if (i < kMetaWords && i != 4) meta[i] = (i == 0) ? nc : 0u;
+    if (i < 6 && i != 4) meta[i] = (i == 0) ? nc : 0u;  // keeps error flags and the endpoint total
 }
 
 __global__ void solve_degree(dy_contact_view cv, uint32_t nc, const float *__restrict__ im, uint32_t *__restrict__ deg) {
@@ -140,67 +139,216 @@ __global__ __launch_bounds__(kSolveThreads) void assign_round(dy_contact_view cv
     finish_class(c, used, depth, serial, state, meta);
 }
 
-// Dataflow assignment in one cooperative launch: each thread owns up to kOwn
-// contacts and advances each one's scan of its neighbour lists as far as the
-// neighbours' classes are known, in round robin, so no thread ever blocks on a
-// single item.  The globally highest-priority open contact is always ready, so
-// with every workgroup resident the loop terminates; the spin bound only
-// guards against a logic error (flagged, never expected).
-__global__ __launch_bounds__(kSolveThreads) void assign_dataflow(dy_contact_view cv, uint32_t nc,
-                                                                 const float *__restrict__ im,
+// Number of higher-priority neighbours of each contact (contacts sharing one
+// of its dynamic bodies that precede it: higher JP priority, or a lower index
+// on the reference-order path).  Contacts with none form the first frontier.
+__global__ __launch_bounds__(kSolveThreads) void pending_init(dy_contact_view cv, uint32_t nc,
+                                                              const float *__restrict__ im,
+                                                              const uint32_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ deg,
+                                                              const uint32_t *__restrict__ adj, int serial,
+                                                              uint32_t *__restrict__ pending,
+                                                              uint32_t *__restrict__ frontier, uint32_t *fcount) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
+    const uint64_t pc = jp_priority(c);
+    uint32_t n = 0;
+    for (int e = 0; e < 2; ++e) {
+        const uint32_t x = ends[e];
+        if (im[x] == 0.f) continue;
+        const uint32_t beg = off[x], cnt = deg[x];
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const uint32_t j = adj[beg + q];
+            n += (j != c && higher_priority(j, c, pc, serial)) ? 1u : 0u;
+        }
+    }
+    pending[c] = n;
+    if (n == 0) frontier[atomicAdd(fcount, 1u)] = c;
+}
+
+// Grid-wide barrier for a cooperative launch: arrival counter + generation.
+// Wave 0 of each workgroup publishes the workgroup's stores (agent release)
+// before arriving and acquires after the generation moves.
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint32_t *err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = gen;
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    atomicOr(err, kErrBarrier);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    gen += 1u;
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(v, d, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Wave-aggregated append: one atomic per wave.  Must be reached by every lane
+// of the wave.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t *out, uint32_t *count) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(mask));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader), 64);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (pred) out[base + static_cast<uint32_t>(__popcll(mask & lt))] = val;
+}
+
+constexpr uint32_t kBatch = 8;  // neighbour entries in flight per lane
+
+// Wave-aggregated append of up to kBatch values per lane: one atomic per wave.
+__device__ __forceinline__ void wave_append_batch(const bool (&pred)[kBatch], const uint32_t (&val)[kBatch],
+                                                  uint32_t *out, uint32_t *count) {
+    uint64_t m[kBatch];
+    uint32_t pre[kBatch], total = 0;
+#pragma unroll
+    for (uint32_t t = 0; t < kBatch; ++t) {
+        m[t] = __ballot(pred[t]);
+        pre[t] = total;
+        total += static_cast<uint32_t>(__popcll(m[t]));
+    }
+    if (total == 0) return;
+    const uint32_t lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, total);
+    base = __shfl(base, 0, 64);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (uint32_t t = 0; t < kBatch; ++t)
+        if (pred[t]) out[base + pre[t] + static_cast<uint32_t>(__popcll(m[t] & lt))] = val[t];
+}
+
+// JP rounds over frontiers in one cooperative launch.  Round r assigns every
+// contact whose higher-priority neighbours were all assigned in rounds < r:
+// exactly the contacts the reference colours in its round r, with the same
+// colour (lowest bit free of those neighbours' colours) and depth r.  Each
+// assigned contact then decrements the pending count of its lower-priority
+// neighbours; the ones that reach zero form the next frontier.  States are
+// read only after the barrier that ends the round that wrote them.
+// fcnt[0..2] rotate as frontier sizes (round r appends to fcnt[r % 3]).
+// meta[2] max class, meta[3] rounds, meta[5] contacts assigned.
+__global__ __launch_bounds__(kSolveThreads) void assign_frontier(dy_contact_view cv, const float *__restrict__ im,
                                                                  const uint32_t *__restrict__ off,
                                                                  const uint32_t *__restrict__ deg,
-                                                                 const uint32_t *__restrict__ adj, uint64_t *state,
-                                                                 uint32_t *meta, int serial) {
+                                                                 const uint32_t *__restrict__ adj, int serial,
+                                                                 uint32_t max_rounds, uint32_t *pending,
+                                                                 uint64_t *state, uint32_t *fr0, uint32_t *fr1,
+                                                                 uint32_t *fcnt, uint32_t *meta, uint32_t *bar,
+                                                                 uint32_t *err) {
+    uint32_t gen = 0;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-    uint32_t cc[kOwn], cur[kOwn], na[kOwn], tot[kOwn], oa[kOwn], ob[kOwn], dep[kOwn];
-    uint64_t used[kOwn], pc[kOwn];
-    bool open[kOwn];
-#pragma unroll
-    for (int o = 0; o < kOwn; ++o) {
-        cc[o] = gtid + o * gstride;
-        open[o] = cc[o] < nc;
-        cur[o] = na[o] = tot[o] = oa[o] = ob[o] = dep[o] = 0;
-        used[o] = 0;
-        pc[o] = 0;
-        if (open[o]) {
-            const uint32_t a = cv.body_a[cc[o]], b = cv.body_b[cc[o]];
-            const uint32_t da = im[a] != 0.f ? deg[a] : 0u, db = im[b] != 0.f ? deg[b] : 0u;
-            oa[o] = off[a];
-            ob[o] = off[b];
-            na[o] = da;
-            tot[o] = da + db;
-            pc[o] = jp_priority(cc[o]);
+    const uint32_t lane = lane_id(), wave0 = gtid & ~63u;
+    uint32_t *cur = fr0, *nxt = fr1;
+    uint32_t ncur = fcnt[0];
+    uint32_t my_max = 0, my_done = 0, round = 1;
+    bool exceeded = false;
+    for (; ncur > 0; ++round) {
+        if (round > max_rounds) {  // deeper than the reference allows
+            exceeded = true;
+            break;
         }
-    }
-    for (uint32_t spin = 0; spin < kSpinLimit; ++spin) {
-        bool any = false;
+        uint32_t *nxt_count = &fcnt[round % 3];
+        if (gtid == 0) fcnt[(round + 1) % 3] = 0u;
+        for (uint32_t i0 = wave0; i0 < ncur; i0 += gstride) {
+            const uint32_t i = i0 + lane;
+            const bool act = i < ncur;
+            const uint32_t c = act ? cur[i] : 0u;
+            uint32_t na = 0, nb = 0, oa = 0, ob = 0;
+            uint64_t pc = 0;
+            if (act) {
+                const uint32_t a = cv.body_a[c], b = cv.body_b[c];
+                na = im[a] != 0.f ? deg[a] : 0u;
+                nb = im[b] != 0.f ? deg[b] : 0u;
+                oa = off[a];
+                ob = off[b];
+                pc = jp_priority(c);
+                uint64_t used = 0;
+                const uint32_t tot = na + nb;
+                for (uint32_t q0 = 0; q0 < tot; q0 += kBatch) {
+                    uint32_t jj[kBatch];
+                    uint64_t sv[kBatch];
 #pragma unroll
-        for (int o = 0; o < kOwn; ++o) {
-            if (!open[o]) continue;
-            while (cur[o] < tot[o]) {
-                const uint32_t q = cur[o];
-                const uint32_t j = q < na[o] ? adj[oa[o] + q] : adj[ob[o] + (q - na[o])];
-                if (j != cc[o] && higher_priority(j, cc[o], pc[o], serial)) {
-                    const uint64_t sj = ld_state(&state[j]);
-                    if (sj == kUnassigned) break;
-                    used[o] |= 1ull << (cls_of(sj) & 63u);
-                    const uint32_t dj = dep_of(sj);
-                    dep[o] = dj > dep[o] ? dj : dep[o];
+                    for (uint32_t t = 0; t < kBatch; ++t) {
+                        const uint32_t q = q0 + t;
+                        jj[t] = q < tot ? (q < na ? adj[oa + q] : adj[ob + (q - na)]) : c;
+                    }
+#pragma unroll
+                    for (uint32_t t = 0; t < kBatch; ++t)
+                        sv[t] = (jj[t] != c && higher_priority(jj[t], c, pc, serial)) ? state[jj[t]] : kUnassigned;
+#pragma unroll
+                    for (uint32_t t = 0; t < kBatch; ++t)
+                        if (sv[t] != kUnassigned) used |= 1ull << (cls_of(sv[t]) & 63u);
                 }
-                cur[o] = q + 1;
+                uint32_t cls;
+                if (serial) {
+                    cls = round - 1u;
+                } else if (used == ~0ull) {
+                    atomicOr(&meta[1], 1u);
+                    cls = 64u;
+                } else {
+                    cls = static_cast<uint32_t>(__builtin_ctzll(~used));
+                }
+                state[c] = (static_cast<uint64_t>(round) << 32) | cls;
+                my_max = cls > my_max ? cls : my_max;
+                ++my_done;
             }
-            if (cur[o] == tot[o]) {
-                finish_class(cc[o], used[o], dep[o], serial, state, meta);
-                open[o] = false;
-            } else {
-                any = true;
+            const uint32_t my_cnt = na + nb;
+            const uint32_t mx = wave_max_u32(my_cnt);
+            for (uint32_t q0 = 0; q0 < mx; q0 += kBatch) {
+                uint32_t jj[kBatch], old[kBatch];
+                bool cand[kBatch], ready[kBatch];
+#pragma unroll
+                for (uint32_t t = 0; t < kBatch; ++t) {
+                    const uint32_t q = q0 + t;
+                    jj[t] = q < my_cnt ? (q < na ? adj[oa + q] : adj[ob + (q - na)]) : 0u;
+                    cand[t] = q < my_cnt && jj[t] != c && !higher_priority(jj[t], c, pc, serial);
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < kBatch; ++t) old[t] = cand[t] ? atomicSub(&pending[jj[t]], 1u) : 0u;
+#pragma unroll
+                for (uint32_t t = 0; t < kBatch; ++t) ready[t] = cand[t] && old[t] == 1u;
+                wave_append_batch(ready, jj, nxt, nxt_count);
             }
         }
-        if (!any) return;
-        __builtin_amdgcn_s_sleep(1);
+        grid_barrier(bar, gen, err);
+        ncur = *nxt_count;
+        uint32_t *t = cur; cur = nxt; nxt = t;
     }
-    atomicOr(&meta[4], kErrTimeout);
+    my_max = wave_max_u32(my_max);
+    my_done = wave_sum_u32(my_done);
+    if (lane == 0) {
+        atomicMax(&meta[2], my_max);
+        atomicAdd(&meta[5], my_done);
+    }
+    if (gtid == 0) meta[3] = exceeded ? max_rounds + 1u : round - 1u;
 }
 
 // Class histogram, aggregated per workgroup in LDS when the class count fits.
@@ -326,33 +474,6 @@ __global__ __launch_bounds__(kSolveThreads) void sweep_class(const uint32_t *__r
         solve_one(i, it, rec, lam, dyn, sta, mu, dt);
 }
 
-// Grid-wide barrier for a cooperative launch: arrival counter + generation.
-// Wave 0 of each workgroup publishes the workgroup's stores (agent release)
-// before arriving and acquires after the generation moves.
-__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint32_t *err) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t g = gen;
-        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, kErrBarrier);
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-    }
-    gen += 1u;
-    __syncthreads();
-}
-
 __global__ __launch_bounds__(kSolveThreads) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_cls,
                                                                   uint32_t n_iters, CRec rec, float *__restrict__ lam,
                                                                   float4 *__restrict__ dyn,
@@ -403,22 +524,49 @@ struct dy_xpbd_s {
     uint32_t *meta = nullptr;  // kMetaWords + barrier words
     uint2 *rec_ab = nullptr;
     float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
-    DevBuf<uint32_t> deg, off, fill, scan_tmp;
+    DevBuf<uint32_t> deg, off, fill, scan_tmp, pending, fr0, fr1;
     DevBuf<float4> dyn, sta;
     dy_solve_stats stats{};
-    uint32_t h_meta[kMetaWords] = {};
+    uint32_t h_meta[kMetaWords + 8] = {};
     int coop_assign_blocks = 0, coop_sweep_max = 0, device_cus = 0;
-    int sweep_blocks_override = 0;
+    int sweep_blocks_override = 0, assign_blocks_override = 0;
     int path = 0;  // 0 persistent sweep, 1 per-class launches
+    // Kernel timing (dy_xpbd_set_timing): event pairs recorded around the
+    // assignment and sweep kernels, read back once the next solve syncs.
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_pending = false, ev_assign = false;
+    double pending_bytes = 0.0;
+    dy_solver_timing acc{};
 };
 
 namespace {
+// Adds the event intervals of the previous solve to the totals (the events
+// have completed once the stream reached a later sync point).
+int collect_timing(dy_xpbd_s *x) {
+    if (!x->ev_pending) return DY_OK;
+    float a = 0.f, b = 0.f;
+    DY_TRY(hipEventSynchronize(x->ev[3]));
+    if (x->ev_assign) DY_TRY(hipEventElapsedTime(&a, x->ev[0], x->ev[1]));
+    DY_TRY(hipEventElapsedTime(&b, x->ev[2], x->ev[3]));
+    x->ev_assign = false;
+    x->acc.solves += 1;
+    x->acc.assign_ms += a;
+    x->acc.sweep_ms += b;
+    x->acc.sweep_bytes += x->pending_bytes;
+    x->ev_pending = false;
+    return DY_OK;
+}
+
 void xp_free(dy_xpbd_s *x) {
+    for (hipEvent_t &e : x->ev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     void *ptrs[] = {x->lam, x->state, x->adj, x->boff, x->bfill, x->hist, x->rec_id, x->meta,
                     x->rec_ab, x->rec_p0, x->rec_p1};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release();
+    x->pending.release(); x->fr0.release(); x->fr1.release();
     x->dyn.release(); x->sta.release();
 }
 
@@ -429,37 +577,62 @@ int query_coop(dy_xpbd_s *x) {
     DY_TRY(hipDeviceGetAttribute(&x->device_cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (!coop) return DY_OK;
     int per_cu = 0;
-    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_dataflow),
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_frontier),
                                                         kSolveThreads, 0));
     x->coop_assign_blocks = per_cu * x->device_cus;
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_persistent),
                                                         kSolveThreads, 0));
     x->coop_sweep_max = per_cu * x->device_cus;
     if (const char *e = std::getenv("DYPHUR_SWEEP_BLOCKS")) x->sweep_blocks_override = std::atoi(e);
+    if (const char *e = std::getenv("DYPHUR_ASSIGN_BLOCKS")) x->assign_blocks_override = std::atoi(e);
     if (const char *e = std::getenv("DYPHUR_SOLVER_PATH")) x->path = std::atoi(e);
     return DY_OK;
 }
 
 // Assigns a class to every contact; leaves meta downloaded in x->h_meta.
 int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint32_t nc, const float *im, int serial) {
-    const uint32_t threads = static_cast<uint32_t>(x->coop_assign_blocks) * kSolveThreads;
-    if (x->coop_assign_blocks > 0 && static_cast<uint64_t>(threads) * kOwn >= nc) {
-        uint32_t blocks = ceil_div(ceil_div(nc, kOwn), kSolveThreads);
+    if (x->coop_assign_blocks > 0) {
+        const uint32_t gc = ceil_div(nc, kSolveThreads);
+        DY_TRY_STATUS(x->pending.ensure(nc));
+        DY_TRY_STATUS(x->fr0.ensure(nc));
+        DY_TRY_STATUS(x->fr1.ensure(nc));
+        uint32_t *fcnt = x->meta + kMetaWords + 4;
+        DY_TRY(hipMemsetAsync(fcnt, 0, 3 * sizeof(uint32_t), st));
+        pending_init<<<gc, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->adj, serial, x->pending.p,
+                                                   x->fr0.p, fcnt);
+        DY_LAUNCH_CHECK();
+        uint32_t *bar = x->meta + kMetaWords;
+        DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
+        uint32_t blocks = x->assign_blocks_override > 0 ? static_cast<uint32_t>(x->assign_blocks_override)
+                                                        : static_cast<uint32_t>(x->device_cus / 8);
+        if (blocks < 1) blocks = 1;
         if (blocks > static_cast<uint32_t>(x->coop_assign_blocks)) blocks = x->coop_assign_blocks;
         dy_contact_view cvv = cv;
-        uint32_t ncv = nc;
         const float *imv = im;
         const uint32_t *offv = x->off.p, *degv = x->deg.p, *adjv = x->adj;
-        uint64_t *statev = x->state;
-        uint32_t *metav = x->meta;
         int serialv = serial;
-        void *args[] = {&cvv, &ncv, &imv, &offv, &degv, &adjv, &statev, &metav, &serialv};
-        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_dataflow), dim3(blocks),
+        uint32_t maxr = serial ? nc + 1u : 256u;
+        uint32_t *pendv = x->pending.p, *f0 = x->fr0.p, *f1 = x->fr1.p, *metav = x->meta;
+        uint64_t *statev = x->state;
+        uint32_t *errv = x->meta + kMetaWords + 2;
+        void *args[] = {&cvv, &imv, &offv, &degv, &adjv, &serialv, &maxr, &pendv, &statev, &f0, &f1, &fcnt,
+                        &metav, &bar, &errv};
+        if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
+        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_frontier), dim3(blocks),
                                           dim3(kSolveThreads), args, 0, st));
+        if (x->timing) {
+            DY_TRY(hipEventRecord(x->ev[1], st));
+            x->ev_assign = true;
+        }
         DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
         DY_TRY(hipStreamSynchronize(st));
-        if (x->h_meta[4] & kErrTimeout) return DY_ERR_INTERNAL;
-        return x->h_meta[0] == 0u ? DY_OK : DY_ERR_INTERNAL;
+        if (x->h_meta[4] || x->h_meta[kMetaWords + 2]) return DY_ERR_INTERNAL;
+        if (x->h_meta[3] > maxr) {
+            if (serial) return DY_ERR_INTERNAL;
+            x->h_meta[3] = 257u;  // the reference would stop after 256 rounds
+            return DY_OK;
+        }
+        return x->h_meta[5] == nc ? DY_OK : DY_ERR_INTERNAL;
     }
     const uint32_t grid = ceil_div(nc, kSolveThreads);
     const uint32_t max_rounds = serial ? nc + 1u : 257u;
@@ -502,11 +675,11 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bfill), ncls * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->hist), ncls * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_id), c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), (kMetaWords + 4) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), (kMetaWords + 8) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_ab), c * sizeof(uint2));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p0), c * sizeof(float4));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p1), c * sizeof(float4));
-    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, (kMetaWords + 4) * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, (kMetaWords + 8) * sizeof(uint32_t), hs(s));
     if (e != hipSuccess) {
         xp_free(x);
         delete x;
@@ -564,13 +737,14 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     const uint32_t gc = ceil_div(nc, kSolveThreads);
     solve_degree<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->deg.p);
     DY_LAUNCH_CHECK();
-    DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, nullptr, x->scan_tmp.p));
+    DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + 6, x->scan_tmp.p));
     solve_fill<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->off.p, x->fill.p, x->adj);
     DY_LAUNCH_CHECK();
     gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
 
     int serial = (x->force_serial || nc <= 256u) ? 1 : 0;
+    DY_TRY_STATUS(collect_timing(x));
     DY_TRY_STATUS(assign_classes(x, st, *cv, nc, bv->inv_mass, serial));
     if (!serial) {
         x->stats.overflow = x->h_meta[1];
@@ -599,6 +773,15 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_LAUNCH_CHECK();
 
     const uint32_t iters = static_cast<uint32_t>(x->n_iters);
+    x->stats.dyn_endpoints = x->h_meta[6];
+    if (x->timing) {
+        // Algorithmic bytes of the sweep: per contact per iteration 36 B
+        // record + 8 B lambda, and per dynamic endpoint 80 B read + 52 B
+        // written; plus the per-iteration renormalisation (32 B per body).
+        x->pending_bytes = static_cast<double>(iters) *
+                           (44.0 * nc + 132.0 * x->h_meta[6] + 32.0 * static_cast<double>(nb));
+        DY_TRY(hipEventRecord(x->ev[2], st));
+    }
     if (iters > 0) {
         const bool persistent = x->path == 0 && x->coop_sweep_max > 0;
         if (persistent) {
@@ -629,8 +812,30 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
                 }
         }
     }
+    if (x->timing) {
+        DY_TRY(hipEventRecord(x->ev[3], st));
+        x->ev_pending = true;
+    }
     scatter_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, iters);
     DY_LAUNCH_CHECK();
+    return DY_OK;
+}
+
+int dy_xpbd_set_timing(dy_xpbd_t x, int enable) {
+    if (!x) return DY_ERR_INVALID_ARG;
+    if (enable && !x->ev[0])
+        for (hipEvent_t &e : x->ev) DY_TRY(hipEventCreate(&e));
+    x->timing = enable != 0;
+    x->ev_pending = false;
+    x->acc = dy_solver_timing{};
+    return DY_OK;
+}
+
+int dy_xpbd_get_timing(dy_xpbd_t x, dy_stream_t s, dy_solver_timing *out) {
+    if (!x || !out) return DY_ERR_INVALID_ARG;
+    DY_TRY(hipStreamSynchronize(hs(s)));
+    DY_TRY_STATUS(collect_timing(x));
+    *out = x->acc;
     return DY_OK;
 }
 

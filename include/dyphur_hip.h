@@ -109,7 +109,18 @@ typedef struct dy_solve_stats {
     uint32_t n_classes;      /* colours (coloured path) or dependency levels (serial path) */
     uint32_t rounds;         /* colouring rounds the reference would run (max dependency depth) */
     uint32_t overflow;       /* 1: a body would need more than 64 colours -> serial fallback */
+    uint32_t dyn_endpoints;  /* contact endpoints on dynamic bodies (sum of per-body contact counts) */
 } dy_solve_stats;
+
+/* Kernel timing of the solver, accumulated over solves while enabled: HIP
+ * events recorded on the solve's stream around the class-assignment kernel and
+ * around the sweep kernel. */
+typedef struct dy_solver_timing {
+    uint32_t solves;          /* solves timed */
+    double assign_ms;         /* total class-assignment kernel time */
+    double sweep_ms;          /* total sweep kernel time */
+    double sweep_bytes;       /* total algorithmic bytes of the timed sweeps */
+} dy_solver_timing;
 
 typedef struct dy_stream_s *dy_stream_t;        /* a hipStream_t */
 typedef struct dy_event_s *dy_event_t;          /* a hipEvent_t */
@@ -227,6 +238,10 @@ int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial);
 int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *contacts, uint32_t n_joints,
                   const dy_body_view *bodies, float dt);
 int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out);
+/* Enable (1) or disable (0) kernel timing; enabling resets the totals. */
+int dy_xpbd_set_timing(dy_xpbd_t x, int enable);
+/* Totals of the timed solves; blocking (waits for the stream's last solve). */
+int dy_xpbd_get_timing(dy_xpbd_t x, dy_stream_t s, dy_solver_timing *out);
 /* Class (colour or level) of each contact of the last solve; blocking. */
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n);
 
