@@ -9,11 +9,13 @@
 //    each constraint the lowest colour unused by its higher-priority
 //    neighbours, i.e. greedy colouring in decreasing priority order; the round
 //    in which the reference colours k is the length of the longest
-//    decreasing-priority chain ending at k.  Both are computed here as a
-//    dataflow over the conflict graph (each contact waits for its
-//    higher-priority neighbours), so the values do not depend on timing.  The
-//    reference's serial fallbacks (a body needing > 64 colours, or more than
-//    256 rounds) are detected from the same data.
+//    decreasing-priority chain ending at k.  Both are computed from per-body
+//    contact lists sorted by priority: a contact's higher-priority neighbours
+//    are the contacts ahead of it on its bodies' lists, so round r colours
+//    the contacts that head all their lists, one per body, reading and
+//    extending one used-colour mask per body.  The values do not depend on
+//    timing.  The reference's serial fallbacks (a contact whose neighbours
+//    use all 64 colours, or more than 256 rounds) are detected on the way.
 //  * Reference-order path (<= 256 contacts or force_serial): run_serial
 //    applies contacts 0..nc-1 in order.  Contacts only interact through shared
 //    dynamic bodies, so class = dependency level (1 + the level of the latest
@@ -46,7 +48,7 @@ constexpr uint32_t kSolveThreads = 256;
 constexpr uint32_t kMetaWords = 16;
 constexpr uint32_t kLdsClasses = 2048;   // block-aggregated class counting limit
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr uint32_t kErrTimeout = 1u, kErrBarrier = 2u, kErrHpCapacity = 4u;
+constexpr uint32_t kErrBarrier = 2u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
 __device__ __forceinline__ uint32_t dep_of(uint64_t s) { return static_cast<uint32_t>(s >> 32); }
@@ -55,9 +57,10 @@ __device__ __forceinline__ uint64_t ld_state(const uint64_t *p) {
 }
 
 __global__ void solve_prep(uint32_t nb, uint32_t nc, uint32_t *__restrict__ deg, uint32_t *__restrict__ fill,
-                           float *__restrict__ lam, uint64_t *__restrict__ state, uint32_t *__restrict__ meta) {
+                           uint64_t *__restrict__ used, float *__restrict__ lam, uint64_t *__restrict__ state,
+                           uint32_t *__restrict__ meta) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) { deg[i] = 0; fill[i] = 0; }
+    if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; }
     if (i < nc) { lam[i] = 0.f; state[i] = kUnassigned; }
     if (i < kMetaWords) meta[i] = (i == 0) ? nc : 0u;
 }
@@ -76,95 +79,30 @@ __global__ void solve_degree(dy_contact_view cv, uint32_t nc, const float *__res
     if (im[b] != 0.f) atomicAdd(&deg[b], 1u);
 }
 
+// Unordered per-body lists of dynamic endpoints: adj = (c << 1 | end), own = body.
 __global__ void solve_fill(dy_contact_view cv, uint32_t nc, const float *__restrict__ im,
-                           const uint32_t *__restrict__ off, uint32_t *__restrict__ fill, uint32_t *__restrict__ adj) {
+                           const uint32_t *__restrict__ off, uint32_t *__restrict__ fill, uint32_t *__restrict__ adj,
+                           uint32_t *__restrict__ own) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const uint32_t a = cv.body_a[c], b = cv.body_b[c];
-    if (im[a] != 0.f) adj[off[a] + atomicAdd(&fill[a], 1u)] = c;
-    if (im[b] != 0.f) adj[off[b] + atomicAdd(&fill[b], 1u)] = c;
-}
-
-// Class and depth of contact c once every higher-priority neighbour has one.
-__device__ __forceinline__ void finish_class(uint32_t c, uint64_t used, uint32_t depth, int serial, uint64_t *state,
-                                             uint32_t *meta) {
-    uint32_t cls;
-    if (serial) {
-        cls = depth;
-    } else if (used == ~0ull) {
-        atomicOr(&meta[1], 1u);
-        cls = 64u;
-    } else {
-        cls = static_cast<uint32_t>(__builtin_ctzll(~used));
+    if (im[a] != 0.f) {
+        const uint32_t p = off[a] + atomicAdd(&fill[a], 1u);
+        adj[p] = c << 1;
+        own[p] = a;
     }
-    const uint32_t d = depth + 1u;
-    __hip_atomic_store(&state[c], (static_cast<uint64_t>(d) << 32) | cls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    atomicMax(&meta[2], cls);
-    atomicMax(&meta[3], d);
-    atomicSub(&meta[0], 1u);
-}
-
-__device__ __forceinline__ bool higher_priority(uint32_t j, uint32_t c, uint64_t pc, int serial) {
-    return serial ? (j < c) : (jp_priority(j) > pc);
-}
-
-// Round-based assignment (fallback when a cooperative launch cannot hold the grid).
-__global__ __launch_bounds__(kSolveThreads) void assign_round(dy_contact_view cv, uint32_t nc, const float *__restrict__ im,
-                                                              const uint32_t *__restrict__ off,
-                                                              const uint32_t *__restrict__ deg,
-                                                              const uint32_t *__restrict__ adj, uint64_t *state,
-                                                              uint32_t *meta, int serial) {
-    if (__hip_atomic_load(&meta[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    if (ld_state(&state[c]) != kUnassigned) return;
-    const uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
-    const uint64_t pc = jp_priority(c);
-    uint64_t used = 0;
-    uint32_t depth = 0;
-    for (int e = 0; e < 2; ++e) {
-        const uint32_t x = ends[e];
-        if (im[x] == 0.f) continue;
-        const uint32_t beg = off[x], cnt = deg[x];
-        for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t j = adj[beg + q];
-            if (j == c || !higher_priority(j, c, pc, serial)) continue;
-            const uint64_t sj = ld_state(&state[j]);
-            if (sj == kUnassigned) return;
-            used |= 1ull << (cls_of(sj) & 63u);
-            const uint32_t dj = dep_of(sj);
-            depth = dj > depth ? dj : depth;
-        }
+    if (im[b] != 0.f) {
+        const uint32_t p = off[b] + atomicAdd(&fill[b], 1u);
+        adj[p] = (c << 1) | 1u;
+        own[p] = b;
     }
-    finish_class(c, used, depth, serial, state, meta);
 }
 
-// Number of higher-priority neighbours of each contact (contacts sharing one
-// of its dynamic bodies that precede it: higher JP priority, or a lower index
-// on the reference-order path).  Contacts with none form the first frontier.
-__global__ __launch_bounds__(kSolveThreads) void pending_init(dy_contact_view cv, uint32_t nc,
-                                                              const float *__restrict__ im,
-                                                              const uint32_t *__restrict__ off,
-                                                              const uint32_t *__restrict__ deg,
-                                                              const uint32_t *__restrict__ adj, int serial,
-                                                              uint32_t *__restrict__ pending,
-                                                              uint32_t *__restrict__ frontier, uint32_t *fcount) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
-    const uint64_t pc = jp_priority(c);
-    uint32_t n = 0;
-    for (int e = 0; e < 2; ++e) {
-        const uint32_t x = ends[e];
-        if (im[x] == 0.f) continue;
-        const uint32_t beg = off[x], cnt = deg[x];
-        for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t j = adj[beg + q];
-            n += (j != c && higher_priority(j, c, pc, serial)) ? 1u : 0u;
-        }
-    }
-    pending[c] = n;
-    if (n == 0) frontier[atomicAdd(fcount, 1u)] = c;
+// Contact order used by the assignment: higher key goes first.  JP priority
+// (hash(k) << 32 | k) on the coloured path; lower index first on the
+// reference-order path.
+__device__ __forceinline__ uint64_t order_key(uint32_t c, int serial) {
+    return serial ? static_cast<uint64_t>(0xffffffffu - c) : jp_priority(c);
 }
 
 // Grid-wide barrier for a cooperative launch: arrival counter + generation.
@@ -208,29 +146,15 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
-// Wave-aggregated append: one atomic per wave.  Must be reached by every lane
-// of the wave.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t *out, uint32_t *count) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    const uint32_t lane = lane_id();
-    const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(mask));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, static_cast<int>(leader), 64);
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    if (pred) out[base + static_cast<uint32_t>(__popcll(mask & lt))] = val;
-}
-
-constexpr uint32_t kBatch = 8;  // neighbour entries in flight per lane
-
-// Wave-aggregated append of up to kBatch values per lane: one atomic per wave.
-__device__ __forceinline__ void wave_append_batch(const bool (&pred)[kBatch], const uint32_t (&val)[kBatch],
-                                                  uint32_t *out, uint32_t *count) {
-    uint64_t m[kBatch];
-    uint32_t pre[kBatch], total = 0;
+// Wave-aggregated append of up to N values per lane: one atomic per wave.
+// Must be reached by every lane of the wave.
+template <int N>
+__device__ __forceinline__ void wave_append(const bool (&pred)[N], const uint32_t (&val)[N], uint32_t *out,
+                                            uint32_t *count) {
+    uint64_t m[N];
+    uint32_t pre[N], total = 0;
 #pragma unroll
-    for (uint32_t t = 0; t < kBatch; ++t) {
+    for (int t = 0; t < N; ++t) {
         m[t] = __ballot(pred[t]);
         pre[t] = total;
         total += static_cast<uint32_t>(__popcll(m[t]));
@@ -242,30 +166,96 @@ __device__ __forceinline__ void wave_append_batch(const bool (&pred)[kBatch], co
     base = __shfl(base, 0, 64);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (uint32_t t = 0; t < kBatch; ++t)
+    for (int t = 0; t < N; ++t)
         if (pred[t]) out[base + pre[t] + static_cast<uint32_t>(__popcll(m[t] & lt))] = val[t];
 }
 
-// JP rounds over frontiers in one cooperative launch.  Round r assigns every
-// contact whose higher-priority neighbours were all assigned in rounds < r:
-// exactly the contacts the reference colours in its round r, with the same
-// colour (lowest bit free of those neighbours' colours) and depth r.  Each
-// assigned contact then decrements the pending count of its lower-priority
-// neighbours; the ones that reach zero form the next frontier.  States are
-// read only after the barrier that ends the round that wrote them.
+// Per-body contact lists.  Every dynamic endpoint (contact c, end e) has an
+// entry adj[p] = (c << 1 | e) in its body's segment [off[x], off[x] + deg[x]),
+// and own[p] = x.  list_rank places c at its rank in the body's order
+// (slot[off[x] + rank]) and records the rank per endpoint (crank[2c + e]).
+// Cost per entry: one pass over the body's segment (L2-resident).
+__global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__restrict__ adj,
+                                                           const uint32_t *__restrict__ own,
+                                                           const uint32_t *__restrict__ off,
+                                                           const uint32_t *__restrict__ deg,
+                                                           const uint32_t *__restrict__ total, int serial,
+                                                           uint32_t *__restrict__ slot, uint32_t *__restrict__ crank) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *total) return;
+    const uint32_t v = adj[p], x = own[p];
+    const uint32_t c = v >> 1;
+    const uint64_t kc = order_key(c, serial);
+    const uint32_t beg = off[x], n = deg[x];
+    const uint32_t self = p - beg;
+    uint32_t rank = 0;
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint64_t kq = order_key(adj[beg + q] >> 1, serial);
+        rank += (kq > kc || (kq == kc && q < self)) ? 1u : 0u;  // ties (a == b) by position
+    }
+    slot[beg + rank] = c;
+    crank[2u * c + (v & 1u)] = rank;
+}
+
+constexpr uint32_t kNone = 0xffffffffu;
+
+// Links each contact to its successor in both endpoint lists and counts the
+// lists it does not head yet (pending).  Contacts heading every list of a
+// dynamic endpoint form the first frontier.  lrec[c] = (a or none, b or none,
+// next on a, next on b); static endpoints are "none".
+__global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, uint32_t nc,
+                                                           const float *__restrict__ im,
+                                                           const uint32_t *__restrict__ off,
+                                                           const uint32_t *__restrict__ deg,
+                                                           const uint32_t *__restrict__ slot,
+                                                           const uint32_t *__restrict__ crank,
+                                                           uint4 *__restrict__ lrec, uint32_t *__restrict__ pending,
+                                                           uint32_t *__restrict__ frontier, uint32_t *fcount) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ready = false;
+    if (c < nc) {
+        uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
+        uint32_t next[2] = {kNone, kNone};
+        uint32_t wait = 0;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const uint32_t x = ends[e];
+            if (im[x] == 0.f) {
+                ends[e] = kNone;
+                continue;
+            }
+            const uint32_t r = crank[2u * c + e];
+            if (r + 1u < deg[x]) next[e] = slot[off[x] + r + 1u];
+            wait += r != 0u ? 1u : 0u;
+        }
+        lrec[c] = make_uint4(ends[0], ends[1], next[0], next[1]);
+        pending[c] = wait;
+        ready = wait == 0u;
+    }
+    const bool pred[1] = {ready};
+    const uint32_t val[1] = {c};
+    wave_append<1>(pred, val, frontier, fcount);
+}
+
+// JP rounds over frontiers in one cooperative launch.  Round r assigns the
+// contacts that head the ordered lists of all their dynamic endpoints: on a
+// body's list every earlier contact is a higher-priority neighbour, so these
+// are exactly the contacts whose higher-priority neighbours were all assigned
+// before round r — the ones the reference colours in its round r.  A body
+// heads at most one contact per round, so the contact owns its bodies' masks
+// of used colours during the round: class = lowest colour free in
+// used[a] | used[b] (round - 1 on the reference-order path), then both masks
+// gain the class and both successors move one list closer to ready.  Values
+// written in a round are read only after the barrier that ends it.
 // fcnt[0..2] rotate as frontier sizes (round r appends to fcnt[r % 3]).
-// meta[2] max class, meta[3] rounds, meta[5] contacts assigned.
-__global__ __launch_bounds__(kSolveThreads) void assign_frontier(dy_contact_view cv, const float *__restrict__ im,
-                                                                 const uint32_t *__restrict__ off,
-                                                                 const uint32_t *__restrict__ deg,
-                                                                 const uint32_t *__restrict__ adj, int serial,
-                                                                 uint32_t max_rounds, uint32_t *pending,
-                                                                 uint64_t *state, uint32_t *fr0, uint32_t *fr1,
-                                                                 uint32_t *fcnt, uint32_t *meta, uint32_t *bar,
-                                                                 uint32_t *err) {
+// meta[1] colour overflow, meta[2] max class, meta[3] rounds, meta[5] assigned.
+__global__ __launch_bounds__(1024) void assign_lists(const uint4 *__restrict__ lrec, int serial, uint32_t max_rounds,
+                                                     uint32_t *pending, uint64_t *used, uint64_t *state, uint32_t *fr0,
+                                                     uint32_t *fr1, uint32_t *fcnt, uint32_t *meta, uint32_t *bar,
+                                                     uint32_t *err) {
     uint32_t gen = 0;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-    const uint32_t lane = lane_id(), wave0 = gtid & ~63u;
+    const uint32_t wave0 = gtid & ~63u, lane = lane_id();
     uint32_t *cur = fr0, *nxt = fr1;
     uint32_t ncur = fcnt[0];
     uint32_t my_max = 0, my_done = 0, round = 1;
@@ -279,64 +269,39 @@ __global__ __launch_bounds__(kSolveThreads) void assign_frontier(dy_contact_view
         if (gtid == 0) fcnt[(round + 1) % 3] = 0u;
         for (uint32_t i0 = wave0; i0 < ncur; i0 += gstride) {
             const uint32_t i = i0 + lane;
-            const bool act = i < ncur;
-            const uint32_t c = act ? cur[i] : 0u;
-            uint32_t na = 0, nb = 0, oa = 0, ob = 0;
-            uint64_t pc = 0;
-            if (act) {
-                const uint32_t a = cv.body_a[c], b = cv.body_b[c];
-                na = im[a] != 0.f ? deg[a] : 0u;
-                nb = im[b] != 0.f ? deg[b] : 0u;
-                oa = off[a];
-                ob = off[b];
-                pc = jp_priority(c);
-                uint64_t used = 0;
-                const uint32_t tot = na + nb;
-                for (uint32_t q0 = 0; q0 < tot; q0 += kBatch) {
-                    uint32_t jj[kBatch];
-                    uint64_t sv[kBatch];
-#pragma unroll
-                    for (uint32_t t = 0; t < kBatch; ++t) {
-                        const uint32_t q = q0 + t;
-                        jj[t] = q < tot ? (q < na ? adj[oa + q] : adj[ob + (q - na)]) : c;
-                    }
-#pragma unroll
-                    for (uint32_t t = 0; t < kBatch; ++t)
-                        sv[t] = (jj[t] != c && higher_priority(jj[t], c, pc, serial)) ? state[jj[t]] : kUnassigned;
-#pragma unroll
-                    for (uint32_t t = 0; t < kBatch; ++t)
-                        if (sv[t] != kUnassigned) used |= 1ull << (cls_of(sv[t]) & 63u);
-                }
+            bool ready[2] = {false, false};
+            uint32_t nx[2] = {0u, 0u};
+            if (i < ncur) {
+                const uint32_t c = cur[i];
+                const uint4 r = lrec[c];
                 uint32_t cls;
                 if (serial) {
                     cls = round - 1u;
-                } else if (used == ~0ull) {
-                    atomicOr(&meta[1], 1u);
-                    cls = 64u;
                 } else {
-                    cls = static_cast<uint32_t>(__builtin_ctzll(~used));
+                    const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
+                    const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
+                    const uint64_t u = ua | ub;
+                    if (u == ~0ull) {
+                        atomicOr(&meta[1], 1u);
+                        cls = 64u;
+                    } else {
+                        cls = static_cast<uint32_t>(__builtin_ctzll(~u));
+                    }
+                    const uint64_t bit = 1ull << (cls & 63u);
+                    if (r.x != kNone) used[r.x] = ua | bit;
+                    if (r.y != kNone) used[r.y] = ub | bit;
                 }
                 state[c] = (static_cast<uint64_t>(round) << 32) | cls;
                 my_max = cls > my_max ? cls : my_max;
                 ++my_done;
+                nx[0] = r.z;
+                nx[1] = r.w;
+                const uint32_t oa = r.z != kNone ? atomicSub(&pending[r.z], 1u) : 0u;
+                const uint32_t ob = r.w != kNone ? atomicSub(&pending[r.w], 1u) : 0u;
+                ready[0] = r.z != kNone && oa == 1u;
+                ready[1] = r.w != kNone && ob == 1u;
             }
-            const uint32_t my_cnt = na + nb;
-            const uint32_t mx = wave_max_u32(my_cnt);
-            for (uint32_t q0 = 0; q0 < mx; q0 += kBatch) {
-                uint32_t jj[kBatch], old[kBatch];
-                bool cand[kBatch], ready[kBatch];
-#pragma unroll
-                for (uint32_t t = 0; t < kBatch; ++t) {
-                    const uint32_t q = q0 + t;
-                    jj[t] = q < my_cnt ? (q < na ? adj[oa + q] : adj[ob + (q - na)]) : 0u;
-                    cand[t] = q < my_cnt && jj[t] != c && !higher_priority(jj[t], c, pc, serial);
-                }
-#pragma unroll
-                for (uint32_t t = 0; t < kBatch; ++t) old[t] = cand[t] ? atomicSub(&pending[jj[t]], 1u) : 0u;
-#pragma unroll
-                for (uint32_t t = 0; t < kBatch; ++t) ready[t] = cand[t] && old[t] == 1u;
-                wave_append_batch(ready, jj, nxt, nxt_count);
-            }
+            wave_append<2>(ready, nx, nxt, nxt_count);
         }
         grid_barrier(bar, gen, err);
         ncur = *nxt_count;
@@ -350,6 +315,58 @@ __global__ __launch_bounds__(kSolveThreads) void assign_frontier(dy_contact_view
     }
     if (gtid == 0) meta[3] = exceeded ? max_rounds + 1u : round - 1u;
 }
+
+// One round per launch (used when a cooperative launch is unavailable or
+// disabled): the frontier of round r is read from fr[r & 1] with its size in
+// fcnt[(r - 1) % 3], the next one written to fr[(r + 1) & 1].
+__global__ __launch_bounds__(kSolveThreads) void assign_lists_round(const uint4 *__restrict__ lrec, int serial,
+                                                                    uint32_t round, uint32_t max_cur,
+                                                                    uint32_t *pending, uint64_t *used,
+                                                                    uint64_t *state, uint32_t *fr0, uint32_t *fr1,
+                                                                    uint32_t *fcnt, uint32_t *meta) {
+    const uint32_t ncur = fcnt[(round - 1u) % 3u];
+    uint32_t *cur = (round & 1u) ? fr0 : fr1;
+    uint32_t *nxt = (round & 1u) ? fr1 : fr0;
+    uint32_t *nxt_count = &fcnt[round % 3u];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && ncur > 0) atomicMax(&meta[3], round);
+    bool ready[2] = {false, false};
+    uint32_t nx[2] = {0u, 0u};
+    if (i < ncur && i < max_cur) {
+        const uint32_t c = cur[i];
+        const uint4 r = lrec[c];
+        uint32_t cls;
+        if (serial) {
+            cls = round - 1u;
+        } else {
+            const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
+            const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
+            const uint64_t u = ua | ub;
+            if (u == ~0ull) {
+                atomicOr(&meta[1], 1u);
+                cls = 64u;
+            } else {
+                cls = static_cast<uint32_t>(__builtin_ctzll(~u));
+            }
+            const uint64_t bit = 1ull << (cls & 63u);
+            if (r.x != kNone) used[r.x] = ua | bit;
+            if (r.y != kNone) used[r.y] = ub | bit;
+        }
+        state[c] = (static_cast<uint64_t>(round) << 32) | cls;
+        atomicMax(&meta[2], cls);
+        atomicAdd(&meta[5], 1u);
+        nx[0] = r.z;
+        nx[1] = r.w;
+        const uint32_t oa = r.z != kNone ? atomicSub(&pending[r.z], 1u) : 0u;
+        const uint32_t ob = r.w != kNone ? atomicSub(&pending[r.w], 1u) : 0u;
+        ready[0] = r.z != kNone && oa == 1u;
+        ready[1] = r.w != kNone && ob == 1u;
+    }
+    wave_append<2>(ready, nx, nxt, nxt_count);
+}
+
+// Clears the frontier count two rounds ahead (launched between rounds).
+__global__ void clear_count(uint32_t *fcnt, uint32_t idx) { fcnt[idx] = 0u; }
 
 // Class histogram, aggregated per workgroup in LDS when the class count fits.
 __global__ __launch_bounds__(kSolveThreads) void class_count(uint32_t nc, uint32_t n_cls,
@@ -474,7 +491,7 @@ __global__ __launch_bounds__(kSolveThreads) void sweep_class(const uint32_t *__r
         solve_one(i, it, rec, lam, dyn, sta, mu, dt);
 }
 
-__global__ __launch_bounds__(kSolveThreads) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_cls,
+__global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_cls,
                                                                   uint32_t n_iters, CRec rec, float *__restrict__ lam,
                                                                   float4 *__restrict__ dyn,
                                                                   const float4 *__restrict__ sta, float mu, float dt,
@@ -520,16 +537,20 @@ struct dy_xpbd_s {
     uint32_t cap = 0;
     float *lam = nullptr;
     uint64_t *state = nullptr;
-    uint32_t *adj = nullptr, *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr;
+    uint32_t *adj = nullptr, *own = nullptr, *slot = nullptr, *crank = nullptr;
+    uint32_t *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr;
+    uint4 *lrec = nullptr;
     uint32_t *meta = nullptr;  // kMetaWords + barrier words
     uint2 *rec_ab = nullptr;
     float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
     DevBuf<uint32_t> deg, off, fill, scan_tmp, pending, fr0, fr1;
+    DevBuf<uint64_t> used;
     DevBuf<float4> dyn, sta;
     dy_solve_stats stats{};
     uint32_t h_meta[kMetaWords + 8] = {};
-    int coop_assign_blocks = 0, coop_sweep_max = 0, device_cus = 0;
+    int coop_assign_max = 0, coop_sweep_max = 0, device_cus = 0;
     int sweep_blocks_override = 0, assign_blocks_override = 0;
+    int assign_threads = 256, sweep_threads = 256;
     int path = 0;  // 0 persistent sweep, 1 per-class launches
     // Kernel timing (dy_xpbd_set_timing): event pairs recorded around the
     // assignment and sweep kernels, read back once the next solve syncs.
@@ -561,65 +582,80 @@ int collect_timing(dy_xpbd_s *x) {
 void xp_free(dy_xpbd_s *x) {
     for (hipEvent_t &e : x->ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
-    void *ptrs[] = {x->lam, x->state, x->adj, x->boff, x->bfill, x->hist, x->rec_id, x->meta,
-                    x->rec_ab, x->rec_p0, x->rec_p1};
+    void *ptrs[] = {x->lam, x->state, x->adj, x->own, x->slot, x->crank, x->lrec, x->boff, x->bfill, x->hist,
+                    x->rec_id, x->meta, x->rec_ab, x->rec_p0, x->rec_p1};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release();
-    x->pending.release(); x->fr0.release(); x->fr1.release();
+    x->pending.release(); x->fr0.release(); x->fr1.release(); x->used.release();
     x->dyn.release(); x->sta.release();
 }
 
+int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// Cooperative-launch limits; DYPHUR_{ASSIGN,SWEEP}_{BLOCKS,THREADS} and
+// DYPHUR_SOLVER_PATH (1 = one launch per round / per class) override the
+// launch shapes for measurement.
 int query_coop(dy_xpbd_s *x) {
     int dev = 0, coop = 0;
     DY_TRY(hipGetDevice(&dev));
     DY_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     DY_TRY(hipDeviceGetAttribute(&x->device_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    x->sweep_blocks_override = env_int("DYPHUR_SWEEP_BLOCKS", 0);
+    x->assign_blocks_override = env_int("DYPHUR_ASSIGN_BLOCKS", 0);
+    x->path = env_int("DYPHUR_SOLVER_PATH", 0);
+    const int at = env_int("DYPHUR_ASSIGN_THREADS", 256), sw = env_int("DYPHUR_SWEEP_THREADS", 256);
+    x->assign_threads = (at == 64 || at == 128 || at == 256 || at == 512 || at == 1024) ? at : 256;
+    x->sweep_threads = (sw == 64 || sw == 128 || sw == 256 || sw == 512 || sw == 1024) ? sw : 256;
     if (!coop) return DY_OK;
     int per_cu = 0;
-    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_frontier),
-                                                        kSolveThreads, 0));
-    x->coop_assign_blocks = per_cu * x->device_cus;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_lists),
+                                                        x->assign_threads, 0));
+    x->coop_assign_max = per_cu * x->device_cus;
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_persistent),
-                                                        kSolveThreads, 0));
+                                                        x->sweep_threads, 0));
     x->coop_sweep_max = per_cu * x->device_cus;
-    if (const char *e = std::getenv("DYPHUR_SWEEP_BLOCKS")) x->sweep_blocks_override = std::atoi(e);
-    if (const char *e = std::getenv("DYPHUR_ASSIGN_BLOCKS")) x->assign_blocks_override = std::atoi(e);
-    if (const char *e = std::getenv("DYPHUR_SOLVER_PATH")) x->path = std::atoi(e);
     return DY_OK;
+}
+
+// Blocks for a cooperative kernel: the override, else device_cus / 8 workgroups
+// of 256 threads' worth, clamped to what can be co-resident.
+uint32_t coop_blocks(const dy_xpbd_s *x, int override_blocks, int threads, int max_blocks) {
+    int b = override_blocks > 0 ? override_blocks : (x->device_cus / 8) * 256 / threads;
+    if (b < 1) b = 1;
+    if (b > max_blocks) b = max_blocks;
+    return static_cast<uint32_t>(b);
 }
 
 // Assigns a class to every contact; leaves meta downloaded in x->h_meta.
 int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint32_t nc, const float *im, int serial) {
-    if (x->coop_assign_blocks > 0) {
-        const uint32_t gc = ceil_div(nc, kSolveThreads);
-        DY_TRY_STATUS(x->pending.ensure(nc));
-        DY_TRY_STATUS(x->fr0.ensure(nc));
-        DY_TRY_STATUS(x->fr1.ensure(nc));
-        uint32_t *fcnt = x->meta + kMetaWords + 4;
-        DY_TRY(hipMemsetAsync(fcnt, 0, 3 * sizeof(uint32_t), st));
-        pending_init<<<gc, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->adj, serial, x->pending.p,
-                                                   x->fr0.p, fcnt);
-        DY_LAUNCH_CHECK();
+    const uint32_t gc = ceil_div(nc, kSolveThreads);
+    uint32_t *fcnt = x->meta + kMetaWords + 4;
+    DY_TRY(hipMemsetAsync(fcnt, 0, 3 * sizeof(uint32_t), st));
+    list_rank<<<ceil_div(2u * nc, kSolveThreads), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p,
+                                                                          x->meta + 6, serial, x->slot, x->crank);
+    DY_LAUNCH_CHECK();
+    list_link<<<gc, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->slot, x->crank, x->lrec,
+                                             x->pending.p, x->fr0.p, fcnt);
+    DY_LAUNCH_CHECK();
+    const uint32_t max_rounds = serial ? nc + 1u : 256u;
+    if (x->coop_assign_max > 0 && x->path == 0) {
         uint32_t *bar = x->meta + kMetaWords;
         DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
-        uint32_t blocks = x->assign_blocks_override > 0 ? static_cast<uint32_t>(x->assign_blocks_override)
-                                                        : static_cast<uint32_t>(x->device_cus / 8);
-        if (blocks < 1) blocks = 1;
-        if (blocks > static_cast<uint32_t>(x->coop_assign_blocks)) blocks = x->coop_assign_blocks;
-        dy_contact_view cvv = cv;
-        const float *imv = im;
-        const uint32_t *offv = x->off.p, *degv = x->deg.p, *adjv = x->adj;
+        const uint32_t blocks = coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
+        const uint4 *lrecv = x->lrec;
         int serialv = serial;
-        uint32_t maxr = serial ? nc + 1u : 256u;
+        uint32_t maxr = max_rounds;
         uint32_t *pendv = x->pending.p, *f0 = x->fr0.p, *f1 = x->fr1.p, *metav = x->meta;
-        uint64_t *statev = x->state;
+        uint64_t *usedv = x->used.p, *statev = x->state;
         uint32_t *errv = x->meta + kMetaWords + 2;
-        void *args[] = {&cvv, &imv, &offv, &degv, &adjv, &serialv, &maxr, &pendv, &statev, &f0, &f1, &fcnt,
-                        &metav, &bar, &errv};
+        void *args[] = {&lrecv, &serialv, &maxr, &pendv, &usedv, &statev, &f0, &f1, &fcnt, &metav, &bar, &errv};
         if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
-        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_frontier), dim3(blocks),
-                                          dim3(kSolveThreads), args, 0, st));
+        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
+                                          dim3(static_cast<uint32_t>(x->assign_threads)), args, 0, st));
         if (x->timing) {
             DY_TRY(hipEventRecord(x->ev[1], st));
             x->ev_assign = true;
@@ -627,33 +663,30 @@ int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint
         DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
         DY_TRY(hipStreamSynchronize(st));
         if (x->h_meta[4] || x->h_meta[kMetaWords + 2]) return DY_ERR_INTERNAL;
-        if (x->h_meta[3] > maxr) {
+        if (x->h_meta[3] > max_rounds) {
             if (serial) return DY_ERR_INTERNAL;
             x->h_meta[3] = 257u;  // the reference would stop after 256 rounds
             return DY_OK;
         }
         return x->h_meta[5] == nc ? DY_OK : DY_ERR_INTERNAL;
     }
-    const uint32_t grid = ceil_div(nc, kSolveThreads);
-    const uint32_t max_rounds = serial ? nc + 1u : 257u;
-    uint32_t rounds = 0, batch = 16;
-    while (true) {
-        for (uint32_t r = 0; r < batch && rounds < max_rounds; ++r, ++rounds) {
-            assign_round<<<grid, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->adj, x->state, x->meta,
-                                                         serial);
-            DY_LAUNCH_CHECK();
+    // One launch per round, checked every 16 rounds.
+    for (uint32_t r = 1; r <= max_rounds; ++r) {
+        clear_count<<<1, 1, 0, st>>>(fcnt, r % 3u);
+        DY_LAUNCH_CHECK();
+        assign_lists_round<<<gc, kSolveThreads, 0, st>>>(x->lrec, serial, r, nc, x->pending.p, x->used.p, x->state,
+                                                          x->fr0.p, x->fr1.p, fcnt, x->meta);
+        DY_LAUNCH_CHECK();
+        if (r % 16u == 0u || r == max_rounds) {
+            DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
+            DY_TRY(hipStreamSynchronize(st));
+            if (x->h_meta[4]) return DY_ERR_INTERNAL;
+            if (x->h_meta[5] == nc) return DY_OK;
         }
-        DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
-        DY_TRY(hipStreamSynchronize(st));
-        if (x->h_meta[0] == 0u) return DY_OK;
-        // Colouring deeper than the reference's 256 rounds: report via depth.
-        if (rounds >= max_rounds) {
-            if (serial) return DY_ERR_INTERNAL;
-            x->h_meta[3] = 257u;
-            return DY_OK;
-        }
-        batch = 16;
     }
+    if (serial) return DY_ERR_INTERNAL;
+    x->h_meta[3] = 257u;
+    return DY_OK;
 }
 }  // namespace
 
@@ -671,6 +704,10 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&x->lam), c * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->state), c * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->adj), 2 * c * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->own), 2 * c * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->slot), 2 * c * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->crank), 2 * c * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->lrec), c * sizeof(uint4));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->boff), ncls * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bfill), ncls * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->hist), ncls * sizeof(uint32_t));
@@ -728,17 +765,22 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_TRY_STATUS(x->deg.ensure(nb));
     DY_TRY_STATUS(x->off.ensure(nb));
     DY_TRY_STATUS(x->fill.ensure(nb));
+    DY_TRY_STATUS(x->used.ensure(nb));
+    DY_TRY_STATUS(x->pending.ensure(nc));
+    DY_TRY_STATUS(x->fr0.ensure(nc));
+    DY_TRY_STATUS(x->fr1.ensure(nc));
     DY_TRY_STATUS(x->dyn.ensure(4 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->sta.ensure(2 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > nc + 66u ? nb : nc + 66u)));
     const uint32_t prep_n = nb > nc ? nb : nc;
-    solve_prep<<<ceil_div(prep_n, 256), 256, 0, st>>>(nb, nc, x->deg.p, x->fill.p, x->lam, x->state, x->meta);
+    solve_prep<<<ceil_div(prep_n, 256), 256, 0, st>>>(nb, nc, x->deg.p, x->fill.p, x->used.p, x->lam, x->state,
+                                                      x->meta);
     DY_LAUNCH_CHECK();
     const uint32_t gc = ceil_div(nc, kSolveThreads);
     solve_degree<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->deg.p);
     DY_LAUNCH_CHECK();
     DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + 6, x->scan_tmp.p));
-    solve_fill<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->off.p, x->fill.p, x->adj);
+    solve_fill<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->off.p, x->fill.p, x->adj, x->own);
     DY_LAUNCH_CHECK();
     gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
@@ -785,10 +827,7 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     if (iters > 0) {
         const bool persistent = x->path == 0 && x->coop_sweep_max > 0;
         if (persistent) {
-            uint32_t blocks = x->sweep_blocks_override > 0 ? static_cast<uint32_t>(x->sweep_blocks_override)
-                                                           : static_cast<uint32_t>(x->device_cus / 8);
-            if (blocks < 1) blocks = 1;
-            if (blocks > static_cast<uint32_t>(x->coop_sweep_max)) blocks = x->coop_sweep_max;
+            const uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max);
             uint32_t *bar = x->meta + kMetaWords;
             DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
             const uint32_t *boffv = x->boff;
@@ -800,7 +839,7 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
             uint32_t *errv = x->meta + kMetaWords + 2;
             void *args[] = {&boffv, &kv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &bar, &errv};
             DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_persistent), dim3(blocks),
-                                              dim3(kSolveThreads), args, 0, st));
+                                              dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
         } else {
             // Per-class launches; each reads its range from the device offsets.
             const uint32_t blocks = gc < 1024u ? gc : 1024u;

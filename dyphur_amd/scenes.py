@@ -202,6 +202,50 @@ def determinism_boxes(nx: int = 2, ny: int = 2, nz: int = 2) -> Scene:
 
 
 # ---------------------------------------------------------------------------
+# One dynamic plate carrying a grid of spheres: every sphere contact shares the
+# plate, so the contacts form one clique and Jones-Plassmann colouring runs out
+# of its 64 colours (core/src/xpbd_solver.cpp:639-651 then falls back to the
+# serial path).
+# ---------------------------------------------------------------------------
+def star_plate(nx: int = 20, nz: int = 15, pitch: float = 0.5, radius: float = 0.2) -> Scene:
+    n_sph = nx * nz
+    n = n_sph + 2
+    gnd_hy = F32(0.5)
+    plate_h = np.array([F32(0.5) * F32(nx) * F32(pitch) + F32(0.25), F32(0.25),
+                        F32(0.5) * F32(nz) * F32(pitch) + F32(0.25)], F32)
+    r = F32(radius)
+    b = _alloc(n)
+    plate_y = gnd_hy + plate_h[1] - F32(0.005)
+    sph_y = plate_y + plate_h[1] + r - F32(0.005)
+    k = 0
+    for ix in range(nx):
+        for iz in range(nz):
+            b["pos"][k] = (F32(ix) - F32(0.5) * F32(nx - 1)) * F32(pitch), sph_y, \
+                          (F32(iz) - F32(0.5) * F32(nz - 1)) * F32(pitch)
+            b["inertia"][k] = _diag(F32(0.4) * r * r)
+            b["shape"][k] = 0
+            k += 1
+    m_plate = F32(50.0)
+    b["pos"][n_sph] = 0.0, plate_y, 0.0
+    b["mass"][n_sph] = m_plate
+    b["inertia"][n_sph] = np.array([m_plate / F32(3.0) * (plate_h[1] * plate_h[1] + plate_h[2] * plate_h[2]), 0, 0,
+                                    0, m_plate / F32(3.0) * (plate_h[0] * plate_h[0] + plate_h[2] * plate_h[2]), 0,
+                                    0, 0, m_plate / F32(3.0) * (plate_h[0] * plate_h[0] + plate_h[1] * plate_h[1])],
+                                   F32)
+    b["shape"][n_sph] = 1
+    b["mass"][n_sph + 1] = 0.0
+    b["shape"][n_sph + 1] = 2
+    b["flags"][n_sph + 1] = FLAG_STATIC
+    gnd = F32(max(float(plate_h[0]), float(plate_h[2])) + 2.0)
+    return Scene(
+        name=f"star_plate_{nx}x{nz}",
+        shape_type=np.array([SHAPE_SPHERE, SHAPE_BOX, SHAPE_BOX], np.uint32),
+        shape_half=np.array([[r, r, r], plate_h, [gnd, gnd_hy, gnd]], F32),
+        bounds=np.array([-gnd - 2.0, -2.0, -gnd - 2.0, gnd + 2.0, 10.0, gnd + 2.0], F32),
+        max_pairs=max(4096, n * 16), max_contacts=max(4096, n * 8), frames=10, **b)
+
+
+# ---------------------------------------------------------------------------
 # Random mixed boxes/spheres in a volume above a ground plate
 # ---------------------------------------------------------------------------
 def _shoemake(rng: Pcg32) -> tuple[float, float, float, float]:

@@ -184,3 +184,63 @@ def test_coloured_run_to_run_identical(gpu):
             w.step()
         hs.append(w.fnv())
     assert hs[0] == hs[1]
+
+
+def test_colour_overflow_falls_back_to_serial(gpu, oracle):
+    """301 contacts sharing one dynamic plate: JP runs out of its 64 colours and
+    the reference solves in contact order; the GPU must take the same path."""
+    sc = scenes.star_plate()
+    b, s = oracle.Bodies(sc), oracle.Shapes(sc)
+    ow = oracle.World.for_scene(sc)
+    gw = gpu.World(sc)
+    for f in range(4):
+        print(f"[star_plate] frame {f}", flush=True)
+        ost = ow.frame(b, s, sc)
+        gst = gw.step()
+        assert (gst["n_pairs"], gst["n_contacts"]) == (ost["n_pairs"], ost["n_contacts"]), f
+        assert bool(gst["used_serial"]) == bool(ost["used_serial"]), f
+        if f == 0:
+            assert gst["n_contacts"] > 256 and gst["used_serial"]
+            assert gw.solver.stats()["overflow"] == 1
+        assert gw.fnv() == b.fnv(), f"frame {f}"
+
+
+LAUNCH_SHAPES = [
+    ("per_round_launches", {"DYPHUR_SOLVER_PATH": "1"}),
+    ("wg1024", {"DYPHUR_ASSIGN_THREADS": "1024", "DYPHUR_SWEEP_THREADS": "1024"}),
+    ("one_block", {"DYPHUR_ASSIGN_BLOCKS": "1", "DYPHUR_SWEEP_BLOCKS": "1"}),
+]
+
+
+@pytest.mark.parametrize("shape", [s[0] for s in LAUNCH_SHAPES])
+@pytest.mark.parametrize("force_serial", [False, True])
+def test_solver_launch_shapes_bitexact(gpu, oracle, stage_inputs, monkeypatch, shape, force_serial):
+    """The class assignment and sweeps give the same bits under every launch
+    shape the solver supports (one launch per round/class, 1024-thread
+    workgroups, a single workgroup)."""
+    for k, v in dict(LAUNCH_SHAPES)[shape].items():
+        monkeypatch.setenv(k, v)
+    sc, ob0, osh = stage_inputs["mixed_3000_f30"]
+    print(f"[solver shape] {shape} serial={force_serial}", flush=True)
+    ob = ob0.copy()
+    w = _contacts_for(oracle, sc, ob, osh)
+    c = w.contacts()
+    st_ref = w.solve(ob, sc.dt, force_serial=force_serial)
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob0.state())
+    ds = gpu.DeviceShapes(sc, s)
+    np_ = gpu.Narrowphase(s, sc.max_contacts)
+    pairs = np.stack([c["body_a"], c["body_b"]], 1).astype(U32)
+    dp = gpu.DeviceArray.from_numpy(pairs.ravel(), s)
+    np_.run(s, dp.ptr, len(pairs), db, ds)
+    solver = gpu.XpbdSolver(s, sc.n_iters, sc.max_contacts, sc.mu)
+    solver.set_force_serial(force_serial)
+    solver.solve(s, np_.contacts_view(), db, sc.dt)
+    st = solver.stats()
+    assert st["used_serial"] == st_ref["used_serial"]
+    if not st_ref["used_serial"]:
+        assert np.array_equal(solver.classes(s, c["n"]), w.colors(c["n"]))
+    got, ref = db.download(s), ob.state()
+    for f in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
+              "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z"):
+        assert np.array_equal(bits(got[f]), bits(ref[f])), f

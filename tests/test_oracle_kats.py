@@ -257,3 +257,28 @@ def test_stress_scene_load_matches_report(oracle):
     assert 5000 < st["n_pairs"] < 12000
     assert 2500 < st["n_contacts"] < 6000
     assert st["used_serial"] == 0 and 4 <= st["n_colors"] <= 64
+
+
+# core/src/xpbd_solver.cpp:639-651: a contact whose higher-priority neighbours
+# hold all 64 colours sends the whole solve to the serial path.  The star_plate
+# contacts all share the plate, so they form one clique of 301 and greedy
+# colouring needs 301 colours; the oracle must report overflow and solve in
+# contact order, i.e. give the same bits as a forced serial solve.
+def test_star_plate_overflows_to_serial(oracle):
+    sc = scenes.star_plate()
+    res = []
+    for serial in (False, True):
+        b, s = oracle.Bodies(sc), oracle.Shapes(sc)
+        oracle.integrate(b, sc.gravity, sc.dt, sc.n_substeps)
+        w = oracle.World.for_scene(sc)
+        n_pairs = w.build_and_query(b, s, sc.bounds)
+        w.sort_pairs(n_pairs)
+        nc = w.narrowphase(w.pairs(n_pairs), b, s)
+        assert nc == 301
+        st = w.solve(b, sc.dt, force_serial=serial)
+        assert st["used_serial"] == 1
+        if not serial:
+            assert st["overflow"] == 1
+        res.append(b.state())
+    for f in ("pos_x", "pos_y", "vel_y", "ang_x", "rot_w"):
+        assert np.array_equal(res[0][f].view(np.uint32), res[1][f].view(np.uint32)), f
