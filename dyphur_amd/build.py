@@ -4,6 +4,7 @@
   dyphur_amd/lib/libdyphur_hostmath.so  the same per-element arithmetic (dy_math.hpp)
                                         compiled for the host, for CPU parity checks
   dyphur_amd/dyphur_py*.so              the Python module over the C++ surface
+  tests/cpp/_build/dyphur_cpp_tests     the C++ API tests (dyphur_amd/cpp/include)
 
 Run ``python -m dyphur_amd.build`` (or __graft_entry__.build()).  Every call
 recompiles.
@@ -28,7 +29,8 @@ ARCH = os.environ.get("DYPHUR_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["runtime.hip", "primitives.hip", "broadphase.hip", "narrowphase.hip", "solver.hip"]
 
 # -ffp-contract=off: parity with the reference requires every float op to round
-# on its own.  Packed-FP32 instructions are disabled.
+# on its own.  Packed-FP32 instructions are disabled: some of their forms have
+# given wrong results on these GPUs.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-rdc", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -96,10 +98,41 @@ def build_python_module(verbose: bool = False) -> str | None:
     return out
 
 
+CPP_TEST_SOURCES = ["main.cpp", "test_stores_integrator.cpp", "test_pipeline.cpp", "test_compute.cpp"]
+
+
+def build_cpp_tests(verbose: bool = False) -> str:
+    """The C++ API tests (tests/cpp) against the header layer in
+    dyphur_amd/cpp/include and libdyphur_hip.so; hipcc, because parallel_for
+    callers compile device lambdas."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    flags = [f"--offload-arch={ARCH}", "-O2", "-std=c++20", "-fno-gpu-rdc", "-Xclang", "-target-feature",
+             "-Xclang", "-packed-fp32-ops", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+             f"-I{os.path.join(PKG, 'cpp', 'include')}"]
+
+    def compile_one(src):
+        obj = os.path.join(out_dir, src.replace(".cpp", ".o"))
+        _run([HIPCC, *flags, "-x", "hip", "-c", os.path.join(src_dir, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(CPP_TEST_SOURCES)) as ex:
+        objs = list(ex.map(compile_one, CPP_TEST_SOURCES))
+    out = os.path.join(out_dir, "dyphur_cpp_tests")
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-o", tmp, *objs, f"-L{LIB}", "-ldyphur_hip",
+          "-Wl,-rpath,$ORIGIN/../../../dyphur_amd/lib"])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_all(verbose: bool = False):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    paths = [build_hip(verbose), build_hostmath(verbose)]
+    paths = [build_hip(verbose), build_hostmath(verbose), build_cpp_tests(verbose)]
     py = build_python_module(verbose)
     if py:
         paths.append(py)

@@ -95,6 +95,7 @@ _SIGS = {
     "dy_memcpy_h2d_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "dy_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "dy_memcpy_d2d_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "dy_memcpy_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "dy_memset_async": (C.c_int, [vp, C.c_int, C.c_size_t, vp]),
     "dy_event_create": (C.c_int, [C.POINTER(vp)]),
     "dy_event_destroy": (C.c_int, [vp]),

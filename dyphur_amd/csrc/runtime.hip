@@ -139,6 +139,13 @@ int dy_memcpy_d2d_async(void *dst, const void *src, size_t bytes, dy_stream_t s)
     return DY_OK;
 }
 
+int dy_memcpy_async(void *dst, const void *src, size_t bytes, dy_stream_t s) {
+    if (!bytes) return DY_OK;
+    if (!dst || !src) return DY_ERR_INVALID_ARG;
+    DY_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, hs(s)));
+    return DY_OK;
+}
+
 int dy_memset_async(void *dst, int value, size_t bytes, dy_stream_t s) {
     if (!bytes) return DY_OK;
     if (!dst) return DY_ERR_INVALID_ARG;

@@ -146,6 +146,9 @@ int dy_free(void *ptr);
 int dy_memcpy_h2d_async(void *dst, const void *src, size_t bytes, dy_stream_t s);
 int dy_memcpy_d2h(void *dst, const void *src, size_t bytes, dy_stream_t s);
 int dy_memcpy_d2d_async(void *dst, const void *src, size_t bytes, dy_stream_t s);
+/* queue.memcpy(dst, src, bytes) (compute/include/compute/buffer.hpp:45-53):
+ * direction taken from the pointers (host pageable, pinned or device). */
+int dy_memcpy_async(void *dst, const void *src, size_t bytes, dy_stream_t s);
 int dy_memset_async(void *dst, int value, size_t bytes, dy_stream_t s);
 /* Event (stream.hpp:6-14), plus timing */
 int dy_event_create(dy_event_t *out);

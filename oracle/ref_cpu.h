@@ -22,6 +22,16 @@
  *   sycl::half(float) = IEEE binary16 round-to-nearest-even;
  *   sycl::rsqrt(x)    = 1.f / sqrtf(x) (correctly rounded sqrt and divide);
  *   sycl::clz(0ull)   = 64.
+ *
+ * Pinning (tests/test_oracle_kats.py, tests/test_oracle_pins.py): the
+ * reference's KATs and golden hash, brute force (Morton keys and order,
+ * Karras tree, pair set), closed forms (leaf boxes, integrator), a direct
+ * simulation of the JP round rule, invariants.  Parity unpinned: the
+ * box-box manifold heuristic, reference-face swap, clip-vertex order and
+ * centroid bits for general poses (core/src/narrowphase.cpp:216-305), and
+ * the exact bits of multi-contact solves beyond the reference's solver KATs
+ * (core/src/xpbd_solver.cpp:413-521,559-587); only a run of the reference
+ * could pin those, and it cannot be built without SYCL.
  */
 #ifndef DYPHUR_REF_CPU_H
 #define DYPHUR_REF_CPU_H
