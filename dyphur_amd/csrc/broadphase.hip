@@ -20,6 +20,8 @@
 //                      rows longer than 16 re-walk the tree for the next 16.
 // The output is the reference's pair set in ascending (a, b) order, so
 // sort_pairs has nothing left to do.
+#include <cstdlib>
+
 #include "dy_internal.hpp"
 
 namespace dy {
@@ -183,6 +185,134 @@ __global__ __launch_bounds__(kMsThreads) void morton_sort(dy_body_view bv, dy_aa
     // Four passes: the sorted data is back in (code0, idx0).
     for (uint32_t i = t; i < n; i += kMsThreads)
         morton64[i] = (static_cast<uint64_t>(code0[i]) << 32) | idx0[i];
+}
+
+// Multi-workgroup form of the same sort for larger n: one cooperative launch,
+// four 8-bit LSD passes over 1024-key tiles, each pass in three phases
+// separated by grid barriers:
+//   A. per-tile digit histograms into mat[digit * n_tiles + tile];
+//   B. per-digit exclusive scan of mat across tiles, row totals into tot;
+//   C. digit bases (scan of tot), then each tile scatters its keys stably:
+//      position = base[d] + mat[d][tile] + rank of the key among the tile's
+//      keys with digit d (wave ballots, waves in index order).
+// The last pass also writes the 64-bit keys (code << 32 | body).
+constexpr uint32_t kSgThreads = 256;
+constexpr uint32_t kSgWaves = kSgThreads / 64;
+constexpr uint32_t kSgItems = 4;
+constexpr uint32_t kSgTile = kSgThreads * kSgItems;
+constexpr uint32_t kErrSortBarrier = 8u;
+
+__global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, dy_aabb bd, uint32_t n,
+                                                               uint32_t n_tiles, uint32_t *code0, uint32_t *idx0,
+                                                               uint32_t *code1, uint32_t *idx1, uint32_t *mat,
+                                                               uint32_t *tot, uint64_t *morton64, uint32_t *bar,
+                                                               uint32_t *err) {
+    __shared__ uint32_t lh[256];
+    __shared__ uint32_t dbase[256];
+    __shared__ uint32_t wcnt[kSgWaves][256];
+    __shared__ uint32_t wsum[17];
+    uint32_t gen = 0;
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t i = blockIdx.x * kSgThreads + t; i < n; i += gridDim.x * kSgThreads) {
+        const uint32_t qx = quantize10(bv.pos_x[i], bd.min_x, bd.max_x);
+        const uint32_t qy = quantize10(bv.pos_y[i], bd.min_y, bd.max_y);
+        const uint32_t qz = quantize10(bv.pos_z[i], bd.min_z, bd.max_z);
+        code0[i] = morton3(qx, qy, qz);
+        idx0[i] = i;
+    }
+    grid_barrier(bar, gen, err, kErrSortBarrier);
+    uint32_t *sk = code0, *sv = idx0, *dk = code1, *dv = idx1;
+    for (uint32_t pass = 0; pass < 4; ++pass) {
+        const uint32_t bit = 8u * pass;
+        const uint32_t mask = pass == 3 ? 0x3fu : 0xffu;
+        // A. tile histograms
+        for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            lh[t] = 0;
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < kSgItems; ++j) {
+                const uint32_t idx = tile * kSgTile + j * kSgThreads + t;
+                if (idx < n) atomicAdd(&lh[(sk[idx] >> bit) & mask], 1u);
+            }
+            __syncthreads();
+            mat[t * n_tiles + tile] = lh[t];
+            __syncthreads();
+        }
+        grid_barrier(bar, gen, err, kErrSortBarrier);
+        // B. per-digit scan across tiles
+        for (uint32_t d = blockIdx.x; d < 256u; d += gridDim.x) {
+            uint32_t carry = 0;
+            for (uint32_t c0 = 0; c0 < n_tiles; c0 += kSgThreads) {
+                const bool in = c0 + t < n_tiles;
+                const uint32_t v = in ? mat[d * n_tiles + c0 + t] : 0u;
+                uint32_t total = 0;
+                const uint32_t ex = block_exclusive_scan(v, wsum, total);
+                if (in) mat[d * n_tiles + c0 + t] = carry + ex;
+                carry += total;
+            }
+            if (t == 0) tot[d] = carry;
+        }
+        grid_barrier(bar, gen, err, kErrSortBarrier);
+        // C. digit bases and stable scatter
+        {
+            uint32_t total = 0;
+            dbase[t] = block_exclusive_scan(tot[t], wsum, total);
+        }
+        for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            for (uint32_t i = t; i < kSgWaves * 256u; i += kSgThreads) (&wcnt[0][0])[i] = 0;
+            __syncthreads();
+            uint32_t kk[kSgItems], vv[kSgItems], dd[kSgItems], rr[kSgItems];
+#pragma unroll
+            for (uint32_t j = 0; j < kSgItems; ++j) {
+                const uint32_t idx = tile * kSgTile + wave * (64 * kSgItems) + j * 64 + lane;
+                const bool valid = idx < n;
+                kk[j] = valid ? sk[idx] : 0u;
+                vv[j] = valid ? sv[idx] : 0u;
+                const uint32_t d = valid ? ((kk[j] >> bit) & mask) : 256u;
+                uint64_t peers = ~0ull;
+#pragma unroll
+                for (int b = 0; b < 9; ++b) {
+                    const bool set = (d >> b) & 1u;
+                    const uint64_t bal = __ballot(set);
+                    peers &= set ? bal : ~bal;
+                }
+                const uint32_t pre = static_cast<uint32_t>(__popcll(peers & lt_mask));
+                const uint32_t c = valid ? wcnt[wave][valid ? d : 0u] : 0u;
+                rr[j] = c + pre;
+                dd[j] = d;
+                if (valid && pre == 0) wcnt[wave][d] = c + static_cast<uint32_t>(__popcll(peers));
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+            {
+                uint32_t run = 0;
+                for (uint32_t w = 0; w < kSgWaves; ++w) {
+                    const uint32_t c = wcnt[w][t];
+                    wcnt[w][t] = run;
+                    run += c;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t j = 0; j < kSgItems; ++j) {
+                if (dd[j] < 256u) {
+                    const uint32_t pos = dbase[dd[j]] + mat[dd[j] * n_tiles + tile] + wcnt[wave][dd[j]] + rr[j];
+                    if (pos >= n) {  // inconsistent histogram: flag instead of writing out of range
+                        atomicOr(err, kErrSortBarrier);
+                        continue;
+                    }
+                    dk[pos] = kk[j];
+                    dv[pos] = vv[j];
+                    if (pass == 3) morton64[pos] = (static_cast<uint64_t>(kk[j]) << 32) | vv[j];
+                }
+            }
+            __syncthreads();
+        }
+        if (pass < 3) grid_barrier(bar, gen, err, kErrSortBarrier);
+        uint32_t *tk = sk; sk = dk; dk = tk;
+        uint32_t *tv = sv; sv = dv; dv = tv;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -456,15 +586,18 @@ struct dy_broadphase_s {
     uint32_t *cnt_top = nullptr, *entries = nullptr, *counters = nullptr;
     uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *ovf = nullptr, *leaf_of = nullptr;
     uint32_t *scan_tmp = nullptr;
+    uint32_t *sort_mat = nullptr, *sort_tot = nullptr;
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
+    uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
 };
 
 namespace {
 void bp_free(dy_broadphase_s *b) {
     void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
                     b->root, b->range, b->nodebox, b->tn, b->cnt_top, b->entries, b->counters,
-                    b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs};
+                    b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs,
+                    b->sort_mat, b->sort_tot};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -491,12 +624,30 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     DY_A(b->morton64, nb);
     DY_A(b->left, nb); DY_A(b->right, nb); DY_A(b->parent, nodes); DY_A(b->root, 1);
     DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes);
-    DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 4);
+    DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 8);
+    DY_A(b->sort_mat, 256 * ceil_div(static_cast<uint32_t>(nb), kSgTile)); DY_A(b->sort_tot, 256);
     DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->ovf, nb * kOvf); DY_A(b->leaf_of, nb);
     DY_A(b->scan_tmp, scan_tmp_words(nb)); DY_A(b->pairs, max_pairs);
 #undef DY_A
     if (e == hipSuccess) b->count = b->counters + 3;
-    if (e == hipSuccess) e = hipMemsetAsync(b->counters, 0, 4 * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) e = hipMemsetAsync(b->counters, 0, 8 * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) {
+        // Workgroups of morton_sort_grid: at most one per 1024 keys, 32 by
+        // default (DYPHUR_SORT_BLOCKS overrides), and no more than can be
+        // co-resident for a cooperative launch.
+        int dev = 0, coop = 0, per_cu = 0, cus = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess && coop)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(morton_sort_grid),
+                                                            kSgThreads, 0);
+        if (e == hipSuccess && coop && per_cu > 0) {
+            int want = 32;
+            if (const char *env = std::getenv("DYPHUR_SORT_BLOCKS")) want = std::atoi(env);
+            b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > per_cu * cus ? per_cu * cus : want));
+        }
+    }
     if (e == hipSuccess) e = hipMemsetAsync(b->root, 0, sizeof(int32_t), hs(s));
     if (e != hipSuccess) {
         bp_free(b);
@@ -521,12 +672,26 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     const uint32_t n = bodies->n;
     if (n > bp->max_bodies || n >= kMortonMax) return DY_ERR_INVALID_ARG;
     hipStream_t st = hs(s);
-    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 2 * sizeof(uint32_t), st));
+    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 4 * sizeof(uint32_t), st));
     if (n < 2) return DY_OK;
     const int ni = static_cast<int>(n);
-    morton_sort<<<1, kMsThreads, 0, st>>>(*bodies, *bounds, n, bp->code0, bp->idx0, bp->code1, bp->idx1,
-                                          bp->morton64);
-    DY_LAUNCH_CHECK();
+    const uint32_t n_tiles = ceil_div(n, kSgTile);
+    if (bp->sort_blocks > 0 && n_tiles > 4) {
+        const uint32_t blocks = n_tiles < bp->sort_blocks ? n_tiles : bp->sort_blocks;
+        dy_body_view bv = *bodies;
+        dy_aabb bd = *bounds;
+        uint32_t nv = n, ntv = n_tiles;
+        uint32_t *c0 = bp->code0, *i0 = bp->idx0, *c1 = bp->code1, *i1 = bp->idx1, *mat = bp->sort_mat,
+                 *tot = bp->sort_tot, *bar = bp->counters + 4, *err = bp->counters + 2;
+        uint64_t *m64 = bp->morton64;
+        void *args[] = {&bv, &bd, &nv, &ntv, &c0, &i0, &c1, &i1, &mat, &tot, &m64, &bar, &err};
+        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(morton_sort_grid), dim3(blocks),
+                                          dim3(kSgThreads), args, 0, st));
+    } else {
+        morton_sort<<<1, kMsThreads, 0, st>>>(*bodies, *bounds, n, bp->code0, bp->idx0, bp->code1, bp->idx1,
+                                              bp->morton64);
+        DY_LAUNCH_CHECK();
+    }
     karras_build<<<ceil_div(n - 1, 256), 256, 0, st>>>(bp->morton64, ni, bp->left, bp->right, bp->parent,
                                                        bp->range, bp->tn, bp->cnt_top, bp->root);
     DY_LAUNCH_CHECK();

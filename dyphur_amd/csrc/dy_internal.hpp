@@ -101,4 +101,35 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *w
     return r;
 }
 
+// Grid-wide barrier for a cooperative launch (every workgroup of the grid
+// co-resident): arrival counter bar[0] + generation bar[1], both zero before
+// the launch.  Thread 0 of each workgroup publishes the workgroup's stores
+// with an agent-scope release before arriving and acquires after the
+// generation moves.  A wait that exceeds kGridSpinLimit polls sets err_bit in
+// *err and gives up, so a broken launch ends instead of hanging.
+constexpr uint32_t kGridSpinLimit = 1u << 22;
+__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint32_t *err, uint32_t err_bit) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = gen;
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kGridSpinLimit) {
+                    atomicOr(err, err_bit);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    gen += 1u;
+    __syncthreads();
+}
+
 }  // namespace dy

@@ -47,7 +47,6 @@ constexpr uint32_t kSolveThreads = 256;
 // meta: [0] remaining [1] colour overflow [2] max class [3] max depth [4] error flags [5] assigned
 constexpr uint32_t kMetaWords = 16;
 constexpr uint32_t kLdsClasses = 2048;   // block-aggregated class counting limit
-constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr uint32_t kErrBarrier = 2u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
@@ -103,33 +102,6 @@ __global__ void solve_fill(dy_contact_view cv, uint32_t nc, const float *__restr
 // reference-order path.
 __device__ __forceinline__ uint64_t order_key(uint32_t c, int serial) {
     return serial ? static_cast<uint64_t>(0xffffffffu - c) : jp_priority(c);
-}
-
-// Grid-wide barrier for a cooperative launch: arrival counter + generation.
-// Wave 0 of each workgroup publishes the workgroup's stores (agent release)
-// before arriving and acquires after the generation moves.
-__device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint32_t *err) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t g = gen;
-        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, kErrBarrier);
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-    }
-    gen += 1u;
-    __syncthreads();
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
@@ -303,7 +275,7 @@ __global__ __launch_bounds__(1024) void assign_lists(const uint4 *__restrict__ l
             }
             wave_append<2>(ready, nx, nxt, nxt_count);
         }
-        grid_barrier(bar, gen, err);
+        grid_barrier(bar, gen, err, kErrBarrier);
         ncur = *nxt_count;
         uint32_t *t = cur; cur = nxt; nxt = t;
     }
@@ -502,7 +474,7 @@ __global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restr
         for (uint32_t k = 0; k < n_cls; ++k) {
             const uint32_t beg = boff[k], end = boff[k + 1];
             for (uint32_t i = beg + gtid; i < end; i += gstride) solve_one(i, it, rec, lam, dyn, sta, mu, dt);
-            if (it + 1 < n_iters || k + 1 < n_cls) grid_barrier(bar, gen, err);
+            if (it + 1 < n_iters || k + 1 < n_cls) grid_barrier(bar, gen, err, kErrBarrier);
         }
     }
 }

@@ -40,6 +40,7 @@ CASES = [
     ("stress_f140", lambda: scenes.stress_blocks(), 140),
     ("mixed_3000", lambda: scenes.random_mixed(3000, seed=11, extent=(12, 10, 12)), 0),
     ("mixed_3000_f30", lambda: scenes.random_mixed(3000, seed=12, extent=(12, 10, 12)), 30),
+    ("mixed_12000", lambda: scenes.random_mixed(12000, seed=13, extent=(24, 16, 24)), 0),
 ]
 
 
@@ -244,3 +245,24 @@ def test_solver_launch_shapes_bitexact(gpu, oracle, stage_inputs, monkeypatch, s
     for f in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
               "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z"):
         assert np.array_equal(bits(got[f]), bits(ref[f])), f
+
+
+@pytest.mark.parametrize("sort_blocks", ["0", "1", "7", "32"])
+def test_morton_sort_workgroup_counts(gpu, oracle, stage_inputs, monkeypatch, sort_blocks):
+    """The Morton sort gives the reference order with one workgroup (0) and
+    with the cooperative multi-workgroup form at any grid size."""
+    monkeypatch.setenv("DYPHUR_SORT_BLOCKS", sort_blocks)
+    sc, ob, osh = stage_inputs["mixed_12000"]
+    print(f"[morton sort] blocks={sort_blocks} n={sc.n}", flush=True)
+    w = oracle.World.for_scene(sc)
+    n_ref = w.build_and_query(ob, osh, sc.bounds)
+    ref = w.bvh(sc.n)
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob.state())
+    ds = gpu.DeviceShapes(sc, s)
+    bp = gpu.Broadphase(s, sc.n, sc.max_pairs)
+    bp.build_and_query(s, db, ds, sc.bounds)
+    assert bp.download_count(s) == n_ref
+    got = bp.bvh(s, sc.n)
+    assert np.array_equal(got["morton"], ref["morton"])
+    assert np.array_equal(got["sorted_idx"], ref["sorted_idx"])
