@@ -6,8 +6,11 @@ U[0.1, 0.5] m, 1/3 spheres with radius U[0.1, 0.4] m, uniform positions in a
 30 x 40 x 30 m volume, uniform random orientations, density 1000 kg/m^3, seed
 20260002) over a static ground plate, FP32, 60 Hz, 1 substep, 10 solver
 iterations.  A step is one frame: integrate -> LBVH broadphase -> pairs ->
-narrowphase -> XPBD solve, through the C ABI exactly as the reference's
-stress driver sequences it (examples/stress_test_blocks/main.cpp:264-273).
+narrowphase -> XPBD solve, through the C ABI in the reference's stress-driver
+order (examples/stress_test_blocks/main.cpp:264-273), using the async-capable
+Narrowphase::run overload so that no frame waits on the host.  An untimed
+replay with per-frame synchronisation then collects the load statistics and
+must end in the same state.
 
 value = body-substeps/s = bodies (ground included) x frames / seconds over the
 timed frames, with all inputs resident in HBM (no host<->device state copies
@@ -44,30 +47,29 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(sc, capi_mod, w, s, steps, window):
-    """Time the CPU oracle (the reference algorithm restated in C++/OpenMP) on
-    four windows of `window` frames spread over frames 0..steps-1.  Each window
-    starts from the GPU trajectory's state at that frame (bit-identical to the
-    oracle's own trajectory), so the sample covers both the falling and the
-    settled phases without running every frame on the CPU.  The oracle's state
-    hash at each window end is compared with the GPU's."""
-    import platform
-    from oracle import ref_cpu as rc
-
-    starts = sorted({0, steps // 3, (2 * steps) // 3, max(0, steps - window)})
-    want = {}
-    for f0 in starts:
-        want[f0] = None
-        want[f0 + window] = None
-    init = w.bodies.host_arrays(sc)
+def replay(w, init, s, steps, want):
+    """Untimed re-run of the same frames with per-frame statistics (one sync
+    per frame) and state snapshots at the frames in `want`."""
     w.bodies.upload_state(init, s)
     s.wait()
-    snaps = {}
-    for f in range(0, max(want) + 1):
+    stats, snaps = [], {}
+    for f in range(steps + 1):
         if f in want:
             snaps[f] = (w.bodies.download(s), w.fnv())
-        if f < max(want):
-            w.step()
+        if f < steps:
+            stats.append(w.step())
+    return stats, snaps, w.fnv()
+
+
+def cpu_baseline(sc, snaps, starts, window):
+    """Time the CPU oracle (the reference algorithm restated in C++/OpenMP) on
+    windows of `window` frames, each starting from the GPU trajectory's state
+    at that frame (bit-identical to the oracle's own trajectory), so the
+    sample covers both the falling and the settled phases without running
+    every frame on the CPU.  The oracle's state hash at each window end is
+    compared with the GPU's."""
+    from oracle import ref_cpu as rc
+
     threads = rc.max_threads()
     ob, osh = rc.Bodies(sc), rc.Shapes(sc)
     ow = rc.World.for_scene(sc)
@@ -89,13 +91,13 @@ def cpu_baseline(sc, capi_mod, w, s, steps, window):
                 cpu_model = line.split(":", 1)[1].strip()
                 break
     except OSError:
-        cpu_model = platform.processor()
+        pass
     return {
         "value": round(sc.n * frames * sc.n_substeps / total, 1),
         "unit": "body-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{frames} frames of the same workload in 4 windows of {window} starting at frames "
+        "sample": f"{frames} frames of the same workload in {len(starts)} windows of {window} starting at frames "
                   f"{starts}, each from the GPU state at that frame; {total:.1f} s on {threads} OpenMP "
                   f"threads ({cpu_model}); oracle state hash equal to the GPU at every window end: {equal}",
         "seconds": round(total, 2),
@@ -107,7 +109,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs the frame step on one GPU only; --gpus {args.gpus} "
-              f"is not supported (see DESIGN.md for how the path would shard).", file=sys.stderr)
+              f"is not supported (a multi-GPU run would shard the scene into Morton-range slabs "
+              f"with a ghost-body exchange per substep).", file=sys.stderr)
         sys.exit(2)
 
     from dyphur_amd import capi, scenes
@@ -127,25 +130,38 @@ def main():
 
     # Warm-up frames (untimed), then restart from the initial state.
     for _ in range(args.warmup):
-        w.step()
+        w.step_async()
+    w.check()
     reset()
 
+    # Timed frames: no host synchronisation inside the loop.
     w.solver.set_timing(True)
     e0, e1 = capi.Event(), capi.Event()
     s.wait()
     t0 = time.perf_counter()
     e0.record(s)
-    pairs, contacts, colors = [], [], []
     for _ in range(args.steps):
-        st = w.step()
-        pairs.append(st["n_pairs"]); contacts.append(st["n_contacts"]); colors.append(st["n_colors"])
+        w.step_async()
     e1.record(s)
     s.wait()
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_ms(e1)
+    w.check()  # raises on any device error flag of the timed frames
     timing = w.solver.timing(s)
     w.solver.set_timing(False)
     final_hash = w.fnv()
+
+    # Untimed replay: per-frame load statistics, CPU-baseline snapshots, and
+    # the same final state as the timed run.
+    window = args.cpu_window if (not args.no_cpu and args.cpu_window > 0) else 0
+    starts = sorted({0, args.steps // 3, (2 * args.steps) // 3, max(0, args.steps - window)}) if window else []
+    want = set(starts) | {f + window for f in starts}
+    stats, snaps, replay_hash = replay(w, init, s, args.steps, want)
+    if replay_hash != final_hash:
+        raise RuntimeError(f"replay state {replay_hash:016x} differs from the timed run {final_hash:016x}")
+    pairs = [st["n_pairs"] for st in stats]
+    contacts = [st["n_contacts"] for st in stats]
+    colors = [st["n_colors"] for st in stats]
     n = sc.n
     value = n * args.steps * sc.n_substeps / wall
     out = {
@@ -194,8 +210,8 @@ def main():
         "other": {k: {"avg_launch_ms": round(v[0], 4), "algorithmic_bytes_per_launch": round(v[1])}
                   for k, v in kernels.items() if k != name},
     }
-    if not args.no_cpu and args.cpu_window > 0:
-        out["cpu_baseline"] = cpu_baseline(sc, capi, w, s, args.steps, args.cpu_window)
+    if window:
+        out["cpu_baseline"] = cpu_baseline(sc, snaps, starts, window)
         out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
     print(json.dumps(out), flush=True)
 

@@ -539,6 +539,28 @@ class World:
         self.solver = XpbdSolver(self.s, scene.n_iters, scene.max_contacts, scene.mu)
         self.last = {}
 
+    def step_async(self, force_serial: bool = False):
+        """One frame with no host synchronisation: the pair count stays on the
+        device (the reference's async-capable Narrowphase::run overload,
+        core/include/core/narrowphase.hpp:39-46) and the solve is enqueued.
+        Errors surface at the next blocking call (step(), fnv(), check())."""
+        sc, s = self.scene, self.s
+        integrate(s, self.bodies, sc.gravity, sc.dt, sc.n_substeps)
+        self.bp.build_and_query(s, self.bodies, self.shapes, sc.bounds)
+        pairs, count = self.bp.pairs_ptr()
+        self.np.run_device_count(s, pairs, count, self.bp.max_pairs, self.bodies, self.shapes)
+        self.solver.set_force_serial(force_serial)
+        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt)
+
+    def check(self) -> dict:
+        """Blocks until the enqueued frames are done and raises on any device
+        error flag; returns the last frame's counts."""
+        n_pairs = self.bp.download_count(self.s)
+        n_contacts = self.np.download_count(self.s)
+        st = self.solver.stats()
+        return dict(n_pairs=n_pairs, n_contacts=n_contacts, n_colors=st["n_classes"],
+                    used_serial=st["used_serial"])
+
     def step(self, force_serial: bool = False) -> dict:
         sc, s = self.scene, self.s
         integrate(s, self.bodies, sc.gravity, sc.dt, sc.n_substeps)

@@ -22,6 +22,12 @@
 //    earlier contact on either dynamic body) reproduces the serial result bit
 //    for bit while running each level in parallel.
 //
+// The whole solve is asynchronous on its stream on the cooperative path: the
+// contact count is read on the device (clamped to capacity), the choice of
+// path, the fallback to the reference order, the class count and the class
+// offsets are made on the device, and the statistics come back with one
+// asynchronous copy that the blocking queries wait for.
+//
 // Data layout during a solve: bodies are gathered into 64 B dynamic + 32 B
 // static AoS records (one or two 128-bit loads per field group instead of 20
 // scattered SoA loads), contacts into class-sorted 40 B records.  All I x K
@@ -36,64 +42,92 @@
 // enters products with a zero inverse inertia, so they are renormalised at
 // the end.  Every body ends with the reference's sequence of operations.
 #include <cstdlib>
-#include <vector>
 
 #include "dy_internal.hpp"
 
 namespace dy {
 
-constexpr uint64_t kUnassigned = ~0ull;
 constexpr uint32_t kSolveThreads = 256;
-// meta: [0] remaining [1] colour overflow [2] max class [3] max depth [4] error flags [5] assigned
-constexpr uint32_t kMetaWords = 16;
-constexpr uint32_t kLdsClasses = 2048;   // block-aggregated class counting limit
-constexpr uint32_t kErrBarrier = 2u;
+constexpr uint32_t kLdsClasses = 2048;  // block-aggregated class counting limit
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kMaxColours = 64;
+constexpr uint32_t kMaxGridBlocks = 1024;  // grid-stride kernels over capacity-sized ranges
+
+// Solver metadata (device words).  solve_prep zeroes all but kMErr, which
+// keeps error flags until the solver is destroyed.
+enum : uint32_t {
+    kMOverflow = 1,      // a contact's neighbours held all 64 colours
+    kMMaxCls = 2,        // largest class
+    kMRounds = 3,        // rounds of the assignment that was used
+    kMErr = 4,           // error flags (sticky)
+    kMAssigned = 5,      // contacts assigned
+    kMDyn = 6,           // dynamic endpoints = entries of the per-body lists
+    kMNcls = 7,          // classes swept
+    kMSerial = 8,        // 1: reference contact order
+    kMNc = 9,            // contacts solved (device count clamped to capacity)
+    kMColourRounds = 10, // colouring rounds before a fallback (257: more than 256)
+    kMBarAssign = 16,    // 2 words: grid barrier of assign_lists
+    kMBarSweep = 18,     // 2 words: grid barrier of sweep_persistent
+    kMFcnt = 20,         // 3 words: frontier sizes
+    kMWords = 32,
+};
+constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
-__device__ __forceinline__ uint32_t dep_of(uint64_t s) { return static_cast<uint32_t>(s >> 32); }
-__device__ __forceinline__ uint64_t ld_state(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+__device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
+__device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Zeroes the per-solve state and publishes the contact count and the path:
+// the reference orders contacts serially at <= 256 contacts or when forced.
+__global__ void solve_prep(uint32_t nb, const uint32_t *__restrict__ ncp, uint32_t cap, int force_serial,
+                           uint32_t *__restrict__ deg, uint32_t *__restrict__ fill, uint64_t *__restrict__ used,
+                           float *__restrict__ lam, uint32_t *__restrict__ hist, uint32_t *__restrict__ meta) {
+    const uint32_t n_raw = *ncp;
+    const uint32_t nc = n_raw < cap ? n_raw : cap;
+    const uint32_t top = nb > nc ? nb : nc;
+    for (uint32_t i = grid_tid(); i < top; i += grid_stride()) {
+        if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; }
+        if (i < nc) lam[i] = 0.f;
+    }
+    const uint32_t i = grid_tid();
+    if (i <= kMaxColours) hist[i] = 0;
+    if (i < kMWords && i != kMErr) {
+        uint32_t v = 0;
+        if (i == kMNc) v = nc;
+        if (i == kMSerial) v = (force_serial || nc <= 256u) ? 1u : 0u;
+        meta[i] = v;
+    }
+    if (i == 0 && n_raw > cap) atomicOr(&meta[kMErr], kErrCapacity);
 }
 
-__global__ void solve_prep(uint32_t nb, uint32_t nc, uint32_t *__restrict__ deg, uint32_t *__restrict__ fill,
-                           uint64_t *__restrict__ used, float *__restrict__ lam, uint64_t *__restrict__ state,
-                           uint32_t *__restrict__ meta) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; }
-    if (i < nc) { lam[i] = 0.f; state[i] = kUnassigned; }
-    if (i < kMetaWords) meta[i] = (i == 0) ? nc : 0u;
-}
-
-__global__ void solve_reassign_prep(uint32_t nc, uint64_t *__restrict__ state, uint32_t *__restrict__ meta) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nc) state[i] = kUnassigned;
-    if (i < 6 && i != 4) meta[i] = (i == 0) ? nc : 0u;  // keeps error flags and the endpoint total
-}
-
-__global__ void solve_degree(dy_contact_view cv, uint32_t nc, const float *__restrict__ im, uint32_t *__restrict__ deg) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t a = cv.body_a[c], b = cv.body_b[c];
-    if (im[a] != 0.f) atomicAdd(&deg[a], 1u);
-    if (im[b] != 0.f) atomicAdd(&deg[b], 1u);
+__global__ void solve_degree(dy_contact_view cv, const uint32_t *__restrict__ meta, const float *__restrict__ im,
+                             uint32_t *__restrict__ deg) {
+    const uint32_t nc = meta[kMNc];
+    for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
+        const uint32_t a = cv.body_a[c], b = cv.body_b[c];
+        if (im[a] != 0.f) atomicAdd(&deg[a], 1u);
+        if (im[b] != 0.f) atomicAdd(&deg[b], 1u);
+    }
 }
 
 // Unordered per-body lists of dynamic endpoints: adj = (c << 1 | end), own = body.
-__global__ void solve_fill(dy_contact_view cv, uint32_t nc, const float *__restrict__ im,
+__global__ void solve_fill(dy_contact_view cv, const uint32_t *__restrict__ meta, const float *__restrict__ im,
                            const uint32_t *__restrict__ off, uint32_t *__restrict__ fill, uint32_t *__restrict__ adj,
                            uint32_t *__restrict__ own) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const uint32_t a = cv.body_a[c], b = cv.body_b[c];
-    if (im[a] != 0.f) {
-        const uint32_t p = off[a] + atomicAdd(&fill[a], 1u);
-        adj[p] = c << 1;
-        own[p] = a;
-    }
-    if (im[b] != 0.f) {
-        const uint32_t p = off[b] + atomicAdd(&fill[b], 1u);
-        adj[p] = (c << 1) | 1u;
-        own[p] = b;
+    const uint32_t nc = meta[kMNc];
+    for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
+        const uint32_t a = cv.body_a[c], b = cv.body_b[c];
+        if (im[a] != 0.f) {
+            const uint32_t p = off[a] + atomicAdd(&fill[a], 1u);
+            adj[p] = c << 1;
+            own[p] = a;
+        }
+        if (im[b] != 0.f) {
+            const uint32_t p = off[b] + atomicAdd(&fill[b], 1u);
+            adj[p] = (c << 1) | 1u;
+            own[p] = b;
+        }
     }
 }
 
@@ -144,17 +178,13 @@ __device__ __forceinline__ void wave_append(const bool (&pred)[N], const uint32_
 
 // Per-body contact lists.  Every dynamic endpoint (contact c, end e) has an
 // entry adj[p] = (c << 1 | e) in its body's segment [off[x], off[x] + deg[x]),
-// and own[p] = x.  list_rank places c at its rank in the body's order
+// and own[p] = x.  rank_entry places c at its rank in the body's order
 // (slot[off[x] + rank]) and records the rank per endpoint (crank[2c + e]).
 // Cost per entry: one pass over the body's segment (L2-resident).
-__global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__restrict__ adj,
-                                                           const uint32_t *__restrict__ own,
-                                                           const uint32_t *__restrict__ off,
-                                                           const uint32_t *__restrict__ deg,
-                                                           const uint32_t *__restrict__ total, int serial,
-                                                           uint32_t *__restrict__ slot, uint32_t *__restrict__ crank) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *total) return;
+__device__ __forceinline__ void rank_entry(uint32_t p, const uint32_t *__restrict__ adj,
+                                           const uint32_t *__restrict__ own, const uint32_t *__restrict__ off,
+                                           const uint32_t *__restrict__ deg, int serial, uint32_t *__restrict__ slot,
+                                           uint32_t *__restrict__ crank) {
     const uint32_t v = adj[p], x = own[p];
     const uint32_t c = v >> 1;
     const uint64_t kc = order_key(c, serial);
@@ -169,13 +199,47 @@ __global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__res
     crank[2u * c + (v & 1u)] = rank;
 }
 
-constexpr uint32_t kNone = 0xffffffffu;
+// Links contact c to its successor in both endpoint lists and counts the
+// lists it does not head yet (pending); returns true when it heads all of
+// them.  lrec[c] = (a or none, b or none, next on a, next on b); static
+// endpoints are "none".
+__device__ __forceinline__ bool link_contact(uint32_t c, const dy_contact_view &cv, const float *__restrict__ im,
+                                             const uint32_t *__restrict__ off, const uint32_t *__restrict__ deg,
+                                             const uint32_t *__restrict__ slot, const uint32_t *__restrict__ crank,
+                                             uint4 *__restrict__ lrec, uint32_t *__restrict__ pending) {
+    uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
+    uint32_t next[2] = {kNone, kNone};
+    uint32_t wait = 0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const uint32_t x = ends[e];
+        if (im[x] == 0.f) {
+            ends[e] = kNone;
+            continue;
+        }
+        const uint32_t r = crank[2u * c + e];
+        if (r + 1u < deg[x]) next[e] = slot[off[x] + r + 1u];
+        wait += r != 0u ? 1u : 0u;
+    }
+    lrec[c] = make_uint4(ends[0], ends[1], next[0], next[1]);
+    pending[c] = wait;
+    return wait == 0u;
+}
 
-// Links each contact to its successor in both endpoint lists and counts the
-// lists it does not head yet (pending).  Contacts heading every list of a
-// dynamic endpoint form the first frontier.  lrec[c] = (a or none, b or none,
-// next on a, next on b); static endpoints are "none".
-__global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, uint32_t nc,
+__global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__restrict__ adj,
+                                                           const uint32_t *__restrict__ own,
+                                                           const uint32_t *__restrict__ off,
+                                                           const uint32_t *__restrict__ deg,
+                                                           const uint32_t *__restrict__ meta,
+                                                           uint32_t *__restrict__ slot, uint32_t *__restrict__ crank) {
+    const uint32_t total = meta[kMDyn];
+    const int serial = static_cast<int>(meta[kMSerial]);
+    for (uint32_t p = grid_tid(); p < total; p += grid_stride()) rank_entry(p, adj, own, off, deg, serial, slot, crank);
+}
+
+// Contacts heading every list of a dynamic endpoint form the first frontier
+// (fr0, size in fcnt[0]).
+__global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, const uint32_t *__restrict__ meta,
                                                            const float *__restrict__ im,
                                                            const uint32_t *__restrict__ off,
                                                            const uint32_t *__restrict__ deg,
@@ -183,31 +247,63 @@ __global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, u
                                                            const uint32_t *__restrict__ crank,
                                                            uint4 *__restrict__ lrec, uint32_t *__restrict__ pending,
                                                            uint32_t *__restrict__ frontier, uint32_t *fcount) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    bool ready = false;
-    if (c < nc) {
-        uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
-        uint32_t next[2] = {kNone, kNone};
-        uint32_t wait = 0;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const uint32_t x = ends[e];
-            if (im[x] == 0.f) {
-                ends[e] = kNone;
-                continue;
-            }
-            const uint32_t r = crank[2u * c + e];
-            if (r + 1u < deg[x]) next[e] = slot[off[x] + r + 1u];
-            wait += r != 0u ? 1u : 0u;
-        }
-        lrec[c] = make_uint4(ends[0], ends[1], next[0], next[1]);
-        pending[c] = wait;
-        ready = wait == 0u;
+    const uint32_t nc = meta[kMNc];
+    const uint32_t lane = lane_id();
+    for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
+        const uint32_t c = c0 + lane;
+        const bool pred[1] = {c < nc && link_contact(c, cv, im, off, deg, slot, crank, lrec, pending)};
+        const uint32_t val[1] = {c};
+        wave_append<1>(pred, val, frontier, fcount);
     }
-    const bool pred[1] = {ready};
-    const uint32_t val[1] = {c};
-    wave_append<1>(pred, val, frontier, fcount);
 }
+
+// Assigns round r's frontier entry c: class = lowest colour free in
+// used[a] | used[b] (round - 1 on the reference-order path), then both
+// successors lose one pending predecessor; those reaching zero are returned
+// in (ready, nx) for the next frontier.
+__device__ __forceinline__ uint32_t assign_one(uint32_t c, uint32_t round, int serial,
+                                               const uint4 *__restrict__ lrec, uint32_t *pending, uint64_t *used,
+                                               uint64_t *state, uint32_t *meta, bool (&ready)[2], uint32_t (&nx)[2]) {
+    const uint4 r = lrec[c];
+    uint32_t cls;
+    if (serial) {
+        cls = round - 1u;
+    } else {
+        const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
+        const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
+        const uint64_t u = ua | ub;
+        if (u == ~0ull) {
+            atomicOr(&meta[kMOverflow], 1u);
+            cls = kMaxColours;
+        } else {
+            cls = static_cast<uint32_t>(__builtin_ctzll(~u));
+        }
+        const uint64_t bit = 1ull << (cls & 63u);
+        if (r.x != kNone) used[r.x] = ua | bit;
+        if (r.y != kNone) used[r.y] = ub | bit;
+    }
+    state[c] = (static_cast<uint64_t>(round) << 32) | cls;
+    nx[0] = r.z;
+    nx[1] = r.w;
+    const uint32_t oa = r.z != kNone ? atomicSub(&pending[r.z], 1u) : 0u;
+    const uint32_t ob = r.w != kNone ? atomicSub(&pending[r.w], 1u) : 0u;
+    ready[0] = r.z != kNone && oa == 1u;
+    ready[1] = r.w != kNone && ob == 1u;
+    return cls;
+}
+
+// Buffers of the cooperative class assignment.
+struct AssignBufs {
+    dy_contact_view cv;
+    const float *im;
+    const uint32_t *adj, *own, *off, *deg;
+    uint32_t *slot, *crank;
+    uint4 *lrec;
+    uint32_t *pending;
+    uint64_t *used, *state;
+    uint32_t *fr0, *fr1;
+    uint32_t *meta, *hist, *boff, *bfill;
+};
 
 // JP rounds over frontiers in one cooperative launch.  Round r assigns the
 // contacts that head the ordered lists of all their dynamic endpoints: on a
@@ -215,150 +311,176 @@ __global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, u
 // are exactly the contacts whose higher-priority neighbours were all assigned
 // before round r — the ones the reference colours in its round r.  A body
 // heads at most one contact per round, so the contact owns its bodies' masks
-// of used colours during the round: class = lowest colour free in
-// used[a] | used[b] (round - 1 on the reference-order path), then both masks
-// gain the class and both successors move one list closer to ready.  Values
-// written in a round are read only after the barrier that ends it.
-// fcnt[0..2] rotate as frontier sizes (round r appends to fcnt[r % 3]).
-// meta[1] colour overflow, meta[2] max class, meta[3] rounds, meta[5] assigned.
-__global__ __launch_bounds__(1024) void assign_lists(const uint4 *__restrict__ lrec, int serial, uint32_t max_rounds,
-                                                     uint32_t *pending, uint64_t *used, uint64_t *state, uint32_t *fr0,
-                                                     uint32_t *fr1, uint32_t *fcnt, uint32_t *meta, uint32_t *bar,
-                                                     uint32_t *err) {
+// of used colours during the round.  Values written in a round are read only
+// after the barrier that ends it.  Frontier sizes rotate over fcnt[0..2]
+// (round r appends to fcnt[r % 3]).
+//
+// When the colouring overflows 64 colours or needs more than 256 rounds, the
+// lists are re-ranked in contact order and the rounds re-run as dependency
+// levels: the reference's run_serial order.  The kernel ends with the class
+// offsets (boff[0..k]), zeroed per-class fill counters and the statistics.
+__global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
+    __shared__ uint32_t lhist[kMaxColours + 1];
     uint32_t gen = 0;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
-    const uint32_t wave0 = gtid & ~63u, lane = lane_id();
-    uint32_t *cur = fr0, *nxt = fr1;
-    uint32_t ncur = fcnt[0];
-    uint32_t my_max = 0, my_done = 0, round = 1;
+    uint32_t *meta = B.meta;
+    uint32_t *bar = meta + kMBarAssign, *fcnt = meta + kMFcnt, *err = meta + kMErr;
+    const uint32_t gtid = grid_tid(), gstride = grid_stride();
+    const uint32_t lane = lane_id(), wave0 = gtid & ~63u;
+    const uint32_t nc = meta[kMNc];
+    int serial = static_cast<int>(meta[kMSerial]);
+    uint32_t my_max = 0, my_done = 0, rounds = 0, running = 0;
     bool exceeded = false;
-    for (; ncur > 0; ++round) {
-        if (round > max_rounds) {  // deeper than the reference allows
-            exceeded = true;
-            break;
-        }
-        uint32_t *nxt_count = &fcnt[round % 3];
-        if (gtid == 0) fcnt[(round + 1) % 3] = 0u;
-        for (uint32_t i0 = wave0; i0 < ncur; i0 += gstride) {
-            const uint32_t i = i0 + lane;
-            bool ready[2] = {false, false};
-            uint32_t nx[2] = {0u, 0u};
-            if (i < ncur) {
-                const uint32_t c = cur[i];
-                const uint4 r = lrec[c];
-                uint32_t cls;
-                if (serial) {
-                    cls = round - 1u;
-                } else {
-                    const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
-                    const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
-                    const uint64_t u = ua | ub;
-                    if (u == ~0ull) {
-                        atomicOr(&meta[1], 1u);
-                        cls = 64u;
-                    } else {
-                        cls = static_cast<uint32_t>(__builtin_ctzll(~u));
-                    }
-                    const uint64_t bit = 1ull << (cls & 63u);
-                    if (r.x != kNone) used[r.x] = ua | bit;
-                    if (r.y != kNone) used[r.y] = ub | bit;
-                }
-                state[c] = (static_cast<uint64_t>(round) << 32) | cls;
-                my_max = cls > my_max ? cls : my_max;
-                ++my_done;
-                nx[0] = r.z;
-                nx[1] = r.w;
-                const uint32_t oa = r.z != kNone ? atomicSub(&pending[r.z], 1u) : 0u;
-                const uint32_t ob = r.w != kNone ? atomicSub(&pending[r.w], 1u) : 0u;
-                ready[0] = r.z != kNone && oa == 1u;
-                ready[1] = r.w != kNone && ob == 1u;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t k = threadIdx.x; k <= kMaxColours; k += blockDim.x) lhist[k] = 0;
+        __syncthreads();
+        my_max = 0;
+        my_done = 0;
+        running = 0;
+        exceeded = false;
+        const uint32_t max_rounds = serial ? nc + 1u : 256u;
+        uint32_t *cur = B.fr0, *nxt = B.fr1;
+        uint32_t ncur = fcnt[0], round = 1;
+        for (; ncur > 0; ++round) {
+            if (round > max_rounds) {  // deeper than the reference allows (uniform)
+                exceeded = true;
+                break;
             }
-            wave_append<2>(ready, nx, nxt, nxt_count);
+            if (serial && gtid == 0) B.boff[round - 1] = running;  // class round-1 = this frontier
+            running += ncur;
+            uint32_t *nxt_count = &fcnt[round % 3];
+            if (gtid == 0) fcnt[(round + 1) % 3] = 0u;
+            for (uint32_t i0 = wave0; i0 < ncur; i0 += gstride) {
+                const uint32_t i = i0 + lane;
+                bool ready[2] = {false, false};
+                uint32_t nx[2] = {0u, 0u};
+                if (i < ncur) {
+                    const uint32_t cls = assign_one(cur[i], round, serial, B.lrec, B.pending, B.used, B.state, meta,
+                                                    ready, nx);
+                    my_max = cls > my_max ? cls : my_max;
+                    ++my_done;
+                    if (!serial) atomicAdd(&lhist[cls < kMaxColours ? cls : kMaxColours], 1u);
+                }
+                wave_append<2>(ready, nx, nxt, nxt_count);
+            }
+            grid_barrier(bar, gen, err, kErrBarrier);
+            ncur = fcnt[round % 3];
+            uint32_t *t = cur; cur = nxt; nxt = t;
+        }
+        rounds = round - 1u;
+        grid_barrier(bar, gen, err, kErrBarrier);  // overflow flags of the last round
+        const bool fallback = !serial && (exceeded || __hip_atomic_load(&meta[kMOverflow], __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT) != 0u);
+        if (!fallback) break;
+        // The reference's serial path: contact order, classes = dependency levels.
+        serial = 1;
+        if (gtid == 0) {
+            meta[kMSerial] = 1u;
+            meta[kMColourRounds] = exceeded ? 257u : rounds;
+            fcnt[0] = 0u;
+            fcnt[1] = 0u;
+            fcnt[2] = 0u;
+        }
+        const uint32_t total = meta[kMDyn];
+        for (uint32_t p = gtid; p < total; p += gstride)
+            rank_entry(p, B.adj, B.own, B.off, B.deg, 1, B.slot, B.crank);
+        grid_barrier(bar, gen, err, kErrBarrier);
+        for (uint32_t c0 = wave0; c0 < nc; c0 += gstride) {
+            const uint32_t c = c0 + lane;
+            const bool pred[1] = {c < nc && link_contact(c, B.cv, B.im, B.off, B.deg, B.slot, B.crank, B.lrec,
+                                                         B.pending)};
+            const uint32_t val[1] = {c};
+            wave_append<1>(pred, val, B.fr0, &fcnt[0]);
         }
         grid_barrier(bar, gen, err, kErrBarrier);
-        ncur = *nxt_count;
-        uint32_t *t = cur; cur = nxt; nxt = t;
+    }
+    // Class sizes: per-block colour histograms, or the frontier sizes.
+    if (!serial) {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k <= kMaxColours; k += blockDim.x)
+            if (lhist[k]) atomicAdd(&B.hist[k], lhist[k]);
     }
     my_max = wave_max_u32(my_max);
     my_done = wave_sum_u32(my_done);
-    if (lane == 0) {
-        atomicMax(&meta[2], my_max);
-        atomicAdd(&meta[5], my_done);
+    if (lane == 0 && my_done) {
+        atomicMax(&meta[kMMaxCls], my_max);
+        atomicAdd(&meta[kMAssigned], my_done);
     }
-    if (gtid == 0) meta[3] = exceeded ? max_rounds + 1u : round - 1u;
+    grid_barrier(bar, gen, err, kErrBarrier);
+    const bool ok = !exceeded && __hip_atomic_load(&meta[kMAssigned], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nc;
+    uint32_t ncls = 0;
+    if (ok) ncls = serial ? rounds : (nc ? meta[kMMaxCls] + 1u : 0u);
+    if (gtid == 0) {
+        if (!ok) atomicOr(err, kErrAssign);
+        meta[kMRounds] = rounds;
+        meta[kMNcls] = ncls;
+        if (serial) {
+            B.boff[ncls] = ok ? running : 0u;
+        } else {
+            uint32_t run = 0;
+            for (uint32_t k = 0; k < ncls; ++k) {
+                B.boff[k] = run;
+                run += B.hist[k];
+            }
+            B.boff[ncls] = run;
+        }
+    }
+    for (uint32_t k = gtid; k < ncls; k += gstride) B.bfill[k] = 0u;
 }
 
 // One round per launch (used when a cooperative launch is unavailable or
 // disabled): the frontier of round r is read from fr[r & 1] with its size in
 // fcnt[(r - 1) % 3], the next one written to fr[(r + 1) & 1].
-__global__ __launch_bounds__(kSolveThreads) void assign_lists_round(const uint4 *__restrict__ lrec, int serial,
-                                                                    uint32_t round, uint32_t max_cur,
+__global__ __launch_bounds__(kSolveThreads) void assign_lists_round(const uint4 *__restrict__ lrec, uint32_t round,
                                                                     uint32_t *pending, uint64_t *used,
                                                                     uint64_t *state, uint32_t *fr0, uint32_t *fr1,
-                                                                    uint32_t *fcnt, uint32_t *meta) {
+                                                                    uint32_t *meta) {
+    uint32_t *fcnt = meta + kMFcnt;
     const uint32_t ncur = fcnt[(round - 1u) % 3u];
+    const int serial = static_cast<int>(meta[kMSerial]);
     uint32_t *cur = (round & 1u) ? fr0 : fr1;
     uint32_t *nxt = (round & 1u) ? fr1 : fr0;
     uint32_t *nxt_count = &fcnt[round % 3u];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && ncur > 0) atomicMax(&meta[3], round);
-    bool ready[2] = {false, false};
-    uint32_t nx[2] = {0u, 0u};
-    if (i < ncur && i < max_cur) {
-        const uint32_t c = cur[i];
-        const uint4 r = lrec[c];
-        uint32_t cls;
-        if (serial) {
-            cls = round - 1u;
-        } else {
-            const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
-            const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
-            const uint64_t u = ua | ub;
-            if (u == ~0ull) {
-                atomicOr(&meta[1], 1u);
-                cls = 64u;
-            } else {
-                cls = static_cast<uint32_t>(__builtin_ctzll(~u));
-            }
-            const uint64_t bit = 1ull << (cls & 63u);
-            if (r.x != kNone) used[r.x] = ua | bit;
-            if (r.y != kNone) used[r.y] = ub | bit;
+    const uint32_t lane = lane_id();
+    if (grid_tid() == 0 && ncur > 0) atomicMax(&meta[kMRounds], round);
+    for (uint32_t i0 = grid_tid() & ~63u; i0 < ncur; i0 += grid_stride()) {
+        const uint32_t i = i0 + lane;
+        bool ready[2] = {false, false};
+        uint32_t nx[2] = {0u, 0u};
+        if (i < ncur) {
+            const uint32_t cls = assign_one(cur[i], round, serial, lrec, pending, used, state, meta, ready, nx);
+            atomicMax(&meta[kMMaxCls], cls);
+            atomicAdd(&meta[kMAssigned], 1u);
         }
-        state[c] = (static_cast<uint64_t>(round) << 32) | cls;
-        atomicMax(&meta[2], cls);
-        atomicAdd(&meta[5], 1u);
-        nx[0] = r.z;
-        nx[1] = r.w;
-        const uint32_t oa = r.z != kNone ? atomicSub(&pending[r.z], 1u) : 0u;
-        const uint32_t ob = r.w != kNone ? atomicSub(&pending[r.w], 1u) : 0u;
-        ready[0] = r.z != kNone && oa == 1u;
-        ready[1] = r.w != kNone && ob == 1u;
+        wave_append<2>(ready, nx, nxt, nxt_count);
     }
-    wave_append<2>(ready, nx, nxt, nxt_count);
 }
 
-// Clears the frontier count two rounds ahead (launched between rounds).
-__global__ void clear_count(uint32_t *fcnt, uint32_t idx) { fcnt[idx] = 0u; }
+// Sets fcnt[idx] = 0 and, when `restart` (1, or 2 after more than 256
+// colouring rounds), switches to the reference order and clears the round
+// statistics (launched between rounds by the host).
+__global__ void round_control(uint32_t *meta, uint32_t idx, int restart) {
+    meta[kMFcnt + idx] = 0u;
+    if (restart) {
+        meta[kMColourRounds] = restart == 2 ? 257u : meta[kMRounds];
+        meta[kMSerial] = 1u;
+        meta[kMRounds] = 0u;
+        meta[kMMaxCls] = 0u;
+        meta[kMAssigned] = 0u;
+        meta[kMFcnt] = 0u;
+        meta[kMFcnt + 1] = 0u;
+        meta[kMFcnt + 2] = 0u;
+    }
+}
 
-// Class histogram, aggregated per workgroup in LDS when the class count fits.
-__global__ __launch_bounds__(kSolveThreads) void class_count(uint32_t nc, uint32_t n_cls,
+// Class histogram for the launch-per-round path.
+__global__ __launch_bounds__(kSolveThreads) void class_count(const uint32_t *__restrict__ meta, uint32_t n_cls,
                                                              const uint64_t *__restrict__ state,
                                                              uint32_t *__restrict__ hist) {
-    __shared__ uint32_t lh[kLdsClasses];
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool local = n_cls <= kLdsClasses;
-    if (local)
-        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x) lh[k] = 0;
-    __syncthreads();
-    if (c < nc) {
+    const uint32_t nc = meta[kMNc];
+    for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
         const uint32_t k = cls_of(state[c]);
-        if (local) atomicAdd(&lh[k], 1u);
-        else atomicAdd(&hist[k], 1u);
+        if (k < n_cls) atomicAdd(&hist[k], 1u);
     }
-    __syncthreads();
-    if (local)
-        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x)
-            if (lh[k]) atomicAdd(&hist[k], lh[k]);
 }
 
 // Contact record for the sweeps, class-sorted: (a, b), (p, n.x), (n.y, n.z, depth, -).
@@ -367,34 +489,49 @@ struct CRec {
     float4 *p0, *p1;
 };
 
-__global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv, uint32_t nc, uint32_t n_cls,
+__global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv, uint32_t *meta,
                                                               const uint64_t *__restrict__ state,
                                                               const uint32_t *__restrict__ boff,
                                                               uint32_t *__restrict__ bfill, CRec rec,
                                                               uint32_t *__restrict__ rec_id) {
     __shared__ uint32_t lh[kLdsClasses];
     __shared__ uint32_t lbase[kLdsClasses];
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nc = meta[kMNc], n_cls = meta[kMNcls];
     const bool local = n_cls <= kLdsClasses;
-    if (local)
-        for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x) lh[k] = 0;
-    __syncthreads();
-    uint32_t k = 0, rank = 0;
-    if (c < nc) {
-        k = cls_of(state[c]);
-        rank = local ? atomicAdd(&lh[k], 1u) : atomicAdd(&bfill[k], 1u);
+    for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += grid_stride()) {
+        const uint32_t c = c0 + threadIdx.x;
+        if (local)
+            for (uint32_t k = threadIdx.x; k < n_cls; k += blockDim.x) lh[k] = 0;
+        __syncthreads();
+        uint32_t k = 0, rank = 0;
+        bool valid = c < nc;
+        if (valid) {
+            k = cls_of(state[c]);
+            if (k >= n_cls) {
+                atomicOr(&meta[kMErr], kErrAssign);
+                valid = false;
+            } else {
+                rank = local ? atomicAdd(&lh[k], 1u) : atomicAdd(&bfill[k], 1u);
+            }
+        }
+        __syncthreads();
+        if (local)
+            for (uint32_t q = threadIdx.x; q < n_cls; q += blockDim.x)
+                lbase[q] = lh[q] ? atomicAdd(&bfill[q], lh[q]) : 0u;
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = boff[k] + (local ? lbase[k] + rank : rank);
+            if (pos < nc) {
+                rec.ab[pos] = make_uint2(cv.body_a[c], cv.body_b[c]);
+                rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
+                rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
+                rec_id[pos] = c;
+            } else {
+                atomicOr(&meta[kMErr], kErrAssign);
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (local)
-        for (uint32_t q = threadIdx.x; q < n_cls; q += blockDim.x)
-            lbase[q] = lh[q] ? atomicAdd(&bfill[q], lh[q]) : 0u;
-    __syncthreads();
-    if (c >= nc) return;
-    const uint32_t pos = boff[k] + (local ? lbase[k] + rank : rank);
-    rec.ab[pos] = make_uint2(cv.body_a[c], cv.body_b[c]);
-    rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
-    rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
-    rec_id[pos] = c;
 }
 
 // AoS body records: dyn = (p.xyz, q.w) (q.xyz, v.x) (v.yz, w.xy) (w.z, renorms, -, -);
@@ -463,13 +600,17 @@ __global__ __launch_bounds__(kSolveThreads) void sweep_class(const uint32_t *__r
         solve_one(i, it, rec, lam, dyn, sta, mu, dt);
 }
 
-__global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_cls,
-                                                                  uint32_t n_iters, CRec rec, float *__restrict__ lam,
-                                                                  float4 *__restrict__ dyn,
-                                                                  const float4 *__restrict__ sta, float mu, float dt,
-                                                                  uint32_t *bar, uint32_t *err) {
+// All I x K phases in one cooperative launch; the class count comes from the
+// assignment (meta), the phases are separated by grid barriers.  Nothing is
+// swept after an assignment error.
+__global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_iters,
+                                                         CRec rec, float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                         const float4 *__restrict__ sta, float mu, float dt,
+                                                         uint32_t *meta) {
     uint32_t gen = 0;
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, gstride = gridDim.x * blockDim.x;
+    uint32_t *bar = meta + kMBarSweep, *err = meta + kMErr;
+    const uint32_t n_cls = (meta[kMErr] & kErrAssign) ? 0u : meta[kMNcls];
+    const uint32_t gtid = grid_tid(), gstride = grid_stride();
     for (uint32_t it = 0; it < n_iters; ++it) {
         for (uint32_t k = 0; k < n_cls; ++k) {
             const uint32_t beg = boff[k], end = boff[k + 1];
@@ -509,44 +650,74 @@ struct dy_xpbd_s {
     uint32_t cap = 0;
     float *lam = nullptr;
     uint64_t *state = nullptr;
-    uint32_t *adj = nullptr, *own = nullptr, *slot = nullptr, *crank = nullptr;
+    uint32_t *adj = nullptr, *own = nullptr, *slot = nullptr, *crank = nullptr, *pending = nullptr;
+    uint32_t *fr0 = nullptr, *fr1 = nullptr;
     uint32_t *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr;
+    uint32_t *meta = nullptr;  // kMWords device words
     uint4 *lrec = nullptr;
-    uint32_t *meta = nullptr;  // kMetaWords + barrier words
     uint2 *rec_ab = nullptr;
     float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
-    DevBuf<uint32_t> deg, off, fill, scan_tmp, pending, fr0, fr1;
+    DevBuf<uint32_t> deg, off, fill, scan_tmp;
     DevBuf<uint64_t> used;
     DevBuf<float4> dyn, sta;
+    // Metadata of the last solve, copied back asynchronously into pinned
+    // memory; ev_meta completes when the copy has landed.
+    uint32_t *h_meta = nullptr;
+    hipEvent_t ev_meta = nullptr;
+    bool meta_pending = false;
+    uint32_t last_nb = 0;
     dy_solve_stats stats{};
-    uint32_t h_meta[kMetaWords + 8] = {};
     int coop_assign_max = 0, coop_sweep_max = 0, device_cus = 0;
     int sweep_blocks_override = 0, assign_blocks_override = 0;
     int assign_threads = 256, sweep_threads = 256;
-    int path = 0;  // 0 persistent sweep, 1 per-class launches
+    int path = 0;  // 0 cooperative kernels, 1 one launch per round / per class
     // Kernel timing (dy_xpbd_set_timing): event pairs recorded around the
     // assignment and sweep kernels, read back once the next solve syncs.
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool ev_pending = false, ev_assign = false;
-    double pending_bytes = 0.0;
+    bool ev_pending = false;
     dy_solver_timing acc{};
 };
 
 namespace {
-// Adds the event intervals of the previous solve to the totals (the events
-// have completed once the stream reached a later sync point).
+
+uint32_t grid_for(uint32_t n) {
+    const uint32_t g = ceil_div(n ? n : 1u, kSolveThreads);
+    return g < kMaxGridBlocks ? g : kMaxGridBlocks;
+}
+
+// Waits for the last solve's metadata copy and fills x->stats; reports the
+// device error flags.
+int sync_meta(dy_xpbd_s *x) {
+    if (!x->meta_pending) return DY_OK;
+    DY_TRY(hipEventSynchronize(x->ev_meta));
+    x->meta_pending = false;
+    const uint32_t *m = x->h_meta;
+    x->stats.n_constraints = m[kMNc];
+    x->stats.used_serial = m[kMSerial];
+    x->stats.n_classes = m[kMNcls];
+    x->stats.rounds = m[kMRounds];
+    x->stats.overflow = m[kMOverflow];
+    x->stats.dyn_endpoints = m[kMDyn];
+    return m[kMErr] ? DY_ERR_INTERNAL : DY_OK;
+}
+
+// Adds the event intervals of the previous solve to the totals.
 int collect_timing(dy_xpbd_s *x) {
     if (!x->ev_pending) return DY_OK;
+    DY_TRY_STATUS(sync_meta(x));
     float a = 0.f, b = 0.f;
     DY_TRY(hipEventSynchronize(x->ev[3]));
-    if (x->ev_assign) DY_TRY(hipEventElapsedTime(&a, x->ev[0], x->ev[1]));
+    DY_TRY(hipEventElapsedTime(&a, x->ev[0], x->ev[1]));
     DY_TRY(hipEventElapsedTime(&b, x->ev[2], x->ev[3]));
-    x->ev_assign = false;
     x->acc.solves += 1;
     x->acc.assign_ms += a;
     x->acc.sweep_ms += b;
-    x->acc.sweep_bytes += x->pending_bytes;
+    // Algorithmic bytes of the sweep: per contact per iteration 36 B record
+    // + 8 B lambda, per dynamic endpoint 80 B read + 52 B written, plus the
+    // per-iteration renormalisation (32 B per body).
+    x->acc.sweep_bytes += static_cast<double>(x->n_iters) *
+                          (44.0 * x->stats.n_constraints + 132.0 * x->stats.dyn_endpoints + 32.0 * x->last_nb);
     x->ev_pending = false;
     return DY_OK;
 }
@@ -554,13 +725,14 @@ int collect_timing(dy_xpbd_s *x) {
 void xp_free(dy_xpbd_s *x) {
     for (hipEvent_t &e : x->ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
-    void *ptrs[] = {x->lam, x->state, x->adj, x->own, x->slot, x->crank, x->lrec, x->boff, x->bfill, x->hist,
-                    x->rec_id, x->meta, x->rec_ab, x->rec_p0, x->rec_p1};
+    if (x->ev_meta) (void)hipEventDestroy(x->ev_meta);
+    if (x->h_meta) (void)hipHostFree(x->h_meta);
+    void *ptrs[] = {x->lam, x->state, x->adj, x->own, x->slot, x->crank, x->pending, x->fr0, x->fr1, x->lrec,
+                    x->boff, x->bfill, x->hist, x->rec_id, x->meta, x->rec_ab, x->rec_p0, x->rec_p1};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release();
-    x->pending.release(); x->fr0.release(); x->fr1.release(); x->used.release();
-    x->dyn.release(); x->sta.release();
+    x->used.release(); x->dyn.release(); x->sta.release();
 }
 
 int env_int(const char *name, int dflt) {
@@ -593,8 +765,8 @@ int query_coop(dy_xpbd_s *x) {
     return DY_OK;
 }
 
-// Blocks for a cooperative kernel: the override, else device_cus / 8 workgroups
-// of 256 threads' worth, clamped to what can be co-resident.
+// Blocks for a cooperative kernel: the override, else device_cus / 8
+// workgroups of 256 threads' worth, clamped to what can be co-resident.
 uint32_t coop_blocks(const dy_xpbd_s *x, int override_blocks, int threads, int max_blocks) {
     int b = override_blocks > 0 ? override_blocks : (x->device_cus / 8) * 256 / threads;
     if (b < 1) b = 1;
@@ -602,64 +774,64 @@ uint32_t coop_blocks(const dy_xpbd_s *x, int override_blocks, int threads, int m
     return static_cast<uint32_t>(b);
 }
 
-// Assigns a class to every contact; leaves meta downloaded in x->h_meta.
-int assign_classes(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, uint32_t nc, const float *im, int serial) {
-    const uint32_t gc = ceil_div(nc, kSolveThreads);
-    uint32_t *fcnt = x->meta + kMetaWords + 4;
-    DY_TRY(hipMemsetAsync(fcnt, 0, 3 * sizeof(uint32_t), st));
-    list_rank<<<ceil_div(2u * nc, kSolveThreads), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p,
-                                                                          x->meta + 6, serial, x->slot, x->crank);
-    DY_LAUNCH_CHECK();
-    list_link<<<gc, kSolveThreads, 0, st>>>(cv, nc, im, x->off.p, x->deg.p, x->slot, x->crank, x->lrec,
-                                             x->pending.p, x->fr0.p, fcnt);
-    DY_LAUNCH_CHECK();
-    const uint32_t max_rounds = serial ? nc + 1u : 256u;
-    if (x->coop_assign_max > 0 && x->path == 0) {
-        uint32_t *bar = x->meta + kMetaWords;
-        DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
-        const uint32_t blocks = coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
-        const uint4 *lrecv = x->lrec;
-        int serialv = serial;
-        uint32_t maxr = max_rounds;
-        uint32_t *pendv = x->pending.p, *f0 = x->fr0.p, *f1 = x->fr1.p, *metav = x->meta;
-        uint64_t *usedv = x->used.p, *statev = x->state;
-        uint32_t *errv = x->meta + kMetaWords + 2;
-        void *args[] = {&lrecv, &serialv, &maxr, &pendv, &usedv, &statev, &f0, &f1, &fcnt, &metav, &bar, &errv};
-        if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
-        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
-                                          dim3(static_cast<uint32_t>(x->assign_threads)), args, 0, st));
-        if (x->timing) {
-            DY_TRY(hipEventRecord(x->ev[1], st));
-            x->ev_assign = true;
-        }
-        DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
+// Launch-per-round assignment: the host drives the rounds and decides the
+// fallback to the reference order, reading the metadata every 16 rounds.
+int assign_by_rounds(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, const float *im) {
+    uint32_t h[kMWords];
+    for (int pass = 0; pass < 2; ++pass) {
+        DY_TRY(hipMemcpyAsync(h, x->meta, sizeof(h), hipMemcpyDeviceToHost, st));
         DY_TRY(hipStreamSynchronize(st));
-        if (x->h_meta[4] || x->h_meta[kMetaWords + 2]) return DY_ERR_INTERNAL;
-        if (x->h_meta[3] > max_rounds) {
-            if (serial) return DY_ERR_INTERNAL;
-            x->h_meta[3] = 257u;  // the reference would stop after 256 rounds
-            return DY_OK;
+        const uint32_t nc = h[kMNc];
+        const int serial = static_cast<int>(h[kMSerial]);
+        const uint32_t max_rounds = serial ? nc + 1u : 256u;
+        const uint32_t grid = grid_for(nc);
+        bool done = false, exceeded = false;
+        for (uint32_t r = 1; !done; ++r) {
+            if (r > max_rounds) {
+                exceeded = true;
+                break;
+            }
+            round_control<<<1, 1, 0, st>>>(x->meta, r % 3u, 0);
+            DY_LAUNCH_CHECK();
+            assign_lists_round<<<grid, kSolveThreads, 0, st>>>(x->lrec, r, x->pending, x->used.p, x->state, x->fr0,
+                                                               x->fr1, x->meta);
+            DY_LAUNCH_CHECK();
+            if (r % 16u == 0u || r == max_rounds) {
+                DY_TRY(hipMemcpyAsync(h, x->meta, sizeof(h), hipMemcpyDeviceToHost, st));
+                DY_TRY(hipStreamSynchronize(st));
+                if (h[kMErr]) return DY_ERR_INTERNAL;
+                done = h[kMAssigned] == nc;
+            }
         }
-        return x->h_meta[5] == nc ? DY_OK : DY_ERR_INTERNAL;
-    }
-    // One launch per round, checked every 16 rounds.
-    for (uint32_t r = 1; r <= max_rounds; ++r) {
-        clear_count<<<1, 1, 0, st>>>(fcnt, r % 3u);
-        DY_LAUNCH_CHECK();
-        assign_lists_round<<<gc, kSolveThreads, 0, st>>>(x->lrec, serial, r, nc, x->pending.p, x->used.p, x->state,
-                                                          x->fr0.p, x->fr1.p, fcnt, x->meta);
-        DY_LAUNCH_CHECK();
-        if (r % 16u == 0u || r == max_rounds) {
-            DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, sizeof(x->h_meta), hipMemcpyDeviceToHost, st));
-            DY_TRY(hipStreamSynchronize(st));
-            if (x->h_meta[4]) return DY_ERR_INTERNAL;
-            if (x->h_meta[5] == nc) return DY_OK;
+        DY_TRY(hipMemcpyAsync(h, x->meta, sizeof(h), hipMemcpyDeviceToHost, st));
+        DY_TRY(hipStreamSynchronize(st));
+        if (!serial && (exceeded || h[kMOverflow])) {
+            // The reference's serial path: re-rank in contact order and rerun.
+            round_control<<<1, 1, 0, st>>>(x->meta, 0, exceeded ? 2 : 1);
+            DY_LAUNCH_CHECK();
+            list_rank<<<grid_for(2u * nc), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->meta,
+                                                                   x->slot, x->crank);
+            DY_LAUNCH_CHECK();
+            list_link<<<grid_for(nc), kSolveThreads, 0, st>>>(cv, x->meta, im, x->off.p, x->deg.p, x->slot, x->crank,
+                                                              x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
+            DY_LAUNCH_CHECK();
+            continue;
         }
+        if (h[kMAssigned] != nc) return DY_ERR_INTERNAL;
+        // Class offsets from a histogram of the classes.
+        const uint32_t k = serial ? h[kMRounds] : (nc ? h[kMMaxCls] + 1u : 0u);
+        DY_TRY(hipMemsetAsync(x->hist, 0, (k + 1) * sizeof(uint32_t), st));
+        DY_TRY(hipMemsetAsync(x->bfill, 0, (k + 1) * sizeof(uint32_t), st));
+        class_count<<<grid_for(nc), kSolveThreads, 0, st>>>(x->meta, k, x->state, x->hist);
+        DY_LAUNCH_CHECK();
+        DY_TRY_STATUS(scan_exclusive(st, x->hist, x->boff, k + 1, nullptr, x->scan_tmp.p));
+        DY_TRY(hipMemcpyAsync(x->meta + kMNcls, &k, sizeof(k), hipMemcpyHostToDevice, st));
+        DY_TRY(hipStreamSynchronize(st));
+        return DY_OK;
     }
-    if (serial) return DY_ERR_INTERNAL;
-    x->h_meta[3] = 257u;
-    return DY_OK;
+    return DY_ERR_INTERNAL;
 }
+
 }  // namespace
 
 extern "C" {
@@ -673,22 +845,32 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     x->cap = max_contacts;
     const size_t c = max_contacts ? max_contacts : 1;
     const size_t ncls = c + 66;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&x->lam), c * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->state), c * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->adj), 2 * c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->own), 2 * c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->slot), 2 * c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->crank), 2 * c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->lrec), c * sizeof(uint4));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->boff), ncls * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->bfill), ncls * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->hist), ncls * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_id), c * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), (kMetaWords + 8) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_ab), c * sizeof(uint2));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p0), c * sizeof(float4));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->rec_p1), c * sizeof(float4));
-    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, (kMetaWords + 8) * sizeof(uint32_t), hs(s));
+    hipError_t e = hipSuccess;
+    auto dalloc = [&e](auto *&p, size_t n) {
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(*p));
+    };
+    dalloc(x->lam, c);
+    dalloc(x->state, c);
+    dalloc(x->adj, 2 * c);
+    dalloc(x->own, 2 * c);
+    dalloc(x->slot, 2 * c);
+    dalloc(x->crank, 2 * c);
+    dalloc(x->pending, c);
+    dalloc(x->fr0, c);
+    dalloc(x->fr1, c);
+    dalloc(x->lrec, c);
+    dalloc(x->boff, ncls);
+    dalloc(x->bfill, ncls);
+    dalloc(x->hist, ncls);
+    dalloc(x->rec_id, c);
+    dalloc(x->meta, kMWords);
+    dalloc(x->rec_ab, c);
+    dalloc(x->rec_p0, c);
+    dalloc(x->rec_p1, c);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&x->h_meta), kMWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&x->ev_meta, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, kMWords * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(hs(s));
     if (e != hipSuccess) {
         xp_free(x);
         delete x;
@@ -720,101 +902,84 @@ int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial) {
 
 int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_t n_joints, const dy_body_view *bv,
                   float dt) {
-    if (!x || !cv || !bv) return DY_ERR_INVALID_ARG;
+    if (!x || !cv || !bv || !cv->n) return DY_ERR_INVALID_ARG;
     if (n_joints != 0) return DY_ERR_UNSUPPORTED;
     hipStream_t st = hs(s);
-    x->stats = dy_solve_stats{};
-    uint32_t head[2] = {0, 0};  // contact count; sweep error flags of the previous solve
-    DY_TRY(hipMemcpyAsync(&head[0], cv->n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    DY_TRY(hipMemcpyAsync(&head[1], x->meta + kMetaWords + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    DY_TRY(hipStreamSynchronize(st));
-    if (head[1]) return DY_ERR_INTERNAL;
-    const uint32_t nc = head[0];
-    x->stats.n_constraints = nc;
-    if (nc == 0) return DY_OK;
-    if (nc > x->cap) return DY_ERR_INVALID_ARG;
-    const uint32_t nb = bv->n;
+    // Errors of earlier solves surface here once their metadata has landed.
+    if (x->timing) {
+        DY_TRY_STATUS(collect_timing(x));
+    } else if (x->meta_pending && hipEventQuery(x->ev_meta) == hipSuccess) {
+        DY_TRY_STATUS(sync_meta(x));
+    }
+    const uint32_t nb = bv->n, cap = x->cap;
     DY_TRY_STATUS(x->deg.ensure(nb));
     DY_TRY_STATUS(x->off.ensure(nb));
     DY_TRY_STATUS(x->fill.ensure(nb));
     DY_TRY_STATUS(x->used.ensure(nb));
-    DY_TRY_STATUS(x->pending.ensure(nc));
-    DY_TRY_STATUS(x->fr0.ensure(nc));
-    DY_TRY_STATUS(x->fr1.ensure(nc));
     DY_TRY_STATUS(x->dyn.ensure(4 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->sta.ensure(2 * static_cast<size_t>(nb)));
-    DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > nc + 66u ? nb : nc + 66u)));
-    const uint32_t prep_n = nb > nc ? nb : nc;
-    solve_prep<<<ceil_div(prep_n, 256), 256, 0, st>>>(nb, nc, x->deg.p, x->fill.p, x->used.p, x->lam, x->state,
-                                                      x->meta);
+    DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > cap + 66u ? nb : cap + 66u)));
+    x->last_nb = nb;
+    uint32_t top = nb > cap ? nb : cap;
+    if (top < kMWords + kMaxColours + 1u) top = kMWords + kMaxColours + 1u;
+    solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(nb, cv->n, cap, x->force_serial, x->deg.p, x->fill.p,
+                                                         x->used.p, x->lam, x->hist, x->meta);
     DY_LAUNCH_CHECK();
-    const uint32_t gc = ceil_div(nc, kSolveThreads);
-    solve_degree<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->deg.p);
+    solve_degree<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->deg.p);
     DY_LAUNCH_CHECK();
-    DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + 6, x->scan_tmp.p));
-    solve_fill<<<gc, kSolveThreads, 0, st>>>(*cv, nc, bv->inv_mass, x->off.p, x->fill.p, x->adj, x->own);
+    DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + kMDyn, x->scan_tmp.p));
+    solve_fill<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->off.p, x->fill.p, x->adj,
+                                                         x->own);
     DY_LAUNCH_CHECK();
     gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
+    list_rank<<<grid_for(2u * cap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->meta, x->slot,
+                                                             x->crank);
+    DY_LAUNCH_CHECK();
+    list_link<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->off.p, x->deg.p, x->slot,
+                                                        x->crank, x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
+    DY_LAUNCH_CHECK();
 
-    int serial = (x->force_serial || nc <= 256u) ? 1 : 0;
-    DY_TRY_STATUS(collect_timing(x));
-    DY_TRY_STATUS(assign_classes(x, st, *cv, nc, bv->inv_mass, serial));
-    if (!serial) {
-        x->stats.overflow = x->h_meta[1];
-        x->stats.rounds = x->h_meta[3];
-        if (x->h_meta[1] || x->h_meta[3] > 256u) {
-            // The reference runs its serial path in these cases.
-            serial = 1;
-            solve_reassign_prep<<<ceil_div(nc, 256), 256, 0, st>>>(nc, x->state, x->meta);
-            DY_LAUNCH_CHECK();
-            DY_TRY_STATUS(assign_classes(x, st, *cv, nc, bv->inv_mass, 1));
-        }
+    const bool coop = x->path == 0 && x->coop_assign_max > 0 && x->coop_sweep_max > 0;
+    if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
+    if (coop) {
+        AssignBufs B{*cv, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
+                     x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill};
+        void *args[] = {&B};
+        const uint32_t blocks = coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
+        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
+                                          dim3(static_cast<uint32_t>(x->assign_threads)), args, 0, st));
+    } else {
+        DY_TRY_STATUS(assign_by_rounds(x, st, *cv, bv->inv_mass));
     }
-    const uint32_t k = x->h_meta[2] + 1u;
-    x->stats.used_serial = static_cast<uint32_t>(serial);
-    x->stats.n_classes = k;
-    if (serial) x->stats.rounds = x->h_meta[3];
+    if (x->timing) DY_TRY(hipEventRecord(x->ev[1], st));
 
     // Class-sorted contact records.
-    DY_TRY(hipMemsetAsync(x->hist, 0, (k + 1) * sizeof(uint32_t), st));
-    DY_TRY(hipMemsetAsync(x->bfill, 0, k * sizeof(uint32_t), st));
-    class_count<<<gc, kSolveThreads, 0, st>>>(nc, k, x->state, x->hist);
-    DY_LAUNCH_CHECK();
-    DY_TRY_STATUS(scan_exclusive(st, x->hist, x->boff, k + 1, nullptr, x->scan_tmp.p));
     CRec rec{x->rec_ab, x->rec_p0, x->rec_p1};
-    class_gather<<<gc, kSolveThreads, 0, st>>>(*cv, nc, k, x->state, x->boff, x->bfill, rec, x->rec_id);
+    class_gather<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id);
     DY_LAUNCH_CHECK();
 
     const uint32_t iters = static_cast<uint32_t>(x->n_iters);
-    x->stats.dyn_endpoints = x->h_meta[6];
-    if (x->timing) {
-        // Algorithmic bytes of the sweep: per contact per iteration 36 B
-        // record + 8 B lambda, and per dynamic endpoint 80 B read + 52 B
-        // written; plus the per-iteration renormalisation (32 B per body).
-        x->pending_bytes = static_cast<double>(iters) *
-                           (44.0 * nc + 132.0 * x->h_meta[6] + 32.0 * static_cast<double>(nb));
-        DY_TRY(hipEventRecord(x->ev[2], st));
-    }
+    if (x->timing) DY_TRY(hipEventRecord(x->ev[2], st));
     if (iters > 0) {
-        const bool persistent = x->path == 0 && x->coop_sweep_max > 0;
-        if (persistent) {
+        if (coop) {
             const uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max);
-            uint32_t *bar = x->meta + kMetaWords;
-            DY_TRY(hipMemsetAsync(bar, 0, 2 * sizeof(uint32_t), st));
             const uint32_t *boffv = x->boff;
-            uint32_t kv = k, iv = iters;
+            uint32_t iv = iters;
             float *lamv = x->lam;
             float4 *dynv = x->dyn.p;
             const float4 *stav = x->sta.p;
             float muv = x->mu, dtv = dt;
-            uint32_t *errv = x->meta + kMetaWords + 2;
-            void *args[] = {&boffv, &kv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &bar, &errv};
+            uint32_t *metav = x->meta;
+            void *args[] = {&boffv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &metav};
             DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_persistent), dim3(blocks),
                                               dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
         } else {
-            // Per-class launches; each reads its range from the device offsets.
-            const uint32_t blocks = gc < 1024u ? gc : 1024u;
+            // Per-class launches; the class count is read back first.
+            uint32_t k = 0;
+            DY_TRY(hipMemcpyAsync(&k, x->meta + kMNcls, sizeof(k), hipMemcpyDeviceToHost, st));
+            DY_TRY(hipStreamSynchronize(st));
+            const uint32_t blocks = grid_for(cap);
             for (uint32_t it = 0; it < iters; ++it)
                 for (uint32_t c = 0; c < k; ++c) {
                     sweep_class<<<blocks, kSolveThreads, 0, st>>>(x->boff, c, it, rec, x->lam, x->dyn.p, x->sta.p,
@@ -829,6 +994,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     }
     scatter_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, iters);
     DY_LAUNCH_CHECK();
+    DY_TRY(hipMemcpyAsync(x->h_meta, x->meta, kMWords * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    DY_TRY(hipEventRecord(x->ev_meta, st));
+    x->meta_pending = true;
     return DY_OK;
 }
 
@@ -852,8 +1020,9 @@ int dy_xpbd_get_timing(dy_xpbd_t x, dy_stream_t s, dy_solver_timing *out) {
 
 int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out) {
     if (!x || !out) return DY_ERR_INVALID_ARG;
+    const int st = sync_meta(x);
     *out = x->stats;
-    return DY_OK;
+    return st;
 }
 
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n) {

@@ -236,10 +236,16 @@ int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial);
  * n_joints must be 0 (joints are out of scope: DY_ERR_UNSUPPORTED otherwise).
  * <= 256 contacts or force_serial: reference contact order (run_serial
  * semantics), executed in parallel over dependency levels; otherwise the
- * reference's Jones-Plassmann colours, swept in colour order.  Blocking (reads
- * the contact count and the class count). */
+ * reference's Jones-Plassmann colours, swept in colour order, falling back to
+ * the contact order where the reference does (a contact needing a 65th
+ * colour, or more than 256 rounds).  The contact count is read on the device
+ * (*contacts->n, clamped to max_contacts; a larger count is an error).
+ * Asynchronous on the stream (the launch-per-round path, DYPHUR_SOLVER_PATH=1,
+ * synchronises).  Errors of a solve are returned by a later solve or by the
+ * blocking queries below. */
 int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *contacts, uint32_t n_joints,
                   const dy_body_view *bodies, float dt);
+/* Statistics of the last solve; blocking (waits for that solve). */
 int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out);
 /* Enable (1) or disable (0) kernel timing; enabling resets the totals. */
 int dy_xpbd_set_timing(dy_xpbd_t x, int enable);
