@@ -192,13 +192,13 @@ def main():
     # Roofline of the dominant kernel, timed with HIP events on the solver's
     # stream over the timed frames.  The sweep's algorithmic bytes per launch
     # (one launch per frame): iterations x (44 B per contact + 132 B per
-    # dynamic contact endpoint + 32 B per body).  Class assignment reads each
-    # contact's record and neighbour lists: 12 B per contact + 16 B per body.
+    # dynamic contact endpoint + 32 B per body).  Class assignment: 12 B per
+    # contact + 16 B per body (the per-stage figure for the colouring).
     n_solves = max(timing["solves"], 1)
     assign_bytes = (12.0 * float(np.sum(contacts)) + 16.0 * n * len(contacts)) / len(contacts)
     kernels = {
         "sweep_persistent": (timing["sweep_ms"] / n_solves, timing["sweep_bytes"] / n_solves),
-        "assign_frontier": (timing["assign_ms"] / n_solves, assign_bytes),
+        "assign_lists": (timing["assign_ms"] / n_solves, assign_bytes),
     }
     name = max(kernels, key=lambda k: kernels[k][0])
     ms, nbytes = kernels[name]
