@@ -69,6 +69,7 @@ enum : uint32_t {
     kMBarAssign = 16,    // 2 words: grid barrier of assign_lists
     kMBarSweep = 18,     // 2 words: grid barrier of sweep_persistent
     kMFcnt = 20,         // 3 words: frontier sizes
+    kMRegSweep = 24,     // 2 words: check-in of sweep_xcd (participants, arrivals)
     kMWords = 32,
 };
 constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u;
@@ -620,6 +621,150 @@ __global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sweep confined to one XCD.  The workgroups of one XCD share its L2: the
+// vector L1 writes through to it and atomics execute in it, so workgroups of
+// one XCD can hand data to each other through the L2 with no cache
+// maintenance, provided every read of data written during the kernel skips
+// the L1 (agent-scope loads) and every writer waits for its stores before
+// arriving.  A grid barrier between the phases then costs one L2 atomic per
+// workgroup plus an L2-atomic poll, instead of the agent-scope write-back and
+// invalidate of the L2 at every phase.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSweepXcc = 0;  // the XCD whose workgroups run the sweep
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    return static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xfu;  // HW_REG_XCC_ID
+}
+
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Current value at the L2 (a returning atomic add of zero, which the compiler
+// would otherwise turn into a load that may hit the L1).
+__device__ __forceinline__ uint32_t l2_read(uint32_t *p) {
+    uint32_t v;
+    const uint32_t zero = 0u;
+    asm volatile("global_atomic_add %0, %1, %2, off sc0\n\ts_waitcnt vmcnt(0)"
+                 : "=v"(v) : "v"(p), "v"(zero) : "memory");
+    return v;
+}
+
+__device__ __forceinline__ float4 ld_agent4(const float4 *p) {
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+    const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float(static_cast<uint32_t>(a)), __uint_as_float(static_cast<uint32_t>(a >> 32)),
+                       __uint_as_float(static_cast<uint32_t>(b)), __uint_as_float(static_cast<uint32_t>(b >> 32)));
+}
+
+// Barrier among the nblk workgroups of one XCD; false on a timeout (err set).
+__device__ __forceinline__ bool xcd_barrier(uint32_t *bar, uint32_t &gen, uint32_t nblk, uint32_t *err) {
+    __shared__ uint32_t s_ok;
+    wait_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = gen;
+        uint32_t ok = 1;
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (arrived == nblk - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            wait_vmem();
+            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            wait_vmem();
+        } else {
+            uint32_t spins = 0;
+            while (l2_read(&bar[1]) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kGridSpinLimit) {
+                    atomicOr(err, kErrBarrier);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        s_ok = ok;
+    }
+    gen += 1u;
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__device__ __forceinline__ BodyState load_aos_agent(const float4 *__restrict__ dyn, const float4 *__restrict__ sta,
+                                                    uint32_t i, uint32_t &renorms) {
+    const float4 d0 = ld_agent4(&dyn[4 * i]), d1 = ld_agent4(&dyn[4 * i + 1]);
+    const float4 d2 = ld_agent4(&dyn[4 * i + 2]), d3 = ld_agent4(&dyn[4 * i + 3]);
+    const float4 s0 = sta[2 * i], s1 = sta[2 * i + 1];
+    BodyState s;
+    s.px = d0.x; s.py = d0.y; s.pz = d0.z; s.qw = d0.w;
+    s.qx = d1.x; s.qy = d1.y; s.qz = d1.z; s.vx = d1.w;
+    s.vy = d2.x; s.vz = d2.y; s.wx = d2.z; s.wy = d2.w;
+    s.wz = d3.x;
+    renorms = __float_as_uint(d3.y);
+    s.im = s0.x; s.ixx = s0.y; s.iyy = s0.z; s.izz = s0.w;
+    s.ixy = s1.x; s.ixz = s1.y; s.iyz = s1.z;
+    return s;
+}
+
+__device__ __forceinline__ void solve_one_agent(uint32_t i, uint32_t it, const CRec &rec, float *__restrict__ lam,
+                                                float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
+                                                float dt) {
+    const uint2 ab = rec.ab[i];
+    const float4 p0 = rec.p0[i], p1 = rec.p1[i];
+    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+    uint32_t ra, rb;
+    BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
+    const bool da = A.im != 0.f, db = B.im != 0.f;
+    if (da) catch_up(A, ra, it);
+    if (db) catch_up(B, rb, it);
+    float l = lam[i];  // lam[i] is only ever touched by this thread
+    project_contact(ci, A, B, mu, l, dt);
+    lam[i] = l;
+    if (da) store_aos(dyn, ab.x, A, it);
+    if (db) store_aos(dyn, ab.y, B, it);
+}
+
+// All I x K phases on the workgroups of XCD kSweepXcc.  The launch has
+// workgroups on every XCD (cooperative, all co-resident); each checks in once
+// at agent scope, the others leave, and the participants number themselves
+// densely.  Contact i of a phase always goes to the same thread.
+__global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ boff, uint32_t n_iters, CRec rec,
+                                                  float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                  const float4 *__restrict__ sta, float mu, float dt,
+                                                  uint32_t *meta) {
+    __shared__ uint32_t s_idx, s_n;
+    uint32_t *reg = meta + kMRegSweep, *bar = meta + kMBarSweep, *err = meta + kMErr;
+    const bool mine = xcc_id() == kSweepXcc;
+    if (threadIdx.x == 0) {
+        s_idx = mine ? __hip_atomic_fetch_add(&reg[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        __hip_atomic_fetch_add(&reg[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        s_n = 0;
+        if (mine) {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&reg[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gridDim.x) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kGridSpinLimit) {
+                    atomicOr(err, kErrBarrier);
+                    break;
+                }
+            }
+            s_n = __hip_atomic_load(&reg[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    if (!mine || s_n == 0) return;
+    const uint32_t nblk = s_n;
+    const uint32_t n_cls = (meta[kMErr] & kErrAssign) ? 0u : meta[kMNcls];
+    const uint32_t gtid = s_idx * blockDim.x + threadIdx.x, gstride = nblk * blockDim.x;
+    uint32_t gen = 0;
+    for (uint32_t it = 0; it < n_iters; ++it) {
+        for (uint32_t k = 0; k < n_cls; ++k) {
+            const uint32_t beg = boff[k], end = boff[k + 1];
+            for (uint32_t i = beg + gtid; i < end; i += gstride) solve_one_agent(i, it, rec, lam, dyn, sta, mu, dt);
+            if ((it + 1 < n_iters || k + 1 < n_cls) && !xcd_barrier(bar, gen, nblk, err)) return;
+        }
+    }
+}
+
 // Back to the SoA view, bringing every body to n_iters renormalisations.
 __global__ void scatter_bodies(dy_body_view bv, const float4 *__restrict__ dyn, uint32_t n_iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -667,7 +812,8 @@ struct dy_xpbd_s {
     bool meta_pending = false;
     uint32_t last_nb = 0;
     dy_solve_stats stats{};
-    int coop_assign_max = 0, coop_sweep_max = 0, device_cus = 0;
+    int coop_assign_max = 0, coop_sweep_max = 0, coop_xcd_max = 0, device_cus = 0;
+    int sweep_mode = 0;  // 0 sweep_xcd, 1 sweep_persistent (whole grid)
     int sweep_blocks_override = 0, assign_blocks_override = 0;
     int assign_threads = 256, sweep_threads = 256;
     int path = 0;  // 0 cooperative kernels, 1 one launch per round / per class
@@ -740,9 +886,10 @@ int env_int(const char *name, int dflt) {
     return e ? std::atoi(e) : dflt;
 }
 
-// Cooperative-launch limits; DYPHUR_{ASSIGN,SWEEP}_{BLOCKS,THREADS} and
-// DYPHUR_SOLVER_PATH (1 = one launch per round / per class) override the
-// launch shapes for measurement.
+// Cooperative-launch limits; DYPHUR_{ASSIGN,SWEEP}_{BLOCKS,THREADS},
+// DYPHUR_SWEEP_MODE (1 = sweep over the whole grid) and DYPHUR_SOLVER_PATH
+// (1 = one launch per round / per class) override the launch shapes for
+// measurement.
 int query_coop(dy_xpbd_s *x) {
     int dev = 0, coop = 0;
     DY_TRY(hipGetDevice(&dev));
@@ -751,6 +898,7 @@ int query_coop(dy_xpbd_s *x) {
     x->sweep_blocks_override = env_int("DYPHUR_SWEEP_BLOCKS", 0);
     x->assign_blocks_override = env_int("DYPHUR_ASSIGN_BLOCKS", 0);
     x->path = env_int("DYPHUR_SOLVER_PATH", 0);
+    x->sweep_mode = env_int("DYPHUR_SWEEP_MODE", 0) == 1 ? 1 : 0;
     const int at = env_int("DYPHUR_ASSIGN_THREADS", 256), sw = env_int("DYPHUR_SWEEP_THREADS", 256);
     x->assign_threads = (at == 64 || at == 128 || at == 256 || at == 512 || at == 1024) ? at : 256;
     x->sweep_threads = (sw == 64 || sw == 128 || sw == 256 || sw == 512 || sw == 1024) ? sw : 256;
@@ -762,6 +910,9 @@ int query_coop(dy_xpbd_s *x) {
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_persistent),
                                                         x->sweep_threads, 0));
     x->coop_sweep_max = per_cu * x->device_cus;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_xcd),
+                                                        x->sweep_threads, 0));
+    x->coop_xcd_max = per_cu * x->device_cus;
     return DY_OK;
 }
 
@@ -963,7 +1114,14 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     if (x->timing) DY_TRY(hipEventRecord(x->ev[2], st));
     if (iters > 0) {
         if (coop) {
-            const uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max);
+            // sweep_xcd: P participants on one XCD out of a launch of P per XCD
+            // (8 XCDs on gfx950); sweep_persistent: P workgroups over the chip.
+            const bool xcd = x->sweep_mode == 0 && x->coop_xcd_max > 0;
+            uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max);
+            if (xcd) {
+                blocks *= 8u;
+                if (blocks > static_cast<uint32_t>(x->coop_xcd_max)) blocks = static_cast<uint32_t>(x->coop_xcd_max);
+            }
             const uint32_t *boffv = x->boff;
             uint32_t iv = iters;
             float *lamv = x->lam;
@@ -972,8 +1130,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
             float muv = x->mu, dtv = dt;
             uint32_t *metav = x->meta;
             void *args[] = {&boffv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &metav};
-            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_persistent), dim3(blocks),
-                                              dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
+            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(xcd ? sweep_xcd : sweep_persistent),
+                                              dim3(blocks), dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0,
+                                              st));
         } else {
             // Per-class launches; the class count is read back first.
             uint32_t k = 0;

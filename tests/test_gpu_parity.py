@@ -210,6 +210,7 @@ LAUNCH_SHAPES = [
     ("per_round_launches", {"DYPHUR_SOLVER_PATH": "1"}),
     ("wg1024", {"DYPHUR_ASSIGN_THREADS": "1024", "DYPHUR_SWEEP_THREADS": "1024"}),
     ("one_block", {"DYPHUR_ASSIGN_BLOCKS": "1", "DYPHUR_SWEEP_BLOCKS": "1"}),
+    ("sweep_whole_grid", {"DYPHUR_SWEEP_MODE": "1"}),
 ]
 
 
@@ -218,7 +219,8 @@ LAUNCH_SHAPES = [
 def test_solver_launch_shapes_bitexact(gpu, oracle, stage_inputs, monkeypatch, shape, force_serial):
     """The class assignment and sweeps give the same bits under every launch
     shape the solver supports (one launch per round/class, 1024-thread
-    workgroups, a single workgroup)."""
+    workgroups, a single workgroup, the sweep over the whole grid instead of
+    one XCD)."""
     for k, v in dict(LAUNCH_SHAPES)[shape].items():
         monkeypatch.setenv(k, v)
     sc, ob0, osh = stage_inputs["mixed_3000_f30"]
