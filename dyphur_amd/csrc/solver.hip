@@ -723,6 +723,43 @@ __device__ __forceinline__ void solve_one_agent(uint32_t i, uint32_t it, const C
     if (db) store_aos(dyn, ab.y, B, it);
 }
 
+// A contact record and its lambda, loaded ahead of the phase that uses them.
+struct CPre {
+    uint2 ab;
+    float4 p0, p1;
+    float l;
+    uint32_t i;
+    bool valid;
+};
+
+__device__ __forceinline__ void pre_load(CPre &c, uint32_t i, uint32_t end, const CRec &rec,
+                                         const float *__restrict__ lam) {
+    c.i = i;
+    c.valid = i < end;
+    if (c.valid) {
+        c.ab = rec.ab[i];
+        c.p0 = rec.p0[i];
+        c.p1 = rec.p1[i];
+        c.l = lam[i];
+    }
+}
+
+__device__ __forceinline__ void solve_pre(const CPre &c, uint32_t it, float *__restrict__ lam,
+                                          float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
+                                          float dt) {
+    const ContactIn ci{c.p0.x, c.p0.y, c.p0.z, c.p0.w, c.p1.x, c.p1.y, c.p1.z};
+    uint32_t ra, rb;
+    BodyState A = load_aos_agent(dyn, sta, c.ab.x, ra), B = load_aos_agent(dyn, sta, c.ab.y, rb);
+    const bool da = A.im != 0.f, db = B.im != 0.f;
+    if (da) catch_up(A, ra, it);
+    if (db) catch_up(B, rb, it);
+    float l = c.l;
+    project_contact(ci, A, B, mu, l, dt);
+    lam[c.i] = l;
+    if (da) store_aos(dyn, c.ab.x, A, it);
+    if (db) store_aos(dyn, c.ab.y, B, it);
+}
+
 // All I x K phases on the workgroups of XCD kSweepXcc.  The launch has
 // workgroups on every XCD (cooperative, all co-resident); each checks in once
 // at agent scope, the others leave, and the participants number themselves
@@ -756,11 +793,21 @@ __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ b
     const uint32_t n_cls = (meta[kMErr] & kErrAssign) ? 0u : meta[kMNcls];
     const uint32_t gtid = s_idx * blockDim.x + threadIdx.x, gstride = nblk * blockDim.x;
     uint32_t gen = 0;
+    // The record and lambda of this thread's first contact of the next phase
+    // do not depend on the phase barrier: they are loaded before it.
+    CPre pf;
+    if (n_cls > 0 && n_iters > 0) pre_load(pf, boff[0] + gtid, boff[1], rec, lam);
     for (uint32_t it = 0; it < n_iters; ++it) {
         for (uint32_t k = 0; k < n_cls; ++k) {
-            const uint32_t beg = boff[k], end = boff[k + 1];
-            for (uint32_t i = beg + gtid; i < end; i += gstride) solve_one_agent(i, it, rec, lam, dyn, sta, mu, dt);
-            if ((it + 1 < n_iters || k + 1 < n_cls) && !xcd_barrier(bar, gen, nblk, err)) return;
+            const uint32_t end = boff[k + 1];
+            const CPre cur = pf;
+            if (cur.valid) solve_pre(cur, it, lam, dyn, sta, mu, dt);
+            for (uint32_t i = cur.i + gstride; i < end; i += gstride) solve_one_agent(i, it, rec, lam, dyn, sta, mu, dt);
+            const bool last_k = k + 1 == n_cls;
+            if (it + 1 == n_iters && last_k) break;
+            const uint32_t kn = last_k ? 0u : k + 1u;
+            pre_load(pf, boff[kn] + gtid, boff[kn + 1], rec, lam);
+            if (!xcd_barrier(bar, gen, nblk, err)) return;
         }
     }
 }
