@@ -89,7 +89,7 @@ def build_python_module(verbose: bool = False) -> str | None:
     out = os.path.join(PKG, "dyphur_py" + suffix)
     tmp = out + ".tmp"
     inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
-           f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(CSRC, 'host')}"]
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(PKG, 'cpp', 'include')}"]
     _run(["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-fvisibility=hidden", *inc, src,
           f"-L{LIB}", "-ldyphur_hip", "-Wl,-rpath,$ORIGIN/lib", "-o", tmp])
     os.replace(tmp, out)

@@ -20,8 +20,8 @@ public:
 
     static Device default_gpu() { return Device(0); }
     static Device default_cpu() {
-        throw Error(DY_ERR_UNSUPPORTED, "Device::default_cpu: this build runs on gfx950 GPUs only "
-                                        "(use Device::default_gpu() or Device::preferred())");
+        throw Error(DY_ERR_NO_DEVICE, "Device::default_cpu: this build has no CPU device, it runs on gfx950 "
+                                      "GPUs (use Device::default_gpu() or Device::preferred())");
     }
     static Device preferred() { return default_gpu(); }
 

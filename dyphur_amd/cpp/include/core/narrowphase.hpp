@@ -63,6 +63,8 @@ public:
     }
     bool overflowed() const { return last_contact_count() > cap_; }
 
+    dy_narrowphase_t handle() const noexcept { return h_.get(); }
+
 private:
     std::shared_ptr<dy_narrowphase_s> h_;
     dy_stream_t s_ = nullptr;
