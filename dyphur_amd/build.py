@@ -129,10 +129,30 @@ def build_cpp_tests(verbose: bool = False) -> str:
     return out
 
 
+def build_examples(verbose: bool = False) -> list:
+    """Drivers over the C++ layer (dyphur_amd/cpp/examples) -> dyphur_amd/bin."""
+    out_dir = os.path.join(PKG, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    src_dir = os.path.join(PKG, "cpp", "examples")
+    outs = []
+    for src in sorted(os.listdir(src_dir)):
+        if not src.endswith(".cpp"):
+            continue
+        out = os.path.join(out_dir, src[:-4])
+        _run(["g++", "-O2", "-std=c++20", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+              f"-I{os.path.join(PKG, 'cpp', 'include')}", os.path.join(src_dir, src), "-o", out + ".tmp",
+              f"-L{LIB}", "-ldyphur_hip", "-Wl,-rpath,$ORIGIN/../lib"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+        if verbose:
+            print("built", out)
+    return outs
+
+
 def build_all(verbose: bool = False):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    paths = [build_hip(verbose), build_hostmath(verbose), build_cpp_tests(verbose)]
+    paths = [build_hip(verbose), build_hostmath(verbose), build_cpp_tests(verbose), *build_examples(verbose)]
     py = build_python_module(verbose)
     if py:
         paths.append(py)
