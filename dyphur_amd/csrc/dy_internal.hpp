@@ -132,4 +132,101 @@ __device__ __forceinline__ void grid_barrier(uint32_t *bar, uint32_t &gen, uint3
     __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Work confined to one XCD.  The workgroups of one XCD share its L2: the
+// vector L1 writes through to it and atomics execute in it, so they can hand
+// data to each other through the L2 with no cache maintenance, provided every
+// read of data written during the kernel skips the L1 (agent-scope loads) and
+// every writer waits for its stores before arriving at a barrier.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcc_id() {
+    return static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xfu;  // HW_REG_XCC_ID
+}
+
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Current value at the L2 (a returning atomic add of zero, which the compiler
+// would otherwise turn into a load that may hit the L1).
+__device__ __forceinline__ uint32_t l2_read(uint32_t *p) {
+    uint32_t v;
+    const uint32_t zero = 0u;
+    asm volatile("global_atomic_add %0, %1, %2, off sc0\n\ts_waitcnt vmcnt(0)"
+                 : "=v"(v) : "v"(p), "v"(zero) : "memory");
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier among the nblk workgroups of one XCD (bar[0] arrivals, bar[1]
+// generation; zero before the launch).  With `fence`, thread 0 also acquires
+// at agent scope after the wait, so that plain loads after the barrier see
+// data written before it by the other workgroups.  False on a timeout (err
+// gets err_bit).
+__device__ __forceinline__ bool xcd_barrier(uint32_t *bar, uint32_t &gen, uint32_t nblk, uint32_t *err,
+                                            uint32_t err_bit, bool fence = false) {
+    __shared__ uint32_t s_ok;
+    wait_vmem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = gen;
+        uint32_t ok = 1;
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (arrived == nblk - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            wait_vmem();
+            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            wait_vmem();
+        } else {
+            uint32_t spins = 0;
+            while (l2_read(&bar[1]) == g) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kGridSpinLimit) {
+                    atomicOr(err, err_bit);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = ok;
+    }
+    gen += 1u;
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// Check-in of a cooperative launch that works on XCD `xcc` only: every
+// workgroup arrives on reg[1] (agent scope); the ones on that XCD take dense
+// indices from reg[0] and wait until all have arrived.  Returns true for a
+// participant, with its index and the participant count.
+__device__ __forceinline__ bool xcd_checkin(uint32_t xcc, uint32_t *reg, uint32_t *err, uint32_t err_bit,
+                                            uint32_t &idx, uint32_t &n) {
+    __shared__ uint32_t s_idx, s_n;
+    const bool mine = xcc_id() == xcc;
+    if (threadIdx.x == 0) {
+        s_idx = mine ? __hip_atomic_fetch_add(&reg[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        __hip_atomic_fetch_add(&reg[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        s_n = 0;
+        if (mine) {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&reg[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gridDim.x) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kGridSpinLimit) {
+                    atomicOr(err, err_bit);
+                    break;
+                }
+            }
+            s_n = __hip_atomic_load(&reg[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    idx = s_idx;
+    n = s_n;
+    return mine && n > 0;
+}
+
+
 }  // namespace dy

@@ -52,6 +52,7 @@ constexpr uint32_t kLdsClasses = 2048;  // block-aggregated class counting limit
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaxColours = 64;
 constexpr uint32_t kMaxGridBlocks = 1024;  // grid-stride kernels over capacity-sized ranges
+constexpr uint32_t kSweepXcc = 0;         // the XCD whose workgroups run the assignment and the sweep
 
 // Solver metadata (device words).  solve_prep zeroes all but kMErr, which
 // keeps error flags until the solver is destroyed.
@@ -70,9 +71,10 @@ enum : uint32_t {
     kMBarSweep = 18,     // 2 words: grid barrier of sweep_persistent
     kMFcnt = 20,         // 3 words: frontier sizes
     kMRegSweep = 24,     // 2 words: check-in of sweep_xcd (participants, arrivals)
+    kMRegAssign = 26,    // 2 words: check-in of assign_lists
     kMWords = 32,
 };
-constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u;
+constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u, kErrFlow = 16u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
 
@@ -83,12 +85,13 @@ __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x 
 // the reference orders contacts serially at <= 256 contacts or when forced.
 __global__ void solve_prep(uint32_t nb, const uint32_t *__restrict__ ncp, uint32_t cap, int force_serial,
                            uint32_t *__restrict__ deg, uint32_t *__restrict__ fill, uint64_t *__restrict__ used,
-                           float *__restrict__ lam, uint32_t *__restrict__ hist, uint32_t *__restrict__ meta) {
+                           uint32_t *__restrict__ ver, float *__restrict__ lam, uint32_t *__restrict__ hist,
+                           uint32_t *__restrict__ meta) {
     const uint32_t n_raw = *ncp;
     const uint32_t nc = n_raw < cap ? n_raw : cap;
     const uint32_t top = nb > nc ? nb : nc;
     for (uint32_t i = grid_tid(); i < top; i += grid_stride()) {
-        if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; }
+        if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; ver[i] = 0; }
         if (i < nc) lam[i] = 0.f;
     }
     const uint32_t i = grid_tid();
@@ -270,8 +273,8 @@ __device__ __forceinline__ uint32_t assign_one(uint32_t c, uint32_t round, int s
     if (serial) {
         cls = round - 1u;
     } else {
-        const uint64_t ua = r.x != kNone ? used[r.x] : 0ull;
-        const uint64_t ub = r.y != kNone ? used[r.y] : 0ull;
+        const uint64_t ua = r.x != kNone ? ld_agent(&used[r.x]) : 0ull;
+        const uint64_t ub = r.y != kNone ? ld_agent(&used[r.y]) : 0ull;
         const uint64_t u = ua | ub;
         if (u == ~0ull) {
             atomicOr(&meta[kMOverflow], 1u);
@@ -320,12 +323,17 @@ struct AssignBufs {
 // lists are re-ranked in contact order and the rounds re-run as dependency
 // levels: the reference's run_serial order.  The kernel ends with the class
 // offsets (boff[0..k]), zeroed per-class fill counters and the statistics.
+// Like the sweep, it runs on the workgroups of one XCD (see xcd_barrier):
+// frontier entries, used-colour masks and counters written during the kernel
+// are read at agent scope (L2), pending counts are L2 atomics.
 __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
     __shared__ uint32_t lhist[kMaxColours + 1];
-    uint32_t gen = 0;
     uint32_t *meta = B.meta;
     uint32_t *bar = meta + kMBarAssign, *fcnt = meta + kMFcnt, *err = meta + kMErr;
-    const uint32_t gtid = grid_tid(), gstride = grid_stride();
+    uint32_t bidx = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegAssign, err, kErrBarrier, bidx, nblk)) return;
+    uint32_t gen = 0;
+    const uint32_t gtid = bidx * blockDim.x + threadIdx.x, gstride = nblk * blockDim.x;
     const uint32_t lane = lane_id(), wave0 = gtid & ~63u;
     const uint32_t nc = meta[kMNc];
     int serial = static_cast<int>(meta[kMSerial]);
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         exceeded = false;
         const uint32_t max_rounds = serial ? nc + 1u : 256u;
         uint32_t *cur = B.fr0, *nxt = B.fr1;
-        uint32_t ncur = fcnt[0], round = 1;
+        uint32_t ncur = ld_agent(&fcnt[0]), round = 1;
         for (; ncur > 0; ++round) {
             if (round > max_rounds) {  // deeper than the reference allows (uniform)
                 exceeded = true;
@@ -355,24 +363,26 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
                 bool ready[2] = {false, false};
                 uint32_t nx[2] = {0u, 0u};
                 if (i < ncur) {
-                    const uint32_t cls = assign_one(cur[i], round, serial, B.lrec, B.pending, B.used, B.state, meta,
-                                                    ready, nx);
+                    const uint32_t cls = assign_one(ld_agent(&cur[i]), round, serial, B.lrec, B.pending, B.used,
+                                                    B.state, meta, ready, nx);
                     my_max = cls > my_max ? cls : my_max;
                     ++my_done;
                     if (!serial) atomicAdd(&lhist[cls < kMaxColours ? cls : kMaxColours], 1u);
                 }
                 wave_append<2>(ready, nx, nxt, nxt_count);
             }
-            grid_barrier(bar, gen, err, kErrBarrier);
-            ncur = fcnt[round % 3];
+            xcd_barrier(bar, gen, nblk, err, kErrBarrier);
+            ncur = ld_agent(&fcnt[round % 3]);
             uint32_t *t = cur; cur = nxt; nxt = t;
         }
         rounds = round - 1u;
-        grid_barrier(bar, gen, err, kErrBarrier);  // overflow flags of the last round
-        const bool fallback = !serial && (exceeded || __hip_atomic_load(&meta[kMOverflow], __ATOMIC_RELAXED,
-                                                                        __HIP_MEMORY_SCOPE_AGENT) != 0u);
+        xcd_barrier(bar, gen, nblk, err, kErrBarrier);  // overflow flags of the last round
+        const bool fallback = !serial && (exceeded || ld_agent(&meta[kMOverflow]) != 0u);
         if (!fallback) break;
-        // The reference's serial path: contact order, classes = dependency levels.
+        // The reference's serial path: contact order, classes = dependency
+        // levels.  The lists are rebuilt here, so the barriers around the
+        // rebuild also make plain loads of them see the other workgroups'
+        // stores.
         serial = 1;
         if (gtid == 0) {
             meta[kMSerial] = 1u;
@@ -384,7 +394,7 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         const uint32_t total = meta[kMDyn];
         for (uint32_t p = gtid; p < total; p += gstride)
             rank_entry(p, B.adj, B.own, B.off, B.deg, 1, B.slot, B.crank);
-        grid_barrier(bar, gen, err, kErrBarrier);
+        xcd_barrier(bar, gen, nblk, err, kErrBarrier, true);
         for (uint32_t c0 = wave0; c0 < nc; c0 += gstride) {
             const uint32_t c = c0 + lane;
             const bool pred[1] = {c < nc && link_contact(c, B.cv, B.im, B.off, B.deg, B.slot, B.crank, B.lrec,
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
             const uint32_t val[1] = {c};
             wave_append<1>(pred, val, B.fr0, &fcnt[0]);
         }
-        grid_barrier(bar, gen, err, kErrBarrier);
+        xcd_barrier(bar, gen, nblk, err, kErrBarrier, true);
     }
     // Class sizes: per-block colour histograms, or the frontier sizes.
     if (!serial) {
@@ -406,10 +416,10 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         atomicMax(&meta[kMMaxCls], my_max);
         atomicAdd(&meta[kMAssigned], my_done);
     }
-    grid_barrier(bar, gen, err, kErrBarrier);
-    const bool ok = !exceeded && __hip_atomic_load(&meta[kMAssigned], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nc;
+    xcd_barrier(bar, gen, nblk, err, kErrBarrier);
+    const bool ok = !exceeded && ld_agent(&meta[kMAssigned]) == nc;
     uint32_t ncls = 0;
-    if (ok) ncls = serial ? rounds : (nc ? meta[kMMaxCls] + 1u : 0u);
+    if (ok) ncls = serial ? rounds : (nc ? ld_agent(&meta[kMMaxCls]) + 1u : 0u);
     if (gtid == 0) {
         if (!ok) atomicOr(err, kErrAssign);
         meta[kMRounds] = rounds;
@@ -420,7 +430,7 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
             uint32_t run = 0;
             for (uint32_t k = 0; k < ncls; ++k) {
                 B.boff[k] = run;
-                run += B.hist[k];
+                run += ld_agent(&B.hist[k]);
             }
             B.boff[ncls] = run;
         }
@@ -494,7 +504,9 @@ __global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv
                                                               const uint64_t *__restrict__ state,
                                                               const uint32_t *__restrict__ boff,
                                                               uint32_t *__restrict__ bfill, CRec rec,
-                                                              uint32_t *__restrict__ rec_id) {
+                                                              uint32_t *__restrict__ rec_id,
+                                                              uint32_t *__restrict__ cpos,
+                                                              uint4 *__restrict__ rseq) {
     __shared__ uint32_t lh[kLdsClasses];
     __shared__ uint32_t lbase[kLdsClasses];
     const uint32_t nc = meta[kMNc], n_cls = meta[kMNcls];
@@ -527,6 +539,8 @@ __global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv
                 rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
                 rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
                 rec_id[pos] = c;
+                cpos[c] = pos;
+                rseq[pos] = make_uint4(0u, 0u, 0u, 0u);
             } else {
                 atomicOr(&meta[kMErr], kErrAssign);
             }
@@ -631,62 +645,12 @@ __global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restr
 // workgroup plus an L2-atomic poll, instead of the agent-scope write-back and
 // invalidate of the L2 at every phase.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kSweepXcc = 0;  // the XCD whose workgroups run the sweep
-
-__device__ __forceinline__ uint32_t xcc_id() {
-    return static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xfu;  // HW_REG_XCC_ID
-}
-
-__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
-// Current value at the L2 (a returning atomic add of zero, which the compiler
-// would otherwise turn into a load that may hit the L1).
-__device__ __forceinline__ uint32_t l2_read(uint32_t *p) {
-    uint32_t v;
-    const uint32_t zero = 0u;
-    asm volatile("global_atomic_add %0, %1, %2, off sc0\n\ts_waitcnt vmcnt(0)"
-                 : "=v"(v) : "v"(p), "v"(zero) : "memory");
-    return v;
-}
-
 __device__ __forceinline__ float4 ld_agent4(const float4 *p) {
     const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
     const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return make_float4(__uint_as_float(static_cast<uint32_t>(a)), __uint_as_float(static_cast<uint32_t>(a >> 32)),
                        __uint_as_float(static_cast<uint32_t>(b)), __uint_as_float(static_cast<uint32_t>(b >> 32)));
-}
-
-// Barrier among the nblk workgroups of one XCD; false on a timeout (err set).
-__device__ __forceinline__ bool xcd_barrier(uint32_t *bar, uint32_t &gen, uint32_t nblk, uint32_t *err) {
-    __shared__ uint32_t s_ok;
-    wait_vmem();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t g = gen;
-        uint32_t ok = 1;
-        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (arrived == nblk - 1) {
-            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            wait_vmem();
-            __hip_atomic_store(&bar[1], g + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            wait_vmem();
-        } else {
-            uint32_t spins = 0;
-            while (l2_read(&bar[1]) == g) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kGridSpinLimit) {
-                    atomicOr(err, kErrBarrier);
-                    ok = 0;
-                    break;
-                }
-            }
-        }
-        s_ok = ok;
-    }
-    gen += 1u;
-    __syncthreads();
-    return s_ok != 0;
 }
 
 __device__ __forceinline__ BodyState load_aos_agent(const float4 *__restrict__ dyn, const float4 *__restrict__ sta,
@@ -768,30 +732,11 @@ __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ b
                                                   float *__restrict__ lam, float4 *__restrict__ dyn,
                                                   const float4 *__restrict__ sta, float mu, float dt,
                                                   uint32_t *meta) {
-    __shared__ uint32_t s_idx, s_n;
-    uint32_t *reg = meta + kMRegSweep, *bar = meta + kMBarSweep, *err = meta + kMErr;
-    const bool mine = xcc_id() == kSweepXcc;
-    if (threadIdx.x == 0) {
-        s_idx = mine ? __hip_atomic_fetch_add(&reg[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        __hip_atomic_fetch_add(&reg[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        s_n = 0;
-        if (mine) {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&reg[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gridDim.x) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > kGridSpinLimit) {
-                    atomicOr(err, kErrBarrier);
-                    break;
-                }
-            }
-            s_n = __hip_atomic_load(&reg[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    __syncthreads();
-    if (!mine || s_n == 0) return;
-    const uint32_t nblk = s_n;
+    uint32_t *bar = meta + kMBarSweep, *err = meta + kMErr;
+    uint32_t b = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
     const uint32_t n_cls = (meta[kMErr] & kErrAssign) ? 0u : meta[kMNcls];
-    const uint32_t gtid = s_idx * blockDim.x + threadIdx.x, gstride = nblk * blockDim.x;
+    const uint32_t gtid = b * blockDim.x + threadIdx.x, gstride = nblk * blockDim.x;
     uint32_t gen = 0;
     // The record and lambda of this thread's first contact of the next phase
     // do not depend on the phase barrier: they are loaded before it.
@@ -807,7 +752,118 @@ __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ b
             if (it + 1 == n_iters && last_k) break;
             const uint32_t kn = last_k ? 0u : k + 1u;
             pre_load(pf, boff[kn] + gtid, boff[kn + 1], rec, lam);
-            if (!xcd_barrier(bar, gen, nblk, err)) return;
+            if (!xcd_barrier(bar, gen, nblk, err, kErrBarrier)) return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow sweep.  Two contacts interact only through a shared dynamic body,
+// and the result is the reference's as long as every body sees its contacts
+// in class order, iteration after iteration.  Each dynamic body x keeps a
+// version ver[x] = contacts applied to it so far; the contact with class rank
+// seq on x (deg[x] contacts on x) runs in iteration it once ver[x] ==
+// it * deg[x] + seq for each of its dynamic bodies, then stores the bodies,
+// waits for the stores and advances the versions.  No phase barriers: a
+// contact waits for its own two predecessors only.
+//
+// Progress: every dependency points to an earlier (iteration, record)
+// position, so the earliest unfinished contact can always run; within a wave
+// the lanes that are ready run while the others keep waiting, so a class
+// boundary inside a wave cannot stall it.  All threads are co-resident
+// (cooperative launch) and every wait is bounded (kErrFlow aborts the sweep).
+// Versions and body records move through the L2 of XCD kSweepXcc (agent-scope
+// loads, stores acknowledged before the version store), as in sweep_xcd.
+// ---------------------------------------------------------------------------
+
+// rseq[pos] = (class rank on a, contacts on a, class rank on b, contacts on
+// b) for the record at class-sorted position pos; zero counts for static
+// ends.  One thread per per-body list entry; classes on one body are distinct.
+__global__ __launch_bounds__(kSolveThreads) void flow_seq(const uint32_t *__restrict__ adj,
+                                                          const uint32_t *__restrict__ own,
+                                                          const uint32_t *__restrict__ off,
+                                                          const uint32_t *__restrict__ deg,
+                                                          const uint64_t *__restrict__ state,
+                                                          const uint32_t *__restrict__ cpos,
+                                                          const uint32_t *__restrict__ meta,
+                                                          uint4 *__restrict__ rseq) {
+    if (meta[kMErr] & kErrAssign) return;
+    const uint32_t total = meta[kMDyn];
+    for (uint32_t p = grid_tid(); p < total; p += grid_stride()) {
+        const uint32_t v = adj[p], x = own[p];
+        const uint32_t c = v >> 1;
+        const uint32_t kc = cls_of(state[c]);
+        const uint32_t beg = off[x], n = deg[x];
+        uint32_t rank = 0;
+        for (uint32_t q = 0; q < n; ++q) rank += cls_of(state[adj[beg + q] >> 1]) < kc ? 1u : 0u;
+        uint32_t *r = reinterpret_cast<uint32_t *>(&rseq[cpos[c]]);
+        r[2u * (v & 1u)] = rank;
+        r[2u * (v & 1u) + 1u] = n;
+    }
+}
+
+__global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
+                                                   float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                   const float4 *__restrict__ sta, uint32_t *ver, float mu,
+                                                   float dt, uint32_t *meta) {
+    uint32_t *err = meta + kMErr;
+    uint32_t b = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
+    if (meta[kMErr] & kErrAssign) return;
+    const uint32_t nc = meta[kMNc];
+    const uint32_t gstride = nblk * blockDim.x;
+    const uint32_t lane = lane_id();
+    const uint32_t wave0 = b * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t it = 0; it < n_iters; ++it) {
+        for (uint32_t i0 = wave0; i0 < nc; i0 += gstride) {
+            const uint32_t i = i0 + lane;
+            bool todo = i < nc;
+            uint2 ab = make_uint2(0u, 0u);
+            uint4 sq = make_uint4(0u, 0u, 0u, 0u);
+            float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+            float l = 0.f;
+            if (todo) {
+                ab = rec.ab[i];
+                sq = rseq[i];
+                p0 = rec.p0[i];
+                p1 = rec.p1[i];
+                l = lam[i];
+            }
+            const uint32_t ta = it * sq.y + sq.x, tb = it * sq.w + sq.z;
+            uint32_t spins = 0;
+            while (__ballot(todo)) {
+                if (todo) {
+                    const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
+                                       (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
+                    if (ready) {
+                        const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+                        uint32_t ra, rb;
+                        BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
+                        const bool da = A.im != 0.f, db = B.im != 0.f;
+                        if (da) catch_up(A, ra, it);
+                        if (db) catch_up(B, rb, it);
+                        project_contact(ci, A, B, mu, l, dt);
+                        lam[i] = l;
+                        if (da) store_aos(dyn, ab.x, A, it);
+                        if (db) store_aos(dyn, ab.y, B, it);
+                        todo = false;
+                    }
+                }
+                wait_vmem();  // body stores acknowledged by the L2 before the versions move
+                if (!todo && i < nc && sq.y != ~0u) {
+                    if (sq.y) __hip_atomic_store(&ver[ab.x], ta + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (sq.w) __hip_atomic_store(&ver[ab.y], tb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    sq.y = ~0u;  // versions published
+                }
+                if (__ballot(todo)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kGridSpinLimit) {
+                        if (lane == 0) atomicOr(err, kErrFlow);
+                        return;
+                    }
+                    if ((spins & 255u) == 0u && (l2_read(err) & kErrFlow)) return;
+                }
+            }
         }
     }
 }
@@ -844,12 +900,12 @@ struct dy_xpbd_s {
     uint64_t *state = nullptr;
     uint32_t *adj = nullptr, *own = nullptr, *slot = nullptr, *crank = nullptr, *pending = nullptr;
     uint32_t *fr0 = nullptr, *fr1 = nullptr;
-    uint32_t *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr;
+    uint32_t *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr, *cpos = nullptr;
     uint32_t *meta = nullptr;  // kMWords device words
-    uint4 *lrec = nullptr;
+    uint4 *lrec = nullptr, *rseq = nullptr;
     uint2 *rec_ab = nullptr;
     float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
-    DevBuf<uint32_t> deg, off, fill, scan_tmp;
+    DevBuf<uint32_t> deg, off, fill, scan_tmp, ver;
     DevBuf<uint64_t> used;
     DevBuf<float4> dyn, sta;
     // Metadata of the last solve, copied back asynchronously into pinned
@@ -859,8 +915,8 @@ struct dy_xpbd_s {
     bool meta_pending = false;
     uint32_t last_nb = 0;
     dy_solve_stats stats{};
-    int coop_assign_max = 0, coop_sweep_max = 0, coop_xcd_max = 0, device_cus = 0;
-    int sweep_mode = 0;  // 0 sweep_xcd, 1 sweep_persistent (whole grid)
+    int coop_assign_max = 0, coop_sweep_max = 0, coop_xcd_max = 0, coop_flow_max = 0, device_cus = 0;
+    int sweep_mode = 2;  // 2 sweep_flow (default), 0 sweep_xcd, 1 sweep_persistent (whole grid)
     int sweep_blocks_override = 0, assign_blocks_override = 0;
     int assign_threads = 256, sweep_threads = 256;
     int path = 0;  // 0 cooperative kernels, 1 one launch per round / per class
@@ -921,10 +977,11 @@ void xp_free(dy_xpbd_s *x) {
     if (x->ev_meta) (void)hipEventDestroy(x->ev_meta);
     if (x->h_meta) (void)hipHostFree(x->h_meta);
     void *ptrs[] = {x->lam, x->state, x->adj, x->own, x->slot, x->crank, x->pending, x->fr0, x->fr1, x->lrec,
-                    x->boff, x->bfill, x->hist, x->rec_id, x->meta, x->rec_ab, x->rec_p0, x->rec_p1};
+                    x->boff, x->bfill, x->hist, x->rec_id, x->cpos, x->rseq, x->meta, x->rec_ab, x->rec_p0,
+                    x->rec_p1};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release();
+    x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release(); x->ver.release();
     x->used.release(); x->dyn.release(); x->sta.release();
 }
 
@@ -934,7 +991,9 @@ int env_int(const char *name, int dflt) {
 }
 
 // Cooperative-launch limits; DYPHUR_{ASSIGN,SWEEP}_{BLOCKS,THREADS},
-// DYPHUR_SWEEP_MODE (1 = sweep over the whole grid) and DYPHUR_SOLVER_PATH
+// DYPHUR_SWEEP_MODE (2 = dataflow sweep, the default; 0 = class phases on one
+// XCD; 1 = class phases over the whole grid) and
+// DYPHUR_SOLVER_PATH
 // (1 = one launch per round / per class) override the launch shapes for
 // measurement.
 int query_coop(dy_xpbd_s *x) {
@@ -945,7 +1004,10 @@ int query_coop(dy_xpbd_s *x) {
     x->sweep_blocks_override = env_int("DYPHUR_SWEEP_BLOCKS", 0);
     x->assign_blocks_override = env_int("DYPHUR_ASSIGN_BLOCKS", 0);
     x->path = env_int("DYPHUR_SOLVER_PATH", 0);
-    x->sweep_mode = env_int("DYPHUR_SWEEP_MODE", 0) == 1 ? 1 : 0;
+    {
+        const int m = env_int("DYPHUR_SWEEP_MODE", 2);
+        x->sweep_mode = (m == 0 || m == 1) ? m : 2;
+    }
     const int at = env_int("DYPHUR_ASSIGN_THREADS", 256), sw = env_int("DYPHUR_SWEEP_THREADS", 256);
     x->assign_threads = (at == 64 || at == 128 || at == 256 || at == 512 || at == 1024) ? at : 256;
     x->sweep_threads = (sw == 64 || sw == 128 || sw == 256 || sw == 512 || sw == 1024) ? sw : 256;
@@ -960,6 +1022,9 @@ int query_coop(dy_xpbd_s *x) {
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_xcd),
                                                         x->sweep_threads, 0));
     x->coop_xcd_max = per_cu * x->device_cus;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_flow),
+                                                        x->sweep_threads, 0));
+    x->coop_flow_max = per_cu * x->device_cus;
     return DY_OK;
 }
 
@@ -1061,6 +1126,8 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     dalloc(x->bfill, ncls);
     dalloc(x->hist, ncls);
     dalloc(x->rec_id, c);
+    dalloc(x->cpos, c);
+    dalloc(x->rseq, c);
     dalloc(x->meta, kMWords);
     dalloc(x->rec_ab, c);
     dalloc(x->rec_p0, c);
@@ -1114,6 +1181,7 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_TRY_STATUS(x->off.ensure(nb));
     DY_TRY_STATUS(x->fill.ensure(nb));
     DY_TRY_STATUS(x->used.ensure(nb));
+    DY_TRY_STATUS(x->ver.ensure(nb));
     DY_TRY_STATUS(x->dyn.ensure(4 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->sta.ensure(2 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > cap + 66u ? nb : cap + 66u)));
@@ -1121,7 +1189,7 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     uint32_t top = nb > cap ? nb : cap;
     if (top < kMWords + kMaxColours + 1u) top = kMWords + kMaxColours + 1u;
     solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(nb, cv->n, cap, x->force_serial, x->deg.p, x->fill.p,
-                                                         x->used.p, x->lam, x->hist, x->meta);
+                                                         x->used.p, x->ver.p, x->lam, x->hist, x->meta);
     DY_LAUNCH_CHECK();
     solve_degree<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->deg.p);
     DY_LAUNCH_CHECK();
@@ -1144,7 +1212,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
         AssignBufs B{*cv, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
                      x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill};
         void *args[] = {&B};
-        const uint32_t blocks = coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
+        // Participants on one XCD out of a launch with as many on every XCD.
+        uint32_t blocks = 8u * coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
+        if (blocks > static_cast<uint32_t>(x->coop_assign_max)) blocks = static_cast<uint32_t>(x->coop_assign_max);
         DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
                                           dim3(static_cast<uint32_t>(x->assign_threads)), args, 0, st));
     } else {
@@ -1154,13 +1224,33 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
 
     // Class-sorted contact records.
     CRec rec{x->rec_ab, x->rec_p0, x->rec_p1};
-    class_gather<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id);
+    class_gather<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id,
+                                                           x->cpos, x->rseq);
     DY_LAUNCH_CHECK();
+    const bool flow = x->path == 0 && x->sweep_mode == 2 && x->coop_flow_max > 0;
+    if (flow) {
+        flow_seq<<<grid_for(2u * cap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->state, x->cpos,
+                                                                x->meta, x->rseq);
+        DY_LAUNCH_CHECK();
+    }
 
     const uint32_t iters = static_cast<uint32_t>(x->n_iters);
     if (x->timing) DY_TRY(hipEventRecord(x->ev[2], st));
     if (iters > 0) {
-        if (coop) {
+        if (coop && flow) {
+            uint32_t blocks = 8u * coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_flow_max);
+            if (blocks > static_cast<uint32_t>(x->coop_flow_max)) blocks = static_cast<uint32_t>(x->coop_flow_max);
+            uint32_t iv = iters;
+            const uint4 *rseqv = x->rseq;
+            float *lamv = x->lam;
+            float4 *dynv = x->dyn.p;
+            const float4 *stav = x->sta.p;
+            uint32_t *verv = x->ver.p, *metav = x->meta;
+            float muv = x->mu, dtv = dt;
+            void *args[] = {&iv, &rec, &rseqv, &lamv, &dynv, &stav, &verv, &muv, &dtv, &metav};
+            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_flow), dim3(blocks),
+                                              dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
+        } else if (coop) {
             // sweep_xcd: P participants on one XCD out of a launch of P per XCD
             // (8 XCDs on gfx950); sweep_persistent: P workgroups over the chip.
             const bool xcd = x->sweep_mode == 0 && x->coop_xcd_max > 0;

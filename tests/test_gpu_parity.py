@@ -161,7 +161,10 @@ def test_golden_box_hash_on_gpu(gpu):
     assert hs[0] == hs[1] == 0xCBCD209C3665C818
 
 
-def test_stress_frames_match_oracle(gpu, oracle):
+@pytest.mark.parametrize("sweep_mode", ["2", "0"])
+def test_stress_frames_match_oracle(gpu, oracle, monkeypatch, sweep_mode):
+    monkeypatch.setenv("DYPHUR_SWEEP_MODE", sweep_mode)
+    print(f"[stress 150 frames] sweep mode {sweep_mode}", flush=True)
     sc = scenes.stress_blocks()
     frames = 150
     b, s = oracle.Bodies(sc), oracle.Shapes(sc)
@@ -210,7 +213,9 @@ LAUNCH_SHAPES = [
     ("per_round_launches", {"DYPHUR_SOLVER_PATH": "1"}),
     ("wg1024", {"DYPHUR_ASSIGN_THREADS": "1024", "DYPHUR_SWEEP_THREADS": "1024"}),
     ("one_block", {"DYPHUR_ASSIGN_BLOCKS": "1", "DYPHUR_SWEEP_BLOCKS": "1"}),
-    ("sweep_whole_grid", {"DYPHUR_SWEEP_MODE": "1"}),
+    ("sweep_phases_whole_grid", {"DYPHUR_SWEEP_MODE": "1"}),
+    ("sweep_phases_one_xcd", {"DYPHUR_SWEEP_MODE": "0"}),
+    ("sweep_phases_one_block", {"DYPHUR_SWEEP_MODE": "0", "DYPHUR_SWEEP_BLOCKS": "1"}),
 ]
 
 
@@ -219,8 +224,8 @@ LAUNCH_SHAPES = [
 def test_solver_launch_shapes_bitexact(gpu, oracle, stage_inputs, monkeypatch, shape, force_serial):
     """The class assignment and sweeps give the same bits under every launch
     shape the solver supports (one launch per round/class, 1024-thread
-    workgroups, a single workgroup, the sweep over the whole grid instead of
-    one XCD)."""
+    workgroups, a single workgroup, and the class-phase sweeps over one XCD or
+    the whole grid beside the default dataflow sweep)."""
     for k, v in dict(LAUNCH_SHAPES)[shape].items():
         monkeypatch.setenv(k, v)
     sc, ob0, osh = stage_inputs["mixed_3000_f30"]
