@@ -44,7 +44,35 @@ def parse():
     ap.add_argument("--cpu-window", type=int, default=20,
                     help="frames per CPU-baseline window (four windows spread over the run)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 10 device-to-device copies of MIB MiB (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> None:
+    from dyphur_amd import capi
+    capi.load()
+    s = capi.Stream()
+    n = mib << 20
+    a = capi.DeviceArray.from_numpy(np.ones(n // 4, np.float32), s)
+    b = capi.DeviceArray.from_numpy(np.zeros(n // 4, np.float32), s)
+    for _ in range(10):
+        capi.check(capi.lib().dy_memcpy_d2d_async(capi.vp(b.ptr), capi.vp(a.ptr), n, s.h), "dy_memcpy_d2d_async")
+    s.wait()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 10}), flush=True)
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known
+    size), from profiles/traffic_<kernel>.json when that measurement exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 def replay(w, init, s, steps, want):
@@ -107,6 +135,9 @@ def cpu_baseline(sc, snaps, starts, window):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs the frame step on one GPU only; --gpus {args.gpus} "
               f"is not supported (a multi-GPU run would shard the scene into Morton-range slabs "
@@ -207,7 +238,7 @@ def main():
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     out["roofline"] = {
         "bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic_from_profiles(name),
         "avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": round(nbytes),
         "other": {k: {"avg_launch_ms": round(v[0], 4), "algorithmic_bytes_per_launch": round(v[1])}
                   for k, v in kernels.items() if k != name},
