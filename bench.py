@@ -227,8 +227,8 @@ def main():
     # contact + 16 B per body (the per-stage figure for the colouring).
     n_solves = max(timing["solves"], 1)
     assign_bytes = (12.0 * float(np.sum(contacts)) + 16.0 * n * len(contacts)) / len(contacts)
-    sweep_kernel = {"0": "sweep_xcd", "1": "sweep_persistent"}.get(os.environ.get("DYPHUR_SWEEP_MODE", "2"),
-                                                                   "sweep_flow")
+    sweep_kernel = {"0": "sweep_xcd", "1": "sweep_persistent", "3": "sweep_flow_lanes"}.get(
+        os.environ.get("DYPHUR_SWEEP_MODE", "2"), "sweep_flow")
     kernels = {
         sweep_kernel: (timing["sweep_ms"] / n_solves, timing["sweep_bytes"] / n_solves),
         "assign_lists": (timing["assign_ms"] / n_solves, assign_bytes),

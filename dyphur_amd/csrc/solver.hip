@@ -868,6 +868,85 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
     }
 }
 
+// Dataflow sweep with a record cursor per lane: a lane moves on to its next
+// record (stride = participating threads, iterations in order) as soon as its
+// current one is done, so the lanes of a wave do not wait for each other.
+// The earliest unfinished (iteration, record) is always at its lane's cursor
+// and ready, so the sweep always progresses.  Waves sleep only when none of
+// their lanes advanced; a wave with no progress for kGridSpinLimit polls
+// aborts the sweep (kErrFlow).
+__global__ __launch_bounds__(1024) void sweep_flow_lanes(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
+                                                         float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                         const float4 *__restrict__ sta, uint32_t *ver, float mu,
+                                                         float dt, uint32_t *meta) {
+    uint32_t *err = meta + kMErr;
+    uint32_t b = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
+    if (meta[kMErr] & kErrAssign) return;
+    const uint32_t nc = meta[kMNc];
+    const uint32_t gstride = nblk * blockDim.x, first = b * blockDim.x + threadIdx.x;
+    const uint32_t lane = lane_id();
+    uint32_t it = 0, i = first;
+    if (i >= nc) it = n_iters;  // nothing for this lane
+    bool loaded = false;
+    uint2 ab = make_uint2(0u, 0u);
+    uint4 sq = make_uint4(0u, 0u, 0u, 0u);
+    float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+    float l = 0.f;
+    uint32_t ta = 0, tb = 0, idle = 0;
+    while (__ballot(it < n_iters)) {
+        bool done_now = false;
+        if (it < n_iters) {
+            if (!loaded) {
+                ab = rec.ab[i];
+                sq = rseq[i];
+                p0 = rec.p0[i];
+                p1 = rec.p1[i];
+                l = lam[i];
+                ta = it * sq.y + sq.x;
+                tb = it * sq.w + sq.z;
+                loaded = true;
+            }
+            const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
+                               (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
+            if (ready) {
+                const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+                uint32_t ra, rb;
+                BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
+                const bool da = A.im != 0.f, db = B.im != 0.f;
+                if (da) catch_up(A, ra, it);
+                if (db) catch_up(B, rb, it);
+                project_contact(ci, A, B, mu, l, dt);
+                lam[i] = l;
+                if (da) store_aos(dyn, ab.x, A, it);
+                if (db) store_aos(dyn, ab.y, B, it);
+                done_now = true;
+            }
+        }
+        wait_vmem();  // body stores acknowledged by the L2 before the versions move
+        if (done_now) {
+            if (sq.y) __hip_atomic_store(&ver[ab.x], ta + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (sq.w) __hip_atomic_store(&ver[ab.y], tb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            loaded = false;
+            i += gstride;
+            if (i >= nc) {
+                i = first;
+                ++it;
+            }
+        }
+        if (__ballot(done_now)) {
+            idle = 0;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+            if (++idle > kGridSpinLimit) {
+                if (lane == 0) atomicOr(err, kErrFlow);
+                return;
+            }
+            if ((idle & 255u) == 0u && (l2_read(err) & kErrFlow)) return;
+        }
+    }
+}
+
 // Back to the SoA view, bringing every body to n_iters renormalisations.
 __global__ void scatter_bodies(dy_body_view bv, const float4 *__restrict__ dyn, uint32_t n_iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1006,7 +1085,7 @@ int query_coop(dy_xpbd_s *x) {
     x->path = env_int("DYPHUR_SOLVER_PATH", 0);
     {
         const int m = env_int("DYPHUR_SWEEP_MODE", 2);
-        x->sweep_mode = (m == 0 || m == 1) ? m : 2;
+        x->sweep_mode = (m == 0 || m == 1 || m == 3) ? m : 2;
     }
     const int at = env_int("DYPHUR_ASSIGN_THREADS", 256), sw = env_int("DYPHUR_SWEEP_THREADS", 256);
     x->assign_threads = (at == 64 || at == 128 || at == 256 || at == 512 || at == 1024) ? at : 256;
@@ -1227,7 +1306,7 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     class_gather<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id,
                                                            x->cpos, x->rseq);
     DY_LAUNCH_CHECK();
-    const bool flow = x->path == 0 && x->sweep_mode == 2 && x->coop_flow_max > 0;
+    const bool flow = x->path == 0 && (x->sweep_mode == 2 || x->sweep_mode == 3) && x->coop_flow_max > 0;
     if (flow) {
         flow_seq<<<grid_for(2u * cap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->state, x->cpos,
                                                                 x->meta, x->rseq);
@@ -1248,7 +1327,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
             uint32_t *verv = x->ver.p, *metav = x->meta;
             float muv = x->mu, dtv = dt;
             void *args[] = {&iv, &rec, &rseqv, &lamv, &dynv, &stav, &verv, &muv, &dtv, &metav};
-            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_flow), dim3(blocks),
+            const void *fn = x->sweep_mode == 3 ? reinterpret_cast<const void *>(sweep_flow_lanes)
+                                                : reinterpret_cast<const void *>(sweep_flow);
+            DY_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks),
                                               dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
         } else if (coop) {
             // sweep_xcd: P participants on one XCD out of a launch of P per XCD

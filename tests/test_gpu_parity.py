@@ -161,7 +161,7 @@ def test_golden_box_hash_on_gpu(gpu):
     assert hs[0] == hs[1] == 0xCBCD209C3665C818
 
 
-@pytest.mark.parametrize("sweep_mode", ["2", "0"])
+@pytest.mark.parametrize("sweep_mode", ["2", "3", "0"])
 def test_stress_frames_match_oracle(gpu, oracle, monkeypatch, sweep_mode):
     monkeypatch.setenv("DYPHUR_SWEEP_MODE", sweep_mode)
     print(f"[stress 150 frames] sweep mode {sweep_mode}", flush=True)
@@ -216,6 +216,8 @@ LAUNCH_SHAPES = [
     ("sweep_phases_whole_grid", {"DYPHUR_SWEEP_MODE": "1"}),
     ("sweep_phases_one_xcd", {"DYPHUR_SWEEP_MODE": "0"}),
     ("sweep_phases_one_block", {"DYPHUR_SWEEP_MODE": "0", "DYPHUR_SWEEP_BLOCKS": "1"}),
+    ("sweep_dataflow_lane_cursors", {"DYPHUR_SWEEP_MODE": "3"}),
+    ("sweep_dataflow_lane_cursors_one_block", {"DYPHUR_SWEEP_MODE": "3", "DYPHUR_SWEEP_BLOCKS": "1"}),
 ]
 
 
