@@ -74,6 +74,16 @@ class SolverTiming(C.Structure):
                 ("sweep_bytes", C.c_double)]
 
 
+class BatchSceneStats(C.Structure):
+    _fields_ = [(k, u32) for k in ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds",
+                                   "overflow", "error", "substeps")]
+
+
+class BatchLayout(C.Structure):
+    _fields_ = [("pairs", vp), ("contacts", ContactView), ("body_offset", C.POINTER(u32)),
+                ("pair_offset", C.POINTER(u32)), ("contact_offset", C.POINTER(u32)), ("n_scenes", u32)]
+
+
 class DyError(RuntimeError):
     def __init__(self, status: int, what: str):
         self.status = status
@@ -136,6 +146,21 @@ _SIGS = {
     "dy_xpbd_set_timing": (C.c_int, [vp, C.c_int]),
     "dy_xpbd_get_timing": (C.c_int, [vp, vp, C.POINTER(SolverTiming)]),
     "dy_state_hash_fnv1a64": (C.c_int, [vp, C.POINTER(BodyView), C.POINTER(C.c_uint64)]),
+    "dy_graph_capture_begin": (C.c_int, [vp]),
+    "dy_graph_capture_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "dy_graph_launch": (C.c_int, [vp, vp]),
+    "dy_graph_destroy": (C.c_int, [vp]),
+    "dy_batch_create": (C.c_int, [vp, u32, vp, vp, vp, C.c_int, f32, C.POINTER(vp)]),
+    "dy_batch_destroy": (C.c_int, [vp]),
+    "dy_batch_set_force_serial": (C.c_int, [vp, C.c_int]),
+    "dy_batch_set_block_threads": (C.c_int, [vp, u32]),
+    "dy_batch_get_block_threads": (C.c_int, [vp, C.POINTER(u32)]),
+    "dy_batch_step": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
+                                C.c_int, C.c_int]),
+    "dy_batch_autotune": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
+                                    C.c_int, C.c_int, C.POINTER(u32), C.POINTER(f32)]),
+    "dy_batch_stats": (C.c_int, [vp, vp, C.POINTER(BatchSceneStats)]),
+    "dy_batch_layout_get": (C.c_int, [vp, C.POINTER(BatchLayout)]),
 }
 
 # Symbols declared in include/dyphur_hip.h (checked by the CPU test suite).
@@ -581,3 +606,147 @@ class World:
 
     def fnv(self) -> int:
         return self.bodies.fnv(self.s)
+
+
+class Graph:
+    """Calls issued on stream s inside ``with Graph.capture(s) as g:`` are
+    recorded into a HIP graph; ``g.launch(s)`` replays them."""
+
+    def __init__(self):
+        self.h = vp()
+
+    @classmethod
+    def capture(cls, s: Stream) -> "_Capture":
+        return _Capture(cls(), s)
+
+    def launch(self, s: Stream):
+        check(lib().dy_graph_launch(self.h, s.h), "dy_graph_launch")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().dy_graph_destroy(self.h)
+                self.h = vp()
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, g: Graph, s: Stream):
+        self.g, self.s = g, s
+
+    def __enter__(self) -> Graph:
+        check(lib().dy_graph_capture_begin(self.s.h), "dy_graph_capture_begin")
+        return self.g
+
+    def __exit__(self, et, ev, tb):
+        st = lib().dy_graph_capture_end(self.s.h, C.byref(self.g.h))
+        if et is None:
+            check(st, "dy_graph_capture_end")
+        return False
+
+
+BATCH_STAT_FIELDS = ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds", "overflow", "error",
+                     "substeps")
+
+
+class Batch:
+    """Many independent scenes stepped together (dy_batch_*): one launch runs
+    every scene's frames, one workgroup per scene.  Body indices in pairs and
+    contacts are scene-local, as each scene's own World would report them."""
+
+    def __init__(self, scene_list, s: Stream | None = None, n_iters: int | None = None,
+                 friction: float | None = None):
+        from . import scenes as _scenes
+        self.s = s or Stream()
+        self.scenes = list(scene_list)
+        self.merged = _scenes.concat(self.scenes)
+        m = self.merged.meta
+        self.body_offset = np.ascontiguousarray(m["body_offset"], np.uint32)
+        self.max_pairs = np.ascontiguousarray(m["max_pairs"], np.uint32)
+        self.max_contacts = np.ascontiguousarray(m["max_contacts"], np.uint32)
+        first = self.scenes[0]
+        self.n_iters = first.n_iters if n_iters is None else n_iters
+        self.mu = first.mu if friction is None else friction
+        self.bodies = DeviceBodies(self.merged, self.s)
+        self.shapes = DeviceShapes(self.merged, self.s)
+        self.h = vp()
+        check(lib().dy_batch_create(self.s.h, len(self.scenes), self.body_offset.ctypes.data,
+                                    self.max_pairs.ctypes.data, self.max_contacts.ctypes.data,
+                                    int(self.n_iters), float(self.mu), C.byref(self.h)), "dy_batch_create")
+        lay = BatchLayout()
+        check(lib().dy_batch_layout_get(self.h, C.byref(lay)), "dy_batch_layout_get")
+        self.layout = lay
+        ns = len(self.scenes)
+        self.pair_offset = np.ctypeslib.as_array(lay.pair_offset, (ns + 1,)).copy()
+        self.contact_offset = np.ctypeslib.as_array(lay.contact_offset, (ns + 1,)).copy()
+
+    @property
+    def n_scenes(self) -> int:
+        return len(self.scenes)
+
+    @property
+    def n_bodies(self) -> int:
+        return int(self.body_offset[-1])
+
+    def set_force_serial(self, v: bool):
+        check(lib().dy_batch_set_force_serial(self.h, 1 if v else 0), "dy_batch_set_force_serial")
+
+    def set_block_threads(self, threads: int):
+        check(lib().dy_batch_set_block_threads(self.h, threads), "dy_batch_set_block_threads")
+
+    def block_threads(self) -> int:
+        out = u32(0)
+        check(lib().dy_batch_get_block_threads(self.h, C.byref(out)), "dy_batch_get_block_threads")
+        return out.value
+
+    def step(self, n_frames: int = 1, dt: float | None = None, n_substeps: int | None = None, gravity=None):
+        sc = self.scenes[0]
+        g = sc.gravity if gravity is None else gravity
+        bv, sv = self.bodies.view(), self.shapes.view()
+        check(lib().dy_batch_step(self.h, self.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2],
+                                  sc.dt if dt is None else dt, sc.n_substeps if n_substeps is None else n_substeps,
+                                  n_frames), "dy_batch_step")
+
+    def autotune(self, trial_frames: int = 5) -> tuple[int, list[float]]:
+        sc = self.scenes[0]
+        g = sc.gravity
+        bv, sv = self.bodies.view(), self.shapes.view()
+        chosen = u32(0)
+        ms = (f32 * 3)()
+        check(lib().dy_batch_autotune(self.h, self.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt,
+                                      sc.n_substeps, trial_frames, C.byref(chosen), ms), "dy_batch_autotune")
+        return chosen.value, [ms[0], ms[1], ms[2]]
+
+    def stats(self, raise_on_error: bool = True) -> dict:
+        """Per-scene statistics; raises DyError when a scene's device error
+        flag is set (hull/mesh shape met, internal check) unless told not to."""
+        arr = (BatchSceneStats * self.n_scenes)()
+        st = lib().dy_batch_stats(self.h, self.s.h, arr)
+        if raise_on_error:
+            check(st, "dy_batch_stats")
+        raw = np.ctypeslib.as_array(C.cast(arr, C.POINTER(u32)), (self.n_scenes, 8)).copy()
+        return {k: raw[:, i] for i, k in enumerate(BATCH_STAT_FIELDS)}
+
+    def pairs_all(self) -> np.ndarray:
+        n = int(self.pair_offset[-1])
+        return download_ptr(self.layout.pairs, 2 * n, np.uint32, self.s).reshape(n, 2)
+
+    def contacts_all(self) -> dict:
+        n = int(self.contact_offset[-1])
+        cv = self.layout.contacts
+        return {k: download_ptr(getattr(cv, k), n, dt, self.s) for k, dt in CONTACT_FIELDS}
+
+    def state(self) -> dict:
+        return self.bodies.download(self.s)
+
+    def scene_slice(self, i: int) -> slice:
+        return slice(int(self.body_offset[i]), int(self.body_offset[i + 1]))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().dy_batch_destroy(self.h)
+                self.h = vp()
+        except Exception:
+            pass

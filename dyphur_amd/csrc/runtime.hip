@@ -185,6 +185,46 @@ int dy_event_elapsed_ms(dy_event_t a, dy_event_t b, float *ms) {
     return DY_OK;
 }
 
+struct dy_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int dy_graph_capture_begin(dy_stream_t s) {
+    if (!s) return DY_ERR_INVALID_ARG;
+    DY_TRY(hipStreamBeginCapture(hs(s), hipStreamCaptureModeThreadLocal));
+    return DY_OK;
+}
+
+int dy_graph_capture_end(dy_stream_t s, dy_graph_t *out) {
+    if (!s || !out) return DY_ERR_INVALID_ARG;
+    *out = nullptr;
+    auto *g = new dy_graph_s();
+    hipError_t e = hipStreamEndCapture(hs(s), &g->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return static_cast<int>(e);
+    }
+    *out = g;
+    return DY_OK;
+}
+
+int dy_graph_launch(dy_graph_t g, dy_stream_t s) {
+    if (!g || !g->exec) return DY_ERR_INVALID_ARG;
+    DY_TRY(hipGraphLaunch(g->exec, hs(s)));
+    return DY_OK;
+}
+
+int dy_graph_destroy(dy_graph_t g) {
+    if (!g) return DY_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return DY_OK;
+}
+
 int dy_body_dynamics_host(uint32_t n, const float *mass, const float *I, const uint32_t *flags_in, float *inv_mass,
                           float *iI, uint32_t *flags_out) {
     if (n && (!mass || !I || !flags_in || !inv_mass || !iI || !flags_out)) return DY_ERR_INVALID_ARG;

@@ -378,6 +378,31 @@ def batched_piles(n_scenes: int = 4096, seed: int = 20260003, median: float = 64
     return [pile_scene(int(c), rng) for c in counts]
 
 
+def concat(scenes: list[Scene], name: str = "batch") -> Scene:
+    """Scenes laid end to end for the batched step: body arrays concatenated,
+    each body's shape index offset into the concatenated catalog.  meta holds
+    body_offset (n_scenes + 1) and the per-scene pair / contact capacities."""
+    if not scenes:
+        raise ValueError("no scenes")
+    body_off = np.zeros(len(scenes) + 1, np.uint32)
+    body_off[1:] = np.cumsum([sc.n for sc in scenes])
+    shape_off = np.zeros(len(scenes) + 1, np.int64)
+    shape_off[1:] = np.cumsum([sc.n_shapes for sc in scenes])
+    cat = lambda key: np.concatenate([getattr(sc, key) for sc in scenes])
+    shape = np.concatenate([sc.shape.astype(np.int64) + shape_off[i] for i, sc in enumerate(scenes)])
+    first = scenes[0]
+    return Scene(name=name, shape_type=cat("shape_type"), shape_half=cat("shape_half"),
+                 pos=cat("pos"), rot=cat("rot"), vel=cat("vel"), ang=cat("ang"), mass=cat("mass"),
+                 inertia=cat("inertia"), shape=shape.astype(np.uint32), flags=cat("flags"),
+                 bounds=first.bounds.copy(), max_pairs=int(sum(sc.max_pairs for sc in scenes)),
+                 max_contacts=int(sum(sc.max_contacts for sc in scenes)), dt=first.dt,
+                 n_substeps=first.n_substeps, n_iters=first.n_iters, mu=first.mu, gravity=first.gravity,
+                 frames=first.frames,
+                 meta=dict(body_offset=body_off,
+                           max_pairs=np.array([sc.max_pairs for sc in scenes], np.uint32),
+                           max_contacts=np.array([sc.max_contacts for sc in scenes], np.uint32)))
+
+
 def two_body(shape_a: int, shape_b: int, half_a, half_b, pos_a, pos_b, mass=(1.0, 1.0),
              flags=(0, 0), rot_a=(1, 0, 0, 0), rot_b=(1, 0, 0, 0), vel_a=(0, 0, 0),
              vel_b=(0, 0, 0), inertia=(1.0, 1.0)) -> Scene:

@@ -14,9 +14,9 @@
  *
  * Blocking calls (host waits for the stream): dy_memcpy_d2h, dy_stream_synchronize,
  * dy_event_synchronize, dy_reduce_sum_f32, *_download_count, dy_narrowphase_last_count,
- * dy_state_hash_fnv1a64, dy_xpbd_solve (it reads the device contact count and the
- * colour count, as the reference solver does).  Everything else is asynchronous
- * on the given stream, in stream order.
+ * dy_state_hash_fnv1a64, dy_xpbd_last_stats, dy_xpbd_get_timing,
+ * dy_xpbd_download_classes, dy_batch_stats, dy_batch_autotune.  Everything else,
+ * dy_xpbd_solve included, is asynchronous on the given stream, in stream order.
  */
 #ifndef DYPHUR_HIP_H
 #define DYPHUR_HIP_H
@@ -127,6 +127,7 @@ typedef struct dy_event_s *dy_event_t;          /* a hipEvent_t */
 typedef struct dy_broadphase_s *dy_broadphase_t;
 typedef struct dy_narrowphase_s *dy_narrowphase_t;
 typedef struct dy_xpbd_s *dy_xpbd_t;
+typedef struct dy_graph_s *dy_graph_t;          /* an instantiated hipGraph */
 
 /* ---- runtime (compute/include/compute/{device,stream,buffer}.hpp) -------- */
 const char *dy_status_string(int status);
@@ -156,6 +157,14 @@ int dy_event_destroy(dy_event_t e);
 int dy_event_record(dy_event_t e, dy_stream_t s);
 int dy_event_synchronize(dy_event_t e);
 int dy_event_elapsed_ms(dy_event_t start, dy_event_t stop, float *ms);
+/* HIP graph capture of a stream (no reference counterpart: the SYCL queue
+ * records nothing).  Calls issued on s between begin and end are recorded,
+ * not run; end instantiates them.  Launching replays them in order on a
+ * stream; destroy frees the graph. */
+int dy_graph_capture_begin(dy_stream_t s);
+int dy_graph_capture_end(dy_stream_t s, dy_graph_t *out);
+int dy_graph_launch(dy_graph_t g, dy_stream_t s);
+int dy_graph_destroy(dy_graph_t g);
 
 /* ---- compute primitives ------------------------------------------------- */
 /* sort_by_key / sort (compute/include/compute/sort.hpp:29,89): ascending, stable.
@@ -253,6 +262,72 @@ int dy_xpbd_set_timing(dy_xpbd_t x, int enable);
 int dy_xpbd_get_timing(dy_xpbd_t x, dy_stream_t s, dy_solver_timing *out);
 /* Class (colour or level) of each contact of the last solve; blocking. */
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n);
+
+/* ---- batched scenes -------------------------------------------------------
+ * Many independent scenes stepped by one launch: the drop-in frame
+ * (examples/stress_test_blocks/main.cpp:264-273) of every scene, one workgroup
+ * per scene.  The reference has no batched form; each scene's results equal
+ * those of a stand-alone world over the same bodies (pairs in canonical
+ * order, contacts in pair order, reference-order or coloured solve by the
+ * reference's rules), with scene-local body indices.  Scene s owns bodies
+ * [body_offset[s], body_offset[s+1]) of the body view passed to each step;
+ * body shape indices address the shape view passed with it.  At most 2048
+ * bodies per scene. */
+typedef struct dy_batch_s *dy_batch_t;
+
+/* Per-scene statistics of the scene's last substep (substeps: running total). */
+typedef struct dy_batch_scene_stats {
+    uint32_t n_pairs;      /* true candidate-pair count (may exceed the scene's pair capacity) */
+    uint32_t n_contacts;   /* true contact count (may exceed the scene's contact capacity) */
+    uint32_t n_classes;    /* colours (coloured path) or dependency levels (reference order) */
+    uint32_t used_serial;  /* 1: reference contact order (<= 256 contacts, forced, or fallback) */
+    uint32_t rounds;       /* class-assignment rounds of the last solve */
+    uint32_t overflow;     /* 1: a body needed a 65th colour */
+    uint32_t error;        /* OR of: 1 hull/mesh shape met, 2 internal check failed */
+    uint32_t substeps;     /* substeps run since creation */
+} dy_batch_scene_stats;
+
+/* Device outputs of the last substep, concatenated over scenes. */
+typedef struct dy_batch_layout {
+    const dy_contact_pair *pairs;    /* scene s: pair_offset[s] .. + min(n_pairs, capacity) */
+    dy_contact_view contacts;        /* scene s: contact_offset[s] ..; contacts.n is NULL */
+    const uint32_t *body_offset;     /* host arrays of n_scenes + 1 entries */
+    const uint32_t *pair_offset;
+    const uint32_t *contact_offset;
+    uint32_t n_scenes;
+} dy_batch_layout;
+
+/* body_offset: host [n_scenes + 1], body_offset[0] = 0, strictly increasing;
+ * max_pairs / max_contacts: host [n_scenes], per-scene capacities as
+ * Broadphase(s, n, max_pairs) / Narrowphase(s, max_contacts) take them;
+ * n_iters / friction as XpbdSolver.  Blocking (uploads the layout). */
+int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offset, const uint32_t *max_pairs,
+                    const uint32_t *max_contacts, int n_iters, float friction, dy_batch_t *out);
+int dy_batch_destroy(dy_batch_t b);
+/* set_force_serial for every scene (xpbd_solver.hpp:48). */
+int dy_batch_set_force_serial(dy_batch_t b, int force_serial);
+/* Workgroup size of the step kernel: 64, 128 or 256 threads (default 64).
+ * Results do not depend on it. */
+int dy_batch_set_block_threads(dy_batch_t b, uint32_t threads);
+int dy_batch_get_block_threads(dy_batch_t b, uint32_t *threads);
+/* n_frames frames of every scene in ONE launch; each frame is n_substeps
+ * substeps of integrate(dt / n_substeps, 1) -> pairs -> narrowphase ->
+ * solve(dt / n_substeps).  Asynchronous; issues no memset or copy, so it can be
+ * captured in a HIP graph. */
+int dy_batch_step(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
+                  float gx, float gy, float gz, float dt, int n_substeps, int n_frames);
+/* Times trial_frames frames at each workgroup size from the current state,
+ * keeps the fastest, and restores the body state and statistics.  Blocking.
+ * chosen (may be NULL) receives the size; ms_out (may be NULL) three times in
+ * ms for 64, 128, 256 threads. */
+int dy_batch_autotune(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
+                      float gx, float gy, float gz, float dt, int n_substeps, int trial_frames,
+                      uint32_t *chosen, float *ms_out);
+/* Per-scene statistics, n_scenes entries; blocking.  Returns
+ * DY_ERR_UNSUPPORTED when a scene met a hull or mesh shape since creation and
+ * DY_ERR_INTERNAL on any other device error flag (out is filled either way). */
+int dy_batch_stats(dy_batch_t b, dy_stream_t s, dy_batch_scene_stats *out);
+int dy_batch_layout_get(dy_batch_t b, dy_batch_layout *out);
 
 /* FNV-1a-64 over pos xyz, rot wxyz of all bodies
  * (examples/stress_test_blocks/main.cpp:34-41,301-311); blocking. */
