@@ -153,14 +153,16 @@ _SIGS = {
     "dy_batch_create": (C.c_int, [vp, u32, vp, vp, vp, C.c_int, f32, C.POINTER(vp)]),
     "dy_batch_destroy": (C.c_int, [vp]),
     "dy_batch_set_force_serial": (C.c_int, [vp, C.c_int]),
-    "dy_batch_set_block_threads": (C.c_int, [vp, u32]),
-    "dy_batch_get_block_threads": (C.c_int, [vp, C.POINTER(u32)]),
+    "dy_batch_set_shape": (C.c_int, [vp, u32, u32]),
+    "dy_batch_get_shape": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32)]),
     "dy_batch_step": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
                                 C.c_int, C.c_int]),
     "dy_batch_autotune": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
-                                    C.c_int, C.c_int, C.POINTER(u32), C.POINTER(f32)]),
+                                    C.c_int, C.c_int, C.POINTER(u32), C.POINTER(u32), C.POINTER(f32)]),
     "dy_batch_stats": (C.c_int, [vp, vp, C.POINTER(BatchSceneStats)]),
     "dy_batch_layout_get": (C.c_int, [vp, C.POINTER(BatchLayout)]),
+    "dy_batch_set_timing": (C.c_int, [vp, vp, C.c_int]),
+    "dy_batch_get_timing": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double)]),
 }
 
 # Symbols declared in include/dyphur_hip.h (checked by the CPU test suite).
@@ -692,13 +694,13 @@ class Batch:
     def set_force_serial(self, v: bool):
         check(lib().dy_batch_set_force_serial(self.h, 1 if v else 0), "dy_batch_set_force_serial")
 
-    def set_block_threads(self, threads: int):
-        check(lib().dy_batch_set_block_threads(self.h, threads), "dy_batch_set_block_threads")
+    def set_shape(self, threads: int, wave_scene_max: int = 128):
+        check(lib().dy_batch_set_shape(self.h, threads, wave_scene_max), "dy_batch_set_shape")
 
-    def block_threads(self) -> int:
-        out = u32(0)
-        check(lib().dy_batch_get_block_threads(self.h, C.byref(out)), "dy_batch_get_block_threads")
-        return out.value
+    def shape(self) -> tuple[int, int]:
+        t, w = u32(0), u32(0)
+        check(lib().dy_batch_get_shape(self.h, C.byref(t), C.byref(w)), "dy_batch_get_shape")
+        return t.value, w.value
 
     def step(self, n_frames: int = 1, dt: float | None = None, n_substeps: int | None = None, gravity=None):
         sc = self.scenes[0]
@@ -708,15 +710,20 @@ class Batch:
                                   sc.dt if dt is None else dt, sc.n_substeps if n_substeps is None else n_substeps,
                                   n_frames), "dy_batch_step")
 
-    def autotune(self, trial_frames: int = 5) -> tuple[int, list[float]]:
+    TUNE_SHAPES = ((64, 0), (128, 64), (128, 128), (256, 64), (256, 128), (256, 256))
+
+    def autotune(self, trial_frames: int = 5) -> tuple[tuple[int, int], list[float]]:
+        """Times the six launch shapes from the current state (which it
+        restores) and keeps the fastest; returns it and the six times in ms."""
         sc = self.scenes[0]
         g = sc.gravity
         bv, sv = self.bodies.view(), self.shapes.view()
-        chosen = u32(0)
-        ms = (f32 * 3)()
+        t, w = u32(0), u32(0)
+        ms = (f32 * 6)()
         check(lib().dy_batch_autotune(self.h, self.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt,
-                                      sc.n_substeps, trial_frames, C.byref(chosen), ms), "dy_batch_autotune")
-        return chosen.value, [ms[0], ms[1], ms[2]]
+                                      sc.n_substeps, trial_frames, C.byref(t), C.byref(w), ms),
+              "dy_batch_autotune")
+        return (t.value, w.value), list(ms)
 
     def stats(self, raise_on_error: bool = True) -> dict:
         """Per-scene statistics; raises DyError when a scene's device error
@@ -727,6 +734,19 @@ class Batch:
             check(st, "dy_batch_stats")
         raw = np.ctypeslib.as_array(C.cast(arr, C.POINTER(u32)), (self.n_scenes, 8)).copy()
         return {k: raw[:, i] for i, k in enumerate(BATCH_STAT_FIELDS)}
+
+    BATCH_STAGES = ("integrate_boxes", "pairs", "narrowphase", "assign", "sweep")
+
+    def set_timing(self, on: bool):
+        check(lib().dy_batch_set_timing(self.h, self.s.h, 1 if on else 0), "dy_batch_set_timing")
+
+    def timing(self) -> dict:
+        """Seconds per stage per scene accumulated while timing was on."""
+        ticks = np.zeros((self.n_scenes, 5), np.uint64)
+        hz = C.c_double(0)
+        check(lib().dy_batch_get_timing(self.h, self.s.h, ticks.ctypes.data, C.byref(hz)), "dy_batch_get_timing")
+        sec = ticks.astype(np.float64) / hz.value
+        return {k: sec[:, i] for i, k in enumerate(self.BATCH_STAGES)}
 
     def pairs_all(self) -> np.ndarray:
         n = int(self.pair_offset[-1])

@@ -33,7 +33,9 @@
 //      per-body masks), then I iterations sweep the classes in order, each
 //      followed by the renormalisation of every quaternion.
 #include <algorithm>
+#include <map>
 #include <numeric>
+#include <utility>
 #include <vector>
 
 #include "dy_internal.hpp"
@@ -41,11 +43,17 @@
 
 namespace dy {
 
-constexpr uint32_t kBatchMaxBodies = 2048;  // LDS boxes: 24 B per body
+constexpr uint32_t kBatchMaxBodies = 2048;  // LDS box records: 32 B per body
+constexpr uint32_t kLdsMaxFloats = 16384;   // 64 KiB of dynamic LDS per workgroup
+constexpr uint32_t kTuneShapes = 6;         // launch shapes the autotuner times
 constexpr uint32_t kBatchSerialMax = 256;   // xpbd_solver.hpp:29 kSerialThreshold
 constexpr uint32_t kBatchMaxRounds = 256;   // xpbd_solver.cpp:640
 constexpr uint32_t kNotAssigned = 0xffffffffu;
 constexpr uint32_t kBatchErrUnsupported = 1u, kBatchErrLists = 2u;
+
+// Timed stages: integrate + leaf boxes, pairs, narrowphase, class
+// assignment, sweep.
+constexpr uint32_t kStages = 5;
 
 enum : uint32_t {
     kStPairs = 0, kStContacts, kStClasses, kStSerial, kStRounds, kStOverflow, kStError, kStSubsteps,
@@ -57,30 +65,39 @@ struct BatchArgs {
     dy_shape_view sv;      // shape catalog; bv.shape holds catalog indices
     dy_contact_view cv;    // contact store, concatenated (cv.n unused)
     dy_contact_pair *pairs;
-    const uint32_t *order; // launch slot -> scene
+    const uint4 *units;    // per workgroup: up to four scenes (kNoScene = empty slot)
+    const uint32_t *unit_kind;  // bit 0: whole-workgroup team; bit 1 + slot: slot's state in LDS
     const uint32_t *body_off, *pair_off, *contact_off;
     const uint64_t *scratch_off;  // 32-bit words, even
     uint32_t *scratch;
     uint32_t *stats;       // kStWords per scene
-    uint32_t max_n;
+    uint64_t *stage_clk;   // kStages per scene when timed
+    uint32_t region_floats;  // LDS floats per team
     float gx, gy, gz, h;
     int n_sub, n_frames, n_iters, force_serial;
     float mu;
 };
 
-// Per-scene scratch, carved from the scene's slice: 6n + 10C + 2 words.
+constexpr uint32_t kNoScene = 0xffffffffu;
+constexpr uint32_t kUnitBlock = 1u;
+// LDS floats per body: a 32 B box record, plus the 20 float fields of the
+// body when the scene's state is held in LDS for the whole launch.
+constexpr uint32_t kBoxFloats = 8, kStateFloats = 20;
+
+// Per-scene scratch, carved from the scene's slice.
 struct SceneScratch {
-    uint64_t *used;                 // [n] colour masks
+    uint64_t *used;                     // [n] colour masks
     uint32_t *row, *deg, *loff, *fill;  // [n]
-    float *lam;                     // [C]
-    int32_t *pred_a, *pred_b;       // [C]
-    uint32_t *rnd, *cls, *order;    // [C]
-    uint32_t *list;                 // [2C]
-    uint32_t *ccnt, *coff;          // [C + 1]
+    float *lam;                         // [C]
+    int32_t *pred_a, *pred_b;           // [C]
+    uint32_t *rnd, *cls, *order;        // [C]
+    uint32_t *list;                     // [2C]
+    uint32_t *ccnt, *coff;              // [C + 1]
+    uint32_t *bits;                     // [n * ceil(n / 32)] overlap rows
 };
 
 __host__ __device__ inline uint64_t scene_scratch_words(uint32_t n, uint32_t c) {
-    const uint64_t w = 6ull * n + 10ull * c + 2ull;
+    const uint64_t w = 6ull * n + 10ull * c + 2ull + static_cast<uint64_t>(n) * ((n + 31u) / 32u);
     return (w + 1ull) & ~1ull;
 }
 
@@ -99,7 +116,8 @@ __device__ __forceinline__ SceneScratch carve(uint32_t *p, uint32_t n, uint32_t 
     s.order = p; p += c;
     s.list = p; p += 2 * c;
     s.ccnt = p; p += c + 1;
-    s.coff = p;
+    s.coff = p; p += c + 1;
+    s.bits = p;
     return s;
 }
 
@@ -146,28 +164,6 @@ __device__ __forceinline__ void store_body(const dy_body_view &v, uint32_t i, co
     v.ang_x[i] = b.wx; v.ang_y[i] = b.wy; v.ang_z[i] = b.wz;
 }
 
-// Exclusive scan of data[0, n) in place by the whole workgroup; returns the
-// total.  Each entry is rewritten by the thread that read it: callers
-// __syncthreads() before other threads read the result.
-__device__ uint32_t scan_in_place(uint32_t *data, uint32_t n, uint32_t *wsum) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < n ? data[i] : 0u;
-        uint32_t tot = 0;
-        const uint32_t ex = block_exclusive_scan(v, wsum, tot);
-        if (i < n) data[i] = carry + ex;
-        carry += tot;
-    }
-    return carry;
-}
-
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *wsum) {
-    uint32_t tot = 0;
-    (void)block_exclusive_scan(v, wsum, tot);
-    return tot;
-}
-
 // Workgroup sum of v and OR of f in one exchange (two barriers).  `wsum`
 // needs 12 words; every thread gets both totals.
 __device__ __forceinline__ uint32_t block_sum_or(uint32_t v, uint32_t f, uint32_t *wsum, uint32_t &f_all) {
@@ -192,13 +188,71 @@ __device__ __forceinline__ uint32_t block_sum_or(uint32_t v, uint32_t f, uint32_
     return sv;
 }
 
-// Reference overlap test on two boxes (broadphase.cpp:322-325): separated on
-// some axis, else touching or overlapping.
-struct LdsBoxes { float *mnx, *mny, *mnz, *mxx, *mxy, *mxz; };
+// A team steps one scene.  WaveTeam: the 64 lanes of one wavefront, whose
+// lanes see each other's memory operations in program order, so its barrier
+// is a wavefront-scope fence pair that orders the compiler and emits no
+// instruction.  BlockTeam: the whole workgroup, __syncthreads().  Each team
+// owns 32 LDS words: [0, 17) scan partials, [20] error bits, [21] class count.
+struct WaveTeam {
+    static constexpr uint32_t kSize = 64;
+    uint32_t rank;
+    uint32_t *sh;
+    __device__ __forceinline__ void sync() const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t &tot) const {
+        const uint32_t inc = wave_inclusive_scan(v);
+        tot = __shfl(inc, 63, 64);
+        return inc - v;
+    }
+    __device__ __forceinline__ uint32_t sum_or(uint32_t v, uint32_t f, uint32_t &f_all) const {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            v += __shfl_xor(v, d, 64);
+            f |= __shfl_xor(f, d, 64);
+        }
+        f_all = f;
+        return v;
+    }
+};
 
-__device__ __forceinline__ bool touch(const float a[6], const LdsBoxes &L, uint32_t b) {
-    return !(L.mnx[b] > a[3] || L.mxx[b] < a[0] || L.mny[b] > a[4] || L.mxy[b] < a[1] ||
-             L.mnz[b] > a[5] || L.mxz[b] < a[2]);
+template <uint32_t NT>
+struct BlockTeam {
+    static constexpr uint32_t kSize = NT;
+    uint32_t rank;
+    uint32_t *sh;
+    __device__ __forceinline__ void sync() const { __syncthreads(); }
+    __device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t &tot) const {
+        return block_exclusive_scan(v, sh, tot);
+    }
+    __device__ __forceinline__ uint32_t sum_or(uint32_t v, uint32_t f, uint32_t &f_all) const {
+        return block_sum_or(v, f, sh, f_all);
+    }
+};
+
+// Exclusive scan of data[0, n) in place by the team; returns the total.  Each
+// entry is rewritten by the thread that read it: callers sync before other
+// threads read the result.
+template <class Team>
+__device__ uint32_t team_scan(const Team &tm, uint32_t *data, uint32_t n) {
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += Team::kSize) {
+        const uint32_t i = base + tm.rank;
+        const uint32_t v = i < n ? data[i] : 0u;
+        uint32_t tot = 0;
+        const uint32_t ex = tm.excl_scan(v, tot);
+        if (i < n) data[i] = carry + ex;
+        carry += tot;
+    }
+    return carry;
+}
+
+// Reference overlap test (broadphase.cpp:322-325): separated on some axis,
+// else touching or overlapping.  Box records: mn xyz, mx xyz, 2 unused.
+__device__ __forceinline__ bool touch(const float q[6], const float4 lo, const float2 hi) {
+    return !(lo.x > q[3] || lo.w < q[0] || lo.y > q[4] || hi.x < q[1] || lo.z > q[5] || hi.y < q[2]);
 }
 
 // Sort one body's contact list and record each contact's predecessor on that
@@ -229,12 +283,13 @@ __device__ __forceinline__ void body_predecessors(uint32_t x, const SceneScratch
     }
 }
 
-// Class assignment in rounds.  Returns the rounds run; `remaining` holds
-// the contacts left unassigned (non-zero only for the coloured path at 256
-// rounds or after a colour overflow, `over`).  Every exit condition is a
-// workgroup-wide total, so all threads leave the loop together.
-__device__ uint32_t assign_rounds(uint32_t nc, bool jp, const SceneScratch &S, const dy_contact_view &cv,
-                                  const float *im, uint32_t *wsum, uint32_t &remaining, uint32_t &over,
+// Class assignment in rounds.  Returns the rounds run; `remaining` holds the
+// contacts left unassigned (non-zero only for the coloured path at 256 rounds
+// or after a colour overflow, `over`).  Every exit condition is a team-wide
+// total, so all threads leave the loop together.
+template <class Team>
+__device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const SceneScratch &S,
+                                  const dy_contact_view &cv, const float *im, uint32_t &remaining, uint32_t &over,
                                   uint32_t &err) {
     uint32_t r = 0;
     remaining = nc;
@@ -242,7 +297,7 @@ __device__ uint32_t assign_rounds(uint32_t nc, bool jp, const SceneScratch &S, c
     while (remaining > 0) {
         if (jp && r >= kBatchMaxRounds) break;
         uint32_t mine = 0, my_over = 0;
-        for (uint32_t k = threadIdx.x; k < nc; k += blockDim.x) {
+        for (uint32_t k = tm.rank; k < nc; k += Team::kSize) {
             if (S.rnd[k] != kNotAssigned) continue;
             const int32_t pa = S.pred_a[k], pb = S.pred_b[k];
             // A predecessor assigned in this round reads as r or as unassigned.
@@ -270,7 +325,7 @@ __device__ uint32_t assign_rounds(uint32_t nc, bool jp, const SceneScratch &S, c
             ++mine;
         }
         uint32_t any_over = 0;
-        const uint32_t got = block_sum_or(mine, my_over, wsum, any_over);
+        const uint32_t got = tm.sum_or(mine, my_over, any_over);
         remaining -= got;
         ++r;
         if (jp && any_over) {
@@ -285,30 +340,55 @@ __device__ uint32_t assign_rounds(uint32_t nc, bool jp, const SceneScratch &S, c
     return r;
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void batch_frames(BatchArgs A) {
-    extern __shared__ float lds_box[];
-    __shared__ uint32_t wsum[17];
-    __shared__ uint32_t s_flags[4];  // [1] error bits, [2] class count
-    const uint32_t t = threadIdx.x;
-    const uint32_t s = A.order[blockIdx.x];
+// One scene for the frames of the launch.  RES: the scene's 20 body fields
+// live in the team's LDS region from the first substep to the last and are
+// written back at the end; otherwise the stages read and write the global
+// arrays.  Per substep: integrate, leaf boxes, pairs, narrowphase, solve.
+template <class Team, bool RES, bool TIMED>
+__device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float *region) {
+    constexpr uint32_t NT = Team::kSize;
+    const uint32_t r = tm.rank;
+    uint32_t *tf = tm.sh;
     const uint32_t b0 = A.body_off[s], n = A.body_off[s + 1] - b0;
     const uint32_t P = A.pair_off[s + 1] - A.pair_off[s];
     const uint32_t C = A.contact_off[s + 1] - A.contact_off[s];
-    const dy_body_view bv = scene_bodies(A.bv, b0, n);
+    const dy_body_view gv = scene_bodies(A.bv, b0, n);
     const dy_contact_view cv = scene_contacts(A.cv, A.contact_off[s], C);
     dy_contact_pair *pairs = A.pairs + A.pair_off[s];
     const SceneScratch S = carve(A.scratch + A.scratch_off[s], n, C);
     uint32_t *st = A.stats + static_cast<size_t>(s) * kStWords;
-    const LdsBoxes L{lds_box, lds_box + A.max_n, lds_box + 2 * A.max_n,
-                     lds_box + 3 * A.max_n, lds_box + 4 * A.max_n, lds_box + 5 * A.max_n};
+    float *box = region;
+    dy_body_view bv = gv;
+    if constexpr (RES) {
+        float *p = region + kBoxFloats * n;
+#define DY_TO_LDS(f, k)                                              \
+    bv.f = p + (k) * n;                                              \
+    for (uint32_t i = r; i < n; i += NT) bv.f[i] = gv.f[i];
+        DY_TO_LDS(pos_x, 0) DY_TO_LDS(pos_y, 1) DY_TO_LDS(pos_z, 2)
+        DY_TO_LDS(rot_w, 3) DY_TO_LDS(rot_x, 4) DY_TO_LDS(rot_y, 5) DY_TO_LDS(rot_z, 6)
+        DY_TO_LDS(vel_x, 7) DY_TO_LDS(vel_y, 8) DY_TO_LDS(vel_z, 9)
+        DY_TO_LDS(ang_x, 10) DY_TO_LDS(ang_y, 11) DY_TO_LDS(ang_z, 12)
+        DY_TO_LDS(inv_mass, 13)
+        DY_TO_LDS(iI_xx, 14) DY_TO_LDS(iI_yy, 15) DY_TO_LDS(iI_zz, 16)
+        DY_TO_LDS(iI_xy, 17) DY_TO_LDS(iI_xz, 18) DY_TO_LDS(iI_yz, 19)
+#undef DY_TO_LDS
+        tm.sync();
+    }
     const float h = A.h;
+    const uint32_t W = (n + 31u) >> 5;
     const uint32_t n_steps = static_cast<uint32_t>(A.n_frames) * static_cast<uint32_t>(A.n_sub);
+    uint64_t clk[kStages + 1] = {};
+    auto mark = [&](int j) {
+        if constexpr (TIMED) {
+            if (r == 0) clk[j] = wall_clock64();
+        }
+    };
 
     for (uint32_t step = 0; step < n_steps; ++step) {
+        mark(0);
         // 1. integrate one substep (integrator.cpp:17-70, n_substeps = 1 at h).
-        for (uint32_t i = t; i < n; i += BS) {
-            if (bv.inv_mass[i] == 0.f || (bv.flags[i] & kFlagSleeping)) continue;
+        for (uint32_t i = r; i < n; i += NT) {
+            if (bv.inv_mass[i] == 0.f || (gv.flags[i] & kFlagSleeping)) continue;
             Kin k{bv.pos_x[i], bv.pos_y[i], bv.pos_z[i], bv.rot_w[i], bv.rot_x[i], bv.rot_y[i],
                   bv.rot_z[i], bv.vel_x[i], bv.vel_y[i], bv.vel_z[i], bv.ang_x[i], bv.ang_y[i], bv.ang_z[i]};
             integrate_body(k, A.gx, A.gy, A.gz, h, 1);
@@ -316,63 +396,95 @@ __global__ __launch_bounds__(BS) void batch_frames(BatchArgs A) {
             bv.rot_w[i] = k.qw; bv.rot_x[i] = k.qx; bv.rot_y[i] = k.qy; bv.rot_z[i] = k.qz;
             bv.vel_x[i] = k.vx; bv.vel_y[i] = k.vy; bv.vel_z[i] = k.vz;
         }
-        if (t < 4) s_flags[t] = 0;
-        __syncthreads();
-
         // 2. leaf boxes (broadphase.cpp:215-251), binary16-rounded.
-        for (uint32_t i = t; i < n; i += BS) {
-            const uint32_t sh = bv.shape[i];
+        for (uint32_t i = r; i < n; i += NT) {
+            const uint32_t sh = gv.shape[i];
             float ex, ey, ez;
             leaf_extent(A.sv.type[sh], A.sv.half_x[sh], A.sv.half_y[sh], A.sv.half_z[sh], bv.rot_w[i],
                         bv.rot_x[i], bv.rot_y[i], bv.rot_z[i], ex, ey, ez);
             const float px = bv.pos_x[i], py = bv.pos_y[i], pz = bv.pos_z[i];
-            L.mnx[i] = f16_to_f32(f32_to_f16(px - ex));
-            L.mny[i] = f16_to_f32(f32_to_f16(py - ey));
-            L.mnz[i] = f16_to_f32(f32_to_f16(pz - ez));
-            L.mxx[i] = f16_to_f32(f32_to_f16(px + ex));
-            L.mxy[i] = f16_to_f32(f32_to_f16(py + ey));
-            L.mxz[i] = f16_to_f32(f32_to_f16(pz + ez));
+            float4 lo, hi;
+            lo.x = f16_to_f32(f32_to_f16(px - ex));
+            lo.y = f16_to_f32(f32_to_f16(py - ey));
+            lo.z = f16_to_f32(f32_to_f16(pz - ez));
+            lo.w = f16_to_f32(f32_to_f16(px + ex));
+            hi.x = f16_to_f32(f32_to_f16(py + ey));
+            hi.y = f16_to_f32(f32_to_f16(pz + ez));
+            hi.z = 0.f;
+            hi.w = 0.f;
+            reinterpret_cast<float4 *>(box)[2 * i] = lo;
+            reinterpret_cast<float4 *>(box)[2 * i + 1] = hi;
         }
-        __syncthreads();
+        if (r < 32) tf[r] = 0;
+        tm.sync();
+        mark(1);
 
-        // 3. candidate pairs: row counts, scan, ordered emission.
-        for (uint32_t a = t; a < n; a += BS) {
-            const float q[6] = {L.mnx[a], L.mny[a], L.mnz[a], L.mxx[a], L.mxy[a], L.mxz[a]};
-            uint32_t c = 0;
-            for (uint32_t b = a + 1; b < n; ++b) c += touch(q, L, b) ? 1u : 0u;
-            S.row[a] = c;
+        // 3. candidate pairs.  Lanes hold rows a; every lane tests the same
+        //    partner b (an LDS broadcast), building 32-bit overlap words of
+        //    the row; then a scan of the row counts and an emit pass over the
+        //    words, in (a, b) order.
+        for (uint32_t base = 0; base < n; base += NT) {
+            const uint32_t a = base + r;
+            const bool have = a < n;
+            float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (have) {
+                const float4 lo = reinterpret_cast<const float4 *>(box)[2 * a];
+                const float4 hi = reinterpret_cast<const float4 *>(box)[2 * a + 1];
+                q[0] = lo.x; q[1] = lo.y; q[2] = lo.z; q[3] = lo.w; q[4] = hi.x; q[5] = hi.y;
+            }
+            uint32_t cnt = 0;
+            for (uint32_t w = (base + 1) >> 5; w < W; ++w) {
+                const uint32_t b_lo = w << 5;
+                const uint32_t m = n - b_lo < 32u ? n - b_lo : 32u;
+                uint32_t word = 0;
+                for (uint32_t j = 0; j < m; ++j) {
+                    const float4 lo = reinterpret_cast<const float4 *>(box)[2 * (b_lo + j)];
+                    const float2 hi = reinterpret_cast<const float2 *>(box)[4 * (b_lo + j) + 2];
+                    word |= (touch(q, lo, hi) ? 1u : 0u) << j;
+                }
+                // keep partners b > a
+                const int32_t lo_bit = static_cast<int32_t>(a) + 1 - static_cast<int32_t>(b_lo);
+                const uint32_t keep = lo_bit <= 0 ? ~0u : (lo_bit >= 32 ? 0u : (~0u << lo_bit));
+                word &= keep;
+                if (have) S.bits[static_cast<size_t>(a) * W + w] = word;
+                cnt += static_cast<uint32_t>(__popc(word));
+            }
+            if (have) S.row[a] = cnt;
         }
-        __syncthreads();
-        const uint32_t n_pairs = scan_in_place(S.row, n, wsum);
-        __syncthreads();
-        for (uint32_t a = t; a < n; a += BS) {
+        tm.sync();
+        const uint32_t n_pairs = team_scan(tm, S.row, n);
+        tm.sync();
+        for (uint32_t a = r; a < n; a += NT) {
             uint32_t pos = S.row[a];
-            if (pos >= P) continue;
-            const float q[6] = {L.mnx[a], L.mny[a], L.mnz[a], L.mxx[a], L.mxy[a], L.mxz[a]};
-            for (uint32_t b = a + 1; b < n && pos < P; ++b) {
-                if (touch(q, L, b)) {
-                    pairs[pos] = dy_contact_pair{a, b};
-                    ++pos;
+            for (uint32_t w = (a + 1) >> 5; w < W && pos < P; ++w) {
+                uint32_t word = S.bits[static_cast<size_t>(a) * W + w];
+                while (word && pos < P) {
+                    const uint32_t j = static_cast<uint32_t>(__builtin_ctz(word));
+                    word &= word - 1u;
+                    pairs[pos++] = dy_contact_pair{a, (w << 5) + j};
                 }
             }
         }
-        __syncthreads();
+        tm.sync();
+        mark(2);
 
         // 4. narrowphase in pair order.
         const uint32_t np_n = n_pairs < P ? n_pairs : P;
         uint32_t run = 0;
-        for (uint32_t base = 0; base < np_n; base += BS) {
-            const uint32_t i = base + t;
+        for (uint32_t base = 0; base < np_n; base += NT) {
+            const uint32_t i = base + r;
             uint32_t hit = 0, unsup = 0;
             Hit hc;
             dy_contact_pair p{0u, 0u};
             if (i < np_n) {
                 p = pairs[i];
-                hit = pair_contact(bv, A.sv, p.a, p.b, hc, unsup) ? 1u : 0u;
+                dy_body_view lv = bv;
+                lv.shape = gv.shape;
+                hit = pair_contact(lv, A.sv, p.a, p.b, hc, unsup) ? 1u : 0u;
             }
-            if (unsup) atomicOr(&s_flags[1], kBatchErrUnsupported);
+            if (unsup) atomicOr(&tf[20], kBatchErrUnsupported);
             uint32_t tot = 0;
-            const uint32_t ex = block_exclusive_scan(hit, wsum, tot);
+            const uint32_t ex = tm.excl_scan(hit, tot);
             const uint32_t slot = run + ex;
             if (hit && slot < C) {
                 cv.body_a[slot] = p.a; cv.body_b[slot] = p.b;
@@ -383,80 +495,84 @@ __global__ __launch_bounds__(BS) void batch_frames(BatchArgs A) {
             run += tot;
         }
         const uint32_t nc = run < C ? run : C;
+        mark(3);
+        mark(4);
+        mark(5);
 
         // 5. solve (xpbd_solver.cpp:409-733), contacts only.
         uint32_t serial = 1, rounds = 0, n_cls = 0, overflow = 0, err = 0;
         if (nc > 0) {
-            for (uint32_t x = t; x < n; x += BS) { S.deg[x] = 0; S.fill[x] = 0; S.used[x] = 0; }
-            for (uint32_t k = t; k < nc; k += BS) {
+            for (uint32_t x = r; x < n; x += NT) { S.deg[x] = 0; S.fill[x] = 0; S.used[x] = 0; }
+            for (uint32_t k = r; k < nc; k += NT) {
                 S.lam[k] = 0.f;
                 S.rnd[k] = kNotAssigned;
                 S.pred_a[k] = -1;
                 S.pred_b[k] = -1;
                 S.cls[k] = 0;
             }
-            __syncthreads();
-            for (uint32_t k = t; k < nc; k += BS) {
+            tm.sync();
+            for (uint32_t k = r; k < nc; k += NT) {
                 const uint32_t a = cv.body_a[k], b = cv.body_b[k];
                 if (bv.inv_mass[a] != 0.f) atomicAdd(&S.deg[a], 1u);
                 if (bv.inv_mass[b] != 0.f) atomicAdd(&S.deg[b], 1u);
             }
-            __syncthreads();
-            for (uint32_t x = t; x < n; x += BS) S.loff[x] = S.deg[x];
-            __syncthreads();
-            (void)scan_in_place(S.loff, n, wsum);
-            __syncthreads();
-            for (uint32_t k = t; k < nc; k += BS) {
+            tm.sync();
+            for (uint32_t x = r; x < n; x += NT) S.loff[x] = S.deg[x];
+            tm.sync();
+            (void)team_scan(tm, S.loff, n);
+            tm.sync();
+            for (uint32_t k = r; k < nc; k += NT) {
                 const uint32_t a = cv.body_a[k], b = cv.body_b[k];
                 if (bv.inv_mass[a] != 0.f) S.list[S.loff[a] + atomicAdd(&S.fill[a], 1u)] = k;
                 if (bv.inv_mass[b] != 0.f) S.list[S.loff[b] + atomicAdd(&S.fill[b], 1u)] = k;
             }
-            __syncthreads();
+            tm.sync();
             const bool jp = !(A.force_serial || nc <= kBatchSerialMax);
-            for (uint32_t x = t; x < n; x += BS) body_predecessors(x, S, cv.body_a, jp);
-            __syncthreads();
+            for (uint32_t x = r; x < n; x += NT) body_predecessors(x, S, cv.body_a, jp);
+            tm.sync();
             uint32_t remaining = 0, over = 0;
-            rounds = assign_rounds(nc, jp, S, cv, bv.inv_mass, wsum, remaining, over, err);
+            rounds = assign_rounds(tm, nc, jp, S, cv, bv.inv_mass, remaining, over, err);
             overflow = over;
             serial = jp ? 0u : 1u;
             if (jp && (remaining > 0 || over)) {
                 // xpbd_solver.cpp:696-699: colouring failed -> reference order.
                 serial = 1;
-                for (uint32_t k = t; k < nc; k += BS) S.rnd[k] = kNotAssigned;
-                __syncthreads();
-                for (uint32_t x = t; x < n; x += BS) body_predecessors(x, S, cv.body_a, false);
-                __syncthreads();
-                (void)assign_rounds(nc, false, S, cv, bv.inv_mass, wsum, remaining, over, err);
+                for (uint32_t k = r; k < nc; k += NT) S.rnd[k] = kNotAssigned;
+                tm.sync();
+                for (uint32_t x = r; x < n; x += NT) body_predecessors(x, S, cv.body_a, false);
+                tm.sync();
+                (void)assign_rounds(tm, nc, false, S, cv, bv.inv_mass, remaining, over, err);
             }
             if (remaining > 0) err |= kBatchErrLists;
             if (err == 0) {
-                for (uint32_t k = t; k < nc; k += BS) atomicMax(&s_flags[2], S.cls[k] + 1u);
+                for (uint32_t k = r; k < nc; k += NT) atomicMax(&tf[21], S.cls[k] + 1u);
             }
-            __syncthreads();
-            n_cls = s_flags[2];
+            tm.sync();
+            n_cls = tf[21];
             if (err == 0) {
                 // classes -> contiguous ranges of `order`
-                for (uint32_t c = t; c < n_cls; c += BS) S.ccnt[c] = 0;
-                __syncthreads();
-                for (uint32_t k = t; k < nc; k += BS) atomicAdd(&S.ccnt[S.cls[k]], 1u);
-                __syncthreads();
-                for (uint32_t c = t; c < n_cls; c += BS) S.coff[c] = S.ccnt[c];
-                __syncthreads();
-                (void)scan_in_place(S.coff, n_cls, wsum);
-                if (t == 0) S.coff[n_cls] = nc;
-                for (uint32_t c = t; c < n_cls; c += BS) S.ccnt[c] = 0;
-                __syncthreads();
-                for (uint32_t k = t; k < nc; k += BS) {
+                for (uint32_t c = r; c < n_cls; c += NT) S.ccnt[c] = 0;
+                tm.sync();
+                for (uint32_t k = r; k < nc; k += NT) atomicAdd(&S.ccnt[S.cls[k]], 1u);
+                tm.sync();
+                for (uint32_t c = r; c < n_cls; c += NT) S.coff[c] = S.ccnt[c];
+                tm.sync();
+                (void)team_scan(tm, S.coff, n_cls);
+                if (r == 0) S.coff[n_cls] = nc;
+                for (uint32_t c = r; c < n_cls; c += NT) S.ccnt[c] = 0;
+                tm.sync();
+                for (uint32_t k = r; k < nc; k += NT) {
                     const uint32_t c = S.cls[k];
                     S.order[S.coff[c] + atomicAdd(&S.ccnt[c], 1u)] = k;
                 }
-                __syncthreads();
+                tm.sync();
+                mark(4);
                 // I iterations: classes in order, then renormalise every body
                 // (xpbd_solver.cpp:578-586,700-733).
                 for (int it = 0; it < A.n_iters; ++it) {
                     for (uint32_t c = 0; c < n_cls; ++c) {
                         const uint32_t beg = S.coff[c], end = S.coff[c + 1];
-                        for (uint32_t j = beg + t; j < end; j += BS) {
+                        for (uint32_t j = beg + r; j < end; j += NT) {
                             const uint32_t k = S.order[j];
                             const uint32_t a = cv.body_a[k], b = cv.body_b[k];
                             const ContactIn ci{cv.pos_x[k], cv.pos_y[k], cv.pos_z[k], cv.norm_x[k],
@@ -469,37 +585,86 @@ __global__ __launch_bounds__(BS) void batch_frames(BatchArgs A) {
                                 if (Bb.im != 0.f) store_body(bv, b, Bb);
                             }
                         }
-                        __syncthreads();
+                        tm.sync();
                     }
-                    for (uint32_t i = t; i < n; i += BS) {
+                    for (uint32_t i = r; i < n; i += NT) {
                         float qw = bv.rot_w[i], qx = bv.rot_x[i], qy = bv.rot_y[i], qz = bv.rot_z[i];
                         renormalise(qw, qx, qy, qz);
                         bv.rot_w[i] = qw; bv.rot_x[i] = qx; bv.rot_y[i] = qy; bv.rot_z[i] = qz;
                     }
-                    __syncthreads();
+                    tm.sync();
                 }
             }
         }
-        if (err) atomicOr(&s_flags[1], err);
-        __syncthreads();
-        if (t == 0) {
+        if (err) atomicOr(&tf[20], err);
+        tm.sync();
+        mark(5);
+        if (r == 0) {
+            if constexpr (TIMED) {
+                for (uint32_t j = 0; j < kStages; ++j) {
+                    const uint64_t x0 = clk[j], x1 = clk[j + 1];
+                    A.stage_clk[static_cast<size_t>(s) * kStages + j] += x1 > x0 ? x1 - x0 : 0ull;
+                }
+            }
             st[kStPairs] = n_pairs;
             st[kStContacts] = run;
             st[kStClasses] = n_cls;
             st[kStSerial] = nc > 0 ? serial : 0u;
             st[kStRounds] = rounds;
             st[kStOverflow] = overflow;
-            st[kStError] |= s_flags[1];
+            st[kStError] |= tf[20];
             st[kStSubsteps] += 1u;
         }
-        __syncthreads();
+        tm.sync();
+    }
+    if constexpr (RES) {
+        // write back the mutable fields: pos, rot, vel, ang
+        for (uint32_t i = r; i < n; i += NT) {
+            gv.pos_x[i] = bv.pos_x[i]; gv.pos_y[i] = bv.pos_y[i]; gv.pos_z[i] = bv.pos_z[i];
+            gv.rot_w[i] = bv.rot_w[i]; gv.rot_x[i] = bv.rot_x[i]; gv.rot_y[i] = bv.rot_y[i];
+            gv.rot_z[i] = bv.rot_z[i];
+            gv.vel_x[i] = bv.vel_x[i]; gv.vel_y[i] = bv.vel_y[i]; gv.vel_z[i] = bv.vel_z[i];
+            gv.ang_x[i] = bv.ang_x[i]; gv.ang_y[i] = bv.ang_y[i]; gv.ang_z[i] = bv.ang_z[i];
+        }
     }
 }
 
-template <int BS>
-int launch_frames(const BatchArgs &a, uint32_t n_scenes, hipStream_t s) {
-    const size_t lds = 6u * a.max_n * sizeof(float);
-    hipLaunchKernelGGL(batch_frames<BS>, dim3(n_scenes), dim3(BS), lds, s, a);
+// One workgroup per unit: either one scene stepped by the whole workgroup,
+// or one scene per wavefront (each wavefront its own team and LDS region).
+// RES: every scene of the launch keeps its state in LDS (the usual case);
+// otherwise the launch holds the scenes too large for that, one per
+// workgroup (or per 64-thread workgroup).
+template <uint32_t NT, bool TIMED, bool RES>
+__global__ __launch_bounds__(NT) void batch_frames(BatchArgs A) {
+    extern __shared__ float lds[];
+    __shared__ uint32_t team_sh[NT / 64][32];
+    const uint4 u = A.units[blockIdx.x];
+    const uint32_t kind = A.unit_kind[blockIdx.x];
+    if constexpr (NT > 64) {
+        if (kind & kUnitBlock) {
+            const BlockTeam<NT> tm{threadIdx.x, team_sh[0]};
+            run_scene<BlockTeam<NT>, RES, TIMED>(tm, A, u.x, lds);
+            return;
+        }
+    }
+    if constexpr (RES || NT == 64) {
+        // Wave-uniform by construction; readfirstlane lets the compiler keep
+        // the scene's pointers in scalar registers.
+        const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(w == 0 ? u.x : w == 1 ? u.y : w == 2 ? u.z : u.w);
+        if (sc == kNoScene) return;  // wavefront teams never wait on each other
+        const WaveTeam tm{lane_id(), team_sh[w]};
+        run_scene<WaveTeam, RES, TIMED>(tm, A, sc, lds + static_cast<size_t>(w) * A.region_floats);
+    }
+}
+
+template <uint32_t NT, bool RES>
+int launch_frames(const BatchArgs &a, uint32_t n_units, size_t lds_bytes, hipStream_t s) {
+    if (n_units == 0) return DY_OK;
+    if (a.stage_clk)
+        hipLaunchKernelGGL((batch_frames<NT, true, RES>), dim3(n_units), dim3(NT), lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL((batch_frames<NT, false, RES>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     DY_LAUNCH_CHECK();
     return DY_OK;
 }
@@ -509,12 +674,27 @@ int launch_frames(const BatchArgs &a, uint32_t n_scenes, hipStream_t s) {
 using namespace dy;
 
 struct dy_batch_s {
-    uint32_t n_scenes = 0, max_n = 0, block = 64;
+    // Launch shape: workgroup threads, and the largest scene (bodies) stepped
+    // by a single wavefront; larger scenes get a whole workgroup.
+    struct Units {
+        uint4 *d_units = nullptr;
+        uint32_t *d_kind = nullptr;
+        uint32_t n_units = 0, region_floats = 0;
+        size_t lds_bytes = 0;
+    };
+    struct Shape {
+        Units res;   // scenes whose state fits in LDS
+        Units glob;  // the others
+    };
+    uint32_t n_scenes = 0, max_n = 0, block = 256, wave_max = 128;
     int n_iters = 10;
     float mu = 0.5f;
     int force_serial = 0;
+    int timing = 0;
+    uint64_t *d_clk = nullptr;  // kStages per scene
     std::vector<uint32_t> body_off, pair_off, contact_off;
-    uint32_t *d_order = nullptr, *d_body_off = nullptr, *d_pair_off = nullptr, *d_contact_off = nullptr;
+    std::map<std::pair<uint32_t, uint32_t>, Shape> shapes;
+    uint32_t *d_body_off = nullptr, *d_pair_off = nullptr, *d_contact_off = nullptr;
     uint64_t *d_scratch_off = nullptr;
     uint32_t *d_scratch = nullptr, *d_stats = nullptr;
     dy_contact_pair *d_pairs = nullptr;
@@ -525,10 +705,15 @@ struct dy_batch_s {
 namespace {
 
 void batch_free(dy_batch_s *b) {
-    void *ps[] = {b->d_order, b->d_body_off, b->d_pair_off, b->d_contact_off, b->d_scratch_off,
-                  b->d_scratch, b->d_stats, b->d_pairs, b->d_contacts};
+    void *ps[] = {b->d_body_off, b->d_pair_off, b->d_contact_off, b->d_scratch_off,
+                  b->d_scratch, b->d_stats, b->d_pairs, b->d_contacts, b->d_clk};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    for (auto &kv : b->shapes)
+        for (auto *u : {&kv.second.res, &kv.second.glob}) {
+            if (u->d_units) (void)hipFree(u->d_units);
+            if (u->d_kind) (void)hipFree(u->d_kind);
+        }
 }
 
 template <typename T>
@@ -542,21 +727,89 @@ int upload(hipStream_t s, void *dst, const void *src, size_t bytes) {
     return DY_OK;
 }
 
+int upload_units(dy_batch_s::Units &u, const std::vector<uint4> &units, const std::vector<uint32_t> &kind,
+                 uint32_t region, uint32_t lds_floats) {
+    u.n_units = static_cast<uint32_t>(units.size());
+    u.region_floats = region;
+    u.lds_bytes = static_cast<size_t>(lds_floats) * sizeof(float);
+    if (units.empty()) return DY_OK;
+    DY_TRY_STATUS(dev_alloc(u.d_units, units.size()));
+    DY_TRY_STATUS(dev_alloc(u.d_kind, kind.size()));
+    DY_TRY(hipMemcpy(u.d_units, units.data(), units.size() * sizeof(uint4), hipMemcpyHostToDevice));
+    DY_TRY(hipMemcpy(u.d_kind, kind.data(), kind.size() * 4, hipMemcpyHostToDevice));
+    return DY_OK;
+}
+
+// Work units of a launch shape.  Scenes of at most wave_max bodies are packed
+// one per wavefront, threads/64 to a workgroup, in descending size (at 64
+// threads every scene is a wavefront team); larger scenes get a workgroup
+// each, largest first.  A wavefront team's LDS region holds its scene's box
+// records and state; a workgroup team's scene keeps its state in LDS where
+// the 64 KiB allow and goes to the second launch (state in global memory)
+// where they do not.
+int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
+    const auto key = std::make_pair(threads, wave_max);
+    if (b->shapes.count(key)) return DY_OK;
+    const uint32_t per = threads / 64;
+    constexpr uint32_t kRes = kStateFloats + kBoxFloats;
+    std::vector<uint32_t> order(b->n_scenes);
+    std::iota(order.begin(), order.end(), 0u);
+    auto nb = [&](uint32_t i) { return b->body_off[i + 1] - b->body_off[i]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return nb(x) > nb(y); });
+    const uint32_t wave_cap = kLdsMaxFloats / per / kRes;  // bodies a wavefront region can hold
+    std::vector<uint32_t> big, small, huge;
+    for (uint32_t i : order) {
+        const bool wave = threads == 64 ? nb(i) <= wave_cap : nb(i) <= std::min(wave_max, wave_cap);
+        if (wave) small.push_back(i);
+        else if (kRes * nb(i) <= kLdsMaxFloats && threads > 64) big.push_back(i);
+        else huge.push_back(i);
+    }
+    const uint32_t region = small.empty() ? 0u : kRes * nb(small.front());
+    uint32_t lds = per * region;
+    if (!big.empty()) lds = std::max(lds, kRes * nb(big.front()));
+    std::vector<uint4> units;
+    std::vector<uint32_t> kind;
+    for (uint32_t i : big) {
+        units.push_back(make_uint4(i, kNoScene, kNoScene, kNoScene));
+        kind.push_back(kUnitBlock);
+    }
+    for (size_t j = 0; j < small.size(); j += per) {
+        uint32_t sc[4] = {kNoScene, kNoScene, kNoScene, kNoScene};
+        for (uint32_t w = 0; w < per && j + w < small.size(); ++w) sc[w] = small[j + w];
+        units.push_back(make_uint4(sc[0], sc[1], sc[2], sc[3]));
+        kind.push_back(0u);
+    }
+    std::vector<uint4> gunits;
+    std::vector<uint32_t> gkind;
+    for (uint32_t i : huge) {
+        gunits.push_back(make_uint4(i, kNoScene, kNoScene, kNoScene));
+        gkind.push_back(threads > 64 ? kUnitBlock : 0u);
+    }
+    const uint32_t glds = huge.empty() ? 0u : kBoxFloats * nb(huge.front());
+    dy_batch_s::Shape sh;
+    int st = upload_units(sh.res, units, kind, region, lds);
+    if (st == DY_OK) st = upload_units(sh.glob, gunits, gkind, glds, glds);
+    b->shapes[key] = sh;  // freed with the batch, also after a failure
+    return st;
+}
+
 int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
                  float gy, float gz, float dt, int n_substeps, int n_frames) {
+    const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
+    if (it == b->shapes.end()) return DY_ERR_INTERNAL;
+    const dy_batch_s::Shape &sh = it->second;
     BatchArgs a{};
     a.bv = bodies;
     a.sv = shapes;
     a.cv = b->cv;
     a.pairs = b->d_pairs;
-    a.order = b->d_order;
     a.body_off = b->d_body_off;
     a.pair_off = b->d_pair_off;
     a.contact_off = b->d_contact_off;
     a.scratch_off = b->d_scratch_off;
     a.scratch = b->d_scratch;
     a.stats = b->d_stats;
-    a.max_n = b->max_n;
+    a.stage_clk = b->timing ? b->d_clk : nullptr;
     a.gx = gx; a.gy = gy; a.gz = gz;
     a.h = dt / static_cast<float>(n_substeps);
     a.n_sub = n_substeps;
@@ -564,12 +817,25 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
     a.n_iters = b->n_iters;
     a.force_serial = b->force_serial;
     a.mu = b->mu;
-    switch (b->block) {
-        case 64: return launch_frames<64>(a, b->n_scenes, s);
-        case 128: return launch_frames<128>(a, b->n_scenes, s);
-        case 256: return launch_frames<256>(a, b->n_scenes, s);
-        default: return DY_ERR_INVALID_ARG;
+    for (int pass = 0; pass < 2; ++pass) {
+        const dy_batch_s::Units &u = pass == 0 ? sh.res : sh.glob;
+        if (u.n_units == 0) continue;
+        a.units = u.d_units;
+        a.unit_kind = u.d_kind;
+        a.region_floats = u.region_floats;
+        int st = DY_ERR_INVALID_ARG;
+        switch (b->block) {
+            case 64: st = pass ? launch_frames<64, false>(a, u.n_units, u.lds_bytes, s)
+                               : launch_frames<64, true>(a, u.n_units, u.lds_bytes, s); break;
+            case 128: st = pass ? launch_frames<128, false>(a, u.n_units, u.lds_bytes, s)
+                                : launch_frames<128, true>(a, u.n_units, u.lds_bytes, s); break;
+            case 256: st = pass ? launch_frames<256, false>(a, u.n_units, u.lds_bytes, s)
+                                : launch_frames<256, true>(a, u.n_units, u.lds_bytes, s); break;
+            default: break;
+        }
+        DY_TRY_STATUS(st);
     }
+    return DY_OK;
 }
 
 int check_step_args(const dy_batch_s *b, const dy_body_view *bodies, const dy_shape_view *shapes, int n_substeps,
@@ -617,12 +883,6 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
     }
     b->pair_off[n_scenes] = static_cast<uint32_t>(pt);
     b->contact_off[n_scenes] = static_cast<uint32_t>(ct);
-    // Largest scenes first, so the longest workgroups start earliest.
-    std::vector<uint32_t> order(n_scenes);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return body_offset[x + 1] - body_offset[x] > body_offset[y + 1] - body_offset[y];
-    });
     const hipStream_t hsm = hs(s);
     const size_t C = ct;
     auto fail = [&](int code) {
@@ -635,7 +895,6 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
         const int e_ = (expr);        \
         if (e_ != DY_OK) return fail(e_); \
     } while (0)
-    DY_BATCH_TRY(dev_alloc(b->d_order, n_scenes));
     DY_BATCH_TRY(dev_alloc(b->d_body_off, n_scenes + 1));
     DY_BATCH_TRY(dev_alloc(b->d_pair_off, n_scenes + 1));
     DY_BATCH_TRY(dev_alloc(b->d_contact_off, n_scenes + 1));
@@ -643,6 +902,7 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
     DY_BATCH_TRY(dev_alloc(b->d_scratch, scratch_off[n_scenes]));
     DY_BATCH_TRY(dev_alloc(b->d_stats, static_cast<size_t>(n_scenes) * kStWords));
     DY_BATCH_TRY(dev_alloc(b->d_pairs, pt));
+    DY_BATCH_TRY(dev_alloc(b->d_clk, static_cast<size_t>(n_scenes) * kStages));
     {
         char *p = nullptr;
         DY_BATCH_TRY(dev_alloc(p, 9 * (C ? C : 1) * 4));
@@ -660,7 +920,6 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
         b->cv.n = nullptr;
         b->cv.capacity = static_cast<uint32_t>(C);
     }
-    DY_BATCH_TRY(upload(hsm, b->d_order, order.data(), n_scenes * 4));
     DY_BATCH_TRY(upload(hsm, b->d_body_off, b->body_off.data(), (n_scenes + 1) * 4));
     DY_BATCH_TRY(upload(hsm, b->d_pair_off, b->pair_off.data(), (n_scenes + 1) * 4));
     DY_BATCH_TRY(upload(hsm, b->d_contact_off, b->contact_off.data(), (n_scenes + 1) * 4));
@@ -668,6 +927,7 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
     if (hipMemsetAsync(b->d_stats, 0, static_cast<size_t>(n_scenes) * kStWords * 4, hsm) != hipSuccess)
         return fail(DY_ERR_INTERNAL);
     if (hipStreamSynchronize(hsm) != hipSuccess) return fail(DY_ERR_INTERNAL);
+    DY_BATCH_TRY(prepare_shape(b, b->block, b->wave_max));
 #undef DY_BATCH_TRY
     *out = b;
     return DY_OK;
@@ -686,15 +946,19 @@ int dy_batch_set_force_serial(dy_batch_t b, int force_serial) {
     return DY_OK;
 }
 
-int dy_batch_set_block_threads(dy_batch_t b, uint32_t threads) {
+int dy_batch_set_shape(dy_batch_t b, uint32_t threads, uint32_t wave_scene_max) {
     if (!b || (threads != 64 && threads != 128 && threads != 256)) return DY_ERR_INVALID_ARG;
+    if (threads == 64) wave_scene_max = 0;
+    DY_TRY_STATUS(prepare_shape(b, threads, wave_scene_max));
     b->block = threads;
+    b->wave_max = wave_scene_max;
     return DY_OK;
 }
 
-int dy_batch_get_block_threads(dy_batch_t b, uint32_t *threads) {
-    if (!b || !threads) return DY_ERR_INVALID_ARG;
+int dy_batch_get_shape(dy_batch_t b, uint32_t *threads, uint32_t *wave_scene_max) {
+    if (!b || !threads || !wave_scene_max) return DY_ERR_INVALID_ARG;
     *threads = b->block;
+    *wave_scene_max = b->wave_max;
     return DY_OK;
 }
 
@@ -707,7 +971,7 @@ int dy_batch_step(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const
 
 int dy_batch_autotune(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
                       float gx, float gy, float gz, float dt, int n_substeps, int trial_frames,
-                      uint32_t *chosen, float *ms_out) {
+                      uint32_t *chosen_threads, uint32_t *chosen_wave_max, float *ms_out) {
     DY_TRY_STATUS(check_step_args(b, bodies, shapes, n_substeps, trial_frames));
     if (trial_frames <= 0) return DY_ERR_INVALID_ARG;
     const hipStream_t hsm = hs(s);
@@ -736,15 +1000,15 @@ int dy_batch_autotune(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, c
                               hipMemcpyDeviceToDevice, hsm));
         return DY_OK;
     };
-    const uint32_t cand[3] = {64, 128, 256};
+    const uint32_t cand[kTuneShapes][2] = {{64, 0}, {128, 64}, {128, 128}, {256, 64}, {256, 128}, {256, 256}};
     float best = 0.f;
-    uint32_t best_bs = b->block;
-    const uint32_t keep = b->block;
+    const uint32_t keep[2] = {b->block, b->wave_max};
+    uint32_t best_shape[2] = {keep[0], keep[1]};
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = DY_ERR_INTERNAL;
     if (st == DY_OK) st = copy_state(true);
-    for (int c = 0; c < 3 && st == DY_OK; ++c) {
-        b->block = cand[c];
-        st = copy_state(false);
+    for (uint32_t c = 0; c < kTuneShapes && st == DY_OK; ++c) {
+        st = dy_batch_set_shape(b, cand[c][0], cand[c][1]);
+        if (st == DY_OK) st = copy_state(false);
         if (st == DY_OK) st = batch_launch(b, hsm, *bodies, *shapes, gx, gy, gz, dt, n_substeps, 1);  // warm-up
         if (st == DY_OK) st = copy_state(false);
         if (st == DY_OK && hipEventRecord(e0, hsm) != hipSuccess) st = DY_ERR_INTERNAL;
@@ -756,17 +1020,20 @@ int dy_batch_autotune(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, c
         if (ms_out) ms_out[c] = ms;
         if (st == DY_OK && (c == 0 || ms < best)) {
             best = ms;
-            best_bs = cand[c];
+            best_shape[0] = cand[c][0];
+            best_shape[1] = cand[c][1];
         }
     }
-    b->block = st == DY_OK ? best_bs : keep;
+    if (st == DY_OK) st = dy_batch_set_shape(b, best_shape[0], best_shape[1]);
+    else (void)dy_batch_set_shape(b, keep[0], keep[1]);
     if (st == DY_OK) st = copy_state(false);
     if (hipStreamSynchronize(hsm) != hipSuccess && st == DY_OK) st = DY_ERR_INTERNAL;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(save);
     (void)hipFree(save_stats);
-    if (st == DY_OK && chosen) *chosen = b->block;
+    if (st == DY_OK && chosen_threads) *chosen_threads = b->block;
+    if (st == DY_OK && chosen_wave_max) *chosen_wave_max = b->wave_max;
     return st;
 }
 
@@ -780,6 +1047,26 @@ int dy_batch_stats(dy_batch_t b, dy_stream_t s, dy_batch_scene_stats *out) {
     for (uint32_t i = 0; i < b->n_scenes; ++i) any |= out[i].error;
     if (any & kBatchErrUnsupported) return DY_ERR_UNSUPPORTED;
     if (any) return DY_ERR_INTERNAL;
+    return DY_OK;
+}
+
+int dy_batch_set_timing(dy_batch_t b, dy_stream_t s, int enable) {
+    if (!b) return DY_ERR_INVALID_ARG;
+    b->timing = enable ? 1 : 0;
+    if (enable)
+        DY_TRY(hipMemsetAsync(b->d_clk, 0, static_cast<size_t>(b->n_scenes) * kStages * 8, hs(s)));
+    return DY_OK;
+}
+
+int dy_batch_get_timing(dy_batch_t b, dy_stream_t s, uint64_t *ticks, double *tick_hz) {
+    if (!b || !ticks || !tick_hz) return DY_ERR_INVALID_ARG;
+    DY_TRY(hipMemcpyAsync(ticks, b->d_clk, static_cast<size_t>(b->n_scenes) * kStages * 8, hipMemcpyDeviceToHost,
+                          hs(s)));
+    DY_TRY(hipStreamSynchronize(hs(s)));
+    int dev = 0, khz = 0;
+    DY_TRY(hipGetDevice(&dev));
+    DY_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    *tick_hz = 1e3 * static_cast<double>(khz);
     return DY_OK;
 }
 

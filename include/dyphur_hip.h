@@ -306,28 +306,37 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
 int dy_batch_destroy(dy_batch_t b);
 /* set_force_serial for every scene (xpbd_solver.hpp:48). */
 int dy_batch_set_force_serial(dy_batch_t b, int force_serial);
-/* Workgroup size of the step kernel: 64, 128 or 256 threads (default 64).
- * Results do not depend on it. */
-int dy_batch_set_block_threads(dy_batch_t b, uint32_t threads);
-int dy_batch_get_block_threads(dy_batch_t b, uint32_t *threads);
+/* Launch shape of the step kernel: workgroup threads (64, 128 or 256) and the
+ * largest scene (bodies, ground included) stepped by one wavefront; larger
+ * scenes get a whole workgroup, smaller ones are packed threads/64 to a
+ * workgroup.  Default 256 / 128.  Results do not depend on the shape. */
+int dy_batch_set_shape(dy_batch_t b, uint32_t threads, uint32_t wave_scene_max);
+int dy_batch_get_shape(dy_batch_t b, uint32_t *threads, uint32_t *wave_scene_max);
 /* n_frames frames of every scene in ONE launch; each frame is n_substeps
  * substeps of integrate(dt / n_substeps, 1) -> pairs -> narrowphase ->
  * solve(dt / n_substeps).  Asynchronous; issues no memset or copy, so it can be
  * captured in a HIP graph. */
 int dy_batch_step(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
                   float gx, float gy, float gz, float dt, int n_substeps, int n_frames);
-/* Times trial_frames frames at each workgroup size from the current state,
- * keeps the fastest, and restores the body state and statistics.  Blocking.
- * chosen (may be NULL) receives the size; ms_out (may be NULL) three times in
- * ms for 64, 128, 256 threads. */
+/* Times trial_frames frames at each of six launch shapes (threads /
+ * wave_scene_max: 64/-, 128/64, 128/128, 256/64, 256/128, 256/256) from the
+ * current state, keeps the fastest, and restores the body state and
+ * statistics.  Blocking.  chosen_* (may be NULL) receive the shape; ms_out
+ * (may be NULL) the six times in ms. */
 int dy_batch_autotune(dy_batch_t b, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
                       float gx, float gy, float gz, float dt, int n_substeps, int trial_frames,
-                      uint32_t *chosen, float *ms_out);
+                      uint32_t *chosen_threads, uint32_t *chosen_wave_max, float *ms_out);
 /* Per-scene statistics, n_scenes entries; blocking.  Returns
  * DY_ERR_UNSUPPORTED when a scene met a hull or mesh shape since creation and
  * DY_ERR_INTERNAL on any other device error flag (out is filled either way). */
 int dy_batch_stats(dy_batch_t b, dy_stream_t s, dy_batch_scene_stats *out);
 int dy_batch_layout_get(dy_batch_t b, dy_batch_layout *out);
+/* Per-stage timing of the step kernel (one clock read per stage boundary by
+ * each workgroup).  Enabling clears the totals (asynchronous on s); get
+ * copies 5 counters per scene (integrate + leaf boxes, pairs, narrowphase,
+ * class assignment, sweep) in ticks of tick_hz, blocking. */
+int dy_batch_set_timing(dy_batch_t b, dy_stream_t s, int enable);
+int dy_batch_get_timing(dy_batch_t b, dy_stream_t s, uint64_t *ticks, double *tick_hz);
 
 /* FNV-1a-64 over pos xyz, rot wxyz of all bodies
  * (examples/stress_test_blocks/main.cpp:34-41,301-311); blocking. */

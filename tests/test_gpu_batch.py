@@ -86,12 +86,12 @@ def _run_oracle_frame(ref, scs, force_serial=False):
     return out
 
 
-@pytest.mark.parametrize("threads", [64, 256])
-def test_batch_matches_oracle_every_frame(gpu, oracle, threads):
+@pytest.mark.parametrize("shape", [(64, 0), (256, 128), (128, 64)])
+def test_batch_matches_oracle_every_frame(gpu, oracle, shape):
     scs = _mixed_batch()
-    print(f"[batch] {len(scs)} scenes threads={threads} frames=40", flush=True)
+    print(f"[batch] {len(scs)} scenes shape={shape} frames=40", flush=True)
     batch = gpu.Batch(scs)
-    batch.set_block_threads(threads)
+    batch.set_shape(*shape)
     ref = _oracle_worlds(oracle, scs)
     saw_jp = saw_serial = False
     for f in range(40):
@@ -129,17 +129,17 @@ def test_batch_one_launch_many_frames_equals_frame_by_frame(gpu):
         assert np.array_equal(ta[k], tb[k]), k
 
 
-def test_batch_workgroup_sizes_agree_and_autotune_restores(gpu):
+def test_batch_launch_shapes_agree_and_autotune_restores(gpu):
     scs = _mixed_batch()
     ref = gpu.Batch(scs)
-    ref.set_block_threads(128)
+    ref.set_shape(128, 128)
     ref.step(10)
     want = ref.state()
     tuned = gpu.Batch(scs)
     before = tuned.state()
     chosen, ms = tuned.autotune(3)
-    assert chosen in (64, 128, 256) and all(m > 0 for m in ms)
-    assert tuned.block_threads() == chosen
+    assert chosen in gpu.Batch.TUNE_SHAPES and all(m > 0 for m in ms)
+    assert tuned.shape() == chosen
     after = tuned.state()
     for k in STATE:
         assert np.array_equal(bits(before[k]), bits(after[k])), k
@@ -148,6 +148,27 @@ def test_batch_workgroup_sizes_agree_and_autotune_restores(gpu):
     got = tuned.state()
     for k in STATE:
         assert np.array_equal(bits(got[k]), bits(want[k])), k
+    for shp in gpu.Batch.TUNE_SHAPES:
+        other = gpu.Batch(scs)
+        other.set_shape(*shp)
+        other.step(10)
+        st = other.state()
+        for k in STATE:
+            assert np.array_equal(bits(st[k]), bits(want[k])), (shp, k)
+
+
+def test_batch_stage_timing_leaves_results_unchanged(gpu):
+    scs = _mixed_batch()
+    plain, timed = gpu.Batch(scs), gpu.Batch(scs)
+    timed.set_timing(True)
+    plain.step(6)
+    timed.step(6)
+    sp, st = plain.state(), timed.state()
+    for k in STATE:
+        assert np.array_equal(bits(sp[k]), bits(st[k])), k
+    tm = timed.timing()
+    assert all(v.shape == (len(scs),) for v in tm.values())
+    assert (tm["pairs"] > 0).all() and tm["sweep"].sum() > 0
 
 
 def test_batch_graph_replay_equals_eager(gpu):
