@@ -16,6 +16,10 @@ value = body-substeps/s = bodies (ground included) x frames / seconds over the
 timed frames, with all inputs resident in HBM (no host<->device state copies
 in the timed region).
 
+The line also carries "batched": the 4096-scene pile workload (seed 20260003)
+stepped 300 frames in one launch, its body-substeps/s and the plain per-scene
+loop of the drop-in API on a sample of the same scenes.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -44,6 +48,10 @@ def parse():
     ap.add_argument("--cpu-window", type=int, default=20,
                     help="frames per CPU-baseline window (four windows spread over the run)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-frames", type=int, default=300,
+                    help="frames of the 4096-scene batched workload (0 skips it)")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="scenes of the batched workload stepped through the plain per-scene loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -130,6 +138,107 @@ def cpu_baseline(sc, snaps, starts, window):
                   f"threads ({cpu_model}); oracle state hash equal to the GPU at every window end: {equal}",
         "seconds": round(total, 2),
         "fnv_equal": equal,
+    }
+
+
+def frame_bytes(n_dyn, n_static, frames, body_solves, pairs, contacts, dyn_endpoints, iters):
+    """Compulsory HBM bytes of reference frames, summed over the frames of a
+    run (per stage: integrate 100 B per dynamic body + 4 per static; Morton
+    keys, key sort, Karras 24 B per body each; leaf boxes 52; refit 56;
+    traversal 52 per body + 8 per pair; narrowphase 76 per pair + 36 per
+    contact; contact scan 8 per pair; colouring 12 per contact + 16 per body;
+    solver, per iteration, 44 per contact + 132 per dynamic contact endpoint,
+    and 32 per body renormalised).  body_solves: bodies x solves that ran;
+    pairs, contacts, dyn_endpoints: totals over the frames."""
+    n = n_dyn + n_static
+    per_frame_n = 100.0 * n_dyn + 4.0 * n_static + (24 + 24 + 24 + 52 + 56 + 52 + 16) * n
+    return (per_frame_n * frames + 32.0 * iters * body_solves + 8.0 * pairs + 76.0 * pairs + 36.0 * contacts
+            + 8.0 * pairs + 12.0 * contacts + iters * (44.0 * contacts + 132.0 * dyn_endpoints))
+
+
+def batched(capi, scenes, frames: int, plain_sample: int) -> dict:
+    """4096 independent pile scenes stepped together: one launch for all
+    frames (launch shape picked by the autotuner on a 10-frame trial from the
+    initial state, which it restores), next to the plain per-scene loop of the
+    drop-in API on a seeded sample of the same scenes."""
+    t0 = time.perf_counter()
+    scs = scenes.batched_piles()
+    b = capi.Batch(scs)
+    b.s.wait()
+    shape, tune_ms = b.autotune(10)
+    e0, e1 = capi.Event(), capi.Event()
+    b.s.wait()
+    w0 = time.perf_counter()
+    e0.record(b.s)
+    b.step(frames)
+    e1.record(b.s)
+    b.s.wait()
+    wall = time.perf_counter() - w0
+    gpu_ms = e0.elapsed_ms(e1)
+    st = b.stats()  # raises on any device error flag
+    n_bodies = b.n_bodies
+    n_static = len(scs)
+    n_per = np.diff(b.body_offset).astype(np.float64)
+    nbytes = frame_bytes(n_bodies - n_static, n_static, frames, float((n_per * st["solves"]).sum()),
+                         float(st["sum_pairs"].sum()), float(st["sum_contacts"].sum()),
+                         float(st["sum_dyn_endpoints"].sum()), scs[0].n_iters)
+    achieved = nbytes / (gpu_ms * 1e-3) / 1e9
+    # The same frames captured as a HIP graph of single-frame launches.
+    g_frames = 10
+    gb = capi.Batch(scs)
+    gb.set_shape(*shape)
+    with capi.Graph.capture(gb.s) as g:
+        for _ in range(g_frames):
+            gb.step(1)
+    gb.s.wait()
+    e0.record(gb.s)
+    for _ in range(frames // g_frames):
+        g.launch(gb.s)
+    e1.record(gb.s)
+    gb.s.wait()
+    graph_ms = e0.elapsed_ms(e1) / (g_frames * (frames // g_frames))
+    same = all(np.array_equal(b.state()[k].view(np.uint32), gb.state()[k].view(np.uint32))
+               for k in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z"))
+    # Plain per-scene loop: the drop-in frame (integrate, build_and_query,
+    # device-count narrowphase, solve) scene after scene on one stream.
+    rng = np.random.default_rng(20260003)
+    idx = rng.choice(len(scs), min(plain_sample, len(scs)), replace=False)
+    s = capi.Stream()
+    worlds = [capi.World(scs[i], s) for i in idx]
+    for w in worlds:
+        w.step_async()
+    s.wait()
+    p_frames = 5
+    p0 = time.perf_counter()
+    for _ in range(p_frames):
+        for w in worlds:
+            w.step_async()
+    s.wait()
+    per_scene_frame = (time.perf_counter() - p0) / (p_frames * len(worlds))
+    plain_frame = per_scene_frame * len(scs)
+    value = n_bodies * frames / wall
+    return {
+        "workload": f"{len(scs)} pile scenes (boxes, lognormal body count median 64, sigma 0.75, "
+                    f"clip [8, 512], own ground each), seed 20260003, {frames} frames at 60 Hz, "
+                    f"1 substep, 10 iterations, all frames in one launch",
+        "value": round(value, 1), "unit": "body-substeps/s", "bodies": n_bodies, "scenes": len(scs),
+        "frames": frames, "ms_per_frame": round(1000.0 * wall / frames, 4),
+        "stream_ms_per_frame": round(gpu_ms / frames, 4), "launches": 1,
+        "shape": {"threads": shape[0], "wave_scene_max": shape[1]},
+        "autotune_ms": {f"{t}/{m}": round(x, 3) for (t, m), x in zip(capi.Batch.TUNE_SHAPES, tune_ms)},
+        "graph_ms_per_frame": round(graph_ms, 4), "graph_state_equal": bool(same),
+        "plain_loop": {"ms_per_scene_frame": round(1000.0 * per_scene_frame, 4),
+                       "sample_scenes": len(worlds), "sample_frames": p_frames,
+                       "ms_per_frame_all_scenes": round(1000.0 * plain_frame, 2),
+                       "value": round(n_bodies / plain_frame, 1)},
+        "vs_plain_loop": round((n_bodies * frames / wall) / (n_bodies / plain_frame), 1),
+        "load": {"pairs_mean": float(st["sum_pairs"].sum()) / frames,
+                 "contacts_mean": float(st["sum_contacts"].sum()) / frames,
+                 "serial_scenes_last_frame": int(st["used_serial"].sum())},
+        "roofline": {"bound": "hbm", "kernel": "batch_frames", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": None, "algorithmic_bytes_per_launch": round(nbytes)},
+        "setup_s": round(w0 - t0, 2),
     }
 
 
@@ -246,6 +355,8 @@ def main():
     if window:
         out["cpu_baseline"] = cpu_baseline(sc, snaps, starts, window)
         out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
+    if args.batched_frames > 0:
+        out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
     print(json.dumps(out), flush=True)
 
 

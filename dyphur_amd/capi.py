@@ -76,7 +76,8 @@ class SolverTiming(C.Structure):
 
 class BatchSceneStats(C.Structure):
     _fields_ = [(k, u32) for k in ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds",
-                                   "overflow", "error", "substeps")]
+                                   "overflow", "error", "substeps", "sum_pairs", "sum_contacts",
+                                   "sum_dyn_endpoints", "solves")]
 
 
 class BatchLayout(C.Structure):
@@ -649,7 +650,7 @@ class _Capture:
 
 
 BATCH_STAT_FIELDS = ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds", "overflow", "error",
-                     "substeps")
+                     "substeps", "sum_pairs", "sum_contacts", "sum_dyn_endpoints", "solves")
 
 
 class Batch:
@@ -732,7 +733,7 @@ class Batch:
         st = lib().dy_batch_stats(self.h, self.s.h, arr)
         if raise_on_error:
             check(st, "dy_batch_stats")
-        raw = np.ctypeslib.as_array(C.cast(arr, C.POINTER(u32)), (self.n_scenes, 8)).copy()
+        raw = np.ctypeslib.as_array(C.cast(arr, C.POINTER(u32)), (self.n_scenes, len(BATCH_STAT_FIELDS))).copy()
         return {k: raw[:, i] for i, k in enumerate(BATCH_STAT_FIELDS)}
 
     BATCH_STAGES = ("integrate_boxes", "pairs", "narrowphase", "assign", "sweep")

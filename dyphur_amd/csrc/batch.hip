@@ -57,7 +57,7 @@ constexpr uint32_t kStages = 5;
 
 enum : uint32_t {
     kStPairs = 0, kStContacts, kStClasses, kStSerial, kStRounds, kStOverflow, kStError, kStSubsteps,
-    kStWords
+    kStSumPairs, kStSumContacts, kStSumDyn, kStSumSolves, kStWords
 };
 
 struct BatchArgs {
@@ -500,7 +500,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         mark(5);
 
         // 5. solve (xpbd_solver.cpp:409-733), contacts only.
-        uint32_t serial = 1, rounds = 0, n_cls = 0, overflow = 0, err = 0;
+        uint32_t serial = 1, rounds = 0, n_cls = 0, overflow = 0, err = 0, n_dyn_end = 0;
         if (nc > 0) {
             for (uint32_t x = r; x < n; x += NT) { S.deg[x] = 0; S.fill[x] = 0; S.used[x] = 0; }
             for (uint32_t k = r; k < nc; k += NT) {
@@ -519,7 +519,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             tm.sync();
             for (uint32_t x = r; x < n; x += NT) S.loff[x] = S.deg[x];
             tm.sync();
-            (void)team_scan(tm, S.loff, n);
+            n_dyn_end = team_scan(tm, S.loff, n);
             tm.sync();
             for (uint32_t k = r; k < nc; k += NT) {
                 const uint32_t a = cv.body_a[k], b = cv.body_b[k];
@@ -614,6 +614,10 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             st[kStOverflow] = overflow;
             st[kStError] |= tf[20];
             st[kStSubsteps] += 1u;
+            st[kStSumPairs] += n_pairs;
+            st[kStSumContacts] += nc;
+            st[kStSumDyn] += n_dyn_end;
+            st[kStSumSolves] += nc > 0 ? 1u : 0u;
         }
         tm.sync();
     }

@@ -275,7 +275,7 @@ int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t
  * bodies per scene. */
 typedef struct dy_batch_s *dy_batch_t;
 
-/* Per-scene statistics of the scene's last substep (substeps: running total). */
+/* Per-scene statistics of the scene's last substep, and running totals. */
 typedef struct dy_batch_scene_stats {
     uint32_t n_pairs;      /* true candidate-pair count (may exceed the scene's pair capacity) */
     uint32_t n_contacts;   /* true contact count (may exceed the scene's contact capacity) */
@@ -285,6 +285,10 @@ typedef struct dy_batch_scene_stats {
     uint32_t overflow;     /* 1: a body needed a 65th colour */
     uint32_t error;        /* OR of: 1 hull/mesh shape met, 2 internal check failed */
     uint32_t substeps;     /* substeps run since creation */
+    uint32_t sum_pairs;    /* totals over those substeps: true pairs, */
+    uint32_t sum_contacts; /* stored contacts, */
+    uint32_t sum_dyn_endpoints;  /* contact endpoints on dynamic bodies, */
+    uint32_t solves;       /* substeps that solved at least one contact */
 } dy_batch_scene_stats;
 
 /* Device outputs of the last substep, concatenated over scenes. */

@@ -94,13 +94,25 @@ def test_batch_matches_oracle_every_frame(gpu, oracle, shape):
     batch.set_shape(*shape)
     ref = _oracle_worlds(oracle, scs)
     saw_jp = saw_serial = False
+    sums = np.zeros((len(scs), 4), np.int64)
     for f in range(40):
         batch.step(1)
         fr = _run_oracle_frame(ref, scs)
         _compare_frame(batch, fr, f)
         saw_jp |= any(not x[3]["used_serial"] and x[3]["n_contacts"] > 0 for x in fr)
         saw_serial |= any(x[3]["used_serial"] for x in fr)
+        for i, (w, b, _sh, fs) in enumerate(fr):
+            c = w.contacts()
+            im = b.arr["inv_mass"]
+            sums[i] += (fs["n_pairs"], c["n"],
+                        int((im[c["body_a"]] != 0).sum() + (im[c["body_b"]] != 0).sum()), int(c["n"] > 0))
     assert saw_jp and saw_serial
+    st = batch.stats()
+    assert np.array_equal(st["sum_pairs"], sums[:, 0])
+    assert np.array_equal(st["sum_contacts"], sums[:, 1])
+    assert np.array_equal(st["sum_dyn_endpoints"], sums[:, 2])
+    assert np.array_equal(st["solves"], sums[:, 3])
+    assert (st["substeps"] == 40).all()
 
 
 def test_batch_force_serial_matches_oracle(gpu, oracle):
