@@ -1,7 +1,7 @@
 // hostmath.cpp — the per-element arithmetic of dy_math.hpp compiled for the
 // host (hipcc --cuda-host-only), exported with a C ABI so the CPU test suite
 // can compare the exact code the kernels run against the oracle without a GPU.
-#include "dy_math.hpp"
+#include "adjoint.hpp"
 
 using namespace dy;
 
@@ -80,5 +80,124 @@ void hm_integrate_body(float *kin, float gx, float gy, float gz, float h, int n_
 }
 
 void hm_renormalise(float *q4) { renormalise(q4[0], q4[1], q4[2], q4[3]); }
+
+}  // extern "C"
+
+// ---- derivatives (adjoint.hpp), float and double --------------------------
+namespace {
+
+template <class S>
+int contact_fwd(uint32_t ta, uint32_t tb, const S *ha, const S *hb, const S *pa, const S *pb, const S *qa,
+                const S *qb, S *out, int *winner) {
+    HitT<S> h{};
+    if (!pair_contact_t(ta, tb, ha, hb, pa, qa, pb, qb, h, -1, winner)) return 0;
+    out[0] = h.px; out[1] = h.py; out[2] = h.pz;
+    out[3] = h.nx; out[4] = h.ny; out[5] = h.nz;
+    out[6] = h.depth;
+    return 1;
+}
+
+template <class S> BodyT<S> body_from(const S *a) {
+    BodyT<S> b;
+    static_assert(sizeof(BodyT<S>) == 20 * sizeof(S), "BodyT layout");
+    __builtin_memcpy(&b, a, sizeof(b));
+    return b;
+}
+
+template <class S> ContactT<S> contact_from(const S *c) {
+    ContactT<S> k;
+    for (int i = 0; i < 3; ++i) { k.p[i] = c[i]; k.n[i] = c[3 + i]; }
+    k.depth = c[6];
+    return k;
+}
+
+template <class S> int project_fwd(const S *c, S *a, S *b, S mu, S *lam, S dt) {
+    BodyT<S> A = body_from(a), B = body_from(b);
+    const bool r = project_t(contact_from(c), A, B, mu, *lam, dt);
+    __builtin_memcpy(a, &A, sizeof(A));
+    __builtin_memcpy(b, &B, sizeof(B));
+    return r ? 1 : 0;
+}
+
+// bar[14] = adjoints of p xyz, q wxyz, v xyz, w xyz, inverse mass.
+template <class S> BodyBar<S> bar_from(const S *x) {
+    BodyBar<S> b;
+    static_assert(sizeof(BodyBar<S>) == 14 * sizeof(S), "BodyBar layout");
+    __builtin_memcpy(&b, x, sizeof(b));
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Contact of one pair through the templated replay (full search); winner
+// receives the box-box axis code.  Same argument order as hm_pair_contact.
+int ha_pair_contact_f(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
+                      const float *qa, const float *qb, float *out, int *winner) {
+    return contact_fwd(ta, tb, ha, hb, pa, pb, qa, qb, out, winner);
+}
+int ha_pair_contact_d(uint32_t ta, uint32_t tb, const double *ha, const double *hb, const double *pa,
+                      const double *pb, const double *qa, const double *qb, double *out, int *winner) {
+    return contact_fwd(ta, tb, ha, hb, pa, pb, qa, qb, out, winner);
+}
+// Contact geometry adjoint: ga[7], gb[7] (+=) = J^T hbar, nt pose tangents per pass (1, 2, 7 or 14).
+int ha_contact_adj_d(uint32_t ta, uint32_t tb, const double *ha, const double *hb, const double *pa,
+                     const double *pb, const double *qa, const double *qb, const double *hbar, double *ga,
+                     double *gb, int nt) {
+    switch (nt) {
+        case 1: return contact_adj<double, 1>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+        case 2: return contact_adj<double, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+        case 7: return contact_adj<double, 7>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+        default: return contact_adj<double, 14>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+    }
+}
+int ha_contact_adj_f(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
+                     const float *qa, const float *qb, const float *hbar, float *ga, float *gb) {
+    return contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+}
+// Contact projection on 20-value bodies (hm_project_contact layout), in place.
+int ha_project_f(const float *c, float *a, float *b, float mu, float *lam, float dt) {
+    return project_fwd(c, a, b, mu, lam, dt);
+}
+int ha_project_d(const double *c, double *a, double *b, double mu, double *lam, double dt) {
+    return project_fwd(c, a, b, mu, lam, dt);
+}
+// Adjoint of the projection from inputs (c, a, b, lam): abar/bbar[14] hold
+// the output adjoints on entry and the input adjoints on return; *lamb
+// likewise; cbar[7] receives the contact's adjoint.
+void ha_project_adj_d(const double *c, const double *a, const double *b, double mu, double lam, double dt,
+                      double *abar, double *bbar, double *lamb, double *cbar) {
+    BodyBar<double> Ab = bar_from(abar), Bb = bar_from(bbar);
+    ContactT<double> cb{};
+    project_adj(contact_from(c), body_from(a), body_from(b), mu, lam, dt, Ab, Bb, *lamb, cb);
+    __builtin_memcpy(abar, &Ab, sizeof(Ab));
+    __builtin_memcpy(bbar, &Bb, sizeof(Bb));
+    for (int i = 0; i < 3; ++i) { cbar[i] = cb.p[i]; cbar[3 + i] = cb.n[i]; }
+    cbar[6] = cb.depth;
+}
+// One integrator substep on 13 values (hm_integrate_body layout) and its adjoint.
+void ha_integrate_f(float *kin, const float *g, float h) {
+    BodyT<float> b{};
+    __builtin_memcpy(&b, kin, 13 * sizeof(float));
+    integrate_t(b, g, h);
+    __builtin_memcpy(kin, &b, 13 * sizeof(float));
+}
+void ha_integrate_d(double *kin, const double *g, double h) {
+    BodyT<double> b{};
+    __builtin_memcpy(&b, kin, 13 * sizeof(double));
+    integrate_t(b, g, h);
+    __builtin_memcpy(kin, &b, 13 * sizeof(double));
+}
+void ha_integrate_adj_d(const double *kin0, const double *g, double h, double *bar) {
+    BodyT<double> b{};
+    __builtin_memcpy(&b, kin0, 13 * sizeof(double));
+    BodyBar<double> bb{};
+    __builtin_memcpy(&bb, bar, 13 * sizeof(double));
+    integrate_adj(b, g, h, bb);
+    __builtin_memcpy(bar, &bb, 13 * sizeof(double));
+}
+void ha_renorm_d(double *q) { renorm_t(q); }
+void ha_renorm_adj_d(const double *q, double *qb) { renorm_adj(q, qb); }
 
 }  // extern "C"
