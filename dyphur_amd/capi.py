@@ -163,6 +163,11 @@ _SIGS = {
     "dy_batch_stats": (C.c_int, [vp, vp, C.POINTER(BatchSceneStats)]),
     "dy_batch_layout_get": (C.c_int, [vp, C.POINTER(BatchLayout)]),
     "dy_batch_set_timing": (C.c_int, [vp, vp, C.c_int]),
+    "dy_diff_create": (C.c_int, [vp, u32, u32, C.POINTER(vp)]),
+    "dy_diff_destroy": (C.c_int, [vp]),
+    "dy_diff_forward": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32, C.c_int]),
+    "dy_diff_backward": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
+                                   C.POINTER(BodyView), C.POINTER(BodyView)]),
     "dy_batch_get_timing": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double)]),
 }
 
@@ -768,6 +773,66 @@ class Batch:
         try:
             if self.h:
                 lib().dy_batch_destroy(self.h)
+                self.h = vp()
+        except Exception:
+            pass
+
+
+MUTABLE_FIELDS = ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
+                  "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z")
+
+
+class Diff:
+    """Reverse-mode derivatives of a Batch's frames (dy_diff_*): forward()
+    steps the batch keeping checkpoints; backward() turns the adjoint of the
+    final state into the adjoint of the initial state and of inv_mass."""
+
+    def __init__(self, batch: Batch, max_frames: int, checkpoint_every: int = 10):
+        self.batch = batch
+        self.h = vp()
+        check(lib().dy_diff_create(batch.h, max_frames, checkpoint_every, C.byref(self.h)), "dy_diff_create")
+        n = batch.n_bodies
+        self.adj = {k: DeviceArray(n, np.float32) for k in MUTABLE_FIELDS + ("inv_mass",)}
+        self.frames = 0
+
+    def forward(self, n_frames: int):
+        b = self.batch
+        sc = b.scenes[0]
+        g = sc.gravity
+        bv, sv = b.bodies.view(), b.shapes.view()
+        check(lib().dy_diff_forward(self.h, b.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt, n_frames),
+              "dy_diff_forward")
+        self.frames = n_frames
+
+    def _adj_view(self) -> BodyView:
+        v = BodyView()
+        for k in BODY_FLOAT_FIELDS:
+            setattr(v, k, self.adj[k].ptr if k in self.adj else None)
+        v.n = self.batch.n_bodies
+        return v
+
+    def backward(self, adj_final: dict, download: bool = True) -> dict | None:
+        """adj_final: host arrays (n,) for any of the 13 mutable fields (others
+        zero).  Returns the adjoint of the initial state's 13 fields and of
+        inv_mass (host arrays) unless download is False."""
+        b = self.batch
+        sc = b.scenes[0]
+        g = sc.gravity
+        n = b.n_bodies
+        for k in MUTABLE_FIELDS:
+            self.adj[k].upload(np.ascontiguousarray(adj_final.get(k, np.zeros(n)), np.float32), b.s)
+        v = self._adj_view()
+        bv, sv = b.bodies.view(), b.shapes.view()
+        check(lib().dy_diff_backward(self.h, b.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt,
+                                     C.byref(v), C.byref(v)), "dy_diff_backward")
+        if not download:
+            return None
+        return {k: self.adj[k].download(b.s) for k in self.adj}
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().dy_diff_destroy(self.h)
                 self.h = vp()
         except Exception:
             pass

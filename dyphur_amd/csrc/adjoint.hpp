@@ -93,6 +93,9 @@ template <class S, int N> DY_HD bool operator<(const Dual<S, N> &a, S b) { retur
 template <class S, int N> DY_HD bool operator>(const Dual<S, N> &a, S b) { return a.v > b; }
 template <class S, int N> DY_HD bool operator<=(const Dual<S, N> &a, S b) { return a.v <= b; }
 template <class S, int N> DY_HD bool operator>=(const Dual<S, N> &a, S b) { return a.v >= b; }
+template <class S, int N> DY_HD bool operator!=(const Dual<S, N> &a, S b) { return a.v != b; }
+template <class S, int N> DY_HD bool operator==(const Dual<S, N> &a, S b) { return a.v == b; }
+template <class S, int N> DY_HD bool operator!=(const Dual<S, N> &a, const Dual<S, N> &b) { return a.v != b.v; }
 template <class S, int N> DY_HD Dual<S, N> sq_root(const Dual<S, N> &a) {
     Dual<S, N> r; r.v = sq_root(a.v);
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / (S(2) * r.v);

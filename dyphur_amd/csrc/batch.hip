@@ -40,6 +40,7 @@
 
 #include "dy_internal.hpp"
 #include "pair_contact.hpp"
+#include "adjoint.hpp"
 
 namespace dy {
 
@@ -72,6 +73,8 @@ struct BatchArgs {
     uint32_t *scratch;
     uint32_t *stats;       // kStWords per scene
     uint64_t *stage_clk;   // kStages per scene when timed
+    float *tape;           // differentiable form: per-frame records (TapeLayout)
+    const uint64_t *tape_off;  // per scene, floats
     uint32_t region_floats;  // LDS floats per team
     float gx, gy, gz, h;
     int n_sub, n_frames, n_iters, force_serial;
@@ -99,6 +102,32 @@ struct SceneScratch {
 __host__ __device__ inline uint64_t scene_scratch_words(uint32_t n, uint32_t c) {
     const uint64_t w = 6ull * n + 10ull * c + 2ull + static_cast<uint64_t>(n) * ((n + 31u) / 32u);
     return (w + 1ull) & ~1ull;
+}
+
+// Per-frame record of one scene for the reverse sweep (floats, each part
+// 16-byte aligned): frame-start state (13 fields x n, SoA), post-integrate
+// pose (7 x n), quaternions before each iteration's renormalisation
+// (iters x 4 x n), a header (contacts solved, classes), class offsets
+// (C + 1), contacts in class order (8 floats: p, n, depth; and the body pair),
+// and each contact's inputs at each iteration (28 floats: both bodies' 13
+// mutable fields, lambda).
+struct TapeLayout {
+    uint64_t st, po, qr, hdr, co, rec, ab, pre, frame;
+};
+
+__host__ __device__ inline TapeLayout tape_layout(uint32_t n, uint32_t c, uint32_t iters) {
+    TapeLayout L;
+    uint64_t o = 0;
+    L.st = o; o = (o + 13ull * n + 3ull) & ~3ull;
+    L.po = o; o = (o + 7ull * n + 3ull) & ~3ull;
+    L.qr = o; o = (o + 4ull * n * iters + 3ull) & ~3ull;
+    L.hdr = o; o += 4;
+    L.co = o; o = (o + c + 1ull + 3ull) & ~3ull;
+    L.rec = o; o += 8ull * c;
+    L.ab = o; o = (o + 2ull * c + 3ull) & ~3ull;
+    L.pre = o; o += 28ull * c * iters;
+    L.frame = o;
+    return L;
 }
 
 __device__ __forceinline__ SceneScratch carve(uint32_t *p, uint32_t n, uint32_t c) {
@@ -344,7 +373,7 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
 // live in the team's LDS region from the first substep to the last and are
 // written back at the end; otherwise the stages read and write the global
 // arrays.  Per substep: integrate, leaf boxes, pairs, narrowphase, solve.
-template <class Team, bool RES, bool TIMED>
+template <class Team, bool RES, bool TIMED, bool TAPE>
 __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float *region) {
     constexpr uint32_t NT = Team::kSize;
     const uint32_t r = tm.rank;
@@ -384,10 +413,34 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         }
     };
 
+    [[maybe_unused]] TapeLayout TL{};
+    [[maybe_unused]] float *tp = nullptr;
+    if constexpr (TAPE) {
+        TL = tape_layout(n, C, static_cast<uint32_t>(A.n_iters));
+        tp = A.tape + A.tape_off[s];
+    }
     for (uint32_t step = 0; step < n_steps; ++step) {
         mark(0);
+        [[maybe_unused]] float *fb = nullptr;
+        if constexpr (TAPE) {
+            fb = tp + step * TL.frame;
+            float *st0 = fb + TL.st;
+            for (uint32_t i = r; i < n; i += NT) {
+                st0[0 * n + i] = bv.pos_x[i]; st0[1 * n + i] = bv.pos_y[i]; st0[2 * n + i] = bv.pos_z[i];
+                st0[3 * n + i] = bv.rot_w[i]; st0[4 * n + i] = bv.rot_x[i]; st0[5 * n + i] = bv.rot_y[i];
+                st0[6 * n + i] = bv.rot_z[i];
+                st0[7 * n + i] = bv.vel_x[i]; st0[8 * n + i] = bv.vel_y[i]; st0[9 * n + i] = bv.vel_z[i];
+                st0[10 * n + i] = bv.ang_x[i]; st0[11 * n + i] = bv.ang_y[i]; st0[12 * n + i] = bv.ang_z[i];
+            }
+        }
         // 1. integrate one substep (integrator.cpp:17-70, n_substeps = 1 at h).
         for (uint32_t i = r; i < n; i += NT) {
+            if constexpr (TAPE) {
+                float *po = fb + TL.po;
+                po[0 * n + i] = bv.pos_x[i]; po[1 * n + i] = bv.pos_y[i]; po[2 * n + i] = bv.pos_z[i];
+                po[3 * n + i] = bv.rot_w[i]; po[4 * n + i] = bv.rot_x[i]; po[5 * n + i] = bv.rot_y[i];
+                po[6 * n + i] = bv.rot_z[i];
+            }
             if (bv.inv_mass[i] == 0.f || (gv.flags[i] & kFlagSleeping)) continue;
             Kin k{bv.pos_x[i], bv.pos_y[i], bv.pos_z[i], bv.rot_w[i], bv.rot_x[i], bv.rot_y[i],
                   bv.rot_z[i], bv.vel_x[i], bv.vel_y[i], bv.vel_z[i], bv.ang_x[i], bv.ang_y[i], bv.ang_z[i]};
@@ -395,6 +448,11 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             bv.pos_x[i] = k.px; bv.pos_y[i] = k.py; bv.pos_z[i] = k.pz;
             bv.rot_w[i] = k.qw; bv.rot_x[i] = k.qx; bv.rot_y[i] = k.qy; bv.rot_z[i] = k.qz;
             bv.vel_x[i] = k.vx; bv.vel_y[i] = k.vy; bv.vel_z[i] = k.vz;
+            if constexpr (TAPE) {
+                float *po = fb + TL.po;
+                po[0 * n + i] = k.px; po[1 * n + i] = k.py; po[2 * n + i] = k.pz;
+                po[3 * n + i] = k.qw; po[4 * n + i] = k.qx; po[5 * n + i] = k.qy; po[6 * n + i] = k.qz;
+            }
         }
         // 2. leaf boxes (broadphase.cpp:215-251), binary16-rounded.
         for (uint32_t i = r; i < n; i += NT) {
@@ -565,6 +623,21 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                     const uint32_t c = S.cls[k];
                     S.order[S.coff[c] + atomicAdd(&S.ccnt[c], 1u)] = k;
                 }
+                if constexpr (TAPE) {
+                    uint32_t *co = reinterpret_cast<uint32_t *>(fb + TL.co);
+                    for (uint32_t c = r; c <= n_cls; c += NT) co[c] = S.coff[c];
+                    float *rec = fb + TL.rec;
+                    uint32_t *ab = reinterpret_cast<uint32_t *>(fb + TL.ab);
+                    for (uint32_t j = r; j < nc; j += NT) {
+                        const uint32_t k = S.order[j];
+                        reinterpret_cast<float4 *>(rec)[2 * j] =
+                            make_float4(cv.pos_x[k], cv.pos_y[k], cv.pos_z[k], cv.norm_x[k]);
+                        reinterpret_cast<float4 *>(rec)[2 * j + 1] =
+                            make_float4(cv.norm_y[k], cv.norm_z[k], cv.depth[k], 0.f);
+                        ab[2 * j] = cv.body_a[k];
+                        ab[2 * j + 1] = cv.body_b[k];
+                    }
+                }
                 tm.sync();
                 mark(4);
                 // I iterations: classes in order, then renormalise every body
@@ -579,6 +652,17 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                                                cv.norm_y[k], cv.norm_z[k], cv.depth[k]};
                             BodyState Ba = load_body(bv, a), Bb = load_body(bv, b);
                             float l = S.lam[k];
+                            if constexpr (TAPE) {
+                                float4 *pr = reinterpret_cast<float4 *>(
+                                    fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + j));
+                                pr[0] = make_float4(Ba.px, Ba.py, Ba.pz, Ba.qw);
+                                pr[1] = make_float4(Ba.qx, Ba.qy, Ba.qz, Ba.vx);
+                                pr[2] = make_float4(Ba.vy, Ba.vz, Ba.wx, Ba.wy);
+                                pr[3] = make_float4(Ba.wz, Bb.px, Bb.py, Bb.pz);
+                                pr[4] = make_float4(Bb.qw, Bb.qx, Bb.qy, Bb.qz);
+                                pr[5] = make_float4(Bb.vx, Bb.vy, Bb.vz, Bb.wx);
+                                pr[6] = make_float4(Bb.wy, Bb.wz, l, 0.f);
+                            }
                             if (project_contact(ci, Ba, Bb, A.mu, l, h)) {
                                 S.lam[k] = l;
                                 if (Ba.im != 0.f) store_body(bv, a, Ba);
@@ -589,6 +673,10 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                     }
                     for (uint32_t i = r; i < n; i += NT) {
                         float qw = bv.rot_w[i], qx = bv.rot_x[i], qy = bv.rot_y[i], qz = bv.rot_z[i];
+                        if constexpr (TAPE) {
+                            float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
+                            qr[i] = qw; qr[n + i] = qx; qr[2 * n + i] = qy; qr[3 * n + i] = qz;
+                        }
                         renormalise(qw, qx, qy, qz);
                         bv.rot_w[i] = qw; bv.rot_x[i] = qx; bv.rot_y[i] = qy; bv.rot_z[i] = qz;
                     }
@@ -599,6 +687,15 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         if (err) atomicOr(&tf[20], err);
         tm.sync();
         mark(5);
+        if constexpr (TAPE) {
+            if (r == 0) {
+                uint32_t *hdr = reinterpret_cast<uint32_t *>(fb + TL.hdr);
+                hdr[0] = err ? 0u : nc;
+                hdr[1] = err ? 0u : n_cls;
+                hdr[2] = err;
+                hdr[3] = 0u;
+            }
+        }
         if (r == 0) {
             if constexpr (TIMED) {
                 for (uint32_t j = 0; j < kStages; ++j) {
@@ -638,7 +735,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
 // RES: every scene of the launch keeps its state in LDS (the usual case);
 // otherwise the launch holds the scenes too large for that, one per
 // workgroup (or per 64-thread workgroup).
-template <uint32_t NT, bool TIMED, bool RES>
+template <uint32_t NT, bool TIMED, bool RES, bool TAPE>
 __global__ __launch_bounds__(NT) void batch_frames(BatchArgs A) {
     extern __shared__ float lds[];
     __shared__ uint32_t team_sh[NT / 64][32];
@@ -647,7 +744,7 @@ __global__ __launch_bounds__(NT) void batch_frames(BatchArgs A) {
     if constexpr (NT > 64) {
         if (kind & kUnitBlock) {
             const BlockTeam<NT> tm{threadIdx.x, team_sh[0]};
-            run_scene<BlockTeam<NT>, RES, TIMED>(tm, A, u.x, lds);
+            run_scene<BlockTeam<NT>, RES, TIMED, TAPE>(tm, A, u.x, lds);
             return;
         }
     }
@@ -658,19 +755,174 @@ __global__ __launch_bounds__(NT) void batch_frames(BatchArgs A) {
         const uint32_t sc = __builtin_amdgcn_readfirstlane(w == 0 ? u.x : w == 1 ? u.y : w == 2 ? u.z : u.w);
         if (sc == kNoScene) return;  // wavefront teams never wait on each other
         const WaveTeam tm{lane_id(), team_sh[w]};
-        run_scene<WaveTeam, RES, TIMED>(tm, A, sc, lds + static_cast<size_t>(w) * A.region_floats);
+        run_scene<WaveTeam, RES, TIMED, TAPE>(tm, A, sc, lds + static_cast<size_t>(w) * A.region_floats);
     }
 }
 
 template <uint32_t NT, bool RES>
 int launch_frames(const BatchArgs &a, uint32_t n_units, size_t lds_bytes, hipStream_t s) {
     if (n_units == 0) return DY_OK;
-    if (a.stage_clk)
-        hipLaunchKernelGGL((batch_frames<NT, true, RES>), dim3(n_units), dim3(NT), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL((batch_frames<NT, false, RES>), dim3(n_units), dim3(NT), lds_bytes, s, a);
+    if (a.tape) {
+        if constexpr (RES)
+            hipLaunchKernelGGL((batch_frames<NT, false, true, true>), dim3(n_units), dim3(NT), lds_bytes, s, a);
+        else
+            return DY_ERR_INVALID_ARG;  // the differentiable form keeps every scene in LDS
+    } else if (a.stage_clk) {
+        hipLaunchKernelGGL((batch_frames<NT, true, RES, false>), dim3(n_units), dim3(NT), lds_bytes, s, a);
+    } else {
+        hipLaunchKernelGGL((batch_frames<NT, false, RES, false>), dim3(n_units), dim3(NT), lds_bytes, s, a);
+    }
     DY_LAUNCH_CHECK();
     return DY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweep of recorded frames (the differentiable form)
+// ---------------------------------------------------------------------------
+struct DiffArgs {
+    dy_body_view adj;  // adjoint of the 13 mutable fields; inv_mass holds the inverse-mass adjoint
+    float *cs;         // per contact: lambda adjoint, then the contact's (p, n, depth) adjoint
+    uint32_t frames;   // recorded frames to reverse, newest first
+};
+
+__device__ __forceinline__ BodyBar<float> load_bar(const dy_body_view &v, uint32_t i) {
+    BodyBar<float> b;
+    b.p[0] = v.pos_x[i]; b.p[1] = v.pos_y[i]; b.p[2] = v.pos_z[i];
+    b.q[0] = v.rot_w[i]; b.q[1] = v.rot_x[i]; b.q[2] = v.rot_y[i]; b.q[3] = v.rot_z[i];
+    b.v[0] = v.vel_x[i]; b.v[1] = v.vel_y[i]; b.v[2] = v.vel_z[i];
+    b.w[0] = v.ang_x[i]; b.w[1] = v.ang_y[i]; b.w[2] = v.ang_z[i];
+    b.im = v.inv_mass[i];
+    return b;
+}
+
+__device__ __forceinline__ void store_bar(const dy_body_view &v, uint32_t i, const BodyBar<float> &b) {
+    v.pos_x[i] = b.p[0]; v.pos_y[i] = b.p[1]; v.pos_z[i] = b.p[2];
+    v.rot_w[i] = b.q[0]; v.rot_x[i] = b.q[1]; v.rot_y[i] = b.q[2]; v.rot_z[i] = b.q[3];
+    v.vel_x[i] = b.v[0]; v.vel_y[i] = b.v[1]; v.vel_z[i] = b.v[2];
+    v.ang_x[i] = b.w[0]; v.ang_y[i] = b.w[1]; v.ang_z[i] = b.w[2];
+    v.inv_mass[i] = b.im;
+}
+
+// Body state as a projection input: 13 recorded fields + the constants.
+__device__ __forceinline__ BodyT<float> body_in(const float *m, const dy_body_view &g, uint32_t i) {
+    BodyT<float> b;
+    for (int k = 0; k < 3; ++k) { b.p[k] = m[k]; b.v[k] = m[7 + k]; b.w[k] = m[10 + k]; }
+    for (int k = 0; k < 4; ++k) b.q[k] = m[3 + k];
+    b.im = g.inv_mass[i];
+    b.I[0] = g.iI_xx[i]; b.I[1] = g.iI_yy[i]; b.I[2] = g.iI_zz[i];
+    b.I[3] = g.iI_xy[i]; b.I[4] = g.iI_xz[i]; b.I[5] = g.iI_yz[i];
+    return b;
+}
+
+// One wavefront per scene reverses its recorded frames, newest first: per
+// frame the solver iterations in reverse (each iteration's renormalisation,
+// then its classes last to first, contacts of a class in parallel: they
+// share no dynamic body), then the contact geometry (class by class), then
+// the integrator.  Static bodies receive no adjoint.
+__global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t r = lane_id();
+    const WaveTeam tm{r, nullptr};
+    const uint32_t b0 = A.body_off[s], n = A.body_off[s + 1] - b0;
+    const uint32_t C = A.contact_off[s + 1] - A.contact_off[s];
+    const dy_body_view gv = scene_bodies(A.bv, b0, n);
+    const dy_body_view av = scene_bodies(D.adj, b0, n);
+    const uint32_t iters = static_cast<uint32_t>(A.n_iters);
+    const TapeLayout TL = tape_layout(n, C, iters);
+    const float *tp = A.tape + A.tape_off[s];
+    float *cs = D.cs + 8ull * A.contact_off[s];
+    const float h = A.h;
+    const float g[3] = {A.gx, A.gy, A.gz};
+    for (int f = static_cast<int>(D.frames) - 1; f >= 0; --f) {
+        const float *fb = tp + static_cast<uint64_t>(f) * TL.frame;
+        const uint32_t *hdr = reinterpret_cast<const uint32_t *>(fb + TL.hdr);
+        const uint32_t nc = hdr[0], ncls = hdr[1];
+        if (nc > 0) {
+            const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
+            const float4 *rec = reinterpret_cast<const float4 *>(fb + TL.rec);
+            const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
+            for (uint32_t j = r; j < 8 * nc; j += 64) cs[j] = 0.f;
+            tm.sync();
+            for (int it = static_cast<int>(iters) - 1; it >= 0; --it) {
+                const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
+                for (uint32_t i = r; i < n; i += 64) {
+                    if (gv.inv_mass[i] == 0.f) continue;
+                    const float q[4] = {qr[i], qr[n + i], qr[2 * n + i], qr[3 * n + i]};
+                    float qb[4] = {av.rot_w[i], av.rot_x[i], av.rot_y[i], av.rot_z[i]};
+                    renorm_adj(q, qb);
+                    av.rot_w[i] = qb[0]; av.rot_x[i] = qb[1]; av.rot_y[i] = qb[2]; av.rot_z[i] = qb[3];
+                }
+                tm.sync();
+                for (int c = static_cast<int>(ncls) - 1; c >= 0; --c) {
+                    const uint32_t end = co[c + 1];
+                    for (uint32_t j = co[c] + r; j < end; j += 64) {
+                        const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
+                        const float4 r0 = rec[2 * j], r1 = rec[2 * j + 1];
+                        ContactT<float> ct;
+                        ct.p[0] = r0.x; ct.p[1] = r0.y; ct.p[2] = r0.z;
+                        ct.n[0] = r0.w; ct.n[1] = r1.x; ct.n[2] = r1.y;
+                        ct.depth = r1.z;
+                        const float *pr = fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + j);
+                        const BodyT<float> A0 = body_in(pr, gv, a), B0 = body_in(pr + 13, gv, b);
+                        BodyBar<float> Ab = load_bar(av, a), Bb = load_bar(av, b);
+                        float lb = cs[8 * j];
+                        ContactT<float> cb;
+                        for (int k = 0; k < 3; ++k) { cb.p[k] = 0.f; cb.n[k] = 0.f; }
+                        cb.depth = 0.f;
+                        project_adj(ct, A0, B0, A.mu, pr[26], h, Ab, Bb, lb, cb);
+                        cs[8 * j] = lb;
+                        for (int k = 0; k < 3; ++k) { cs[8 * j + 1 + k] += cb.p[k]; cs[8 * j + 4 + k] += cb.n[k]; }
+                        cs[8 * j + 7] += cb.depth;
+                        if (A0.im != 0.f) store_bar(av, a, Ab);
+                        if (B0.im != 0.f) store_bar(av, b, Bb);
+                    }
+                    tm.sync();
+                }
+            }
+            // contact geometry: (p, n, depth) adjoints to the post-integrate poses
+            const float *po = fb + TL.po;
+            for (uint32_t c = 0; c < ncls; ++c) {
+                const uint32_t end = co[c + 1];
+                for (uint32_t j = co[c] + r; j < end; j += 64) {
+                    const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
+                    const uint32_t sa = gv.shape[a], sb = gv.shape[b];
+                    const float ha[3] = {A.sv.half_x[sa], A.sv.half_y[sa], A.sv.half_z[sa]};
+                    const float hb[3] = {A.sv.half_x[sb], A.sv.half_y[sb], A.sv.half_z[sb]};
+                    const float pa[3] = {po[a], po[n + a], po[2 * n + a]};
+                    const float qa[4] = {po[3 * n + a], po[4 * n + a], po[5 * n + a], po[6 * n + a]};
+                    const float pb[3] = {po[b], po[n + b], po[2 * n + b]};
+                    const float qb[4] = {po[3 * n + b], po[4 * n + b], po[5 * n + b], po[6 * n + b]};
+                    float hbar[7];
+                    for (int k = 0; k < 7; ++k) hbar[k] = cs[8 * j + 1 + k];
+                    float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    (void)contact_adj<float, 2>(A.sv.type[sa], A.sv.type[sb], ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
+                    if (gv.inv_mass[a] != 0.f) {
+                        av.pos_x[a] += ga[0]; av.pos_y[a] += ga[1]; av.pos_z[a] += ga[2];
+                        av.rot_w[a] += ga[3]; av.rot_x[a] += ga[4]; av.rot_y[a] += ga[5]; av.rot_z[a] += ga[6];
+                    }
+                    if (gv.inv_mass[b] != 0.f) {
+                        av.pos_x[b] += gb2[0]; av.pos_y[b] += gb2[1]; av.pos_z[b] += gb2[2];
+                        av.rot_w[b] += gb2[3]; av.rot_x[b] += gb2[4]; av.rot_y[b] += gb2[5]; av.rot_z[b] += gb2[6];
+                    }
+                }
+                tm.sync();
+            }
+        }
+        // integrator substep
+        const float *st0 = fb + TL.st;
+        for (uint32_t i = r; i < n; i += 64) {
+            if (gv.inv_mass[i] == 0.f || (gv.flags[i] & kFlagSleeping)) continue;
+            BodyT<float> k0;
+            for (int q = 0; q < 3; ++q) {
+                k0.p[q] = st0[q * n + i]; k0.v[q] = st0[(7 + q) * n + i]; k0.w[q] = st0[(10 + q) * n + i];
+            }
+            for (int q = 0; q < 4; ++q) k0.q[q] = st0[(3 + q) * n + i];
+            BodyBar<float> bb = load_bar(av, i);
+            integrate_adj(k0, g, h, bb);
+            store_bar(av, i, bb);
+        }
+        tm.sync();
+    }
 }
 
 }  // namespace dy
@@ -797,11 +1049,8 @@ int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
     return st;
 }
 
-int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
-                 float gy, float gz, float dt, int n_substeps, int n_frames) {
-    const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
-    if (it == b->shapes.end()) return DY_ERR_INTERNAL;
-    const dy_batch_s::Shape &sh = it->second;
+BatchArgs batch_args(dy_batch_s *b, const dy_body_view &bodies, const dy_shape_view &shapes, float gx, float gy,
+                     float gz, float dt, int n_substeps, int n_frames) {
     BatchArgs a{};
     a.bv = bodies;
     a.sv = shapes;
@@ -821,6 +1070,19 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
     a.n_iters = b->n_iters;
     a.force_serial = b->force_serial;
     a.mu = b->mu;
+    return a;
+}
+
+int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
+                 float gy, float gz, float dt, int n_substeps, int n_frames, float *tape = nullptr,
+                 const uint64_t *tape_off = nullptr) {
+    const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
+    if (it == b->shapes.end()) return DY_ERR_INTERNAL;
+    const dy_batch_s::Shape &sh = it->second;
+    BatchArgs a = batch_args(b, bodies, shapes, gx, gy, gz, dt, n_substeps, n_frames);
+    a.tape = tape;
+    a.tape_off = tape_off;
+    if (tape && sh.glob.n_units) return DY_ERR_INVALID_ARG;
     for (int pass = 0; pass < 2; ++pass) {
         const dy_batch_s::Units &u = pass == 0 ? sh.res : sh.glob;
         if (u.n_units == 0) continue;
@@ -1082,6 +1344,151 @@ int dy_batch_layout_get(dy_batch_t b, dy_batch_layout *out) {
     out->body_offset = b->body_off.data();
     out->pair_offset = b->pair_off.data();
     out->contact_offset = b->contact_off.data();
+    return DY_OK;
+}
+
+}  // extern "C"
+
+// ---- differentiable form -------------------------------------------------
+struct dy_diff_s {
+    dy_batch_s *b = nullptr;
+    uint32_t max_frames = 0, every = 0, frames = 0;
+    size_t n_total = 0;
+    float *d_tape = nullptr;
+    uint64_t *d_tape_off = nullptr;
+    float *d_ckpt = nullptr;  // checkpoints: 13 x n_total floats each
+    float *d_work = nullptr;  // recompute state: 13 x n_total
+    float *d_cs = nullptr;    // 8 per contact
+};
+
+namespace {
+
+float *const *mutable_fields(const dy_body_view &v, float *out[13]) {
+    float *f[13] = {v.pos_x, v.pos_y, v.pos_z, v.rot_w, v.rot_x, v.rot_y, v.rot_z,
+                    v.vel_x, v.vel_y, v.vel_z, v.ang_x, v.ang_y, v.ang_z};
+    for (int k = 0; k < 13; ++k) out[k] = f[k];
+    return out;
+}
+
+void set_mutable_fields(dy_body_view &v, float *base, size_t n) {
+    float **f[13] = {&v.pos_x, &v.pos_y, &v.pos_z, &v.rot_w, &v.rot_x, &v.rot_y, &v.rot_z,
+                     &v.vel_x, &v.vel_y, &v.vel_z, &v.ang_x, &v.ang_y, &v.ang_z};
+    for (int k = 0; k < 13; ++k) *f[k] = base + k * n;
+}
+
+int copy_state(hipStream_t s, float *const dst[13], float *const src[13], size_t n) {
+    for (int k = 0; k < 13; ++k)
+        DY_TRY(hipMemcpyAsync(dst[k], src[k], n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return DY_OK;
+}
+
+void diff_free(dy_diff_s *d) {
+    void *ps[] = {d->d_tape, d->d_tape_off, d->d_ckpt, d->d_work, d->d_cs};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, dy_diff_t *out) {
+    if (!b || !out || max_frames == 0 || checkpoint_every == 0) return DY_ERR_INVALID_ARG;
+    *out = nullptr;
+    auto *d = new dy_diff_s();
+    d->b = b;
+    d->max_frames = max_frames;
+    d->every = checkpoint_every;
+    d->n_total = b->body_off.back();
+    std::vector<uint64_t> off(b->n_scenes + 1, 0ull);
+    for (uint32_t i = 0; i < b->n_scenes; ++i) {
+        const uint32_t n = b->body_off[i + 1] - b->body_off[i];
+        const uint32_t c = b->contact_off[i + 1] - b->contact_off[i];
+        off[i + 1] = off[i] + static_cast<uint64_t>(checkpoint_every) * tape_layout(n, c, b->n_iters).frame;
+    }
+    const uint32_t n_ckpt = (max_frames + checkpoint_every - 1) / checkpoint_every;
+    int st = DY_OK;
+    if (st == DY_OK) st = dev_alloc(d->d_tape, off.back());
+    if (st == DY_OK) st = dev_alloc(d->d_tape_off, off.size());
+    if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
+    if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
+    if (st == DY_OK) st = dev_alloc(d->d_cs, 8ull * std::max<uint32_t>(b->contact_off.back(), 1u));
+    if (st == DY_OK && hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+        st = DY_ERR_INTERNAL;
+    if (st != DY_OK) {
+        diff_free(d);
+        delete d;
+        return st;
+    }
+    *out = d;
+    return DY_OK;
+}
+
+int dy_diff_destroy(dy_diff_t d) {
+    if (!d) return DY_OK;
+    diff_free(d);
+    delete d;
+    return DY_OK;
+}
+
+int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes, float gx,
+                    float gy, float gz, float dt, int n_frames) {
+    if (!d) return DY_ERR_INVALID_ARG;
+    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, 1, n_frames));
+    if (n_frames <= 0 || static_cast<uint32_t>(n_frames) > d->max_frames) return DY_ERR_INVALID_ARG;
+    const hipStream_t hsm = hs(s);
+    float *src[13];
+    mutable_fields(*bodies, src);
+    for (uint32_t f0 = 0, k = 0; f0 < static_cast<uint32_t>(n_frames); f0 += d->every, ++k) {
+        dy_body_view ck{};
+        set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * d->n_total, d->n_total);
+        float *dst[13];
+        mutable_fields(ck, dst);
+        DY_TRY_STATUS(copy_state(hsm, dst, src, d->n_total));
+        const int nf = static_cast<int>(std::min<uint32_t>(d->every, static_cast<uint32_t>(n_frames) - f0));
+        DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, 1, nf));
+    }
+    d->frames = static_cast<uint32_t>(n_frames);
+    return DY_OK;
+}
+
+int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes, float gx,
+                     float gy, float gz, float dt, const dy_body_view *adj_final, dy_body_view *adj_initial) {
+    if (!d || !adj_final || !adj_initial || d->frames == 0) return DY_ERR_INVALID_ARG;
+    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, 1, 1));
+    if (adj_final->n != d->n_total || adj_initial->n != d->n_total) return DY_ERR_INVALID_ARG;
+    const hipStream_t hsm = hs(s);
+    const size_t n = d->n_total;
+    float *fin[13], *ini[13];
+    mutable_fields(*adj_final, fin);
+    mutable_fields(*adj_initial, ini);
+    if (fin[0] != ini[0]) DY_TRY_STATUS(copy_state(hsm, ini, fin, n));
+    DY_TRY(hipMemsetAsync(adj_initial->inv_mass, 0, n * sizeof(float), hsm));
+    dy_body_view work = *bodies;
+    set_mutable_fields(work, d->d_work, n);
+    float *wk[13];
+    mutable_fields(work, wk);
+    const uint32_t n_seg = (d->frames + d->every - 1) / d->every;
+    DiffArgs D{};
+    D.adj = *adj_initial;
+    D.cs = d->d_cs;
+    for (int k = static_cast<int>(n_seg) - 1; k >= 0; --k) {
+        const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
+        const uint32_t nf = std::min(d->every, d->frames - f0);
+        dy_body_view ck{};
+        set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * n, n);
+        float *cp[13];
+        mutable_fields(ck, cp);
+        DY_TRY_STATUS(copy_state(hsm, wk, cp, n));
+        DY_TRY_STATUS(batch_launch(d->b, hsm, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape,
+                                   d->d_tape_off));
+        BatchArgs a = batch_args(d->b, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf));
+        a.tape = d->d_tape;
+        a.tape_off = d->d_tape_off;
+        D.frames = nf;
+        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), 0, hsm, a, D);
+        DY_LAUNCH_CHECK();
+    }
     return DY_OK;
 }
 

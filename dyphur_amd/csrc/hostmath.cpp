@@ -3,6 +3,8 @@
 // can compare the exact code the kernels run against the oracle without a GPU.
 #include "adjoint.hpp"
 
+#include <vector>
+
 using namespace dy;
 
 extern "C" {
@@ -199,5 +201,191 @@ void ha_integrate_adj_d(const double *kin0, const double *g, double h, double *b
 }
 void ha_renorm_d(double *q) { renorm_t(q); }
 void ha_renorm_adj_d(const double *q, double *qb) { renorm_adj(q, qb); }
+
+}  // extern "C"
+
+// ---- a host reference of whole frames, float64 or forward-mode -------------
+namespace {
+
+template <class S> S val(S x) { return x; }
+template <class S, int N> S val(const Dual<S, N> &x) { return x.v; }
+
+// frames of one scene: st[13][n] SoA state (pos, rot, vel, ang), im[n],
+// I6[6][n], per-body shape type and half extents hs[3][n].  The frame of the
+// stress driver, the pair set from binary16 boxes of the float-rounded state,
+// contacts in pair order, reference contact order for <= 256 contacts (or
+// forced) and Jones-Plassmann colours otherwise (with the reference's
+// fallbacks).
+template <class T, class S>
+void host_frames(int n, T *st, const T *im, const double *I6, const uint32_t *stype, const double *hs_d,
+                 const uint32_t *flags, const double *g, double dt, int frames, int iters, double mu, int force_serial,
+                 uint32_t max_pairs, uint32_t max_contacts) {
+    auto F = [&](int f, int i) -> T & { return st[static_cast<size_t>(f) * n + i]; };
+    std::vector<S> hs(3 * static_cast<size_t>(n));
+    for (size_t i = 0; i < hs.size(); ++i) hs[i] = static_cast<S>(hs_d[i]);
+    auto body = [&](int i) {
+        BodyT<T> b;
+        for (int k = 0; k < 3; ++k) { b.p[k] = F(k, i); b.v[k] = F(7 + k, i); b.w[k] = F(10 + k, i); }
+        for (int k = 0; k < 4; ++k) b.q[k] = F(3 + k, i);
+        b.im = im[i];
+        for (int k = 0; k < 6; ++k) b.I[k] = T(static_cast<S>(I6[static_cast<size_t>(k) * n + i]));
+        return b;
+    };
+    auto put = [&](int i, const BodyT<T> &b) {
+        for (int k = 0; k < 3; ++k) { F(k, i) = b.p[k]; F(7 + k, i) = b.v[k]; F(10 + k, i) = b.w[k]; }
+        for (int k = 0; k < 4; ++k) F(3 + k, i) = b.q[k];
+    };
+    const T gT[3] = {T(static_cast<S>(g[0])), T(static_cast<S>(g[1])), T(static_cast<S>(g[2]))};
+    const T dtT = T(static_cast<S>(dt)), muT = T(static_cast<S>(mu));
+    for (int f = 0; f < frames; ++f) {
+        for (int i = 0; i < n; ++i) {
+            if (val(im[i]) == 0.0 || (flags[i] & kFlagSleeping)) continue;
+            BodyT<T> b = body(i);
+            integrate_t(b, gT, dtT);
+            put(i, b);
+        }
+        // pairs from the binary16 boxes of the float-rounded state
+        std::vector<float> bx(6 * n);
+        for (int i = 0; i < n; ++i) {
+            float ex, ey, ez;
+            leaf_extent(stype[i], static_cast<float>(hs[i]), static_cast<float>(hs[n + i]),
+                        static_cast<float>(hs[2 * n + i]), static_cast<float>(val(F(3, i))),
+                        static_cast<float>(val(F(4, i))), static_cast<float>(val(F(5, i))),
+                        static_cast<float>(val(F(6, i))), ex, ey, ez);
+            const float p[3] = {static_cast<float>(val(F(0, i))), static_cast<float>(val(F(1, i))),
+                                static_cast<float>(val(F(2, i)))};
+            const float e[3] = {ex, ey, ez};
+            for (int k = 0; k < 3; ++k) {
+                bx[6 * i + k] = f16_to_f32(f32_to_f16(p[k] - e[k]));
+                bx[6 * i + 3 + k] = f16_to_f32(f32_to_f16(p[k] + e[k]));
+            }
+        }
+        std::vector<std::pair<int, int>> pairs;
+        for (int a = 0; a < n; ++a)
+            for (int b = a + 1; b < n; ++b) {
+                const float *A = &bx[6 * a], *B = &bx[6 * b];
+                const bool sep = B[0] > A[3] || B[3] < A[0] || B[1] > A[4] || B[4] < A[1] || B[2] > A[5] ||
+                                 B[5] < A[2];
+                if (!sep && pairs.size() < max_pairs) pairs.emplace_back(a, b);
+            }
+        std::vector<ContactT<T>> con;
+        std::vector<std::pair<int, int>> cab;
+        for (auto &pr : pairs) {
+            const int a = pr.first, b = pr.second;
+            const S ha[3] = {hs[a], hs[n + a], hs[2 * n + a]}, hb[3] = {hs[b], hs[n + b], hs[2 * n + b]};
+            const T pa[3] = {F(0, a), F(1, a), F(2, a)}, pb[3] = {F(0, b), F(1, b), F(2, b)};
+            const T qa[4] = {F(3, a), F(4, a), F(5, a), F(6, a)}, qb[4] = {F(3, b), F(4, b), F(5, b), F(6, b)};
+            HitT<T> h;
+            if (!pair_contact_t(stype[a], stype[b], ha, hb, pa, qa, pb, qb, h, -1, nullptr)) continue;
+            if (con.size() >= max_contacts) continue;
+            ContactT<T> c;
+            c.p[0] = h.px; c.p[1] = h.py; c.p[2] = h.pz;
+            c.n[0] = h.nx; c.n[1] = h.ny; c.n[2] = h.nz;
+            c.depth = h.depth;
+            con.push_back(c);
+            cab.emplace_back(a, b);
+        }
+        const int nc = static_cast<int>(con.size());
+        if (nc == 0) continue;
+        // solve order: reference order, or colours
+        std::vector<int> order(nc);
+        for (int k = 0; k < nc; ++k) order[k] = k;
+        if (!(force_serial || nc <= 256)) {
+            std::vector<int> col(nc, -1);
+            std::vector<uint64_t> mask(n, 0ull);
+            int left = nc, rounds = 0;
+            bool over = false;
+            while (left > 0 && rounds < 256 && !over) {
+                ++rounds;
+                std::vector<uint64_t> best(n, 0ull);
+                for (int k = 0; k < nc; ++k) {
+                    if (col[k] >= 0) continue;
+                    const uint64_t pk = jp_priority(static_cast<uint32_t>(k));
+                    for (int e : {cab[k].first, cab[k].second})
+                        if (val(im[e]) != 0.0 && pk > best[e]) best[e] = pk;
+                }
+                std::vector<int> win;
+                for (int k = 0; k < nc; ++k) {
+                    if (col[k] >= 0) continue;
+                    const uint64_t pk = jp_priority(static_cast<uint32_t>(k));
+                    bool ok = true;
+                    for (int e : {cab[k].first, cab[k].second})
+                        if (val(im[e]) != 0.0 && best[e] != pk) ok = false;
+                    if (ok) win.push_back(k);
+                }
+                for (int k : win) {
+                    uint64_t used = 0;
+                    for (int e : {cab[k].first, cab[k].second})
+                        if (val(im[e]) != 0.0) used |= mask[e];
+                    if (used == ~0ull) { over = true; continue; }
+                    const int c = __builtin_ctzll(~used);
+                    col[k] = c;
+                    for (int e : {cab[k].first, cab[k].second})
+                        if (val(im[e]) != 0.0) mask[e] |= 1ull << c;
+                    --left;
+                }
+            }
+            if (left == 0 && !over) {
+                std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return col[x] < col[y]; });
+            }
+        }
+        std::vector<T> lam(nc, T(S(0)));
+        for (int it = 0; it < iters; ++it) {
+            for (int k : order) {
+                BodyT<T> A = body(cab[k].first), B = body(cab[k].second);
+                if (project_t(con[k], A, B, muT, lam[k], dtT)) {
+                    if (val(A.im) != 0.0) put(cab[k].first, A);
+                    if (val(B.im) != 0.0) put(cab[k].second, B);
+                }
+            }
+            for (int i = 0; i < n; ++i) {
+                T q[4] = {F(3, i), F(4, i), F(5, i), F(6, i)};
+                renorm_t(q);
+                for (int k = 0; k < 4; ++k) F(3 + k, i) = q[k];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// float64 frames of one scene, state in place (13 x n SoA).
+void ha_sim_d(int n, double *st, const double *im, const double *I6, const uint32_t *stype, const double *hs,
+              const uint32_t *flags, const double *g, double dt, int frames, int iters, double mu, int force_serial,
+              uint32_t max_pairs, uint32_t max_contacts) {
+    host_frames<double, double>(n, st, im, I6, stype, hs, flags, g, dt, frames, iters, mu, force_serial, max_pairs,
+                                max_contacts);
+}
+
+// Forward mode: the same frames carrying one tangent; st_dot[13 x n] and
+// im_dot[n] seed it, st_dot receives the final tangent.
+void ha_sim_dir_d(int n, double *st, double *st_dot, const double *im, const double *im_dot, const double *I6,
+                  const uint32_t *stype, const double *hs, const uint32_t *flags, const double *g, double dt,
+                  int frames, int iters, double mu, int force_serial, uint32_t max_pairs, uint32_t max_contacts) {
+    typedef Dual<double, 1> D;
+    std::vector<D> s(13 * static_cast<size_t>(n)), m(n);
+    for (size_t i = 0; i < s.size(); ++i) { s[i] = D(st[i]); s[i].d[0] = st_dot[i]; }
+    for (int i = 0; i < n; ++i) { m[i] = D(im[i]); m[i].d[0] = im_dot[i]; }
+    host_frames<D, double>(n, s.data(), m.data(), I6, stype, hs, flags, g, dt, frames, iters, mu, force_serial,
+                           max_pairs, max_contacts);
+    for (size_t i = 0; i < s.size(); ++i) { st[i] = s[i].v; st_dot[i] = s[i].d[0]; }
+}
+
+// The same in float: values follow the gfx950 kernels' float arithmetic
+// (identical operations), so this is the float forward-mode derivative of
+// the exact trajectory the kernels compute.
+void ha_sim_dir_f(int n, double *st, double *st_dot, const double *im, const double *im_dot, const double *I6,
+                  const uint32_t *stype, const double *hs, const uint32_t *flags, const double *g, double dt,
+                  int frames, int iters, double mu, int force_serial, uint32_t max_pairs, uint32_t max_contacts) {
+    typedef Dual<float, 1> D;
+    std::vector<D> s(13 * static_cast<size_t>(n)), m(n);
+    for (size_t i = 0; i < s.size(); ++i) { s[i] = D(static_cast<float>(st[i])); s[i].d[0] = static_cast<float>(st_dot[i]); }
+    for (int i = 0; i < n; ++i) { m[i] = D(static_cast<float>(im[i])); m[i].d[0] = static_cast<float>(im_dot[i]); }
+    host_frames<D, float>(n, s.data(), m.data(), I6, stype, hs, flags, g, dt, frames, iters, mu, force_serial,
+                          max_pairs, max_contacts);
+    for (size_t i = 0; i < s.size(); ++i) { st[i] = s[i].v; st_dot[i] = s[i].d[0]; }
+}
 
 }  // extern "C"

@@ -378,6 +378,46 @@ def batched_piles(n_scenes: int = 4096, seed: int = 20260003, median: float = 64
     return [pile_scene(int(c), rng) for c in counts]
 
 
+def diff_stack(rng: Pcg32, n_side: int = 4, half: float = 0.2, pitch: float = 0.45, lift: float = 1.0,
+               v_sigma: float = 1.0, w_sigma: float = 2.0, target_sigma: float = 0.5) -> Scene:
+    """A differentiable-form scene: n_side^3 boxes (half extent `half`, 1 kg)
+    on a `pitch` grid whose lowest layer is `lift` m above the ground's top,
+    thrown with linear velocity N(0, v_sigma) and angular velocity
+    N(0, w_sigma) per axis; the static ground (half 10 x 0.5 x 10) last.
+    meta["target"] holds the loss targets x* = x0 + N(0, target_sigma)."""
+    n_dyn = n_side ** 3
+    n = n_dyn + 1
+    b = _alloc(n)
+    inertia = 1.0 / 3.0 * (2.0 * half * half)
+    k = 0
+    for iy in range(n_side):
+        for ix in range(n_side):
+            for iz in range(n_side):
+                b["pos"][k] = ((ix - (n_side - 1) / 2.0) * pitch, 0.5 + lift + half + iy * pitch,
+                               (iz - (n_side - 1) / 2.0) * pitch)
+                b["inertia"][k] = _diag(inertia)
+                k += 1
+    for k in range(n_dyn):
+        b["vel"][k] = (v_sigma * rng.normal(), v_sigma * rng.normal(), v_sigma * rng.normal())
+        b["ang"][k] = (w_sigma * rng.normal(), w_sigma * rng.normal(), w_sigma * rng.normal())
+    target = b["pos"][:n_dyn].astype(np.float64) + np.array(
+        [[target_sigma * rng.normal() for _ in range(3)] for _ in range(n_dyn)])
+    shape_type = np.zeros(2, np.uint32)
+    shape_half = np.array([[half, half, half], [10.0, 0.5, 10.0]], F32)
+    b["shape"][:n_dyn] = 0
+    b["shape"][n_dyn] = 1
+    b["mass"][n_dyn] = 0.0
+    b["flags"][n_dyn] = FLAG_STATIC
+    return Scene(name=f"diff_stack_{n_dyn}", shape_type=shape_type, shape_half=shape_half,
+                 bounds=np.array([-15, -2, -15, 15, 12, 15], F32), max_pairs=max(256, 48 * n),
+                 max_contacts=max(64, 16 * n), frames=120, meta={"target": target}, **b)
+
+
+def diff_batch(n_scenes: int = 256, seed: int = 20260004) -> list[Scene]:
+    rng = Pcg32(seed)
+    return [diff_stack(rng) for _ in range(n_scenes)]
+
+
 def concat(scenes: list[Scene], name: str = "batch") -> Scene:
     """Scenes laid end to end for the batched step: body arrays concatenated,
     each body's shape index offset into the concatenated catalog.  meta holds

@@ -342,6 +342,35 @@ int dy_batch_layout_get(dy_batch_t b, dy_batch_layout *out);
 int dy_batch_set_timing(dy_batch_t b, dy_stream_t s, int enable);
 int dy_batch_get_timing(dy_batch_t b, dy_stream_t s, uint64_t *ticks, double *tick_hz);
 
+/* ---- differentiable batched form ------------------------------------------
+ * No reference counterpart.  Reverse-mode derivatives of a batch's frames
+ * (one substep per frame) with respect to the initial state and the
+ * inverse masses.  Forward: the batch's frames, the mutable state saved every
+ * checkpoint_every frames.  Backward: per checkpoint segment, newest first,
+ * the segment's frames are recomputed with a record of every intermediate
+ * the adjoints need, then reversed: hand-written adjoints of the integrator,
+ * the contact projection and the quaternion renormalisation, and the contact
+ * geometry's Jacobian with respect to the two poses.  The contact set, pair
+ * list and solve order are those of the forward (piecewise-smooth
+ * derivative).  Every scene must keep its state in LDS (at most 585 bodies).
+ * Statistics of the batch count the recomputed frames too. */
+typedef struct dy_diff_s *dy_diff_t;
+/* Asynchronous allocations are done here (blocking). */
+int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, dy_diff_t *out);
+int dy_diff_destroy(dy_diff_t d);
+/* n_frames (<= max_frames) frames of every scene from the state in bodies,
+ * which ends in the final state.  Asynchronous. */
+int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
+                    float gx, float gy, float gz, float dt, int n_frames);
+/* After dy_diff_forward: adj_final holds the loss's adjoint of the final
+ * state (its 13 mutable arrays: pos, rot, vel, ang); adj_initial receives the
+ * adjoint of the initial state in the same 13 arrays and the adjoint of the
+ * inverse masses in its inv_mass array (the two views may share arrays).
+ * Other fields of both views are unused.  Asynchronous. */
+int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
+                     float gx, float gy, float gz, float dt, const dy_body_view *adj_final,
+                     dy_body_view *adj_initial);
+
 /* FNV-1a-64 over pos xyz, rot wxyz of all bodies
  * (examples/stress_test_blocks/main.cpp:34-41,301-311); blocking. */
 int dy_state_hash_fnv1a64(dy_stream_t s, const dy_body_view *bodies, uint64_t *out);

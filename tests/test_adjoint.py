@@ -270,3 +270,66 @@ def test_contact_geometry_adjoint_matches_central_differences(hm):
         seen[kind] = seen.get(kind, 0) + 1
     # every dispatch branch and both box-box kinds (face of A / of B, edge) exercised
     assert {(0, 0, 0), (0, 0, 1), (1, 0, -1), (0, 1, -1), (1, 1, -1)} <= set(seen), seen
+
+
+# ---- the float64 / forward-mode frame reference used by the GPU gradient test
+def _sim_args(hm, sc):
+    L = hm
+    vp_, d, i = C.c_void_p, C.c_double, C.c_int
+    L.ha_sim_d.argtypes = [i] + [vp_] * 7 + [d, i, i, d, i, C.c_uint32, C.c_uint32]
+    L.ha_sim_dir_d.argtypes = [i] + [vp_] * 9 + [d, i, i, d, i, C.c_uint32, C.c_uint32]
+    from dyphur_amd import capi
+    inv, iI, fl = capi.body_dynamics(sc.mass, sc.inertia, sc.flags)
+    n = sc.n
+    st = np.concatenate([sc.pos.T, sc.rot.T, sc.vel.T, sc.ang.T]).astype(np.float64).ravel()
+    I6 = iI.T.astype(np.float64).ravel()
+    stype = np.ascontiguousarray(sc.shape_type[sc.shape], np.uint32)
+    hs = sc.shape_half[sc.shape].T.astype(np.float64).ravel()
+    return dict(n=n, st=st, im=inv.astype(np.float64), I6=I6, stype=stype, hs=hs,
+                flags=np.ascontiguousarray(fl, np.uint32), g=np.array(sc.gravity, np.float64))
+
+
+def _sim(hm, a, frames, st=None, im=None):
+    s = (a["st"] if st is None else st).copy()
+    m = a["im"] if im is None else im
+    hm.ha_sim_d(a["n"], P(s), P(m), P(a["I6"]), P(a["stype"]), P(a["hs"]), P(a["flags"]), P(a["g"]),
+                1 / 60, frames, 10, 0.5, 0, 1 << 20, 1 << 20)
+    return s
+
+
+def test_host_reference_tracks_the_oracle(hm, oracle):
+    from dyphur_amd import scenes
+    sc = scenes.diff_stack(scenes.Pcg32(7))
+    a = _sim_args(hm, sc)
+    ob, osh = oracle.Bodies(sc), oracle.Shapes(sc)
+    w = oracle.World.for_scene(sc)
+    s = a["st"].copy()
+    for f in range(30):
+        w.frame(ob, osh, sc)
+        s = _sim(hm, a, 1, st=s)
+    ref = ob.state()
+    n = sc.n
+    got = s.reshape(13, n)
+    for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+        assert np.abs(got[k] - ref[name]).max() < 1e-3, name
+
+
+def test_host_forward_mode_matches_central_differences(hm):
+    from dyphur_amd import scenes
+    sc = scenes.diff_stack(scenes.Pcg32(8))
+    a = _sim_args(hm, sc)
+    n = sc.n
+    rng = np.random.default_rng(9)
+    u_st = np.zeros(13 * n)
+    u_st[7 * n:10 * n] = rng.normal(size=3 * n)  # velocity directions
+    u_st[7 * n + n - 1::n] = 0.0                  # not the ground
+    u_im = np.concatenate([rng.normal(size=n - 1) * 0.1, [0.0]])
+    frames = 12
+    st, sd = a["st"].copy(), u_st.copy()
+    hm.ha_sim_dir_d(n, P(st), P(sd), P(a["im"]), P(u_im), P(a["I6"]), P(a["stype"]), P(a["hs"]),
+                    P(a["flags"]), P(a["g"]), 1 / 60, frames, 10, 0.5, 0, 1 << 20, 1 << 20)
+    eps = 1e-6
+    sp = _sim(hm, a, frames, st=a["st"] + eps * u_st, im=a["im"] + eps * u_im)
+    sm = _sim(hm, a, frames, st=a["st"] - eps * u_st, im=a["im"] - eps * u_im)
+    fd = (sp - sm) / (2 * eps)
+    assert np.abs(sd - fd).max() <= 1e-4 * max(1.0, np.abs(fd).max())

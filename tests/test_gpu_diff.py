@@ -1,0 +1,123 @@
+"""Differentiable batched form (dy_diff_*): gradients of a thrown-stack loss
+with respect to initial velocities and inverse masses, from the gfx950
+reverse sweep (float32), against a float64 host forward-mode reference of the
+same frames (tests/test_adjoint.py pins that reference against central
+differences and against the oracle's trajectory).
+
+Loss L = sum over dynamic bodies of |x_T - x*|^2 (config 4's loss).  Per
+scene, a random direction u over (v0, inverse mass) of its dynamic bodies:
+the GPU's u . grad L must equal the reference's dL/du within 1e-3 relative.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dyphur_amd import scenes
+from tests import _hostmath
+from tests.test_adjoint import P, _sim_args
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _gpu_grads(gpu, scs, frames, every):
+    b = gpu.Batch(scs)
+    d = gpu.Diff(b, frames, every)
+    d.forward(frames)
+    st = b.state()
+    n = b.n_bodies
+    adj = {k: np.zeros(n, np.float32) for k in ("pos_x", "pos_y", "pos_z")}
+    for i, sc in enumerate(scs):
+        sl = b.scene_slice(i)
+        nd = sc.n - 1
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+            x = st[name][sl][:nd].astype(np.float64)
+            adj[name][sl.start:sl.start + nd] = 2.0 * (x - sc.meta["target"][:, k])
+    g = d.backward(adj)
+    return b, st, g
+
+
+def _host_directional(hm, sc, frames, u_v, u_im, fn):
+    """dL/du by forward mode over the same frames (fn: the float64 or the
+    float reference); returns it and the final state (13 x n)."""
+    a = _sim_args(hm, sc)
+    n = sc.n
+    st, sd = a["st"].copy(), np.zeros(13 * n)
+    for k in range(3):
+        sd[(7 + k) * n:(8 + k) * n] = u_v[:, k]
+    im_dot = u_im.astype(np.float64)
+    fn(n, P(st), P(sd), P(a["im"]), P(im_dot), P(a["I6"]), P(a["stype"]), P(a["hs"]), P(a["flags"]), P(a["g"]),
+       sc.dt, frames, sc.n_iters, sc.mu, 0, sc.max_pairs, sc.max_contacts)
+    nd = n - 1
+    x = st.reshape(13, n)[0:3, :nd].T
+    xd = sd.reshape(13, n)[0:3, :nd].T
+    return float(np.sum(2.0 * (x - sc.meta["target"]) * xd)), st.reshape(13, n)
+
+
+@pytest.fixture(scope="module")
+def hm():
+    L = _hostmath.lib()
+    vp, d, i = C.c_void_p, C.c_double, C.c_int
+    L.ha_sim_dir_d.argtypes = [i] + [vp] * 9 + [d, i, i, d, i, C.c_uint32, C.c_uint32]
+    L.ha_sim_dir_f.argtypes = [i] + [vp] * 9 + [d, i, i, d, i, C.c_uint32, C.c_uint32]
+    return L
+
+
+def test_diff_forward_equals_batch_step(gpu):
+    scs = scenes.diff_batch(6, seed=11)
+    a = gpu.Batch(scs)
+    a.step(23)
+    b = gpu.Batch(scs)
+    d = gpu.Diff(b, 40, 8)
+    d.forward(23)
+    sa, sb = a.state(), b.state()
+    for k in gpu.MUTABLE_FIELDS:
+        assert np.array_equal(bits(sa[k]), bits(sb[k])), k
+
+
+@pytest.mark.parametrize("frames,every", [(30, 8), (60, 10)])
+def test_diff_gradient_matches_forward_mode(gpu, hm, frames, every):
+    """Against the float reference (the kernels' own trajectory, bit for bit):
+    the reverse sweep's u . grad L equals forward mode within 1e-3.  Against
+    the float64 reference the difference is the gradient's sensitivity to
+    float32 rounding of the trajectory: bounded by 1e-2 over the 30 frames
+    before the stacks reach the ground; after ground impact the derivative
+    grows by orders of magnitude (contact events) and float32 and float64
+    trajectories no longer share it, so it is only printed there."""
+    scs = scenes.diff_batch(4, seed=20260004)
+    print(f"[diff] {len(scs)} scenes frames={frames} checkpoint={every}", flush=True)
+    b, st, g = _gpu_grads(gpu, scs, frames, every)
+    rng = np.random.default_rng(5)
+    e32, e64 = [], []
+    for i, sc in enumerate(scs):
+        sl = b.scene_slice(i)
+        nd = sc.n - 1
+        u_v = rng.normal(size=(nd, 3))
+        u_im = np.concatenate([rng.normal(size=nd) * 0.2, [0.0]])
+        gv = np.stack([g["vel_x"][sl][:nd], g["vel_y"][sl][:nd], g["vel_z"][sl][:nd]], 1).astype(np.float64)
+        gpu_d = float(np.sum(gv * u_v) + np.sum(g["inv_mass"][sl].astype(np.float64) * u_im))
+        uv = np.vstack([u_v, np.zeros((1, 3))])
+        f32_d, f32_st = _host_directional(hm, sc, frames, uv, u_im, hm.ha_sim_dir_f)
+        f64_d, _ = _host_directional(hm, sc, frames, uv, u_im, hm.ha_sim_dir_d)
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z")):
+            assert np.array_equal(bits(f32_st[k].astype(np.float32)), bits(st[name][sl])), (i, name)
+        e32.append(abs(gpu_d - f32_d) / max(abs(f32_d), 1e-12))
+        e64.append(abs(gpu_d - f64_d) / max(abs(f64_d), 1e-12))
+        print(f"[diff] scene {i}: gpu {gpu_d:.6e} f32-fwd {f32_d:.6e} ({e32[-1]:.1e}) "
+              f"f64-fwd {f64_d:.6e} ({e64[-1]:.1e})", flush=True)
+    assert max(e32) <= 1e-3, e32
+    if frames <= 30:
+        assert max(e64) <= 1e-2, e64
+
+
+def test_diff_static_ground_gets_no_gradient(gpu):
+    scs = scenes.diff_batch(2, seed=3)
+    b, _, g = _gpu_grads(gpu, scs, 40, 10)
+    for i, sc in enumerate(scs):
+        ground = b.scene_slice(i).stop - 1
+        for k in g:
+            assert g[k][ground] == 0.0, k
