@@ -42,9 +42,11 @@ struct Dual {
     S v;
     S d[N];
     DY_HD Dual() : v(S(0)) {
+        #pragma unroll
         for (int i = 0; i < N; ++i) d[i] = S(0);
     }
     DY_HD Dual(S x) : v(x) {
+        #pragma unroll
         for (int i = 0; i < N; ++i) d[i] = S(0);
     }
 };
@@ -52,26 +54,31 @@ struct Dual {
 template <class S, int N> DY_HD S value(const Dual<S, N> &a) { return a.v; }
 template <class S, int N> DY_HD Dual<S, N> operator+(const Dual<S, N> &a, const Dual<S, N> &b) {
     Dual<S, N> r; r.v = a.v + b.v;
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
     return r;
 }
 template <class S, int N> DY_HD Dual<S, N> operator-(const Dual<S, N> &a, const Dual<S, N> &b) {
     Dual<S, N> r; r.v = a.v - b.v;
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
     return r;
 }
 template <class S, int N> DY_HD Dual<S, N> operator-(const Dual<S, N> &a) {
     Dual<S, N> r; r.v = -a.v;
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
     return r;
 }
 template <class S, int N> DY_HD Dual<S, N> operator*(const Dual<S, N> &a, const Dual<S, N> &b) {
     Dual<S, N> r; r.v = a.v * b.v;
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
     return r;
 }
 template <class S, int N> DY_HD Dual<S, N> operator/(const Dual<S, N> &a, const Dual<S, N> &b) {
     Dual<S, N> r; r.v = a.v / b.v;
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
     return r;
 }
@@ -98,6 +105,7 @@ template <class S, int N> DY_HD bool operator==(const Dual<S, N> &a, S b) { retu
 template <class S, int N> DY_HD bool operator!=(const Dual<S, N> &a, const Dual<S, N> &b) { return a.v != b.v; }
 template <class S, int N> DY_HD Dual<S, N> sq_root(const Dual<S, N> &a) {
     Dual<S, N> r; r.v = sq_root(a.v);
+    #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / (S(2) * r.v);
     return r;
 }
@@ -109,6 +117,21 @@ template <class S, int N> DY_HD Dual<S, N> absval(const Dual<S, N> &a) { return 
 // ---------------------------------------------------------------------------
 template <class T> struct M3T { T m[3][3]; };
 template <class T> struct HitT { T px, py, pz, nx, ny, nz, depth; };
+
+// Element i of a 3-array and row i of a matrix for a run-time i, by selects:
+// no run-time indexing, so everything stays in registers.
+template <class T> DY_HD T pick3(const T a[3], int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+template <class T> DY_HD T pickm(const M3T<T> &M, int r, int c) {
+    return r == 0 ? M.m[0][c] : (r == 1 ? M.m[1][c] : M.m[2][c]);
+}
+template <class T> DY_HD M3T<T> pick_mat(bool first, const M3T<T> &A, const M3T<T> &B) {
+    M3T<T> R;
+    #pragma unroll
+    for (int i = 0; i < 3; ++i)
+        #pragma unroll
+        for (int j = 0; j < 3; ++j) R.m[i][j] = first ? A.m[i][j] : B.m[i][j];
+    return R;
+}
 
 template <class T, class S>
 DY_HD M3T<T> axes_t(const T &qw, const T &qx, const T &qy, const T &qz, S one) {
@@ -148,18 +171,21 @@ DY_HD bool sphere_box_t(const T &sx, const T &sy, const T &sz, S r, const T &cx,
                         const M3T<T> &E, const S he[3], HitT<T> &h) {
     const T dx = sx - cx, dy = sy - cy, dz = sz - cz;
     T q[3], cl[3];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) q[a] = dx * E.m[a][0] + dy * E.m[a][1] + dz * E.m[a][2];
+    #pragma unroll
     for (int a = 0; a < 3; ++a) cl[a] = q[a] < -he[a] ? T(-he[a]) : (q[a] > he[a] ? T(he[a]) : q[a]);
     const T lx = q[0] - cl[0], ly = q[1] - cl[1], lz = q[2] - cl[2];
     const T d2 = lx * lx + ly * ly + lz * lz;
     if (d2 < S(1e-12f)) {
         T pen[3];
+        #pragma unroll
         for (int a = 0; a < 3; ++a) pen[a] = he[a] - absval(q[a]);
         const int f = (pen[0] <= pen[1] && pen[0] <= pen[2]) ? 0 : (pen[1] <= pen[2] ? 1 : 2);
-        const T qf = q[f];
-        const T pf = pen[f];
+        const T qf = pick3(q, f);
+        const T pf = pick3(pen, f);
         const S sg = qf >= S(0) ? S(1) : S(-1);
-        h.nx = -(sg * E.m[f][0]); h.ny = -(sg * E.m[f][1]); h.nz = -(sg * E.m[f][2]);
+        h.nx = -(sg * pickm(E, f, 0)); h.ny = -(sg * pickm(E, f, 1)); h.nz = -(sg * pickm(E, f, 2));
         h.depth = r + pf;
         h.px = sx - h.nx * r; h.py = sy - h.ny * r; h.pz = sz - h.nz * r;
         return true;
@@ -207,6 +233,22 @@ DY_HD bool sat_axis_t(SatT<T> &s, const M3T<T> &A, const S ah[3], const M3T<T> &
     return true;
 }
 
+// Edge-edge axis A_i x B_j of the separating-axis test (skipped, as a pass,
+// when the edges are parallel).
+template <class T, class S>
+DY_HD bool edge_axis_t(SatT<T> &s, const M3T<T> &A, const S ah[3], const M3T<T> &B, const S bh[3], const T &dx,
+                       const T &dy, const T &dz, int i, int j, S kPar) {
+    const T a0 = pickm(A, i, 0), a1 = pickm(A, i, 1), a2 = pickm(A, i, 2);
+    const T b0 = pickm(B, j, 0), b1 = pickm(B, j, 1), b2 = pickm(B, j, 2);
+    const T cx = a1 * b2 - a2 * b1;
+    const T cy = a2 * b0 - a0 * b2;
+    const T cz = a0 * b1 - a1 * b0;
+    const T l2 = cx * cx + cy * cy + cz * cz;
+    if (l2 < kPar * kPar) return true;
+    const T il = S(1) / sq_root(l2);
+    return sat_axis_t(s, A, ah, B, bh, dx, dy, dz, cx * il, cy * il, cz * il, 2, i, j);
+}
+
 // Box-box contact (dy_math.hpp box_box).  axis < 0: full separating-axis
 // search; otherwise only the axis type*16 + i*4 + j is evaluated (the one the
 // search picked).  *winner (may be null) receives the axis code.
@@ -217,33 +259,32 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
     const S kPar = S(1e-5f);
     const T dx = bcx - acx, dy = bcy - acy, dz = bcz - acz;
     T C[3][3];
+    #pragma unroll
     for (int i = 0; i < 3; ++i)
+        #pragma unroll
         for (int j = 0; j < 3; ++j)
             C[i][j] = A.m[i][0] * B.m[j][0] + A.m[i][1] * B.m[j][1] + A.m[i][2] * B.m[j][2];
     SatT<T> s{T(S(1e30f)), -1, 0, 0, T(S(0)), T(S(1)), T(S(0))};
-    auto edge = [&](int i, int j) -> bool {
-        const T cx = A.m[i][1] * B.m[j][2] - A.m[i][2] * B.m[j][1];
-        const T cy = A.m[i][2] * B.m[j][0] - A.m[i][0] * B.m[j][2];
-        const T cz = A.m[i][0] * B.m[j][1] - A.m[i][1] * B.m[j][0];
-        const T l2 = cx * cx + cy * cy + cz * cz;
-        if (l2 < kPar * kPar) return true;
-        const T il = S(1) / sq_root(l2);
-        return sat_axis_t(s, A, ah, B, bh, dx, dy, dz, cx * il, cy * il, cz * il, 2, i, j);
-    };
     if (axis < 0) {
+        #pragma unroll
         for (int i = 0; i < 3; ++i)
             if (!sat_axis_t(s, A, ah, B, bh, dx, dy, dz, A.m[i][0], A.m[i][1], A.m[i][2], 0, i, 0)) return false;
+        #pragma unroll
         for (int j = 0; j < 3; ++j)
             if (!sat_axis_t(s, A, ah, B, bh, dx, dy, dz, B.m[j][0], B.m[j][1], B.m[j][2], 1, j, 0)) return false;
+        #pragma unroll
         for (int i = 0; i < 3; ++i)
+            #pragma unroll
             for (int j = 0; j < 3; ++j)
-                if (!edge(i, j)) return false;
+                if (!edge_axis_t(s, A, ah, B, bh, dx, dy, dz, i, j, kPar)) return false;
     } else {
         const int ty = axis >> 4, ai = (axis >> 2) & 3, aj = axis & 3;
         bool ok = true;
-        if (ty == 0) ok = sat_axis_t(s, A, ah, B, bh, dx, dy, dz, A.m[ai][0], A.m[ai][1], A.m[ai][2], 0, ai, 0);
-        else if (ty == 1) ok = sat_axis_t(s, A, ah, B, bh, dx, dy, dz, B.m[ai][0], B.m[ai][1], B.m[ai][2], 1, ai, 0);
-        else ok = edge(ai, aj);
+        if (ty == 0)
+            ok = sat_axis_t(s, A, ah, B, bh, dx, dy, dz, pickm(A, ai, 0), pickm(A, ai, 1), pickm(A, ai, 2), 0, ai, 0);
+        else if (ty == 1)
+            ok = sat_axis_t(s, A, ah, B, bh, dx, dy, dz, pickm(B, ai, 0), pickm(B, ai, 1), pickm(B, ai, 2), 1, ai, 0);
+        else ok = edge_axis_t(s, A, ah, B, bh, dx, dy, dz, ai, aj, kPar);
         if (!ok || s.type < 0) return false;
     }
     if (winner) *winner = s.type * 16 + s.i * 4 + s.j;
@@ -254,25 +295,31 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
         int type = s.type, ref = s.i;
         if (type == 0) {
             const int ta0 = (ref + 1) % 3, ta1 = (ref + 2) % 3;
+            T cr[3];
+            #pragma unroll
+            for (int j = 0; j < 3; ++j) cr[j] = ref == 0 ? C[0][j] : (ref == 1 ? C[1][j] : C[2][j]);
             int jb = 0;
-            for (int j = 1; j < 3; ++j)
-                if (absval(C[ref][j]) > absval(C[ref][jb])) jb = j;
+            if (absval(cr[1]) > absval(cr[0])) jb = 1;
+            if (absval(cr[2]) > absval(pick3(cr, jb))) jb = 2;
             const int tb0 = (jb + 1) % 3, tb1 = (jb + 2) % 3;
-            if (bh[tb0] * bh[tb1] > ah[ta0] * ah[ta1]) { type = 1; ref = jb; }
+            if (pick3(bh, tb0) * pick3(bh, tb1) > pick3(ah, ta0) * pick3(ah, ta1)) { type = 1; ref = jb; }
         }
         const bool a_ref = type == 0;
-        const M3T<T> &R = a_ref ? A : B;
-        const M3T<T> &I = a_ref ? B : A;
-        const S *rh = a_ref ? ah : bh;
-        const S *ih = a_ref ? bh : ah;
+        const M3T<T> R = pick_mat(a_ref, A, B);
+        const M3T<T> I = pick_mat(a_ref, B, A);
+        S rh[3], ih[3];
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) { rh[c] = a_ref ? ah[c] : bh[c]; ih[c] = a_ref ? bh[c] : ah[c]; }
         const T rcx = a_ref ? acx : bcx, rcy = a_ref ? acy : bcy, rcz = a_ref ? acz : bcz;
         const T icx = a_ref ? bcx : acx, icy = a_ref ? bcy : acy, icz = a_ref ? bcz : acz;
         const int t0 = (ref + 1) % 3, t1 = (ref + 2) % 3;
         const T ref_n = rcx * nx + rcy * ny + rcz * nz;
         const S sdir = a_ref ? S(1) : S(-1);
-        const S rh_ref = rh[ref], rh_t0 = rh[t0], rh_t1 = rh[t1];
+        const S rh_ref = pick3(rh, ref), rh_t0 = pick3(rh, t0), rh_t1 = pick3(rh, t1);
         T r0[3], r1[3];
-        for (int c = 0; c < 3; ++c) { r0[c] = R.m[t0][c]; r1[c] = R.m[t1][c]; }
+        #pragma unroll
+        for (int c = 0; c < 3; ++c) { r0[c] = pickm(R, t0, c); r1[c] = pickm(R, t1, c); }
+        #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const S s0 = (k & 1) ? S(1) : S(-1), s1 = (k & 2) ? S(1) : S(-1), s2 = (k & 4) ? S(1) : S(-1);
             const T vx = icx + s0 * ih[0] * I.m[0][0] + s1 * ih[1] * I.m[1][0] + s2 * ih[2] * I.m[2][0];
@@ -297,29 +344,40 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
     } else {
         const int i = s.i, j = s.j;
         const int ta0 = (i + 1) % 3, ta1 = (i + 2) % 3;
-        const S sa0 = (-nx * A.m[ta0][0] - ny * A.m[ta0][1] - nz * A.m[ta0][2]) >= S(0) ? S(1) : S(-1);
-        const S sa1 = (-nx * A.m[ta1][0] - ny * A.m[ta1][1] - nz * A.m[ta1][2]) >= S(0) ? S(1) : S(-1);
-        const T pax = acx + sa0 * ah[ta0] * A.m[ta0][0] + sa1 * ah[ta1] * A.m[ta1][0];
-        const T pay = acy + sa0 * ah[ta0] * A.m[ta0][1] + sa1 * ah[ta1] * A.m[ta1][1];
-        const T paz = acz + sa0 * ah[ta0] * A.m[ta0][2] + sa1 * ah[ta1] * A.m[ta1][2];
+        T A0[3], A1[3], Ai[3], B0[3], B1[3], Bj[3];
+        #pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            A0[k] = pickm(A, ta0, k); A1[k] = pickm(A, ta1, k); Ai[k] = pickm(A, i, k);
+        }
+        const S sa0 = (-nx * A0[0] - ny * A0[1] - nz * A0[2]) >= S(0) ? S(1) : S(-1);
+        const S sa1 = (-nx * A1[0] - ny * A1[1] - nz * A1[2]) >= S(0) ? S(1) : S(-1);
+        const S ah0 = pick3(ah, ta0), ah1 = pick3(ah, ta1);
+        const T pax = acx + sa0 * ah0 * A0[0] + sa1 * ah1 * A1[0];
+        const T pay = acy + sa0 * ah0 * A0[1] + sa1 * ah1 * A1[1];
+        const T paz = acz + sa0 * ah0 * A0[2] + sa1 * ah1 * A1[2];
         const int tb0 = (j + 1) % 3, tb1 = (j + 2) % 3;
-        const S sb0 = (nx * B.m[tb0][0] + ny * B.m[tb0][1] + nz * B.m[tb0][2]) >= S(0) ? S(1) : S(-1);
-        const S sb1 = (nx * B.m[tb1][0] + ny * B.m[tb1][1] + nz * B.m[tb1][2]) >= S(0) ? S(1) : S(-1);
-        const T pbx = bcx + sb0 * bh[tb0] * B.m[tb0][0] + sb1 * bh[tb1] * B.m[tb1][0];
-        const T pby = bcy + sb0 * bh[tb0] * B.m[tb0][1] + sb1 * bh[tb1] * B.m[tb1][1];
-        const T pbz = bcz + sb0 * bh[tb0] * B.m[tb0][2] + sb1 * bh[tb1] * B.m[tb1][2];
-        const T c = C[i][j];
+        #pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            B0[k] = pickm(B, tb0, k); B1[k] = pickm(B, tb1, k); Bj[k] = pickm(B, j, k);
+        }
+        const S sb0 = (nx * B0[0] + ny * B0[1] + nz * B0[2]) >= S(0) ? S(1) : S(-1);
+        const S sb1 = (nx * B1[0] + ny * B1[1] + nz * B1[2]) >= S(0) ? S(1) : S(-1);
+        const S bh0 = pick3(bh, tb0), bh1 = pick3(bh, tb1);
+        const T pbx = bcx + sb0 * bh0 * B0[0] + sb1 * bh1 * B1[0];
+        const T pby = bcy + sb0 * bh0 * B0[1] + sb1 * bh1 * B1[1];
+        const T pbz = bcz + sb0 * bh0 * B0[2] + sb1 * bh1 * B1[2];
+        const T c = Ai[0] * Bj[0] + Ai[1] * Bj[1] + Ai[2] * Bj[2];
         const T den = S(1) - c * c;
         const T wx = pax - pbx, wy = pay - pby, wz = paz - pbz;
-        const T d1 = wx * A.m[i][0] + wy * A.m[i][1] + wz * A.m[i][2];
-        const T d2 = wx * B.m[j][0] + wy * B.m[j][1] + wz * B.m[j][2];
+        const T d1 = wx * Ai[0] + wy * Ai[1] + wz * Ai[2];
+        const T d2 = wx * Bj[0] + wy * Bj[1] + wz * Bj[2];
         T u = T(S(0)), v = T(S(0));
         if (absval(den) > S(1e-6f)) {
             u = (c * d2 - d1) / den;
             v = (d2 - c * d1) / den;
         }
-        const T qax = pax + u * A.m[i][0], qay = pay + u * A.m[i][1], qaz = paz + u * A.m[i][2];
-        const T qbx = pbx + v * B.m[j][0], qby = pby + v * B.m[j][1], qbz = pbz + v * B.m[j][2];
+        const T qax = pax + u * Ai[0], qay = pay + u * Ai[1], qaz = paz + u * Ai[2];
+        const T qbx = pbx + v * Bj[0], qby = pby + v * Bj[1], qbz = pbz + v * Bj[2];
         sx += S(0.5f) * (qax + qbx);
         sy += S(0.5f) * (qay + qby);
         sz += S(0.5f) * (qaz + qbz);
@@ -372,28 +430,180 @@ DY_HD bool contact_adj(uint32_t ta, uint32_t tb, const S ha[3], const S hb[3], c
     }
     for (int k0 = 0; k0 < 14; k0 += NT) {
         D dpa[3], dqa[4], dpb[3], dqb[4];
+        #pragma unroll
         for (int k = 0; k < 3; ++k) dpa[k] = D(pa[k]);
+        #pragma unroll
         for (int k = 0; k < 4; ++k) dqa[k] = D(qa[k]);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) dpb[k] = D(pb[k]);
+        #pragma unroll
         for (int k = 0; k < 4; ++k) dqb[k] = D(qb[k]);
-        for (int t = 0; t < NT && k0 + t < 14; ++t) {
-            const int k = k0 + t;
-            if (k < 3) dpa[k].d[t] = S(1);
-            else if (k < 7) dqa[k - 3].d[t] = S(1);
-            else if (k < 10) dpb[k - 7].d[t] = S(1);
-            else dqb[k - 10].d[t] = S(1);
+        #pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            #pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                dpa[k].d[t] = k == k0 + t ? S(1) : S(0);
+                dpb[k].d[t] = k + 7 == k0 + t ? S(1) : S(0);
+            }
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                dqa[k].d[t] = k + 3 == k0 + t ? S(1) : S(0);
+                dqb[k].d[t] = k + 10 == k0 + t ? S(1) : S(0);
+            }
         }
         HitT<D> h;
         if (!pair_contact_t(ta, tb, ha, hb, dpa, dqa, dpb, dqb, h, axis, nullptr)) return false;
-        const D *out[7] = {&h.px, &h.py, &h.pz, &h.nx, &h.ny, &h.nz, &h.depth};
-        for (int t = 0; t < NT && k0 + t < 14; ++t) {
-            const int k = k0 + t;
-            S acc = S(0);
-            for (int o = 0; o < 7; ++o) acc += hbar[o] * out[o]->d[t];
-            if (k < 7) ga[k] += acc;
-            else gb[k - 7] += acc;
+        #pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const S acc = hbar[0] * h.px.d[t] + hbar[1] * h.py.d[t] + hbar[2] * h.pz.d[t] + hbar[3] * h.nx.d[t] +
+                          hbar[4] * h.ny.d[t] + hbar[5] * h.nz.d[t] + hbar[6] * h.depth.d[t];
+            #pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                ga[k] += k == k0 + t ? acc : S(0);
+                gb[k] += k + 7 == k0 + t ? acc : S(0);
+            }
         }
     }
+    return true;
+}
+
+// Reverse mode for the box-box face contact (the usual case of resting
+// boxes), written out: the manifold's point and depth are means over the kept
+// incident vertices v_k = c_I + sum_m s_km h_m I_m, and the normal is a signed
+// row of one box's axes, so the adjoint is linear in the kept vertices.  The
+// float search and the manifold's decisions (winning axis, reference-face
+// swap, kept vertices) are recomputed with the forward's arithmetic.  Returns
+// false for any other contact kind (the caller falls back to contact_adj).
+template <class S>
+DY_HD bool box_face_adj(const S ha[3], const S hb[3], const S pa[3], const S qa[4], const S pb[3], const S qb[4],
+                        const S hbar[7], S ga[7], S gb[7]) {
+    const S one = S(1);
+    const M3T<S> A = axes_t(qa[0], qa[1], qa[2], qa[3], one);
+    const M3T<S> B = axes_t(qb[0], qb[1], qb[2], qb[3], one);
+    int axis = -1;
+    {
+        HitT<S> h;
+        if (!box_box_t(pa[0], pa[1], pa[2], A, ha, pb[0], pb[1], pb[2], B, hb, h, -1, &axis)) return false;
+    }
+    const int ty = axis >> 4, ai = (axis >> 2) & 3;
+    if (ty == 2) return false;
+    // normal: sg * row ai of A (ty 0) or of B (ty 1), sign from the centre offset
+    const S dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
+    S l[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) l[c] = ty == 0 ? pickm(A, ai, c) : pickm(B, ai, c);
+    const S sg = (dx * l[0] + dy * l[1] + dz * l[2]) >= S(0) ? S(-1) : S(1);
+    const S n[3] = {sg * l[0], sg * l[1], sg * l[2]};
+    // reference-face swap (box_box_t)
+    int type = ty, ref = ai;
+    if (type == 0) {
+        const int ta0 = (ref + 1) % 3, ta1 = (ref + 2) % 3;
+        S cr[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            cr[j] = pickm(A, ref, 0) * B.m[j][0] + pickm(A, ref, 1) * B.m[j][1] + pickm(A, ref, 2) * B.m[j][2];
+        int jb = 0;
+        if (absval(cr[1]) > absval(cr[0])) jb = 1;
+        if (absval(cr[2]) > absval(pick3(cr, jb))) jb = 2;
+        const int tb0 = (jb + 1) % 3, tb1 = (jb + 2) % 3;
+        if (pick3(hb, tb0) * pick3(hb, tb1) > pick3(ha, ta0) * pick3(ha, ta1)) { type = 1; ref = jb; }
+    }
+    const bool a_ref = type == 0;
+    const M3T<S> R = pick_mat(a_ref, A, B);
+    const M3T<S> I = pick_mat(a_ref, B, A);
+    S rh[3], ih[3], rc[3], ic[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        rh[c] = a_ref ? ha[c] : hb[c];
+        ih[c] = a_ref ? hb[c] : ha[c];
+        rc[c] = a_ref ? pa[c] : pb[c];
+        ic[c] = a_ref ? pb[c] : pa[c];
+    }
+    const int t0 = (ref + 1) % 3, t1 = (ref + 2) % 3;
+    const S ref_n = rc[0] * n[0] + rc[1] * n[1] + rc[2] * n[2];
+    const S sdir = a_ref ? S(1) : S(-1);
+    const S rh_ref = pick3(rh, ref), rh_t0 = pick3(rh, t0), rh_t1 = pick3(rh, t1);
+    S r0[3], r1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { r0[c] = pickm(R, t0, c); r1[c] = pickm(R, t1, c); }
+    // forward decisions: kept vertices (first four in k order) and their dv
+    uint32_t kept = 0;
+    S dvk[8];
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        dvk[k] = S(0);
+        const S s0 = (k & 1) ? S(1) : S(-1), s1 = (k & 2) ? S(1) : S(-1), s2 = (k & 4) ? S(1) : S(-1);
+        const S vx = ic[0] + s0 * ih[0] * I.m[0][0] + s1 * ih[1] * I.m[1][0] + s2 * ih[2] * I.m[2][0];
+        const S vy = ic[1] + s0 * ih[0] * I.m[0][1] + s1 * ih[1] * I.m[1][1] + s2 * ih[2] * I.m[2][1];
+        const S vz = ic[2] + s0 * ih[0] * I.m[0][2] + s1 * ih[1] * I.m[1][2] + s2 * ih[2] * I.m[2][2];
+        const S vn = vx * n[0] + vy * n[1] + vz * n[2];
+        const S dv = rh_ref + sdir * (vn - ref_n);
+        if (dv <= S(0)) continue;
+        const S ux = vx - rc[0], uy = vy - rc[1], uz = vz - rc[2];
+        const S q0 = ux * r0[0] + uy * r0[1] + uz * r0[2];
+        const S q1 = ux * r1[0] + uy * r1[1] + uz * r1[2];
+        if (absval(q0) > rh_t0 + S(1e-3f)) continue;
+        if (absval(q1) > rh_t1 + S(1e-3f)) continue;
+        if (m < 4) {
+            kept |= 1u << k;
+            dvk[k] = dv;
+            ++m;
+        }
+    }
+    if (m == 0) return false;
+    // reverse: p = mean(v - sdir dv n), depth = mean(dv)
+    const S inv = S(1) / static_cast<S>(m);
+    const S sb[3] = {hbar[0] * inv, hbar[1] * inv, hbar[2] * inv};
+    const S sdb = hbar[6] * inv;
+    const S sbn = sb[0] * n[0] + sb[1] * n[1] + sb[2] * n[2];
+    S nb[3] = {hbar[3], hbar[4], hbar[5]}, icb[3] = {S(0), S(0), S(0)}, rcb[3] = {S(0), S(0), S(0)};
+    M3T<S> Ib{};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!((kept >> k) & 1u)) continue;
+        const S sk[3] = {(k & 1) ? S(1) : S(-1), (k & 2) ? S(1) : S(-1), (k & 4) ? S(1) : S(-1)};
+        S v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            v[c] = ic[c] + sk[0] * ih[0] * I.m[0][c] + sk[1] * ih[1] * I.m[1][c] + sk[2] * ih[2] * I.m[2][c];
+        const S dv = dvk[k];
+        const S dvb = sdb - sdir * sbn;
+        S vb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nb[c] += -sdir * dv * sb[c] + sdir * dvb * (v[c] - rc[c]);
+            vb[c] = sb[c] + sdir * dvb * n[c];
+            rcb[c] += -sdir * dvb * n[c];
+            icb[c] += vb[c];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Ib.m[r][c] += sk[r] * ih[r] * vb[c];
+    }
+    // normal n = sg * l: the adjoint of that axis row
+    M3T<S> Ab{}, Bb{};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S lb = (r == ai) ? sg * nb[c] : S(0);
+            if (ty == 0) Ab.m[r][c] += lb;
+            else Bb.m[r][c] += lb;
+            if (a_ref) Bb.m[r][c] += Ib.m[r][c];
+            else Ab.m[r][c] += Ib.m[r][c];
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ga[c] += a_ref ? rcb[c] : icb[c];
+        gb[c] += a_ref ? icb[c] : rcb[c];
+    }
+    S qab[4] = {S(0), S(0), S(0), S(0)}, qbb[4] = {S(0), S(0), S(0), S(0)};
+    axes_adj(qa, Ab, qab);
+    axes_adj(qb, Bb, qbb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { ga[3 + k] += qab[k]; gb[3 + k] += qbb[k]; }
     return true;
 }
 
@@ -414,7 +624,9 @@ template <class T> struct BodyBar {
 };
 
 template <class T> DY_HD void zero(BodyBar<T> &b) {
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { b.p[k] = T(0); b.v[k] = T(0); b.w[k] = T(0); }
+    #pragma unroll
     for (int k = 0; k < 4; ++k) b.q[k] = T(0);
     b.im = T(0);
 }
@@ -450,18 +662,24 @@ template <class T> DY_HD void iw_apply(const M3T<T> &M, const T I[6], const T v[
 template <class T> DY_HD void iw_adj(const M3T<T> &M, const T I[6], const T v[3], const T ob[3], T vbar[3],
                                      M3T<T> &Mbar) {
     T l[3], j[3], jb[3], lb[3];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) l[k] = M.m[0][k] * v[0] + M.m[1][k] * v[1] + M.m[2][k] * v[2];
     j[0] = I[0] * l[0] + I[3] * l[1] + I[4] * l[2];
     j[1] = I[3] * l[0] + I[1] * l[1] + I[5] * l[2];
     j[2] = I[4] * l[0] + I[5] * l[1] + I[2] * l[2];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) jb[k] = M.m[0][k] * ob[0] + M.m[1][k] * ob[1] + M.m[2][k] * ob[2];
+    #pragma unroll
     for (int i = 0; i < 3; ++i)
+        #pragma unroll
         for (int k = 0; k < 3; ++k) Mbar.m[i][k] += ob[i] * j[k];
     lb[0] = I[0] * jb[0] + I[3] * jb[1] + I[4] * jb[2];
     lb[1] = I[3] * jb[0] + I[1] * jb[1] + I[5] * jb[2];
     lb[2] = I[4] * jb[0] + I[5] * jb[1] + I[2] * jb[2];
+    #pragma unroll
     for (int i = 0; i < 3; ++i) {
         vbar[i] += M.m[i][0] * lb[0] + M.m[i][1] * lb[1] + M.m[i][2] * lb[2];
+        #pragma unroll
         for (int k = 0; k < 3; ++k) Mbar.m[i][k] += v[i] * lb[k];
     }
 }
@@ -503,19 +721,24 @@ template <class T> DY_HD void rot_adj(const T q[4], const T w[3], const T q1b[4]
 // Quaternion renormalisation q' = q / |q| and its adjoint (in place on qb).
 template <class T> DY_HD void renorm_t(T q[4]) {
     const T s = T(1) / sq_root(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    #pragma unroll
     for (int k = 0; k < 4; ++k) q[k] = q[k] * s;
 }
 template <class T> DY_HD void renorm_adj(const T q[4], T qb[4]) {
     const T s = T(1) / sq_root(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     T u[4];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) u[k] = q[k] * s;
     const T d = u[0] * qb[0] + u[1] * qb[1] + u[2] * qb[2] + u[3] * qb[3];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) qb[k] = s * (qb[k] - u[k] * d);
 }
 
 // One integrator substep of a dynamic body (integrate_body) and its adjoint.
 template <class T> DY_HD void integrate_t(BodyT<T> &b, const T g[3], T h) {
+    #pragma unroll
     for (int k = 0; k < 3; ++k) b.v[k] = b.v[k] + g[k] * h;
+    #pragma unroll
     for (int k = 0; k < 3; ++k) b.p[k] += b.v[k] * h;
     const T hh = T(0.5f) * h;
     const T qw = b.q[0], qx = b.q[1], qy = b.q[2], qz = b.q[3];
@@ -534,9 +757,11 @@ template <class T> DY_HD void integrate_adj(const BodyT<T> &b0, const T g[3], T 
     const T e[4] = {-(w[0] * q[1] + w[1] * q[2] + w[2] * q[3]), (w[0] * q[0] + w[1] * q[3] - w[2] * q[2]),
                     (w[1] * q[0] + w[2] * q[1] - w[0] * q[3]), (w[2] * q[0] + w[0] * q[2] - w[1] * q[1])};
     T qt[4];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) qt[k] = q[k] + hh * e[k];
     renorm_adj(qt, bb.q);  // bb.q is now the adjoint of qt
     T eb[4];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) eb[k] = hh * bb.q[k];
     // q-bar: identity part stays in bb.q; add Omega(w)^T eb
     T qb[4] = {bb.q[0], bb.q[1], bb.q[2], bb.q[3]};
@@ -547,7 +772,9 @@ template <class T> DY_HD void integrate_adj(const BodyT<T> &b0, const T g[3], T 
     bb.w[0] += -q[1] * eb[0] + q[0] * eb[1] - q[3] * eb[2] + q[2] * eb[3];
     bb.w[1] += -q[2] * eb[0] + q[3] * eb[1] + q[0] * eb[2] - q[1] * eb[3];
     bb.w[2] += -q[3] * eb[0] - q[2] * eb[1] + q[1] * eb[2] + q[0] * eb[3];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) bb.q[k] = qb[k];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) bb.v[k] += h * bb.p[k];  // p1 = p + v1 h; v1 = v + g h
 }
 
@@ -567,6 +794,7 @@ template <class T> DY_HD bool project_t(const ContactT<T> &c, BodyT<T> &A, BodyT
     const T *n = c.n;
     const T wa = A.im, wb = B.im;
     T ra[3], rb[3], ca[3], cb[3], ka[3], kb[3];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { ra[k] = c.p[k] - A.p[k]; rb[k] = c.p[k] - B.p[k]; }
     cross3(ra, n, ca);
     cross3(rb, n, cb);
@@ -584,6 +812,7 @@ template <class T> DY_HD bool project_t(const ContactT<T> &c, BodyT<T> &A, BodyT
         T va[3], vb[3], rv[3];
         point_vel(A.v, A.w, ra, va);
         point_vel(B.v, B.w, rb, vb);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) rv[k] = va[k] - vb[k];
         const T vn = dot3(rv, n);
         if (vn < T(0)) {
@@ -599,13 +828,16 @@ template <class T> DY_HD bool project_t(const ContactT<T> &c, BodyT<T> &A, BodyT
         T va[3], vb[3], u[3], t0[3];
         point_vel(A.v, A.w, ra, va);
         point_vel(B.v, B.w, rb, vb);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) u[k] = va[k] - vb[k];
         const T un = dot3(u, n);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) t0[k] = u[k] - un * n[k];
         const T t2 = dot3(t0, t0);
         if (t2 > T(1e-12f)) {
             const T tl = sq_root(t2);
             T t[3], da[3], db[3], fa[3], fb[3];
+            #pragma unroll
             for (int k = 0; k < 3; ++k) t[k] = t0[k] / tl;
             cross3(ra, t, da);
             cross3(rb, t, db);
@@ -637,6 +869,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     const T wa = A0.im, wb = B0.im;
     const bool dA = wa != T(0), dB = wb != T(0);
     T ra[3], rb[3], ca[3], cbv[3], ka[3], kb[3];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { ra[k] = c.p[k] - A0.p[k]; rb[k] = c.p[k] - B0.p[k]; }
     cross3(ra, n, ca);
     cross3(rb, n, cbv);
@@ -650,7 +883,9 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     const T lam1 = lam0 + rem;
     // forward states after the position stage
     T qA1[4], qB1[4], omA[3], omB[3];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) { qA1[k] = A0.q[k]; qB1[k] = B0.q[k]; }
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { omA[k] = dl * ka[k]; omB[k] = -dl * kb[k]; }
     if (dA) rot_apply(qA1, omA);
     if (dB) rot_apply(qB1, omB);
@@ -662,6 +897,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         T va[3], vb[3];
         point_vel(A0.v, A0.w, ra, va);
         point_vel(B0.v, B0.w, rb, vb);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) rv[k] = va[k] - vb[k];
     }
     const T vn = dot3(rv, n);
@@ -669,6 +905,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     const T jn = vel ? -vn / w : T(0);
     iw_apply(MA1, A0.I, ca, ga);
     iw_apply(MB1, B0.I, cbv, gb);
+    #pragma unroll
     for (int k = 0; k < 3; ++k) {
         vA1[k] = A0.v[k]; wA1[k] = A0.w[k]; vB1[k] = B0.v[k]; wB1[k] = B0.w[k];
         if (vel && dA) { vA1[k] += wa * jn * n[k]; wA1[k] += jn * ga[k]; }
@@ -682,12 +919,15 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         T va[3], vb[3];
         point_vel(vA1, wA1, ra, va);
         point_vel(vB1, wB1, rb, vb);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) u[k] = va[k] - vb[k];
         un = dot3(u, n);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) t0[k] = u[k] - un * n[k];
         const T t2 = dot3(t0, t0);
         if (t2 > T(1e-12f)) {
             tl = sq_root(t2);
+            #pragma unroll
             for (int k = 0; k < 3; ++k) t[k] = t0[k] / tl;
             cross3(ra, t, da);
             cross3(rb, t, db);
@@ -711,6 +951,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     M3T<T> MAb{}, MBb{}, MA1b{}, MB1b{};
     // running adjoints of the velocity state (start: outputs)
     T avb[3], awb[3], bvb[3], bwb[3];
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { avb[k] = Ab.v[k]; awb[k] = Ab.w[k]; bvb[k] = Bb.v[k]; bwb[k] = Bb.w[k]; }
     T lam1b = lamb;
     if (fric) {
@@ -719,12 +960,14 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
             const T dvt = dot3(avb, t);
             jtb += -wa * dvt - dot3(awb, fa);
             Ab.im += -jt * dvt;
+            #pragma unroll
             for (int k = 0; k < 3; ++k) { tb[k] += -(wa * jt) * avb[k]; fab[k] += -jt * awb[k]; }
         }
         if (dB) {
             const T dvt = dot3(bvb, t);
             jtb += wb * dvt + dot3(bwb, fb);
             Bb.im += jt * dvt;
+            #pragma unroll
             for (int k = 0; k < 3; ++k) { tb[k] += (wb * jt) * bvb[k]; fbb[k] += jt * bwb[k]; }
         }
         T jt0b = T(0), jcapb = T(0);
@@ -740,6 +983,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         Ab.im += wtb;
         Bb.im += wtb;
         T dab[3], dbb[3];
+        #pragma unroll
         for (int k = 0; k < 3; ++k) {
             dab[k] = wtb * fa[k];
             fab[k] += wtb * da[k];
@@ -757,16 +1001,20 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         // t = t0 / tl
         T t0b[3];
         T s = T(0);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) { t0b[k] = tb[k] / tl; s += tb[k] * t0[k]; }
         tlb += -s / (tl * tl);
         // tl = sqrt(t2); t2 = t0.t0
         const T t2b = tlb / (T(2) * tl);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) t0b[k] += T(2) * t2b * t0[k];
         // t0 = u - un n
         T ub[3];
         const T unb = -dot3(t0b, n);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) { ub[k] = t0b[k] + unb * n[k]; cb.n[k] += -un * t0b[k] + unb * u[k]; }
         // u = (vA1 + wA1 x ra) - (vB1 + wB1 x rb)
+        #pragma unroll
         for (int k = 0; k < 3; ++k) { avb[k] += ub[k]; bvb[k] -= ub[k]; }
         cross3(ra, ub, tmp); for (int k = 0; k < 3; ++k) awb[k] += tmp[k];
         cross3(ub, wA1, tmp); for (int k = 0; k < 3; ++k) rab[k] += tmp[k];
@@ -782,12 +1030,14 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
             const T dvn = dot3(avb, n);
             jnb += wa * dvn + dot3(awb, ga);
             Ab.im += jn * dvn;
+            #pragma unroll
             for (int k = 0; k < 3; ++k) { cb.n[k] += (wa * jn) * avb[k]; gab[k] += jn * awb[k]; }
         }
         if (dB) {
             const T dvn = dot3(bvb, n);
             jnb += -wb * dvn - dot3(bwb, gb);
             Bb.im += -jn * dvn;
+            #pragma unroll
             for (int k = 0; k < 3; ++k) { cb.n[k] += -(wb * jn) * bvb[k]; gbb[k] += -jn * bwb[k]; }
         }
         iw_adj(MA1, A0.I, ca, gab, cab, MA1b);
@@ -797,7 +1047,9 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         wbar += jnb * vn / (w * w);
         // vn = rv . n; rv = (vA + wA x ra) - (vB + wB x rb)
         T rvb[3], tmp[3];
+        #pragma unroll
         for (int k = 0; k < 3; ++k) { rvb[k] = vnb * n[k]; cb.n[k] += vnb * rv[k]; }
+        #pragma unroll
         for (int k = 0; k < 3; ++k) { avb[k] += rvb[k]; bvb[k] -= rvb[k]; }
         cross3(ra, rvb, tmp); for (int k = 0; k < 3; ++k) awb[k] += tmp[k];
         cross3(rvb, A0.w, tmp); for (int k = 0; k < 3; ++k) rab[k] += tmp[k];
@@ -805,9 +1057,11 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         cross3(rb, nrv, tmp); for (int k = 0; k < 3; ++k) bwb[k] += tmp[k];
         cross3(nrv, B0.w, tmp); for (int k = 0; k < 3; ++k) rbb[k] += tmp[k];
     }
+    #pragma unroll
     for (int k = 0; k < 3; ++k) { Ab.v[k] = avb[k]; Ab.w[k] = awb[k]; Bb.v[k] = bvb[k]; Bb.w[k] = bwb[k]; }
     // quaternions: q1-bar = output adjoint + axes adjoint at q1, then rotate_by reverse
     T qA1b[4], qB1b[4];
+    #pragma unroll
     for (int k = 0; k < 4; ++k) { qA1b[k] = Ab.q[k]; qB1b[k] = Bb.q[k]; }
     axes_adj(qA1, MA1b, qA1b);
     axes_adj(qB1, MB1b, qB1b);
@@ -817,16 +1071,20 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         T omb[3] = {T(0), T(0), T(0)};
         rot_adj(A0.q, omA, qA1b, qAb, omb);
         dlb += dot3(omb, ka);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) kab[k] += dl * omb[k];
     } else {
+        #pragma unroll
         for (int k = 0; k < 4; ++k) qAb[k] = qA1b[k];
     }
     if (dB) {
         T omb[3] = {T(0), T(0), T(0)};
         rot_adj(B0.q, omB, qB1b, qBb, omb);
         dlb += -dot3(omb, kb);
+        #pragma unroll
         for (int k = 0; k < 3; ++k) kbb[k] += -dl * omb[k];
     } else {
+        #pragma unroll
         for (int k = 0; k < 4; ++k) qBb[k] = qB1b[k];
     }
     // positions: pA1 = pA + wa dl n; pB1 = pB - wb dl n
@@ -834,12 +1092,14 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         const T dpn = dot3(Ab.p, n);
         Ab.im += dl * dpn;
         dlb += wa * dpn;
+        #pragma unroll
         for (int k = 0; k < 3; ++k) cb.n[k] += (wa * dl) * Ab.p[k];
     }
     if (dB) {
         const T dpn = dot3(Bb.p, n);
         Bb.im += -dl * dpn;
         dlb += -wb * dpn;
+        #pragma unroll
         for (int k = 0; k < 3; ++k) cb.n[k] += -(wb * dl) * Bb.p[k];
     }
     // lam1 = lam0 + rem; dl = rem / w
@@ -850,6 +1110,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     // w = wa + wb + ca.ka + cb.kb
     Ab.im += wbar;
     Bb.im += wbar;
+    #pragma unroll
     for (int k = 0; k < 3; ++k) {
         cab[k] += wbar * ka[k];
         kab[k] += wbar * ca[k];
@@ -860,6 +1121,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     iw_adj(MB, B0.I, cbv, kbb, cbb, MBb);
     axes_adj(A0.q, MAb, qAb);
     axes_adj(B0.q, MBb, qBb);
+    #pragma unroll
     for (int k = 0; k < 4; ++k) { Ab.q[k] = qAb[k]; Bb.q[k] = qBb[k]; }
     // ca = ra x n: ra-bar += n x ca-bar; n-bar += ca-bar x ra
     T tmp[3];
@@ -868,6 +1130,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     cross3(n, cbb, tmp); for (int k = 0; k < 3; ++k) rbb[k] += tmp[k];
     cross3(cbb, rb, tmp); for (int k = 0; k < 3; ++k) cb.n[k] += tmp[k];
     // ra = p - pA; rb = p - pB
+    #pragma unroll
     for (int k = 0; k < 3; ++k) {
         cb.p[k] += rab[k] + rbb[k];
         Ab.p[k] += -rab[k];

@@ -779,10 +779,15 @@ int launch_frames(const BatchArgs &a, uint32_t n_units, size_t lds_bytes, hipStr
 // ---------------------------------------------------------------------------
 // Reverse sweep of recorded frames (the differentiable form)
 // ---------------------------------------------------------------------------
+// Per-contact floats of the reverse sweep: lambda adjoint, the contact's
+// (p, n, depth) adjoint, and its pose adjoints for both bodies (7 + 7).
+constexpr uint32_t kCsStride = 22;
+
 struct DiffArgs {
     dy_body_view adj;  // adjoint of the 13 mutable fields; inv_mass holds the inverse-mass adjoint
-    float *cs;         // per contact: lambda adjoint, then the contact's (p, n, depth) adjoint
+    float *cs;         // kCsStride per contact
     uint32_t frames;   // recorded frames to reverse, newest first
+    uint32_t max_n;    // largest scene: LDS holds 21 floats per body
 };
 
 __device__ __forceinline__ BodyBar<float> load_bar(const dy_body_view &v, uint32_t i) {
@@ -820,17 +825,38 @@ __device__ __forceinline__ BodyT<float> body_in(const float *m, const dy_body_vi
 // share no dynamic body), then the contact geometry (class by class), then
 // the integrator.  Static bodies receive no adjoint.
 __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
+    extern __shared__ float lds_adj[];
     const uint32_t s = blockIdx.x;
     const uint32_t r = lane_id();
     const WaveTeam tm{r, nullptr};
     const uint32_t b0 = A.body_off[s], n = A.body_off[s + 1] - b0;
     const uint32_t C = A.contact_off[s + 1] - A.contact_off[s];
-    const dy_body_view gv = scene_bodies(A.bv, b0, n);
-    const dy_body_view av = scene_bodies(D.adj, b0, n);
+    const dy_body_view gl = scene_bodies(A.bv, b0, n);
+    const dy_body_view ag = scene_bodies(D.adj, b0, n);
+    // The scene's adjoint (13 fields + inverse mass) and its constants
+    // (inverse mass, inverse inertia) live in LDS during the launch.
+    dy_body_view av{}, gv = gl;
+    {
+        float *p = lds_adj;
+#define DY_ADJ_LDS(dst, src, k)                                         \
+    dst = p + (k) * n;                                                  \
+    for (uint32_t i = r; i < n; i += 64) dst[i] = src[i];
+        DY_ADJ_LDS(av.pos_x, ag.pos_x, 0) DY_ADJ_LDS(av.pos_y, ag.pos_y, 1) DY_ADJ_LDS(av.pos_z, ag.pos_z, 2)
+        DY_ADJ_LDS(av.rot_w, ag.rot_w, 3) DY_ADJ_LDS(av.rot_x, ag.rot_x, 4) DY_ADJ_LDS(av.rot_y, ag.rot_y, 5)
+        DY_ADJ_LDS(av.rot_z, ag.rot_z, 6)
+        DY_ADJ_LDS(av.vel_x, ag.vel_x, 7) DY_ADJ_LDS(av.vel_y, ag.vel_y, 8) DY_ADJ_LDS(av.vel_z, ag.vel_z, 9)
+        DY_ADJ_LDS(av.ang_x, ag.ang_x, 10) DY_ADJ_LDS(av.ang_y, ag.ang_y, 11) DY_ADJ_LDS(av.ang_z, ag.ang_z, 12)
+        DY_ADJ_LDS(av.inv_mass, ag.inv_mass, 13)
+        DY_ADJ_LDS(gv.inv_mass, gl.inv_mass, 14)
+        DY_ADJ_LDS(gv.iI_xx, gl.iI_xx, 15) DY_ADJ_LDS(gv.iI_yy, gl.iI_yy, 16) DY_ADJ_LDS(gv.iI_zz, gl.iI_zz, 17)
+        DY_ADJ_LDS(gv.iI_xy, gl.iI_xy, 18) DY_ADJ_LDS(gv.iI_xz, gl.iI_xz, 19) DY_ADJ_LDS(gv.iI_yz, gl.iI_yz, 20)
+#undef DY_ADJ_LDS
+    }
+    tm.sync();
     const uint32_t iters = static_cast<uint32_t>(A.n_iters);
     const TapeLayout TL = tape_layout(n, C, iters);
     const float *tp = A.tape + A.tape_off[s];
-    float *cs = D.cs + 8ull * A.contact_off[s];
+    float *cs = D.cs + static_cast<uint64_t>(kCsStride) * A.contact_off[s];
     const float h = A.h;
     const float g[3] = {A.gx, A.gy, A.gz};
     for (int f = static_cast<int>(D.frames) - 1; f >= 0; --f) {
@@ -841,7 +867,7 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
             const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
             const float4 *rec = reinterpret_cast<const float4 *>(fb + TL.rec);
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
-            for (uint32_t j = r; j < 8 * nc; j += 64) cs[j] = 0.f;
+            for (uint32_t j = r; j < kCsStride * nc; j += 64) cs[j] = 0.f;
             tm.sync();
             for (int it = static_cast<int>(iters) - 1; it >= 0; --it) {
                 const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
@@ -865,37 +891,54 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                         const float *pr = fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + j);
                         const BodyT<float> A0 = body_in(pr, gv, a), B0 = body_in(pr + 13, gv, b);
                         BodyBar<float> Ab = load_bar(av, a), Bb = load_bar(av, b);
-                        float lb = cs[8 * j];
+                        float lb = cs[kCsStride * j];
                         ContactT<float> cb;
                         for (int k = 0; k < 3; ++k) { cb.p[k] = 0.f; cb.n[k] = 0.f; }
                         cb.depth = 0.f;
                         project_adj(ct, A0, B0, A.mu, pr[26], h, Ab, Bb, lb, cb);
-                        cs[8 * j] = lb;
-                        for (int k = 0; k < 3; ++k) { cs[8 * j + 1 + k] += cb.p[k]; cs[8 * j + 4 + k] += cb.n[k]; }
-                        cs[8 * j + 7] += cb.depth;
+                        cs[kCsStride * j] = lb;
+                        for (int k = 0; k < 3; ++k) {
+                            cs[kCsStride * j + 1 + k] += cb.p[k];
+                            cs[kCsStride * j + 4 + k] += cb.n[k];
+                        }
+                        cs[kCsStride * j + 7] += cb.depth;
                         if (A0.im != 0.f) store_bar(av, a, Ab);
                         if (B0.im != 0.f) store_bar(av, b, Bb);
                     }
                     tm.sync();
                 }
             }
-            // contact geometry: (p, n, depth) adjoints to the post-integrate poses
+            // contact geometry: (p, n, depth) adjoints to the post-integrate
+            // poses, every contact in parallel (written-out face adjoint,
+            // dual-number replay for the other kinds), then added to the
+            // bodies class by class
             const float *po = fb + TL.po;
+            for (uint32_t j = r; j < nc; j += 64) {
+                const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
+                const uint32_t sa = gl.shape[a], sb = gl.shape[b];
+                const uint32_t ta = A.sv.type[sa], tb = A.sv.type[sb];
+                const float ha[3] = {A.sv.half_x[sa], A.sv.half_y[sa], A.sv.half_z[sa]};
+                const float hb[3] = {A.sv.half_x[sb], A.sv.half_y[sb], A.sv.half_z[sb]};
+                const float pa[3] = {po[a], po[n + a], po[2 * n + a]};
+                const float qa[4] = {po[3 * n + a], po[4 * n + a], po[5 * n + a], po[6 * n + a]};
+                const float pb[3] = {po[b], po[n + b], po[2 * n + b]};
+                const float qb[4] = {po[3 * n + b], po[4 * n + b], po[5 * n + b], po[6 * n + b]};
+                float hbar[7];
+                for (int k = 0; k < 7; ++k) hbar[k] = cs[kCsStride * j + 1 + k];
+                float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                const bool done = ta == kBox && tb == kBox && box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
+                if (!done) (void)contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
+                for (int k = 0; k < 7; ++k) {
+                    cs[kCsStride * j + 8 + k] = ga[k];
+                    cs[kCsStride * j + 15 + k] = gb2[k];
+                }
+            }
+            tm.sync();
             for (uint32_t c = 0; c < ncls; ++c) {
                 const uint32_t end = co[c + 1];
                 for (uint32_t j = co[c] + r; j < end; j += 64) {
                     const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
-                    const uint32_t sa = gv.shape[a], sb = gv.shape[b];
-                    const float ha[3] = {A.sv.half_x[sa], A.sv.half_y[sa], A.sv.half_z[sa]};
-                    const float hb[3] = {A.sv.half_x[sb], A.sv.half_y[sb], A.sv.half_z[sb]};
-                    const float pa[3] = {po[a], po[n + a], po[2 * n + a]};
-                    const float qa[4] = {po[3 * n + a], po[4 * n + a], po[5 * n + a], po[6 * n + a]};
-                    const float pb[3] = {po[b], po[n + b], po[2 * n + b]};
-                    const float qb[4] = {po[3 * n + b], po[4 * n + b], po[5 * n + b], po[6 * n + b]};
-                    float hbar[7];
-                    for (int k = 0; k < 7; ++k) hbar[k] = cs[8 * j + 1 + k];
-                    float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                    (void)contact_adj<float, 2>(A.sv.type[sa], A.sv.type[sb], ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
+                    const float *ga = cs + kCsStride * j + 8, *gb2 = cs + kCsStride * j + 15;
                     if (gv.inv_mass[a] != 0.f) {
                         av.pos_x[a] += ga[0]; av.pos_y[a] += ga[1]; av.pos_z[a] += ga[2];
                         av.rot_w[a] += ga[3]; av.rot_x[a] += ga[4]; av.rot_y[a] += ga[5]; av.rot_z[a] += ga[6];
@@ -923,6 +966,7 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
         }
         tm.sync();
     }
+    for (uint32_t i = r; i < n; i += 64) store_bar(ag, i, load_bar(av, i));
 }
 
 }  // namespace dy
@@ -1358,7 +1402,7 @@ struct dy_diff_s {
     uint64_t *d_tape_off = nullptr;
     float *d_ckpt = nullptr;  // checkpoints: 13 x n_total floats each
     float *d_work = nullptr;  // recompute state: 13 x n_total
-    float *d_cs = nullptr;    // 8 per contact
+    float *d_cs = nullptr;    // kCsStride per contact
 };
 
 namespace {
@@ -1412,7 +1456,7 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     if (st == DY_OK) st = dev_alloc(d->d_tape_off, off.size());
     if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
-    if (st == DY_OK) st = dev_alloc(d->d_cs, 8ull * std::max<uint32_t>(b->contact_off.back(), 1u));
+    if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(b->contact_off.back(), 1ull));
     if (st == DY_OK && hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         st = DY_ERR_INTERNAL;
     if (st != DY_OK) {
@@ -1486,7 +1530,9 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         a.tape = d->d_tape;
         a.tape_off = d->d_tape_off;
         D.frames = nf;
-        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), 0, hsm, a, D);
+        D.max_n = d->b->max_n;
+        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), 21ull * d->b->max_n * sizeof(float), hsm,
+                           a, D);
         DY_LAUNCH_CHECK();
     }
     return DY_OK;

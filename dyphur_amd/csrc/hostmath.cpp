@@ -154,6 +154,10 @@ int ha_contact_adj_d(uint32_t ta, uint32_t tb, const double *ha, const double *h
         default: return contact_adj<double, 14>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
     }
 }
+int ha_box_face_adj_d(const double *ha, const double *hb, const double *pa, const double *pb, const double *qa,
+                      const double *qb, const double *hbar, double *ga, double *gb) {
+    return box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+}
 int ha_contact_adj_f(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
                      const float *qa, const float *qb, const float *hbar, float *ga, float *gb) {
     return contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;

@@ -29,6 +29,8 @@ def hm():
     L.ha_pair_contact_d.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 8
     L.ha_contact_adj_d.restype = i
     L.ha_contact_adj_d.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 9 + [i]
+    L.ha_box_face_adj_d.restype = i
+    L.ha_box_face_adj_d.argtypes = [vp] * 9
     L.ha_contact_adj_f.restype = i
     L.ha_contact_adj_f.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 9
     L.ha_project_f.restype = i
@@ -333,3 +335,30 @@ def test_host_forward_mode_matches_central_differences(hm):
     sm = _sim(hm, a, frames, st=a["st"] - eps * u_st, im=a["im"] - eps * u_im)
     fd = (sp - sm) / (2 * eps)
     assert np.abs(sd - fd).max() <= 1e-4 * max(1.0, np.abs(fd).max())
+
+
+def test_box_face_adjoint_equals_dual_jacobian(hm):
+    """The written-out face-contact adjoint equals the dual-number Jacobian
+    on every box-box face contact (and declines edge contacts)."""
+    rng = np.random.default_rng(12)
+    faces = edges = 0
+    for _ in range(3000):
+        ha, hb, pa, pb, qa, qb = _pair(rng, BOX, BOX)
+        out = np.zeros(7)
+        w = C.c_int(-1)
+        if not hm.ha_pair_contact_d(BOX, BOX, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(out), C.byref(w)):
+            continue
+        hbar = rng.normal(size=7)
+        g1a, g1b, g2a, g2b = np.zeros(7), np.zeros(7), np.zeros(7), np.zeros(7)
+        hm.ha_contact_adj_d(BOX, BOX, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g1a), P(g1b), 14)
+        ok = hm.ha_box_face_adj_d(P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g2a), P(g2b))
+        if (w.value >> 4) == 2:
+            assert not ok
+            edges += 1
+            continue
+        assert ok
+        faces += 1
+        want = np.concatenate([g1a, g1b])
+        got = np.concatenate([g2a, g2b])
+        assert np.abs(got - want).max() <= 1e-10 * max(1.0, np.abs(want).max()), (got, want)
+    assert faces > 500 and edges > 10
