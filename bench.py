@@ -18,7 +18,9 @@ in the timed region).
 
 The line also carries "batched": the 4096-scene pile workload (seed 20260003)
 stepped 300 frames in one launch, its body-substeps/s and the plain per-scene
-loop of the drop-in API on a sample of the same scenes.
+loop of the drop-in API on a sample of the same scenes; and
+"differentiable": the 256-scene thrown-stack workload (seed 20260004), its
+forward and forward + adjoint times per frame.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -50,6 +52,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-frames", type=int, default=300,
                     help="frames of the 4096-scene batched workload (0 skips it)")
+    ap.add_argument("--diff-frames", type=int, default=120,
+                    help="frames of the differentiable 256-scene workload (0 skips it)")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="scenes of the batched workload stepped through the plain per-scene loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,6 +246,59 @@ def batched(capi, scenes, frames: int, plain_sample: int) -> dict:
     }
 
 
+def differentiable(capi, scenes, frames: int = 120, every: int = 10) -> dict:
+    """Config 4: 256 thrown 4x4x4 box stacks, loss sum |x_T - x*|^2.
+    Forward alone (one launch of all frames) against forward with
+    checkpoints + reverse sweep (recompute each checkpoint segment with a
+    record, then the adjoints), both timed with HIP events on the stream."""
+    scs = scenes.diff_batch(256)
+    b = capi.Batch(scs)
+    init = b.bodies.host_arrays(b.merged)
+    e0, e1 = capi.Event(), capi.Event()
+    b.step(1)
+    b.bodies.upload_state(init, b.s)
+    e0.record(b.s)
+    b.step(frames)
+    e1.record(b.s)
+    e1.sync()
+    fwd = e0.elapsed_ms(e1)
+    d = capi.Diff(b, frames, every)
+    b.bodies.upload_state(init, b.s)
+    d.forward(frames)
+    st = b.state()
+    n = b.n_bodies
+    adj = {k: np.zeros(n, np.float32) for k in ("pos_x", "pos_y", "pos_z")}
+    for i, sc in enumerate(scs):
+        lo = int(b.body_offset[i])
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+            adj[name][lo:lo + sc.n - 1] = 2.0 * (st[name][lo:lo + sc.n - 1] - sc.meta["target"][:, k])
+    d.backward(adj, download=False)  # warm-up
+    b.bodies.upload_state(init, b.s)
+    b.s.wait()
+    e0.record(b.s)
+    d.forward(frames)
+    e1.record(b.s)
+    e1.sync()
+    dfw = e0.elapsed_ms(e1)
+    e0.record(b.s)
+    d.backward(adj, download=False)
+    e1.record(b.s)
+    e1.sync()
+    bwd = e0.elapsed_ms(e1)
+    return {
+        "workload": f"256 scenes of 64 boxes (4x4x4 stack, half 0.2 m, 1 kg, 0.45 m pitch, 1 m above the "
+                    f"ground) thrown with v0 ~ N(0,1), w0 ~ N(0,2), seed 20260004; {frames} frames at 60 Hz; "
+                    f"loss sum |x_T - x*|^2; gradients w.r.t. v0 and inverse mass; checkpoint every {every} frames",
+        "bodies": n, "frames": frames, "checkpoint_every": every,
+        "forward_ms_per_frame": round(fwd / frames, 4),
+        "forward_checkpointed_ms_per_frame": round(dfw / frames, 4),
+        "adjoint_ms_per_frame": round(bwd / frames, 4),
+        "fwd_adjoint_ms_per_frame": round((dfw + bwd) / frames, 4),
+        "fwd_adjoint_over_fwd": round((dfw + bwd) / fwd, 2),
+        "value": round(n * frames / ((dfw + bwd) * 1e-3), 1), "unit": "body-substeps/s (forward + adjoint)",
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -357,6 +414,8 @@ def main():
         out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
     if args.batched_frames > 0:
         out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
+    if args.diff_frames > 0:
+        out["differentiable"] = differentiable(capi, scenes, args.diff_frames)
     print(json.dumps(out), flush=True)
 
 
