@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DYPHUR_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["runtime.hip", "primitives.hip", "broadphase.hip", "narrowphase.hip", "solver.hip",
-               "batch.hip"]
+               "batch.hip", "ray_query.hip"]
 
 # -ffp-contract=off: parity with the reference requires every float op to round
 # on its own.  Packed-FP32 instructions are disabled: some of their forms have
@@ -99,7 +99,7 @@ def build_python_module(verbose: bool = False) -> str | None:
     return out
 
 
-CPP_TEST_SOURCES = ["main.cpp", "test_stores_integrator.cpp", "test_pipeline.cpp", "test_compute.cpp"]
+CPP_TEST_SOURCES = ["main.cpp", "test_stores_integrator.cpp", "test_pipeline.cpp", "test_compute.cpp", "test_ray_query.cpp"]
 
 
 def build_cpp_tests(verbose: bool = False) -> str:

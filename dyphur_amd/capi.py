@@ -147,6 +147,7 @@ _SIGS = {
     "dy_xpbd_set_timing": (C.c_int, [vp, C.c_int]),
     "dy_xpbd_get_timing": (C.c_int, [vp, vp, C.POINTER(SolverTiming)]),
     "dy_state_hash_fnv1a64": (C.c_int, [vp, C.POINTER(BodyView), C.POINTER(C.c_uint64)]),
+    "dy_ray_query": (C.c_int, [vp, C.POINTER(BvhView), C.POINTER(BodyView), C.POINTER(ShapeView), vp, vp, u32]),
     "dy_graph_capture_begin": (C.c_int, [vp]),
     "dy_graph_capture_end": (C.c_int, [vp, C.POINTER(vp)]),
     "dy_graph_launch": (C.c_int, [vp, vp]),
@@ -614,6 +615,23 @@ class World:
 
     def fnv(self) -> int:
         return self.bodies.fnv(self.s)
+
+
+def ray_query(s: Stream, bp: "Broadphase", n: int, bodies: "DeviceBodies", shapes: "DeviceShapes",
+              rays: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Closest hits of rays (m, 7: origin xyz, direction xyz, t_max) over the
+    BVH of bp's last build_and_query over n bodies: (t[m], body[m])."""
+    r = np.ascontiguousarray(rays, np.float32)
+    m = len(r)
+    d_rays = DeviceArray.from_numpy(r.ravel(), s)
+    d_hits = DeviceArray(2 * max(m, 1), np.uint32)
+    bvh = BvhView()
+    check(lib().dy_broadphase_bvh_view(bp.h, n, C.byref(bvh)), "dy_broadphase_bvh_view")
+    bv, sv = bodies.view(), shapes.view()
+    check(lib().dy_ray_query(s.h, C.byref(bvh), C.byref(bv), C.byref(sv), d_rays.ptr, d_hits.ptr, m),
+          "dy_ray_query")
+    out = d_hits.download(s, 2 * m).reshape(m, 2)
+    return out[:, 0].view(np.float32).copy(), out[:, 1].copy()
 
 
 class Graph:

@@ -263,6 +263,19 @@ int dy_xpbd_get_timing(dy_xpbd_t x, dy_stream_t s, dy_solver_timing *out);
 /* Class (colour or level) of each contact of the last solve; blocking. */
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n);
 
+/* ---- ray queries ------------------------------------------------------------
+ * Ray / RayHit (core/include/core/ray_query.hpp:9-18): direction need not be
+ * normalised, t is in direction units; a hit is recorded only for t < t_max. */
+typedef struct dy_ray { float ox, oy, oz, dx, dy, dz, t_max; } dy_ray;
+typedef struct dy_ray_hit { float t; uint32_t body_idx; } dy_ray_hit;
+/* ray_query(s, bvh, bodies, shapes, d_rays, d_hits, n_rays)
+ * (core/include/core/ray_query.hpp:35-41, core/src/ray_query.cpp:126-195):
+ * closest exact hit (oriented box, sphere) over the BVH of the last
+ * build_and_query (dy_broadphase_bvh_view); body_idx 0xFFFFFFFF and t = t_max
+ * when nothing is hit.  Device arrays; asynchronous. */
+int dy_ray_query(dy_stream_t s, const dy_bvh_view *bvh, const dy_body_view *bodies, const dy_shape_view *shapes,
+                 const dy_ray *d_rays, dy_ray_hit *d_hits, uint32_t n_rays);
+
 /* ---- batched scenes -------------------------------------------------------
  * Many independent scenes stepped by one launch: the drop-in frame
  * (examples/stress_test_blocks/main.cpp:264-273) of every scene, one workgroup

@@ -1089,6 +1089,135 @@ void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *boun
     if (st) { st->n_pairs = np; st->n_contacts = nc; st->n_colors = ss.n_colors; st->used_serial = ss.used_serial; }
 }
 
+// ---- ray queries over the broadphase's LBVH (core/src/ray_query.cpp:12-195) ----
+namespace {
+
+// Slab test of a ray against a box (ray_query.cpp:14-36): entered before
+// best_t and not entirely behind the origin.
+bool rq_aabb_hit(float ox, float oy, float oz, float ix, float iy, float iz, float mnx, float mny, float mnz,
+                 float mxx, float mxy, float mxz, float best_t) {
+    const float t1x = (mnx - ox) * ix, t2x = (mxx - ox) * ix;
+    const float t1y = (mny - oy) * iy, t2y = (mxy - oy) * iy;
+    const float t1z = (mnz - oz) * iz, t2z = (mxz - oz) * iz;
+    float tmin = t1x < t2x ? t1x : t2x;
+    float tmax = t1x < t2x ? t2x : t1x;
+    const float ylo = t1y < t2y ? t1y : t2y, yhi = t1y < t2y ? t2y : t1y;
+    tmin = tmin > ylo ? tmin : ylo;
+    tmax = tmax < yhi ? tmax : yhi;
+    const float zlo = t1z < t2z ? t1z : t2z, zhi = t1z < t2z ? t2z : t1z;
+    tmin = tmin > zlo ? tmin : zlo;
+    tmax = tmax < zhi ? tmax : zhi;
+    return tmax >= 0.f && tmax >= tmin && tmin < best_t;
+}
+
+// v rotated by the conjugate of q (quat.hpp:31,39-42: t = 2 u x v,
+// v + w t + u x t with u = -xyz).
+void rq_rotate_inv(float w, float x, float y, float z, const float v[3], float o[3]) {
+    const float ux = -x, uy = -y, uz = -z;
+    const float tx = (uy * v[2] - uz * v[1]) * 2.f;
+    const float ty = (uz * v[0] - ux * v[2]) * 2.f;
+    const float tz = (ux * v[1] - uy * v[0]) * 2.f;
+    o[0] = (v[0] + tx * w) + (uy * tz - uz * ty);
+    o[1] = (v[1] + ty * w) + (uz * tx - ux * tz);
+    o[2] = (v[2] + tz * w) + (ux * ty - uy * tx);
+}
+
+// Oriented box (ray_query.cpp:39-77); -1 on a miss.
+float rq_box_hit(const float o[3], const float d[3], const float p[3], const float q[4], const float h[3],
+                 float best_t) {
+    const float oc[3] = {o[0] - p[0], o[1] - p[1], o[2] - p[2]};
+    float lo[3], ld[3];
+    rq_rotate_inv(q[0], q[1], q[2], q[3], oc, lo);
+    rq_rotate_inv(q[0], q[1], q[2], q[3], d, ld);
+    const float ix = 1.f / ld[0], iy = 1.f / ld[1], iz = 1.f / ld[2];
+    const float t1x = (-h[0] - lo[0]) * ix, t2x = (h[0] - lo[0]) * ix;
+    const float t1y = (-h[1] - lo[1]) * iy, t2y = (h[1] - lo[1]) * iy;
+    const float t1z = (-h[2] - lo[2]) * iz, t2z = (h[2] - lo[2]) * iz;
+    float tmin = t1x < t2x ? t1x : t2x;
+    float tmax = t1x < t2x ? t2x : t1x;
+    const float ylo = t1y < t2y ? t1y : t2y, yhi = t1y < t2y ? t2y : t1y;
+    tmin = tmin > ylo ? tmin : ylo;
+    tmax = tmax < yhi ? tmax : yhi;
+    const float zlo = t1z < t2z ? t1z : t2z, zhi = t1z < t2z ? t2z : t1z;
+    tmin = tmin > zlo ? tmin : zlo;
+    tmax = tmax < zhi ? tmax : zhi;
+    if (tmax < 0.f || tmin > tmax) return -1.f;
+    const float t = tmin >= 0.f ? tmin : tmax;
+    return (t >= 0.f && t < best_t) ? t : -1.f;
+}
+
+// Sphere (ray_query.cpp:80-95); -1 on a miss.
+float rq_sphere_hit(const float o[3], const float d[3], const float c[3], float r, float best_t) {
+    const float ocx = o[0] - c[0], ocy = o[1] - c[1], ocz = o[2] - c[2];
+    const float a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const float b = d[0] * ocx + d[1] * ocy + d[2] * ocz;
+    const float cc = ocx * ocx + ocy * ocy + ocz * ocz - r * r;
+    const float disc = b * b - a * cc;
+    if (disc < 0.f) return -1.f;
+    const float sq = sqrtf(disc);
+    float t = (-b - sq) / a;
+    if (t < 0.f) t = (-b + sq) / a;
+    return (t >= 0.f && t < best_t) ? t : -1.f;
+}
+
+void rq_leaf(uint32_t bi, const float o[3], const float d[3], const rc_bodies &b, const rc_shapes &s, float &best_t,
+             uint32_t &best) {
+    const uint32_t sh = b.shape[bi];
+    const float p[3] = {b.pos_x[bi], b.pos_y[bi], b.pos_z[bi]};
+    float t;
+    if (s.type[sh] == 1u) {
+        t = rq_sphere_hit(o, d, p, s.half_x[sh], best_t);
+    } else {
+        const float q[4] = {b.rot_w[bi], b.rot_x[bi], b.rot_y[bi], b.rot_z[bi]};
+        const float h[3] = {s.half_x[sh], s.half_y[sh], s.half_z[sh]};
+        t = rq_box_hit(o, d, p, q, h, best_t);
+    }
+    if (t >= 0.f) {
+        best_t = t;
+        best = bi;
+    }
+}
+
+}  // namespace
+
+void rc_ray_query(const rc_world *w, uint32_t n, const rc_bodies *b, const rc_shapes *s, const rc_ray *rays,
+                  rc_ray_hit *hits, uint32_t n_rays) {
+    // ray_query.cpp:126-195: depth-first from the root with a 64-entry stack,
+    // right child pushed first so the left subtree is visited first.
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < static_cast<long long>(n_rays); ++ii) {
+        const rc_ray r = rays[ii];
+        const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+        float best_t = r.t_max;
+        uint32_t best = 0xFFFFFFFFu;
+        const float ix = 1.f / r.dx, iy = 1.f / r.dy, iz = 1.f / r.dz;
+        if (n == 1) {
+            rq_leaf(0u, o, d, *b, *s, best_t, best);
+        } else if (n > 1) {
+            int32_t stack[64];
+            int top = 0;
+            stack[top++] = w->root;
+            const uint32_t n_int = n - 1u;
+            while (top > 0) {
+                const int32_t node = stack[--top];
+                const uint16_t *bx = &w->aabb[8 * static_cast<size_t>(node)];
+                const float mn[3] = {rc_half_to_float(bx[0]), rc_half_to_float(bx[1]), rc_half_to_float(bx[2])};
+                const float mx[3] = {rc_half_to_float(bx[3]), rc_half_to_float(bx[4]), rc_half_to_float(bx[5])};
+                if (!rq_aabb_hit(o[0], o[1], o[2], ix, iy, iz, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], best_t))
+                    continue;
+                if (static_cast<uint32_t>(node) >= n_int) {
+                    rq_leaf(w->sorted_idx[node - static_cast<int32_t>(n_int)], o, d, *b, *s, best_t, best);
+                } else if (top + 1 < 64) {
+                    stack[top++] = w->right[node];
+                    stack[top++] = w->left[node];
+                }
+            }
+        }
+        hits[ii].t = best_t;
+        hits[ii].body = best;
+    }
+}
+
 int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
                     const float *pb, const float *qa, const float *qb, float *out) {
     float px[2] = {pa[0], pb[0]}, py[2] = {pa[1], pb[1]}, pz[2] = {pa[2], pb[2]};

@@ -17,6 +17,7 @@
  *                                          (rc_body_setup)
  *   compute/include/compute/sort.hpp:68-85 (rc_bitonic_sort_*)
  *   examples/stress_test_blocks/main.cpp:34-41,301-311 (rc_fnv1a64_state)
+ *   core/src/ray_query.cpp:12-195          (rc_ray_query)
  *
  * Semantic choices at the SYCL boundary (AdaptiveCpp is not available here):
  *   sycl::half(float) = IEEE binary16 round-to-nearest-even;
@@ -157,6 +158,14 @@ void rc_project_contact(const float *contact, float *a, float *b, float mu, floa
 void rc_leaf_extent(uint32_t stype, const float *half3, const float *q4, float *out3);
 /* Karras node i (broadphase.cpp:164-174): out = lo, hi, split, left, right. */
 void rc_karras_node(const uint64_t *codes, int n, int i, int32_t *out);
+
+/* Ray queries over the last build_and_query's LBVH (core/src/ray_query.cpp:126-195,
+ * core/include/core/ray_query.hpp:9-18): closest exact hit (oriented box slab test,
+ * analytic sphere) with t < t_max; body 0xFFFFFFFF and t = t_max when none. */
+typedef struct rc_ray { float ox, oy, oz, dx, dy, dz, t_max; } rc_ray;
+typedef struct rc_ray_hit { float t; uint32_t body; } rc_ray_hit;
+void rc_ray_query(const rc_world *w, uint32_t n, const rc_bodies *b, const rc_shapes *s, const rc_ray *rays,
+                  rc_ray_hit *hits, uint32_t n_rays);
 
 int rc_max_threads(void);
 void rc_set_threads(int n);

@@ -112,6 +112,7 @@ def lib():
         L.rc_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
         L.rc_leaf_extent.argtypes = [C.c_uint32, vp, vp, vp]
         L.rc_karras_node.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.rc_ray_query.argtypes = [vp, C.c_uint32, C.POINTER(RcBodies), C.POINTER(RcShapes), vp, vp, C.c_uint32]
         L.rc_max_threads.restype = C.c_int
         L.rc_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -278,6 +279,15 @@ class World:
         out = np.zeros(n, np.uint32)
         self.L.rc_solve_colors(self.h, out.ctypes.data, n)
         return out
+
+    def ray_query(self, bodies: Bodies, shapes: Shapes, n: int, rays: np.ndarray):
+        """Closest hits of rays (m, 7: origin, direction, t_max) against the
+        BVH of the last build_and_query over n bodies: (t[m], body[m])."""
+        r = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(r), 2), np.uint32)
+        self.L.rc_ray_query(self.h, n, C.byref(bodies.view()), C.byref(shapes.view()), r.ctypes.data,
+                            out.ctypes.data, len(r))
+        return out[:, 0].view(np.float32).copy(), out[:, 1].copy()
 
     def frame(self, bodies: Bodies, shapes: Shapes, sc, force_serial: bool = False) -> dict:
         st = RcFrameStats()
