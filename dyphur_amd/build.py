@@ -99,7 +99,8 @@ def build_python_module(verbose: bool = False) -> str | None:
     return out
 
 
-CPP_TEST_SOURCES = ["main.cpp", "test_stores_integrator.cpp", "test_pipeline.cpp", "test_compute.cpp", "test_ray_query.cpp"]
+CPP_TEST_SOURCES = ["main.cpp", "test_stores_integrator.cpp", "test_pipeline.cpp", "test_compute.cpp", "test_ray_query.cpp",
+                    "test_joints.cpp"]
 
 
 def build_cpp_tests(verbose: bool = False) -> str:

@@ -50,6 +50,16 @@ class ContactView(C.Structure):
     _fields_ = [(f, vp) for f, _ in CONTACT_FIELDS] + [("n", vp), ("capacity", u32)]
 
 
+JOINT_F32 = ("anchor_px", "anchor_py", "anchor_pz", "anchor_cx", "anchor_cy", "anchor_cz",
+             "axis_px", "axis_py", "axis_pz", "limit_lo", "limit_hi", "target_pos", "target_vel",
+             "stiffness", "damping", "compliance_pos", "compliance_ang")
+
+
+class JointView(C.Structure):
+    _fields_ = [("body_parent", vp), ("body_child", vp)] + [(f, vp) for f in JOINT_F32] + [
+        ("type", vp), ("n", u32)]
+
+
 class Aabb(C.Structure):
     _fields_ = [(n, f32) for n in ("min_x", "min_y", "min_z", "max_x", "max_y", "max_z")]
 
@@ -141,7 +151,7 @@ _SIGS = {
     "dy_xpbd_create": (C.c_int, [vp, C.c_int, u32, f32, C.POINTER(vp)]),
     "dy_xpbd_destroy": (C.c_int, [vp]),
     "dy_xpbd_set_force_serial": (C.c_int, [vp, C.c_int]),
-    "dy_xpbd_solve": (C.c_int, [vp, vp, C.POINTER(ContactView), u32, C.POINTER(BodyView), f32]),
+    "dy_xpbd_solve": (C.c_int, [vp, vp, C.POINTER(ContactView), C.POINTER(JointView), C.POINTER(BodyView), f32]),
     "dy_xpbd_last_stats": (C.c_int, [vp, C.POINTER(SolveStats)]),
     "dy_xpbd_download_classes": (C.c_int, [vp, vp, vp, u32]),
     "dy_xpbd_set_timing": (C.c_int, [vp, C.c_int]),
@@ -388,6 +398,36 @@ class DeviceBodies:
         return out.value
 
 
+class DeviceJoints:
+    """Device SoA joint arrays (the reference JointStore, JointView layout)
+    from a dict of host arrays: body_parent, body_child (uint32), type
+    (uint8: 0 fixed, 1 revolute, 2 prismatic, 3 ball) and the float32
+    fields of JOINT_F32."""
+
+    def __init__(self, joints: dict, s: Stream):
+        self.n = len(joints["body_parent"])
+        self.a = {"body_parent": DeviceArray.from_numpy(np.asarray(joints["body_parent"], np.uint32), s),
+                  "body_child": DeviceArray.from_numpy(np.asarray(joints["body_child"], np.uint32), s),
+                  "type": DeviceArray.from_numpy(np.asarray(joints["type"], np.uint8), s)}
+        for f in JOINT_F32:
+            a = np.asarray(joints[f], np.float32)
+            if len(a) != self.n:
+                raise ValueError(f"joint field {f} has {len(a)} entries, expected {self.n}")
+            self.a[f] = DeviceArray.from_numpy(a, s)
+
+    def set_targets(self, pos: np.ndarray, vel: np.ndarray, s: Stream):
+        """JointStore::set_targets (core/include/core/joint_store.hpp:31)."""
+        self.a["target_pos"].upload(np.asarray(pos, np.float32), s)
+        self.a["target_vel"].upload(np.asarray(vel, np.float32), s)
+
+    def view(self) -> JointView:
+        v = JointView()
+        for k, d in self.a.items():
+            setattr(v, k, d.ptr)
+        v.n = self.n
+        return v
+
+
 class DeviceShapes:
     def __init__(self, scene, s: Stream):
         self.n = scene.n_shapes
@@ -527,9 +567,11 @@ class XpbdSolver:
     def set_force_serial(self, v: bool):
         check(lib().dy_xpbd_set_force_serial(self.h, int(bool(v))), "dy_xpbd_set_force_serial")
 
-    def solve(self, s: Stream, contacts: ContactView, bodies: DeviceBodies, dt: float, n_joints: int = 0):
+    def solve(self, s: Stream, contacts: ContactView, bodies: DeviceBodies, dt: float,
+              joints: "DeviceJoints | None" = None):
         bv = bodies.view()
-        check(lib().dy_xpbd_solve(self.h, s.h, C.byref(contacts), n_joints, C.byref(bv), dt), "dy_xpbd_solve")
+        jv = C.byref(joints.view()) if joints is not None and joints.n else None
+        check(lib().dy_xpbd_solve(self.h, s.h, C.byref(contacts), jv, C.byref(bv), dt), "dy_xpbd_solve")
 
     def stats(self) -> dict:
         st = SolveStats()
@@ -571,6 +613,9 @@ class World:
         self.bp = Broadphase(self.s, scene.n, scene.max_pairs)
         self.np = Narrowphase(self.s, scene.max_contacts)
         self.solver = XpbdSolver(self.s, scene.n_iters, scene.max_contacts, scene.mu)
+        joints = getattr(scene, "joints", None)
+        self.joints = DeviceJoints(joints, self.s) if joints is not None else None
+        self.n_joints = self.joints.n if self.joints is not None else 0
         self.last = {}
 
     def step_async(self, force_serial: bool = False):
@@ -584,7 +629,7 @@ class World:
         pairs, count = self.bp.pairs_ptr()
         self.np.run_device_count(s, pairs, count, self.bp.max_pairs, self.bodies, self.shapes)
         self.solver.set_force_serial(force_serial)
-        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt)
+        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt, self.joints)
 
     def check(self) -> dict:
         """Blocks until the enqueued frames are done and raises on any device
@@ -604,9 +649,9 @@ class World:
         pairs, _ = self.bp.pairs_ptr()
         self.np.run(s, pairs, min(n_pairs, self.bp.max_pairs), self.bodies, self.shapes)
         self.solver.set_force_serial(force_serial)
-        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt)
+        self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt, self.joints)
         st = self.solver.stats()
-        self.last = dict(n_pairs=n_pairs, n_contacts=st["n_constraints"], n_colors=st["n_classes"],
+        self.last = dict(n_pairs=n_pairs, n_contacts=st["n_constraints"] - self.n_joints, n_colors=st["n_classes"],
                          used_serial=st["used_serial"], dyn_endpoints=st["dyn_endpoints"])
         return self.last
 
@@ -684,8 +729,10 @@ class Batch:
     def __init__(self, scene_list, s: Stream | None = None, n_iters: int | None = None,
                  friction: float | None = None):
         from . import scenes as _scenes
-        self.s = s or Stream()
         self.scenes = list(scene_list)
+        if any(getattr(sc, "joints", None) is not None for sc in self.scenes):
+            raise ValueError("the batched form solves contacts only; step scenes with joints with World")
+        self.s = s or Stream()
         self.merged = _scenes.concat(self.scenes)
         m = self.merged.meta
         self.body_offset = np.ascontiguousarray(m["body_offset"], np.uint32)

@@ -1,8 +1,10 @@
 #pragma once
 // XpbdSolver (core/include/core/xpbd_solver.hpp:29-48) over the gfx950 class
-// sweeps: <= kSerialThreshold contacts (or set_force_serial) gives the
-// reference's contact-order results, larger scenes its Jones-Plassmann colour
-// order.  solve() blocks (it reads the contact count).  Joints throw.
+// sweeps: <= kSerialThreshold constraints (contacts + joints) or
+// set_force_serial gives the reference's serial-order results, larger scenes
+// its Jones-Plassmann colour order.  solve() is asynchronous on the stream;
+// errors (invalid joints among them) surface at the next solve or
+// last_stats().
 #include "solver.hpp"
 
 #include <memory>
@@ -19,7 +21,8 @@ public:
 
     void solve(Stream &s, const ContactView &contacts, const JointView &joints, BodyView bodies,
                float dt) override {
-        check(dy_xpbd_solve(h_.get(), s.handle(), c_view(contacts), joints.n, c_view(bodies), dt),
+        check(dy_xpbd_solve(h_.get(), s.handle(), c_view(contacts), joints.n ? c_view(joints) : nullptr,
+                            c_view(bodies), dt),
               "XpbdSolver::solve");
     }
 

@@ -3,9 +3,10 @@
 // (Device, Stream, AABB, ShapeType, ShapeParams, ShapeStore, BodyParams,
 // BodyStore, Broadphase, Narrowphase, ContactSensor), with zero-copy numpy
 // views of the sensor arrays.  Additions: Device.default_gpu / preferred,
-// IntegratorParams + integrate, XpbdSolver, step (one frame of substeps),
-// state_hash, BodyStore.state (host copy of the body arrays).  This build has
-// no CPU device: Device.default_cpu raises.
+// IntegratorParams + integrate, JointType / JointParams / JointStore,
+// XpbdSolver (contacts and joints), step (one frame of substeps), state_hash,
+// BodyStore.state (host copy of the body arrays).  This build has no CPU
+// device: Device.default_cpu raises.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -16,6 +17,7 @@
 #include <core/broadphase.hpp>
 #include <core/contact_sensor.hpp>
 #include <core/integrator.hpp>
+#include <core/joint_store.hpp>
 #include <core/narrowphase.hpp>
 #include <core/shape_store.hpp>
 #include <core/step.hpp>
@@ -188,12 +190,51 @@ PYBIND11_MODULE(dyphur_py, m) {
 
     m.def("integrate", &integrate, "s"_a, "bodies"_a, "params"_a);
 
+    py::enum_<JointType>(m, "JointType")
+        .value("Fixed", JointType::Fixed)
+        .value("Revolute", JointType::Revolute)
+        .value("Prismatic", JointType::Prismatic)
+        .value("Ball", JointType::Ball);
+
+    py::class_<JointParams>(m, "JointParams")
+        .def(py::init<>())
+        .def_readwrite("body_parent", &JointParams::body_parent)
+        .def_readwrite("body_child", &JointParams::body_child)
+        .def("set_anchor_parent", [](JointParams &p, float x, float y, float z) { p.anchor_parent = Vec3f{x, y, z}; },
+             "x"_a, "y"_a, "z"_a)
+        .def("set_anchor_child", [](JointParams &p, float x, float y, float z) { p.anchor_child = Vec3f{x, y, z}; },
+             "x"_a, "y"_a, "z"_a)
+        .def("set_axis", [](JointParams &p, float x, float y, float z) { p.axis = Vec3f{x, y, z}; }, "x"_a, "y"_a,
+             "z"_a)
+        .def_readwrite("limit_lo", &JointParams::limit_lo)
+        .def_readwrite("limit_hi", &JointParams::limit_hi)
+        .def_readwrite("target_pos", &JointParams::target_pos)
+        .def_readwrite("target_vel", &JointParams::target_vel)
+        .def_readwrite("stiffness", &JointParams::stiffness)
+        .def_readwrite("damping", &JointParams::damping)
+        .def_readwrite("compliance_pos", &JointParams::compliance_pos)
+        .def_readwrite("compliance_ang", &JointParams::compliance_ang)
+        .def_readwrite("type", &JointParams::type);
+
+    py::class_<JointStore>(m, "JointStore")
+        .def(py::init<Stream &, uint32_t>(), "s"_a, "capacity"_a)
+        .def("add", &JointStore::add, "p"_a)
+        .def("upload", &JointStore::upload)
+        .def("set_targets", [](JointStore &js, const std::vector<float> &pos, const std::vector<float> &vel) {
+                 if (pos.size() != js.count() || vel.size() != js.count())
+                     throw py::value_error("set_targets: one target per joint");
+                 js.set_targets(pos.data(), vel.data());
+             }, "pos"_a, "vel"_a)
+        .def("count", &JointStore::count)
+        .def("capacity", &JointStore::capacity);
+
     py::class_<XpbdSolver>(m, "XpbdSolver")
         .def(py::init<Stream &, int, uint32_t, float>(), "s"_a, "n_iters"_a = 10, "max_contacts"_a = 16384,
              "friction"_a = 0.5f)
-        .def("solve", [](XpbdSolver &x, Stream &s, Narrowphase &np, BodyView bv, float dt) {
-                 x.solve(s, np.contacts(), JointView{}, bv, dt);
-             }, "s"_a, "np"_a, "bodies"_a, "dt"_a, "Solve the narrowphase's contacts (joints are not supported).")
+        .def("solve", [](XpbdSolver &x, Stream &s, Narrowphase &np, BodyView bv, float dt, JointStore *joints) {
+                 x.solve(s, np.contacts(), joints ? joints->view() : JointView{}, bv, dt);
+             }, "s"_a, "np"_a, "bodies"_a, "dt"_a, "joints"_a = nullptr,
+             "Solve the narrowphase's contacts and the joints of a JointStore (None: contacts only).")
         .def("set_force_serial", &XpbdSolver::set_force_serial)
         .def("last_stats", [](const XpbdSolver &x) {
                  const dy_solve_stats st = x.last_stats();

@@ -676,6 +676,7 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     if (n < 2) return DY_OK;
     const int ni = static_cast<int>(n);
     const uint32_t n_tiles = ceil_div(n, kSgTile);
+    bool sorted = false;
     if (bp->sort_blocks > 0 && n_tiles > 4) {
         const uint32_t blocks = n_tiles < bp->sort_blocks ? n_tiles : bp->sort_blocks;
         dy_body_view bv = *bodies;
@@ -685,9 +686,11 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
                  *tot = bp->sort_tot, *bar = bp->counters + 4, *err = bp->counters + 2;
         uint64_t *m64 = bp->morton64;
         void *args[] = {&bv, &bd, &nv, &ntv, &c0, &i0, &c1, &i1, &mat, &tot, &m64, &bar, &err};
-        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(morton_sort_grid), dim3(blocks),
-                                          dim3(kSgThreads), args, 0, st));
-    } else {
+        sorted = try_coop_launch(reinterpret_cast<const void *>(morton_sort_grid), dim3(blocks), dim3(kSgThreads),
+                                 args, st) == hipSuccess;
+        if (!sorted) bp->sort_blocks = 0;  // refused: the single-workgroup sort from now on
+    }
+    if (!sorted) {
         morton_sort<<<1, kMsThreads, 0, st>>>(*bodies, *bounds, n, bp->code0, bp->idx0, bp->code1, bp->idx1,
                                               bp->morton64);
         DY_LAUNCH_CHECK();

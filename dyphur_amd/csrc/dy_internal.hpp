@@ -22,6 +22,17 @@
 
 #define DY_LAUNCH_CHECK() DY_TRY(hipGetLastError())
 
+// A cooperative launch that may be refused: another process that has made
+// cooperative launches on the device holds them (hipErrorIllegalState), or
+// the grid does not fit.  Nothing is enqueued then; the runtime's last error
+// is cleared so the refusal does not resurface at the next launch check, and
+// the caller takes its non-cooperative path.
+inline hipError_t try_coop_launch(const void *fn, dim3 grid, dim3 block, void **args, hipStream_t st) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, st);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
+}
+
 namespace dy {
 
 inline hipStream_t hs(dy_stream_t s) { return reinterpret_cast<hipStream_t>(s); }

@@ -2,6 +2,7 @@
 // host (hipcc --cuda-host-only), exported with a C ABI so the CPU test suite
 // can compare the exact code the kernels run against the oracle without a GPU.
 #include "adjoint.hpp"
+#include "joint_math.hpp"
 
 #include <vector>
 
@@ -82,6 +83,34 @@ void hm_integrate_body(float *kin, float gx, float gy, float gz, float h, int n_
 }
 
 void hm_renormalise(float *q4) { renormalise(q4[0], q4[1], q4[2], q4[3]); }
+
+float hm_atan2_cr(float y, float x) { return atan2_cr(y, x); }
+void hm_atan2_cr_array(const float *y, const float *x, float *out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = atan2_cr(y[i], x[i]);
+}
+
+// A two-body solve with one joint (parent a, child b; body[20] as above):
+// the motor once, then iters x (the joint, renormalisation of both bodies),
+// the reference's run_serial sequence.  jp[17] = anchor_p xyz, anchor_c xyz,
+// axis xyz, lo, hi, target_pos, target_vel, stiffness, damping,
+// compliance_pos, compliance_ang.
+void hm_joint_pair(float *a, float *b, const float *jp, uint32_t type, float dt, int iters) {
+    JointIn j;
+    static_assert(sizeof(JointIn) == 18 * sizeof(float), "JointIn layout");
+    __builtin_memcpy(&j, jp, 17 * sizeof(float));
+    j.type = type;
+    BodyState A, B;
+    __builtin_memcpy(&A, a, sizeof(A));
+    __builtin_memcpy(&B, b, sizeof(B));
+    joint_motor(A, B, j, dt);
+    for (int it = 0; it < iters; ++it) {
+        solve_joint(j, A, B, dt);
+        renormalise(A.qw, A.qx, A.qy, A.qz);
+        renormalise(B.qw, B.qx, B.qy, B.qz);
+    }
+    __builtin_memcpy(a, &A, sizeof(A));
+    __builtin_memcpy(b, &B, sizeof(B));
+}
 
 }  // extern "C"
 

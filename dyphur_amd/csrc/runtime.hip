@@ -34,7 +34,7 @@ const char *dy_status_string(int status) {
     switch (status) {
         case DY_OK: return "success";
         case DY_ERR_INVALID_ARG: return "invalid argument or capacity exceeded";
-        case DY_ERR_UNSUPPORTED: return "unsupported input (joints, convex-hull or triangle-mesh shapes)";
+        case DY_ERR_UNSUPPORTED: return "unsupported input (convex-hull or triangle-mesh shapes)";
         case DY_ERR_NO_DEVICE: return "no HIP device";
         case DY_ERR_INTERNAL: return "internal consistency check failed";
         default: break;

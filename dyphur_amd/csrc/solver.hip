@@ -44,6 +44,7 @@
 #include <cstdlib>
 
 #include "dy_internal.hpp"
+#include "joint_math.hpp"
 
 namespace dy {
 
@@ -65,8 +66,11 @@ enum : uint32_t {
     kMDyn = 6,           // dynamic endpoints = entries of the per-body lists
     kMNcls = 7,          // classes swept
     kMSerial = 8,        // 1: reference contact order
-    kMNc = 9,            // contacts solved (device count clamped to capacity)
+    kMNc = 9,            // constraints solved: contacts (count clamped to capacity) + joints
     kMColourRounds = 10, // colouring rounds before a fallback (257: more than 256)
+    kMNcc = 11,          // contacts solved; constraint k >= kMNcc is joint k - kMNcc
+    kMNj = 12,           // joints solved
+    kMJBad = 13,         // a joint of this solve failed validation
     kMBarAssign = 16,    // 2 words: grid barrier of assign_lists
     kMBarSweep = 18,     // 2 words: grid barrier of sweep_persistent
     kMFcnt = 20,         // 3 words: frontier sizes
@@ -74,54 +78,123 @@ enum : uint32_t {
     kMRegAssign = 26,    // 2 words: check-in of assign_lists
     kMWords = 32,
 };
-constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u, kErrFlow = 16u;
+constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u, kErrFlow = 16u, kErrJoint = 32u;
 
 __device__ __forceinline__ uint32_t cls_of(uint64_t s) { return static_cast<uint32_t>(s & 0xffffffffu); }
 
 __device__ __forceinline__ uint32_t grid_stride() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// Zeroes the per-solve state and publishes the contact count and the path:
-// the reference orders contacts serially at <= 256 contacts or when forced.
-__global__ void solve_prep(uint32_t nb, const uint32_t *__restrict__ ncp, uint32_t cap, int force_serial,
-                           uint32_t *__restrict__ deg, uint32_t *__restrict__ fill, uint64_t *__restrict__ used,
-                           uint32_t *__restrict__ ver, float *__restrict__ lam, uint32_t *__restrict__ hist,
-                           uint32_t *__restrict__ meta) {
+// Zeroes the per-solve state and publishes the constraint counts and the
+// path: the reference runs its constraints serially at <= 256 constraints
+// (contacts + joints) or when forced (xpbd_solver.cpp:611-619).
+__global__ void solve_prep(uint32_t nb, const uint32_t *__restrict__ ncp, uint32_t cap, uint32_t nj,
+                           int force_serial, uint32_t *__restrict__ deg, uint32_t *__restrict__ fill,
+                           uint64_t *__restrict__ used, uint32_t *__restrict__ ver, float *__restrict__ lam,
+                           uint32_t *__restrict__ hist, uint32_t *__restrict__ meta) {
     const uint32_t n_raw = *ncp;
     const uint32_t nc = n_raw < cap ? n_raw : cap;
-    const uint32_t top = nb > nc ? nb : nc;
+    const uint32_t ncon = nc + nj;
+    const uint32_t top = nb > ncon ? nb : ncon;
     for (uint32_t i = grid_tid(); i < top; i += grid_stride()) {
         if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; ver[i] = 0; }
-        if (i < nc) lam[i] = 0.f;
+        if (i < ncon) lam[i] = 0.f;
     }
     const uint32_t i = grid_tid();
     if (i <= kMaxColours) hist[i] = 0;
     if (i < kMWords && i != kMErr) {
         uint32_t v = 0;
-        if (i == kMNc) v = nc;
-        if (i == kMSerial) v = (force_serial || nc <= 256u) ? 1u : 0u;
+        if (i == kMNc) v = ncon;
+        if (i == kMNcc) v = nc;
+        if (i == kMNj) v = nj;
+        if (i == kMSerial) v = (force_serial || ncon <= 256u) ? 1u : 0u;
         meta[i] = v;
     }
     if (i == 0 && n_raw > cap) atomicOr(&meta[kMErr], kErrCapacity);
 }
 
-__global__ void solve_degree(dy_contact_view cv, const uint32_t *__restrict__ meta, const float *__restrict__ im,
+// Constraint k of a solve: contact k (k < kMNcc) or joint k - kMNcc, whose
+// validated (parent, child) joint_pack wrote to jend.
+struct Cons {
+    dy_contact_view cv;
+    const uint2 *jend;
+};
+
+__device__ __forceinline__ void cons_ends(const Cons &cs, uint32_t k, uint32_t ncc, uint32_t &a, uint32_t &b) {
+    if (k < ncc) {
+        a = cs.cv.body_a[k];
+        b = cs.cv.body_b[k];
+    } else {
+        const uint2 e = cs.jend[k - ncc];
+        a = e.x;
+        b = e.y;
+    }
+}
+
+// Joint parameters as 5 float4 per joint: (anchor_p xyz, anchor_c x)
+// (anchor_c yz, axis xy) (axis z, lo, hi, target_pos) (target_vel,
+// stiffness, damping, compliance_pos) (compliance_ang, type, -, -).  A joint
+// with a body index out of range, parent == child or a type above 3 is
+// flagged (kMJBad, kErrJoint) and the solve then runs without joints.
+__global__ void joint_pack(dy_joint_view jv, uint32_t nb, float4 *__restrict__ jpk, uint2 *__restrict__ jend,
+                           uint32_t *__restrict__ meta) {
+    const uint32_t j = grid_tid();
+    if (j >= jv.n) return;
+    const uint32_t p = jv.body_parent[j], c = jv.body_child[j], t = jv.type[j];
+    if (p >= nb || c >= nb || p == c || t > kJointBall) {
+        atomicOr(&meta[kMJBad], 1u);
+        atomicOr(&meta[kMErr], kErrJoint);
+        jend[j] = make_uint2(0u, 0u);
+        return;
+    }
+    jend[j] = make_uint2(p, c);
+    jpk[5 * j + 0] = make_float4(jv.anchor_px[j], jv.anchor_py[j], jv.anchor_pz[j], jv.anchor_cx[j]);
+    jpk[5 * j + 1] = make_float4(jv.anchor_cy[j], jv.anchor_cz[j], jv.axis_px[j], jv.axis_py[j]);
+    jpk[5 * j + 2] = make_float4(jv.axis_pz[j], jv.limit_lo[j], jv.limit_hi[j], jv.target_pos[j]);
+    jpk[5 * j + 3] = make_float4(jv.target_vel[j], jv.stiffness[j], jv.damping[j], jv.compliance_pos[j]);
+    jpk[5 * j + 4] = make_float4(jv.compliance_ang[j], __uint_as_float(t), 0.f, 0.f);
+}
+
+// After a failed joint validation: the solve proceeds with the contacts only.
+__global__ void joint_commit(int force_serial, uint32_t *__restrict__ meta) {
+    if (threadIdx.x != 0 || meta[kMJBad] == 0u) return;
+    const uint32_t nc = meta[kMNcc];
+    meta[kMNj] = 0u;
+    meta[kMNc] = nc;
+    meta[kMSerial] = (force_serial || nc <= 256u) ? 1u : 0u;
+}
+
+__device__ __forceinline__ JointIn load_joint(const float4 *__restrict__ jpk, uint32_t j) {
+    const float4 a = jpk[5 * j], b = jpk[5 * j + 1], c = jpk[5 * j + 2], d = jpk[5 * j + 3], e = jpk[5 * j + 4];
+    JointIn r;
+    r.apx = a.x; r.apy = a.y; r.apz = a.z; r.acx = a.w;
+    r.acy = b.x; r.acz = b.y; r.lx = b.z; r.ly = b.w;
+    r.lz = c.x; r.lo = c.y; r.hi = c.z; r.target_pos = c.w;
+    r.target_vel = d.x; r.stiffness = d.y; r.damping = d.z; r.compliance_pos = d.w;
+    r.compliance_ang = e.x;
+    r.type = __float_as_uint(e.y);
+    return r;
+}
+
+__global__ void solve_degree(Cons cs, const uint32_t *__restrict__ meta, const float *__restrict__ im,
                              uint32_t *__restrict__ deg) {
-    const uint32_t nc = meta[kMNc];
+    const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
     for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
-        const uint32_t a = cv.body_a[c], b = cv.body_b[c];
+        uint32_t a, b;
+        cons_ends(cs, c, ncc, a, b);
         if (im[a] != 0.f) atomicAdd(&deg[a], 1u);
         if (im[b] != 0.f) atomicAdd(&deg[b], 1u);
     }
 }
 
 // Unordered per-body lists of dynamic endpoints: adj = (c << 1 | end), own = body.
-__global__ void solve_fill(dy_contact_view cv, const uint32_t *__restrict__ meta, const float *__restrict__ im,
+__global__ void solve_fill(Cons cs, const uint32_t *__restrict__ meta, const float *__restrict__ im,
                            const uint32_t *__restrict__ off, uint32_t *__restrict__ fill, uint32_t *__restrict__ adj,
                            uint32_t *__restrict__ own) {
-    const uint32_t nc = meta[kMNc];
+    const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
     for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
-        const uint32_t a = cv.body_a[c], b = cv.body_b[c];
+        uint32_t a, b;
+        cons_ends(cs, c, ncc, a, b);
         if (im[a] != 0.f) {
             const uint32_t p = off[a] + atomicAdd(&fill[a], 1u);
             adj[p] = c << 1;
@@ -207,11 +280,12 @@ __device__ __forceinline__ void rank_entry(uint32_t p, const uint32_t *__restric
 // lists it does not head yet (pending); returns true when it heads all of
 // them.  lrec[c] = (a or none, b or none, next on a, next on b); static
 // endpoints are "none".
-__device__ __forceinline__ bool link_contact(uint32_t c, const dy_contact_view &cv, const float *__restrict__ im,
+__device__ __forceinline__ bool link_contact(uint32_t c, const Cons &cs, uint32_t ncc, const float *__restrict__ im,
                                              const uint32_t *__restrict__ off, const uint32_t *__restrict__ deg,
                                              const uint32_t *__restrict__ slot, const uint32_t *__restrict__ crank,
                                              uint4 *__restrict__ lrec, uint32_t *__restrict__ pending) {
-    uint32_t ends[2] = {cv.body_a[c], cv.body_b[c]};
+    uint32_t ends[2];
+    cons_ends(cs, c, ncc, ends[0], ends[1]);
     uint32_t next[2] = {kNone, kNone};
     uint32_t wait = 0;
 #pragma unroll
@@ -243,7 +317,7 @@ __global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__res
 
 // Contacts heading every list of a dynamic endpoint form the first frontier
 // (fr0, size in fcnt[0]).
-__global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, const uint32_t *__restrict__ meta,
+__global__ __launch_bounds__(kSolveThreads) void list_link(Cons cs, const uint32_t *__restrict__ meta,
                                                            const float *__restrict__ im,
                                                            const uint32_t *__restrict__ off,
                                                            const uint32_t *__restrict__ deg,
@@ -251,11 +325,11 @@ __global__ __launch_bounds__(kSolveThreads) void list_link(dy_contact_view cv, c
                                                            const uint32_t *__restrict__ crank,
                                                            uint4 *__restrict__ lrec, uint32_t *__restrict__ pending,
                                                            uint32_t *__restrict__ frontier, uint32_t *fcount) {
-    const uint32_t nc = meta[kMNc];
+    const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
     const uint32_t lane = lane_id();
     for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
         const uint32_t c = c0 + lane;
-        const bool pred[1] = {c < nc && link_contact(c, cv, im, off, deg, slot, crank, lrec, pending)};
+        const bool pred[1] = {c < nc && link_contact(c, cs, ncc, im, off, deg, slot, crank, lrec, pending)};
         const uint32_t val[1] = {c};
         wave_append<1>(pred, val, frontier, fcount);
     }
@@ -298,7 +372,7 @@ __device__ __forceinline__ uint32_t assign_one(uint32_t c, uint32_t round, int s
 
 // Buffers of the cooperative class assignment.
 struct AssignBufs {
-    dy_contact_view cv;
+    Cons cons;
     const float *im;
     const uint32_t *adj, *own, *off, *deg;
     uint32_t *slot, *crank;
@@ -397,8 +471,8 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         xcd_barrier(bar, gen, nblk, err, kErrBarrier, true);
         for (uint32_t c0 = wave0; c0 < nc; c0 += gstride) {
             const uint32_t c = c0 + lane;
-            const bool pred[1] = {c < nc && link_contact(c, B.cv, B.im, B.off, B.deg, B.slot, B.crank, B.lrec,
-                                                         B.pending)};
+            const bool pred[1] = {c < nc && link_contact(c, B.cons, meta[kMNcc], B.im, B.off, B.deg, B.slot,
+                                                         B.crank, B.lrec, B.pending)};
             const uint32_t val[1] = {c};
             wave_append<1>(pred, val, B.fr0, &fcnt[0]);
         }
@@ -494,13 +568,16 @@ __global__ __launch_bounds__(kSolveThreads) void class_count(const uint32_t *__r
     }
 }
 
-// Contact record for the sweeps, class-sorted: (a, b), (p, n.x), (n.y, n.z, depth, -).
+// Constraint record for the sweeps, class-sorted: (a, b), (p, n.x), (n.y,
+// n.z, depth, 0) for a contact; (parent, child), (joint index, -, -, -),
+// (-, -, -, kRecJoint) for a joint.
+constexpr uint32_t kRecJoint = 1u;
 struct CRec {
     uint2 *ab;
     float4 *p0, *p1;
 };
 
-__global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv, uint32_t *meta,
+__global__ __launch_bounds__(kSolveThreads) void class_gather(Cons cs, uint32_t *meta,
                                                               const uint64_t *__restrict__ state,
                                                               const uint32_t *__restrict__ boff,
                                                               uint32_t *__restrict__ bfill, CRec rec,
@@ -509,7 +586,7 @@ __global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv
                                                               uint4 *__restrict__ rseq) {
     __shared__ uint32_t lh[kLdsClasses];
     __shared__ uint32_t lbase[kLdsClasses];
-    const uint32_t nc = meta[kMNc], n_cls = meta[kMNcls];
+    const uint32_t nc = meta[kMNc], n_cls = meta[kMNcls], ncc = meta[kMNcc];
     const bool local = n_cls <= kLdsClasses;
     for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += grid_stride()) {
         const uint32_t c = c0 + threadIdx.x;
@@ -535,9 +612,16 @@ __global__ __launch_bounds__(kSolveThreads) void class_gather(dy_contact_view cv
         if (valid) {
             const uint32_t pos = boff[k] + (local ? lbase[k] + rank : rank);
             if (pos < nc) {
-                rec.ab[pos] = make_uint2(cv.body_a[c], cv.body_b[c]);
-                rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
-                rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
+                const dy_contact_view &cv = cs.cv;
+                if (c < ncc) {
+                    rec.ab[pos] = make_uint2(cv.body_a[c], cv.body_b[c]);
+                    rec.p0[pos] = make_float4(cv.pos_x[c], cv.pos_y[c], cv.pos_z[c], cv.norm_x[c]);
+                    rec.p1[pos] = make_float4(cv.norm_y[c], cv.norm_z[c], cv.depth[c], 0.f);
+                } else {
+                    rec.ab[pos] = cs.jend[c - ncc];
+                    rec.p0[pos] = make_float4(__uint_as_float(c - ncc), 0.f, 0.f, 0.f);
+                    rec.p1[pos] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kRecJoint));
+                }
                 rec_id[pos] = c;
                 cpos[c] = pos;
                 rseq[pos] = make_uint4(0u, 0u, 0u, 0u);
@@ -588,40 +672,89 @@ __device__ __forceinline__ void catch_up(BodyState &s, uint32_t done, uint32_t t
     for (uint32_t r = done; r < target; ++r) renormalise(s.qw, s.qx, s.qy, s.qz);
 }
 
+// Applies record (p0, p1) in iteration it to the loaded bodies, first
+// brought to `it` renormalisations.  For a joint this includes a static
+// body, in this private copy only: the joint reads its orientation, which
+// the reference renormalises after every iteration like any other body's.
+template <bool JOINTS>
+__device__ __forceinline__ void apply_record(const float4 &p0, const float4 &p1, BodyState &A, BodyState &B,
+                                             uint32_t ra, uint32_t rb, uint32_t it, float &l, float mu, float dt,
+                                             const float4 *__restrict__ jpk) {
+    if (JOINTS && __float_as_uint(p1.w) == kRecJoint) {
+        catch_up(A, ra, it);
+        catch_up(B, rb, it);
+        solve_joint(load_joint(jpk, __float_as_uint(p0.x)), A, B, dt);
+        return;
+    }
+    if (A.im != 0.f) catch_up(A, ra, it);
+    if (B.im != 0.f) catch_up(B, rb, it);
+    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+    project_contact(ci, A, B, mu, l, dt);
+}
+
+template <bool JOINTS>
 __device__ __forceinline__ void solve_one(uint32_t i, uint32_t it, const CRec &rec, float *__restrict__ lam,
                                           float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
-                                          float dt) {
+                                          float dt, const float4 *__restrict__ jpk) {
     const uint2 ab = rec.ab[i];
     const float4 p0 = rec.p0[i], p1 = rec.p1[i];
-    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
     uint32_t ra, rb;
     BodyState A = load_aos(dyn, sta, ab.x, ra), B = load_aos(dyn, sta, ab.y, rb);
     const bool da = A.im != 0.f, db = B.im != 0.f;
-    if (da) catch_up(A, ra, it);
-    if (db) catch_up(B, rb, it);
     float l = lam[i];
-    project_contact(ci, A, B, mu, l, dt);
+    apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
     lam[i] = l;
     if (da) store_aos(dyn, ab.x, A, it);
     if (db) store_aos(dyn, ab.y, B, it);
 }
 
+// Joint motors, once per solve before the first iteration, class by class
+// (xpbd_solver.cpp:566-577 in joint order, 703-719 in colour order; a
+// body's joints run in index order on the reference-order path because the
+// dependency levels follow it).  One workgroup: a class's joints share no
+// dynamic body, and the workgroup barrier orders the classes.
+__global__ __launch_bounds__(1024) void joint_motors(const uint32_t *__restrict__ boff, CRec rec,
+                                                     float4 *__restrict__ dyn, const float4 *__restrict__ sta,
+                                                     const float4 *__restrict__ jpk, float dt,
+                                                     const uint32_t *__restrict__ meta) {
+    if ((meta[kMErr] & kErrAssign) || meta[kMNj] == 0u) return;
+    const uint32_t n_cls = meta[kMNcls];
+    for (uint32_t k = 0; k < n_cls; ++k) {
+        const uint32_t end = boff[k + 1];
+        for (uint32_t i = boff[k] + threadIdx.x; i < end; i += blockDim.x) {
+            if (__float_as_uint(rec.p1[i].w) != kRecJoint) continue;
+            const JointIn j = load_joint(jpk, __float_as_uint(rec.p0[i].x));
+            if (j.type != kJointRevolute && j.type != kJointPrismatic) continue;
+            const uint2 ab = rec.ab[i];
+            uint32_t ra, rb;
+            BodyState A = load_aos(dyn, sta, ab.x, ra), B = load_aos(dyn, sta, ab.y, rb);
+            joint_motor(A, B, j, dt);
+            if (A.im != 0.f) store_aos(dyn, ab.x, A, ra);
+            if (B.im != 0.f) store_aos(dyn, ab.y, B, rb);
+        }
+        __syncthreads();
+    }
+}
+
 // One class (fallback path: one launch per class).
+template <bool JOINTS>
 __global__ __launch_bounds__(kSolveThreads) void sweep_class(const uint32_t *__restrict__ boff, uint32_t k, uint32_t it,
                                                              CRec rec, float *__restrict__ lam, float4 *__restrict__ dyn,
-                                                             const float4 *__restrict__ sta, float mu, float dt) {
+                                                             const float4 *__restrict__ sta, float mu, float dt,
+                                                             const float4 *__restrict__ jpk) {
     const uint32_t beg = boff[k], end = boff[k + 1];
     for (uint32_t i = beg + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x)
-        solve_one(i, it, rec, lam, dyn, sta, mu, dt);
+        solve_one<JOINTS>(i, it, rec, lam, dyn, sta, mu, dt, jpk);
 }
 
 // All I x K phases in one cooperative launch; the class count comes from the
 // assignment (meta), the phases are separated by grid barriers.  Nothing is
 // swept after an assignment error.
+template <bool JOINTS>
 __global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restrict__ boff, uint32_t n_iters,
                                                          CRec rec, float *__restrict__ lam, float4 *__restrict__ dyn,
                                                          const float4 *__restrict__ sta, float mu, float dt,
-                                                         uint32_t *meta) {
+                                                         uint32_t *meta, const float4 *__restrict__ jpk) {
     uint32_t gen = 0;
     uint32_t *bar = meta + kMBarSweep, *err = meta + kMErr;
     const uint32_t n_cls = (meta[kMErr] & kErrAssign) ? 0u : meta[kMNcls];
@@ -629,7 +762,8 @@ __global__ __launch_bounds__(1024) void sweep_persistent(const uint32_t *__restr
     for (uint32_t it = 0; it < n_iters; ++it) {
         for (uint32_t k = 0; k < n_cls; ++k) {
             const uint32_t beg = boff[k], end = boff[k + 1];
-            for (uint32_t i = beg + gtid; i < end; i += gstride) solve_one(i, it, rec, lam, dyn, sta, mu, dt);
+            for (uint32_t i = beg + gtid; i < end; i += gstride)
+                solve_one<JOINTS>(i, it, rec, lam, dyn, sta, mu, dt, jpk);
             if (it + 1 < n_iters || k + 1 < n_cls) grid_barrier(bar, gen, err, kErrBarrier);
         }
     }
@@ -669,19 +803,17 @@ __device__ __forceinline__ BodyState load_aos_agent(const float4 *__restrict__ d
     return s;
 }
 
+template <bool JOINTS>
 __device__ __forceinline__ void solve_one_agent(uint32_t i, uint32_t it, const CRec &rec, float *__restrict__ lam,
                                                 float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
-                                                float dt) {
+                                                float dt, const float4 *__restrict__ jpk) {
     const uint2 ab = rec.ab[i];
     const float4 p0 = rec.p0[i], p1 = rec.p1[i];
-    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
     uint32_t ra, rb;
     BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
     const bool da = A.im != 0.f, db = B.im != 0.f;
-    if (da) catch_up(A, ra, it);
-    if (db) catch_up(B, rb, it);
     float l = lam[i];  // lam[i] is only ever touched by this thread
-    project_contact(ci, A, B, mu, l, dt);
+    apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
     lam[i] = l;
     if (da) store_aos(dyn, ab.x, A, it);
     if (db) store_aos(dyn, ab.y, B, it);
@@ -708,17 +840,15 @@ __device__ __forceinline__ void pre_load(CPre &c, uint32_t i, uint32_t end, cons
     }
 }
 
+template <bool JOINTS>
 __device__ __forceinline__ void solve_pre(const CPre &c, uint32_t it, float *__restrict__ lam,
                                           float4 *__restrict__ dyn, const float4 *__restrict__ sta, float mu,
-                                          float dt) {
-    const ContactIn ci{c.p0.x, c.p0.y, c.p0.z, c.p0.w, c.p1.x, c.p1.y, c.p1.z};
+                                          float dt, const float4 *__restrict__ jpk) {
     uint32_t ra, rb;
     BodyState A = load_aos_agent(dyn, sta, c.ab.x, ra), B = load_aos_agent(dyn, sta, c.ab.y, rb);
     const bool da = A.im != 0.f, db = B.im != 0.f;
-    if (da) catch_up(A, ra, it);
-    if (db) catch_up(B, rb, it);
     float l = c.l;
-    project_contact(ci, A, B, mu, l, dt);
+    apply_record<JOINTS>(c.p0, c.p1, A, B, ra, rb, it, l, mu, dt, jpk);
     lam[c.i] = l;
     if (da) store_aos(dyn, c.ab.x, A, it);
     if (db) store_aos(dyn, c.ab.y, B, it);
@@ -728,10 +858,11 @@ __device__ __forceinline__ void solve_pre(const CPre &c, uint32_t it, float *__r
 // workgroups on every XCD (cooperative, all co-resident); each checks in once
 // at agent scope, the others leave, and the participants number themselves
 // densely.  Contact i of a phase always goes to the same thread.
+template <bool JOINTS>
 __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ boff, uint32_t n_iters, CRec rec,
                                                   float *__restrict__ lam, float4 *__restrict__ dyn,
                                                   const float4 *__restrict__ sta, float mu, float dt,
-                                                  uint32_t *meta) {
+                                                  uint32_t *meta, const float4 *__restrict__ jpk) {
     uint32_t *bar = meta + kMBarSweep, *err = meta + kMErr;
     uint32_t b = 0, nblk = 0;
     if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
@@ -746,8 +877,9 @@ __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ b
         for (uint32_t k = 0; k < n_cls; ++k) {
             const uint32_t end = boff[k + 1];
             const CPre cur = pf;
-            if (cur.valid) solve_pre(cur, it, lam, dyn, sta, mu, dt);
-            for (uint32_t i = cur.i + gstride; i < end; i += gstride) solve_one_agent(i, it, rec, lam, dyn, sta, mu, dt);
+            if (cur.valid) solve_pre<JOINTS>(cur, it, lam, dyn, sta, mu, dt, jpk);
+            for (uint32_t i = cur.i + gstride; i < end; i += gstride)
+                solve_one_agent<JOINTS>(i, it, rec, lam, dyn, sta, mu, dt, jpk);
             const bool last_k = k + 1 == n_cls;
             if (it + 1 == n_iters && last_k) break;
             const uint32_t kn = last_k ? 0u : k + 1u;
@@ -802,10 +934,11 @@ __global__ __launch_bounds__(kSolveThreads) void flow_seq(const uint32_t *__rest
     }
 }
 
+template <bool JOINTS>
 __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
                                                    float *__restrict__ lam, float4 *__restrict__ dyn,
                                                    const float4 *__restrict__ sta, uint32_t *ver, float mu,
-                                                   float dt, uint32_t *meta) {
+                                                   float dt, uint32_t *meta, const float4 *__restrict__ jpk) {
     uint32_t *err = meta + kMErr;
     uint32_t b = 0, nblk = 0;
     if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
@@ -836,13 +969,10 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
                     const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
                                        (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
                     if (ready) {
-                        const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
                         uint32_t ra, rb;
                         BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
                         const bool da = A.im != 0.f, db = B.im != 0.f;
-                        if (da) catch_up(A, ra, it);
-                        if (db) catch_up(B, rb, it);
-                        project_contact(ci, A, B, mu, l, dt);
+                        apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
                         lam[i] = l;
                         if (da) store_aos(dyn, ab.x, A, it);
                         if (db) store_aos(dyn, ab.y, B, it);
@@ -875,10 +1005,11 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
 // and ready, so the sweep always progresses.  Waves sleep only when none of
 // their lanes advanced; a wave with no progress for kGridSpinLimit polls
 // aborts the sweep (kErrFlow).
+template <bool JOINTS>
 __global__ __launch_bounds__(1024) void sweep_flow_lanes(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
                                                          float *__restrict__ lam, float4 *__restrict__ dyn,
                                                          const float4 *__restrict__ sta, uint32_t *ver, float mu,
-                                                         float dt, uint32_t *meta) {
+                                                         float dt, uint32_t *meta, const float4 *__restrict__ jpk) {
     uint32_t *err = meta + kMErr;
     uint32_t b = 0, nblk = 0;
     if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, b, nblk)) return;
@@ -910,13 +1041,10 @@ __global__ __launch_bounds__(1024) void sweep_flow_lanes(uint32_t n_iters, CRec 
             const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
                                (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
             if (ready) {
-                const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
                 uint32_t ra, rb;
                 BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
                 const bool da = A.im != 0.f, db = B.im != 0.f;
-                if (da) catch_up(A, ra, it);
-                if (db) catch_up(B, rb, it);
-                project_contact(ci, A, B, mu, l, dt);
+                apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
                 lam[i] = l;
                 if (da) store_aos(dyn, ab.x, A, it);
                 if (db) store_aos(dyn, ab.y, B, it);
@@ -994,11 +1122,20 @@ struct dy_xpbd_s {
     bool meta_pending = false;
     uint32_t last_nb = 0;
     dy_solve_stats stats{};
-    int coop_assign_max = 0, coop_sweep_max = 0, coop_xcd_max = 0, coop_flow_max = 0, device_cus = 0;
+    // Co-resident workgroup limits of the cooperative kernels, [0] contacts
+    // only, [1] the variants that also solve joints.
+    int coop_assign_max = 0, coop_sweep_max[2] = {0, 0}, coop_xcd_max[2] = {0, 0}, coop_flow_max[2] = {0, 0},
+        coop_lanes_max[2] = {0, 0}, device_cus = 0;
+    // Constraint capacity (contacts + joints) of the constraint-indexed
+    // buffers, and the joint capacity of jpk / jend.
+    uint32_t ccap = 0, jcap = 0;
+    float4 *jpk = nullptr;
+    uint2 *jend = nullptr;
     int sweep_mode = 2;  // 2 sweep_flow (default), 0 sweep_xcd, 1 sweep_persistent (whole grid)
     int sweep_blocks_override = 0, assign_blocks_override = 0;
     int assign_threads = 256, sweep_threads = 256;
     int path = 0;  // 0 cooperative kernels, 1 one launch per round / per class
+    bool coop_refused = false;  // a cooperative launch was refused: launches per round / per class from now on
     // Kernel timing (dy_xpbd_set_timing): event pairs recorded around the
     // assignment and sweep kernels, read back once the next solve syncs.
     bool timing = false;
@@ -1027,6 +1164,7 @@ int sync_meta(dy_xpbd_s *x) {
     x->stats.rounds = m[kMRounds];
     x->stats.overflow = m[kMOverflow];
     x->stats.dyn_endpoints = m[kMDyn];
+    if (m[kMErr] == kErrJoint) return DY_ERR_INVALID_ARG;
     return m[kMErr] ? DY_ERR_INTERNAL : DY_OK;
 }
 
@@ -1050,14 +1188,76 @@ int collect_timing(dy_xpbd_s *x) {
     return DY_OK;
 }
 
+// The constraint-indexed buffers, for c constraints (contacts + joints).
+void free_constraint_bufs(dy_xpbd_s *x) {
+    void **ptrs[] = {(void **)&x->lam, (void **)&x->state, (void **)&x->adj, (void **)&x->own, (void **)&x->slot,
+                     (void **)&x->crank, (void **)&x->pending, (void **)&x->fr0, (void **)&x->fr1,
+                     (void **)&x->lrec, (void **)&x->boff, (void **)&x->bfill, (void **)&x->hist,
+                     (void **)&x->rec_id, (void **)&x->cpos, (void **)&x->rseq, (void **)&x->rec_ab,
+                     (void **)&x->rec_p0, (void **)&x->rec_p1};
+    for (void **p : ptrs)
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    x->ccap = 0;
+}
+
+hipError_t alloc_constraint_bufs(dy_xpbd_s *x, uint32_t n_con) {
+    free_constraint_bufs(x);
+    const size_t c = n_con ? n_con : 1;
+    const size_t ncls = c + 66;  // dependency levels <= constraints; colours <= 65
+    hipError_t e = hipSuccess;
+    auto dalloc = [&e](auto *&p, size_t n) {
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(*p));
+    };
+    dalloc(x->lam, c);
+    dalloc(x->state, c);
+    dalloc(x->adj, 2 * c);
+    dalloc(x->own, 2 * c);
+    dalloc(x->slot, 2 * c);
+    dalloc(x->crank, 2 * c);
+    dalloc(x->pending, c);
+    dalloc(x->fr0, c);
+    dalloc(x->fr1, c);
+    dalloc(x->lrec, c);
+    dalloc(x->boff, ncls);
+    dalloc(x->bfill, ncls);
+    dalloc(x->hist, ncls);
+    dalloc(x->rec_id, c);
+    dalloc(x->cpos, c);
+    dalloc(x->rseq, c);
+    dalloc(x->rec_ab, c);
+    dalloc(x->rec_p0, c);
+    dalloc(x->rec_p1, c);
+    if (e == hipSuccess) x->ccap = static_cast<uint32_t>(c);
+    return e;
+}
+
+// Room for nj joints: joint parameters and the constraint buffers for
+// max_contacts + nj constraints (grown, never shrunk).
+int ensure_joints(dy_xpbd_s *x, uint32_t nj) {
+    if (nj <= x->jcap) return DY_OK;
+    DY_TRY(hipDeviceSynchronize());  // nothing in flight still uses the buffers
+    if (x->jpk) (void)hipFree(x->jpk);
+    if (x->jend) (void)hipFree(x->jend);
+    x->jpk = nullptr;
+    x->jend = nullptr;
+    x->jcap = 0;
+    DY_TRY(hipMalloc(reinterpret_cast<void **>(&x->jpk), 5 * static_cast<size_t>(nj) * sizeof(float4)));
+    DY_TRY(hipMalloc(reinterpret_cast<void **>(&x->jend), static_cast<size_t>(nj) * sizeof(uint2)));
+    DY_TRY(alloc_constraint_bufs(x, x->cap + nj));
+    x->jcap = nj;
+    return DY_OK;
+}
+
 void xp_free(dy_xpbd_s *x) {
     for (hipEvent_t &e : x->ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (x->ev_meta) (void)hipEventDestroy(x->ev_meta);
     if (x->h_meta) (void)hipHostFree(x->h_meta);
-    void *ptrs[] = {x->lam, x->state, x->adj, x->own, x->slot, x->crank, x->pending, x->fr0, x->fr1, x->lrec,
-                    x->boff, x->bfill, x->hist, x->rec_id, x->cpos, x->rseq, x->meta, x->rec_ab, x->rec_p0,
-                    x->rec_p1};
+    free_constraint_bufs(x);
+    void *ptrs[] = {x->meta, x->jpk, x->jend};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release(); x->ver.release();
@@ -1095,15 +1295,18 @@ int query_coop(dy_xpbd_s *x) {
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_lists),
                                                         x->assign_threads, 0));
     x->coop_assign_max = per_cu * x->device_cus;
-    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_persistent),
-                                                        x->sweep_threads, 0));
-    x->coop_sweep_max = per_cu * x->device_cus;
-    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_xcd),
-                                                        x->sweep_threads, 0));
-    x->coop_xcd_max = per_cu * x->device_cus;
-    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_flow),
-                                                        x->sweep_threads, 0));
-    x->coop_flow_max = per_cu * x->device_cus;
+    const void *fns[4][2] = {
+        {reinterpret_cast<const void *>(sweep_persistent<false>), reinterpret_cast<const void *>(sweep_persistent<true>)},
+        {reinterpret_cast<const void *>(sweep_xcd<false>), reinterpret_cast<const void *>(sweep_xcd<true>)},
+        {reinterpret_cast<const void *>(sweep_flow<false>), reinterpret_cast<const void *>(sweep_flow<true>)},
+        {reinterpret_cast<const void *>(sweep_flow_lanes<false>),
+         reinterpret_cast<const void *>(sweep_flow_lanes<true>)}};
+    int *outs[4] = {x->coop_sweep_max, x->coop_xcd_max, x->coop_flow_max, x->coop_lanes_max};
+    for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 2; ++j) {
+            DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[k][j], x->sweep_threads, 0));
+            outs[k][j] = per_cu * x->device_cus;
+        }
     return DY_OK;
 }
 
@@ -1118,7 +1321,7 @@ uint32_t coop_blocks(const dy_xpbd_s *x, int override_blocks, int threads, int m
 
 // Launch-per-round assignment: the host drives the rounds and decides the
 // fallback to the reference order, reading the metadata every 16 rounds.
-int assign_by_rounds(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, const float *im) {
+int assign_by_rounds(dy_xpbd_s *x, hipStream_t st, const Cons &cv, const float *im) {
     uint32_t h[kMWords];
     for (int pass = 0; pass < 2; ++pass) {
         DY_TRY(hipMemcpyAsync(h, x->meta, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1141,7 +1344,7 @@ int assign_by_rounds(dy_xpbd_s *x, hipStream_t st, const dy_contact_view &cv, co
             if (r % 16u == 0u || r == max_rounds) {
                 DY_TRY(hipMemcpyAsync(h, x->meta, sizeof(h), hipMemcpyDeviceToHost, st));
                 DY_TRY(hipStreamSynchronize(st));
-                if (h[kMErr]) return DY_ERR_INTERNAL;
+                if (h[kMErr] & ~kErrJoint) return DY_ERR_INTERNAL;  // a rejected joint is reported by the queries
                 done = h[kMAssigned] == nc;
             }
         }
@@ -1185,32 +1388,8 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     x->n_iters = n_iters;
     x->mu = friction;
     x->cap = max_contacts;
-    const size_t c = max_contacts ? max_contacts : 1;
-    const size_t ncls = c + 66;
-    hipError_t e = hipSuccess;
-    auto dalloc = [&e](auto *&p, size_t n) {
-        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(*p));
-    };
-    dalloc(x->lam, c);
-    dalloc(x->state, c);
-    dalloc(x->adj, 2 * c);
-    dalloc(x->own, 2 * c);
-    dalloc(x->slot, 2 * c);
-    dalloc(x->crank, 2 * c);
-    dalloc(x->pending, c);
-    dalloc(x->fr0, c);
-    dalloc(x->fr1, c);
-    dalloc(x->lrec, c);
-    dalloc(x->boff, ncls);
-    dalloc(x->bfill, ncls);
-    dalloc(x->hist, ncls);
-    dalloc(x->rec_id, c);
-    dalloc(x->cpos, c);
-    dalloc(x->rseq, c);
-    dalloc(x->meta, kMWords);
-    dalloc(x->rec_ab, c);
-    dalloc(x->rec_p0, c);
-    dalloc(x->rec_p1, c);
+    hipError_t e = alloc_constraint_bufs(x, max_contacts);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), kMWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&x->h_meta), kMWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&x->ev_meta, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, kMWords * sizeof(uint32_t), hs(s));
@@ -1244,10 +1423,15 @@ int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial) {
     return DY_OK;
 }
 
-int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_t n_joints, const dy_body_view *bv,
-                  float dt) {
+int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const dy_joint_view *jv,
+                  const dy_body_view *bv, float dt) {
     if (!x || !cv || !bv || !cv->n) return DY_ERR_INVALID_ARG;
-    if (n_joints != 0) return DY_ERR_UNSUPPORTED;
+    const uint32_t nj = jv ? jv->n : 0u;
+    if (nj && (!jv->body_parent || !jv->body_child || !jv->type || !jv->anchor_px || !jv->anchor_py ||
+               !jv->anchor_pz || !jv->anchor_cx || !jv->anchor_cy || !jv->anchor_cz || !jv->axis_px ||
+               !jv->axis_py || !jv->axis_pz || !jv->limit_lo || !jv->limit_hi || !jv->target_pos ||
+               !jv->target_vel || !jv->stiffness || !jv->damping || !jv->compliance_pos || !jv->compliance_ang))
+        return DY_ERR_INVALID_ARG;
     hipStream_t st = hs(s);
     // Errors of earlier solves surface here once their metadata has landed.
     if (x->timing) {
@@ -1255,7 +1439,8 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     } else if (x->meta_pending && hipEventQuery(x->ev_meta) == hipSuccess) {
         DY_TRY_STATUS(sync_meta(x));
     }
-    const uint32_t nb = bv->n, cap = x->cap;
+    DY_TRY_STATUS(ensure_joints(x, nj));
+    const uint32_t nb = bv->n, cap = x->cap, ncap = cap + nj;
     DY_TRY_STATUS(x->deg.ensure(nb));
     DY_TRY_STATUS(x->off.ensure(nb));
     DY_TRY_STATUS(x->fill.ensure(nb));
@@ -1263,107 +1448,135 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, uint32_
     DY_TRY_STATUS(x->ver.ensure(nb));
     DY_TRY_STATUS(x->dyn.ensure(4 * static_cast<size_t>(nb)));
     DY_TRY_STATUS(x->sta.ensure(2 * static_cast<size_t>(nb)));
-    DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > cap + 66u ? nb : cap + 66u)));
+    DY_TRY_STATUS(x->scan_tmp.ensure(scan_tmp_words(nb > ncap + 66u ? nb : ncap + 66u)));
     x->last_nb = nb;
-    uint32_t top = nb > cap ? nb : cap;
+    uint32_t top = nb > ncap ? nb : ncap;
     if (top < kMWords + kMaxColours + 1u) top = kMWords + kMaxColours + 1u;
-    solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(nb, cv->n, cap, x->force_serial, x->deg.p, x->fill.p,
+    solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(nb, cv->n, cap, nj, x->force_serial, x->deg.p, x->fill.p,
                                                          x->used.p, x->ver.p, x->lam, x->hist, x->meta);
     DY_LAUNCH_CHECK();
-    solve_degree<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->deg.p);
+    if (nj) {
+        joint_pack<<<ceil_div(nj, 256), 256, 0, st>>>(*jv, nb, x->jpk, x->jend, x->meta);
+        DY_LAUNCH_CHECK();
+        joint_commit<<<1, 64, 0, st>>>(x->force_serial, x->meta);
+        DY_LAUNCH_CHECK();
+    }
+    const Cons cs{*cv, x->jend};
+    solve_degree<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, bv->inv_mass, x->deg.p);
     DY_LAUNCH_CHECK();
     DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + kMDyn, x->scan_tmp.p));
-    solve_fill<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->off.p, x->fill.p, x->adj,
-                                                         x->own);
+    solve_fill<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, bv->inv_mass, x->off.p, x->fill.p, x->adj,
+                                                          x->own);
     DY_LAUNCH_CHECK();
     gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
-    list_rank<<<grid_for(2u * cap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->meta, x->slot,
-                                                             x->crank);
+    list_rank<<<grid_for(2u * ncap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->meta, x->slot,
+                                                              x->crank);
     DY_LAUNCH_CHECK();
-    list_link<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, bv->inv_mass, x->off.p, x->deg.p, x->slot,
-                                                        x->crank, x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
+    list_link<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, bv->inv_mass, x->off.p, x->deg.p, x->slot,
+                                                         x->crank, x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
     DY_LAUNCH_CHECK();
 
-    const bool coop = x->path == 0 && x->coop_assign_max > 0 && x->coop_sweep_max > 0;
+    const int J = nj ? 1 : 0;
+    bool coop = x->path == 0 && !x->coop_refused && x->coop_assign_max > 0 && x->coop_sweep_max[J] > 0;
     if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
     if (coop) {
-        AssignBufs B{*cv, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
+        AssignBufs B{cs, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
                      x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill};
         void *args[] = {&B};
         // Participants on one XCD out of a launch with as many on every XCD.
         uint32_t blocks = 8u * coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
         if (blocks > static_cast<uint32_t>(x->coop_assign_max)) blocks = static_cast<uint32_t>(x->coop_assign_max);
-        DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
-                                          dim3(static_cast<uint32_t>(x->assign_threads)), args, 0, st));
-    } else {
-        DY_TRY_STATUS(assign_by_rounds(x, st, *cv, bv->inv_mass));
+        if (try_coop_launch(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
+                            dim3(static_cast<uint32_t>(x->assign_threads)), args, st) != hipSuccess) {
+            x->coop_refused = true;
+            coop = false;
+        }
     }
+    if (!coop) DY_TRY_STATUS(assign_by_rounds(x, st, cs, bv->inv_mass));
     if (x->timing) DY_TRY(hipEventRecord(x->ev[1], st));
 
-    // Class-sorted contact records.
+    // Class-sorted constraint records.
     CRec rec{x->rec_ab, x->rec_p0, x->rec_p1};
-    class_gather<<<grid_for(cap), kSolveThreads, 0, st>>>(*cv, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id,
-                                                           x->cpos, x->rseq);
+    class_gather<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id,
+                                                            x->cpos, x->rseq);
     DY_LAUNCH_CHECK();
-    const bool flow = x->path == 0 && (x->sweep_mode == 2 || x->sweep_mode == 3) && x->coop_flow_max > 0;
+    const int *flow_max = x->sweep_mode == 3 ? x->coop_lanes_max : x->coop_flow_max;
+    const bool flow = coop && (x->sweep_mode == 2 || x->sweep_mode == 3) && flow_max[J] > 0;
     if (flow) {
-        flow_seq<<<grid_for(2u * cap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->state, x->cpos,
-                                                                x->meta, x->rseq);
+        flow_seq<<<grid_for(2u * ncap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->state,
+                                                                 x->cpos, x->meta, x->rseq);
+        DY_LAUNCH_CHECK();
+    }
+    if (nj) {
+        joint_motors<<<1, 1024, 0, st>>>(x->boff, rec, x->dyn.p, x->sta.p, x->jpk, dt, x->meta);
         DY_LAUNCH_CHECK();
     }
 
     const uint32_t iters = static_cast<uint32_t>(x->n_iters);
+    const float4 *jpkv = x->jpk;
     if (x->timing) DY_TRY(hipEventRecord(x->ev[2], st));
-    if (iters > 0) {
-        if (coop && flow) {
-            uint32_t blocks = 8u * coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_flow_max);
-            if (blocks > static_cast<uint32_t>(x->coop_flow_max)) blocks = static_cast<uint32_t>(x->coop_flow_max);
-            uint32_t iv = iters;
-            const uint4 *rseqv = x->rseq;
-            float *lamv = x->lam;
-            float4 *dynv = x->dyn.p;
-            const float4 *stav = x->sta.p;
-            uint32_t *verv = x->ver.p, *metav = x->meta;
-            float muv = x->mu, dtv = dt;
-            void *args[] = {&iv, &rec, &rseqv, &lamv, &dynv, &stav, &verv, &muv, &dtv, &metav};
-            const void *fn = x->sweep_mode == 3 ? reinterpret_cast<const void *>(sweep_flow_lanes)
-                                                : reinterpret_cast<const void *>(sweep_flow);
-            DY_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks),
-                                              dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0, st));
-        } else if (coop) {
-            // sweep_xcd: P participants on one XCD out of a launch of P per XCD
-            // (8 XCDs on gfx950); sweep_persistent: P workgroups over the chip.
-            const bool xcd = x->sweep_mode == 0 && x->coop_xcd_max > 0;
-            uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max);
-            if (xcd) {
-                blocks *= 8u;
-                if (blocks > static_cast<uint32_t>(x->coop_xcd_max)) blocks = static_cast<uint32_t>(x->coop_xcd_max);
-            }
-            const uint32_t *boffv = x->boff;
-            uint32_t iv = iters;
-            float *lamv = x->lam;
-            float4 *dynv = x->dyn.p;
-            const float4 *stav = x->sta.p;
-            float muv = x->mu, dtv = dt;
-            uint32_t *metav = x->meta;
-            void *args[] = {&boffv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &metav};
-            DY_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(xcd ? sweep_xcd : sweep_persistent),
-                                              dim3(blocks), dim3(static_cast<uint32_t>(x->sweep_threads)), args, 0,
-                                              st));
-        } else {
-            // Per-class launches; the class count is read back first.
-            uint32_t k = 0;
-            DY_TRY(hipMemcpyAsync(&k, x->meta + kMNcls, sizeof(k), hipMemcpyDeviceToHost, st));
-            DY_TRY(hipStreamSynchronize(st));
-            const uint32_t blocks = grid_for(cap);
-            for (uint32_t it = 0; it < iters; ++it)
-                for (uint32_t c = 0; c < k; ++c) {
-                    sweep_class<<<blocks, kSolveThreads, 0, st>>>(x->boff, c, it, rec, x->lam, x->dyn.p, x->sta.p,
-                                                                  x->mu, dt);
-                    DY_LAUNCH_CHECK();
-                }
+    bool swept = iters == 0;
+    if (!swept && coop && flow) {
+        uint32_t blocks = 8u * coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, flow_max[J]);
+        if (blocks > static_cast<uint32_t>(flow_max[J])) blocks = static_cast<uint32_t>(flow_max[J]);
+        uint32_t iv = iters;
+        const uint4 *rseqv = x->rseq;
+        float *lamv = x->lam;
+        float4 *dynv = x->dyn.p;
+        const float4 *stav = x->sta.p;
+        uint32_t *verv = x->ver.p, *metav = x->meta;
+        float muv = x->mu, dtv = dt;
+        void *args[] = {&iv, &rec, &rseqv, &lamv, &dynv, &stav, &verv, &muv, &dtv, &metav, &jpkv};
+        const void *fn;
+        if (x->sweep_mode == 3)
+            fn = J ? reinterpret_cast<const void *>(sweep_flow_lanes<true>)
+                   : reinterpret_cast<const void *>(sweep_flow_lanes<false>);
+        else
+            fn = J ? reinterpret_cast<const void *>(sweep_flow<true>) : reinterpret_cast<const void *>(sweep_flow<false>);
+        swept = try_coop_launch(fn, dim3(blocks), dim3(static_cast<uint32_t>(x->sweep_threads)), args, st) == hipSuccess;
+    } else if (!swept && coop) {
+        // sweep_xcd: P participants on one XCD out of a launch of P per XCD
+        // (8 XCDs on gfx950); sweep_persistent: P workgroups over the chip.
+        const bool xcd = x->sweep_mode == 0 && x->coop_xcd_max[J] > 0;
+        uint32_t blocks = coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, x->coop_sweep_max[J]);
+        if (xcd) {
+            blocks *= 8u;
+            if (blocks > static_cast<uint32_t>(x->coop_xcd_max[J])) blocks = static_cast<uint32_t>(x->coop_xcd_max[J]);
         }
+        const uint32_t *boffv = x->boff;
+        uint32_t iv = iters;
+        float *lamv = x->lam;
+        float4 *dynv = x->dyn.p;
+        const float4 *stav = x->sta.p;
+        float muv = x->mu, dtv = dt;
+        uint32_t *metav = x->meta;
+        void *args[] = {&boffv, &iv, &rec, &lamv, &dynv, &stav, &muv, &dtv, &metav, &jpkv};
+        const void *fn;
+        if (xcd)
+            fn = J ? reinterpret_cast<const void *>(sweep_xcd<true>) : reinterpret_cast<const void *>(sweep_xcd<false>);
+        else
+            fn = J ? reinterpret_cast<const void *>(sweep_persistent<true>)
+                   : reinterpret_cast<const void *>(sweep_persistent<false>);
+        swept = try_coop_launch(fn, dim3(blocks), dim3(static_cast<uint32_t>(x->sweep_threads)), args, st) == hipSuccess;
+    }
+    if (!swept) {
+        if (coop) x->coop_refused = true;
+        // Per-class launches; the class count is read back first.
+        uint32_t k = 0;
+        DY_TRY(hipMemcpyAsync(&k, x->meta + kMNcls, sizeof(k), hipMemcpyDeviceToHost, st));
+        DY_TRY(hipStreamSynchronize(st));
+        const uint32_t blocks = grid_for(ncap);
+        for (uint32_t it = 0; it < iters; ++it)
+            for (uint32_t c = 0; c < k; ++c) {
+                if (J)
+                    sweep_class<true><<<blocks, kSolveThreads, 0, st>>>(x->boff, c, it, rec, x->lam, x->dyn.p,
+                                                                        x->sta.p, x->mu, dt, jpkv);
+                else
+                    sweep_class<false><<<blocks, kSolveThreads, 0, st>>>(x->boff, c, it, rec, x->lam, x->dyn.p,
+                                                                         x->sta.p, x->mu, dt, jpkv);
+                DY_LAUNCH_CHECK();
+            }
     }
     if (x->timing) {
         DY_TRY(hipEventRecord(x->ev[3], st));
@@ -1404,7 +1617,7 @@ int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out) {
 
 int dy_xpbd_download_classes(dy_xpbd_t x, dy_stream_t s, uint32_t *out, uint32_t n) {
     if (!x || (!out && n)) return DY_ERR_INVALID_ARG;
-    if (n > x->cap) return DY_ERR_INVALID_ARG;
+    if (n > x->ccap) return DY_ERR_INVALID_ARG;
     if (n == 0) return DY_OK;
     unpack_classes<<<ceil_div(n, 256), 256, 0, hs(s)>>>(n, x->state, x->rec_id);
     DY_LAUNCH_CHECK();

@@ -15,6 +15,10 @@ Scenes:
     Shoemake quaternions, density 1000 kg/m^3 (32 768 bodies by default).
   * ``batched_piles``: independent scenes with lognormal body counts, each with
     its own ground, stacked in jittered columns.
+  * ``articulated_arm``: core/tests/sim_determinism_test.cpp:130-190 (static
+    base and two revolute links with PD motors; the golden articulation hash).
+  * ``joint_chains``: chains of boxes and spheres hanging from static bases,
+    every joint type with seeded limits, motors and compliance, over a ground.
 
 Randomness: PCG32 (XSH-RR, O'Neill 2014) seeded through splitmix64, so a seed
 gives the same scene on every machine with this numpy.
@@ -109,6 +113,8 @@ class Scene:
     gravity: tuple = (0.0, -9.81, 0.0)
     frames: int = 300
     meta: dict = field(default_factory=dict)
+    # JointView arrays (body_parent, body_child, type, JOINT_F32 fields) or None
+    joints: dict | None = None
 
     @property
     def n(self) -> int:
@@ -441,6 +447,150 @@ def concat(scenes: list[Scene], name: str = "batch") -> Scene:
                  meta=dict(body_offset=body_off,
                            max_pairs=np.array([sc.max_pairs for sc in scenes], np.uint32),
                            max_contacts=np.array([sc.max_contacts for sc in scenes], np.uint32)))
+
+
+JOINT_F32 = ("anchor_px", "anchor_py", "anchor_pz", "anchor_cx", "anchor_cy", "anchor_cz",
+             "axis_px", "axis_py", "axis_pz", "limit_lo", "limit_hi", "target_pos", "target_vel",
+             "stiffness", "damping", "compliance_pos", "compliance_ang")
+JOINT_FIXED, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_BALL = 0, 1, 2, 3
+
+
+def joint_arrays(n: int) -> dict:
+    """n joints with JointParams' defaults (core/include/core/articulation.hpp:
+    16-42): axis +y, limits disabled (lo 1 > hi -1), no motor, rigid."""
+    d = {f: np.zeros(n, F32) for f in JOINT_F32}
+    d["body_parent"] = np.zeros(n, np.uint32)
+    d["body_child"] = np.zeros(n, np.uint32)
+    d["type"] = np.zeros(n, np.uint8)
+    d["axis_py"][:] = 1.0
+    d["limit_lo"][:] = 1.0
+    d["limit_hi"][:] = -1.0
+    return d
+
+
+# ---------------------------------------------------------------------------
+# core/tests/sim_determinism_test.cpp:130-190: static base + two 1 kg links,
+# revolute about x with PD motors (targets +0.5, -0.5 rad) and limits +-2.9.
+# The reference runs integrate + solve only (no collision), 20 frames.
+# ---------------------------------------------------------------------------
+def articulated_arm() -> Scene:
+    n = 3
+    hy = F32(0.15)
+    sep = F32(2.0) * hy
+    i_diag = F32(1.0) / F32(3.0) * F32(2.0) * hy * hy
+    b = _alloc(n)
+    for i in (1, 2):
+        b["pos"][i] = (0.0, F32(i) * sep, 0.0)
+        b["inertia"][i] = _diag(i_diag)
+    b["mass"][:] = (0.0, 1.0, 1.0)
+    b["flags"][0] = FLAG_STATIC
+    j = joint_arrays(2)
+    j["body_parent"][:] = (0, 1)
+    j["body_child"][:] = (1, 2)
+    j["anchor_py"][:] = hy
+    j["anchor_cy"][:] = -hy
+    j["axis_px"][:], j["axis_py"][:] = 1.0, 0.0
+    j["limit_lo"][:], j["limit_hi"][:] = -2.9, 2.9
+    j["target_pos"][:] = (0.5, -0.5)
+    j["stiffness"][:], j["damping"][:] = 200.0, 30.0
+    j["type"][:] = JOINT_REVOLUTE
+    return Scene(name="articulated_arm", shape_type=np.array([SHAPE_SPHERE], np.uint32),
+                 shape_half=np.array([[0.05, 0.05, 0.05]], F32), bounds=np.array([-4, -4, -4, 4, 4, 4], F32),
+                 max_pairs=32, max_contacts=16, frames=20, joints=j, **b)
+
+
+# ---------------------------------------------------------------------------
+# Chains hanging from static bases over a static ground: links are boxes
+# (half 0.08 x 0.15 x 0.08) or spheres (r 0.12) joined parent-above-child by
+# a seeded mix of the four joint types, with limits around the rest pose,
+# PD motors and compliance on some, and small initial velocities.  Prismatic
+# joints carry a holding motor and, only with prismatic_limits, limits: the
+# reference's prismatic limit correction (core/src/xpbd_solver.cpp:338-345)
+# moves the bodies along +-axis with the sign of the anchor-difference
+# constraint while its coordinate is child minus parent, so an active limit
+# grows instead of closing; it is reproduced as written.  Fixed and
+# prismatic joints hang only from the static bases: the reference's
+# orientation lock (solve_fixed_ang, core/src/xpbd_solver.cpp:234-252, also
+# part of every prismatic joint) takes q_b conj(q_a), already a world-frame
+# rotation, and rotates it by q_a once more, which skews its axis whenever
+# the parent turns and lets chains of them diverge.  Chains
+# stand 0.32 apart, so neighbours and the lowest links touch: contacts and
+# joints share bodies.
+# ---------------------------------------------------------------------------
+def joint_chains(n_chains: int = 6, links: int = 5, seed: int = 20260005, spacing: float = 0.32,
+                 prismatic_limits: bool = False) -> Scene:
+    rng = Pcg32(seed)
+    pitch = 0.34
+    n = n_chains * (links + 1) + 1
+    b = _alloc(n)
+    side = max(1, int(math.ceil(math.sqrt(n_chains))))
+    top = 0.6 + links * pitch
+    shape_type = np.array([SHAPE_BOX, SHAPE_SPHERE, SHAPE_BOX, SHAPE_BOX], np.uint32)
+    shape_half = np.array([[0.08, 0.15, 0.08], [0.12, 0.12, 0.12], [0.1, 0.1, 0.1],
+                           [0.5 * side * spacing + 2.0, 0.5, 0.5 * side * spacing + 2.0]], F32)
+    j = joint_arrays(n_chains * links)
+    k = 0
+    for c in range(n_chains):
+        cx = (c % side - (side - 1) / 2.0) * spacing
+        cz = (c // side - (side - 1) / 2.0) * spacing
+        base = c * (links + 1)
+        b["pos"][base] = (cx, top, cz)
+        b["mass"][base] = 0.0
+        b["flags"][base] = FLAG_STATIC
+        b["shape"][base] = 2
+        for li in range(links):
+            i = base + 1 + li
+            sph = rng.uniform() < 0.35
+            b["shape"][i] = 1 if sph else 0
+            b["pos"][i] = (cx + rng.uniform_range(-0.01, 0.01), top - (li + 1) * pitch, cz + rng.uniform_range(-0.01, 0.01))
+            m = 1000.0 * (4.0 / 3.0 * math.pi * 0.12 ** 3 if sph else 8.0 * 0.08 * 0.15 * 0.08)
+            b["mass"][i] = m
+            if sph:
+                b["inertia"][i] = _diag(0.4 * m * 0.12 * 0.12)
+            else:
+                b["inertia"][i] = np.array([m / 3.0 * (0.15 ** 2 + 0.08 ** 2), 0, 0, 0, m / 3.0 * (2 * 0.08 ** 2), 0,
+                                            0, 0, m / 3.0 * (0.15 ** 2 + 0.08 ** 2)], F32)
+            b["vel"][i] = (0.3 * rng.normal(), 0.1 * rng.normal(), 0.3 * rng.normal())
+            b["ang"][i] = (0.5 * rng.normal(), 0.5 * rng.normal(), 0.5 * rng.normal())
+            parent = base + li
+            j["body_parent"][k], j["body_child"][k] = parent, i
+            j["anchor_py"][k], j["anchor_cy"][k] = -0.5 * pitch, 0.5 * pitch  # midway between the centres
+            t = int(rng.uniform() * 4.0) % 4
+            if li > 0 and t == JOINT_FIXED:
+                t = JOINT_BALL
+            elif li > 0 and t == JOINT_PRISMATIC:
+                t = JOINT_REVOLUTE
+            j["type"][k] = t
+            ax = int(rng.uniform() * 3.0) % 3
+            axis = [0.0, 0.0, 0.0]
+            axis[ax] = 1.0
+            if t == JOINT_PRISMATIC:
+                axis = [0.0, 1.0, 0.0]
+            j["axis_px"][k], j["axis_py"][k], j["axis_pz"][k] = axis
+            if t == JOINT_REVOLUTE and rng.uniform() < 0.7:
+                w = rng.uniform_range(0.2, 1.2)
+                j["limit_lo"][k], j["limit_hi"][k] = -w, w
+            if t == JOINT_PRISMATIC and prismatic_limits and rng.uniform() < 0.7:
+                rest = -pitch
+                j["limit_lo"][k], j["limit_hi"][k] = rest - 0.04, rest + 0.04
+            if t == JOINT_PRISMATIC or (t == JOINT_REVOLUTE and rng.uniform() < 0.6):
+                j["stiffness"][k] = rng.uniform_range(0.0, 150.0) if t == JOINT_REVOLUTE else rng.uniform_range(150.0, 400.0)
+                j["damping"][k] = rng.uniform_range(0.0, 20.0) if t == JOINT_REVOLUTE else rng.uniform_range(10.0, 30.0)
+                j["target_pos"][k] = rng.uniform_range(-0.4, 0.4) if t == JOINT_REVOLUTE else -pitch + rng.uniform_range(-0.03, 0.03)
+                j["target_vel"][k] = rng.uniform_range(-0.5, 0.5)
+            if rng.uniform() < 0.4:
+                j["compliance_pos"][k] = rng.uniform_range(1e-6, 1e-4)
+                j["compliance_ang"][k] = rng.uniform_range(1e-6, 1e-4)
+            k += 1
+    g = n - 1
+    b["mass"][g] = 0.0
+    b["flags"][g] = FLAG_STATIC
+    b["shape"][g] = 3
+    b["pos"][g] = (0.0, -0.5 + 0.05, 0.0)
+    half = float(shape_half[3, 0]) + 2.0
+    return Scene(name=f"joint_chains_{n_chains}x{links}", shape_type=shape_type, shape_half=shape_half,
+                 bounds=np.array([-half, -2.0, -half, half, top + 2.0, half], F32),
+                 max_pairs=max(256, 48 * n), max_contacts=max(64, 16 * n), frames=60, joints=j, **b)
 
 
 def two_body(shape_a: int, shape_b: int, half_a, half_b, pos_a, pos_b, mass=(1.0, 1.0),

@@ -30,7 +30,7 @@ extern "C" {
 
 #define DY_OK 0
 #define DY_ERR_INVALID_ARG (-1)  /* null/oversized argument, capacity exceeded */
-#define DY_ERR_UNSUPPORTED (-2)  /* joints, convex-hull or triangle-mesh shapes */
+#define DY_ERR_UNSUPPORTED (-2)  /* convex-hull or triangle-mesh shapes */
 #define DY_ERR_NO_DEVICE   (-3)  /* no HIP device */
 #define DY_ERR_INTERNAL    (-4)  /* a device-side consistency check failed */
 
@@ -75,6 +75,23 @@ typedef struct dy_contact_view {
     uint32_t capacity;
 } dy_contact_view;
 
+/* JointView (core/include/core/articulation.hpp:47-70): device SoA arrays,
+ * n joints.  type: 0 fixed, 1 revolute, 2 prismatic, 3 ball; anchors in the
+ * parent's / child's body frame, axis in the parent's frame; limit_lo >
+ * limit_hi disables the limits. */
+typedef struct dy_joint_view {
+    uint32_t *body_parent, *body_child;
+    float *anchor_px, *anchor_py, *anchor_pz;
+    float *anchor_cx, *anchor_cy, *anchor_cz;
+    float *axis_px, *axis_py, *axis_pz;
+    float *limit_lo, *limit_hi;
+    float *target_pos, *target_vel;
+    float *stiffness, *damping;
+    float *compliance_pos, *compliance_ang;
+    uint8_t *type;
+    uint32_t n;
+} dy_joint_view;
+
 /* core/include/core/aabb.hpp:7-31 */
 typedef struct dy_aabb { float min_x, min_y, min_z, max_x, max_y, max_z; } dy_aabb;
 
@@ -104,7 +121,7 @@ typedef struct dy_device_info {
 } dy_device_info;
 
 typedef struct dy_solve_stats {
-    uint32_t n_constraints;  /* contacts solved (joints are rejected) */
+    uint32_t n_constraints;  /* constraints solved: contacts + joints */
     uint32_t used_serial;    /* 1: reference-order (run_serial) semantics */
     uint32_t n_classes;      /* colours (coloured path) or dependency levels (serial path) */
     uint32_t rounds;         /* colouring rounds the reference would run (max dependency depth) */
@@ -241,18 +258,24 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
 int dy_xpbd_destroy(dy_xpbd_t x);
 /* set_force_serial (xpbd_solver.hpp:48) */
 int dy_xpbd_set_force_serial(dy_xpbd_t x, int force_serial);
-/* solve(s, contacts, joints, bodies, dt) (core/include/core/solver.hpp:17-21).
- * n_joints must be 0 (joints are out of scope: DY_ERR_UNSUPPORTED otherwise).
- * <= 256 contacts or force_serial: reference contact order (run_serial
- * semantics), executed in parallel over dependency levels; otherwise the
- * reference's Jones-Plassmann colours, swept in colour order, falling back to
- * the contact order where the reference does (a contact needing a 65th
- * colour, or more than 256 rounds).  The contact count is read on the device
- * (*contacts->n, clamped to max_contacts; a larger count is an error).
+/* solve(s, contacts, joints, bodies, dt) (core/include/core/solver.hpp:17-21,
+ * core/src/xpbd_solver.cpp:590-733).  Constraint k is contact k (k < nc) or
+ * joint k - nc; joints may be NULL (= JointView{}).  <= 256 constraints or
+ * force_serial: the reference's serial order (run_serial: joint motors in
+ * joint order, then per iteration contacts, joints, renormalisation),
+ * executed in parallel over dependency levels; otherwise the reference's
+ * Jones-Plassmann colours over contacts and joints, motors then constraints
+ * swept in colour order, falling back to the serial order where the
+ * reference does (a constraint needing a 65th colour, or more than 256
+ * rounds).  The contact count is read on the device (*contacts->n, clamped
+ * to max_contacts; a larger count is an error).  Joints are validated on the
+ * device: a body index out of range, parent == child or a type above 3 makes
+ * that solve run without joints and the solver report DY_ERR_INVALID_ARG.
  * Asynchronous on the stream (the launch-per-round path, DYPHUR_SOLVER_PATH=1,
- * synchronises).  Errors of a solve are returned by a later solve or by the
- * blocking queries below. */
-int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *contacts, uint32_t n_joints,
+ * synchronises; a solve with more joints than any before it synchronises the
+ * device to grow its buffers).  Errors of a solve are returned by a later
+ * solve or by the blocking queries below. */
+int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *contacts, const dy_joint_view *joints,
                   const dy_body_view *bodies, float dt);
 /* Statistics of the last solve; blocking (waits for that solve). */
 int dy_xpbd_last_stats(dy_xpbd_t x, dy_solve_stats *out);

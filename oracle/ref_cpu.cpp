@@ -532,6 +532,289 @@ void project_contact(uint32_t c, const ContactSoA &cs, rc_bodies &bv, float mu, 
     }
 }
 
+// ----------------------------------------------------------------------------
+// XPBD joints (xpbd_solver.cpp:60-99 helpers, 100-407 constraints, 523-556
+// solve_joint).  Bodies are read and written through the SoA arrays in the
+// reference's order; bodies with inv_mass == 0 are never written.
+// ----------------------------------------------------------------------------
+enum : uint8_t { kJFixed = 0, kJRevolute = 1, kJPrismatic = 2, kJBall = 3 };
+
+// atan2 of joint_current_value (xpbd_solver.cpp:317).  The reference calls
+// sycl::atan2, on its OpenMP backend the C library's float atan2, which is
+// not correctly rounded everywhere.  The oracle's atan2 is the correctly
+// rounded one (the float64 atan2 rounded once to float), the function the
+// gfx950 kernels reproduce; rc_set_libc_atan2(1) switches to the C library's
+// float atan2 to reproduce the reference's own golden articulation hash.
+int g_libc_atan2 = 0;
+inline float joint_atan2(float y, float x) {
+    if (g_libc_atan2) return std::atan2(y, x);
+    return static_cast<float>(std::atan2(static_cast<double>(y), static_cast<double>(x)));
+}
+
+// Body-local v in world axes with body i's current quaternion
+// (qrot_to_world, xpbd_solver.cpp:62-73): the columns of box_axes.
+inline void to_world(const rc_bodies &b, uint32_t i, float vx, float vy, float vz, float &ox, float &oy,
+                     float &oz) {
+    float E[3][3];
+    box_axes(b.rot_w[i], b.rot_x[i], b.rot_y[i], b.rot_z[i], E);
+    ox = E[0][0] * vx + E[1][0] * vy + E[2][0] * vz;
+    oy = E[0][1] * vx + E[1][1] * vy + E[2][1] * vz;
+    oz = E[0][2] * vx + E[1][2] * vy + E[2][2] * vz;
+}
+
+// inv_mass + (r x n) . I_w^-1 (r x n)  (gen_inv_mass, xpbd_solver.cpp:90-98).
+inline float gen_inv_mass(const rc_bodies &b, uint32_t i, float rx, float ry, float rz, float nx, float ny,
+                          float nz) {
+    const float cx = ry * nz - rz * ny, cy = rz * nx - rx * nz, cz = rx * ny - ry * nx;
+    float kx, ky, kz;
+    inv_inertia_world(b, i, cx, cy, cz, kx, ky, kz);
+    return b.inv_mass[i] + (cx * kx + cy * ky + cz * kz);
+}
+
+// Positional impulse sign * dl * n through lever arm r: position, orientation
+// and the matching velocity change dx / h (apply_pos_impulse, :103-133).
+inline void pos_impulse(rc_bodies &b, uint32_t i, float sign, float dl, float nx, float ny, float nz, float rx,
+                        float ry, float rz, float inv_dt) {
+    const float im = b.inv_mass[i];
+    if (im == 0.f) return;
+    const float dx = sign * im * dl * nx, dy = sign * im * dl * ny, dz = sign * im * dl * nz;
+    b.pos_x[i] += dx; b.pos_y[i] += dy; b.pos_z[i] += dz;
+    b.vel_x[i] += dx * inv_dt; b.vel_y[i] += dy * inv_dt; b.vel_z[i] += dz * inv_dt;
+    const float cx = ry * nz - rz * ny, cy = rz * nx - rx * nz, cz = rx * ny - ry * nx;
+    float kx, ky, kz;
+    inv_inertia_world(b, i, cx, cy, cz, kx, ky, kz);
+    const float ax = sign * dl * kx, ay = sign * dl * ky, az = sign * dl * kz;
+    rotate_by(b, i, ax, ay, az);
+    b.ang_x[i] += ax * inv_dt; b.ang_y[i] += ay * inv_dt; b.ang_z[i] += az * inv_dt;
+}
+
+// Anchor coincidence (solve_pos_constraint, :137-164).
+inline void pos_constraint(rc_bodies &b, uint32_t ia, uint32_t ib, float rax, float ray, float raz, float rbx,
+                           float rby, float rbz, float alpha, float inv_dt) {
+    const float eps = 1e-7f;
+    const float cx = (b.pos_x[ia] + rax) - (b.pos_x[ib] + rbx);
+    const float cy = (b.pos_y[ia] + ray) - (b.pos_y[ib] + rby);
+    const float cz = (b.pos_z[ia] + raz) - (b.pos_z[ib] + rbz);
+    const float c2 = cx * cx + cy * cy + cz * cz;
+    if (c2 < eps * eps) return;
+    const float cl = std::sqrt(c2);
+    const float nx = cx / cl, ny = cy / cl, nz = cz / cl;
+    const float wa = gen_inv_mass(b, ia, rax, ray, raz, nx, ny, nz);
+    const float wb = gen_inv_mass(b, ib, rbx, rby, rbz, nx, ny, nz);
+    const float w = wa + wb + alpha;
+    if (w < 1e-10f) return;
+    const float dl = -cl / w;
+    pos_impulse(b, ia, +1.f, dl, nx, ny, nz, rax, ray, raz, inv_dt);
+    pos_impulse(b, ib, -1.f, dl, nx, ny, nz, rbx, rby, rbz, inv_dt);
+}
+
+// Anchor coincidence across the axis only: the prismatic slide stays free
+// (solve_transverse_pos, :169-196).
+inline void transverse_constraint(rc_bodies &b, uint32_t ia, uint32_t ib, float rax, float ray, float raz,
+                                  float rbx, float rby, float rbz, float ux, float uy, float uz, float alpha,
+                                  float inv_dt) {
+    const float cx = (b.pos_x[ia] + rax) - (b.pos_x[ib] + rbx);
+    const float cy = (b.pos_y[ia] + ray) - (b.pos_y[ib] + rby);
+    const float cz = (b.pos_z[ia] + raz) - (b.pos_z[ib] + rbz);
+    const float along = cx * ux + cy * uy + cz * uz;
+    const float tx = cx - along * ux, ty = cy - along * uy, tz = cz - along * uz;
+    const float t2 = tx * tx + ty * ty + tz * tz;
+    if (t2 < 1e-14f) return;
+    const float tl = std::sqrt(t2);
+    const float nx = tx / tl, ny = ty / tl, nz = tz / tl;
+    const float wa = gen_inv_mass(b, ia, rax, ray, raz, nx, ny, nz);
+    const float wb = gen_inv_mass(b, ib, rbx, rby, rbz, nx, ny, nz);
+    const float w = wa + wb + alpha;
+    if (w < 1e-10f) return;
+    const float dl = -tl / w;
+    pos_impulse(b, ia, +1.f, dl, nx, ny, nz, rax, ray, raz, inv_dt);
+    pos_impulse(b, ib, -1.f, dl, nx, ny, nz, rbx, rby, rbz, inv_dt);
+}
+
+// Rotation correction along the world vector c, |c| = error angle
+// (solve_ang_constraint, :200-230).  The angular velocities change by the
+// rotation times inv_dt (0 for the limit correction).
+inline void ang_constraint(rc_bodies &b, uint32_t ia, uint32_t ib, float cx, float cy, float cz, float alpha,
+                           float inv_dt) {
+    const float c2 = cx * cx + cy * cy + cz * cz;
+    if (c2 < 1e-14f) return;
+    const float cl = std::sqrt(c2);
+    const float nx = cx / cl, ny = cy / cl, nz = cz / cl;
+    float ax, ay, az, bx, by, bz;
+    inv_inertia_world(b, ia, nx, ny, nz, ax, ay, az);
+    inv_inertia_world(b, ib, nx, ny, nz, bx, by, bz);
+    const float wa = nx * ax + ny * ay + nz * az;
+    const float wb = nx * bx + ny * by + nz * bz;
+    const float w = wa + wb + alpha;
+    if (w < 1e-10f) return;
+    const float dl = -cl / w;
+    rotate_by(b, ia, -dl * ax, -dl * ay, -dl * az);
+    rotate_by(b, ib, dl * bx, dl * by, dl * bz);
+    if (b.inv_mass[ia] != 0.f) {
+        b.ang_x[ia] += (-dl * ax) * inv_dt; b.ang_y[ia] += (-dl * ay) * inv_dt; b.ang_z[ia] += (-dl * az) * inv_dt;
+    }
+    if (b.inv_mass[ib] != 0.f) {
+        b.ang_x[ib] += (dl * bx) * inv_dt; b.ang_y[ib] += (dl * by) * inv_dt; b.ang_z[ib] += (dl * bz) * inv_dt;
+    }
+}
+
+// Relative orientation locked: error 2 * vec(conj(q_a) q_b) in a's world
+// axes, shortest arc (solve_fixed_ang, :234-252).
+inline void fixed_ang(rc_bodies &b, uint32_t ia, uint32_t ib, float alpha, float inv_dt) {
+    const float aw = b.rot_w[ia], ax = b.rot_x[ia], ay = b.rot_y[ia], az = b.rot_z[ia];
+    const float bw = b.rot_w[ib], bx = b.rot_x[ib], by = b.rot_y[ib], bz = b.rot_z[ib];
+    const float rw = aw * bw + ax * bx + ay * by + az * bz;
+    float rx = -ax * bw + aw * bx - az * by + ay * bz;
+    float ry = -ay * bw + az * bx + aw * by - ax * bz;
+    float rz = -az * bw - ay * bx + ax * by + aw * bz;
+    if (rw < 0.f) { rx = -rx; ry = -ry; rz = -rz; }
+    float cx, cy, cz;
+    to_world(b, ia, rx, ry, rz, cx, cy, cz);
+    cx *= 2.f; cy *= 2.f; cz *= 2.f;
+    ang_constraint(b, ia, ib, cx, cy, cz, alpha, inv_dt);
+}
+
+// Hinge axes aligned: error = axis_a x axis_b, same hemisphere
+// (solve_revolute_ang, :257-273).
+inline void revolute_ang(rc_bodies &b, uint32_t ia, uint32_t ib, float lx, float ly, float lz, float alpha,
+                         float inv_dt) {
+    float ax, ay, az, bx, by, bz;
+    to_world(b, ia, lx, ly, lz, ax, ay, az);
+    to_world(b, ib, lx, ly, lz, bx, by, bz);
+    if (ax * bx + ay * by + az * bz < 0.f) { bx = -bx; by = -by; bz = -bz; }
+    const float cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    ang_constraint(b, ia, ib, cx, cy, cz, alpha, inv_dt);
+}
+
+// Joint coordinate: offset along the axis (prismatic) or the hinge angle
+// between a reference direction seen from both bodies (revolute)
+// (joint_current_value, :278-318).
+inline float joint_value(const rc_bodies &b, uint32_t ia, uint32_t ib, float lx, float ly, float lz,
+                         uint8_t type) {
+    float ux, uy, uz;
+    to_world(b, ia, lx, ly, lz, ux, uy, uz);
+    if (type == kJPrismatic)
+        return (b.pos_x[ib] - b.pos_x[ia]) * ux + (b.pos_y[ib] - b.pos_y[ia]) * uy + (b.pos_z[ib] - b.pos_z[ia]) * uz;
+    float px, py, pz;
+    if (std::fabs(lx) < std::fabs(ly) && std::fabs(lx) < std::fabs(lz)) {
+        const float len = std::sqrt(ly * ly + lz * lz);
+        px = 0.f; py = -lz / len; pz = ly / len;
+    } else if (std::fabs(ly) < std::fabs(lz)) {
+        const float len = std::sqrt(lx * lx + lz * lz);
+        px = lz / len; py = 0.f; pz = -lx / len;
+    } else {
+        const float len = std::sqrt(lx * lx + ly * ly);
+        px = -ly / len; py = lx / len; pz = 0.f;
+    }
+    float rx, ry, rz, cx, cy, cz;
+    to_world(b, ia, px, py, pz, rx, ry, rz);
+    to_world(b, ib, px, py, pz, cx, cy, cz);
+    const float sx = ry * cz - rz * cy, sy = rz * cx - rx * cz, sz = rx * cy - ry * cx;
+    const float sn = sx * ux + sy * uy + sz * uz;
+    const float cs = rx * cx + ry * cy + rz * cz;
+    return joint_atan2(sn, cs);
+}
+
+// One-sided limit on the joint coordinate (apply_limit, :322-352).
+inline void joint_limit(rc_bodies &b, uint32_t ia, uint32_t ib, float lx, float ly, float lz, float lo, float hi,
+                        uint8_t type, float inv_dt) {
+    if (lo > hi) return;
+    const float v = joint_value(b, ia, ib, lx, ly, lz, type);
+    float viol = 0.f;
+    if (v < lo) viol = v - lo;
+    else if (v > hi) viol = v - hi;
+    if (std::fabs(viol) < 1e-7f) return;
+    float ux, uy, uz;
+    to_world(b, ia, lx, ly, lz, ux, uy, uz);
+    if (type == kJPrismatic) {
+        const float wa = gen_inv_mass(b, ia, 0.f, 0.f, 0.f, ux, uy, uz);
+        const float wb = gen_inv_mass(b, ib, 0.f, 0.f, 0.f, ux, uy, uz);
+        const float w = wa + wb;
+        if (w < 1e-10f) return;
+        const float dl = -viol / w;
+        pos_impulse(b, ia, +1.f, dl, ux, uy, uz, 0.f, 0.f, 0.f, inv_dt);
+        pos_impulse(b, ib, -1.f, dl, ux, uy, uz, 0.f, 0.f, 0.f, inv_dt);
+    } else {
+        ang_constraint(b, ia, ib, viol * ux, viol * uy, viol * uz, 0.f, 0.f);
+    }
+}
+
+// PD motor: one velocity impulse along the axis per solve (apply_motor,
+// :356-407).
+inline void joint_motor(rc_bodies &b, uint32_t ia, uint32_t ib, float lx, float ly, float lz, float target_pos,
+                        float target_vel, float stiffness, float damping, float h, uint8_t type) {
+    if (stiffness < 1e-6f && damping < 1e-6f) return;
+    float ux, uy, uz;
+    to_world(b, ia, lx, ly, lz, ux, uy, uz);
+    float vrel;
+    if (type == kJPrismatic) {
+        const float vx = b.vel_x[ib] - b.vel_x[ia], vy = b.vel_y[ib] - b.vel_y[ia], vz = b.vel_z[ib] - b.vel_z[ia];
+        vrel = vx * ux + vy * uy + vz * uz;
+    } else {
+        const float ox = b.ang_x[ib] - b.ang_x[ia], oy = b.ang_y[ib] - b.ang_y[ia], oz = b.ang_z[ib] - b.ang_z[ia];
+        vrel = ox * ux + oy * uy + oz * uz;
+    }
+    float pos_err = 0.f;
+    if (stiffness > 1e-6f) pos_err = target_pos - joint_value(b, ia, ib, lx, ly, lz, type);
+    const float vel_err = target_vel - vrel;
+    const float ima = b.inv_mass[ia], imb = b.inv_mass[ib];
+    if (type == kJPrismatic) {
+        const float w = ima + imb;
+        if (w < 1e-10f) return;
+        const float j = (stiffness * pos_err + damping * vel_err) * h / w;
+        if (ima != 0.f) { b.vel_x[ia] -= ima * j * ux; b.vel_y[ia] -= ima * j * uy; b.vel_z[ia] -= ima * j * uz; }
+        if (imb != 0.f) { b.vel_x[ib] += imb * j * ux; b.vel_y[ib] += imb * j * uy; b.vel_z[ib] += imb * j * uz; }
+    } else {
+        float ax, ay, az, bx, by, bz;
+        inv_inertia_world(b, ia, ux, uy, uz, ax, ay, az);
+        inv_inertia_world(b, ib, ux, uy, uz, bx, by, bz);
+        const float wa = ux * ax + uy * ay + uz * az;
+        const float wb = ux * bx + uy * by + uz * bz;
+        const float w = wa + wb;
+        if (w < 1e-10f) return;
+        const float j = (stiffness * pos_err + damping * vel_err) * h / w;
+        if (ima != 0.f) { b.ang_x[ia] -= j * ax; b.ang_y[ia] -= j * ay; b.ang_z[ia] -= j * az; }
+        if (imb != 0.f) { b.ang_x[ib] += j * bx; b.ang_y[ib] += j * by; b.ang_z[ib] += j * bz; }
+    }
+}
+
+// The motor of joint j where it has one (revolute, prismatic).
+inline void motor_of(const rc_joints &J, uint32_t j, rc_bodies &b, float dt) {
+    const uint8_t t = J.type[j];
+    if (t != kJRevolute && t != kJPrismatic) return;
+    joint_motor(b, J.body_parent[j], J.body_child[j], J.axis_px[j], J.axis_py[j], J.axis_pz[j], J.target_pos[j],
+                J.target_vel[j], J.stiffness[j], J.damping[j], dt, t);
+}
+
+// One joint constraint (solve_joint, :524-556).  The lever arms are taken
+// once, before any correction of this joint.
+inline void solve_joint(const rc_joints &J, uint32_t j, rc_bodies &b, float dt) {
+    const uint32_t ia = J.body_parent[j], ib = J.body_child[j];
+    const uint8_t t = J.type[j];
+    float rax, ray, raz, rbx, rby, rbz;
+    to_world(b, ia, J.anchor_px[j], J.anchor_py[j], J.anchor_pz[j], rax, ray, raz);
+    to_world(b, ib, J.anchor_cx[j], J.anchor_cy[j], J.anchor_cz[j], rbx, rby, rbz);
+    const float alpha_pos = J.compliance_pos[j] / (dt * dt);
+    const float alpha_ang = J.compliance_ang[j] / (dt * dt);
+    const float lx = J.axis_px[j], ly = J.axis_py[j], lz = J.axis_pz[j];
+    const float inv_dt = 1.f / dt;
+    if (t == kJPrismatic) {
+        float ux, uy, uz;
+        to_world(b, ia, lx, ly, lz, ux, uy, uz);
+        fixed_ang(b, ia, ib, alpha_ang, inv_dt);
+        transverse_constraint(b, ia, ib, rax, ray, raz, rbx, rby, rbz, ux, uy, uz, alpha_pos, inv_dt);
+        joint_limit(b, ia, ib, lx, ly, lz, J.limit_lo[j], J.limit_hi[j], t, inv_dt);
+    } else {
+        pos_constraint(b, ia, ib, rax, ray, raz, rbx, rby, rbz, alpha_pos, inv_dt);
+        if (t == kJFixed) {
+            fixed_ang(b, ia, ib, alpha_ang, inv_dt);
+        } else if (t == kJRevolute) {
+            revolute_ang(b, ia, ib, lx, ly, lz, alpha_ang, inv_dt);
+            joint_limit(b, ia, ib, lx, ly, lz, J.limit_lo[j], J.limit_hi[j], t, inv_dt);
+        }
+    }
+}
+
 }  // namespace
 
 // ============================================================================
@@ -974,23 +1257,37 @@ void rc_np_set_contacts(rc_world *w, uint32_t n, const uint32_t *a, const uint32
     w->last_count = n;
 }
 
-static void run_serial(rc_world *w, rc_bodies *bv, float dt, uint32_t nc) {
-    // xpbd_solver.cpp:559-587 with no joints: one work-item.
+static void run_serial(rc_world *w, rc_bodies *bv, const rc_joints &J, float dt, uint32_t nc) {
+    // xpbd_solver.cpp:559-587 on one work-item: motors once, then per
+    // iteration contacts, joints, renormalisation of every body.
+    for (uint32_t j = 0; j < J.n; ++j) motor_of(J, j, *bv, dt);
     for (int it = 0; it < w->n_iters; ++it) {
         for (uint32_t c = 0; c < nc; ++c) project_contact(c, w->store, *bv, w->mu, w->lambda.data(), dt);
+        for (uint32_t j = 0; j < J.n; ++j) solve_joint(J, j, *bv, dt);
         for (uint32_t i = 0; i < bv->n; ++i) renormalise(*bv, i);
     }
 }
 
 void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_stats *st) {
+    rc_solve_joints(w, bv, nullptr, dt, force_serial, st);
+}
+
+void rc_set_libc_atan2(int on) { g_libc_atan2 = on ? 1 : 0; }
+float rc_joint_atan2(float y, float x) { return joint_atan2(y, x); }
+
+void rc_solve_joints(rc_world *w, rc_bodies *bv, const rc_joints *joints, float dt, int force_serial,
+                     rc_solve_stats *st) {
+    rc_joints none{};
+    const rc_joints &J = joints ? *joints : none;
     std::fill(w->lambda.begin(), w->lambda.end(), 0.f);
-    const uint32_t nc = w->n_contacts;
-    const uint32_t n_con = nc;  // no joints
+    const uint32_t nc = w->n_contacts, nj = J.n;
+    // Constraint k: contact k for k < nc, joint k - nc after (xpbd_solver.cpp:611-614).
+    const uint32_t n_con = nc + nj;
     rc_solve_stats local{};
     local.n_constraints = n_con;
     if (n_con == 0) { if (st) *st = local; return; }
     if (force_serial || n_con <= 256u) {
-        run_serial(w, bv, dt, nc);
+        run_serial(w, bv, J, dt, nc);
         local.used_serial = 1;
         if (st) *st = local;
         return;
@@ -1008,6 +1305,10 @@ void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_s
     uint32_t meta_max = 0, meta_over = 0, meta_rem = n_con;
     const ContactSoA &cs = w->store;
     const float *im = bv->inv_mass;
+    auto ends = [&](uint32_t k, uint32_t &a, uint32_t &b) {
+        if (k < nc) { a = cs.a[k]; b = cs.b[k]; }
+        else { a = J.body_parent[k - nc]; b = J.body_child[k - nc]; }
+    };
     uint32_t r = 0, h_rem = n_con;
     while (h_rem > 0 && r < 256u) {
         for (uint32_t bi = 0; bi < 8u && r < 256u; ++bi, ++r) {
@@ -1016,7 +1317,8 @@ void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_s
             for (long long kk = 0; kk < static_cast<long long>(n_con); ++kk) {
                 const uint32_t k = static_cast<uint32_t>(kk);
                 if (color[k] != kUncol) continue;
-                const uint32_t a = cs.a[k], b = cs.b[k];
+                uint32_t a, b;
+                ends(k, a, b);
                 const uint64_t pk = jp_priority(k);
                 if (im[a] != 0.f) atomic_max_u64(&bpri[a], pk);
                 if (im[b] != 0.f) atomic_max_u64(&bpri[b], pk);
@@ -1025,7 +1327,8 @@ void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_s
             for (long long kk = 0; kk < static_cast<long long>(n_con); ++kk) {
                 const uint32_t k = static_cast<uint32_t>(kk);
                 if (color[k] != kUncol) continue;
-                const uint32_t a = cs.a[k], b = cs.b[k];
+                uint32_t a, b;
+                ends(k, a, b);
                 const bool da = im[a] != 0.f, db = im[b] != 0.f;
                 const uint64_t pk = jp_priority(k);
                 if ((da && bpri[a] != pk) || (db && bpri[b] != pk)) continue;
@@ -1047,8 +1350,7 @@ void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_s
     local.rounds = r;
     local.overflow = meta_over;
     if (h_rem > 0 || meta_over) {
-        std::fill(w->lambda.begin(), w->lambda.end(), 0.f);
-        run_serial(w, bv, dt, nc);
+        run_serial(w, bv, J, dt, nc);
         local.used_serial = 1;
         if (st) *st = local;
         return;
@@ -1056,13 +1358,24 @@ void rc_solve(rc_world *w, rc_bodies *bv, float dt, int force_serial, rc_solve_s
     const uint32_t ncol = meta_max + 1;
     local.n_colors = ncol;
     float *lam = w->lambda.data();
+    // Motor pre-pass, colour by colour, joints only (xpbd_solver.cpp:703-719).
+    if (nj > 0) {
+        for (uint32_t ci = 0; ci < ncol; ++ci) {
+#pragma omp parallel for schedule(static)
+            for (long long kk = nc; kk < static_cast<long long>(n_con); ++kk) {
+                const uint32_t k = static_cast<uint32_t>(kk);
+                if (color[k] == ci) motor_of(J, k - nc, *bv, dt);
+            }
+        }
+    }
     for (int it = 0; it < w->n_iters; ++it) {
         for (uint32_t ci = 0; ci < ncol; ++ci) {
 #pragma omp parallel for schedule(static)
             for (long long kk = 0; kk < static_cast<long long>(n_con); ++kk) {
                 const uint32_t k = static_cast<uint32_t>(kk);
                 if (color[k] != ci) continue;
-                project_contact(k, cs, *bv, w->mu, lam, dt);
+                if (k < nc) project_contact(k, cs, *bv, w->mu, lam, dt);
+                else solve_joint(J, k - nc, *bv, dt);
             }
         }
 #pragma omp parallel for schedule(static)
@@ -1079,13 +1392,19 @@ void rc_solve_colors(const rc_world *w, uint32_t *out, uint32_t n) {
 void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *bounds,
               float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
               rc_frame_stats *st) {
+    rc_frame_joints(w, b, s, bounds, nullptr, gx, gy, gz, dt, n_substeps, force_serial, st);
+}
+
+void rc_frame_joints(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *bounds, const rc_joints *joints,
+                     float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
+                     rc_frame_stats *st) {
     rc_integrate(b, gx, gy, gz, dt, n_substeps);
     const uint32_t np = rc_bp_build_and_query(w, b, s, bounds);
     rc_bp_sort_pairs(w, np);
     const uint32_t npc = np > w->max_pairs ? w->max_pairs : np;
     const uint32_t nc = rc_np_run(w, w->pairs.data(), npc, b, s);
     rc_solve_stats ss{};
-    rc_solve(w, b, dt, force_serial, &ss);
+    rc_solve_joints(w, b, joints, dt, force_serial, &ss);
     if (st) { st->n_pairs = np; st->n_contacts = nc; st->n_colors = ss.n_colors; st->used_serial = ss.used_serial; }
 }
 

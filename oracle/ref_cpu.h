@@ -13,6 +13,7 @@
  *   core/src/broadphase.cpp:14-365        (rc_bp_*)
  *   core/src/narrowphase.cpp:12-306,912-1127 (rc_np_run)
  *   core/src/xpbd_solver.cpp:1-134,409-521,558-733 (rc_solve)
+ *   core/src/xpbd_solver.cpp:60-407,523-556 (joints: rc_solve_joints)
  *   core/src/body_store.cpp:28-69, core/include/core/math/mat3.hpp:61-79
  *                                          (rc_body_setup)
  *   compute/include/compute/sort.hpp:68-85 (rc_bitonic_sort_*)
@@ -22,7 +23,9 @@
  * Semantic choices at the SYCL boundary (AdaptiveCpp is not available here):
  *   sycl::half(float) = IEEE binary16 round-to-nearest-even;
  *   sycl::rsqrt(x)    = 1.f / sqrtf(x) (correctly rounded sqrt and divide);
- *   sycl::clz(0ull)   = 64.
+ *   sycl::clz(0ull)   = 64;
+ *   sycl::atan2(y, x) = the correctly rounded float atan2 (the C library's
+ *                       float atan2 under rc_set_libc_atan2(1)).
  *
  * Pinning (tests/test_oracle_kats.py, tests/test_oracle_pins.py): the
  * reference's KATs and golden hash, brute force (Morton keys and order,
@@ -134,8 +137,32 @@ void rc_np_set_contacts(rc_world *w, uint32_t n, const uint32_t *body_a, const u
                         const float *px, const float *py, const float *pz,
                         const float *nx, const float *ny, const float *nz, const float *depth);
 
+/* Joint SoA, field order of the reference JointView
+ * (core/include/core/articulation.hpp:47-70); type 0 fixed, 1 revolute,
+ * 2 prismatic, 3 ball. */
+typedef struct rc_joints {
+    const uint32_t *body_parent, *body_child;
+    const float *anchor_px, *anchor_py, *anchor_pz;
+    const float *anchor_cx, *anchor_cy, *anchor_cz;
+    const float *axis_px, *axis_py, *axis_pz;
+    const float *limit_lo, *limit_hi;
+    const float *target_pos, *target_vel;
+    const float *stiffness, *damping;
+    const float *compliance_pos, *compliance_ang;
+    const uint8_t *type;
+    uint32_t n;
+} rc_joints;
+
 /* XpbdSolver::solve with JointView{} (n = 0). */
 void rc_solve(rc_world *w, rc_bodies *b, float dt, int force_serial, rc_solve_stats *stats);
+/* XpbdSolver::solve(s, contacts, joints, bodies, dt) (xpbd_solver.cpp:590-733):
+ * constraint k is contact k (k < nc) or joint k - nc; joints may be NULL. */
+void rc_solve_joints(rc_world *w, rc_bodies *b, const rc_joints *joints, float dt, int force_serial,
+                     rc_solve_stats *stats);
+/* The atan2 of the revolute joint angle: correctly rounded (default), or the
+ * C library's float atan2 (on = 1), which the reference itself calls. */
+void rc_set_libc_atan2(int on);
+float rc_joint_atan2(float y, float x);
 /* Colour array of the last coloured solve (n_constraints entries). */
 void rc_solve_colors(const rc_world *w, uint32_t *out, uint32_t n);
 
@@ -145,6 +172,10 @@ typedef struct rc_frame_stats { uint32_t n_pairs, n_contacts, n_colors, used_ser
 void rc_frame(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *bounds,
               float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
               rc_frame_stats *st);
+/* rc_frame with joints (NULL = none). */
+void rc_frame_joints(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aabb *bounds, const rc_joints *joints,
+                     float gx, float gy, float gz, float dt, int n_substeps, int force_serial,
+                     rc_frame_stats *stats);
 
 /* ---- per-element entry points (for function-level parity checks) ----- */
 /* Contact of one pair (narrowphase.cpp:920-1000): ha/hb half extents, pa/pb

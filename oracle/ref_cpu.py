@@ -46,6 +46,16 @@ class RcSolveStats(C.Structure):
                                           "rounds", "overflow")]
 
 
+JOINT_F32 = ("anchor_px", "anchor_py", "anchor_pz", "anchor_cx", "anchor_cy", "anchor_cz",
+             "axis_px", "axis_py", "axis_pz", "limit_lo", "limit_hi", "target_pos", "target_vel",
+             "stiffness", "damping", "compliance_pos", "compliance_ang")
+
+
+class RcJoints(C.Structure):
+    _fields_ = [("body_parent", u32p), ("body_child", u32p)] + [(n, f32p) for n in JOINT_F32] + [
+        ("type", C.POINTER(C.c_uint8)), ("n", C.c_uint32)]
+
+
 class RcFrameStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("n_pairs", "n_contacts", "n_colors", "used_serial")]
 
@@ -103,10 +113,18 @@ def lib():
         L.rc_np_contacts.argtypes = [vp, C.POINTER(RcContacts)]
         L.rc_np_set_contacts.argtypes = [vp, C.c_uint32] + [vp] * 9
         L.rc_solve.argtypes = [vp, C.POINTER(RcBodies), C.c_float, C.c_int, C.POINTER(RcSolveStats)]
+        L.rc_solve_joints.argtypes = [vp, C.POINTER(RcBodies), C.POINTER(RcJoints), C.c_float, C.c_int,
+                                      C.POINTER(RcSolveStats)]
+        L.rc_set_libc_atan2.argtypes = [C.c_int]
+        L.rc_joint_atan2.restype = C.c_float
+        L.rc_joint_atan2.argtypes = [C.c_float, C.c_float]
         L.rc_solve_colors.argtypes = [vp, vp, C.c_uint32]
         L.rc_frame.argtypes = [vp, C.POINTER(RcBodies), C.POINTER(RcShapes), C.POINTER(RcAabb),
                                C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
                                C.POINTER(RcFrameStats)]
+        L.rc_frame_joints.argtypes = [vp, C.POINTER(RcBodies), C.POINTER(RcShapes), C.POINTER(RcAabb),
+                                      C.POINTER(RcJoints), C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                      C.c_int, C.POINTER(RcFrameStats)]
         L.rc_pair_contact.restype = C.c_int
         L.rc_pair_contact.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
         L.rc_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
@@ -197,6 +215,33 @@ class Shapes:
         return v
 
 
+class Joints:
+    """Host joint SoA (the oracle's JointStore): a dict of arrays with the
+    reference JointView's fields (body_parent, body_child: uint32; type:
+    uint8; the rest float32)."""
+
+    def __init__(self, arrays: dict):
+        n = len(arrays["body_parent"])
+        self.n = n
+        self.arr = {"body_parent": np.ascontiguousarray(arrays["body_parent"], np.uint32),
+                    "body_child": np.ascontiguousarray(arrays["body_child"], np.uint32),
+                    "type": np.ascontiguousarray(arrays["type"], np.uint8)}
+        for f in JOINT_F32:
+            self.arr[f] = np.ascontiguousarray(arrays[f], np.float32)
+            assert len(self.arr[f]) == n, f
+
+    def view(self) -> RcJoints:
+        v = RcJoints()
+        v.body_parent = _ptr(self.arr["body_parent"], C.c_uint32)
+        v.body_child = _ptr(self.arr["body_child"], C.c_uint32)
+        for f in JOINT_F32:
+            setattr(v, f, _ptr(self.arr[f], C.c_float))
+        v.type = _ptr(self.arr["type"], C.c_uint8)
+        v.n = self.n
+        self._keep = v
+        return v
+
+
 def aabb(bounds) -> RcAabb:
     return RcAabb(*[float(x) for x in bounds])
 
@@ -270,9 +315,13 @@ class World:
             ("norm_y", np.float32), ("norm_z", np.float32), ("depth", np.float32))]
         self.L.rc_np_set_contacts(self.h, n, *[a.ctypes.data for a in arrs])
 
-    def solve(self, bodies: Bodies, dt: float, force_serial: bool = False) -> dict:
+    def solve(self, bodies: Bodies, dt: float, force_serial: bool = False, joints: "Joints | None" = None) -> dict:
         st = RcSolveStats()
-        self.L.rc_solve(self.h, C.byref(bodies.view()), dt, int(force_serial), C.byref(st))
+        if joints is None:
+            self.L.rc_solve(self.h, C.byref(bodies.view()), dt, int(force_serial), C.byref(st))
+        else:
+            self.L.rc_solve_joints(self.h, C.byref(bodies.view()), C.byref(joints.view()), dt, int(force_serial),
+                                   C.byref(st))
         return {k: getattr(st, k) for k, _ in RcSolveStats._fields_}
 
     def colors(self, n: int) -> np.ndarray:
@@ -289,12 +338,20 @@ class World:
                             out.ctypes.data, len(r))
         return out[:, 0].view(np.float32).copy(), out[:, 1].copy()
 
-    def frame(self, bodies: Bodies, shapes: Shapes, sc, force_serial: bool = False) -> dict:
+    def frame(self, bodies: Bodies, shapes: Shapes, sc, force_serial: bool = False,
+              joints: "Joints | None" = None) -> dict:
         st = RcFrameStats()
         bd = aabb(sc.bounds)
         g = sc.gravity
-        self.L.rc_frame(self.h, C.byref(bodies.view()), C.byref(shapes.view()), C.byref(bd),
-                        g[0], g[1], g[2], sc.dt, sc.n_substeps, int(force_serial), C.byref(st))
+        if joints is None and getattr(sc, "joints", None) is not None:
+            joints = Joints(sc.joints)
+        if joints is None:
+            self.L.rc_frame(self.h, C.byref(bodies.view()), C.byref(shapes.view()), C.byref(bd),
+                            g[0], g[1], g[2], sc.dt, sc.n_substeps, int(force_serial), C.byref(st))
+        else:
+            self.L.rc_frame_joints(self.h, C.byref(bodies.view()), C.byref(shapes.view()), C.byref(bd),
+                                   C.byref(joints.view()), g[0], g[1], g[2], sc.dt, sc.n_substeps,
+                                   int(force_serial), C.byref(st))
         return {k: getattr(st, k) for k, _ in RcFrameStats._fields_}
 
 

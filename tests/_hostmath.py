@@ -36,6 +36,10 @@ def lib():
         L.hm_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
         L.hm_integrate_body.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
         L.hm_renormalise.argtypes = [vp]
+        L.hm_atan2_cr.restype = C.c_float
+        L.hm_atan2_cr.argtypes = [C.c_float, C.c_float]
+        L.hm_atan2_cr_array.argtypes = [vp, vp, vp, C.c_size_t]
+        L.hm_joint_pair.argtypes = [vp, vp, vp, C.c_uint32, C.c_float, C.c_int]
         _lib = L
     return _lib
 

@@ -243,7 +243,7 @@ TEST_CASE("XpbdSolver: static body is not moved", "[solver]") {
     REQUIRE(y[1] >= 1.f - 1e-3f);
 }
 
-TEST_CASE("XpbdSolver: joints are rejected", "[solver]") {
+TEST_CASE("XpbdSolver: a joint view without arrays is rejected", "[solver]") {
     Scene sc(1, 1);
     add_spheres(sc, {{0.f, 0.f, 0.f}}, 1.f);
     Narrowphase np(sc.s, 4);
