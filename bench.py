@@ -54,6 +54,8 @@ def parse():
                     help="frames of the 4096-scene batched workload (0 skips it)")
     ap.add_argument("--diff-frames", type=int, default=120,
                     help="frames of the differentiable 256-scene workload (0 skips it)")
+    ap.add_argument("--joint-frames", type=int, default=120,
+                    help="frames of the articulated workload (0 skips it)")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="scenes of the batched workload stepped through the plain per-scene loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -246,6 +248,68 @@ def batched(capi, scenes, frames: int, plain_sample: int) -> dict:
     }
 
 
+def articulated(capi, scenes, frames: int, check_frames: int = 20) -> dict:
+    """Joint chains with contacts (1024 chains of 8 links over a ground,
+    every joint type, limits, PD motors, compliance): the drop-in frame with
+    a JointView, no host synchronisation inside the timed frames.  The oracle
+    runs check_frames of the same scene on the host; its final state hash
+    must equal the GPU's after as many frames."""
+    sc = scenes.joint_chains(1024, 8, seed=20260006)
+    n_joints = len(sc.joints["type"])
+    warm = capi.World(sc)
+    for _ in range(3):
+        warm.step_async()
+    warm.check()
+    w = capi.World(sc)
+    e0, e1 = capi.Event(), capi.Event()
+    w.s.wait()
+    t0 = time.perf_counter()
+    e0.record(w.s)
+    for _ in range(frames):
+        w.step_async()
+    e1.record(w.s)
+    w.s.wait()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1)
+    last = w.check()  # raises on any device error flag
+    # the solver's split, on further frames (its event timing waits for each solve)
+    w.solver.set_timing(True)
+    for _ in range(20):
+        w.step_async()
+    tm = w.solver.timing(w.s)
+    # the oracle on the host, same frames from the same initial state
+    from oracle import ref_cpu
+    ref_cpu.build()
+    ob, osh = ref_cpu.Bodies(sc), ref_cpu.Shapes(sc)
+    ow = ref_cpu.World.for_scene(sc)
+    js = ref_cpu.Joints(sc.joints)
+    c0 = time.perf_counter()
+    for _ in range(check_frames):
+        ow.frame(ob, osh, sc, joints=js)
+    cpu_s = time.perf_counter() - c0
+    g = capi.World(sc)
+    for _ in range(check_frames):
+        g.step_async()
+    g.check()
+    same = g.fnv() == ob.fnv()
+    return {
+        "workload": f"joint chains: 1024 static bases x 8 links (boxes, spheres), {n_joints} joints "
+                    f"(fixed, revolute, prismatic, ball; limits, PD motors, compliance) + contacts "
+                    f"over a ground, seed 20260006, {frames} frames at 60 Hz, 10 iterations",
+        "bodies": sc.n, "joints": n_joints, "frames": frames,
+        "value": round(sc.n * frames / wall, 1), "unit": "body-substeps/s",
+        "ms_per_frame": round(1000.0 * wall / frames, 4), "stream_ms_per_frame": round(gpu_ms / frames, 4),
+        "solver_ms_per_frame": {"assign": round(tm["assign_ms"] / max(tm["solves"], 1), 4),
+                                "sweep": round(tm["sweep_ms"] / max(tm["solves"], 1), 4)},
+        "last_frame": {"contacts": last["n_contacts"], "classes": last["n_colors"],
+                       "used_serial": last["used_serial"]},
+        "cpu_baseline": {"value": round(sc.n * check_frames / cpu_s, 1), "unit": "body-substeps/s",
+                         "cores": ref_cpu.lib().rc_max_threads(), "kind": "port",
+                         "sample": f"the oracle's frames 0..{check_frames - 1} of the same scene"},
+        "state_hash_equal_after_check_frames": bool(same),
+    }
+
+
 def differentiable(capi, scenes, frames: int = 120, every: int = 10) -> dict:
     """Config 4: 256 thrown 4x4x4 box stacks, loss sum |x_T - x*|^2.
     Forward alone (one launch of all frames) against forward with
@@ -416,6 +480,8 @@ def main():
         out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
     if args.diff_frames > 0:
         out["differentiable"] = differentiable(capi, scenes, args.diff_frames)
+    if args.joint_frames > 0:
+        out["articulated"] = articulated(capi, scenes, args.joint_frames)
     print(json.dumps(out), flush=True)
 
 

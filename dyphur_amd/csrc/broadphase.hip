@@ -240,18 +240,23 @@ __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, 
             __syncthreads();
         }
         grid_barrier(bar, gen, err, kErrSortBarrier);
-        // B. per-digit scan across tiles
-        for (uint32_t d = blockIdx.x; d < 256u; d += gridDim.x) {
+        // B. per-digit scan across tiles, one wavefront per digit (no
+        // workgroup barriers: the 256 digits spread over every wave of the grid)
+        for (uint32_t d = blockIdx.x * kSgWaves + wave; d < 256u; d += gridDim.x * kSgWaves) {
             uint32_t carry = 0;
-            for (uint32_t c0 = 0; c0 < n_tiles; c0 += kSgThreads) {
-                const bool in = c0 + t < n_tiles;
-                const uint32_t v = in ? mat[d * n_tiles + c0 + t] : 0u;
-                uint32_t total = 0;
-                const uint32_t ex = block_exclusive_scan(v, wsum, total);
-                if (in) mat[d * n_tiles + c0 + t] = carry + ex;
-                carry += total;
+            for (uint32_t c0 = 0; c0 < n_tiles; c0 += 64u) {
+                const bool in = c0 + lane < n_tiles;
+                const uint32_t v = in ? mat[d * n_tiles + c0 + lane] : 0u;
+                uint32_t x = v;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t y = __shfl_up(x, static_cast<unsigned>(o), 64);
+                    if (lane >= static_cast<uint32_t>(o)) x += y;
+                }
+                if (in) mat[d * n_tiles + c0 + lane] = carry + x - v;
+                carry += __shfl(x, 63, 64);
             }
-            if (t == 0) tot[d] = carry;
+            if (lane == 0) tot[d] = carry;
         }
         grid_barrier(bar, gen, err, kErrSortBarrier);
         // C. digit bases and stable scatter
@@ -678,7 +683,7 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     const uint32_t n_tiles = ceil_div(n, kSgTile);
     bool sorted = false;
     if (bp->sort_blocks > 0 && n_tiles > 4) {
-        const uint32_t blocks = n_tiles < bp->sort_blocks ? n_tiles : bp->sort_blocks;
+        const uint32_t blocks = bp->sort_blocks;  // all of them: the digit scans use every wave
         dy_body_view bv = *bodies;
         dy_aabb bd = *bounds;
         uint32_t nv = n, ntv = n_tiles;

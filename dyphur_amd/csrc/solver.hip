@@ -76,6 +76,8 @@ enum : uint32_t {
     kMFcnt = 20,         // 3 words: frontier sizes
     kMRegSweep = 24,     // 2 words: check-in of sweep_xcd (participants, arrivals)
     kMRegAssign = 26,    // 2 words: check-in of assign_lists
+    kMRegMotor = 28,     // 2 words: check-in of joint_motors_xcd
+    kMBarMotor = 30,     // 2 words: barrier of joint_motors_xcd
     kMWords = 32,
 };
 constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u, kErrFlow = 16u, kErrJoint = 32u;
@@ -854,6 +856,48 @@ __device__ __forceinline__ void solve_pre(const CPre &c, uint32_t it, float *__r
     if (db) store_aos(dyn, c.ab.y, B, it);
 }
 
+// Joint motors on the workgroups of XCD kSweepXcc (cooperative, like
+// sweep_xcd): a class's motor joints spread over every participant, an L2
+// barrier between classes, body records read at agent scope (other
+// workgroups wrote them in earlier classes).  On the reference-order path
+// the classes are dependency levels, possibly many and mostly small: the
+// first participant alone runs them, with workgroup barriers.
+__global__ __launch_bounds__(1024) void joint_motors_xcd(const uint32_t *__restrict__ boff, CRec rec,
+                                                         float4 *__restrict__ dyn, const float4 *__restrict__ sta,
+                                                         const float4 *__restrict__ jpk, float dt, uint32_t *meta) {
+    uint32_t *bar = meta + kMBarMotor, *err = meta + kMErr;
+    uint32_t b = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegMotor, err, kErrBarrier, b, nblk)) return;
+    if ((meta[kMErr] & kErrAssign) || meta[kMNj] == 0u) return;
+    const uint32_t n_cls = meta[kMNcls];
+    const bool serial = meta[kMSerial] != 0u;
+    if (serial && b != 0) return;
+    const uint32_t first = serial ? threadIdx.x : b * blockDim.x + threadIdx.x;
+    const uint32_t stride = serial ? blockDim.x : nblk * blockDim.x;
+    uint32_t gen = 0;
+    for (uint32_t k = 0; k < n_cls; ++k) {
+        const uint32_t end = boff[k + 1];
+        for (uint32_t i = boff[k] + first; i < end; i += stride) {
+            if (__float_as_uint(rec.p1[i].w) != kRecJoint) continue;
+            const JointIn j = load_joint(jpk, __float_as_uint(rec.p0[i].x));
+            if (j.type != kJointRevolute && j.type != kJointPrismatic) continue;
+            const uint2 ab = rec.ab[i];
+            uint32_t ra, rb;
+            BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
+            joint_motor(A, B, j, dt);
+            if (A.im != 0.f) store_aos(dyn, ab.x, A, ra);
+            if (B.im != 0.f) store_aos(dyn, ab.y, B, rb);
+        }
+        if (k + 1 == n_cls) break;
+        if (serial) {
+            wait_vmem();
+            __syncthreads();
+        } else if (!xcd_barrier(bar, gen, nblk, err, kErrBarrier)) {
+            return;
+        }
+    }
+}
+
 // All I x K phases on the workgroups of XCD kSweepXcc.  The launch has
 // workgroups on every XCD (cooperative, all co-resident); each checks in once
 // at agent scope, the others leave, and the participants number themselves
@@ -1125,7 +1169,7 @@ struct dy_xpbd_s {
     // Co-resident workgroup limits of the cooperative kernels, [0] contacts
     // only, [1] the variants that also solve joints.
     int coop_assign_max = 0, coop_sweep_max[2] = {0, 0}, coop_xcd_max[2] = {0, 0}, coop_flow_max[2] = {0, 0},
-        coop_lanes_max[2] = {0, 0}, device_cus = 0;
+        coop_lanes_max[2] = {0, 0}, coop_motor_max = 0, device_cus = 0;
     // Constraint capacity (contacts + joints) of the constraint-indexed
     // buffers, and the joint capacity of jpk / jend.
     uint32_t ccap = 0, jcap = 0;
@@ -1307,6 +1351,9 @@ int query_coop(dy_xpbd_s *x) {
             DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[k][j], x->sweep_threads, 0));
             outs[k][j] = per_cu * x->device_cus;
         }
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(joint_motors_xcd),
+                                                        x->sweep_threads, 0));
+    x->coop_motor_max = per_cu * x->device_cus;
     return DY_OK;
 }
 
@@ -1509,8 +1556,28 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
         DY_LAUNCH_CHECK();
     }
     if (nj) {
-        joint_motors<<<1, 1024, 0, st>>>(x->boff, rec, x->dyn.p, x->sta.p, x->jpk, dt, x->meta);
-        DY_LAUNCH_CHECK();
+        bool motors = false;
+        if (coop && x->coop_motor_max > 0) {
+            // P participants on one XCD out of a launch of P per XCD, as sweep_xcd.
+            uint32_t blocks = 8u * coop_blocks(x, 0, x->sweep_threads, x->coop_motor_max);
+            if (blocks > static_cast<uint32_t>(x->coop_motor_max)) blocks = static_cast<uint32_t>(x->coop_motor_max);
+            const uint32_t *boffv = x->boff;
+            float4 *dynv = x->dyn.p;
+            const float4 *stav = x->sta.p, *jpkv0 = x->jpk;
+            float dtv = dt;
+            uint32_t *metav = x->meta;
+            void *args[] = {&boffv, &rec, &dynv, &stav, &jpkv0, &dtv, &metav};
+            motors = try_coop_launch(reinterpret_cast<const void *>(joint_motors_xcd), dim3(blocks),
+                                     dim3(static_cast<uint32_t>(x->sweep_threads)), args, st) == hipSuccess;
+            if (!motors) {
+                x->coop_refused = true;
+                coop = false;
+            }
+        }
+        if (!motors) {
+            joint_motors<<<1, 1024, 0, st>>>(x->boff, rec, x->dyn.p, x->sta.p, x->jpk, dt, x->meta);
+            DY_LAUNCH_CHECK();
+        }
     }
 
     const uint32_t iters = static_cast<uint32_t>(x->n_iters);
