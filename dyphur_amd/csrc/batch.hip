@@ -785,10 +785,33 @@ constexpr uint32_t kCsStride = 22;
 
 struct DiffArgs {
     dy_body_view adj;  // adjoint of the 13 mutable fields; inv_mass holds the inverse-mass adjoint
-    float *cs;         // kCsStride per contact
+    float *cs;         // kCsStride per contact of capacity: 8 accumulators, then 14 pose gradients
     uint32_t frames;   // recorded frames to reverse, newest first
     uint32_t max_n;    // largest scene: LDS holds 21 floats per body
+    uint32_t acc_lds;  // 1: the 8 accumulators per contact live in LDS after the body adjoints
 };
+
+// The tape's inputs of one contact projection, loaded one phase ahead.
+struct PreRec {
+    uint32_t j, a, b;
+    float4 r0, r1;
+    float4 m[7];  // both bodies' 13 fields and lambda (28 floats)
+};
+
+__device__ __forceinline__ void pre_fetch(PreRec &P, const float *fb, const TapeLayout &TL, const uint32_t *co,
+                                          uint32_t C, uint32_t it, uint32_t c, uint32_t r) {
+    P.j = co[c] + r;
+    if (P.j >= co[c + 1]) return;
+    const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
+    const float4 *rec = reinterpret_cast<const float4 *>(fb + TL.rec);
+    const float4 *pr = reinterpret_cast<const float4 *>(fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + P.j));
+    P.a = ab[2 * P.j];
+    P.b = ab[2 * P.j + 1];
+    P.r0 = rec[2 * P.j];
+    P.r1 = rec[2 * P.j + 1];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) P.m[k] = pr[k];
+}
 
 __device__ __forceinline__ BodyBar<float> load_bar(const dy_body_view &v, uint32_t i) {
     BodyBar<float> b;
@@ -824,6 +847,33 @@ __device__ __forceinline__ BodyT<float> body_in(const float *m, const dy_body_vi
 // then its classes last to first, contacts of a class in parallel: they
 // share no dynamic body), then the contact geometry (class by class), then
 // the integrator.  Static bodies receive no adjoint.
+// One contact's projection adjoint: inputs m (28 floats: both bodies' 13
+// fields and lambda), accumulators acc (lambda adjoint, then the contact's
+// p, n, depth adjoints), body adjoints in LDS.
+__device__ __forceinline__ void contact_phase_adj(const BatchArgs &A, float h, const dy_body_view &gv,
+                                                  dy_body_view &av, uint32_t a, uint32_t b, const float4 &r0,
+                                                  const float4 &r1, const float *m, float *acc) {
+    ContactT<float> ct;
+    ct.p[0] = r0.x; ct.p[1] = r0.y; ct.p[2] = r0.z;
+    ct.n[0] = r0.w; ct.n[1] = r1.x; ct.n[2] = r1.y;
+    ct.depth = r1.z;
+    const BodyT<float> A0 = body_in(m, gv, a), B0 = body_in(m + 13, gv, b);
+    BodyBar<float> Ab = load_bar(av, a), Bb = load_bar(av, b);
+    float lb = acc[0];
+    ContactT<float> cb;
+    for (int k = 0; k < 3; ++k) { cb.p[k] = 0.f; cb.n[k] = 0.f; }
+    cb.depth = 0.f;
+    project_adj(ct, A0, B0, A.mu, m[26], h, Ab, Bb, lb, cb);
+    acc[0] = lb;
+    for (int k = 0; k < 3; ++k) {
+        acc[1 + k] += cb.p[k];
+        acc[4 + k] += cb.n[k];
+    }
+    acc[7] += cb.depth;
+    if (A0.im != 0.f) store_bar(av, a, Ab);
+    if (B0.im != 0.f) store_bar(av, b, Bb);
+}
+
 __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
     extern __shared__ float lds_adj[];
     const uint32_t s = blockIdx.x;
@@ -857,6 +907,9 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
     const TapeLayout TL = tape_layout(n, C, iters);
     const float *tp = A.tape + A.tape_off[s];
     float *cs = D.cs + static_cast<uint64_t>(kCsStride) * A.contact_off[s];
+    // per contact: 8 accumulators (LDS when they fit), 14 pose gradients
+    float *acc = D.acc_lds ? lds_adj + 21u * n : cs;
+    float *pg = cs + 8ull * C;
     const float h = A.h;
     const float g[3] = {A.gx, A.gy, A.gz};
     for (int f = static_cast<int>(D.frames) - 1; f >= 0; --f) {
@@ -865,10 +918,14 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
         const uint32_t nc = hdr[0], ncls = hdr[1];
         if (nc > 0) {
             const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
-            const float4 *rec = reinterpret_cast<const float4 *>(fb + TL.rec);
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
-            for (uint32_t j = r; j < kCsStride * nc; j += 64) cs[j] = 0.f;
+            for (uint32_t j = r; j < 8u * nc; j += 64) acc[j] = 0.f;
             tm.sync();
+            // phases (it, c) newest first; each lane's first contact of the
+            // next phase is loaded while this one computes
+            PreRec cur, nxt;
+            cur.j = ~0u;
+            if (iters > 0) pre_fetch(cur, fb, TL, co, C, iters - 1u, ncls - 1u, r);
             for (int it = static_cast<int>(iters) - 1; it >= 0; --it) {
                 const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
                 for (uint32_t i = r; i < n; i += 64) {
@@ -880,32 +937,22 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                 }
                 tm.sync();
                 for (int c = static_cast<int>(ncls) - 1; c >= 0; --c) {
+                    const bool more = c > 0 || it > 0;
+                    if (more)
+                        pre_fetch(nxt, fb, TL, co, C, c > 0 ? static_cast<uint32_t>(it) : static_cast<uint32_t>(it - 1),
+                                  c > 0 ? static_cast<uint32_t>(c - 1) : ncls - 1u, r);
                     const uint32_t end = co[c + 1];
-                    for (uint32_t j = co[c] + r; j < end; j += 64) {
-                        const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
-                        const float4 r0 = rec[2 * j], r1 = rec[2 * j + 1];
-                        ContactT<float> ct;
-                        ct.p[0] = r0.x; ct.p[1] = r0.y; ct.p[2] = r0.z;
-                        ct.n[0] = r0.w; ct.n[1] = r1.x; ct.n[2] = r1.y;
-                        ct.depth = r1.z;
-                        const float *pr = fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + j);
-                        const BodyT<float> A0 = body_in(pr, gv, a), B0 = body_in(pr + 13, gv, b);
-                        BodyBar<float> Ab = load_bar(av, a), Bb = load_bar(av, b);
-                        float lb = cs[kCsStride * j];
-                        ContactT<float> cb;
-                        for (int k = 0; k < 3; ++k) { cb.p[k] = 0.f; cb.n[k] = 0.f; }
-                        cb.depth = 0.f;
-                        project_adj(ct, A0, B0, A.mu, pr[26], h, Ab, Bb, lb, cb);
-                        cs[kCsStride * j] = lb;
-                        for (int k = 0; k < 3; ++k) {
-                            cs[kCsStride * j + 1 + k] += cb.p[k];
-                            cs[kCsStride * j + 4 + k] += cb.n[k];
-                        }
-                        cs[kCsStride * j + 7] += cb.depth;
-                        if (A0.im != 0.f) store_bar(av, a, Ab);
-                        if (B0.im != 0.f) store_bar(av, b, Bb);
+                    if (cur.j < end)
+                        contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
+                                          reinterpret_cast<const float *>(cur.m), acc + 8u * cur.j);
+                    for (uint32_t j = cur.j + 64; j < end; j += 64) {  // classes of more than 64 contacts
+                        PreRec x;
+                        pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(it), static_cast<uint32_t>(c), r + (j - cur.j));
+                        contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1, reinterpret_cast<const float *>(x.m),
+                                          acc + 8u * j);
                     }
                     tm.sync();
+                    if (more) cur = nxt;
                 }
             }
             // contact geometry: (p, n, depth) adjoints to the post-integrate
@@ -924,13 +971,13 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                 const float pb[3] = {po[b], po[n + b], po[2 * n + b]};
                 const float qb[4] = {po[3 * n + b], po[4 * n + b], po[5 * n + b], po[6 * n + b]};
                 float hbar[7];
-                for (int k = 0; k < 7; ++k) hbar[k] = cs[kCsStride * j + 1 + k];
+                for (int k = 0; k < 7; ++k) hbar[k] = acc[8u * j + 1 + k];
                 float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 const bool done = ta == kBox && tb == kBox && box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
                 if (!done) (void)contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
                 for (int k = 0; k < 7; ++k) {
-                    cs[kCsStride * j + 8 + k] = ga[k];
-                    cs[kCsStride * j + 15 + k] = gb2[k];
+                    pg[14u * j + k] = ga[k];
+                    pg[14u * j + 7 + k] = gb2[k];
                 }
             }
             tm.sync();
@@ -938,7 +985,7 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                 const uint32_t end = co[c + 1];
                 for (uint32_t j = co[c] + r; j < end; j += 64) {
                     const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
-                    const float *ga = cs + kCsStride * j + 8, *gb2 = cs + kCsStride * j + 15;
+                    const float *ga = pg + 14u * j, *gb2 = pg + 14u * j + 7;
                     if (gv.inv_mass[a] != 0.f) {
                         av.pos_x[a] += ga[0]; av.pos_y[a] += ga[1]; av.pos_z[a] += ga[2];
                         av.rot_w[a] += ga[3]; av.rot_x[a] += ga[4]; av.rot_y[a] += ga[5]; av.rot_z[a] += ga[6];
@@ -1115,6 +1162,13 @@ BatchArgs batch_args(dy_batch_s *b, const dy_body_view &bodies, const dy_shape_v
     a.force_serial = b->force_serial;
     a.mu = b->mu;
     return a;
+}
+
+// Every scene of the current launch shape runs LDS-resident: the condition
+// for recording a tape.
+bool batch_tape_ok(const dy_batch_s *b) {
+    const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
+    return it != b->shapes.end() && it->second.glob.n_units == 0;
 }
 
 int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
@@ -1398,11 +1452,16 @@ struct dy_diff_s {
     dy_batch_s *b = nullptr;
     uint32_t max_frames = 0, every = 0, frames = 0;
     size_t n_total = 0;
-    float *d_tape = nullptr;
+    // Two tapes of one segment each: the reverse sweep of segment k reads one
+    // while segment k - 1 is recomputed into the other on the recompute stream.
+    float *d_tape[2] = {nullptr, nullptr};
     uint64_t *d_tape_off = nullptr;
     float *d_ckpt = nullptr;  // checkpoints: 13 x n_total floats each
     float *d_work = nullptr;  // recompute state: 13 x n_total
     float *d_cs = nullptr;    // kCsStride per contact
+    hipStream_t rs = nullptr;  // recompute stream (default priority)
+    hipEvent_t ev_start = nullptr, ev_tape[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    int newest = -1;  // tape holding the last forward's newest segment, or -1
 };
 
 namespace {
@@ -1427,9 +1486,14 @@ int copy_state(hipStream_t s, float *const dst[13], float *const src[13], size_t
 }
 
 void diff_free(dy_diff_s *d) {
-    void *ps[] = {d->d_tape, d->d_tape_off, d->d_ckpt, d->d_work, d->d_cs};
+    if (d->rs) (void)hipStreamSynchronize(d->rs);
+    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_ckpt, d->d_work, d->d_cs};
     for (void *p : ps)
         if (p) (void)hipFree(p);
+    hipEvent_t evs[] = {d->ev_start, d->ev_tape[0], d->ev_tape[1], d->ev_free[0], d->ev_free[1]};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (d->rs) (void)hipStreamDestroy(d->rs);
 }
 
 }  // namespace
@@ -1452,8 +1516,13 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     }
     const uint32_t n_ckpt = (max_frames + checkpoint_every - 1) / checkpoint_every;
     int st = DY_OK;
-    if (st == DY_OK) st = dev_alloc(d->d_tape, off.back());
+    if (st == DY_OK) st = dev_alloc(d->d_tape[0], off.back());
+    if (st == DY_OK) st = dev_alloc(d->d_tape[1], off.back());
     if (st == DY_OK) st = dev_alloc(d->d_tape_off, off.size());
+    if (st == DY_OK && hipStreamCreateWithFlags(&d->rs, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
+    hipEvent_t *evs[] = {&d->ev_start, &d->ev_tape[0], &d->ev_tape[1], &d->ev_free[0], &d->ev_free[1]};
+    for (hipEvent_t *e : evs)
+        if (st == DY_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) st = DY_ERR_INTERNAL;
     if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(b->contact_off.back(), 1ull));
@@ -1483,6 +1552,8 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
     const hipStream_t hsm = hs(s);
     float *src[13];
     mutable_fields(*bodies, src);
+    d->newest = -1;
+    const uint32_t n_seg = (static_cast<uint32_t>(n_frames) + d->every - 1) / d->every;
     for (uint32_t f0 = 0, k = 0; f0 < static_cast<uint32_t>(n_frames); f0 += d->every, ++k) {
         dy_body_view ck{};
         set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * d->n_total, d->n_total);
@@ -1490,7 +1561,12 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
         mutable_fields(ck, dst);
         DY_TRY_STATUS(copy_state(hsm, dst, src, d->n_total));
         const int nf = static_cast<int>(std::min<uint32_t>(d->every, static_cast<uint32_t>(n_frames) - f0));
-        DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, 1, nf));
+        // The newest segment is recorded as it runs: the backward starts
+        // from it without recomputing it.
+        const bool newest = k + 1 == n_seg && batch_tape_ok(d->b);
+        DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, 1, nf,
+                                   newest ? d->d_tape[k % 2] : nullptr, newest ? d->d_tape_off : nullptr));
+        if (newest) d->newest = static_cast<int>(k % 2);
     }
     d->frames = static_cast<uint32_t>(n_frames);
     return DY_OK;
@@ -1516,25 +1592,54 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     DiffArgs D{};
     D.adj = *adj_initial;
     D.cs = d->d_cs;
-    for (int k = static_cast<int>(n_seg) - 1; k >= 0; --k) {
-        const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
+    // The 8 per-contact accumulators go to LDS beside the body adjoints when
+    // the largest scene's share fits in 64 KiB.
+    uint64_t max_c = 0;
+    for (size_t k = 0; k + 1 < d->b->contact_off.size(); ++k)
+        max_c = std::max<uint64_t>(max_c, d->b->contact_off[k + 1] - d->b->contact_off[k]);
+    const uint64_t lds_floats = 21ull * d->b->max_n + 8ull * max_c;
+    D.acc_lds = lds_floats * sizeof(float) <= 65536u ? 1u : 0u;
+    const size_t lds_bytes = (D.acc_lds ? lds_floats : 21ull * d->b->max_n) * sizeof(float);
+    // Segments newest first.  Segment k's reverse sweep (stream s) reads tape
+    // k % 2 while segment k - 1 is recomputed from its checkpoint into the
+    // other tape on the recompute stream; events order the hand-overs.  The
+    // reverse sweep reads only the constant body fields of `work`, which the
+    // recompute does not write.
+    DY_TRY(hipEventRecord(d->ev_start, hsm));
+    DY_TRY(hipStreamWaitEvent(d->rs, d->ev_start, 0));
+    auto recompute = [&](uint32_t k) -> int {
+        const uint32_t f0 = k * d->every;
         const uint32_t nf = std::min(d->every, d->frames - f0);
+        const int t = static_cast<int>(k % 2);
+        DY_TRY(hipStreamWaitEvent(d->rs, d->ev_free[t], 0));  // the reverse sweep of segment k + 2 is done with it
         dy_body_view ck{};
         set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * n, n);
         float *cp[13];
         mutable_fields(ck, cp);
-        DY_TRY_STATUS(copy_state(hsm, wk, cp, n));
-        DY_TRY_STATUS(batch_launch(d->b, hsm, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape,
+        DY_TRY_STATUS(copy_state(d->rs, wk, cp, n));
+        DY_TRY_STATUS(batch_launch(d->b, d->rs, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape[t],
                                    d->d_tape_off));
+        DY_TRY(hipEventRecord(d->ev_tape[t], d->rs));
+        DY_TRY(hipStreamWaitEvent(hsm, d->ev_tape[t], 0));
+        return DY_OK;
+    };
+    const uint32_t last = n_seg - 1;
+    if (d->newest != static_cast<int>(last % 2)) DY_TRY_STATUS(recompute(last));
+    for (int k = static_cast<int>(last); k >= 0; --k) {
+        const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
+        const uint32_t nf = std::min(d->every, d->frames - f0);
+        const int t = k % 2;
         BatchArgs a = batch_args(d->b, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf));
-        a.tape = d->d_tape;
+        a.tape = d->d_tape[t];
         a.tape_off = d->d_tape_off;
         D.frames = nf;
         D.max_n = d->b->max_n;
-        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), 21ull * d->b->max_n * sizeof(float), hsm,
-                           a, D);
+        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), lds_bytes, hsm, a, D);
         DY_LAUNCH_CHECK();
+        DY_TRY(hipEventRecord(d->ev_free[t], hsm));
+        if (k > 0) DY_TRY_STATUS(recompute(static_cast<uint32_t>(k - 1)));
     }
+    d->newest = -1;  // the tapes now hold older segments
     return DY_OK;
 }
 
