@@ -75,6 +75,7 @@ struct BatchArgs {
     uint64_t *stage_clk;   // kStages per scene when timed
     float *tape;           // differentiable form: per-frame records (TapeLayout)
     const uint64_t *tape_off;  // per scene, floats
+    int keep_input;        // 1: the final state is not written back (a recompute from a checkpoint)
     uint32_t region_floats;  // LDS floats per team
     float gx, gy, gz, h;
     int n_sub, n_frames, n_iters, force_serial;
@@ -720,7 +721,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
     }
     if constexpr (RES) {
         // write back the mutable fields: pos, rot, vel, ang
-        for (uint32_t i = r; i < n; i += NT) {
+        for (uint32_t i = r; i < n && !A.keep_input; i += NT) {
             gv.pos_x[i] = bv.pos_x[i]; gv.pos_y[i] = bv.pos_y[i]; gv.pos_z[i] = bv.pos_z[i];
             gv.rot_w[i] = bv.rot_w[i]; gv.rot_x[i] = bv.rot_x[i]; gv.rot_y[i] = bv.rot_y[i];
             gv.rot_z[i] = bv.rot_z[i];
@@ -1173,14 +1174,16 @@ bool batch_tape_ok(const dy_batch_s *b) {
 
 int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
                  float gy, float gz, float dt, int n_substeps, int n_frames, float *tape = nullptr,
-                 const uint64_t *tape_off = nullptr) {
+                 const uint64_t *tape_off = nullptr, bool keep_input = false) {
     const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
     if (it == b->shapes.end()) return DY_ERR_INTERNAL;
     const dy_batch_s::Shape &sh = it->second;
     BatchArgs a = batch_args(b, bodies, shapes, gx, gy, gz, dt, n_substeps, n_frames);
     a.tape = tape;
     a.tape_off = tape_off;
+    a.keep_input = keep_input ? 1 : 0;
     if (tape && sh.glob.n_units) return DY_ERR_INVALID_ARG;
+    if (keep_input && sh.glob.n_units) return DY_ERR_INVALID_ARG;
     for (int pass = 0; pass < 2; ++pass) {
         const dy_batch_s::Units &u = pass == 0 ? sh.res : sh.glob;
         if (u.n_units == 0) continue;
@@ -1457,7 +1460,6 @@ struct dy_diff_s {
     float *d_tape[2] = {nullptr, nullptr};
     uint64_t *d_tape_off = nullptr;
     float *d_ckpt = nullptr;  // checkpoints: 13 x n_total floats each
-    float *d_work = nullptr;  // recompute state: 13 x n_total
     float *d_cs = nullptr;    // kCsStride per contact
     hipStream_t rs = nullptr;  // recompute stream (default priority)
     hipEvent_t ev_start = nullptr, ev_tape[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
@@ -1487,7 +1489,7 @@ int copy_state(hipStream_t s, float *const dst[13], float *const src[13], size_t
 
 void diff_free(dy_diff_s *d) {
     if (d->rs) (void)hipStreamSynchronize(d->rs);
-    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_ckpt, d->d_work, d->d_cs};
+    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_ckpt, d->d_cs};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     hipEvent_t evs[] = {d->ev_start, d->ev_tape[0], d->ev_tape[1], d->ev_free[0], d->ev_free[1]};
@@ -1524,7 +1526,6 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     for (hipEvent_t *e : evs)
         if (st == DY_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) st = DY_ERR_INTERNAL;
     if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
-    if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(b->contact_off.back(), 1ull));
     if (st == DY_OK && hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
         st = DY_ERR_INTERNAL;
@@ -1584,10 +1585,6 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     mutable_fields(*adj_initial, ini);
     if (fin[0] != ini[0]) DY_TRY_STATUS(copy_state(hsm, ini, fin, n));
     DY_TRY(hipMemsetAsync(adj_initial->inv_mass, 0, n * sizeof(float), hsm));
-    dy_body_view work = *bodies;
-    set_mutable_fields(work, d->d_work, n);
-    float *wk[13];
-    mutable_fields(work, wk);
     const uint32_t n_seg = (d->frames + d->every - 1) / d->every;
     DiffArgs D{};
     D.adj = *adj_initial;
@@ -1603,8 +1600,8 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     // Segments newest first.  Segment k's reverse sweep (stream s) reads tape
     // k % 2 while segment k - 1 is recomputed from its checkpoint into the
     // other tape on the recompute stream; events order the hand-overs.  The
-    // reverse sweep reads only the constant body fields of `work`, which the
-    // recompute does not write.
+    // reverse sweep reads only the constant body fields (inverse mass and
+    // inertia, shape, flags) of the body view.
     DY_TRY(hipEventRecord(d->ev_start, hsm));
     DY_TRY(hipStreamWaitEvent(d->rs, d->ev_start, 0));
     auto recompute = [&](uint32_t k) -> int {
@@ -1612,13 +1609,11 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         const uint32_t nf = std::min(d->every, d->frames - f0);
         const int t = static_cast<int>(k % 2);
         DY_TRY(hipStreamWaitEvent(d->rs, d->ev_free[t], 0));  // the reverse sweep of segment k + 2 is done with it
-        dy_body_view ck{};
+        // straight from checkpoint k, which the recompute reads but does not overwrite
+        dy_body_view ck = *bodies;
         set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * n, n);
-        float *cp[13];
-        mutable_fields(ck, cp);
-        DY_TRY_STATUS(copy_state(d->rs, wk, cp, n));
-        DY_TRY_STATUS(batch_launch(d->b, d->rs, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape[t],
-                                   d->d_tape_off));
+        DY_TRY_STATUS(batch_launch(d->b, d->rs, ck, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape[t],
+                                   d->d_tape_off, true));
         DY_TRY(hipEventRecord(d->ev_tape[t], d->rs));
         DY_TRY(hipStreamWaitEvent(hsm, d->ev_tape[t], 0));
         return DY_OK;
@@ -1629,7 +1624,7 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
         const uint32_t nf = std::min(d->every, d->frames - f0);
         const int t = k % 2;
-        BatchArgs a = batch_args(d->b, work, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf));
+        BatchArgs a = batch_args(d->b, *bodies, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf));
         a.tape = d->d_tape[t];
         a.tape_off = d->d_tape_off;
         D.frames = nf;
