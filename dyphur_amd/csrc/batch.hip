@@ -789,7 +789,7 @@ struct DiffArgs {
     float *cs;         // kCsStride per contact of capacity: 8 accumulators, then 14 pose gradients
     uint32_t frames;   // recorded frames to reverse, newest first
     uint32_t max_n;    // largest scene: LDS holds 21 floats per body
-    uint32_t acc_lds;  // 1: the 8 accumulators per contact live in LDS after the body adjoints
+    uint32_t acc_lds;  // contacts whose 8 accumulators live in LDS after the body adjoints (the rest: cs)
 };
 
 // The tape's inputs of one contact projection, loaded one phase ahead.
@@ -908,8 +908,11 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
     const TapeLayout TL = tape_layout(n, C, iters);
     const float *tp = A.tape + A.tape_off[s];
     float *cs = D.cs + static_cast<uint64_t>(kCsStride) * A.contact_off[s];
-    // per contact: 8 accumulators (LDS when they fit), 14 pose gradients
-    float *acc = D.acc_lds ? lds_adj + 21u * n : cs;
+    // per contact: 8 accumulators (the first D.acc_lds in LDS, the others in
+    // cs), 14 pose gradients (cs)
+    float *const acc_l = lds_adj + 21u * n;
+    const uint32_t n_lds = D.acc_lds;
+    auto acc_of = [&](uint32_t j) -> float * { return j < n_lds ? acc_l + 8u * j : cs + 8u * j; };
     float *pg = cs + 8ull * C;
     const float h = A.h;
     const float g[3] = {A.gx, A.gy, A.gz};
@@ -920,7 +923,7 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
         if (nc > 0) {
             const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
-            for (uint32_t j = r; j < 8u * nc; j += 64) acc[j] = 0.f;
+            for (uint32_t j = r; j < 8u * nc; j += 64) acc_of(j >> 3)[j & 7u] = 0.f;
             tm.sync();
             // phases (it, c) newest first; each lane's first contact of the
             // next phase is loaded while this one computes
@@ -945,12 +948,12 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                     const uint32_t end = co[c + 1];
                     if (cur.j < end)
                         contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
-                                          reinterpret_cast<const float *>(cur.m), acc + 8u * cur.j);
+                                          reinterpret_cast<const float *>(cur.m), acc_of(cur.j));
                     for (uint32_t j = cur.j + 64; j < end; j += 64) {  // classes of more than 64 contacts
                         PreRec x;
                         pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(it), static_cast<uint32_t>(c), r + (j - cur.j));
                         contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1, reinterpret_cast<const float *>(x.m),
-                                          acc + 8u * j);
+                                          acc_of(j));
                     }
                     tm.sync();
                     if (more) cur = nxt;
@@ -972,7 +975,8 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
                 const float pb[3] = {po[b], po[n + b], po[2 * n + b]};
                 const float qb[4] = {po[3 * n + b], po[4 * n + b], po[5 * n + b], po[6 * n + b]};
                 float hbar[7];
-                for (int k = 0; k < 7; ++k) hbar[k] = acc[8u * j + 1 + k];
+                const float *aj = acc_of(j);
+                for (int k = 0; k < 7; ++k) hbar[k] = aj[1 + k];
                 float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 const bool done = ta == kBox && tb == kBox && box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
                 if (!done) (void)contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
@@ -1589,14 +1593,16 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     DiffArgs D{};
     D.adj = *adj_initial;
     D.cs = d->d_cs;
-    // The 8 per-contact accumulators go to LDS beside the body adjoints when
-    // the largest scene's share fits in 64 KiB.
+    // The 8 per-contact accumulators of a scene's first kAccLds contacts go
+    // to LDS beside the body adjoints (a modest share, so that the reverse
+    // sweep's workgroups leave LDS for the recompute running beside them).
+    constexpr uint64_t kAccLds = 384;
     uint64_t max_c = 0;
     for (size_t k = 0; k + 1 < d->b->contact_off.size(); ++k)
         max_c = std::max<uint64_t>(max_c, d->b->contact_off[k + 1] - d->b->contact_off[k]);
-    const uint64_t lds_floats = 21ull * d->b->max_n + 8ull * max_c;
-    D.acc_lds = lds_floats * sizeof(float) <= 65536u ? 1u : 0u;
-    const size_t lds_bytes = (D.acc_lds ? lds_floats : 21ull * d->b->max_n) * sizeof(float);
+    D.acc_lds = static_cast<uint32_t>(std::min(max_c, kAccLds));
+    const size_t lds_bytes = (21ull * d->b->max_n + 8ull * D.acc_lds) * sizeof(float);
+    if (lds_bytes > 65536u) return DY_ERR_INVALID_ARG;
     // Segments newest first.  Segment k's reverse sweep (stream s) reads tape
     // k % 2 while segment k - 1 is recomputed from its checkpoint into the
     // other tape on the recompute stream; events order the hand-overs.  The
