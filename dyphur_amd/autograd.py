@@ -20,9 +20,10 @@ from . import capi
 
 
 class Rollout:
-    def __init__(self, scene_list, frames: int, checkpoint_every: int = 10):
+    def __init__(self, scene_list, frames: int, checkpoint_every: int = 10, n_substeps: int = 1, **tape):
+        """tape: tape_contacts_per_body / tape_bytes, as capi.Diff takes them."""
         self.batch = capi.Batch(scene_list)
-        self.diff = capi.Diff(self.batch, frames, checkpoint_every)
+        self.diff = capi.Diff(self.batch, frames, checkpoint_every, n_substeps=n_substeps, **tape)
         self.frames = frames
         self.init = self.batch.bodies.host_arrays(self.batch.merged)
         self.static = self.init["inv_mass"] == 0

@@ -174,7 +174,8 @@ _SIGS = {
     "dy_batch_stats": (C.c_int, [vp, vp, C.POINTER(BatchSceneStats)]),
     "dy_batch_layout_get": (C.c_int, [vp, C.POINTER(BatchLayout)]),
     "dy_batch_set_timing": (C.c_int, [vp, vp, C.c_int]),
-    "dy_diff_create": (C.c_int, [vp, u32, u32, C.POINTER(vp)]),
+    "dy_diff_create": (C.c_int, [vp, u32, u32, u32, f32, C.c_uint64, C.POINTER(vp)]),
+    "dy_diff_tape_steps": (C.c_int, [vp, C.POINTER(u32), C.POINTER(C.c_uint64)]),
     "dy_diff_destroy": (C.c_int, [vp]),
     "dy_diff_forward": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32, C.c_int]),
     "dy_diff_backward": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
@@ -852,10 +853,13 @@ class Diff:
     steps the batch keeping checkpoints; backward() turns the adjoint of the
     final state into the adjoint of the initial state and of inv_mass."""
 
-    def __init__(self, batch: Batch, max_frames: int, checkpoint_every: int = 10):
+    def __init__(self, batch: Batch, max_frames: int, checkpoint_every: int = 10, n_substeps: int = 1,
+                 tape_contacts_per_body: float = 0.0, tape_bytes: int = 0):
         self.batch = batch
         self.h = vp()
-        check(lib().dy_diff_create(batch.h, max_frames, checkpoint_every, C.byref(self.h)), "dy_diff_create")
+        self.n_substeps = n_substeps
+        check(lib().dy_diff_create(batch.h, max_frames, checkpoint_every, n_substeps, tape_contacts_per_body,
+                                   tape_bytes, C.byref(self.h)), "dy_diff_create")
         n = batch.n_bodies
         self.adj = {k: DeviceArray(n, np.float32) for k in MUTABLE_FIELDS + ("inv_mass",)}
         self.frames = 0
@@ -868,6 +872,15 @@ class Diff:
         check(lib().dy_diff_forward(self.h, b.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt, n_frames),
               "dy_diff_forward")
         self.frames = n_frames
+
+    def tape_steps(self) -> int:
+        return self.tape_layout()[0]
+
+    def tape_layout(self) -> tuple[int, int]:
+        """(substeps one tape holds, bytes of one recorded substep)."""
+        steps, nbytes = u32(0), C.c_uint64(0)
+        check(lib().dy_diff_tape_steps(self.h, C.byref(steps), C.byref(nbytes)), "dy_diff_tape_steps")
+        return steps.value, nbytes.value
 
     def _adj_view(self) -> BodyView:
         v = BodyView()

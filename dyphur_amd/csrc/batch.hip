@@ -33,8 +33,10 @@
 //      per-body masks), then I iterations sweep the classes in order, each
 //      followed by the renormalisation of every quaternion.
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <numeric>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -50,7 +52,10 @@ constexpr uint32_t kTuneShapes = 6;         // launch shapes the autotuner times
 constexpr uint32_t kBatchSerialMax = 256;   // xpbd_solver.hpp:29 kSerialThreshold
 constexpr uint32_t kBatchMaxRounds = 256;   // xpbd_solver.cpp:640
 constexpr uint32_t kNotAssigned = 0xffffffffu;
-constexpr uint32_t kBatchErrUnsupported = 1u, kBatchErrLists = 2u;
+// Scene error bits (stats word kStError, and a tape frame's header): a hull
+// or mesh shape met; an internal check failed; more contacts in a step than
+// the tape holds (the step ran, its record is incomplete).
+constexpr uint32_t kBatchErrUnsupported = 1u, kBatchErrLists = 2u, kBatchErrTape = 4u;
 
 // Timed stages: integrate + leaf boxes, pairs, narrowphase, class
 // assignment, sweep.
@@ -75,6 +80,7 @@ struct BatchArgs {
     uint64_t *stage_clk;   // kStages per scene when timed
     float *tape;           // differentiable form: per-frame records (TapeLayout)
     const uint64_t *tape_off;  // per scene, floats
+    const uint32_t *tape_cap;  // per scene: contacts a tape step holds
     int keep_input;        // 1: the final state is not written back (a recompute from a checkpoint)
     uint32_t region_floats;  // LDS floats per team
     float gx, gy, gz, h;
@@ -416,8 +422,10 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
 
     [[maybe_unused]] TapeLayout TL{};
     [[maybe_unused]] float *tp = nullptr;
+    [[maybe_unused]] uint32_t Ct = 0;  // contacts a tape step holds
     if constexpr (TAPE) {
-        TL = tape_layout(n, C, static_cast<uint32_t>(A.n_iters));
+        Ct = A.tape_cap[s];
+        TL = tape_layout(n, Ct, static_cast<uint32_t>(A.n_iters));
         tp = A.tape + A.tape_off[s];
     }
     for (uint32_t step = 0; step < n_steps; ++step) {
@@ -624,7 +632,15 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                     const uint32_t c = S.cls[k];
                     S.order[S.coff[c] + atomicAdd(&S.ccnt[c], 1u)] = k;
                 }
+                [[maybe_unused]] bool taped = false;
                 if constexpr (TAPE) {
+                    taped = nc <= Ct;
+                    if (!taped && r == 0) atomicOr(&tf[20], kBatchErrTape);
+                    // the record gathers `order`, which other wavefronts of
+                    // a workgroup team have just scattered
+                    tm.sync();
+                }
+                if (taped) {
                     uint32_t *co = reinterpret_cast<uint32_t *>(fb + TL.co);
                     for (uint32_t c = r; c <= n_cls; c += NT) co[c] = S.coff[c];
                     float *rec = fb + TL.rec;
@@ -653,9 +669,9 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                                                cv.norm_y[k], cv.norm_z[k], cv.depth[k]};
                             BodyState Ba = load_body(bv, a), Bb = load_body(bv, b);
                             float l = S.lam[k];
-                            if constexpr (TAPE) {
+                            if (taped) {
                                 float4 *pr = reinterpret_cast<float4 *>(
-                                    fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + j));
+                                    fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * Ct + j));
                                 pr[0] = make_float4(Ba.px, Ba.py, Ba.pz, Ba.qw);
                                 pr[1] = make_float4(Ba.qx, Ba.qy, Ba.qz, Ba.vx);
                                 pr[2] = make_float4(Ba.vy, Ba.vz, Ba.wx, Ba.wy);
@@ -690,10 +706,11 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         mark(5);
         if constexpr (TAPE) {
             if (r == 0) {
+                const uint32_t terr = err | (nc > Ct ? kBatchErrTape : 0u);
                 uint32_t *hdr = reinterpret_cast<uint32_t *>(fb + TL.hdr);
-                hdr[0] = err ? 0u : nc;
-                hdr[1] = err ? 0u : n_cls;
-                hdr[2] = err;
+                hdr[0] = terr ? 0u : nc;
+                hdr[1] = terr ? 0u : n_cls;
+                hdr[2] = terr;
                 hdr[3] = 0u;
             }
         }
@@ -764,10 +781,7 @@ template <uint32_t NT, bool RES>
 int launch_frames(const BatchArgs &a, uint32_t n_units, size_t lds_bytes, hipStream_t s) {
     if (n_units == 0) return DY_OK;
     if (a.tape) {
-        if constexpr (RES)
-            hipLaunchKernelGGL((batch_frames<NT, false, true, true>), dim3(n_units), dim3(NT), lds_bytes, s, a);
-        else
-            return DY_ERR_INVALID_ARG;  // the differentiable form keeps every scene in LDS
+        hipLaunchKernelGGL((batch_frames<NT, false, RES, true>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     } else if (a.stage_clk) {
         hipLaunchKernelGGL((batch_frames<NT, true, RES, false>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     } else {
@@ -786,10 +800,12 @@ constexpr uint32_t kCsStride = 22;
 
 struct DiffArgs {
     dy_body_view adj;  // adjoint of the 13 mutable fields; inv_mass holds the inverse-mass adjoint
-    float *cs;         // kCsStride per contact of capacity: 8 accumulators, then 14 pose gradients
-    uint32_t frames;   // recorded frames to reverse, newest first
-    uint32_t max_n;    // largest scene: LDS holds 21 floats per body
+    float *cs;         // kCsStride per tape contact: 8 accumulators, then 14 pose gradients
+    const uint64_t *cs_off;  // per scene, first tape contact (kCsStride floats each)
+    uint32_t frames;   // recorded steps to reverse, newest first
+    uint32_t lds_floats;  // dynamic LDS per workgroup
     uint32_t acc_lds;  // contacts whose 8 accumulators live in LDS after the body adjoints (the rest: cs)
+    uint32_t *err;     // OR of the reversed steps' header errors
 };
 
 // The tape's inputs of one contact projection, loaded one phase ahead.
@@ -875,20 +891,22 @@ __device__ __forceinline__ void contact_phase_adj(const BatchArgs &A, float h, c
     if (B0.im != 0.f) store_bar(av, b, Bb);
 }
 
-__global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
-    extern __shared__ float lds_adj[];
-    const uint32_t s = blockIdx.x;
+// The reverse sweep of one scene.  !GLB: the scene's adjoint (13 fields +
+// inverse mass) and its constants (inverse mass, inverse inertia) live in
+// LDS during the launch, beside its first contacts' accumulators; GLB (a
+// scene too large for that): they stay in global memory, and the team
+// barrier is the workgroup's, which orders global memory as well.
+template <bool GLB>
+__device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s, float *lds) {
     const uint32_t r = lane_id();
-    const WaveTeam tm{r, nullptr};
+    using Team = std::conditional_t<GLB, BlockTeam<64>, WaveTeam>;
+    const Team tm{r, nullptr};
     const uint32_t b0 = A.body_off[s], n = A.body_off[s + 1] - b0;
-    const uint32_t C = A.contact_off[s + 1] - A.contact_off[s];
     const dy_body_view gl = scene_bodies(A.bv, b0, n);
     const dy_body_view ag = scene_bodies(D.adj, b0, n);
-    // The scene's adjoint (13 fields + inverse mass) and its constants
-    // (inverse mass, inverse inertia) live in LDS during the launch.
-    dy_body_view av{}, gv = gl;
-    {
-        float *p = lds_adj;
+    dy_body_view av = ag, gv = gl;
+    if (!GLB) {
+        float *p = lds;
 #define DY_ADJ_LDS(dst, src, k)                                         \
     dst = p + (k) * n;                                                  \
     for (uint32_t i = r; i < n; i += 64) dst[i] = src[i];
@@ -905,13 +923,14 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
     }
     tm.sync();
     const uint32_t iters = static_cast<uint32_t>(A.n_iters);
+    const uint32_t C = A.tape_cap[s];  // contacts a tape step holds
     const TapeLayout TL = tape_layout(n, C, iters);
     const float *tp = A.tape + A.tape_off[s];
-    float *cs = D.cs + static_cast<uint64_t>(kCsStride) * A.contact_off[s];
-    // per contact: 8 accumulators (the first D.acc_lds in LDS, the others in
-    // cs), 14 pose gradients (cs)
-    float *const acc_l = lds_adj + 21u * n;
-    const uint32_t n_lds = D.acc_lds;
+    float *cs = D.cs + kCsStride * D.cs_off[s];
+    // per contact: 8 accumulators (the first n_lds in LDS, the others in cs),
+    // 14 pose gradients (cs)
+    float *const acc_l = lds + 21u * n;
+    const uint32_t n_lds = GLB ? 0u : D.acc_lds;
     auto acc_of = [&](uint32_t j) -> float * { return j < n_lds ? acc_l + 8u * j : cs + 8u * j; };
     float *pg = cs + 8ull * C;
     const float h = A.h;
@@ -920,6 +939,7 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
         const float *fb = tp + static_cast<uint64_t>(f) * TL.frame;
         const uint32_t *hdr = reinterpret_cast<const uint32_t *>(fb + TL.hdr);
         const uint32_t nc = hdr[0], ncls = hdr[1];
+        if (r == 0 && hdr[2] != 0u) atomicOr(D.err, hdr[2]);
         if (nc > 0) {
             const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
@@ -1018,7 +1038,25 @@ __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
         }
         tm.sync();
     }
-    for (uint32_t i = r; i < n; i += 64) store_bar(ag, i, load_bar(av, i));
+    if (!GLB)
+        for (uint32_t i = r; i < n; i += 64) store_bar(ag, i, load_bar(av, i));
+}
+
+// One 64-lane workgroup per scene.  A scene whose adjoint and accumulators
+// fit the launch's LDS is reversed by diff_backward, a larger one by
+// diff_backward_global (each kernel leaves the other's scenes alone).
+__device__ __forceinline__ bool reverse_in_lds(const BatchArgs &A, const DiffArgs &D, uint32_t s) {
+    const uint32_t n = A.body_off[s + 1] - A.body_off[s];
+    return 21u * n + 8u * D.acc_lds <= D.lds_floats;
+}
+
+__global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
+    extern __shared__ float lds_adj[];
+    if (reverse_in_lds(A, D, blockIdx.x)) reverse_scene<false>(A, D, blockIdx.x, lds_adj);
+}
+
+__global__ __launch_bounds__(64) void diff_backward_global(BatchArgs A, DiffArgs D) {
+    if (!reverse_in_lds(A, D, blockIdx.x)) reverse_scene<true>(A, D, blockIdx.x, nullptr);
 }
 
 }  // namespace dy
@@ -1169,8 +1207,9 @@ BatchArgs batch_args(dy_batch_s *b, const dy_body_view &bodies, const dy_shape_v
     return a;
 }
 
-// Every scene of the current launch shape runs LDS-resident: the condition
-// for recording a tape.
+// Every scene of the current launch shape runs LDS-resident (its state is
+// read once and written back at the end): a recompute can then read a
+// checkpoint in place.
 bool batch_tape_ok(const dy_batch_s *b) {
     const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
     return it != b->shapes.end() && it->second.glob.n_units == 0;
@@ -1178,15 +1217,16 @@ bool batch_tape_ok(const dy_batch_s *b) {
 
 int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const dy_shape_view &shapes, float gx,
                  float gy, float gz, float dt, int n_substeps, int n_frames, float *tape = nullptr,
-                 const uint64_t *tape_off = nullptr, bool keep_input = false) {
+                 const uint64_t *tape_off = nullptr, bool keep_input = false, const uint32_t *tape_cap = nullptr) {
     const auto it = b->shapes.find(std::make_pair(b->block, b->wave_max));
     if (it == b->shapes.end()) return DY_ERR_INTERNAL;
     const dy_batch_s::Shape &sh = it->second;
     BatchArgs a = batch_args(b, bodies, shapes, gx, gy, gz, dt, n_substeps, n_frames);
     a.tape = tape;
     a.tape_off = tape_off;
+    a.tape_cap = tape_cap;
     a.keep_input = keep_input ? 1 : 0;
-    if (tape && sh.glob.n_units) return DY_ERR_INVALID_ARG;
+    if (tape && !tape_cap) return DY_ERR_INVALID_ARG;
     if (keep_input && sh.glob.n_units) return DY_ERR_INVALID_ARG;
     for (int pass = 0; pass < 2; ++pass) {
         const dy_batch_s::Units &u = pass == 0 ? sh.res : sh.glob;
@@ -1457,14 +1497,23 @@ int dy_batch_layout_get(dy_batch_t b, dy_batch_layout *out) {
 // ---- differentiable form -------------------------------------------------
 struct dy_diff_s {
     dy_batch_s *b = nullptr;
-    uint32_t max_frames = 0, every = 0, frames = 0;
+    uint32_t max_frames = 0, every = 0, frames = 0, n_sub = 1;
+    uint32_t tape_steps = 0;  // substeps one tape holds (whole frames)
+    uint64_t step_bytes = 0;  // one recorded substep of every scene
     size_t n_total = 0;
-    // Two tapes of one segment each: the reverse sweep of segment k reads one
-    // while segment k - 1 is recomputed into the other on the recompute stream.
+    bool resident = true;     // every scene LDS-resident: recomputes read checkpoints in place
+    bool cap_reduced = false; // some scene's tape holds fewer contacts than its store
+    // Two tapes: the reverse sweep of one unit reads one while the next
+    // (older) unit is recomputed into the other on the recompute stream.
     float *d_tape[2] = {nullptr, nullptr};
-    uint64_t *d_tape_off = nullptr;
+    uint64_t *d_tape_off = nullptr, *d_cs_off = nullptr;
+    uint32_t *d_tape_cap = nullptr;  // per scene: contacts a tape step holds
+    uint32_t *d_err = nullptr;       // header errors of the reversed steps
+    std::vector<uint32_t> tape_cap;
     float *d_ckpt = nullptr;  // checkpoints: 13 x n_total floats each
-    float *d_cs = nullptr;    // kCsStride per contact
+    float *d_sub = nullptr;   // sub-checkpoints inside a segment longer than a tape
+    float *d_work = nullptr;  // recompute state when scenes run from global memory
+    float *d_cs = nullptr;    // kCsStride per tape contact
     hipStream_t rs = nullptr;  // recompute stream (default priority)
     hipEvent_t ev_start = nullptr, ev_tape[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
     int newest = -1;  // tape holding the last forward's newest segment, or -1
@@ -1491,9 +1540,17 @@ int copy_state(hipStream_t s, float *const dst[13], float *const src[13], size_t
     return DY_OK;
 }
 
+// A 13-field state stored contiguously (checkpoint, sub-checkpoint, work).
+dy_body_view state_view(const dy_body_view &bodies, float *base, size_t n) {
+    dy_body_view v = bodies;
+    set_mutable_fields(v, base, n);
+    return v;
+}
+
 void diff_free(dy_diff_s *d) {
     if (d->rs) (void)hipStreamSynchronize(d->rs);
-    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_ckpt, d->d_cs};
+    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_cs_off, d->d_tape_cap, d->d_err,
+                  d->d_ckpt, d->d_sub, d->d_work, d->d_cs};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     hipEvent_t evs[] = {d->ev_start, d->ev_tape[0], d->ev_tape[1], d->ev_free[0], d->ev_free[1]};
@@ -1506,32 +1563,72 @@ void diff_free(dy_diff_s *d) {
 
 extern "C" {
 
-int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, dy_diff_t *out) {
-    if (!b || !out || max_frames == 0 || checkpoint_every == 0) return DY_ERR_INVALID_ARG;
+int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, uint32_t n_substeps,
+                   float tape_contacts_per_body, uint64_t tape_bytes, dy_diff_t *out) {
+    if (!b || !out || max_frames == 0 || checkpoint_every == 0 || n_substeps == 0) return DY_ERR_INVALID_ARG;
     *out = nullptr;
     auto *d = new dy_diff_s();
     d->b = b;
     d->max_frames = max_frames;
     d->every = checkpoint_every;
+    d->n_sub = n_substeps;
     d->n_total = b->body_off.back();
-    std::vector<uint64_t> off(b->n_scenes + 1, 0ull);
-    for (uint32_t i = 0; i < b->n_scenes; ++i) {
+    d->resident = batch_tape_ok(b);
+    // Per scene: the contacts one tape step holds (the contact store's
+    // capacity, or fewer on request), and the floats of one recorded step.
+    const uint32_t ns = b->n_scenes;
+    d->tape_cap.resize(ns);
+    std::vector<uint64_t> step_floats(ns), cs_off(ns + 1, 0ull);
+    uint64_t step_total = 0;
+    for (uint32_t i = 0; i < ns; ++i) {
         const uint32_t n = b->body_off[i + 1] - b->body_off[i];
         const uint32_t c = b->contact_off[i + 1] - b->contact_off[i];
-        off[i + 1] = off[i] + static_cast<uint64_t>(checkpoint_every) * tape_layout(n, c, b->n_iters).frame;
+        uint32_t ct = c;
+        if (tape_contacts_per_body > 0.f) {
+            const double want = std::ceil(static_cast<double>(tape_contacts_per_body) * n);
+            ct = static_cast<uint32_t>(std::min<double>(c, std::max(1.0, want)));
+        }
+        d->cap_reduced = d->cap_reduced || ct < c;
+        d->tape_cap[i] = ct;
+        step_floats[i] = tape_layout(n, ct, b->n_iters).frame;
+        step_total += step_floats[i];
+        cs_off[i + 1] = cs_off[i] + ct;
     }
+    // Substeps per tape: whole frames, at most a segment, within the budget
+    // (24 GiB per tape unless given).
+    const uint64_t budget = tape_bytes ? tape_bytes : (24ull << 30);
+    const uint64_t seg_steps = static_cast<uint64_t>(checkpoint_every) * n_substeps;
+    uint64_t fit = budget / (std::max<uint64_t>(step_total, 1ull) * sizeof(float));
+    fit = fit / n_substeps * n_substeps;
+    if (fit == 0) {
+        delete d;
+        return DY_ERR_INVALID_ARG;  // one frame's record exceeds the tape budget
+    }
+    d->tape_steps = static_cast<uint32_t>(std::min(fit, seg_steps));
+    d->step_bytes = step_total * sizeof(float);
+    std::vector<uint64_t> off(ns + 1, 0ull);
+    for (uint32_t i = 0; i < ns; ++i) off[i + 1] = off[i] + d->tape_steps * step_floats[i];
     const uint32_t n_ckpt = (max_frames + checkpoint_every - 1) / checkpoint_every;
+    const uint32_t tape_frames = d->tape_steps / n_substeps;
+    const uint32_t n_subck = (checkpoint_every + tape_frames - 1) / tape_frames;
     int st = DY_OK;
     if (st == DY_OK) st = dev_alloc(d->d_tape[0], off.back());
     if (st == DY_OK) st = dev_alloc(d->d_tape[1], off.back());
     if (st == DY_OK) st = dev_alloc(d->d_tape_off, off.size());
+    if (st == DY_OK) st = dev_alloc(d->d_cs_off, cs_off.size());
+    if (st == DY_OK) st = dev_alloc(d->d_tape_cap, ns);
+    if (st == DY_OK) st = dev_alloc(d->d_err, 1);
+    if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
+    if (st == DY_OK && n_subck > 1) st = dev_alloc(d->d_sub, static_cast<size_t>(n_subck) * 13 * d->n_total);
+    if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
+    if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(cs_off.back(), 1ull));
     if (st == DY_OK && hipStreamCreateWithFlags(&d->rs, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
     hipEvent_t *evs[] = {&d->ev_start, &d->ev_tape[0], &d->ev_tape[1], &d->ev_free[0], &d->ev_free[1]};
     for (hipEvent_t *e : evs)
         if (st == DY_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) st = DY_ERR_INTERNAL;
-    if (st == DY_OK) st = dev_alloc(d->d_ckpt, static_cast<size_t>(n_ckpt) * 13 * d->n_total);
-    if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(b->contact_off.back(), 1ull));
-    if (st == DY_OK && hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    if (st == DY_OK && (hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(d->d_cs_off, cs_off.data(), cs_off.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(d->d_tape_cap, d->tape_cap.data(), ns * 4, hipMemcpyHostToDevice) != hipSuccess))
         st = DY_ERR_INTERNAL;
     if (st != DY_OK) {
         diff_free(d);
@@ -1549,16 +1646,24 @@ int dy_diff_destroy(dy_diff_t d) {
     return DY_OK;
 }
 
+int dy_diff_tape_steps(dy_diff_t d, uint32_t *steps, uint64_t *bytes_per_step) {
+    if (!d || !steps) return DY_ERR_INVALID_ARG;
+    *steps = d->tape_steps;
+    if (bytes_per_step) *bytes_per_step = d->step_bytes;
+    return DY_OK;
+}
+
 int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes, float gx,
                     float gy, float gz, float dt, int n_frames) {
     if (!d) return DY_ERR_INVALID_ARG;
-    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, 1, n_frames));
+    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, static_cast<int>(d->n_sub), n_frames));
     if (n_frames <= 0 || static_cast<uint32_t>(n_frames) > d->max_frames) return DY_ERR_INVALID_ARG;
     const hipStream_t hsm = hs(s);
     float *src[13];
     mutable_fields(*bodies, src);
     d->newest = -1;
     const uint32_t n_seg = (static_cast<uint32_t>(n_frames) + d->every - 1) / d->every;
+    const int sub = static_cast<int>(d->n_sub);
     for (uint32_t f0 = 0, k = 0; f0 < static_cast<uint32_t>(n_frames); f0 += d->every, ++k) {
         dy_body_view ck{};
         set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * d->n_total, d->n_total);
@@ -1566,12 +1671,14 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
         mutable_fields(ck, dst);
         DY_TRY_STATUS(copy_state(hsm, dst, src, d->n_total));
         const int nf = static_cast<int>(std::min<uint32_t>(d->every, static_cast<uint32_t>(n_frames) - f0));
-        // The newest segment is recorded as it runs: the backward starts
-        // from it without recomputing it.
-        const bool newest = k + 1 == n_seg && batch_tape_ok(d->b);
-        DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, 1, nf,
-                                   newest ? d->d_tape[k % 2] : nullptr, newest ? d->d_tape_off : nullptr));
-        if (newest) d->newest = static_cast<int>(k % 2);
+        // The newest segment is recorded as it runs when one tape holds it:
+        // the backward then starts from it without recomputing it.
+        const bool newest = k + 1 == n_seg && static_cast<uint32_t>(nf) * d->n_sub <= d->tape_steps;
+        const uint32_t t = 0u;  // the tape the backward's first unit reads
+        DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, sub, nf,
+                                   newest ? d->d_tape[t] : nullptr, newest ? d->d_tape_off : nullptr, false,
+                                   newest ? d->d_tape_cap : nullptr));
+        if (newest) d->newest = static_cast<int>(t);
     }
     d->frames = static_cast<uint32_t>(n_frames);
     return DY_OK;
@@ -1580,67 +1687,131 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
 int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes, float gx,
                      float gy, float gz, float dt, const dy_body_view *adj_final, dy_body_view *adj_initial) {
     if (!d || !adj_final || !adj_initial || d->frames == 0) return DY_ERR_INVALID_ARG;
-    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, 1, 1));
+    DY_TRY_STATUS(check_step_args(d->b, bodies, shapes, static_cast<int>(d->n_sub), 1));
     if (adj_final->n != d->n_total || adj_initial->n != d->n_total) return DY_ERR_INVALID_ARG;
     const hipStream_t hsm = hs(s);
     const size_t n = d->n_total;
+    const int sub = static_cast<int>(d->n_sub);
     float *fin[13], *ini[13];
     mutable_fields(*adj_final, fin);
     mutable_fields(*adj_initial, ini);
     if (fin[0] != ini[0]) DY_TRY_STATUS(copy_state(hsm, ini, fin, n));
     DY_TRY(hipMemsetAsync(adj_initial->inv_mass, 0, n * sizeof(float), hsm));
-    const uint32_t n_seg = (d->frames + d->every - 1) / d->every;
+    DY_TRY(hipMemsetAsync(d->d_err, 0, sizeof(uint32_t), hsm));
     DiffArgs D{};
     D.adj = *adj_initial;
     D.cs = d->d_cs;
-    // The 8 per-contact accumulators of a scene's first kAccLds contacts go
-    // to LDS beside the body adjoints (a modest share, so that the reverse
-    // sweep's workgroups leave LDS for the recompute running beside them).
-    constexpr uint64_t kAccLds = 384;
-    uint64_t max_c = 0;
-    for (size_t k = 0; k + 1 < d->b->contact_off.size(); ++k)
-        max_c = std::max<uint64_t>(max_c, d->b->contact_off[k + 1] - d->b->contact_off[k]);
-    D.acc_lds = static_cast<uint32_t>(std::min(max_c, kAccLds));
-    const size_t lds_bytes = (21ull * d->b->max_n + 8ull * D.acc_lds) * sizeof(float);
-    if (lds_bytes > 65536u) return DY_ERR_INVALID_ARG;
-    // Segments newest first.  Segment k's reverse sweep (stream s) reads tape
-    // k % 2 while segment k - 1 is recomputed from its checkpoint into the
-    // other tape on the recompute stream; events order the hand-overs.  The
-    // reverse sweep reads only the constant body fields (inverse mass and
-    // inertia, shape, flags) of the body view.
+    D.cs_off = d->d_cs_off;
+    D.err = d->d_err;
+    // Per scene in LDS: the adjoint and constants (21 floats per body) and
+    // the accumulators of its first 384 contacts; scenes beyond 64 KiB are
+    // reversed from global memory.
+    constexpr uint32_t kAccLds = 384;
+    uint32_t max_ct = 0;
+    for (uint32_t c : d->tape_cap) max_ct = std::max(max_ct, c);
+    D.acc_lds = std::min(max_ct, kAccLds);
+    const uint64_t want = 21ull * d->b->max_n + 8ull * D.acc_lds;
+    D.lds_floats = static_cast<uint32_t>(std::min<uint64_t>(want, 65536u / sizeof(float)));
+    const size_t lds_bytes = static_cast<size_t>(D.lds_floats) * sizeof(float);
+    bool any_global = false;
+    for (uint32_t i = 0; i < d->b->n_scenes; ++i)
+        any_global = any_global || 21ull * (d->b->body_off[i + 1] - d->b->body_off[i]) + 8ull * D.acc_lds > D.lds_floats;
+    const uint32_t tape_frames = d->tape_steps / d->n_sub;
+
+    // Units of at most one tape, newest first: segment k (checkpoint every
+    // `every` frames) in chunks of tape_frames frames.  A unit's reverse
+    // sweep (stream s) reads tape u % 2 while the next unit is recomputed
+    // into the other tape on the recompute stream; events order the
+    // hand-overs.  A segment longer than a tape is first replayed from its
+    // checkpoint, saving a sub-checkpoint per chunk.  The reverse sweep reads
+    // only the constant fields (inverse mass and inertia, shape, flags) of
+    // the body view.
+    struct Unit {
+        uint32_t seg, f0, nf;  // frames [f0, f0 + nf) of segment seg (absolute frames)
+        const float *src;      // state at f0: a checkpoint or a sub-checkpoint
+    };
     DY_TRY(hipEventRecord(d->ev_start, hsm));
     DY_TRY(hipStreamWaitEvent(d->rs, d->ev_start, 0));
-    auto recompute = [&](uint32_t k) -> int {
-        const uint32_t f0 = k * d->every;
-        const uint32_t nf = std::min(d->every, d->frames - f0);
-        const int t = static_cast<int>(k % 2);
-        DY_TRY(hipStreamWaitEvent(d->rs, d->ev_free[t], 0));  // the reverse sweep of segment k + 2 is done with it
-        // straight from checkpoint k, which the recompute reads but does not overwrite
-        dy_body_view ck = *bodies;
-        set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * n, n);
-        DY_TRY_STATUS(batch_launch(d->b, d->rs, ck, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf), d->d_tape[t],
-                                   d->d_tape_off, true));
+    const uint32_t n_seg = (d->frames + d->every - 1) / d->every;
+    uint32_t u_index = 0;  // units enqueued so far (tape = u_index % 2)
+    auto seg_units = [&](uint32_t k, std::vector<Unit> &out) -> int {
+        const uint32_t f0 = k * d->every, nf = std::min(d->every, d->frames - f0);
+        float *ck = d->d_ckpt + static_cast<size_t>(k) * 13 * n;
+        out.clear();
+        if (nf <= tape_frames) {
+            out.push_back(Unit{k, f0, nf, ck});
+            return DY_OK;
+        }
+        // replay the segment on the recompute stream, saving each chunk's start
+        DY_TRY(hipMemcpyAsync(d->d_work, ck, 13 * n * sizeof(float), hipMemcpyDeviceToDevice, d->rs));
+        const dy_body_view wv = state_view(*bodies, d->d_work, n);
+        uint32_t c = 0;
+        for (uint32_t g = 0; g < nf; g += tape_frames, ++c) {
+            float *sc = d->d_sub + static_cast<size_t>(c) * 13 * n;
+            DY_TRY(hipMemcpyAsync(sc, d->d_work, 13 * n * sizeof(float), hipMemcpyDeviceToDevice, d->rs));
+            const uint32_t cf = std::min(tape_frames, nf - g);
+            out.push_back(Unit{k, f0 + g, cf, sc});
+            if (g + cf < nf)
+                DY_TRY_STATUS(batch_launch(d->b, d->rs, wv, *shapes, gx, gy, gz, dt, sub, static_cast<int>(cf)));
+        }
+        std::reverse(out.begin(), out.end());
+        return DY_OK;
+    };
+    auto recompute = [&](const Unit &u, int t) -> int {
+        DY_TRY(hipStreamWaitEvent(d->rs, d->ev_free[t], 0));  // the reverse sweep two units back is done with it
+        if (d->resident) {
+            // straight from the saved state, which the recompute reads but does not overwrite
+            const dy_body_view sv = state_view(*bodies, const_cast<float *>(u.src), n);
+            DY_TRY_STATUS(batch_launch(d->b, d->rs, sv, *shapes, gx, gy, gz, dt, sub, static_cast<int>(u.nf),
+                                       d->d_tape[t], d->d_tape_off, true, d->d_tape_cap));
+        } else {
+            DY_TRY(hipMemcpyAsync(d->d_work, u.src, 13 * n * sizeof(float), hipMemcpyDeviceToDevice, d->rs));
+            const dy_body_view wv = state_view(*bodies, d->d_work, n);
+            DY_TRY_STATUS(batch_launch(d->b, d->rs, wv, *shapes, gx, gy, gz, dt, sub, static_cast<int>(u.nf),
+                                       d->d_tape[t], d->d_tape_off, false, d->d_tape_cap));
+        }
         DY_TRY(hipEventRecord(d->ev_tape[t], d->rs));
         DY_TRY(hipStreamWaitEvent(hsm, d->ev_tape[t], 0));
         return DY_OK;
     };
-    const uint32_t last = n_seg - 1;
-    if (d->newest != static_cast<int>(last % 2)) DY_TRY_STATUS(recompute(last));
-    for (int k = static_cast<int>(last); k >= 0; --k) {
-        const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
-        const uint32_t nf = std::min(d->every, d->frames - f0);
-        const int t = k % 2;
-        BatchArgs a = batch_args(d->b, *bodies, *shapes, gx, gy, gz, dt, 1, static_cast<int>(nf));
+    auto reverse = [&](const Unit &u, int t) -> int {
+        BatchArgs a = batch_args(d->b, *bodies, *shapes, gx, gy, gz, dt, sub, static_cast<int>(u.nf));
         a.tape = d->d_tape[t];
         a.tape_off = d->d_tape_off;
-        D.frames = nf;
-        D.max_n = d->b->max_n;
+        a.tape_cap = d->d_tape_cap;
+        D.frames = u.nf * d->n_sub;
         hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), lds_bytes, hsm, a, D);
         DY_LAUNCH_CHECK();
+        if (any_global) {
+            hipLaunchKernelGGL(diff_backward_global, dim3(d->b->n_scenes), dim3(64), 0, hsm, a, D);
+            DY_LAUNCH_CHECK();
+        }
         DY_TRY(hipEventRecord(d->ev_free[t], hsm));
-        if (k > 0) DY_TRY_STATUS(recompute(static_cast<uint32_t>(k - 1)));
+        return DY_OK;
+    };
+    std::vector<Unit> units;
+    bool first = true;
+    for (int k = static_cast<int>(n_seg) - 1; k >= 0; --k) {
+        DY_TRY_STATUS(seg_units(static_cast<uint32_t>(k), units));
+        for (const Unit &u : units) {
+            const int t = static_cast<int>(u_index % 2u);
+            // the forward taped the newest segment when one tape held it
+            const bool taped = first && d->newest == t && units.size() == 1;
+            if (!taped) DY_TRY_STATUS(recompute(u, t));
+            DY_TRY_STATUS(reverse(u, t));
+            first = false;
+            ++u_index;
+        }
     }
-    d->newest = -1;  // the tapes now hold older segments
+    d->newest = -1;  // the tapes now hold older units
+    if (d->cap_reduced) {
+        // a step with more contacts than a reduced tape holds cannot be reversed
+        uint32_t err = 0;
+        DY_TRY(hipMemcpyAsync(&err, d->d_err, sizeof(err), hipMemcpyDeviceToHost, hsm));
+        DY_TRY(hipStreamSynchronize(hsm));
+        if (err & kBatchErrTape) return DY_ERR_INVALID_ARG;
+        if (err) return DY_ERR_INTERNAL;
+    }
     return DY_OK;
 }
 

@@ -19,6 +19,9 @@ Scenes:
     base and two revolute links with PD motors; the golden articulation hash).
   * ``joint_chains``: chains of boxes and spheres hanging from static bases,
     every joint type with seeded limits, motors and compliance, over a ground.
+  * ``diff_at_scale``: the batched + differentiable configuration at scale
+    (lognormal body counts up to 1 024, config 2's mix at half scale,
+    4 substeps per frame).
 
 Randomness: PCG32 (XSH-RR, O'Neill 2014) seeded through splitmix64, so a seed
 gives the same scene on every machine with this numpy.
@@ -422,6 +425,31 @@ def diff_stack(rng: Pcg32, n_side: int = 4, half: float = 0.2, pitch: float = 0.
 def diff_batch(n_scenes: int = 256, seed: int = 20260004) -> list[Scene]:
     rng = Pcg32(seed)
     return [diff_stack(rng) for _ in range(n_scenes)]
+
+
+# ---------------------------------------------------------------------------
+# Batched + differentiable at scale: config 2's mix of boxes and spheres at
+# half scale per scene, lognormal body counts, 4 substeps per frame.
+# ---------------------------------------------------------------------------
+def diff_mixed(n_dyn: int, rng: Pcg32, scale: float = 0.5, extent=(30.0, 40.0, 30.0), frames: int = 240,
+               n_substeps: int = 4, target_sigma: float = 0.5) -> Scene:
+    """random_mixed(n_dyn) at `scale` (its seed drawn from rng), with
+    meta["target"] = x0 + N(0, target_sigma) for the dynamic bodies."""
+    sc = random_mixed(n_dyn, seed=int(rng.next_u32()), scale=scale, extent=extent, frames=frames)
+    sc.n_substeps = n_substeps
+    tgt = sc.pos[:n_dyn].astype(np.float64).copy()
+    for k in range(n_dyn):
+        for a in range(3):
+            tgt[k, a] += target_sigma * rng.normal()
+    sc.meta["target"] = tgt
+    return sc
+
+
+def diff_at_scale(n_scenes: int = 1024, seed: int = 20260005, median: float = 128.0, sigma: float = 0.75,
+                  lo: int = 16, hi: int = 1024) -> list[Scene]:
+    rng = Pcg32(seed)
+    counts = lognormal_counts(n_scenes, median, sigma, lo, hi, rng)
+    return [diff_mixed(int(c), rng) for c in counts]
 
 
 def concat(scenes: list[Scene], name: str = "batch") -> Scene:

@@ -380,20 +380,30 @@ int dy_batch_get_timing(dy_batch_t b, dy_stream_t s, uint64_t *ticks, double *ti
 
 /* ---- differentiable batched form ------------------------------------------
  * No reference counterpart.  Reverse-mode derivatives of a batch's frames
- * (one substep per frame) with respect to the initial state and the
+ * (n_substeps substeps each) with respect to the initial state and the
  * inverse masses.  Forward: the batch's frames, the mutable state saved every
  * checkpoint_every frames.  Backward: per checkpoint segment, newest first,
- * the segment's frames are recomputed with a record of every intermediate
- * the adjoints need, then reversed: hand-written adjoints of the integrator,
- * the contact projection and the quaternion renormalisation, and the contact
- * geometry's Jacobian with respect to the two poses.  The contact set, pair
- * list and solve order are those of the forward (piecewise-smooth
- * derivative).  Every scene must keep its state in LDS (at most 585 bodies).
- * Statistics of the batch count the recomputed frames too. */
+ * the segment's substeps are recomputed with a record (the tape) of every
+ * intermediate the adjoints need, then reversed: hand-written adjoints of
+ * the integrator, the contact projection and the quaternion
+ * renormalisation, and the contact geometry's Jacobian with respect to the
+ * two poses.  The contact set, pair list and solve order are those of the
+ * forward (piecewise-smooth derivative).  A segment longer than a tape is
+ * replayed once to save a state per tape-sized chunk, then recomputed and
+ * reversed chunk by chunk.  Statistics of the batch count the recomputed
+ * substeps too. */
 typedef struct dy_diff_s *dy_diff_t;
-/* Asynchronous allocations are done here (blocking). */
-int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, dy_diff_t *out);
+/* A tape step holds, per scene, tape_contacts_per_body x bodies contacts
+ * (<= 0: the scene's contact capacity); a step with more contacts makes the
+ * backward fail with DY_ERR_INVALID_ARG.  Each of the two tapes takes at most
+ * tape_bytes (0: 24 GiB), in whole frames; DY_ERR_INVALID_ARG if one frame
+ * does not fit.  Allocations are done here (blocking). */
+int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, uint32_t n_substeps,
+                   float tape_contacts_per_body, uint64_t tape_bytes, dy_diff_t *out);
 int dy_diff_destroy(dy_diff_t d);
+/* Substeps one tape holds, and the bytes one recorded substep of every scene
+ * takes (may be NULL). */
+int dy_diff_tape_steps(dy_diff_t d, uint32_t *steps, uint64_t *bytes_per_step);
 /* n_frames (<= max_frames) frames of every scene from the state in bodies,
  * which ends in the final state.  Asynchronous. */
 int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
@@ -402,7 +412,8 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
  * state (its 13 mutable arrays: pos, rot, vel, ang); adj_initial receives the
  * adjoint of the initial state in the same 13 arrays and the adjoint of the
  * inverse masses in its inv_mass array (the two views may share arrays).
- * Other fields of both views are unused.  Asynchronous. */
+ * Other fields of both views are unused.  Asynchronous, except with a
+ * reduced tape (it then waits to report an overfull step). */
 int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, const dy_shape_view *shapes,
                      float gx, float gy, float gz, float dt, const dy_body_view *adj_final,
                      dy_body_view *adj_initial);

@@ -154,3 +154,114 @@ def test_torch_autograd_rollout_matches_reverse_sweep(gpu, hm):
     want, _ = _host_directional(hm, sc, frames, np.vstack([u_v, np.zeros((1, 3))]), u_im, hm.ha_sim_dir_f)
     got = float(np.sum(vel.grad.numpy()[:nd] * u_v) + np.sum(inv.grad.numpy()[:sc.n] * u_im))
     assert abs(got - want) <= 1e-3 * abs(want), (got, want)
+
+
+# ---- at scale: substeps, a scene reversed from global memory, tape chunks ----
+def _mixed_batch(frames):
+    """Three scenes of config 5's kind (mixed boxes and spheres at half
+    scale), packed densely so contacts start at once; the 700-body scene is
+    too large to keep in LDS, so it is recorded and reversed from global
+    memory.  2 substeps per frame."""
+    rng = scenes.Pcg32(20260005)
+    return [scenes.diff_mixed(n, rng, extent=(7.0, 7.0, 7.0), frames=frames, n_substeps=2) for n in (40, 700, 120)]
+
+
+def _grads(gpu, scs, frames, every, **kw):
+    b = gpu.Batch(scs)
+    d = gpu.Diff(b, frames, every, n_substeps=scs[0].n_substeps, **kw)
+    d.forward(frames)
+    st = b.state()
+    n = b.n_bodies
+    adj = {k: np.zeros(n, np.float32) for k in ("pos_x", "pos_y", "pos_z")}
+    for i, sc in enumerate(scs):
+        sl = b.scene_slice(i)
+        nd = sc.n - 1
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+            x = st[name][sl][:nd].astype(np.float64)
+            adj[name][sl.start:sl.start + nd] = 2.0 * (x - sc.meta["target"][:, k])
+    return b, d, st, d.backward(adj)
+
+
+def test_diff_substeps_and_global_scene_match_forward_mode(gpu, hm):
+    """Gradients through 2 substeps per frame, with the 700-body scene's
+    record and reverse sweep in global memory, against the host float
+    forward mode over the same substeps (dt / 2, 2 x frames), whose
+    trajectory equals the kernels' bit for bit."""
+    frames, every = 6, 3
+    scs = _mixed_batch(frames)
+    print(f"[diff] mixed scenes {[sc.n for sc in scs]}, {frames} frames x 2 substeps", flush=True)
+    b, d, st, g = _grads(gpu, scs, frames, every)
+    rng = np.random.default_rng(12)
+    for i, sc in enumerate(scs):
+        sl = b.scene_slice(i)
+        nd = sc.n - 1
+        u_v = rng.normal(size=(nd, 3))
+        u_im = np.concatenate([rng.normal(size=nd) * 0.2, [0.0]])
+        gv = np.stack([g["vel_x"][sl][:nd], g["vel_y"][sl][:nd], g["vel_z"][sl][:nd]], 1).astype(np.float64)
+        gpu_d = float(np.sum(gv * u_v) + np.sum(g["inv_mass"][sl].astype(np.float64) * u_im))
+        uv = np.vstack([u_v, np.zeros((1, 3))])
+        sub = _substep_scene(sc)
+        want, f32_st = _host_directional(hm, sub, frames * sc.n_substeps, uv, u_im, hm.ha_sim_dir_f)
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z")):
+            assert np.array_equal(bits(f32_st[k].astype(np.float32)), bits(st[name][sl])), (i, name)
+        err = abs(gpu_d - want) / max(abs(want), 1e-12)
+        print(f"[diff] scene {i} ({sc.n} bodies): gpu {gpu_d:.6e} f32-fwd {want:.6e} ({err:.1e})", flush=True)
+        assert err <= 1e-3, (i, gpu_d, want)
+
+
+def test_diff_workgroup_and_wavefront_teams_give_the_same_bits(gpu):
+    """A 300-body scene stepped by a 256-thread workgroup team and by a
+    single-wavefront team records tapes that differ only in the order of
+    contacts inside a class, which the reverse sweep does not see: the
+    gradients agree bit for bit (a workgroup team must finish scattering
+    the class order before the record gathers it)."""
+    frames, every = 4, 2
+    rng = scenes.Pcg32(20260005)
+    scs = [scenes.diff_mixed(300, rng, extent=(7.0, 7.0, 7.0), frames=frames, n_substeps=2)]
+    got = []
+    for shape in ((256, 128), (64, 128)):
+        b = gpu.Batch(scs)
+        b.set_shape(*shape)
+        d = gpu.Diff(b, frames, every, n_substeps=2)
+        d.forward(frames)
+        st = b.state()
+        nd = scs[0].n - 1
+        adj = {k: np.zeros(b.n_bodies, np.float32) for k in ("pos_x", "pos_y", "pos_z")}
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+            adj[name][:nd] = 2.0 * (st[name][:nd].astype(np.float64) - scs[0].meta["target"][:, k])
+        got.append(d.backward(adj))
+    for k in got[0]:
+        assert np.array_equal(bits(got[0][k]), bits(got[1][k])), k
+
+
+def _substep_scene(sc):
+    """The same scene stepped at the substep: dt / n_substeps per frame."""
+    import copy
+    s2 = copy.copy(sc)
+    s2.dt = sc.dt / sc.n_substeps
+    s2.n_substeps = 1
+    return s2
+
+
+def test_diff_tape_chunks_give_the_same_bits(gpu):
+    """A tape of one frame (segments replayed into sub-checkpoints, then
+    recomputed and reversed frame by frame) gives the one-tape-per-segment
+    gradients bit for bit."""
+    frames, every = 6, 3
+    scs = _mixed_batch(frames)
+    _, d1, _, g1 = _grads(gpu, scs, frames, every)
+    steps, step_bytes = d1.tape_layout()
+    assert steps == every * 2
+    _, d2, _, g2 = _grads(gpu, scs, frames, every, tape_bytes=2 * step_bytes + 64)
+    assert d2.tape_steps() == 2  # one frame of 2 substeps per tape
+    for k in g1:
+        assert np.array_equal(bits(g1[k]), bits(g2[k])), k
+
+
+def test_diff_overfull_reduced_tape_is_an_error(gpu):
+    """A tape that holds fewer contacts than a step produced: the backward
+    reports it instead of returning a wrong gradient."""
+    scs = _mixed_batch(4)
+    with pytest.raises(gpu.DyError) as e:
+        _grads(gpu, scs, 4, 2, tape_contacts_per_body=0.01)
+    assert e.value.status == -1
