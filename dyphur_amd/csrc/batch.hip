@@ -805,6 +805,7 @@ struct DiffArgs {
     uint32_t frames;   // recorded steps to reverse, newest first
     uint32_t lds_floats;  // dynamic LDS per workgroup
     uint32_t acc_lds;  // contacts whose 8 accumulators live in LDS after the body adjoints (the rest: cs)
+    const uint32_t *scenes;  // the scenes this launch reverses, one per workgroup
     uint32_t *err;     // OR of the reversed steps' header errors
 };
 
@@ -891,16 +892,16 @@ __device__ __forceinline__ void contact_phase_adj(const BatchArgs &A, float h, c
     if (B0.im != 0.f) store_bar(av, b, Bb);
 }
 
-// The reverse sweep of one scene.  !GLB: the scene's adjoint (13 fields +
+// The reverse sweep of one scene by a team (a wavefront, or a 256-thread
+// workgroup for a larger scene).  !GLB: the scene's adjoint (13 fields +
 // inverse mass) and its constants (inverse mass, inverse inertia) live in
 // LDS during the launch, beside its first contacts' accumulators; GLB (a
 // scene too large for that): they stay in global memory, and the team
 // barrier is the workgroup's, which orders global memory as well.
-template <bool GLB>
-__device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s, float *lds) {
-    const uint32_t r = lane_id();
-    using Team = std::conditional_t<GLB, BlockTeam<64>, WaveTeam>;
-    const Team tm{r, nullptr};
+template <class Team, bool GLB>
+__device__ void reverse_scene(const Team &tm, const BatchArgs &A, const DiffArgs &D, uint32_t s, float *lds) {
+    constexpr uint32_t NT = Team::kSize;
+    const uint32_t r = tm.rank;
     const uint32_t b0 = A.body_off[s], n = A.body_off[s + 1] - b0;
     const dy_body_view gl = scene_bodies(A.bv, b0, n);
     const dy_body_view ag = scene_bodies(D.adj, b0, n);
@@ -909,7 +910,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
         float *p = lds;
 #define DY_ADJ_LDS(dst, src, k)                                         \
     dst = p + (k) * n;                                                  \
-    for (uint32_t i = r; i < n; i += 64) dst[i] = src[i];
+    for (uint32_t i = r; i < n; i += NT) dst[i] = src[i];
         DY_ADJ_LDS(av.pos_x, ag.pos_x, 0) DY_ADJ_LDS(av.pos_y, ag.pos_y, 1) DY_ADJ_LDS(av.pos_z, ag.pos_z, 2)
         DY_ADJ_LDS(av.rot_w, ag.rot_w, 3) DY_ADJ_LDS(av.rot_x, ag.rot_x, 4) DY_ADJ_LDS(av.rot_y, ag.rot_y, 5)
         DY_ADJ_LDS(av.rot_z, ag.rot_z, 6)
@@ -943,7 +944,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
         if (nc > 0) {
             const uint32_t *co = reinterpret_cast<const uint32_t *>(fb + TL.co);
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
-            for (uint32_t j = r; j < 8u * nc; j += 64) acc_of(j >> 3)[j & 7u] = 0.f;
+            for (uint32_t j = r; j < 8u * nc; j += NT) acc_of(j >> 3)[j & 7u] = 0.f;
             tm.sync();
             // phases (it, c) newest first; each lane's first contact of the
             // next phase is loaded while this one computes
@@ -952,7 +953,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
             if (iters > 0) pre_fetch(cur, fb, TL, co, C, iters - 1u, ncls - 1u, r);
             for (int it = static_cast<int>(iters) - 1; it >= 0; --it) {
                 const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
-                for (uint32_t i = r; i < n; i += 64) {
+                for (uint32_t i = r; i < n; i += NT) {
                     if (gv.inv_mass[i] == 0.f) continue;
                     const float q[4] = {qr[i], qr[n + i], qr[2 * n + i], qr[3 * n + i]};
                     float qb[4] = {av.rot_w[i], av.rot_x[i], av.rot_y[i], av.rot_z[i]};
@@ -969,7 +970,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
                     if (cur.j < end)
                         contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
                                           reinterpret_cast<const float *>(cur.m), acc_of(cur.j));
-                    for (uint32_t j = cur.j + 64; j < end; j += 64) {  // classes of more than 64 contacts
+                    for (uint32_t j = cur.j + NT; j < end; j += NT) {  // classes larger than the team
                         PreRec x;
                         pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(it), static_cast<uint32_t>(c), r + (j - cur.j));
                         contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1, reinterpret_cast<const float *>(x.m),
@@ -984,7 +985,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
             // dual-number replay for the other kinds), then added to the
             // bodies class by class
             const float *po = fb + TL.po;
-            for (uint32_t j = r; j < nc; j += 64) {
+            for (uint32_t j = r; j < nc; j += NT) {
                 const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
                 const uint32_t sa = gl.shape[a], sb = gl.shape[b];
                 const uint32_t ta = A.sv.type[sa], tb = A.sv.type[sb];
@@ -1008,7 +1009,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
             tm.sync();
             for (uint32_t c = 0; c < ncls; ++c) {
                 const uint32_t end = co[c + 1];
-                for (uint32_t j = co[c] + r; j < end; j += 64) {
+                for (uint32_t j = co[c] + r; j < end; j += NT) {
                     const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
                     const float *ga = pg + 14u * j, *gb2 = pg + 14u * j + 7;
                     if (gv.inv_mass[a] != 0.f) {
@@ -1025,7 +1026,7 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
         }
         // integrator substep
         const float *st0 = fb + TL.st;
-        for (uint32_t i = r; i < n; i += 64) {
+        for (uint32_t i = r; i < n; i += NT) {
             if (gv.inv_mass[i] == 0.f || (gv.flags[i] & kFlagSleeping)) continue;
             BodyT<float> k0;
             for (int q = 0; q < 3; ++q) {
@@ -1039,24 +1040,34 @@ __device__ void reverse_scene(const BatchArgs &A, const DiffArgs &D, uint32_t s,
         tm.sync();
     }
     if (!GLB)
-        for (uint32_t i = r; i < n; i += 64) store_bar(ag, i, load_bar(av, i));
+        for (uint32_t i = r; i < n; i += NT) store_bar(ag, i, load_bar(av, i));
 }
 
-// One 64-lane workgroup per scene.  A scene whose adjoint and accumulators
-// fit the launch's LDS is reversed by diff_backward, a larger one by
-// diff_backward_global (each kernel leaves the other's scenes alone).
-__device__ __forceinline__ bool reverse_in_lds(const BatchArgs &A, const DiffArgs &D, uint32_t s) {
-    const uint32_t n = A.body_off[s + 1] - A.body_off[s];
-    return 21u * n + 8u * D.acc_lds <= D.lds_floats;
-}
+// Reverse kernels, one workgroup per scene of D.scenes.  diff_backward: a
+// wavefront per small scene, its adjoint in LDS.  diff_backward_block: a
+// 256-thread workgroup per larger scene whose adjoint (21 n floats) and
+// first accumulators fit the launch's LDS.  diff_backward_global: the same
+// for the scenes beyond that, their adjoint in global memory.  The launches
+// cover disjoint scenes and run concurrently (separate kernels: one holding
+// both 256-thread variants spills three times as much).
+constexpr uint32_t kRevBlock = 256;
+constexpr uint32_t kRevWaveMax = 128;  // bodies: larger scenes get a workgroup
 
 __global__ __launch_bounds__(64) void diff_backward(BatchArgs A, DiffArgs D) {
     extern __shared__ float lds_adj[];
-    if (reverse_in_lds(A, D, blockIdx.x)) reverse_scene<false>(A, D, blockIdx.x, lds_adj);
+    const WaveTeam tm{threadIdx.x, nullptr};
+    reverse_scene<WaveTeam, false>(tm, A, D, D.scenes[blockIdx.x], lds_adj);
 }
 
-__global__ __launch_bounds__(64) void diff_backward_global(BatchArgs A, DiffArgs D) {
-    if (!reverse_in_lds(A, D, blockIdx.x)) reverse_scene<true>(A, D, blockIdx.x, nullptr);
+__global__ __launch_bounds__(kRevBlock) void diff_backward_block(BatchArgs A, DiffArgs D) {
+    extern __shared__ float lds_adj[];
+    const BlockTeam<kRevBlock> tm{threadIdx.x, nullptr};
+    reverse_scene<BlockTeam<kRevBlock>, false>(tm, A, D, D.scenes[blockIdx.x], lds_adj);
+}
+
+__global__ __launch_bounds__(kRevBlock) void diff_backward_global(BatchArgs A, DiffArgs D) {
+    const BlockTeam<kRevBlock> tm{threadIdx.x, nullptr};
+    reverse_scene<BlockTeam<kRevBlock>, true>(tm, A, D, D.scenes[blockIdx.x], nullptr);
 }
 
 }  // namespace dy
@@ -1515,7 +1526,14 @@ struct dy_diff_s {
     float *d_work = nullptr;  // recompute state when scenes run from global memory
     float *d_cs = nullptr;    // kCsStride per tape contact
     hipStream_t rs = nullptr;  // recompute stream (default priority)
+    // Reverse launches (wavefront, workgroup in LDS, workgroup in global
+    // memory): scenes of each (largest first, concatenated), LDS sizing, and
+    // the streams beside the caller's that run the second and third.
+    uint32_t *d_rev_scenes = nullptr;
+    uint32_t n_rev[3] = {0, 0, 0}, acc_rev[3] = {0, 0, 0}, lds_rev[3] = {0, 0, 0};
+    hipStream_t rev_st[2] = {nullptr, nullptr};  // default priority
     hipEvent_t ev_start = nullptr, ev_tape[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
     int newest = -1;  // tape holding the last forward's newest segment, or -1
 };
 
@@ -1549,14 +1567,19 @@ dy_body_view state_view(const dy_body_view &bodies, float *base, size_t n) {
 
 void diff_free(dy_diff_s *d) {
     if (d->rs) (void)hipStreamSynchronize(d->rs);
+    for (hipStream_t x : d->rev_st)
+        if (x) (void)hipStreamSynchronize(x);
     void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_cs_off, d->d_tape_cap, d->d_err,
-                  d->d_ckpt, d->d_sub, d->d_work, d->d_cs};
+                  d->d_ckpt, d->d_sub, d->d_work, d->d_cs, d->d_rev_scenes};
     for (void *p : ps)
         if (p) (void)hipFree(p);
-    hipEvent_t evs[] = {d->ev_start, d->ev_tape[0], d->ev_tape[1], d->ev_free[0], d->ev_free[1]};
+    hipEvent_t evs[] = {d->ev_start, d->ev_tape[0], d->ev_tape[1], d->ev_free[0], d->ev_free[1],
+                        d->ev_fork,  d->ev_join[0], d->ev_join[1]};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
     if (d->rs) (void)hipStreamDestroy(d->rs);
+    for (hipStream_t x : d->rev_st)
+        if (x) (void)hipStreamDestroy(x);
 }
 
 }  // namespace
@@ -1622,8 +1645,40 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     if (st == DY_OK && n_subck > 1) st = dev_alloc(d->d_sub, static_cast<size_t>(n_subck) * 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_work, 13 * d->n_total);
     if (st == DY_OK) st = dev_alloc(d->d_cs, kCsStride * std::max<uint64_t>(cs_off.back(), 1ull));
+    // Reverse launches: scenes of at most kRevWaveMax bodies one per
+    // wavefront, the others one per workgroup, in LDS where their adjoint and
+    // constants (21 floats per body) and up to 384 contacts' accumulators fit
+    // 64 KiB, else in global memory; each list largest first.
+    constexpr uint32_t kAccLds = 384, kLdsFloats = 65536u / sizeof(float);
+    std::vector<uint32_t> rev[3];
+    uint32_t n_max[3] = {0, 0, 0}, c_max[3] = {0, 0, 0};
+    auto nb = [&](uint32_t i) { return b->body_off[i + 1] - b->body_off[i]; };
+    uint32_t acc_big = 0;  // the workgroup launch's LDS accumulators (one count for all its scenes)
+    for (uint32_t i = 0; i < ns; ++i)
+        if (nb(i) > kRevWaveMax) acc_big = std::max(acc_big, std::min(d->tape_cap[i], kAccLds));
+    for (uint32_t i = 0; i < ns; ++i) {
+        const int k = nb(i) <= kRevWaveMax ? 0 : (21ull * nb(i) + 8ull * acc_big <= kLdsFloats ? 1 : 2);
+        rev[k].push_back(i);
+        n_max[k] = std::max(n_max[k], nb(i));
+        c_max[k] = std::max(c_max[k], d->tape_cap[i]);
+    }
+    std::vector<uint32_t> all;
+    for (int k = 0; k < 3; ++k) {
+        std::stable_sort(rev[k].begin(), rev[k].end(), [&](uint32_t x, uint32_t y) { return nb(x) > nb(y); });
+        all.insert(all.end(), rev[k].begin(), rev[k].end());
+        d->n_rev[k] = static_cast<uint32_t>(rev[k].size());
+        // every scene of launch k fits 21 n_max[k] + 8 acc_rev[k] floats (<= 64 KiB)
+        d->acc_rev[k] = k == 0 ? std::min(c_max[0], kAccLds) : (k == 1 ? acc_big : 0u);
+        d->lds_rev[k] = k == 2 ? 0u : 21u * n_max[k] + 8u * d->acc_rev[k];
+    }
+    if (st == DY_OK) st = dev_alloc(d->d_rev_scenes, all.size());
+    if (st == DY_OK && hipMemcpy(d->d_rev_scenes, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        st = DY_ERR_INTERNAL;
     if (st == DY_OK && hipStreamCreateWithFlags(&d->rs, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
-    hipEvent_t *evs[] = {&d->ev_start, &d->ev_tape[0], &d->ev_tape[1], &d->ev_free[0], &d->ev_free[1]};
+    for (hipStream_t &x : d->rev_st)
+        if (st == DY_OK && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
+    hipEvent_t *evs[] = {&d->ev_start, &d->ev_tape[0], &d->ev_tape[1], &d->ev_free[0], &d->ev_free[1],
+                         &d->ev_fork,  &d->ev_join[0], &d->ev_join[1]};
     for (hipEvent_t *e : evs)
         if (st == DY_OK && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) st = DY_ERR_INTERNAL;
     if (st == DY_OK && (hipMemcpy(d->d_tape_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1703,19 +1758,12 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     D.cs = d->d_cs;
     D.cs_off = d->d_cs_off;
     D.err = d->d_err;
-    // Per scene in LDS: the adjoint and constants (21 floats per body) and
-    // the accumulators of its first 384 contacts; scenes beyond 64 KiB are
-    // reversed from global memory.
-    constexpr uint32_t kAccLds = 384;
-    uint32_t max_ct = 0;
-    for (uint32_t c : d->tape_cap) max_ct = std::max(max_ct, c);
-    D.acc_lds = std::min(max_ct, kAccLds);
-    const uint64_t want = 21ull * d->b->max_n + 8ull * D.acc_lds;
-    D.lds_floats = static_cast<uint32_t>(std::min<uint64_t>(want, 65536u / sizeof(float)));
-    const size_t lds_bytes = static_cast<size_t>(D.lds_floats) * sizeof(float);
-    bool any_global = false;
-    for (uint32_t i = 0; i < d->b->n_scenes; ++i)
-        any_global = any_global || 21ull * (d->b->body_off[i + 1] - d->b->body_off[i]) + 8ull * D.acc_lds > D.lds_floats;
+    DiffArgs Dk[3] = {D, D, D};
+    for (int k = 0, o = 0; k < 3; o += static_cast<int>(d->n_rev[k]), ++k) {
+        Dk[k].scenes = d->d_rev_scenes + o;
+        Dk[k].acc_lds = d->acc_rev[k];
+        Dk[k].lds_floats = d->lds_rev[k];
+    }
     const uint32_t tape_frames = d->tape_steps / d->n_sub;
 
     // Units of at most one tape, newest first: segment k (checkpoint every
@@ -1779,12 +1827,30 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         a.tape = d->d_tape[t];
         a.tape_off = d->d_tape_off;
         a.tape_cap = d->d_tape_cap;
-        D.frames = u.nf * d->n_sub;
-        hipLaunchKernelGGL(diff_backward, dim3(d->b->n_scenes), dim3(64), lds_bytes, hsm, a, D);
-        DY_LAUNCH_CHECK();
-        if (any_global) {
-            hipLaunchKernelGGL(diff_backward_global, dim3(d->b->n_scenes), dim3(64), 0, hsm, a, D);
-            DY_LAUNCH_CHECK();
+        // the non-empty launches: the first on the caller's stream, the
+        // others on the side streams, forked from and joined back into it
+        const void *kern[3] = {reinterpret_cast<const void *>(&diff_backward),
+                               reinterpret_cast<const void *>(&diff_backward_block),
+                               reinterpret_cast<const void *>(&diff_backward_global)};
+        const uint32_t threads[3] = {64, kRevBlock, kRevBlock};
+        int n_launch = 0;
+        for (int k = 0; k < 3; ++k) {
+            if (d->n_rev[k] == 0) continue;
+            Dk[k].frames = u.nf * d->n_sub;
+            hipStream_t st = hsm;
+            if (n_launch > 0) {
+                st = d->rev_st[n_launch - 1];
+                if (n_launch == 1) DY_TRY(hipEventRecord(d->ev_fork, hsm));
+                DY_TRY(hipStreamWaitEvent(st, d->ev_fork, 0));
+            }
+            void *args[] = {&a, &Dk[k]};
+            DY_TRY(hipLaunchKernel(kern[k], dim3(d->n_rev[k]), dim3(threads[k]), args,
+                                   static_cast<size_t>(d->lds_rev[k]) * sizeof(float), st));
+            ++n_launch;
+        }
+        for (int j = 0; j + 1 < n_launch; ++j) {
+            DY_TRY(hipEventRecord(d->ev_join[j], d->rev_st[j]));
+            DY_TRY(hipStreamWaitEvent(hsm, d->ev_join[j], 0));
         }
         DY_TRY(hipEventRecord(d->ev_free[t], hsm));
         return DY_OK;
