@@ -311,7 +311,7 @@ def articulated(capi, scenes, frames: int, check_frames: int = 20) -> dict:
 
 
 def differentiable(capi, scenes, frames: int = 120, every: int = 10) -> dict:
-    """Config 4: 256 thrown 4x4x4 box stacks, loss sum |x_T - x*|^2.
+    """256 thrown 4x4x4 box stacks, loss sum |x_T - x*|^2.
     Forward alone (one launch of all frames) against forward with
     checkpoints + reverse sweep (recompute each checkpoint segment with a
     record, then the adjoints), both timed with HIP events on the stream."""

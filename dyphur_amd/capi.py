@@ -166,6 +166,7 @@ _SIGS = {
     "dy_batch_destroy": (C.c_int, [vp]),
     "dy_batch_set_force_serial": (C.c_int, [vp, C.c_int]),
     "dy_batch_set_shape": (C.c_int, [vp, u32, u32]),
+    "dy_batch_set_lds_limit": (C.c_int, [vp, u32]),
     "dy_batch_get_shape": (C.c_int, [vp, C.POINTER(u32), C.POINTER(u32)]),
     "dy_batch_step": (C.c_int, [vp, vp, C.POINTER(BodyView), C.POINTER(ShapeView), f32, f32, f32, f32,
                                 C.c_int, C.c_int]),
@@ -768,6 +769,11 @@ class Batch:
 
     def set_shape(self, threads: int, wave_scene_max: int = 128):
         check(lib().dy_batch_set_shape(self.h, threads, wave_scene_max), "dy_batch_set_shape")
+
+    def set_lds_limit(self, nbytes: int):
+        """LDS one workgroup may take (0: the device's maximum); beyond it a
+        scene's state stays in global memory."""
+        check(lib().dy_batch_set_lds_limit(self.h, int(nbytes)), "dy_batch_set_lds_limit")
 
     def shape(self) -> tuple[int, int]:
         t, w = u32(0), u32(0)

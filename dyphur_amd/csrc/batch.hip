@@ -777,14 +777,27 @@ __global__ __launch_bounds__(NT) void batch_frames(BatchArgs A) {
     }
 }
 
+// Dynamic LDS beyond 64 KiB (gfx950 gives a workgroup up to 160 KB) is
+// declared to the runtime once per kernel instance.
+template <class K>
+int allow_lds(K kernel, size_t lds_bytes) {
+    if (lds_bytes <= 65536) return DY_OK;
+    DY_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(lds_bytes)));
+    return DY_OK;
+}
+
 template <uint32_t NT, bool RES>
 int launch_frames(const BatchArgs &a, uint32_t n_units, size_t lds_bytes, hipStream_t s) {
     if (n_units == 0) return DY_OK;
     if (a.tape) {
+        DY_TRY_STATUS(allow_lds(batch_frames<NT, false, RES, true>, lds_bytes));
         hipLaunchKernelGGL((batch_frames<NT, false, RES, true>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     } else if (a.stage_clk) {
+        DY_TRY_STATUS(allow_lds(batch_frames<NT, true, RES, false>, lds_bytes));
         hipLaunchKernelGGL((batch_frames<NT, true, RES, false>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     } else {
+        DY_TRY_STATUS(allow_lds(batch_frames<NT, false, RES, false>, lds_bytes));
         hipLaunchKernelGGL((batch_frames<NT, false, RES, false>), dim3(n_units), dim3(NT), lds_bytes, s, a);
     }
     DY_LAUNCH_CHECK();
@@ -1084,10 +1097,12 @@ struct dy_batch_s {
         size_t lds_bytes = 0;
     };
     struct Shape {
-        Units res;   // scenes whose state fits in LDS
-        Units glob;  // the others
+        Units res;   // scenes whose state fits 64 KiB of LDS
+        Units big;   // scenes whose state fits the larger LDS of one workgroup (one per workgroup)
+        Units glob;  // the others: state in global memory
     };
     uint32_t n_scenes = 0, max_n = 0, block = 256, wave_max = 128;
+    uint32_t lds_max_floats = kLdsMaxFloats;  // dynamic LDS one workgroup may take on this device
     int n_iters = 10;
     float mu = 0.5f;
     int force_serial = 0;
@@ -1101,6 +1116,11 @@ struct dy_batch_s {
     dy_contact_pair *d_pairs = nullptr;
     void *d_contacts = nullptr;  // 9 arrays of contact_off.back() words
     dy_contact_view cv{};
+    // The launch of the scenes kept in global memory runs beside the
+    // LDS-resident one: a side stream forked from and joined back into the
+    // caller's (a graph capture records the two as parallel nodes).
+    hipStream_t side[2] = {nullptr, nullptr};  // default priority
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1111,10 +1131,17 @@ void batch_free(dy_batch_s *b) {
     for (void *p : ps)
         if (p) (void)hipFree(p);
     for (auto &kv : b->shapes)
-        for (auto *u : {&kv.second.res, &kv.second.glob}) {
+        for (auto *u : {&kv.second.res, &kv.second.big, &kv.second.glob}) {
             if (u->d_units) (void)hipFree(u->d_units);
             if (u->d_kind) (void)hipFree(u->d_kind);
         }
+    for (hipStream_t x : b->side)
+        if (x) {
+            (void)hipStreamSynchronize(x);
+            (void)hipStreamDestroy(x);
+        }
+    for (hipEvent_t e : {b->ev_fork, b->ev_join[0], b->ev_join[1]})
+        if (e) (void)hipEventDestroy(e);
 }
 
 template <typename T>
@@ -1146,8 +1173,11 @@ int upload_units(dy_batch_s::Units &u, const std::vector<uint4> &units, const st
 // threads every scene is a wavefront team); larger scenes get a workgroup
 // each, largest first.  A wavefront team's LDS region holds its scene's box
 // records and state; a workgroup team's scene keeps its state in LDS where
-// the 64 KiB allow and goes to the second launch (state in global memory)
-// where they do not.
+// 64 KiB allow.  Scenes beyond that go to a second launch, one per
+// workgroup, whose LDS is sized for its largest scene up to what one
+// workgroup may take (160 KB on gfx950), so that the first launch keeps its
+// occupancy; scenes beyond that go to a third launch with their state in
+// global memory.
 int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
     const auto key = std::make_pair(threads, wave_max);
     if (b->shapes.count(key)) return DY_OK;
@@ -1158,11 +1188,12 @@ int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
     auto nb = [&](uint32_t i) { return b->body_off[i + 1] - b->body_off[i]; };
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return nb(x) > nb(y); });
     const uint32_t wave_cap = kLdsMaxFloats / per / kRes;  // bodies a wavefront region can hold
-    std::vector<uint32_t> big, small, huge;
+    std::vector<uint32_t> big, small, large, huge;
     for (uint32_t i : order) {
         const bool wave = threads == 64 ? nb(i) <= wave_cap : nb(i) <= std::min(wave_max, wave_cap);
         if (wave) small.push_back(i);
         else if (kRes * nb(i) <= kLdsMaxFloats && threads > 64) big.push_back(i);
+        else if (kRes * nb(i) <= b->lds_max_floats) large.push_back(i);
         else huge.push_back(i);
     }
     const uint32_t region = small.empty() ? 0u : kRes * nb(small.front());
@@ -1180,15 +1211,21 @@ int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
         units.push_back(make_uint4(sc[0], sc[1], sc[2], sc[3]));
         kind.push_back(0u);
     }
-    std::vector<uint4> gunits;
-    std::vector<uint32_t> gkind;
-    for (uint32_t i : huge) {
-        gunits.push_back(make_uint4(i, kNoScene, kNoScene, kNoScene));
-        gkind.push_back(threads > 64 ? kUnitBlock : 0u);
-    }
+    auto singles = [&](const std::vector<uint32_t> &list, std::vector<uint4> &u, std::vector<uint32_t> &k) {
+        for (uint32_t i : list) {
+            u.push_back(make_uint4(i, kNoScene, kNoScene, kNoScene));
+            k.push_back(threads > 64 ? kUnitBlock : 0u);
+        }
+    };
+    std::vector<uint4> lunits, gunits;
+    std::vector<uint32_t> lkind, gkind;
+    singles(large, lunits, lkind);
+    singles(huge, gunits, gkind);
+    const uint32_t llds = large.empty() ? 0u : kRes * nb(large.front());
     const uint32_t glds = huge.empty() ? 0u : kBoxFloats * nb(huge.front());
     dy_batch_s::Shape sh;
     int st = upload_units(sh.res, units, kind, region, lds);
+    if (st == DY_OK) st = upload_units(sh.big, lunits, lkind, llds, llds);
     if (st == DY_OK) st = upload_units(sh.glob, gunits, gkind, glds, glds);
     b->shapes[key] = sh;  // freed with the batch, also after a failure
     return st;
@@ -1239,23 +1276,40 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
     a.keep_input = keep_input ? 1 : 0;
     if (tape && !tape_cap) return DY_ERR_INVALID_ARG;
     if (keep_input && sh.glob.n_units) return DY_ERR_INVALID_ARG;
-    for (int pass = 0; pass < 2; ++pass) {
-        const dy_batch_s::Units &u = pass == 0 ? sh.res : sh.glob;
+    // Up to three launches (LDS-resident, larger LDS-resident, global
+    // memory): the first non-empty one on the caller's stream, the others on
+    // the side streams, forked from it and joined back into it.
+    const dy_batch_s::Units *passes[3] = {&sh.res, &sh.big, &sh.glob};
+    int n_launch = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+        const dy_batch_s::Units &u = *passes[pass];
         if (u.n_units == 0) continue;
+        hipStream_t ls = s;
+        if (n_launch > 0) {
+            ls = b->side[n_launch - 1];
+            if (n_launch == 1) DY_TRY(hipEventRecord(b->ev_fork, s));
+            DY_TRY(hipStreamWaitEvent(ls, b->ev_fork, 0));
+        }
+        ++n_launch;
         a.units = u.d_units;
         a.unit_kind = u.d_kind;
         a.region_floats = u.region_floats;
+        const bool res = pass < 2;
         int st = DY_ERR_INVALID_ARG;
         switch (b->block) {
-            case 64: st = pass ? launch_frames<64, false>(a, u.n_units, u.lds_bytes, s)
-                               : launch_frames<64, true>(a, u.n_units, u.lds_bytes, s); break;
-            case 128: st = pass ? launch_frames<128, false>(a, u.n_units, u.lds_bytes, s)
-                                : launch_frames<128, true>(a, u.n_units, u.lds_bytes, s); break;
-            case 256: st = pass ? launch_frames<256, false>(a, u.n_units, u.lds_bytes, s)
-                                : launch_frames<256, true>(a, u.n_units, u.lds_bytes, s); break;
+            case 64: st = res ? launch_frames<64, true>(a, u.n_units, u.lds_bytes, ls)
+                              : launch_frames<64, false>(a, u.n_units, u.lds_bytes, ls); break;
+            case 128: st = res ? launch_frames<128, true>(a, u.n_units, u.lds_bytes, ls)
+                               : launch_frames<128, false>(a, u.n_units, u.lds_bytes, ls); break;
+            case 256: st = res ? launch_frames<256, true>(a, u.n_units, u.lds_bytes, ls)
+                               : launch_frames<256, false>(a, u.n_units, u.lds_bytes, ls); break;
             default: break;
         }
         DY_TRY_STATUS(st);
+    }
+    for (int j = 0; j + 1 < n_launch; ++j) {
+        DY_TRY(hipEventRecord(b->ev_join[j], b->side[j]));
+        DY_TRY(hipStreamWaitEvent(s, b->ev_join[j], 0));
     }
     return DY_OK;
 }
@@ -1349,6 +1403,18 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
     if (hipMemsetAsync(b->d_stats, 0, static_cast<size_t>(n_scenes) * kStWords * 4, hsm) != hipSuccess)
         return fail(DY_ERR_INTERNAL);
     if (hipStreamSynchronize(hsm) != hipSuccess) return fail(DY_ERR_INTERNAL);
+    for (hipStream_t &x : b->side)
+        if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) return fail(DY_ERR_INTERNAL);
+    for (hipEvent_t *e : {&b->ev_fork, &b->ev_join[0], &b->ev_join[1]})
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(DY_ERR_INTERNAL);
+    {
+        // dynamic LDS one workgroup may take, less 4 KiB for the kernels' static LDS
+        int dev = 0, lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return fail(DY_ERR_INTERNAL);
+        b->lds_max_floats = std::max<uint32_t>(kLdsMaxFloats, static_cast<uint32_t>(std::max(lds - 4096, 0)) / 4u);
+    }
     DY_BATCH_TRY(prepare_shape(b, b->block, b->wave_max));
 #undef DY_BATCH_TRY
     *out = b;
@@ -1375,6 +1441,26 @@ int dy_batch_set_shape(dy_batch_t b, uint32_t threads, uint32_t wave_scene_max) 
     b->block = threads;
     b->wave_max = wave_scene_max;
     return DY_OK;
+}
+
+int dy_batch_set_lds_limit(dy_batch_t b, uint32_t bytes) {
+    if (!b) return DY_ERR_INVALID_ARG;
+    int dev = 0, lds = 0;
+    DY_TRY(hipGetDevice(&dev));
+    DY_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const uint32_t dev_floats = std::max<uint32_t>(kLdsMaxFloats, static_cast<uint32_t>(std::max(lds - 4096, 0)) / 4u);
+    const uint32_t want = bytes == 0 ? dev_floats : std::min(dev_floats, std::max(kLdsMaxFloats, bytes / 4u));
+    if (want == b->lds_max_floats) return DY_OK;
+    // the launch plans depend on the limit: drop them and plan the current shape again
+    DY_TRY(hipDeviceSynchronize());
+    for (auto &kv : b->shapes)
+        for (auto *u : {&kv.second.res, &kv.second.big, &kv.second.glob}) {
+            if (u->d_units) (void)hipFree(u->d_units);
+            if (u->d_kind) (void)hipFree(u->d_kind);
+        }
+    b->shapes.clear();
+    b->lds_max_floats = want;
+    return prepare_shape(b, b->block, b->wave_max);
 }
 
 int dy_batch_get_shape(dy_batch_t b, uint32_t *threads, uint32_t *wave_scene_max) {
@@ -1648,8 +1734,11 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     // Reverse launches: scenes of at most kRevWaveMax bodies one per
     // wavefront, the others one per workgroup, in LDS where their adjoint and
     // constants (21 floats per body) and up to 384 contacts' accumulators fit
-    // 64 KiB, else in global memory; each list largest first.
-    constexpr uint32_t kAccLds = 384, kLdsFloats = 65536u / sizeof(float);
+    // what one workgroup may take (160 KB on gfx950: the workgroup fills its
+    // CU at one wavefront per SIMD in any case), else in global memory; each
+    // list largest first.
+    constexpr uint32_t kAccLds = 384;
+    const uint32_t kLdsFloats = b->lds_max_floats;
     std::vector<uint32_t> rev[3];
     uint32_t n_max[3] = {0, 0, 0}, c_max[3] = {0, 0, 0};
     auto nb = [&](uint32_t i) { return b->body_off[i + 1] - b->body_off[i]; };
@@ -1667,7 +1756,7 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
         std::stable_sort(rev[k].begin(), rev[k].end(), [&](uint32_t x, uint32_t y) { return nb(x) > nb(y); });
         all.insert(all.end(), rev[k].begin(), rev[k].end());
         d->n_rev[k] = static_cast<uint32_t>(rev[k].size());
-        // every scene of launch k fits 21 n_max[k] + 8 acc_rev[k] floats (<= 64 KiB)
+        // every scene of launch k fits 21 n_max[k] + 8 acc_rev[k] floats (<= kLdsFloats)
         d->acc_rev[k] = k == 0 ? std::min(c_max[0], kAccLds) : (k == 1 ? acc_big : 0u);
         d->lds_rev[k] = k == 2 ? 0u : 21u * n_max[k] + 8u * d->acc_rev[k];
     }
@@ -1844,6 +1933,7 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
                 DY_TRY(hipStreamWaitEvent(st, d->ev_fork, 0));
             }
             void *args[] = {&a, &Dk[k]};
+            DY_TRY_STATUS(allow_lds(kern[k], static_cast<size_t>(d->lds_rev[k]) * sizeof(float)));
             DY_TRY(hipLaunchKernel(kern[k], dim3(d->n_rev[k]), dim3(threads[k]), args,
                                    static_cast<size_t>(d->lds_rev[k]) * sizeof(float), st));
             ++n_launch;

@@ -19,9 +19,10 @@ Scenes:
     base and two revolute links with PD motors; the golden articulation hash).
   * ``joint_chains``: chains of boxes and spheres hanging from static bases,
     every joint type with seeded limits, motors and compliance, over a ground.
-  * ``diff_at_scale``: the batched + differentiable configuration at scale
-    (lognormal body counts up to 1 024, config 2's mix at half scale,
-    4 substeps per frame).
+  * ``diff_at_scale``: 1 024 scenes for the batched + differentiable form at
+    scale: lognormal body counts (median 128, clipped to [16, 1 024]), each
+    scene random_mixed's 2/3 boxes and 1/3 spheres at half scale, 4 substeps
+    per frame.
 
 Randomness: PCG32 (XSH-RR, O'Neill 2014) seeded through splitmix64, so a seed
 gives the same scene on every machine with this numpy.
@@ -428,8 +429,8 @@ def diff_batch(n_scenes: int = 256, seed: int = 20260004) -> list[Scene]:
 
 
 # ---------------------------------------------------------------------------
-# Batched + differentiable at scale: config 2's mix of boxes and spheres at
-# half scale per scene, lognormal body counts, 4 substeps per frame.
+# Batched + differentiable at scale: random_mixed's boxes and spheres at half
+# scale per scene, lognormal body counts, 4 substeps per frame.
 # ---------------------------------------------------------------------------
 def diff_mixed(n_dyn: int, rng: Pcg32, scale: float = 0.5, extent=(30.0, 40.0, 30.0), frames: int = 240,
                n_substeps: int = 4, target_sigma: float = 0.5) -> Scene:

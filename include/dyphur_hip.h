@@ -352,6 +352,14 @@ int dy_batch_set_force_serial(dy_batch_t b, int force_serial);
  * workgroup.  Default 256 / 128.  Results do not depend on the shape. */
 int dy_batch_set_shape(dy_batch_t b, uint32_t threads, uint32_t wave_scene_max);
 int dy_batch_get_shape(dy_batch_t b, uint32_t *threads, uint32_t *wave_scene_max);
+/* LDS one workgroup of the step may take, in bytes (0: the device's maximum,
+ * the default: 160 KB on gfx950; values below 64 KiB read as 64 KiB).  Scenes
+ * whose state (112 B per body) fits 64 KiB share launches; larger ones that
+ * fit the limit run one per workgroup in a launch of their own; the rest keep
+ * their state in global memory.  The reverse sweep of a dy_diff_t created
+ * afterwards is planned with the same limit.  Results do not depend on it.
+ * No counterpart in the reference (its scenes never leave global memory). */
+int dy_batch_set_lds_limit(dy_batch_t b, uint32_t bytes);
 /* n_frames frames of every scene in ONE launch; each frame is n_substeps
  * substeps of integrate(dt / n_substeps, 1) -> pairs -> narrowphase ->
  * solve(dt / n_substeps).  Asynchronous; issues no memset or copy, so it can be

@@ -115,6 +115,23 @@ def test_batch_matches_oracle_every_frame(gpu, oracle, shape):
     assert (st["substeps"] == 40).all()
 
 
+@pytest.mark.parametrize("lds_limit", [0, 65536])
+def test_batch_large_and_global_scenes_match_oracle(gpu, oracle, lds_limit):
+    """A 700-body dense scene (78 KB of state) runs in the launch of its own
+    with the device's LDS limit, and from global memory at 64 KiB, beside
+    small piles in the LDS-resident launch; every frame equals the oracle."""
+    rng = scenes.Pcg32(20261208)
+    scs = [scenes.random_mixed(700, seed=41, extent=(6, 5, 6)), scenes.pile_scene(40, rng),
+           scenes.pile_scene(12, rng)]
+    print(f"[batch] 700-body scene beside piles, lds limit {lds_limit}", flush=True)
+    batch = gpu.Batch(scs)
+    batch.set_lds_limit(lds_limit)
+    ref = _oracle_worlds(oracle, scs)
+    for f in range(10):
+        batch.step(1)
+        _compare_frame(batch, _run_oracle_frame(ref, scs), f)
+
+
 def test_batch_force_serial_matches_oracle(gpu, oracle):
     scs = _mixed_batch()[-3:]
     batch = gpu.Batch(scs)

@@ -4,7 +4,7 @@ reverse sweep (float32), against a float64 host forward-mode reference of the
 same frames (tests/test_adjoint.py pins that reference against central
 differences and against the oracle's trajectory).
 
-Loss L = sum over dynamic bodies of |x_T - x*|^2 (config 4's loss).  Per
+Loss L = sum over dynamic bodies of |x_T - x*|^2 (the thrown-stack loss).  Per
 scene, a random direction u over (v0, inverse mass) of its dynamic bodies:
 the GPU's u . grad L must equal the reference's dL/du within 1e-3 relative.
 """
@@ -158,16 +158,17 @@ def test_torch_autograd_rollout_matches_reverse_sweep(gpu, hm):
 
 # ---- at scale: substeps, a scene reversed from global memory, tape chunks ----
 def _mixed_batch(frames):
-    """Three scenes of config 5's kind (mixed boxes and spheres at half
-    scale), packed densely so contacts start at once; the 700-body scene is
-    too large to keep in LDS, so it is recorded and reversed from global
-    memory.  2 substeps per frame."""
+    """Three scenes of mixed boxes and spheres at half scale (as the
+    batched + differentiable workload at scale has them), packed densely so
+    contacts start at once; the 700-body scene's state (78 KB) exceeds
+    64 KiB of LDS.  2 substeps per frame."""
     rng = scenes.Pcg32(20260005)
     return [scenes.diff_mixed(n, rng, extent=(7.0, 7.0, 7.0), frames=frames, n_substeps=2) for n in (40, 700, 120)]
 
 
-def _grads(gpu, scs, frames, every, **kw):
+def _grads(gpu, scs, frames, every, lds_limit=0, **kw):
     b = gpu.Batch(scs)
+    b.set_lds_limit(lds_limit)
     d = gpu.Diff(b, frames, every, n_substeps=scs[0].n_substeps, **kw)
     d.forward(frames)
     st = b.state()
@@ -182,15 +183,18 @@ def _grads(gpu, scs, frames, every, **kw):
     return b, d, st, d.backward(adj)
 
 
-def test_diff_substeps_and_global_scene_match_forward_mode(gpu, hm):
-    """Gradients through 2 substeps per frame, with the 700-body scene's
-    record and reverse sweep in global memory, against the host float
-    forward mode over the same substeps (dt / 2, 2 x frames), whose
-    trajectory equals the kernels' bit for bit."""
+@pytest.mark.parametrize("lds_limit", [0, 65536])
+def test_diff_substeps_and_global_scene_match_forward_mode(gpu, hm, lds_limit):
+    """Gradients through 2 substeps per frame against the host float forward
+    mode over the same substeps (dt / 2, 2 x frames), whose trajectory
+    equals the kernels' bit for bit.  The 700-body scene is recorded and
+    reversed by a workgroup in LDS under the device's limit, and from global
+    memory under a 64 KiB limit."""
     frames, every = 6, 3
     scs = _mixed_batch(frames)
-    print(f"[diff] mixed scenes {[sc.n for sc in scs]}, {frames} frames x 2 substeps", flush=True)
-    b, d, st, g = _grads(gpu, scs, frames, every)
+    print(f"[diff] mixed scenes {[sc.n for sc in scs]}, {frames} frames x 2 substeps, lds limit {lds_limit}",
+          flush=True)
+    b, d, st, g = _grads(gpu, scs, frames, every, lds_limit=lds_limit)
     rng = np.random.default_rng(12)
     for i, sc in enumerate(scs):
         sl = b.scene_slice(i)
