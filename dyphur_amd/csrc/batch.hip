@@ -272,14 +272,14 @@ struct BlockTeam {
 // entry is rewritten by the thread that read it: callers sync before other
 // threads read the result.
 template <class Team>
-__device__ uint32_t team_scan(const Team &tm, uint32_t *data, uint32_t n) {
+__device__ uint32_t team_scan(const Team &tm, uint32_t *data, uint32_t n, uint32_t stride = 1) {
     uint32_t carry = 0;
     for (uint32_t base = 0; base < n; base += Team::kSize) {
         const uint32_t i = base + tm.rank;
-        const uint32_t v = i < n ? data[i] : 0u;
+        const uint32_t v = i < n ? data[i * stride] : 0u;
         uint32_t tot = 0;
         const uint32_t ex = tm.excl_scan(v, tot);
-        if (i < n) data[i] = carry + ex;
+        if (i < n) data[i * stride] = carry + ex;
         carry += tot;
     }
     return carry;
@@ -289,6 +289,235 @@ __device__ uint32_t team_scan(const Team &tm, uint32_t *data, uint32_t n) {
 // else touching or overlapping.  Box records: mn xyz, mx xyz, 2 unused.
 __device__ __forceinline__ bool touch(const float q[6], const float4 lo, const float2 hi) {
     return !(lo.x > q[3] || lo.w < q[0] || lo.y > q[4] || hi.x < q[1] || lo.z > q[5] || hi.y < q[2]);
+}
+
+// ---- candidate pairs of a larger scene on a uniform x-z grid --------------
+// Order-preserving map of floats to u32 (the max of keys is the key of the max).
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(v, d, 64)));
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+constexpr uint32_t kGridMinBodies = 160;  // smaller scenes test every pair
+constexpr uint32_t kRowList = 16;         // partners a gridded row may list instead of bit words
+
+struct PairGrid {
+    float x0, z0, wx, wz;
+    uint32_t kx, kz;
+    __device__ static uint32_t cells(float span, float w) {
+        const float f = floorf(span / w);
+        return f >= 65535.f ? 65536u : static_cast<uint32_t>(f) + 1u;
+    }
+    __device__ static uint32_t cell(float v, float v0, float w, uint32_t k) {
+        const float u = floorf((v - v0) / w);
+        return u > 0.f ? static_cast<uint32_t>(fminf(u, static_cast<float>(k - 1u))) : 0u;
+    }
+    __device__ uint32_t cx(float x) const { return cell(x, x0, wx, kx); }
+    __device__ uint32_t cz(float z) const { return cell(z, z0, wz, kz); }
+};
+
+// The rows of S.bits (partners b > a whose boxes touch a's) and their counts
+// S.row, found on a grid: every box more than 4 times the mean extent (a
+// ground) is "large" and tested against every body; the others are binned by
+// the x and z of their lower corner into cells 1.01 times the largest small
+// extent wide, so that a small body's small partners lie in the 3 x 3 cells
+// around its own (|lo_b - lo_a| <= that extent on each axis when the boxes
+// touch; the 1 % covers the rounding of the cell coordinates).  The test and
+// the words are the all-pairs pass's, so the pairs are the same.  Scratch:
+// the box records' two unused words (8k + 7: cell counts, then offsets;
+// 8p + 6: bodies by cell, then the large ones), S.fill (cell cursors), S.deg
+// (1: the row is a sorted list of its partners, else bit words) and the team
+// words tf[0, 4) and tf[23, 31).  False, before writing any row, when the grid
+// does not apply (no small body, non-finite boxes): the caller tests every
+// pair.  Every exit is team-uniform.  Kept out of line: inlined into the
+// step kernel, this compiler (ROCm 7.2, gfx950) fails instruction selection
+// ("Operand has incorrect register class" on a compare with the LDS aperture).
+template <class Team>
+__device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, uint32_t n, uint32_t W, const SceneScratch &S, uint32_t *tf) {
+    constexpr uint32_t NT = Team::kSize;
+    const uint32_t r = tm.rank;
+    const float4 *b4 = reinterpret_cast<const float4 *>(box);
+    const float2 *b2 = reinterpret_cast<const float2 *>(box);
+    uint32_t *bu = reinterpret_cast<uint32_t *>(box);
+    // 1. the large-box threshold: 4 x the mean of the x and z extents
+    float se = 0.f;
+    for (uint32_t i = r; i < n; i += NT) {
+        const float4 lo = b4[2 * i];
+        const float2 hi = b2[4 * i + 2];
+        se += (lo.w - lo.x) + (hi.y - lo.z);
+    }
+    se = wave_sum_f32(se);
+    if (lane_id() == 0) tf[r >> 6] = __float_as_uint(se);  // per-wavefront partials (scan words, free here)
+    tm.sync();
+    float st = 0.f;
+    for (uint32_t w = 0; w < NT / 64; ++w) st += __uint_as_float(tf[w]);
+    const float T = 2.f * st / static_cast<float>(n);
+    auto small = [&](float ex, float ez) { return ex <= T && ez <= T; };
+    // 2. over the small boxes: largest extents, lower-corner ranges, count
+    uint32_t k[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // Ex, Ez, -min x, max x, -min z, max z (keys)
+    uint32_t ns = 0;
+    for (uint32_t i = r; i < n; i += NT) {
+        const float4 lo = b4[2 * i];
+        const float2 hi = b2[4 * i + 2];
+        const float ex = lo.w - lo.x, ez = hi.y - lo.z;
+        if (!small(ex, ez)) continue;
+        k[0] = max(k[0], fkey(ex));
+        k[1] = max(k[1], fkey(ez));
+        k[2] = max(k[2], ~fkey(lo.x));
+        k[3] = max(k[3], fkey(lo.x));
+        k[4] = max(k[4], ~fkey(lo.z));
+        k[5] = max(k[5], fkey(lo.z));
+        ++ns;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) k[j] = wave_max_u32(k[j]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) ns += __shfl_xor(ns, d, 64);
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) atomicMax(&tf[23 + j], k[j]);
+        atomicAdd(&tf[29], ns);
+    }
+    tm.sync();
+    ns = tf[29];
+    if (ns == 0) return false;
+    const float Ex = funkey(tf[23]), Ez = funkey(tf[24]);
+    const float xmin = funkey(~tf[25]), xmax = funkey(tf[26]), zmin = funkey(~tf[27]), zmax = funkey(tf[28]);
+    const float sx = xmax - xmin, sz = zmax - zmin;
+    if (!(isfinite(sx) && isfinite(sz) && isfinite(Ex) && isfinite(Ez))) return false;
+    PairGrid g;
+    g.x0 = xmin;
+    g.z0 = zmin;
+    g.wx = fmaxf(1.01f * Ex, 1e-4f);
+    g.wz = fmaxf(1.01f * Ez, 1e-4f);
+    g.kx = PairGrid::cells(sx, g.wx);
+    g.kz = PairGrid::cells(sz, g.wz);
+    while (static_cast<uint64_t>(g.kx) * g.kz > n - 1u) {  // offsets k <= K live in words 8k + 7, k < n
+        if (g.kx >= g.kz) {
+            g.wx *= 2.f;
+            g.kx = PairGrid::cells(sx, g.wx);
+        } else {
+            g.wz *= 2.f;
+            g.kz = PairGrid::cells(sz, g.wz);
+        }
+    }
+    const uint32_t K = g.kx * g.kz;
+    // 3. counting sort of the small bodies by cell; the large ones follow
+    for (uint32_t c = r; c <= K; c += NT) {
+        bu[8 * c + 7] = 0u;
+        S.fill[c] = 0u;
+    }
+    tm.sync();
+    for (uint32_t i = r; i < n; i += NT) {
+        const float4 lo = b4[2 * i];
+        const float2 hi = b2[4 * i + 2];
+        if (small(lo.w - lo.x, hi.y - lo.z)) atomicAdd(&bu[8 * (g.cz(lo.z) * g.kx + g.cx(lo.x)) + 7], 1u);
+    }
+    tm.sync();
+    (void)team_scan(tm, bu + 7, K + 1u, 8u);  // exclusive; offset K = ns
+    tm.sync();
+    for (uint32_t i = r; i < n; i += NT) {
+        const float4 lo = b4[2 * i];
+        const float2 hi = b2[4 * i + 2];
+        uint32_t p;
+        if (small(lo.w - lo.x, hi.y - lo.z)) {
+            const uint32_t c = g.cz(lo.z) * g.kx + g.cx(lo.x);
+            p = bu[8 * c + 7] + atomicAdd(&S.fill[c], 1u);
+        } else {
+            p = ns + atomicAdd(&tf[30], 1u);
+        }
+        bu[8 * p + 6] = i;
+    }
+    tm.sync();
+    // 4. the rows of the small bodies.  A row of at most kRowList partners
+    //    that fit its words is stored as their sorted list (S.deg[a] = 1),
+    //    else as bit words; a large body's row is cleared here and filled by
+    //    the whole team below.
+    for (uint32_t a = r; a < n; a += NT) {
+        const float4 lo = b4[2 * a];
+        const float2 hi = b2[4 * a + 2];
+        const float q[6] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
+        const uint32_t w0 = (a + 1) >> 5;
+        uint32_t *row = S.bits + static_cast<size_t>(a) * W;
+        const bool sm = small(lo.w - lo.x, hi.y - lo.z);
+        uint32_t x0 = 0, x1 = 0, z0 = 1, z1 = 0;  // no cells for a large row
+        if (sm) {
+            const uint32_t cx = g.cx(lo.x), cz = g.cz(lo.z);
+            x0 = cx ? cx - 1u : 0u;
+            x1 = cx + 1u < g.kx ? cx + 1u : cx;
+            z0 = cz ? cz - 1u : 0u;
+            z1 = cz + 1u < g.kz ? cz + 1u : cz;
+        }
+        // visit(b) for every candidate partner b > a whose box touches a's
+        auto each_hit = [&](auto visit) {
+            auto test = [&](uint32_t b) {
+                if (b > a && touch(q, b4[2 * b], b2[4 * b + 2])) visit(b);
+            };
+            for (uint32_t z = z0; z <= z1; ++z) {
+                const uint32_t p1 = bu[8 * (z * g.kx + x1 + 1u) + 7];
+                for (uint32_t p = bu[8 * (z * g.kx + x0) + 7]; p < p1; ++p) test(bu[8 * p + 6]);
+            }
+            if (sm)
+                for (uint32_t p = ns; p < n; ++p) test(bu[8 * p + 6]);
+        };
+        uint32_t h[kRowList];  // the first hits, ascending
+        uint32_t cnt = 0;
+        each_hit([&](uint32_t b) {
+            uint32_t c = b;
+#pragma unroll
+            for (uint32_t j = 0; j < kRowList; ++j) {
+                if (j < cnt) {
+                    const uint32_t lo_ = min(h[j], c), hi_ = max(h[j], c);
+                    h[j] = lo_;
+                    c = hi_;
+                } else if (j == cnt) {
+                    h[j] = c;
+                }
+            }
+            ++cnt;
+        });
+        const bool listed = sm && cnt <= kRowList && cnt <= W - w0;
+        if (listed) {
+#pragma unroll
+            for (uint32_t j = 0; j < kRowList; ++j)
+                if (j < cnt) row[w0 + j] = h[j];
+        } else {
+            for (uint32_t w = w0; w < W; ++w) row[w] = 0u;
+            each_hit([&](uint32_t b) { row[b >> 5] |= 1u << (b & 31u); });
+        }
+        S.deg[a] = listed ? 1u : 0u;
+        S.row[a] = cnt;
+    }
+    tm.sync();
+    // 5. the rows of the large bodies, each by the whole team
+    for (uint32_t p = ns; p < n; ++p) {
+        const uint32_t a = bu[8 * p + 6];
+        const float4 lo = b4[2 * a];
+        const float2 hi = b2[4 * a + 2];
+        const float q[6] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
+        uint32_t cnt = 0;
+        for (uint32_t b = a + 1u + r; b < n; b += NT) {
+            if (touch(q, b4[2 * b], b2[4 * b + 2])) {
+                atomicOr(&S.bits[static_cast<size_t>(a) * W + (b >> 5)], 1u << (b & 31u));
+                ++cnt;
+            }
+        }
+        if (cnt) atomicAdd(&S.row[a], cnt);
+    }
+    return true;
 }
 
 // Sort one body's contact list and record each contact's predecessor on that
@@ -489,8 +718,10 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         // 3. candidate pairs.  Lanes hold rows a; every lane tests the same
         //    partner b (an LDS broadcast), building 32-bit overlap words of
         //    the row; then a scan of the row counts and an emit pass over the
-        //    words, in (a, b) order.
-        for (uint32_t base = 0; base < n; base += NT) {
+        //    words, in (a, b) order.  A scene of kGridMinBodies or more
+        //    fills the same words from a grid of its boxes instead.
+        const bool gridded = n >= kGridMinBodies && grid_rows(tm, box, n, W, S, tf);
+        for (uint32_t base = 0; base < n && !gridded; base += NT) {
             const uint32_t a = base + r;
             const bool have = a < n;
             float q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -523,6 +754,12 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         tm.sync();
         for (uint32_t a = r; a < n; a += NT) {
             uint32_t pos = S.row[a];
+            if (gridded && S.deg[a]) {  // a listed row: its partners, ascending
+                const uint32_t *row = S.bits + static_cast<size_t>(a) * W + ((a + 1) >> 5);
+                const uint32_t cnt = (a + 1 < n ? S.row[a + 1] : n_pairs) - pos;
+                for (uint32_t j = 0; j < cnt && pos < P; ++j) pairs[pos++] = dy_contact_pair{a, row[j]};
+                continue;
+            }
             for (uint32_t w = (a + 1) >> 5; w < W && pos < P; ++w) {
                 uint32_t word = S.bits[static_cast<size_t>(a) * W + w];
                 while (word && pos < P) {
@@ -1276,10 +1513,16 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
     a.keep_input = keep_input ? 1 : 0;
     if (tape && !tape_cap) return DY_ERR_INVALID_ARG;
     if (keep_input && sh.glob.n_units) return DY_ERR_INVALID_ARG;
-    // Up to three launches (LDS-resident, larger LDS-resident, global
-    // memory): the first non-empty one on the caller's stream, the others on
-    // the side streams, forked from it and joined back into it.
-    const dy_batch_s::Units *passes[3] = {&sh.res, &sh.big, &sh.glob};
+    // Up to three launches (larger LDS-resident, global memory, LDS-resident;
+    // the ones holding the largest scenes are enqueued first so that their
+    // workgroups are placed first): the first non-empty one on the caller's
+    // stream, the others on the side streams, forked from the caller's
+    // stream before any of them and joined back into it.
+    const dy_batch_s::Units *passes[3] = {&sh.big, &sh.glob, &sh.res};
+    const bool residents[3] = {true, false, true};
+    int n_nonempty = 0;
+    for (const dy_batch_s::Units *u : passes) n_nonempty += u->n_units ? 1 : 0;
+    if (n_nonempty > 1) DY_TRY(hipEventRecord(b->ev_fork, s));
     int n_launch = 0;
     for (int pass = 0; pass < 3; ++pass) {
         const dy_batch_s::Units &u = *passes[pass];
@@ -1287,14 +1530,13 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
         hipStream_t ls = s;
         if (n_launch > 0) {
             ls = b->side[n_launch - 1];
-            if (n_launch == 1) DY_TRY(hipEventRecord(b->ev_fork, s));
             DY_TRY(hipStreamWaitEvent(ls, b->ev_fork, 0));
         }
         ++n_launch;
         a.units = u.d_units;
         a.unit_kind = u.d_kind;
         a.region_floats = u.region_floats;
-        const bool res = pass < 2;
+        const bool res = residents[pass];
         int st = DY_ERR_INVALID_ARG;
         switch (b->block) {
             case 64: st = res ? launch_frames<64, true>(a, u.n_units, u.lds_bytes, ls)
@@ -1918,18 +2160,22 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         a.tape_cap = d->d_tape_cap;
         // the non-empty launches: the first on the caller's stream, the
         // others on the side streams, forked from and joined back into it
+        // (the largest scenes' launches first, so that their workgroups are
+        // placed first; the fork is recorded before any of them)
         const void *kern[3] = {reinterpret_cast<const void *>(&diff_backward),
                                reinterpret_cast<const void *>(&diff_backward_block),
                                reinterpret_cast<const void *>(&diff_backward_global)};
         const uint32_t threads[3] = {64, kRevBlock, kRevBlock};
+        int n_nonempty = 0;
+        for (uint32_t c : d->n_rev) n_nonempty += c ? 1 : 0;
+        if (n_nonempty > 1) DY_TRY(hipEventRecord(d->ev_fork, hsm));
         int n_launch = 0;
-        for (int k = 0; k < 3; ++k) {
+        for (int k : {2, 1, 0}) {
             if (d->n_rev[k] == 0) continue;
             Dk[k].frames = u.nf * d->n_sub;
             hipStream_t st = hsm;
             if (n_launch > 0) {
                 st = d->rev_st[n_launch - 1];
-                if (n_launch == 1) DY_TRY(hipEventRecord(d->ev_fork, hsm));
                 DY_TRY(hipStreamWaitEvent(st, d->ev_fork, 0));
             }
             void *args[] = {&a, &Dk[k]};
