@@ -20,7 +20,9 @@ The line also carries "batched": the 4096-scene pile workload (seed 20260003)
 stepped 300 frames in one launch, its body-substeps/s and the plain per-scene
 loop of the drop-in API on a sample of the same scenes; and
 "differentiable": the 256-scene thrown-stack workload (seed 20260004), its
-forward and forward + adjoint times per frame.
+forward and forward + adjoint times per frame; "differentiable_at_scale": the
+same for 1 024 mixed scenes of up to 1 024 bodies at 4 substeps per frame
+(seed 20260005); and "articulated": the joint-chain workload.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -54,6 +56,8 @@ def parse():
                     help="frames of the 4096-scene batched workload (0 skips it)")
     ap.add_argument("--diff-frames", type=int, default=120,
                     help="frames of the differentiable 256-scene workload (0 skips it)")
+    ap.add_argument("--diff-scale-frames", type=int, default=240,
+                    help="frames of the 1 024-scene differentiable leg (0 skips it)")
     ap.add_argument("--joint-frames", type=int, default=120,
                     help="frames of the articulated workload (0 skips it)")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -363,6 +367,69 @@ def differentiable(capi, scenes, frames: int = 120, every: int = 10) -> dict:
     }
 
 
+def differentiable_at_scale(capi, scenes, frames: int = 240, every: int = 20) -> dict:
+    """1 024 scenes of lognormal body counts (median 128, clipped to [16,
+    1 024]) of mixed boxes and spheres at half scale, 4 substeps per frame:
+    the forward alone against the checkpointed forward + the backward
+    (recompute from each checkpoint with a record, then the reverse sweep),
+    timed with HIP events on the stream.  Tapes hold 4 contacts per body
+    (the workload peaks near 1.2) and 80 GiB each, so that one tape holds a
+    20-frame segment; a step with more contacts would make the backward
+    fail, not return a wrong gradient."""
+    sub = 4
+    scs = scenes.diff_at_scale()
+    b = capi.Batch(scs)
+    init = b.bodies.host_arrays(b.merged)
+    e0, e1 = capi.Event(), capi.Event()
+    b.step(1, n_substeps=sub)
+    b.bodies.upload_state(init, b.s)
+    e0.record(b.s)
+    b.step(frames, n_substeps=sub)
+    e1.record(b.s)
+    e1.sync()
+    fwd = e0.elapsed_ms(e1)
+    d = capi.Diff(b, frames, every, n_substeps=sub, tape_contacts_per_body=4.0, tape_bytes=80 << 30)
+    steps, step_bytes = d.tape_layout()
+    b.bodies.upload_state(init, b.s)
+    d.forward(frames)
+    st = b.state()
+    n = b.n_bodies
+    adj = {k: np.zeros(n, np.float32) for k in ("pos_x", "pos_y", "pos_z")}
+    for i, sc in enumerate(scs):
+        lo = int(b.body_offset[i])
+        for k, name in enumerate(("pos_x", "pos_y", "pos_z")):
+            adj[name][lo:lo + sc.n - 1] = 2.0 * (st[name][lo:lo + sc.n - 1] - sc.meta["target"][:, k])
+    d.backward(adj, download=False)  # warm-up
+    b.bodies.upload_state(init, b.s)
+    b.s.wait()
+    e0.record(b.s)
+    d.forward(frames)
+    e1.record(b.s)
+    e1.sync()
+    dfw = e0.elapsed_ms(e1)
+    e0.record(b.s)
+    d.backward(adj, download=False)
+    e1.record(b.s)
+    e1.sync()
+    bwd = e0.elapsed_ms(e1)
+    counts = np.array([sc.n for sc in scs])
+    return {
+        "workload": f"{len(scs)} scenes of mixed boxes (2/3) and spheres (1/3) at half scale, lognormal body counts "
+                    f"(median 128, sigma 0.75, clip [16, 1024]) + a ground each, seed 20260005; {frames} frames at "
+                    f"60 Hz, {sub} substeps, 10 iterations; loss sum |x_T - x*|^2; gradients w.r.t. v0 and inverse "
+                    f"mass; checkpoint every {every} frames",
+        "bodies": n, "scenes": len(scs), "max_scene_bodies": int(counts.max()), "frames": frames, "substeps": sub,
+        "checkpoint_every": every, "tape_substeps": steps, "tape_gib": round(steps * step_bytes / 2**30, 1),
+        "forward_ms_per_frame": round(fwd / frames, 4),
+        "forward_body_substeps_per_s": round(n * frames * sub / (fwd * 1e-3), 1),
+        "forward_checkpointed_ms_per_frame": round(dfw / frames, 4),
+        "adjoint_ms_per_frame": round(bwd / frames, 4),
+        "fwd_adjoint_ms_per_frame": round((dfw + bwd) / frames, 4),
+        "fwd_adjoint_over_fwd": round((dfw + bwd) / fwd, 2),
+        "value": round(n * frames * sub / ((dfw + bwd) * 1e-3), 1), "unit": "body-substeps/s (forward + adjoint)",
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -480,6 +547,8 @@ def main():
         out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
     if args.diff_frames > 0:
         out["differentiable"] = differentiable(capi, scenes, args.diff_frames)
+    if args.diff_scale_frames > 0:
+        out["differentiable_at_scale"] = differentiable_at_scale(capi, scenes, args.diff_scale_frames)
     if args.joint_frames > 0:
         out["articulated"] = articulated(capi, scenes, args.joint_frames)
     print(json.dumps(out), flush=True)

@@ -340,11 +340,12 @@ struct PairGrid {
 // the box records' two unused words (8k + 7: cell counts, then offsets;
 // 8p + 6: bodies by cell, then the large ones), S.fill (cell cursors), S.deg
 // (1: the row is a sorted list of its partners, else bit words) and the team
-// words tf[0, 4) and tf[23, 31).  False, before writing any row, when the grid
-// does not apply (no small body, non-finite boxes): the caller tests every
-// pair.  Every exit is team-uniform.  Kept out of line: inlined into the
-// step kernel, this compiler (ROCm 7.2, gfx950) fails instruction selection
-// ("Operand has incorrect register class" on a compare with the LDS aperture).
+// words tf[0, 4) and tf[23, 31).  False, before writing any row, when the
+// grid does not apply (no small body, non-finite boxes) or would not pay
+// (too few cells): the caller tests every pair.  Every exit is team-uniform.
+// Kept out of line: inlined into the step kernel, this compiler (ROCm 7.2,
+// gfx950) fails instruction selection ("Operand has incorrect register
+// class" on a compare with the LDS aperture).
 template <class Team>
 __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, uint32_t n, uint32_t W, const SceneScratch &S, uint32_t *tf) {
     constexpr uint32_t NT = Team::kSize;
@@ -415,6 +416,10 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
         }
     }
     const uint32_t K = g.kx * g.kz;
+    // The grid pays when a row's 3 x 3 cells hold far fewer boxes than the
+    // row would test otherwise (about n / 2): 9 ns / K <= n / 8.  A compact
+    // pile spans few cells and tests every pair.
+    if (72ull * ns > static_cast<uint64_t>(n) * K) return false;
     // 3. counting sort of the small bodies by cell; the large ones follow
     for (uint32_t c = r; c <= K; c += NT) {
         bu[8 * c + 7] = 0u;
@@ -719,8 +724,11 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         //    partner b (an LDS broadcast), building 32-bit overlap words of
         //    the row; then a scan of the row counts and an emit pass over the
         //    words, in (a, b) order.  A scene of kGridMinBodies or more
-        //    fills the same words from a grid of its boxes instead.
-        const bool gridded = n >= kGridMinBodies && grid_rows(tm, box, n, W, S, tf);
+        //    stepped by a workgroup fills the same words from a grid of its
+        //    boxes instead (for a single wavefront's scenes, at most a few
+        //    hundred bodies, testing every pair measured faster).
+        bool gridded = false;
+        if constexpr (NT > 64) gridded = n >= kGridMinBodies && grid_rows(tm, box, n, W, S, tf);
         for (uint32_t base = 0; base < n && !gridded; base += NT) {
             const uint32_t a = base + r;
             const bool have = a < n;
