@@ -1853,6 +1853,12 @@ struct dy_diff_s {
     // Two tapes: the reverse sweep of one unit reads one while the next
     // (older) unit is recomputed into the other on the recompute stream.
     float *d_tape[2] = {nullptr, nullptr};
+    // When every segment's record fits the memory of the two tapes, the
+    // forward records them all (one tape per segment, tape_floats each) and
+    // the backward only reverses: no recompute.
+    bool record_all = false;
+    float *d_all = nullptr;
+    uint64_t tape_floats = 0;
     uint64_t *d_tape_off = nullptr, *d_cs_off = nullptr;
     uint32_t *d_tape_cap = nullptr;  // per scene: contacts a tape step holds
     uint32_t *d_err = nullptr;       // header errors of the reversed steps
@@ -1905,7 +1911,7 @@ void diff_free(dy_diff_s *d) {
     if (d->rs) (void)hipStreamSynchronize(d->rs);
     for (hipStream_t x : d->rev_st)
         if (x) (void)hipStreamSynchronize(x);
-    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_tape_off, d->d_cs_off, d->d_tape_cap, d->d_err,
+    void *ps[] = {d->d_tape[0], d->d_tape[1], d->d_all, d->d_tape_off, d->d_cs_off, d->d_tape_cap, d->d_err,
                   d->d_ckpt, d->d_sub, d->d_work, d->d_cs, d->d_rev_scenes};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -1970,9 +1976,16 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     const uint32_t n_ckpt = (max_frames + checkpoint_every - 1) / checkpoint_every;
     const uint32_t tape_frames = d->tape_steps / n_substeps;
     const uint32_t n_subck = (checkpoint_every + tape_frames - 1) / tape_frames;
+    d->tape_floats = off.back();
+    d->record_all = d->tape_steps == seg_steps &&
+                    static_cast<uint64_t>(n_ckpt) * d->tape_floats * sizeof(float) <= 2 * budget;
     int st = DY_OK;
-    if (st == DY_OK) st = dev_alloc(d->d_tape[0], off.back());
-    if (st == DY_OK) st = dev_alloc(d->d_tape[1], off.back());
+    if (d->record_all) {
+        if (st == DY_OK) st = dev_alloc(d->d_all, static_cast<size_t>(n_ckpt) * d->tape_floats);
+    } else {
+        if (st == DY_OK) st = dev_alloc(d->d_tape[0], d->tape_floats);
+        if (st == DY_OK) st = dev_alloc(d->d_tape[1], d->tape_floats);
+    }
     if (st == DY_OK) st = dev_alloc(d->d_tape_off, off.size());
     if (st == DY_OK) st = dev_alloc(d->d_cs_off, cs_off.size());
     if (st == DY_OK) st = dev_alloc(d->d_tape_cap, ns);
@@ -2059,12 +2072,18 @@ int dy_diff_forward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, cons
     const uint32_t n_seg = (static_cast<uint32_t>(n_frames) + d->every - 1) / d->every;
     const int sub = static_cast<int>(d->n_sub);
     for (uint32_t f0 = 0, k = 0; f0 < static_cast<uint32_t>(n_frames); f0 += d->every, ++k) {
+        const int nf = static_cast<int>(std::min<uint32_t>(d->every, static_cast<uint32_t>(n_frames) - f0));
+        if (d->record_all) {  // every segment recorded as it runs; no checkpoints needed
+            DY_TRY_STATUS(batch_launch(d->b, hsm, *bodies, *shapes, gx, gy, gz, dt, sub, nf,
+                                       d->d_all + static_cast<size_t>(k) * d->tape_floats, d->d_tape_off, false,
+                                       d->d_tape_cap));
+            continue;
+        }
         dy_body_view ck{};
         set_mutable_fields(ck, d->d_ckpt + static_cast<size_t>(k) * 13 * d->n_total, d->n_total);
         float *dst[13];
         mutable_fields(ck, dst);
         DY_TRY_STATUS(copy_state(hsm, dst, src, d->n_total));
-        const int nf = static_cast<int>(std::min<uint32_t>(d->every, static_cast<uint32_t>(n_frames) - f0));
         // The newest segment is recorded as it runs when one tape holds it:
         // the backward then starts from it without recomputing it.
         const bool newest = k + 1 == n_seg && static_cast<uint32_t>(nf) * d->n_sub <= d->tape_steps;
@@ -2105,8 +2124,10 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
     }
     const uint32_t tape_frames = d->tape_steps / d->n_sub;
 
-    // Units of at most one tape, newest first: segment k (checkpoint every
-    // `every` frames) in chunks of tape_frames frames.  A unit's reverse
+    // All segments recorded by the forward: reverse each from its own tape,
+    // newest first.  Otherwise units of at most one tape, newest first:
+    // segment k (checkpoint every `every` frames) in chunks of tape_frames
+    // frames.  A unit's reverse
     // sweep (stream s) reads tape u % 2 while the next unit is recomputed
     // into the other tape on the recompute stream; events order the
     // hand-overs.  A segment longer than a tape is first replayed from its
@@ -2161,9 +2182,9 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
         DY_TRY(hipStreamWaitEvent(hsm, d->ev_tape[t], 0));
         return DY_OK;
     };
-    auto reverse = [&](const Unit &u, int t) -> int {
+    auto reverse = [&](const Unit &u, float *tape, int t) -> int {  // t: the tape to release, or -1
         BatchArgs a = batch_args(d->b, *bodies, *shapes, gx, gy, gz, dt, sub, static_cast<int>(u.nf));
-        a.tape = d->d_tape[t];
+        a.tape = tape;
         a.tape_off = d->d_tape_off;
         a.tape_cap = d->d_tape_cap;
         // the non-empty launches: the first on the caller's stream, the
@@ -2196,19 +2217,24 @@ int dy_diff_backward(dy_diff_t d, dy_stream_t s, const dy_body_view *bodies, con
             DY_TRY(hipEventRecord(d->ev_join[j], d->rev_st[j]));
             DY_TRY(hipStreamWaitEvent(hsm, d->ev_join[j], 0));
         }
-        DY_TRY(hipEventRecord(d->ev_free[t], hsm));
+        if (t >= 0) DY_TRY(hipEventRecord(d->ev_free[t], hsm));
         return DY_OK;
     };
     std::vector<Unit> units;
     bool first = true;
-    for (int k = static_cast<int>(n_seg) - 1; k >= 0; --k) {
+    for (int k = static_cast<int>(n_seg) - 1; k >= 0 && d->record_all; --k) {
+        const uint32_t f0 = static_cast<uint32_t>(k) * d->every;
+        const Unit u{static_cast<uint32_t>(k), f0, std::min(d->every, d->frames - f0), nullptr};
+        DY_TRY_STATUS(reverse(u, d->d_all + static_cast<size_t>(k) * d->tape_floats, -1));
+    }
+    for (int k = static_cast<int>(n_seg) - 1; k >= 0 && !d->record_all; --k) {
         DY_TRY_STATUS(seg_units(static_cast<uint32_t>(k), units));
         for (const Unit &u : units) {
             const int t = static_cast<int>(u_index % 2u);
             // the forward taped the newest segment when one tape held it
             const bool taped = first && d->newest == t && units.size() == 1;
             if (!taped) DY_TRY_STATUS(recompute(u, t));
-            DY_TRY_STATUS(reverse(u, t));
+            DY_TRY_STATUS(reverse(u, d->d_tape[t], t));
             first = false;
             ++u_index;
         }

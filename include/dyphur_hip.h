@@ -405,7 +405,10 @@ typedef struct dy_diff_s *dy_diff_t;
  * (<= 0: the scene's contact capacity); a step with more contacts makes the
  * backward fail with DY_ERR_INVALID_ARG.  Each of the two tapes takes at most
  * tape_bytes (0: 24 GiB), in whole frames; DY_ERR_INVALID_ARG if one frame
- * does not fit.  Allocations are done here (blocking). */
+ * does not fit.  When the records of all ceil(max_frames / checkpoint_every)
+ * segments fit 2 x tape_bytes, the forward records every segment instead and
+ * the backward recomputes nothing (same gradient bits).  Allocations are
+ * done here (blocking). */
 int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every, uint32_t n_substeps,
                    float tape_contacts_per_body, uint64_t tape_bytes, dy_diff_t *out);
 int dy_diff_destroy(dy_diff_t d);

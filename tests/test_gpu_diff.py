@@ -262,6 +262,22 @@ def test_diff_tape_chunks_give_the_same_bits(gpu):
         assert np.array_equal(bits(g1[k]), bits(g2[k])), k
 
 
+def test_diff_recorded_horizon_and_recompute_give_the_same_bits(gpu):
+    """Three 2-frame segments: by default all three records fit the two
+    tapes' budget, so the forward records them and the backward only
+    reverses; with a budget of one segment per tape the backward recomputes
+    each older segment from its checkpoint.  Same gradient bits."""
+    frames, every = 6, 2
+    scs = _mixed_batch(frames)
+    _, d1, _, g1 = _grads(gpu, scs, frames, every)
+    steps, step_bytes = d1.tape_layout()
+    assert steps == every * 2
+    _, d2, _, g2 = _grads(gpu, scs, frames, every, tape_bytes=steps * step_bytes)
+    assert d2.tape_steps() == steps
+    for k in g1:
+        assert np.array_equal(bits(g1[k]), bits(g2[k])), k
+
+
 def test_diff_overfull_reduced_tape_is_an_error(gpu):
     """A tape that holds fewer contacts than a step produced: the backward
     reports it instead of returning a wrong gradient."""
