@@ -564,17 +564,32 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
     uint32_t r = 0;
     remaining = nc;
     over = 0;
+    constexpr uint32_t kChunk = 8;  // contacts a thread decides on before committing any
     while (remaining > 0) {
         if (jp && r >= kBatchMaxRounds) break;
         uint32_t mine = 0, my_over = 0;
-        for (uint32_t k = tm.rank; k < nc; k += Team::kSize) {
-            if (S.rnd[k] != kNotAssigned) continue;
-            const int32_t pa = S.pred_a[k], pb = S.pred_b[k];
-            // A predecessor assigned in this round reads as r or as unassigned.
-            if (pa >= 0 && __hip_atomic_load(&S.rnd[pa], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= r)
-                continue;
-            if (pb >= 0 && __hip_atomic_load(&S.rnd[pb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= r)
-                continue;
+        for (uint32_t base = tm.rank; base < nc; base += kChunk * Team::kSize) {
+            // decide: which of this thread's next kChunk contacts have every
+            // predecessor assigned in an earlier round (independent loads,
+            // in flight together; a predecessor assigned in this round reads
+            // as r or as unassigned)
+            uint32_t ready = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < kChunk; ++j) {
+                const uint32_t k = base + j * Team::kSize;
+                if (k >= nc) break;
+                if (__hip_atomic_load(&S.rnd[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kNotAssigned)
+                    continue;
+                const int32_t pa = S.pred_a[k], pb = S.pred_b[k];
+                const bool ok_a =
+                    pa < 0 || __hip_atomic_load(&S.rnd[pa], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < r;
+                const bool ok_b =
+                    pb < 0 || __hip_atomic_load(&S.rnd[pb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < r;
+                if (ok_a && ok_b) ready |= 1u << j;
+            }
+            // commit the ready ones (contacts of one round share no dynamic body)
+            for (uint32_t rem_ = ready; rem_; rem_ &= rem_ - 1u) {
+            const uint32_t k = base + static_cast<uint32_t>(__builtin_ctz(rem_)) * Team::kSize;
             if (jp) {
                 const uint32_t a = cv.body_a[k], b = cv.body_b[k];
                 const bool da = im[a] != 0.f, db = im[b] != 0.f;
@@ -593,6 +608,7 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
             }
             __hip_atomic_store(&S.rnd[k], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             ++mine;
+            }
         }
         uint32_t any_over = 0;
         const uint32_t got = tm.sum_or(mine, my_over, any_over);
@@ -1473,6 +1489,13 @@ int prepare_shape(dy_batch_s *b, uint32_t threads, uint32_t wave_max) {
     if (st == DY_OK) st = upload_units(sh.big, lunits, lkind, llds, llds);
     if (st == DY_OK) st = upload_units(sh.glob, gunits, gkind, glds, glds);
     b->shapes[key] = sh;  // freed with the batch, also after a failure
+    // one side stream per launch beyond the first, made here (set-up, never
+    // inside a graph capture) and only as many as a shape needs: a process's
+    // streams share HIP's few hardware queues
+    const int n_launch = (units.empty() ? 0 : 1) + (lunits.empty() ? 0 : 1) + (gunits.empty() ? 0 : 1);
+    for (int j = 0; j + 1 < n_launch && st == DY_OK; ++j)
+        if (!b->side[j] && hipStreamCreateWithFlags(&b->side[j], hipStreamNonBlocking) != hipSuccess)
+            st = DY_ERR_INTERNAL;
     return st;
 }
 
@@ -1537,7 +1560,7 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
         if (u.n_units == 0) continue;
         hipStream_t ls = s;
         if (n_launch > 0) {
-            ls = b->side[n_launch - 1];
+            ls = b->side[n_launch - 1];  // made by prepare_shape
             DY_TRY(hipStreamWaitEvent(ls, b->ev_fork, 0));
         }
         ++n_launch;
@@ -1653,8 +1676,6 @@ int dy_batch_create(dy_stream_t s, uint32_t n_scenes, const uint32_t *body_offse
     if (hipMemsetAsync(b->d_stats, 0, static_cast<size_t>(n_scenes) * kStWords * 4, hsm) != hipSuccess)
         return fail(DY_ERR_INTERNAL);
     if (hipStreamSynchronize(hsm) != hipSuccess) return fail(DY_ERR_INTERNAL);
-    for (hipStream_t &x : b->side)
-        if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) return fail(DY_ERR_INTERNAL);
     for (hipEvent_t *e : {&b->ev_fork, &b->ev_join[0], &b->ev_join[1]})
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(DY_ERR_INTERNAL);
     {
@@ -2027,8 +2048,13 @@ int dy_diff_create(dy_batch_t b, uint32_t max_frames, uint32_t checkpoint_every,
     if (st == DY_OK && hipMemcpy(d->d_rev_scenes, all.data(), all.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         st = DY_ERR_INTERNAL;
     if (st == DY_OK && hipStreamCreateWithFlags(&d->rs, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
-    for (hipStream_t &x : d->rev_st)
-        if (st == DY_OK && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) st = DY_ERR_INTERNAL;
+    // one side stream per reverse launch beyond the first (a process's
+    // streams share HIP's few hardware queues, so idle ones are not made)
+    int n_rev_launches = 0;
+    for (uint32_t c : d->n_rev) n_rev_launches += c ? 1 : 0;
+    for (int j = 0; j + 1 < n_rev_launches; ++j)
+        if (st == DY_OK && hipStreamCreateWithFlags(&d->rev_st[j], hipStreamNonBlocking) != hipSuccess)
+            st = DY_ERR_INTERNAL;
     hipEvent_t *evs[] = {&d->ev_start, &d->ev_tape[0], &d->ev_tape[1], &d->ev_free[0], &d->ev_free[1],
                          &d->ev_fork,  &d->ev_join[0], &d->ev_join[1]};
     for (hipEvent_t *e : evs)
