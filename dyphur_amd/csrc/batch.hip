@@ -532,6 +532,40 @@ __device__ __forceinline__ void body_predecessors(uint32_t x, const SceneScratch
                                                   bool jp) {
     uint32_t *L = S.list + S.loff[x];
     const uint32_t d = S.deg[x];
+    constexpr uint32_t kRegSort = 8;
+    if (d <= kRegSort) {
+        // in registers: the list's loads in flight together, an odd-even
+        // transposition network (the keys are distinct, so the order is the
+        // insertion sort's), then the stores
+        uint32_t v[kRegSort];
+        uint64_t key[kRegSort];
+#pragma unroll
+        for (uint32_t i = 0; i < kRegSort; ++i) v[i] = i < d ? L[i] : 0u;
+#pragma unroll
+        for (uint32_t i = 0; i < kRegSort; ++i) key[i] = i < d ? (jp ? ~jp_priority(v[i]) : v[i]) : ~0ull;
+#pragma unroll
+        for (uint32_t round = 0; round < kRegSort; ++round) {
+#pragma unroll
+            for (uint32_t i = round & 1u; i + 1 < kRegSort; i += 2) {
+                if (key[i] > key[i + 1]) {  // strict: padding (~0) never passes a list entry
+                    const uint64_t tk = key[i]; key[i] = key[i + 1]; key[i + 1] = tk;
+                    const uint32_t tv = v[i]; v[i] = v[i + 1]; v[i + 1] = tv;
+                }
+            }
+        }
+        uint32_t own[kRegSort];
+#pragma unroll
+        for (uint32_t p = 0; p < kRegSort; ++p) own[p] = p < d ? ca[v[p]] : 0u;
+#pragma unroll
+        for (uint32_t p = 0; p < kRegSort; ++p) {
+            if (p >= d) break;
+            L[p] = v[p];
+            const int32_t pred = p ? static_cast<int32_t>(v[p - 1]) : -1;
+            if (own[p] == x) S.pred_a[v[p]] = pred;
+            else S.pred_b[v[p]] = pred;
+        }
+        return;
+    }
     for (uint32_t i = 1; i < d; ++i) {
         const uint32_t k = L[i];
         const uint64_t kk = jp ? ~jp_priority(k) : k;
@@ -551,6 +585,15 @@ __device__ __forceinline__ void body_predecessors(uint32_t x, const SceneScratch
         if (ca[k] == x) S.pred_a[k] = pred;
         else S.pred_b[k] = pred;
     }
+}
+
+// Every body's predecessors, by the team.  Out of line: the in-register
+// sort's arrays would otherwise raise the step kernel's register peak (and
+// halve its occupancy); one call per solve.
+template <class Team>
+__device__ __attribute__((noinline)) void team_predecessors(const Team &tm, uint32_t n, const SceneScratch &S,
+                                                            const uint32_t *ca, bool jp) {
+    for (uint32_t x = tm.rank; x < n; x += Team::kSize) body_predecessors(x, S, ca, jp);
 }
 
 // Class assignment in rounds.  Returns the rounds run; `remaining` holds the
@@ -856,7 +899,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             }
             tm.sync();
             const bool jp = !(A.force_serial || nc <= kBatchSerialMax);
-            for (uint32_t x = r; x < n; x += NT) body_predecessors(x, S, cv.body_a, jp);
+            team_predecessors(tm, n, S, cv.body_a, jp);
             tm.sync();
             uint32_t remaining = 0, over = 0;
             rounds = assign_rounds(tm, nc, jp, S, cv, bv.inv_mass, remaining, over, err);
@@ -867,7 +910,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                 serial = 1;
                 for (uint32_t k = r; k < nc; k += NT) S.rnd[k] = kNotAssigned;
                 tm.sync();
-                for (uint32_t x = r; x < n; x += NT) body_predecessors(x, S, cv.body_a, false);
+                team_predecessors(tm, n, S, cv.body_a, false);
                 tm.sync();
                 (void)assign_rounds(tm, nc, false, S, cv, bv.inv_mass, remaining, over, err);
             }
