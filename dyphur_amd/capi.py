@@ -87,7 +87,7 @@ class SolverTiming(C.Structure):
 class BatchSceneStats(C.Structure):
     _fields_ = [(k, u32) for k in ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds",
                                    "overflow", "error", "substeps", "sum_pairs", "sum_contacts",
-                                   "sum_dyn_endpoints", "solves")]
+                                   "sum_dyn_endpoints", "solves", "grid_substeps")]
 
 
 class BatchLayout(C.Structure):
@@ -720,7 +720,7 @@ class _Capture:
 
 
 BATCH_STAT_FIELDS = ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds", "overflow", "error",
-                     "substeps", "sum_pairs", "sum_contacts", "sum_dyn_endpoints", "solves")
+                     "substeps", "sum_pairs", "sum_contacts", "sum_dyn_endpoints", "solves", "grid_substeps")
 
 
 class Batch:

@@ -63,7 +63,7 @@ constexpr uint32_t kStages = 5;
 
 enum : uint32_t {
     kStPairs = 0, kStContacts, kStClasses, kStSerial, kStRounds, kStOverflow, kStError, kStSubsteps,
-    kStSumPairs, kStSumContacts, kStSumDyn, kStSumSolves, kStWords
+    kStSumPairs, kStSumContacts, kStSumDyn, kStSumSolves, kStGrid, kStWords
 };
 
 struct BatchArgs {
@@ -1037,6 +1037,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             st[kStSumContacts] += nc;
             st[kStSumDyn] += n_dyn_end;
             st[kStSumSolves] += nc > 0 ? 1u : 0u;
+            st[kStGrid] += gridded ? 1u : 0u;
         }
         tm.sync();
     }

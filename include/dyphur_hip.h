@@ -325,6 +325,7 @@ typedef struct dy_batch_scene_stats {
     uint32_t sum_contacts; /* stored contacts, */
     uint32_t sum_dyn_endpoints;  /* contact endpoints on dynamic bodies, */
     uint32_t solves;       /* substeps that solved at least one contact */
+    uint32_t grid_substeps;  /* substeps whose pairs came from the scene's x-z grid rather than every pair */
 } dy_batch_scene_stats;
 
 /* Device outputs of the last substep, concatenated over scenes. */

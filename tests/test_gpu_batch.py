@@ -132,6 +132,26 @@ def test_batch_large_and_global_scenes_match_oracle(gpu, oracle, lds_limit):
         _compare_frame(batch, _run_oracle_frame(ref, scs), f)
 
 
+def test_batch_grid_pairs_match_oracle(gpu, oracle):
+    """A sparse 600-body scene stepped by a workgroup finds its pairs on the
+    x-z grid (the statistics count those substeps) and still equals the
+    oracle's pair list, contacts and state every frame; a compact pile of the
+    same batch tests every pair."""
+    rng = scenes.Pcg32(20261209)
+    scs = [scenes.random_mixed(600, seed=43, extent=(20, 4, 20)), scenes.pile_scene(300, rng),
+           scenes.pile_scene(30, rng)]
+    batch = gpu.Batch(scs)
+    batch.set_shape(256, 128)
+    ref = _oracle_worlds(oracle, scs)
+    for f in range(8):
+        batch.step(1)
+        _compare_frame(batch, _run_oracle_frame(ref, scs), f)
+    st = batch.stats()
+    print(f"[batch] grid substeps per scene: {list(st['grid_substeps'])}", flush=True)
+    assert st["grid_substeps"][0] == 8
+    assert st["grid_substeps"][2] == 0  # 31 bodies: a wavefront tests every pair
+
+
 def test_batch_force_serial_matches_oracle(gpu, oracle):
     scs = _mixed_batch()[-3:]
     batch = gpu.Batch(scs)
