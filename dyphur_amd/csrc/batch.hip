@@ -338,7 +338,7 @@ struct PairGrid {
 // touch; the 1 % covers the rounding of the cell coordinates).  The test and
 // the words are the all-pairs pass's, so the pairs are the same.  Scratch:
 // the box records' two unused words (8k + 7: cell counts, then offsets;
-// 8p + 6: bodies by cell, then the large ones), S.fill (cell cursors), S.deg
+// 8p + 6: bodies by cell, then the large ones), S.deg
 // (1: the row is a sorted list of its partners, else bit words) and the team
 // words tf[0, 4) and tf[23, 31).  False, before writing any row, when the
 // grid does not apply (no small body, non-finite boxes) or would not pay
@@ -346,18 +346,31 @@ struct PairGrid {
 // Kept out of line: inlined into the step kernel, this compiler (ROCm 7.2,
 // gfx950) fails instruction selection ("Operand has incorrect register
 // class" on a compare with the LDS aperture).
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// A box record from LDS (out of line, the pointer's address space is
+// explicit so that the reads are LDS instructions, not flat ones).
+__device__ __forceinline__ float4 lds_lo(const lds_f32 *box, uint32_t i) {
+    const lds_f32 *p = box + 8u * i;
+    return make_float4(p[0], p[1], p[2], p[3]);
+}
+__device__ __forceinline__ float2 lds_hi(const lds_f32 *box, uint32_t i) {
+    const lds_f32 *p = box + 8u * i + 4u;
+    return make_float2(p[0], p[1]);
+}
+
 template <class Team>
-__device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, uint32_t n, uint32_t W, const SceneScratch &S, uint32_t *tf) {
+__device__ __attribute__((noinline)) bool grid_rows(const Team &tm, lds_f32 *box, uint32_t n, uint32_t W,
+                                                    const SceneScratch &S, lds_u32 *tf) {
     constexpr uint32_t NT = Team::kSize;
     const uint32_t r = tm.rank;
-    const float4 *b4 = reinterpret_cast<const float4 *>(box);
-    const float2 *b2 = reinterpret_cast<const float2 *>(box);
-    uint32_t *bu = reinterpret_cast<uint32_t *>(box);
+    lds_u32 *bu = reinterpret_cast<lds_u32 *>(box);
     // 1. the large-box threshold: 4 x the mean of the x and z extents
     float se = 0.f;
     for (uint32_t i = r; i < n; i += NT) {
-        const float4 lo = b4[2 * i];
-        const float2 hi = b2[4 * i + 2];
+        const float4 lo = lds_lo(box, i);
+        const float2 hi = lds_hi(box, i);
         se += (lo.w - lo.x) + (hi.y - lo.z);
     }
     se = wave_sum_f32(se);
@@ -371,8 +384,8 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
     uint32_t k[6] = {0u, 0u, 0u, 0u, 0u, 0u};  // Ex, Ez, -min x, max x, -min z, max z (keys)
     uint32_t ns = 0;
     for (uint32_t i = r; i < n; i += NT) {
-        const float4 lo = b4[2 * i];
-        const float2 hi = b2[4 * i + 2];
+        const float4 lo = lds_lo(box, i);
+        const float2 hi = lds_hi(box, i);
         const float ex = lo.w - lo.x, ez = hi.y - lo.z;
         if (!small(ex, ez)) continue;
         k[0] = max(k[0], fkey(ex));
@@ -389,8 +402,8 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
     for (int d = 32; d >= 1; d >>= 1) ns += __shfl_xor(ns, d, 64);
     if (lane_id() == 0) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) atomicMax(&tf[23 + j], k[j]);
-        atomicAdd(&tf[29], ns);
+        for (int j = 0; j < 6; ++j) atomicMax((uint32_t *)&tf[23 + j], k[j]);
+        atomicAdd((uint32_t *)&tf[29], ns);
     }
     tm.sync();
     ns = tf[29];
@@ -420,30 +433,36 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
     // row would test otherwise (about n / 2): 9 ns / K <= n / 8.  A compact
     // pile spans few cells and tests every pair.
     if (72ull * ns > static_cast<uint64_t>(n) * K) return false;
-    // 3. counting sort of the small bodies by cell; the large ones follow
-    for (uint32_t c = r; c <= K; c += NT) {
-        bu[8 * c + 7] = 0u;
-        S.fill[c] = 0u;
-    }
+    // 3. counting sort of the small bodies by cell (a body's place in its
+    //    cell is the count its histogram increment returned, kept in a
+    //    register: at most kBatchMaxBodies / NT bodies per thread); the large
+    //    ones follow
+    constexpr uint32_t kPer = kBatchMaxBodies / NT;
+    for (uint32_t c = r; c <= K; c += NT) bu[8 * c + 7] = 0u;
     tm.sync();
-    for (uint32_t i = r; i < n; i += NT) {
-        const float4 lo = b4[2 * i];
-        const float2 hi = b2[4 * i + 2];
-        if (small(lo.w - lo.x, hi.y - lo.z)) atomicAdd(&bu[8 * (g.cz(lo.z) * g.kx + g.cx(lo.x)) + 7], 1u);
-    }
-    tm.sync();
-    (void)team_scan(tm, bu + 7, K + 1u, 8u);  // exclusive; offset K = ns
-    tm.sync();
-    for (uint32_t i = r; i < n; i += NT) {
-        const float4 lo = b4[2 * i];
-        const float2 hi = b2[4 * i + 2];
-        uint32_t p;
+    uint32_t cell_of[kPer], place[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t i = r + j * NT;
+        cell_of[j] = ~0u;
+        place[j] = 0u;
+        if (i >= n) continue;
+        const float4 lo = lds_lo(box, i);
+        const float2 hi = lds_hi(box, i);
         if (small(lo.w - lo.x, hi.y - lo.z)) {
-            const uint32_t c = g.cz(lo.z) * g.kx + g.cx(lo.x);
-            p = bu[8 * c + 7] + atomicAdd(&S.fill[c], 1u);
-        } else {
-            p = ns + atomicAdd(&tf[30], 1u);
+            cell_of[j] = g.cz(lo.z) * g.kx + g.cx(lo.x);
+            place[j] = atomicAdd((uint32_t *)&bu[8 * cell_of[j] + 7], 1u);
         }
+    }
+    tm.sync();
+    (void)team_scan(tm, (uint32_t *)(bu + 7), K + 1u, 8u);  // exclusive; offset K = ns
+    tm.sync();
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t i = r + j * NT;
+        if (i >= n) continue;
+        const uint32_t p = cell_of[j] != ~0u ? bu[8 * cell_of[j] + 7] + place[j]
+                                             : ns + atomicAdd((uint32_t *)&tf[30], 1u);
         bu[8 * p + 6] = i;
     }
     tm.sync();
@@ -452,8 +471,8 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
     //    else as bit words; a large body's row is cleared here and filled by
     //    the whole team below.
     for (uint32_t a = r; a < n; a += NT) {
-        const float4 lo = b4[2 * a];
-        const float2 hi = b2[4 * a + 2];
+        const float4 lo = lds_lo(box, a);
+        const float2 hi = lds_hi(box, a);
         const float q[6] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
         const uint32_t w0 = (a + 1) >> 5;
         uint32_t *row = S.bits + static_cast<size_t>(a) * W;
@@ -468,15 +487,31 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
         }
         // visit(b) for every candidate partner b > a whose box touches a's
         auto each_hit = [&](auto visit) {
-            auto test = [&](uint32_t b) {
-                if (b > a && touch(q, b4[2 * b], b2[4 * b + 2])) visit(b);
+            // list positions [p, p1): four candidates' reads issued together
+            auto range = [&](uint32_t p, uint32_t p1) {
+                for (; p + 4u <= p1; p += 4u) {
+                    uint32_t bb[4];
+                    float4 lo4[4];
+                    float2 hi4[4];
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) bb[j] = bu[8 * (p + j) + 6];
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j) {
+                        lo4[j] = lds_lo(box, bb[j]);
+                        hi4[j] = lds_hi(box, bb[j]);
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < 4; ++j)
+                        if (bb[j] > a && touch(q, lo4[j], hi4[j])) visit(bb[j]);
+                }
+                for (; p < p1; ++p) {
+                    const uint32_t b = bu[8 * p + 6];
+                    if (b > a && touch(q, lds_lo(box, b), lds_hi(box, b))) visit(b);
+                }
             };
-            for (uint32_t z = z0; z <= z1; ++z) {
-                const uint32_t p1 = bu[8 * (z * g.kx + x1 + 1u) + 7];
-                for (uint32_t p = bu[8 * (z * g.kx + x0) + 7]; p < p1; ++p) test(bu[8 * p + 6]);
-            }
-            if (sm)
-                for (uint32_t p = ns; p < n; ++p) test(bu[8 * p + 6]);
+            for (uint32_t z = z0; z <= z1; ++z)
+                range(bu[8 * (z * g.kx + x0) + 7], bu[8 * (z * g.kx + x1 + 1u) + 7]);
+            if (sm) range(ns, n);
         };
         uint32_t h[kRowList];  // the first hits, ascending
         uint32_t cnt = 0;
@@ -510,12 +545,12 @@ __device__ __attribute__((noinline)) bool grid_rows(const Team &tm, float *box, 
     // 5. the rows of the large bodies, each by the whole team
     for (uint32_t p = ns; p < n; ++p) {
         const uint32_t a = bu[8 * p + 6];
-        const float4 lo = b4[2 * a];
-        const float2 hi = b2[4 * a + 2];
+        const float4 lo = lds_lo(box, a);
+        const float2 hi = lds_hi(box, a);
         const float q[6] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
         uint32_t cnt = 0;
         for (uint32_t b = a + 1u + r; b < n; b += NT) {
-            if (touch(q, b4[2 * b], b2[4 * b + 2])) {
+            if (touch(q, lds_lo(box, b), lds_hi(box, b))) {
                 atomicOr(&S.bits[static_cast<size_t>(a) * W + (b >> 5)], 1u << (b & 31u));
                 ++cnt;
             }
@@ -787,7 +822,8 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
         //    boxes instead (for a single wavefront's scenes, at most a few
         //    hundred bodies, testing every pair measured faster).
         bool gridded = false;
-        if constexpr (NT > 64) gridded = n >= kGridMinBodies && grid_rows(tm, box, n, W, S, tf);
+        if constexpr (NT > 64)
+            gridded = n >= kGridMinBodies && grid_rows(tm, (lds_f32 *)box, n, W, S, (lds_u32 *)tf);
         for (uint32_t base = 0; base < n && !gridded; base += NT) {
             const uint32_t a = base + r;
             const bool have = a < n;
