@@ -543,6 +543,11 @@ def main():
     if window:
         out["cpu_baseline"] = cpu_baseline(sc, snaps, starts, window)
         out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
+    # The other legs do not use the single-scene world: release it and its
+    # stream (a process's streams share HIP's few hardware queues).
+    del w, e0, e1
+    s.close()
+    del s
     if args.batched_frames > 0:
         out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
     if args.diff_frames > 0:
