@@ -704,6 +704,14 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
     return r;
 }
 
+// A record the step writes once and only the reverse sweep reads, much later:
+// a non-temporal store, so that it does not displace the step's working set.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void nt_store4(float4 *p, float a, float b, float c, float d) {
+    const v4f_t v = {a, b, c, d};
+    __builtin_nontemporal_store(v, reinterpret_cast<v4f_t *>(p));
+}
+
 // One scene for the frames of the launch.  RES: the scene's 20 body fields
 // live in the team's LDS region from the first substep to the last and are
 // written back at the end; otherwise the stages read and write the global
@@ -1012,13 +1020,13 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                             if (taped) {
                                 float4 *pr = reinterpret_cast<float4 *>(
                                     fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * Ct + j));
-                                pr[0] = make_float4(Ba.px, Ba.py, Ba.pz, Ba.qw);
-                                pr[1] = make_float4(Ba.qx, Ba.qy, Ba.qz, Ba.vx);
-                                pr[2] = make_float4(Ba.vy, Ba.vz, Ba.wx, Ba.wy);
-                                pr[3] = make_float4(Ba.wz, Bb.px, Bb.py, Bb.pz);
-                                pr[4] = make_float4(Bb.qw, Bb.qx, Bb.qy, Bb.qz);
-                                pr[5] = make_float4(Bb.vx, Bb.vy, Bb.vz, Bb.wx);
-                                pr[6] = make_float4(Bb.wy, Bb.wz, l, 0.f);
+                                nt_store4(pr + 0, Ba.px, Ba.py, Ba.pz, Ba.qw);
+                                nt_store4(pr + 1, Ba.qx, Ba.qy, Ba.qz, Ba.vx);
+                                nt_store4(pr + 2, Ba.vy, Ba.vz, Ba.wx, Ba.wy);
+                                nt_store4(pr + 3, Ba.wz, Bb.px, Bb.py, Bb.pz);
+                                nt_store4(pr + 4, Bb.qw, Bb.qx, Bb.qy, Bb.qz);
+                                nt_store4(pr + 5, Bb.vx, Bb.vy, Bb.vz, Bb.wx);
+                                nt_store4(pr + 6, Bb.wy, Bb.wz, l, 0.f);
                             }
                             if (project_contact(ci, Ba, Bb, A.mu, l, h)) {
                                 S.lam[k] = l;
