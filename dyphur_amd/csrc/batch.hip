@@ -89,6 +89,7 @@ struct BatchArgs {
 };
 
 constexpr uint32_t kNoScene = 0xffffffffu;
+constexpr uint32_t kBigThreads = 512;  // workgroup of the larger-LDS launch (8 wavefronts: <= 256 VGPRs each)
 constexpr uint32_t kUnitBlock = 1u;
 // LDS floats per body: a 32 B box record, plus the 20 float fields of the
 // body when the scene's state is held in LDS for the whole launch.
@@ -201,7 +202,7 @@ __device__ __forceinline__ void store_body(const dy_body_view &v, uint32_t i, co
 }
 
 // Workgroup sum of v and OR of f in one exchange (two barriers).  `wsum`
-// needs 12 words; every thread gets both totals.
+// needs 8 + waves words; every thread gets both totals.
 __device__ __forceinline__ uint32_t block_sum_or(uint32_t v, uint32_t f, uint32_t *wsum, uint32_t &f_all) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1657,6 +1658,12 @@ int batch_launch(dy_batch_s *b, hipStream_t s, const dy_body_view &bodies, const
         a.region_floats = u.region_floats;
         const bool res = residents[pass];
         int st = DY_ERR_INVALID_ARG;
+        // the largest LDS-resident scenes (one per workgroup, the launch's
+        // critical path) get 512 threads when the shape uses workgroup teams
+        if (passes[pass] == &sh.big && b->block > 64) {
+            DY_TRY_STATUS((launch_frames<kBigThreads, true>(a, u.n_units, u.lds_bytes, ls)));
+            continue;
+        }
         switch (b->block) {
             case 64: st = res ? launch_frames<64, true>(a, u.n_units, u.lds_bytes, ls)
                               : launch_frames<64, false>(a, u.n_units, u.lds_bytes, ls); break;
