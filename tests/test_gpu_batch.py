@@ -132,6 +132,21 @@ def test_batch_large_and_global_scenes_match_oracle(gpu, oracle, lds_limit):
         _compare_frame(batch, _run_oracle_frame(ref, scs), f)
 
 
+def test_batch_lds_limit_below_64k_reads_as_64k(gpu):
+    """A limit under 64 KiB is taken as 64 KiB: the same launch plan, so the
+    same state bits as an explicit 64 KiB after the same frames."""
+    rng = scenes.Pcg32(20261210)
+    scs = [scenes.random_mixed(700, seed=47, extent=(6, 5, 6)), scenes.pile_scene(20, rng)]
+    got = []
+    for limit in (1, 65536):
+        b = gpu.Batch(scs)
+        b.set_lds_limit(limit)
+        b.step(4)
+        got.append(b.state())
+    for k in STATE:
+        assert np.array_equal(bits(got[0][k]), bits(got[1][k])), k
+
+
 def test_batch_grid_pairs_match_oracle(gpu, oracle):
     """A sparse 600-body scene stepped by a workgroup finds its pairs on the
     x-z grid (the statistics count those substeps) and still equals the

@@ -156,6 +156,29 @@ def test_torch_autograd_rollout_matches_reverse_sweep(gpu, hm):
     assert abs(got - want) <= 1e-3 * abs(want), (got, want)
 
 
+def test_torch_rollout_with_substeps_equals_the_c_abi(gpu):
+    """autograd.Rollout passes substeps and the tape options through: its
+    gradients equal capi.Diff's over the same frames at 2 substeps."""
+    torch = pytest.importorskip("torch")
+    from dyphur_amd import autograd as ag
+    rng = scenes.Pcg32(20260006)
+    scs = [scenes.diff_mixed(n, rng, extent=(7.0, 7.0, 7.0), frames=4, n_substeps=2) for n in (30, 60)]
+    r = ag.Rollout(scs, 4, 2, n_substeps=2, tape_contacts_per_body=8.0)
+    v0, im0 = r.initial()
+    vel = torch.tensor(v0, dtype=torch.float64, requires_grad=True)
+    inv = torch.tensor(im0, dtype=torch.float64, requires_grad=True)
+    x = ag.rollout(vel, inv, r)
+    (x ** 2).sum().backward()
+    b = gpu.Batch(scs)
+    d = gpu.Diff(b, 4, 2, n_substeps=2, tape_contacts_per_body=8.0)
+    d.forward(4)
+    st = b.state()
+    g = d.backward({k: (2.0 * st[k].astype(np.float64)).astype(np.float32) for k in ("pos_x", "pos_y", "pos_z")})
+    assert np.array_equal(x.detach().numpy().astype(np.float32)[:, 1], st["pos_y"])
+    assert np.array_equal(vel.grad.numpy().astype(np.float32)[:, 2], g["vel_z"])
+    assert np.array_equal(inv.grad.numpy().astype(np.float32), g["inv_mass"])
+
+
 # ---- at scale: substeps, a scene reversed from global memory, tape chunks ----
 def _mixed_batch(frames):
     """Three scenes of mixed boxes and spheres at half scale (as the
