@@ -661,6 +661,7 @@ template <class T> DY_HD void iw_apply(const M3T<T> &M, const T I[6], const T v[
 // Adjoint of iw_apply for output adjoint ob: vbar += M D M^T ob, Mbar += ...
 template <class T> DY_HD void iw_adj(const M3T<T> &M, const T I[6], const T v[3], const T ob[3], T vbar[3],
                                      M3T<T> &Mbar) {
+#pragma clang fp contract(fast)  // adjoint-only arithmetic (see project_adj)
     T l[3], j[3], jb[3], lb[3];
     #pragma unroll
     for (int k = 0; k < 3; ++k) l[k] = M.m[0][k] * v[0] + M.m[1][k] * v[1] + M.m[2][k] * v[2];
@@ -685,6 +686,7 @@ template <class T> DY_HD void iw_adj(const M3T<T> &M, const T I[6], const T v[3]
 }
 // Adjoint of axes_t: qbar += dM/dq^T Mbar (q = w x y z).
 template <class T> DY_HD void axes_adj(const T q[4], const M3T<T> &B, T qb[4]) {
+#pragma clang fp contract(fast)  // adjoint-only arithmetic (see project_adj)
     const T w = q[0], x = q[1], y = q[2], z = q[3];
     const T two = T(2), four = T(4);
     qb[0] += two * (z * B.m[0][1] - y * B.m[0][2] - z * B.m[1][0] + x * B.m[1][2] + y * B.m[2][0] - x * B.m[2][1]);
@@ -707,6 +709,7 @@ template <class T> DY_HD void rot_apply(T q[4], const T w[3]) {
 }
 // Given q (before), w and the adjoint q1b of the result: adds to qb and wb.
 template <class T> DY_HD void rot_adj(const T q[4], const T w[3], const T q1b[4], T qb[4], T wb[3]) {
+#pragma clang fp contract(fast)  // adjoint-only arithmetic (see project_adj)
     const T h = T(0.5f);
     const T bw = q1b[0], bx = q1b[1], by = q1b[2], bz = q1b[3];
     qb[0] += bw + h * (w[0] * bx + w[1] * by + w[2] * bz);
@@ -945,6 +948,12 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     }
 
     // ---- reverse ----
+    // Contracted (fused multiply-adds): the forward replay above must repeat
+    // the forward's bits, so that every branch is the forward's; the
+    // adjoint's own rounding is pinned by no reference and only shortens the
+    // dependent chain.
+    {
+#pragma clang fp contract(fast)
     T rab[3] = {T(0), T(0), T(0)}, rbb[3] = {T(0), T(0), T(0)};
     T cab[3] = {T(0), T(0), T(0)}, cbb[3] = {T(0), T(0), T(0)};
     T wbar = T(0), dlb = T(0), remb = T(0);
@@ -1139,6 +1148,7 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     // rem = depth - lam0
     cb.depth += remb;
     lamb += -remb;
+    }
 }
 
 }  // namespace dy
