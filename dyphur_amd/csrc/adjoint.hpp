@@ -954,6 +954,8 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     // dependent chain.
     {
 #pragma clang fp contract(fast)
+    // one reciprocal per denominator (the forward replay divides exactly)
+    const T rw = T(1) / w;
     T rab[3] = {T(0), T(0), T(0)}, rbb[3] = {T(0), T(0), T(0)};
     T cab[3] = {T(0), T(0), T(0)}, cbb[3] = {T(0), T(0), T(0)};
     T wbar = T(0), dlb = T(0), remb = T(0);
@@ -983,11 +985,12 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         if (cap) jcapb = mu * jtb;
         else jt0b = jtb;
         // jcap = lam1 / (dt w)
-        lam1b += jcapb / (dt * w);
-        wbar += -jcapb * jcap / w;
+        lam1b += jcapb * rw / dt;
+        wbar += -jcapb * jcap * rw;
         // jt0 = tl / wt
-        T tlb = jt0b / wt;
-        const T wtb = -jt0b * jt0 / wt;
+        const T rwt = T(1) / wt;
+        T tlb = jt0b * rwt;
+        const T wtb = -jt0b * jt0 * rwt;
         // wt = wa + wb + da.fa + db.fb
         Ab.im += wtb;
         Bb.im += wtb;
@@ -1010,11 +1013,12 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         // t = t0 / tl
         T t0b[3];
         T s = T(0);
+        const T rtl = T(1) / tl;
         #pragma unroll
-        for (int k = 0; k < 3; ++k) { t0b[k] = tb[k] / tl; s += tb[k] * t0[k]; }
-        tlb += -s / (tl * tl);
+        for (int k = 0; k < 3; ++k) { t0b[k] = tb[k] * rtl; s += tb[k] * t0[k]; }
+        tlb += -s * rtl * rtl;
         // tl = sqrt(t2); t2 = t0.t0
-        const T t2b = tlb / (T(2) * tl);
+        const T t2b = tlb * (T(0.5f) * rtl);
         #pragma unroll
         for (int k = 0; k < 3; ++k) t0b[k] += T(2) * t2b * t0[k];
         // t0 = u - un n
@@ -1052,8 +1056,8 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
         iw_adj(MA1, A0.I, ca, gab, cab, MA1b);
         iw_adj(MB1, B0.I, cbv, gbb, cbb, MB1b);
         // jn = -vn / w
-        const T vnb = -jnb / w;
-        wbar += jnb * vn / (w * w);
+        const T vnb = -jnb * rw;
+        wbar += jnb * vn * rw * rw;
         // vn = rv . n; rv = (vA + wA x ra) - (vB + wB x rb)
         T rvb[3], tmp[3];
         #pragma unroll
@@ -1114,8 +1118,8 @@ DY_HD void project_adj(const ContactT<T> &c, const BodyT<T> &A0, const BodyT<T> 
     // lam1 = lam0 + rem; dl = rem / w
     lamb = lam1b;
     remb += lam1b;
-    remb += dlb / w;
-    wbar += -dlb * dl / w;
+    remb += dlb * rw;
+    wbar += -dlb * dl * rw;
     // w = wa + wb + ca.ka + cb.kb
     Ab.im += wbar;
     Bb.im += wbar;
