@@ -359,6 +359,7 @@ int dy_batch_get_shape(dy_batch_t b, uint32_t *threads, uint32_t *wave_scene_max
  * fit the limit run one per workgroup in a launch of their own; the rest keep
  * their state in global memory.  The reverse sweep of a dy_diff_t created
  * afterwards is planned with the same limit.  Results do not depend on it.
+ * Blocking when the limit changes (waits for the device, then re-plans).
  * No counterpart in the reference (its scenes never leave global memory). */
 int dy_batch_set_lds_limit(dy_batch_t b, uint32_t bytes);
 /* n_frames frames of every scene in ONE launch; each frame is n_substeps
