@@ -118,9 +118,10 @@ __host__ __device__ inline uint64_t scene_scratch_words(uint32_t n, uint32_t c) 
 // (iters x 4 x n), a header (contacts solved, classes), class offsets
 // (C + 1), contacts in class order (8 floats: p, n, depth; and the body pair),
 // and each contact's inputs at each iteration (28 floats: both bodies' 13
-// mutable fields, lambda).
+// mutable fields, lambda; written only for a live projection, see below),
+// and each contact's lambda at each iteration.
 struct TapeLayout {
-    uint64_t st, po, qr, hdr, co, rec, ab, pre, frame;
+    uint64_t st, po, qr, hdr, co, rec, ab, pre, lam, frame;
 };
 
 __host__ __device__ inline TapeLayout tape_layout(uint32_t n, uint32_t c, uint32_t iters) {
@@ -134,6 +135,7 @@ __host__ __device__ inline TapeLayout tape_layout(uint32_t n, uint32_t c, uint32
     L.rec = o; o += 8ull * c;
     L.ab = o; o = (o + 2ull * c + 3ull) & ~3ull;
     L.pre = o; o += 28ull * c * iters;
+    L.lam = o; o = (o + static_cast<uint64_t>(c) * iters + 3ull) & ~3ull;
     L.frame = o;
     return L;
 }
@@ -705,6 +707,13 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
     return r;
 }
 
+// The projection's own first test (xpbd_solver.cpp:413-521, project_contact):
+// a contact with depth <= 0, or whose depth lambda already covers, changes
+// nothing.  Same float operations, so the same decision.
+__device__ __forceinline__ bool contact_live(float depth, float lam) {
+    return !(depth <= 0.f) && !(depth - lam <= 1e-6f);
+}
+
 // A record the step writes once and only the reverse sweep reads, much later:
 // a non-temporal store, so that it does not displace the step's working set.
 typedef float v4f_t __attribute__((ext_vector_type(4)));
@@ -1016,8 +1025,13 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                             const uint32_t a = cv.body_a[k], b = cv.body_b[k];
                             const ContactIn ci{cv.pos_x[k], cv.pos_y[k], cv.pos_z[k], cv.norm_x[k],
                                                cv.norm_y[k], cv.norm_z[k], cv.depth[k]};
-                            BodyState Ba = load_body(bv, a), Bb = load_body(bv, b);
                             float l = S.lam[k];
+                            if (taped) fb[TL.lam + static_cast<uint64_t>(it) * Ct + j] = l;
+                            // project_contact returns at once, changing nothing, unless the
+                            // contact is live (its depth not yet covered by lambda: after its
+                            // first projection lambda == depth for the rest of the substep)
+                            if (!contact_live(ci.depth, l)) continue;
+                            BodyState Ba = load_body(bv, a), Bb = load_body(bv, b);
                             if (taped) {
                                 float4 *pr = reinterpret_cast<float4 *>(
                                     fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * Ct + j));
@@ -1175,21 +1189,26 @@ struct DiffArgs {
 // The tape's inputs of one contact projection, loaded one phase ahead.
 struct PreRec {
     uint32_t j, a, b;
+    bool live;    // the projection changed something (else its adjoint is zero)
     float4 r0, r1;
-    float4 m[7];  // both bodies' 13 fields and lambda (28 floats)
+    float4 m[7];  // both bodies' 13 fields and lambda (28 floats), live only
 };
 
 __device__ __forceinline__ void pre_fetch(PreRec &P, const float *fb, const TapeLayout &TL, const uint32_t *co,
                                           uint32_t C, uint32_t it, uint32_t c, uint32_t r) {
     P.j = co[c] + r;
+    P.live = false;
     if (P.j >= co[c + 1]) return;
-    const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
     const float4 *rec = reinterpret_cast<const float4 *>(fb + TL.rec);
-    const float4 *pr = reinterpret_cast<const float4 *>(fb + TL.pre + 28ull * (static_cast<uint64_t>(it) * C + P.j));
+    const uint64_t at = static_cast<uint64_t>(it) * C + P.j;
+    P.r1 = rec[2 * P.j + 1];
+    P.live = contact_live(P.r1.z, fb[TL.lam + at]);
+    if (!P.live) return;
+    const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
+    const float4 *pr = reinterpret_cast<const float4 *>(fb + TL.pre + 28ull * at);
     P.a = ab[2 * P.j];
     P.b = ab[2 * P.j + 1];
     P.r0 = rec[2 * P.j];
-    P.r1 = rec[2 * P.j + 1];
 #pragma unroll
     for (int k = 0; k < 7; ++k) P.m[k] = pr[k];
 }
@@ -1330,14 +1349,15 @@ __device__ void reverse_scene(const Team &tm, const BatchArgs &A, const DiffArgs
                         pre_fetch(nxt, fb, TL, co, C, c > 0 ? static_cast<uint32_t>(it) : static_cast<uint32_t>(it - 1),
                                   c > 0 ? static_cast<uint32_t>(c - 1) : ncls - 1u, r);
                     const uint32_t end = co[c + 1];
-                    if (cur.j < end)
+                    if (cur.j < end && cur.live)
                         contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
                                           reinterpret_cast<const float *>(cur.m), acc_of(cur.j));
                     for (uint32_t j = cur.j + NT; j < end; j += NT) {  // classes larger than the team
                         PreRec x;
                         pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(it), static_cast<uint32_t>(c), r + (j - cur.j));
-                        contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1, reinterpret_cast<const float *>(x.m),
-                                          acc_of(j));
+                        if (x.live)
+                            contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1,
+                                              reinterpret_cast<const float *>(x.m), acc_of(j));
                     }
                     tm.sync();
                     if (more) cur = nxt;
