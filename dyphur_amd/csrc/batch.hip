@@ -707,13 +707,6 @@ __device__ uint32_t assign_rounds(const Team &tm, uint32_t nc, bool jp, const Sc
     return r;
 }
 
-// The projection's own first test (xpbd_solver.cpp:413-521, project_contact):
-// a contact with depth <= 0, or whose depth lambda already covers, changes
-// nothing.  Same float operations, so the same decision.
-__device__ __forceinline__ bool contact_live(float depth, float lam) {
-    return !(depth <= 0.f) && !(depth - lam <= 1e-6f);
-}
-
 // A record the step writes once and only the reverse sweep reads, much later:
 // a non-temporal store, so that it does not displace the step's working set.
 typedef float v4f_t __attribute__((ext_vector_type(4)));

@@ -398,6 +398,12 @@ struct ContactIn { float px, py, pz, nx, ny, nz, depth; };
 // One contact projection: position, inelastic normal velocity, Coulomb
 // friction.  lam is the contact's accumulated depth; returns true when the
 // bodies were modified.
+// project_contact's own first test: a contact with depth <= 0, or whose depth
+// lambda already covers (after its first projection in a substep, lambda ==
+// depth), changes nothing.  Callers may skip such a contact's loads and
+// stores; the operations are project_contact's, so the decision is its.
+DY_HD inline bool contact_live(float depth, float lam) { return !(depth <= 0.f) && !(depth - lam <= 1e-6f); }
+
 DY_HD bool project_contact(const ContactIn &c, BodyState &A, BodyState &B, float mu, float &lam,
                            float dt) {
     if (c.depth <= 0.f) return false;

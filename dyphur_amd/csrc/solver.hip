@@ -1012,7 +1012,11 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
                 if (todo) {
                     const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
                                        (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
-                    if (ready) {
+                    // a contact that changes nothing (contact_live: every contact after
+                    // its first projection) moves its bodies' versions only; their
+                    // deferred renormalisations are applied by the next live record
+                    const bool live = (JOINTS && __float_as_uint(p1.w) == kRecJoint) || contact_live(p1.z, l);
+                    if (ready && live) {
                         uint32_t ra, rb;
                         BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
                         const bool da = A.im != 0.f, db = B.im != 0.f;
@@ -1020,8 +1024,8 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
                         lam[i] = l;
                         if (da) store_aos(dyn, ab.x, A, it);
                         if (db) store_aos(dyn, ab.y, B, it);
-                        todo = false;
                     }
+                    if (ready) todo = false;
                 }
                 wait_vmem();  // body stores acknowledged by the L2 before the versions move
                 if (!todo && i < nc && sq.y != ~0u) {
