@@ -12,9 +12,10 @@
 //                      counters, workgroup scope); the last workgroup to finish
 //                      (agent-scope release/acquire around a ticket) merges the
 //                      few nodes whose ranges cross chunks.
-//   4. traverse_count  one thread per leaf in Morton order: stackless walk,
-//                      counting partners b > a and keeping the 16 smallest in
-//                      registers (sorted).
+//   4. traverse_count  one thread per leaf in Morton order, 64-thread
+//                      workgroups with contiguous leaf runs per XCD: stackless
+//                      walk, counting partners b > a and keeping the 16
+//                      smallest in registers (sorted).
 //   5. scan of counts  row offsets per body a; the total is the true pair count.
 //   6. emit_pairs      one thread per body: writes its row (a, b0 < b1 < ...);
 //                      rows longer than 16 re-walk the tree for the next 16.
@@ -30,6 +31,7 @@ constexpr uint32_t kRow = 16;            // partners kept in registers per walk
 constexpr uint32_t kOvf = 128;           // further partners kept per body in an overflow buffer
 constexpr uint32_t kChunk = 1024;        // leaves per refit workgroup
 constexpr uint32_t kMortonMax = 1u << 30;
+constexpr uint32_t kXcds = 8;           // XCDs of an MI355X; workgroups are dealt to them round-robin
 // A Karras tree over distinct 64-bit keys is at most 65 levels deep; walks
 // that exceed this bound flag an internal error instead of looping.
 constexpr int kMaxHeight = 96;
@@ -507,8 +509,16 @@ __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, co
 __global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ tn, int n,
                                                       uint32_t *__restrict__ counts, uint32_t *__restrict__ row,
                                                       uint32_t *__restrict__ ovf, uint32_t *__restrict__ leaf_of,
-                                                      uint32_t *err) {
-    const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+                                                      uint32_t *err, uint32_t xcd_map) {
+    // With xcd_map the launch's workgroups, which the hardware deals out to
+    // the 8 XCDs round-robin, take contiguous runs of leaves per XCD: walks of
+    // neighbouring leaves share subtrees, so each XCD's L2 holds one region.
+    uint32_t g = blockIdx.x;
+    if (xcd_map) {
+        const uint32_t G = gridDim.x, per = G / kXcds, rem = G % kXcds, x = g % kXcds, slot = g / kXcds;
+        g = x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+    }
+    const int k = static_cast<int>(g * blockDim.x + threadIdx.x);
     if (k >= n) return;
     const TNode leaf = tn[n - 1 + k];
     const uint32_t a = static_cast<uint32_t>(leaf.child);
@@ -595,6 +605,13 @@ struct dy_broadphase_s {
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
     uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
+    // traverse_count launch shape: 64-thread workgroups spread the walks over
+    // all CUs (256-thread ones left half of them idle at 32k bodies) and the
+    // XCD map keeps each XCD on one region of the tree; together 1.9 % off
+    // the 32k-body frame.  DYPHUR_TRAVERSE_THREADS / DYPHUR_TRAVERSE_XCD
+    // override them.
+    uint32_t trav_threads = 64;
+    uint32_t trav_xcd = 1;
 };
 
 namespace {
@@ -653,6 +670,11 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
             b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > per_cu * cus ? per_cu * cus : want));
         }
     }
+    if (const char *env = std::getenv("DYPHUR_TRAVERSE_THREADS")) {
+        const int t = std::atoi(env);
+        if (t == 64 || t == 128 || t == 256) b->trav_threads = static_cast<uint32_t>(t);
+    }
+    if (const char *env = std::getenv("DYPHUR_TRAVERSE_XCD")) b->trav_xcd = std::atoi(env) ? 1u : 0u;
     if (e == hipSuccess) e = hipMemsetAsync(b->root, 0, sizeof(int32_t), hs(s));
     if (e != hipSuccess) {
         bp_free(b);
@@ -707,8 +729,8 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
                                                        bp->right, bp->range, bp->nodebox, bp->tn, bp->cnt_top,
                                                        bp->entries, bp->counters);
     DY_LAUNCH_CHECK();
-    traverse_count<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of,
-                                                     bp->counters + 2);
+    traverse_count<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(
+        bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
     DY_LAUNCH_CHECK();
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
     emit_pairs<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
