@@ -605,7 +605,8 @@ struct dy_broadphase_s {
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
     uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
-    // traverse_count launch shape: 64-thread workgroups spread the walks over
+    // traverse_count launch shape (emit_pairs uses the same workgroup size):
+    // 64-thread workgroups spread the walks over
     // all CUs (256-thread ones left half of them idle at 32k bodies) and the
     // XCD map keeps each XCD on one region of the tree; together 1.9 % off
     // the 32k-body frame.  DYPHUR_TRAVERSE_THREADS / DYPHUR_TRAVERSE_XCD
@@ -733,7 +734,7 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
         bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
     DY_LAUNCH_CHECK();
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
-    emit_pairs<<<ceil_div(n, 256), 256, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
+    emit_pairs<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
                                                  bp->pairs, bp->max_pairs, bp->counters + 2);
     DY_LAUNCH_CHECK();
     return DY_OK;
