@@ -1535,8 +1535,13 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
         AssignBufs B{cs, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
                      x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill};
         void *args[] = {&B};
-        // Participants on one XCD out of a launch with as many on every XCD.
-        uint32_t blocks = 8u * coop_blocks(x, x->assign_blocks_override, x->assign_threads, x->coop_assign_max);
+        // Participants on one XCD out of a launch with as many on every XCD:
+        // by default 16 workgroups of 256 threads per XCD on a 256-CU device,
+        // half the solver's usual share.  On the 32k-body frame 16 beat 8, 24
+        // and 32 (fewer workgroups meet at each round barrier).
+        const int assign_dflt = (x->device_cus / 16) * 256 / x->assign_threads;
+        uint32_t blocks = 8u * coop_blocks(x, x->assign_blocks_override > 0 ? x->assign_blocks_override : assign_dflt,
+                                           x->assign_threads, x->coop_assign_max);
         if (blocks > static_cast<uint32_t>(x->coop_assign_max)) blocks = static_cast<uint32_t>(x->coop_assign_max);
         if (try_coop_launch(reinterpret_cast<const void *>(assign_lists), dim3(blocks),
                             dim3(static_cast<uint32_t>(x->assign_threads)), args, st) != hipSuccess) {
