@@ -5,24 +5,32 @@ Workload: a single scene of 32 768 dynamic bodies (2/3 boxes with half extents
 U[0.1, 0.5] m, 1/3 spheres with radius U[0.1, 0.4] m, uniform positions in a
 30 x 40 x 30 m volume, uniform random orientations, density 1000 kg/m^3, seed
 20260002) over a static ground plate, FP32, 60 Hz, 1 substep, 10 solver
-iterations.  A step is one frame: integrate -> LBVH broadphase -> pairs ->
-narrowphase -> XPBD solve, through the C ABI in the reference's stress-driver
-order (examples/stress_test_blocks/main.cpp:264-273), using the async-capable
-Narrowphase::run overload so that no frame waits on the host.  An untimed
-replay with per-frame synchronisation then collects the load statistics and
-must end in the same state.
+iterations, 600 frames.  A step is one frame: integrate -> LBVH broadphase ->
+pairs -> narrowphase -> XPBD solve, through the C ABI in the reference's
+stress-driver order (examples/stress_test_blocks/main.cpp:264-273), using the
+async-capable Narrowphase::run overload so that no frame waits on the host.
+The timed frames are the last --steps frames of the 600-frame run (the
+frames before them run untimed), so a short run times the settled pile.  An
+untimed pass times every kernel over the same frames (HIP events; the
+roofline names the slowest), and an untimed replay of the whole run collects
+the load statistics and must end in the same state.
 
 value = body-substeps/s = bodies (ground included) x frames / seconds over the
 timed frames, with all inputs resident in HBM (no host<->device state copies
 in the timed region).
 
-The line also carries "batched": the 4096-scene pile workload (seed 20260003)
-stepped 300 frames in one launch, its body-substeps/s and the plain per-scene
-loop of the drop-in API on a sample of the same scenes; and
-"differentiable": the 256-scene thrown-stack workload (seed 20260004), its
-forward and forward + adjoint times per frame; "differentiable_at_scale": the
-same for 1 024 mixed scenes of up to 1 024 bodies at 4 substeps per frame
-(seed 20260005); and "articulated": the joint-chain workload.
+The line also carries "stress_c1": the reference's own stress_test_blocks
+scene (1 024 boxes, 300 frames) on the GPU with and without the reference's
+trajectory I/O, the CPU oracle at all host threads and at 4 (the reference
+publishes 205.5 fps on 4 threads), the largest penetration after each solve
+and the settled kinetic energy against the oracle's, plus the 0.85 m
+variant; "batched": the 4096-scene pile workload (seed 20260003) stepped 300
+frames in one launch, its body-substeps/s and the plain per-scene loop of the
+drop-in API on a sample of the same scenes; "differentiable": the 256-scene
+thrown-stack workload (seed 20260004), its forward and forward + adjoint
+times per frame; "differentiable_at_scale": the same for 1 024 mixed scenes
+of up to 1 024 bodies at 4 substeps per frame (seed 20260005); and
+"articulated": the joint-chain workload.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
@@ -43,10 +51,15 @@ METRIC = "body-substeps/s on 1 MI355X: plain vs batched; fwd+adjoint ms/frame; H
 HBM_PEAK_GBS = 8000.0
 
 
+C2_FRAMES = 600  # frames of the 32 768-body workload
+C1_FRAMES = 300  # frames of the stress stack (examples/stress_test_blocks/main.cpp:52)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=600,
+                    help="timed frames: the last STEPS frames of the 600-frame 32 768-body run")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--bodies", type=int, default=32768)
     ap.add_argument("--cpu-window", type=int, default=20,
@@ -58,6 +71,8 @@ def parse():
                     help="frames of the differentiable 256-scene workload (0 skips it)")
     ap.add_argument("--diff-scale-frames", type=int, default=240,
                     help="frames of the 1 024-scene differentiable leg (0 skips it)")
+    ap.add_argument("--c1-frames", type=int, default=C1_FRAMES,
+                    help="frames of the stress_test_blocks leg (0 skips it)")
     ap.add_argument("--joint-frames", type=int, default=120,
                     help="frames of the articulated workload (0 skips it)")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -93,51 +108,85 @@ def traffic_from_profiles(kernel: str):
     return d.get("hbm_bytes_per_launch")
 
 
-def replay(w, init, s, steps, want):
-    """Untimed re-run of the same frames with per-frame statistics (one sync
-    per frame) and state snapshots at the frames in `want`."""
+def plan_frames(steps: int, config_frames: int, cpu_window: int, n_windows: int = 4) -> dict:
+    """Which frames the single-scene leg times and where the CPU windows sit.
+
+    The run is the workload's config_frames frames (more if steps asks for
+    more); the timed frames are its last `steps` frames, so a short run times
+    the settled pile rather than the free fall at the start.  The CPU baseline
+    runs n_windows windows of `window` frames spread over the whole run, each
+    from the GPU state at its first frame; every window ends inside the run."""
+    steps = max(int(steps), 1)
+    total = max(int(config_frames), steps)
+    window = max(0, min(int(cpu_window), total // n_windows))
+    starts = []
+    if window:
+        span = total - window
+        starts = sorted({(k * span) // (n_windows - 1) for k in range(n_windows)}) if n_windows > 1 else [span]
+    return {"total": total, "start": total - steps, "steps": steps, "window": window, "starts": starts}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def replay(w, init, s, total, want):
+    """Untimed re-run of frames 0..total-1 from the initial state with
+    per-frame statistics (host synchronisation every frame) and state
+    snapshots at the frames in `want`."""
     w.bodies.upload_state(init, s)
     s.wait()
     stats, snaps = [], {}
-    for f in range(steps + 1):
+    for f in range(total + 1):
         if f in want:
             snaps[f] = (w.bodies.download(s), w.fnv())
-        if f < steps:
+        if f < total:
             stats.append(w.step())
-    return stats, snaps, w.fnv()
+    return stats, snaps
 
 
-def cpu_baseline(sc, snaps, starts, window):
+def cpu_baseline(sc, snaps, starts, window, threads_alt: int = 4):
     """Time the CPU oracle (the reference algorithm restated in C++/OpenMP) on
     windows of `window` frames, each starting from the GPU trajectory's state
     at that frame (bit-identical to the oracle's own trajectory), so the
     sample covers both the falling and the settled phases without running
     every frame on the CPU.  The oracle's state hash at each window end is
-    compared with the GPU's."""
+    compared with the GPU's.  The last window is timed again on
+    `threads_alt` threads."""
     from oracle import ref_cpu as rc
 
+    rc.build()
     threads = rc.max_threads()
     ob, osh = rc.Bodies(sc), rc.Shapes(sc)
     ow = rc.World.for_scene(sc)
-    total, frames, equal = 0.0, 0, True
-    for f0 in starts:
+
+    def run(f0):
         st, _ = snaps[f0]
         for k in rc.BODY_FIELDS:
             ob.arr[k][:] = st[k]
         t0 = time.perf_counter()
         for _ in range(window):
             ow.frame(ob, osh, sc)
-        total += time.perf_counter() - t0
+        return time.perf_counter() - t0, ob.fnv() == snaps[f0 + window][1]
+
+    total, frames, equal = 0.0, 0, True
+    for f0 in starts:
+        dt, eq = run(f0)
+        total += dt
         frames += window
-        equal = equal and (ob.fnv() == snaps[f0 + window][1])
-    cpu_model = ""
+        equal = equal and eq
+    rc.set_threads(threads_alt)
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+        alt_s, eq = run(starts[-1])
+    finally:
+        rc.set_threads(threads)
+    equal = equal and eq
     return {
         "value": round(sc.n * frames * sc.n_substeps / total, 1),
         "unit": "body-substeps/s",
@@ -145,10 +194,256 @@ def cpu_baseline(sc, snaps, starts, window):
         "kind": "port",
         "sample": f"{frames} frames of the same workload in {len(starts)} windows of {window} starting at frames "
                   f"{starts}, each from the GPU state at that frame; {total:.1f} s on {threads} OpenMP "
-                  f"threads ({cpu_model}); oracle state hash equal to the GPU at every window end: {equal}",
+                  f"threads ({cpu_model()}); oracle state hash equal to the GPU at every window end: {equal}",
         "seconds": round(total, 2),
         "fnv_equal": equal,
+        f"value_{threads_alt}_threads": round(sc.n * window * sc.n_substeps / alt_s, 1),
+        f"sample_{threads_alt}_threads": f"the window at frame {starts[-1]} on {threads_alt} OpenMP threads "
+                                         f"({alt_s:.1f} s)",
     }
+
+
+def stage_kernels(capi, w, sc, s, n_frames: int) -> dict:
+    """Per-kernel time and algorithmic bytes over n_frames frames from the
+    world's current state (untimed: the stage events and the solver's kernel
+    timing wait on the host every frame).  Returns {kernel: (ms per launch,
+    bytes per launch)}; the broadphase's launches come from its own kernel
+    timing, integrate and the narrowphase from events around their calls."""
+    n = sc.n
+    n_static = int(np.count_nonzero(sc.flags & 1))
+    e = [capi.Event() for _ in range(4)]
+    w.bp.set_timing(True)
+    w.solver.set_timing(True)
+    integ = narrow = 0.0
+    P = C = 0.0
+    try:
+        for _ in range(n_frames):
+            e[0].record(s)
+            capi.integrate(s, w.bodies, sc.gravity, sc.dt, sc.n_substeps)
+            e[1].record(s)
+            w.bp.build_and_query(s, w.bodies, w.shapes, sc.bounds)
+            pairs, count = w.bp.pairs_ptr()
+            e[2].record(s)
+            w.np.run_device_count(s, pairs, count, w.bp.max_pairs, w.bodies, w.shapes)
+            e[3].record(s)
+            w.solver.solve(s, w.np.contacts_view(), w.bodies, sc.dt, w.joints)
+            s.wait()
+            integ += e[0].elapsed_ms(e[1])
+            narrow += e[2].elapsed_ms(e[3])
+            P += min(w.bp.download_count(s), w.bp.max_pairs)
+            C += w.np.download_count(s)
+        bt = w.bp.timing()
+        tm = w.solver.timing(s)
+    finally:
+        w.bp.set_timing(False)
+        w.solver.set_timing(False)
+    f = float(n_frames)
+    P, C = P / f, C / f
+    nb = float(max(bt["builds"], 1))
+    ns = float(max(tm["solves"], 1))
+    return {
+        "integrate_kernel": (integ / f, 100.0 * (n - n_static) + 4.0 * n_static),
+        "morton_sort_grid": (bt["sort_ms"] / nb, 48.0 * n),
+        "karras_build": (bt["karras_ms"] / nb, 24.0 * n),
+        "leaf_refit": (bt["refit_ms"] / nb, 108.0 * n),
+        "traverse_count": (bt["traverse_ms"] / nb, 52.0 * n),
+        "emit_pairs": (bt["emit_ms"] / nb, 8.0 * P),
+        "narrowphase": (narrow / f, 84.0 * P + 36.0 * C),
+        "assign_lists": (tm["assign_ms"] / ns, 12.0 * C + 16.0 * n),
+        "sweep_flow": (tm["sweep_ms"] / ns, tm["sweep_bytes"] / ns),
+    }
+
+
+def roofline_of(kernels: dict) -> dict:
+    """The dominant kernel (largest time per frame) with its achieved
+    algorithmic GB/s against the HBM peak, and the whole frame's."""
+    name = max(kernels, key=lambda k: kernels[k][0])
+    ms, nbytes = kernels[name]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    tot_ms = sum(v[0] for v in kernels.values())
+    tot_b = sum(v[1] for v in kernels.values())
+    return {
+        "bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic_from_profiles(name),
+        "avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": round(nbytes),
+        "frame": {"kernel_ms": round(tot_ms, 4), "algorithmic_bytes": round(tot_b),
+                  "achieved": round(tot_b / (tot_ms * 1e-3) / 1e9, 2) if tot_ms > 0 else 0.0},
+        "kernels": {k: {"avg_launch_ms": round(v[0], 4), "algorithmic_bytes_per_launch": round(v[1]),
+                        "achieved": round(v[1] / (v[0] * 1e-3) / 1e9, 2) if v[0] > 0 else 0.0}
+                    for k, v in kernels.items()},
+    }
+
+
+def kinetic_energy(st: dict, sc) -> float:
+    """Sum over dynamic bodies of m|v|^2/2 + w.(R I_body R^T w)/2 (float64)."""
+    dyn = (sc.flags & 1) == 0
+    m = sc.mass.astype(np.float64)[dyn]
+    v = np.stack([st[k] for k in ("vel_x", "vel_y", "vel_z")], 1).astype(np.float64)[dyn]
+    w = np.stack([st[k] for k in ("ang_x", "ang_y", "ang_z")], 1).astype(np.float64)[dyn]
+    q = np.stack([st[k] for k in ("rot_w", "rot_x", "rot_y", "rot_z")], 1).astype(np.float64)[dyn]
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    a, b, c, d = q.T
+    R = np.stack([1 - 2 * (c * c + d * d), 2 * (b * c - a * d), 2 * (b * d + a * c),
+                  2 * (b * c + a * d), 1 - 2 * (b * b + d * d), 2 * (c * d - a * b),
+                  2 * (b * d - a * c), 2 * (c * d + a * b), 1 - 2 * (b * b + c * c)], 1).reshape(-1, 3, 3)
+    I = sc.inertia.astype(np.float64)[dyn].reshape(-1, 3, 3)
+    wb = np.einsum("nji,nj->ni", R, w)          # body-frame angular velocity R^T w
+    rot = 0.5 * np.einsum("ni,nij,nj->n", wb, I, wb)
+    return float((0.5 * m * (v * v).sum(1) + rot).sum())
+
+
+def run_driver(binary: str, prefix: str, frames: int, pitch: float | None = None) -> dict | None:
+    """The C++ drop-in stress driver (the reference's main loop: a pair-count
+    download every frame and a 7-array state download + trajectory write
+    every second frame) as a child process; its fps and final state hash."""
+    import subprocess
+    cmd = [binary, prefix, str(frames)] + (["--pitch", str(pitch)] if pitch is not None else [])
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
+    except (OSError, subprocess.TimeoutExpired) as ex:
+        return {"error": str(ex)}
+    if r.returncode != 0:
+        return {"error": f"rc {r.returncode}: {r.stdout[-400:]}"}
+    try:
+        m = json.load(open(prefix + ".metrics.json"))
+        h = open(prefix + ".golden").read().strip()
+    except (OSError, ValueError) as ex:
+        return {"error": str(ex)}
+    return {"fps": m["frames_per_second"], "wall_s": m["wall_clock_seconds"], "state_fnv1a64": h,
+            "avg_contacts_last_frame": m.get("avg_contacts_per_frame")}
+
+
+def stress_c1(capi, scenes, drivers: dict, frames: int = C1_FRAMES, settle_frames: int = 30) -> dict:
+    """The reference's stress_test_blocks scene (1 024 boxes + ground, 1.0 m
+    pitch as shipped, 300 frames, 1 substep, 10 iterations): the GPU without
+    trajectory I/O (frames enqueued back to back) and the drop-in C++ driver
+    with the reference's I/O (measured before this process touched the GPU);
+    the CPU oracle on all host threads and on 4, without and with the same
+    I/O; the solver-quality gates (largest penetration after each frame's
+    solve, settled kinetic energy over the last frames against the oracle's)
+    and the 0.85 m labelled variant."""
+    import tempfile
+    from dyphur_amd import formats
+    from oracle import ref_cpu as rc
+
+    out = {"workload": f"stress_test_blocks: 8x16x8 boxes (half 0.4 m, 1 kg) on a 1.0 m pitch + static ground, "
+                       f"{frames} frames at 60 Hz, 1 substep, 10 iterations, friction 0.5",
+           "penetration_note": "deepest contact left after each frame's solve (a second broadphase + narrowphase "
+                               "over the solved state); the reference algorithm run by the oracle gives the same "
+                               "depth bit for bit every frame, so the 5 mm bound is the reference solver's own "
+                               "result on this scene",
+           "published_fps": {"cpu_openmp_4_threads_xeon_e5_2667v4": 205.5, "cuda_rtx3060_warm": 233.0}}
+    res = {}
+    for label, pitch in (("pitch_1.0", 1.0), ("pitch_0.85", 0.85)):
+        sc = scenes.stress_blocks(pitch=pitch)
+        s = capi.Stream()
+        w = capi.World(sc, s)
+        init = w.bodies.host_arrays(sc)
+        for _ in range(5):
+            w.step_async()
+        w.check()
+        w.bodies.upload_state(init, s)
+        s.wait()
+        e0, e1 = capi.Event(), capi.Event()
+        t0 = time.perf_counter()
+        e0.record(s)
+        for _ in range(frames):
+            w.step_async()
+        e1.record(s)
+        s.wait()
+        wall = time.perf_counter() - t0
+        gpu_ms = e0.elapsed_ms(e1)
+        w.check()
+        h_gpu = w.fnv()
+        # Gates: per frame, the deepest contact the narrowphase found before
+        # the solve and the deepest one left after it (a second broadphase +
+        # narrowphase over the solved state); kinetic energy over the last
+        # settle_frames frames.
+        w.bodies.upload_state(init, s)
+        s.wait()
+        pbp = capi.Broadphase(s, sc.n, sc.max_pairs)
+        pnp = capi.Narrowphase(s, sc.max_contacts)
+        pre, post, ke = [], [], []
+        for f in range(frames):
+            w.step()
+            c = w.np.contacts(s)
+            pre.append(float(c["depth"].max()) if c["n"] else 0.0)
+            pbp.build_and_query(s, w.bodies, w.shapes, sc.bounds)
+            n_p = pbp.download_count(s)
+            pp, _ = pbp.pairs_ptr()
+            pnp.run(s, pp, min(n_p, pbp.max_pairs), w.bodies, w.shapes)
+            c2 = pnp.contacts(s)
+            post.append(float(c2["depth"].max()) if c2["n"] else 0.0)
+            if f >= frames - settle_frames:
+                ke.append(kinetic_energy(w.state(), sc))
+        r = {"gpu_fps_no_io": round(frames / wall, 1), "gpu_stream_ms_per_frame": round(gpu_ms / frames, 4),
+             "gpu_body_substeps_per_s": round(sc.n * frames / wall, 1), "state_fnv1a64": f"{h_gpu:016x}",
+             "max_penetration_m": {"after_solve": round(max(post), 6), "before_solve": round(max(pre), 6),
+                                   "after_solve_last_frame": round(post[-1], 6)},
+             "gate_penetration_5mm": bool(max(post) <= 0.005),
+             "frames_deeper_than_5mm": int(sum(1 for x in post if x > 0.005)),
+             "settled_ke_j": float(np.mean(ke)), "ke_frames": f"{frames - settle_frames}..{frames - 1}"}
+        d = drivers.get(label)
+        if d:
+            r["driver_with_io"] = d
+            if "fps" in d:
+                r["driver_hash_equal"] = d["state_fnv1a64"] == f"{h_gpu:016x}"
+        # Reference order on the GPU (forced serial), its settled energy.
+        w.bodies.upload_state(init, s)
+        s.wait()
+        ke_s = []
+        for f in range(frames):
+            w.step(force_serial=True)
+            if f >= frames - settle_frames:
+                ke_s.append(kinetic_energy(w.state(), sc))
+        r["settled_ke_reference_order_j"] = float(np.mean(ke_s))
+        del pbp, pnp, w
+        s.close()
+        res[label] = (r, sc)
+    out.update({k: v[0] for k, v in res.items()})
+
+    # CPU oracle on the shipped scene: all threads and 4, without and with I/O.
+    r, sc = res["pitch_1.0"]
+    rc.build()
+    threads = rc.max_threads()
+    cpu = {"cores": threads, "cpu_model": cpu_model(), "kind": "port"}
+    tmp = tempfile.mkdtemp(prefix="dyphur_c1_")
+    for th in (threads, 4):
+        rc.set_threads(th)
+        for io in (False, True):
+            ob, osh = rc.Bodies(sc), rc.Shapes(sc)
+            ow = rc.World.for_scene(sc)
+            snap = None
+            tw = formats.TrajectoryWriter(os.path.join(tmp, "c1"), sc.n, frames) if io else None
+            t0 = time.perf_counter()
+            for f in range(frames):
+                ow.frame(ob, osh, sc)
+                if io and f % 2 == 0:
+                    tw.write_frame({k: ob.arr[k].copy() for k in
+                                    ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z")})
+                if f == frames - settle_frames - 1:
+                    snap = ob.copy()
+            dt = time.perf_counter() - t0
+            if tw:
+                tw.close()
+            key = f"fps_{th}_threads" + ("_with_io" if io else "")
+            cpu[key] = round(frames / dt, 1)
+            if th == threads and not io:
+                cpu["state_fnv1a64"] = f"{ob.fnv():016x}"
+                cpu["hash_equal_gpu"] = cpu["state_fnv1a64"] == r["state_fnv1a64"]
+                # the oracle's settled energy: the last frames again from the snapshot
+                kes = []
+                for _ in range(settle_frames):
+                    ow.frame(snap, osh, sc)
+                    kes.append(kinetic_energy(snap.state(), sc))
+                cpu["settled_ke_j"] = float(np.mean(kes))
+    rc.set_threads(threads)
+    cpu["body_substeps_per_s"] = round(sc.n * cpu[f"fps_{threads}_threads"], 1)
+    out["cpu_oracle"] = cpu
+    ke_g, ke_o = r["settled_ke_j"], cpu["settled_ke_j"]
+    out["gate_settled_ke_within_5pct"] = bool(abs(ke_g - ke_o) <= 0.05 * abs(ke_o))
+    out["gpu_over_cpu_no_io"] = round(r["gpu_fps_no_io"] / cpu[f"fps_{threads}_threads"], 2)
+    return out
 
 
 def frame_bytes(n_dyn, n_static, frames, body_solves, pairs, contacts, dyn_endpoints, iters):
@@ -441,13 +736,27 @@ def main():
               f"with a ghost-body exchange per substep).", file=sys.stderr)
         sys.exit(2)
 
+    # The drop-in C++ stress driver first, in child processes, before this
+    # process touches the GPU (a process's cooperative launches are refused
+    # while another process on the device holds them).
+    drivers = {}
+    if args.c1_frames > 0:
+        import tempfile
+        binary = os.path.join(ROOT, "dyphur_amd", "bin", "stress_test_blocks")
+        if os.path.exists(binary):
+            tmp = tempfile.mkdtemp(prefix="dyphur_drv_")
+            drivers["pitch_1.0"] = run_driver(binary, os.path.join(tmp, "p100"), args.c1_frames)
+            drivers["pitch_0.85"] = run_driver(binary, os.path.join(tmp, "p085"), args.c1_frames, 0.85)
+
     from dyphur_amd import capi, scenes
 
     capi.load()
     if capi.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
-    sc = scenes.random_mixed(args.bodies, seed=20260002, frames=args.steps)
+    plan = plan_frames(args.steps, C2_FRAMES, 0 if args.no_cpu else args.cpu_window)
+    total, start, steps = plan["total"], plan["start"], plan["steps"]
+    sc = scenes.random_mixed(args.bodies, seed=20260002, frames=total)
     s = capi.Stream()
     w = capi.World(sc, s)
     init = w.bodies.host_arrays(sc)
@@ -456,98 +765,93 @@ def main():
         w.bodies.upload_state(init, s)
         s.wait()
 
-    # Warm-up frames (untimed), then restart from the initial state.
+    # Warm-up frames (untimed), then restart from the initial state and
+    # advance, untimed, to the first timed frame.
     for _ in range(args.warmup):
         w.step_async()
     w.check()
     reset()
+    for _ in range(start):
+        w.step_async()
+    w.check()
 
-    # Timed frames: no host synchronisation inside the loop.
-    w.solver.set_timing(True)
+    # Timed frames: no host synchronisation and no timing mode inside the loop.
     e0, e1 = capi.Event(), capi.Event()
     s.wait()
     t0 = time.perf_counter()
     e0.record(s)
-    for _ in range(args.steps):
+    for _ in range(steps):
         w.step_async()
     e1.record(s)
     s.wait()
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_ms(e1)
     w.check()  # raises on any device error flag of the timed frames
-    timing = w.solver.timing(s)
-    w.solver.set_timing(False)
     final_hash = w.fnv()
 
-    # Untimed replay: per-frame load statistics, CPU-baseline snapshots, and
-    # the same final state as the timed run.
-    window = args.cpu_window if (not args.no_cpu and args.cpu_window > 0) else 0
-    starts = sorted({0, args.steps // 3, (2 * args.steps) // 3, max(0, args.steps - window)}) if window else []
-    want = set(starts) | {f + window for f in starts}
-    stats, snaps, replay_hash = replay(w, init, s, args.steps, want)
+    # Per-kernel times over the same frames, in a separate untimed pass.
+    reset()
+    for _ in range(start):
+        w.step_async()
+    w.check()
+    kernels = stage_kernels(capi, w, sc, s, steps)
+    if w.fnv() != final_hash:
+        raise RuntimeError("the kernel-timing pass ended in a different state than the timed run")
+
+    # Untimed replay of the whole run: per-frame load statistics, the CPU
+    # windows' snapshots, and the same final state as the timed run.
+    window, starts = plan["window"], plan["starts"]
+    want = (set(starts) | {f + window for f in starts}) if window else set()
+    stats, snaps = replay(w, init, s, total, want)
+    replay_hash = w.fnv()
     if replay_hash != final_hash:
         raise RuntimeError(f"replay state {replay_hash:016x} differs from the timed run {final_hash:016x}")
-    pairs = [st["n_pairs"] for st in stats]
-    contacts = [st["n_contacts"] for st in stats]
-    colors = [st["n_colors"] for st in stats]
+    timed = stats[start:]
     n = sc.n
-    value = n * args.steps * sc.n_substeps / wall
+    value = n * steps * sc.n_substeps / wall
     out = {
         "metric": METRIC,
         "value": round(value, 1),
         "unit": "body-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * wall / args.steps, 4),
+        "ms_per_step": round(1000.0 * wall / steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"single scene, {args.bodies} mixed boxes/spheres + static ground, "
-                        f"frames 0..{args.steps - 1} at 60 Hz, 1 substep, 10 iterations",
-            "bodies": n, "frames": args.steps, "substeps": sc.n_substeps, "iterations": sc.n_iters,
-            "seed": 20260002, "max_pairs": sc.max_pairs, "max_contacts": sc.max_contacts,
+            "workload": f"single scene, {args.bodies} mixed boxes/spheres + static ground, seed 20260002, "
+                        f"{total}-frame run, timed frames {start}..{total - 1} at 60 Hz, 1 substep, 10 iterations",
+            "bodies": n, "frames": total, "timed_frames": [start, total - 1], "substeps": sc.n_substeps,
+            "iterations": sc.n_iters, "seed": 20260002, "max_pairs": sc.max_pairs, "max_contacts": sc.max_contacts,
         },
-        "stream_ms_per_step": round(gpu_ms / args.steps, 4),
-        "load": {"pairs_mean": float(np.mean(pairs)), "contacts_mean": float(np.mean(contacts)),
-                 "colors_max": int(np.max(colors)), "pairs_max": int(np.max(pairs)),
-                 "contacts_max": int(np.max(contacts))},
+        "stream_ms_per_step": round(gpu_ms / steps, 4),
+        "load": {"timed_pairs_mean": float(np.mean([t["n_pairs"] for t in timed])),
+                 "timed_contacts_mean": float(np.mean([t["n_contacts"] for t in timed])),
+                 "run_pairs_mean": float(np.mean([t["n_pairs"] for t in stats])),
+                 "run_contacts_mean": float(np.mean([t["n_contacts"] for t in stats])),
+                 "colors_max": int(max(t["n_colors"] for t in stats)),
+                 "pairs_max": int(max(t["n_pairs"] for t in stats)),
+                 "contacts_max": int(max(t["n_contacts"] for t in stats))},
         "final_state_fnv1a64": f"{final_hash:016x}",
     }
-    # Roofline of the dominant kernel, timed with HIP events on the solver's
-    # stream over the timed frames.  The sweep's algorithmic bytes per launch
-    # (one launch per frame): iterations x (44 B per contact + 132 B per
-    # dynamic contact endpoint + 32 B per body).  Class assignment: 12 B per
-    # contact + 16 B per body (the per-stage figure for the colouring).
-    n_solves = max(timing["solves"], 1)
-    assign_bytes = (12.0 * float(np.sum(contacts)) + 16.0 * n * len(contacts)) / len(contacts)
-    sweep_kernel = {"0": "sweep_xcd", "1": "sweep_persistent", "3": "sweep_flow_lanes"}.get(
-        os.environ.get("DYPHUR_SWEEP_MODE", "2"), "sweep_flow")
-    kernels = {
-        sweep_kernel: (timing["sweep_ms"] / n_solves, timing["sweep_bytes"] / n_solves),
-        "assign_lists": (timing["assign_ms"] / n_solves, assign_bytes),
-    }
-    name = max(kernels, key=lambda k: kernels[k][0])
-    ms, nbytes = kernels[name]
-    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    out["roofline"] = {
-        "bound": "hbm", "kernel": name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic_from_profiles(name),
-        "avg_launch_ms": round(ms, 4), "algorithmic_bytes_per_launch": round(nbytes),
-        "other": {k: {"avg_launch_ms": round(v[0], 4), "algorithmic_bytes_per_launch": round(v[1])}
-                  for k, v in kernels.items() if k != name},
-    }
+    # Roofline of the dominant kernel of the frame: per-kernel event times
+    # over the timed frames' replay, algorithmic bytes per launch from the
+    # per-stage figures and the frames' own pair and contact counts.
+    out["roofline"] = roofline_of(kernels)
     if window:
         out["cpu_baseline"] = cpu_baseline(sc, snaps, starts, window)
         out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 2)
     # The other legs do not use the single-scene world: release it and its
     # stream (a process's streams share HIP's few hardware queues).
-    del w, e0, e1
+    del w, e0, e1, snaps
     s.close()
     del s
+    if args.c1_frames > 0:
+        out["stress_c1"] = stress_c1(capi, scenes, drivers, args.c1_frames)
     if args.batched_frames > 0:
         out["batched"] = batched(capi, scenes, args.batched_frames, args.plain_sample)
     if args.diff_frames > 0:

@@ -84,6 +84,11 @@ class SolverTiming(C.Structure):
                 ("sweep_bytes", C.c_double)]
 
 
+class BroadphaseTiming(C.Structure):
+    _fields_ = [("builds", u32), ("sort_ms", C.c_double), ("karras_ms", C.c_double), ("refit_ms", C.c_double),
+                ("traverse_ms", C.c_double), ("emit_ms", C.c_double)]
+
+
 class BatchSceneStats(C.Structure):
     _fields_ = [(k, u32) for k in ("n_pairs", "n_contacts", "n_classes", "used_serial", "rounds",
                                    "overflow", "error", "substeps", "sum_pairs", "sum_contacts",
@@ -139,6 +144,8 @@ _SIGS = {
     "dy_broadphase_sort_pairs": (C.c_int, [vp, vp, u32]),
     "dy_broadphase_download_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
     "dy_broadphase_bvh_view": (C.c_int, [vp, u32, C.POINTER(BvhView)]),
+    "dy_broadphase_set_timing": (C.c_int, [vp, C.c_int]),
+    "dy_broadphase_get_timing": (C.c_int, [vp, C.POINTER(BroadphaseTiming)]),
     "dy_broadphase_morton_keys": (C.c_int, [vp, C.POINTER(vp)]),
     "dy_narrowphase_create": (C.c_int, [vp, u32, C.POINTER(vp)]),
     "dy_narrowphase_destroy": (C.c_int, [vp]),
@@ -148,6 +155,8 @@ _SIGS = {
     "dy_narrowphase_contacts": (C.c_int, [vp, C.POINTER(ContactView)]),
     "dy_narrowphase_download_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
     "dy_narrowphase_last_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
+    "dy_narrowphase_set_features": (C.c_int, [vp, C.c_int]),
+    "dy_narrowphase_features": (C.c_int, [vp, C.POINTER(vp)]),
     "dy_xpbd_create": (C.c_int, [vp, C.c_int, u32, f32, C.POINTER(vp)]),
     "dy_xpbd_destroy": (C.c_int, [vp]),
     "dy_xpbd_set_force_serial": (C.c_int, [vp, C.c_int]),
@@ -480,6 +489,14 @@ class Broadphase:
     def sort_pairs(self, s: Stream, n_pairs: int):
         check(lib().dy_broadphase_sort_pairs(self.h, s.h, n_pairs), "dy_broadphase_sort_pairs")
 
+    def set_timing(self, on: bool):
+        check(lib().dy_broadphase_set_timing(self.h, int(bool(on))), "dy_broadphase_set_timing")
+
+    def timing(self) -> dict:
+        t = BroadphaseTiming()
+        check(lib().dy_broadphase_get_timing(self.h, C.byref(t)), "dy_broadphase_get_timing")
+        return {k: getattr(t, k) for k, _ in BroadphaseTiming._fields_}
+
     def pairs(self, s: Stream, n: int) -> np.ndarray:
         n = min(n, self.max_pairs)
         p, _ = self.pairs_ptr()
@@ -541,6 +558,16 @@ class Narrowphase:
         out = u32(0)
         check(lib().dy_narrowphase_last_count(self.h, s.h, C.byref(out)), "dy_narrowphase_last_count")
         return out.value
+
+    def set_features(self, on: bool):
+        check(lib().dy_narrowphase_set_features(self.h, int(bool(on))), "dy_narrowphase_set_features")
+
+    def features(self, s: Stream) -> np.ndarray:
+        """Feature words of the stored contacts of the last run (set_features(True) first)."""
+        n = self.download_count(s)
+        p = vp()
+        check(lib().dy_narrowphase_features(self.h, C.byref(p)), "dy_narrowphase_features")
+        return download_ptr(p.value, n, np.uint32, s)
 
     def contacts(self, s: Stream) -> dict:
         n = self.download_count(s)

@@ -613,6 +613,11 @@ struct dy_broadphase_s {
     // override them.
     uint32_t trav_threads = 64;
     uint32_t trav_xcd = 1;
+    // Kernel timing (dy_broadphase_set_timing): events between the launches
+    // of one build, read back when the next build starts or on request.
+    bool timing = false, ev_pending = false;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    dy_broadphase_timing acc{};
 };
 
 namespace {
@@ -623,6 +628,26 @@ void bp_free(dy_broadphase_s *b) {
                     b->sort_mat, b->sort_tot};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t &e : b->ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+}
+// Adds the event intervals of the last timed build to the totals.
+int bp_collect_timing(dy_broadphase_s *b) {
+    if (!b->ev_pending) return DY_OK;
+    DY_TRY(hipEventSynchronize(b->ev[5]));
+    float ms[5];
+    for (int k = 0; k < 5; ++k) DY_TRY(hipEventElapsedTime(&ms[k], b->ev[k], b->ev[k + 1]));
+    b->acc.builds += 1;
+    b->acc.sort_ms += ms[0];
+    b->acc.karras_ms += ms[1];
+    b->acc.refit_ms += ms[2];
+    b->acc.traverse_ms += ms[3];
+    b->acc.emit_ms += ms[4];
+    b->ev_pending = false;
+    return DY_OK;
 }
 template <typename T>
 hipError_t dalloc(T *&p, size_t n) {
@@ -700,8 +725,11 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     const uint32_t n = bodies->n;
     if (n > bp->max_bodies || n >= kMortonMax) return DY_ERR_INVALID_ARG;
     hipStream_t st = hs(s);
+    if (bp->timing) DY_TRY_STATUS(bp_collect_timing(bp));
     DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 4 * sizeof(uint32_t), st));
     if (n < 2) return DY_OK;
+    const bool tm = bp->timing;
+    if (tm) DY_TRY(hipEventRecord(bp->ev[0], st));
     const int ni = static_cast<int>(n);
     const uint32_t n_tiles = ceil_div(n, kSgTile);
     bool sorted = false;
@@ -723,20 +751,45 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
                                               bp->morton64);
         DY_LAUNCH_CHECK();
     }
+    if (tm) DY_TRY(hipEventRecord(bp->ev[1], st));
     karras_build<<<ceil_div(n - 1, 256), 256, 0, st>>>(bp->morton64, ni, bp->left, bp->right, bp->parent,
                                                        bp->range, bp->tn, bp->cnt_top, bp->root);
     DY_LAUNCH_CHECK();
+    if (tm) DY_TRY(hipEventRecord(bp->ev[2], st));
     leaf_refit<<<ceil_div(n, kChunk), kChunk, 0, st>>>(*bodies, *shapes, ni, bp->idx0, bp->parent, bp->left,
                                                        bp->right, bp->range, bp->nodebox, bp->tn, bp->cnt_top,
                                                        bp->entries, bp->counters);
     DY_LAUNCH_CHECK();
+    if (tm) DY_TRY(hipEventRecord(bp->ev[3], st));
     traverse_count<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(
         bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
     DY_LAUNCH_CHECK();
+    if (tm) DY_TRY(hipEventRecord(bp->ev[4], st));
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
     emit_pairs<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
                                                  bp->pairs, bp->max_pairs, bp->counters + 2);
     DY_LAUNCH_CHECK();
+    if (tm) {
+        DY_TRY(hipEventRecord(bp->ev[5], st));
+        bp->ev_pending = true;
+    }
+    return DY_OK;
+}
+
+int dy_broadphase_set_timing(dy_broadphase_t bp, int enable) {
+    if (!bp) return DY_ERR_INVALID_ARG;
+    if (enable && !bp->ev[0])
+        for (hipEvent_t &e : bp->ev) DY_TRY(hipEventCreate(&e));
+    DY_TRY_STATUS(bp_collect_timing(bp));  // a build timed before this call is not counted
+    bp->timing = enable != 0;
+    bp->acc = dy_broadphase_timing{};
+    return DY_OK;
+}
+
+int dy_broadphase_get_timing(dy_broadphase_t bp, dy_broadphase_timing *out) {
+    if (!bp || !out) return DY_ERR_INVALID_ARG;
+    DY_TRY_STATUS(bp_collect_timing(bp));
+    *out = bp->acc;
     return DY_OK;
 }
 

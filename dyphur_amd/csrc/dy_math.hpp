@@ -139,7 +139,16 @@ DY_HD void leaf_extent(uint32_t stype, float hx, float hy, float hz, float qw, f
 // ---------------------------------------------------------------------------
 // Narrowphase primitives (core/src/narrowphase.cpp:46-306)
 // ---------------------------------------------------------------------------
-struct Hit { float px, py, pz, nx, ny, nz, depth; };
+// feat: the contact's feature word (dy_contact_feature in dyphur_hip.h: kind,
+// axes, incident-vertex mask, SAT axis); the reference's box_box computes
+// these and keeps only the contact (narrowphase.cpp:153-305).
+struct Hit { float px, py, pz, nx, ny, nz, depth; uint32_t feat; };
+
+constexpr uint32_t kFeatFaceA = 0, kFeatFaceB = 1, kFeatEdge = 2, kFeatSphSph = 3, kFeatSphBoxOut = 4,
+                   kFeatSphBoxIn = 5;
+DY_HD uint32_t feat_word(uint32_t kind, uint32_t i, uint32_t j, uint32_t mask, uint32_t sat) {
+    return kind | (i << 3) | (j << 5) | (mask << 8) | (sat << 16);
+}
 
 DY_HD bool sphere_sphere(float ax, float ay, float az, float ra, float bx, float by, float bz,
                          float rb, Hit &h) {
@@ -154,6 +163,7 @@ DY_HD bool sphere_sphere(float ax, float ay, float az, float ra, float bx, float
     h.px = 0.5f * ((ax - h.nx * ra) + (bx + h.nx * rb));
     h.py = 0.5f * ((ay - h.ny * ra) + (by + h.ny * rb));
     h.pz = 0.5f * ((az - h.nz * ra) + (bz + h.nz * rb));
+    h.feat = feat_word(kFeatSphSph, 0, 0, 0, 0);
     return true;
 }
 
@@ -178,6 +188,7 @@ DY_HD bool sphere_box(float sx, float sy, float sz, float r, float cx, float cy,
         h.nx = -(sg * E.m[f][0]); h.ny = -(sg * E.m[f][1]); h.nz = -(sg * E.m[f][2]);
         h.depth = r + pf;
         h.px = sx - h.nx * r; h.py = sy - h.ny * r; h.pz = sz - h.nz * r;
+        h.feat = feat_word(kFeatSphBoxIn, static_cast<uint32_t>(f), 0, 0, 0);
         return true;
     }
     if (d2 >= r * r) return false;
@@ -191,6 +202,7 @@ DY_HD bool sphere_box(float sx, float sy, float sz, float r, float cx, float cy,
     h.px = cx + cl[0] * E.m[0][0] + cl[1] * E.m[1][0] + cl[2] * E.m[2][0];
     h.py = cy + cl[0] * E.m[0][1] + cl[1] * E.m[1][1] + cl[2] * E.m[2][1];
     h.pz = cz + cl[0] * E.m[0][2] + cl[1] * E.m[1][2] + cl[2] * E.m[2][2];
+    h.feat = feat_word(kFeatSphBoxOut, 0, 0, 0, 0);
     return true;
 }
 
@@ -262,6 +274,8 @@ DY_HD bool box_box(float acx, float acy, float acz, const M3 &A, const float ah[
     const float nx = s.nx, ny = s.ny, nz = s.nz;
     float sx = 0.f, sy = 0.f, sz = 0.f, sd = 0.f;
     int m = 0;
+    const uint32_t sat = static_cast<uint32_t>(s.type == 0 ? s.i : (s.type == 1 ? 3 + s.i : 6 + 3 * s.i + s.j));
+    uint32_t feat;
     if (s.type == 0 || s.type == 1) {
         int type = s.type, ref = s.i;
         if (type == 0) {
@@ -286,6 +300,7 @@ DY_HD bool box_box(float acx, float acy, float acz, const M3 &A, const float ah[
         float r0[3], r1[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) { r0[c] = R.m[t0][c]; r1[c] = R.m[t1][c]; }
+        uint32_t mask = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float s0 = (k & 1) ? 1.f : -1.f, s1 = (k & 2) ? 1.f : -1.f, s2 = (k & 4) ? 1.f : -1.f;
@@ -300,6 +315,7 @@ DY_HD bool box_box(float acx, float acy, float acz, const M3 &A, const float ah[
             const float q1 = ux * r1[0] + uy * r1[1] + uz * r1[2];
             if (fabsf(q0) > rh_t0 + 1e-3f) continue;
             if (fabsf(q1) > rh_t1 + 1e-3f) continue;
+            mask |= 1u << k;
             if (m < 4) {
                 // Running centroid sums in emission order (narrowphase.cpp:993).
                 sx += vx - sdir * dv * nx;
@@ -309,6 +325,7 @@ DY_HD bool box_box(float acx, float acy, float acz, const M3 &A, const float ah[
                 ++m;
             }
         }
+        feat = feat_word(a_ref ? kFeatFaceA : kFeatFaceB, static_cast<uint32_t>(ref), 0, mask, sat);
     } else {
         const int i = s.i, j = s.j;
         const int ta0 = (i + 1) % 3, ta1 = (i + 2) % 3;
@@ -340,12 +357,14 @@ DY_HD bool box_box(float acx, float acy, float acz, const M3 &A, const float ah[
         sz += 0.5f * (qaz + qbz);
         sd += s.best;
         m = 1;
+        feat = feat_word(kFeatEdge, static_cast<uint32_t>(i), static_cast<uint32_t>(j), 0, sat);
     }
     if (m == 0) return false;
     const float inv = 1.f / static_cast<float>(m);
     h.px = sx * inv; h.py = sy * inv; h.pz = sz * inv;
     h.nx = nx; h.ny = ny; h.nz = nz;
     h.depth = sd * inv;
+    h.feat = feat;
     return true;
 }
 

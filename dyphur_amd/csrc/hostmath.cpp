@@ -33,9 +33,10 @@ void hm_leaf_extent(uint32_t stype, const float *half3, const float *q4, float *
 }
 
 // Contact of one pair: types, half extents (3 each), centres (3 each),
-// quaternions wxyz (4 each).  out[7] = p xyz, n xyz, depth.  Returns 1 on hit.
-int hm_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
-                    const float *qa, const float *qb, float *out) {
+// quaternions wxyz (4 each).  out[7] = p xyz, n xyz, depth; *feat (may be
+// null) the feature word.  Returns 1 on hit.
+int hm_pair_contact_feat(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                         const float *pb, const float *qa, const float *qb, float *out, uint32_t *feat) {
     Hit h{};
     bool hit = false;
     if (ta == kSphere && tb == kSphere) {
@@ -56,7 +57,13 @@ int hm_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, 
     out[0] = h.px; out[1] = h.py; out[2] = h.pz;
     out[3] = h.nx; out[4] = h.ny; out[5] = h.nz;
     out[6] = h.depth;
+    if (feat) *feat = h.feat;
     return 1;
+}
+
+int hm_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
+                    const float *qa, const float *qb, float *out) {
+    return hm_pair_contact_feat(ta, tb, ha, hb, pa, pb, qa, qb, out, nullptr);
 }
 
 // One contact projection.  contact[7] = p xyz, n xyz, depth; body[20] =

@@ -3,71 +3,138 @@
 //
 // Each in-scope pair yields at most one contact (box-box manifolds reduce to
 // their centroid), so the reference's append + (pair, sub) key sort + gather
-// reduces to: one pair per lane into fixed per-pair slots with a hit flag and
-// per-workgroup hit counts (np_test), a scan of the workgroup counts
-// (np_offsets), and an ordered compaction (np_compact).  Contacts come out in
-// pair order by construction; no sort, no atomics on the output.
-#include "dy_internal.hpp"
+// (narrowphase.cpp:1077-1127) reduces to an order-preserving compaction of the
+// pair list.  It is done in one pass: workgroups claim 256-pair tiles in
+// ascending order from a ticket, test one pair per lane, scan the hits of the
+// tile and find the tile's output offset by decoupled look-back over the
+// earlier tiles' published sums (Merrill & Garland, "Single-pass Parallel
+// Prefix Scan with Decoupled Look-back", 2016); each contact is then written
+// straight to its slot.  Contacts come out in pair order by construction; no
+// per-pair staging, no sort.  The launch is bounded by the device-held pair
+// count: tiles past it are never claimed.
+#include "narrowphase_pass.hpp"
 #include "pair_contact.hpp"
 
 namespace dy {
 
-constexpr uint32_t kNpThreads = 256;
+constexpr uint32_t kNpThreads = 256;   // pairs per tile
+constexpr uint32_t kNpMaxGrid = 2048;  // persistent workgroups at most (8 per CU on 256 CUs)
+// Tile status word: value in bits 0-31, state in bits 32-33 (0 not ready,
+// 1 the tile's own hit count, 2 the inclusive count of tiles 0..t).
+constexpr uint64_t kTileAgg = 1ull << 32, kTileInc = 2ull << 32;
+constexpr uint32_t kNpSpinLimit = 1u << 24;
 
-struct PairSlots {
-    float *px, *py, *pz, *nx, *ny, *nz, *depth;
-    uint32_t *hit;
-};
-
-__global__ __launch_bounds__(kNpThreads) void np_test(const dy_contact_pair *__restrict__ pairs, uint32_t n_bound,
-                                                      const uint32_t *__restrict__ d_n, dy_body_view bv,
-                                                      dy_shape_view sv, PairSlots slots,
-                                                      uint32_t *__restrict__ block_hits, uint32_t *__restrict__ err) {
-    __shared__ uint32_t red[kNpThreads / 64];
-    const uint32_t i = blockIdx.x * kNpThreads + threadIdx.x;
-    const uint32_t n = d_n ? min(*d_n, n_bound) : n_bound;
-    uint32_t hit = 0, unsup = 0;
-    if (i < n) {
-        const dy_contact_pair p = pairs[i];
-        Hit h;
-        if (pair_contact(bv, sv, p.a, p.b, h, unsup)) {
-            hit = 1;
-            slots.px[i] = h.px; slots.py[i] = h.py; slots.pz[i] = h.pz;
-            slots.nx[i] = h.nx; slots.ny[i] = h.ny; slots.nz[i] = h.nz;
-            slots.depth[i] = h.depth;
+// Exclusive prefix of tile t: sum of the hit counts of tiles 0..t-1, read by
+// wave 0 from the status words of the 64 preceding tiles at a time.
+__device__ __forceinline__ uint32_t np_lookback(uint64_t *status, uint32_t t, uint32_t *err) {
+    const uint32_t lane = lane_id();
+    uint32_t prefix = 0;
+    int64_t base = static_cast<int64_t>(t) - 1;
+    uint32_t spins = 0;
+    while (base >= 0) {
+        const int64_t j = base - lane;
+        uint64_t w = j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kTileInc;
+        // every tile in the window must have published at least its own count
+        if (__ballot(w < kTileAgg) != 0) {
+            if (++spins > kNpSpinLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
         }
-    }
-    if (i < n_bound) slots.hit[i] = hit;
-    if (unsup) atomicOr(err, 1u);
-    uint32_t c = hit;
+        const uint64_t inc = __ballot(w >= kTileInc);  // lanes whose tile holds an inclusive count
+        const uint32_t stop = inc ? static_cast<uint32_t>(__builtin_ctzll(inc)) : 64u;
+        uint32_t v = lane <= stop ? static_cast<uint32_t>(w) : 0u;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) block_hits[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        prefix += v;
+        if (stop < 64) break;
+        base -= 64;
+    }
+    return prefix;
 }
 
-// counts[0] = stored (clamped), counts[1] = true count.
-__global__ void np_clamp(uint32_t *__restrict__ counts, uint32_t cap) {
-    const uint32_t t = counts[1];
-    counts[0] = t < cap ? t : cap;
-}
-
-__global__ __launch_bounds__(kNpThreads) void np_compact(const dy_contact_pair *__restrict__ pairs, uint32_t n_bound,
-                                                         PairSlots slots, const uint32_t *__restrict__ block_offs,
-                                                         dy_contact_view cv) {
+// Persistent pass: claim tile t, test its pairs, publish the tile's count,
+// look back for its offset, publish the inclusive count, write the contacts.
+__global__ __launch_bounds__(kNpThreads) void np_fused(const dy_contact_pair *__restrict__ pairs, uint32_t n_bound,
+                                                       const uint32_t *__restrict__ d_n, dy_body_view bv,
+                                                       dy_shape_view sv, uint64_t *status, uint32_t *ticket,
+                                                       NpOut out) {
     __shared__ uint32_t wsum[17];
-    const uint32_t i = blockIdx.x * kNpThreads + threadIdx.x;
-    const uint32_t hit = i < n_bound ? slots.hit[i] : 0u;
-    uint32_t total;
-    const uint32_t pos = block_offs[blockIdx.x] + block_exclusive_scan(hit, wsum, total);
-    if (!hit || pos >= cv.capacity) return;
-    const dy_contact_pair p = pairs[i];
-    cv.body_a[pos] = p.a;
-    cv.body_b[pos] = p.b;
-    cv.pos_x[pos] = slots.px[i]; cv.pos_y[pos] = slots.py[i]; cv.pos_z[pos] = slots.pz[i];
-    cv.norm_x[pos] = slots.nx[i]; cv.norm_y[pos] = slots.ny[i]; cv.norm_z[pos] = slots.nz[i];
-    cv.depth[pos] = slots.depth[i];
+    __shared__ uint32_t s_tile, s_prefix;
+    const uint32_t n = d_n ? min(*d_n, n_bound) : n_bound;
+    const uint32_t n_tiles = (n + kNpThreads - 1) / kNpThreads;
+    const uint32_t cap = out.cv.capacity;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        if (t >= n_tiles) break;  // uniform: every wave of the workgroup leaves here
+        const uint32_t i = t * kNpThreads + threadIdx.x;
+        uint32_t hit = 0, unsup = 0;
+        Hit h{};
+        dy_contact_pair p{0u, 0u};
+        if (i < n) {
+            p = pairs[i];
+            hit = pair_contact(bv, sv, p.a, p.b, h, unsup) ? 1u : 0u;
+        }
+        if (unsup) atomicOr(&out.counts[2], 1u);
+        uint32_t total;
+        const uint32_t local = block_exclusive_scan(hit, wsum, total);
+        if (threadIdx.x < 64) {
+            uint32_t prefix = 0;
+            if (t == 0) {
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[0], kTileInc | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[t], kTileAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                prefix = np_lookback(status, t, &out.counts[3]);
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[t], kTileInc | (prefix + total), __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (threadIdx.x == 0) {
+                s_prefix = prefix;
+                if (t == n_tiles - 1) {  // the last tile knows the true count
+                    const uint32_t all = prefix + total;
+                    out.counts[1] = all;
+                    out.counts[0] = all < cap ? all : cap;
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t pos = s_prefix + local;
+        if (hit && pos < cap) {
+            const dy_contact_view &cv = out.cv;
+            cv.body_a[pos] = p.a;
+            cv.body_b[pos] = p.b;
+            cv.pos_x[pos] = h.px; cv.pos_y[pos] = h.py; cv.pos_z[pos] = h.pz;
+            cv.norm_x[pos] = h.nx; cv.norm_y[pos] = h.ny; cv.norm_z[pos] = h.nz;
+            cv.depth[pos] = h.depth;
+            if (out.feat) out.feat[pos] = h.feat;
+        }
+        __syncthreads();  // s_tile and s_prefix are rewritten by the next claim
+    }
+}
+
+// Persistent workgroups for a launch over at most n_bound pairs.
+uint32_t np_grid(uint32_t n_bound) {
+    const uint32_t tiles = ceil_div(n_bound, kNpThreads);
+    return tiles < kNpMaxGrid ? tiles : kNpMaxGrid;
+}
+
+int np_enqueue(hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound, const uint32_t *d_n,
+               const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t *ticket, NpOut out) {
+    DY_TRY(hipMemsetAsync(out.counts, 0, kNpCounts * sizeof(uint32_t), st));
+    if (n_bound == 0) return DY_OK;
+    const uint32_t tiles = ceil_div(n_bound, kNpThreads);
+    DY_TRY(hipMemsetAsync(status, 0, tiles * sizeof(uint64_t), st));
+    DY_TRY(hipMemsetAsync(ticket, 0, sizeof(uint32_t), st));
+    np_fused<<<np_grid(n_bound), kNpThreads, 0, st>>>(pairs, n_bound, d_n, bv, sv, status, ticket, out);
+    DY_LAUNCH_CHECK();
+    return DY_OK;
 }
 
 }  // namespace dy
@@ -79,51 +146,32 @@ struct dy_narrowphase_s {
     // store
     uint32_t *body_a = nullptr, *body_b = nullptr;
     float *px = nullptr, *py = nullptr, *pz = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr, *depth = nullptr;
-    uint32_t *counts = nullptr;  // [0] stored, [1] true, [2] unsupported flag
-    // per-pair scratch (grows)
-    DevBuf<float> s_px, s_py, s_pz, s_nx, s_ny, s_nz, s_depth;
-    DevBuf<uint32_t> s_hit, block_hits, block_offs, scan_tmp;
+    uint32_t *counts = nullptr;  // kNpCounts words
+    uint32_t *feat = nullptr;    // feature words (dy_narrowphase_set_features), else null
+    bool want_feat = false;
+    // tile status (grows with the launch bound) + ticket
+    DevBuf<uint64_t> status;
+    uint32_t *ticket = nullptr;
 };
 
 namespace {
-int np_ensure(dy_narrowphase_s *np, uint32_t n_pairs) {
-    const uint32_t nb = ceil_div(n_pairs > 0 ? n_pairs : 1, kNpThreads);
-    DY_TRY_STATUS(np->s_px.ensure(n_pairs)); DY_TRY_STATUS(np->s_py.ensure(n_pairs));
-    DY_TRY_STATUS(np->s_pz.ensure(n_pairs)); DY_TRY_STATUS(np->s_nx.ensure(n_pairs));
-    DY_TRY_STATUS(np->s_ny.ensure(n_pairs)); DY_TRY_STATUS(np->s_nz.ensure(n_pairs));
-    DY_TRY_STATUS(np->s_depth.ensure(n_pairs)); DY_TRY_STATUS(np->s_hit.ensure(n_pairs));
-    DY_TRY_STATUS(np->block_hits.ensure(nb)); DY_TRY_STATUS(np->block_offs.ensure(nb));
-    DY_TRY_STATUS(np->scan_tmp.ensure(scan_tmp_words(nb)));
-    return DY_OK;
-}
-
 int np_launch(dy_narrowphase_s *np, hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound,
               const uint32_t *d_n, const dy_body_view &bv, const dy_shape_view &sv) {
-    DY_TRY(hipMemsetAsync(np->counts, 0, 3 * sizeof(uint32_t), st));
-    if (n_bound == 0) return DY_OK;
-    DY_TRY_STATUS(np_ensure(np, n_bound));
-    const uint32_t nb = ceil_div(n_bound, kNpThreads);
-    PairSlots slots{np->s_px.p, np->s_py.p, np->s_pz.p, np->s_nx.p, np->s_ny.p, np->s_nz.p, np->s_depth.p, np->s_hit.p};
-    np_test<<<nb, kNpThreads, 0, st>>>(pairs, n_bound, d_n, bv, sv, slots, np->block_hits.p, np->counts + 2);
-    DY_LAUNCH_CHECK();
-    DY_TRY_STATUS(scan_exclusive(st, np->block_hits.p, np->block_offs.p, nb, np->counts + 1, np->scan_tmp.p));
-    np_clamp<<<1, 1, 0, st>>>(np->counts, np->cap);
-    DY_LAUNCH_CHECK();
-    dy_contact_view cv{np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth,
-                       np->counts, np->cap};
-    np_compact<<<nb, kNpThreads, 0, st>>>(pairs, n_bound, slots, np->block_offs.p, cv);
-    DY_LAUNCH_CHECK();
-    return DY_OK;
+    DY_TRY_STATUS(np->status.ensure(ceil_div(n_bound > 0 ? n_bound : 1u, kNpThreads)));
+    const dy_contact_view cv{np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth,
+                             np->counts, np->cap};
+    return np_enqueue(st, pairs, n_bound, d_n, bv, sv, np->status.p, np->ticket,
+                      NpOut{cv, np->counts, np->want_feat ? np->feat : nullptr});
 }
 
 void np_free(dy_narrowphase_s *np) {
-    void *ptrs[] = {np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth, np->counts};
+    void *ptrs[] = {np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth,
+                    np->counts, np->feat, np->ticket};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    np->s_px.release(); np->s_py.release(); np->s_pz.release(); np->s_nx.release();
-    np->s_ny.release(); np->s_nz.release(); np->s_depth.release(); np->s_hit.release();
-    np->block_hits.release(); np->block_offs.release(); np->scan_tmp.release();
+    np->status.release();
 }
+
 }  // namespace
 
 extern "C" {
@@ -141,8 +189,9 @@ int dy_narrowphase_create(dy_stream_t s, uint32_t max_contacts, dy_narrowphase_t
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), c * sizeof(uint32_t));
     for (auto p : f)
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), c * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&np->counts), 4 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemsetAsync(np->counts, 0, 4 * sizeof(uint32_t), hs(s));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&np->counts), kNpCounts * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&np->ticket), sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(np->counts, 0, kNpCounts * sizeof(uint32_t), hs(s));
     if (e != hipSuccess) {
         np_free(np);
         delete np;
@@ -180,21 +229,37 @@ int dy_narrowphase_contacts(dy_narrowphase_t np, dy_contact_view *out) {
     return DY_OK;
 }
 
+int dy_narrowphase_set_features(dy_narrowphase_t np, int enable) {
+    if (!np) return DY_ERR_INVALID_ARG;
+    if (enable && !np->feat)
+        DY_TRY(hipMalloc(reinterpret_cast<void **>(&np->feat), (np->cap ? np->cap : 1u) * sizeof(uint32_t)));
+    np->want_feat = enable != 0;
+    return DY_OK;
+}
+
+int dy_narrowphase_features(dy_narrowphase_t np, const uint32_t **d_features) {
+    if (!np || !d_features) return DY_ERR_INVALID_ARG;
+    *d_features = np->want_feat ? np->feat : nullptr;
+    return np->want_feat ? DY_OK : DY_ERR_INVALID_ARG;
+}
+
 int dy_narrowphase_download_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out) {
     if (!np || !out) return DY_ERR_INVALID_ARG;
-    uint32_t h[3];
+    uint32_t h[kNpCounts];
     DY_TRY(hipMemcpyAsync(h, np->counts, sizeof(h), hipMemcpyDeviceToHost, hs(s)));
     DY_TRY(hipStreamSynchronize(hs(s)));
     *out = h[0];
+    if (h[3]) return DY_ERR_INTERNAL;
     return h[2] ? DY_ERR_UNSUPPORTED : DY_OK;
 }
 
 int dy_narrowphase_last_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out) {
     if (!np || !out) return DY_ERR_INVALID_ARG;
-    uint32_t h[3];
+    uint32_t h[kNpCounts];
     DY_TRY(hipMemcpyAsync(h, np->counts, sizeof(h), hipMemcpyDeviceToHost, hs(s)));
     DY_TRY(hipStreamSynchronize(hs(s)));
     *out = h[1];
+    if (h[3]) return DY_ERR_INTERNAL;
     return h[2] ? DY_ERR_UNSUPPORTED : DY_OK;
 }
 

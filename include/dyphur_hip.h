@@ -14,6 +14,7 @@
  *
  * Blocking calls (host waits for the stream): dy_memcpy_d2h, dy_stream_synchronize,
  * dy_event_synchronize, dy_reduce_sum_f32, *_download_count, dy_narrowphase_last_count,
+ * dy_broadphase_get_timing,
  * dy_state_hash_fnv1a64, dy_xpbd_last_stats, dy_xpbd_get_timing,
  * dy_xpbd_download_classes, dy_batch_stats, dy_batch_autotune.  Everything else,
  * dy_xpbd_solve included, is asynchronous on the given stream, in stream order.
@@ -139,6 +140,19 @@ typedef struct dy_solver_timing {
     double sweep_bytes;       /* total algorithmic bytes of the timed sweeps */
 } dy_solver_timing;
 
+/* Kernel timing of the broadphase, accumulated over builds while enabled: HIP
+ * events recorded on the build's stream between its launches.  No reference
+ * counterpart (the reference's profile mode fences whole stages:
+ * examples/stress_test_blocks/main.cpp:198-257). */
+typedef struct dy_broadphase_timing {
+    uint32_t builds;          /* builds timed */
+    double sort_ms;           /* Morton keys + key sort */
+    double karras_ms;         /* Karras tree */
+    double refit_ms;          /* fp16 leaf boxes + bottom-up refit */
+    double traverse_ms;       /* per-leaf traversal, pair counts and sorted rows */
+    double emit_ms;           /* row-count scan + pair emission */
+} dy_broadphase_timing;
+
 typedef struct dy_stream_s *dy_stream_t;        /* a hipStream_t */
 typedef struct dy_event_s *dy_event_t;          /* a hipEvent_t */
 typedef struct dy_broadphase_s *dy_broadphase_t;
@@ -229,6 +243,10 @@ int dy_broadphase_pairs(dy_broadphase_t bp, const dy_contact_pair **pairs, const
 int dy_broadphase_sort_pairs(dy_broadphase_t bp, dy_stream_t s, uint32_t n_pairs);
 /* download_count (broadphase.hpp:78): blocking; the true count. */
 int dy_broadphase_download_count(dy_broadphase_t bp, dy_stream_t s, uint32_t *out);
+/* Enable (1) or disable (0) kernel timing; enabling resets the totals. */
+int dy_broadphase_set_timing(dy_broadphase_t bp, int enable);
+/* Totals of the timed builds; blocking (waits for the last timed build). */
+int dy_broadphase_get_timing(dy_broadphase_t bp, dy_broadphase_timing *out);
 /* bvh_view (broadphase.hpp:82) */
 int dy_broadphase_bvh_view(dy_broadphase_t bp, uint32_t n_active, dy_bvh_view *out);
 
@@ -252,6 +270,25 @@ int dy_narrowphase_contacts(dy_narrowphase_t np, dy_contact_view *out);
 int dy_narrowphase_download_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out);
 /* last_contact_count (narrowphase.hpp:57): blocking here; the true count. */
 int dy_narrowphase_last_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out);
+/* Feature words (no reference counterpart; core/src/narrowphase.cpp:153-305
+ * computes their parts and keeps only the contact).  set_features(np, 1)
+ * makes later runs also store, per stored contact in contact order, a word:
+ *   bits 0-2   kind: 0 box-box with A's face as the reference face, 1 box-box
+ *              with B's face, 2 box-box edge-edge, 3 sphere-sphere, 4 sphere-box
+ *              with the centre outside (closest point), 5 sphere-box with the
+ *              centre inside
+ *   bits 3-4   the reference-face axis (kinds 0, 1), A's edge axis (2), the face
+ *              of least penetration (5); else 0
+ *   bits 5-6   B's edge axis (kind 2); else 0
+ *   bits 8-15  kinds 0, 1: the incident box's vertices k = 0..7 (signs of the
+ *              half extents x, y, z by bits 0, 1, 2 of k) that passed the depth
+ *              and both side tests; the first four in k order form the manifold
+ *   bits 16-19 box-box: the SAT axis of least overlap before the reference-face
+ *              choice (0-2 A's faces, 3-5 B's faces, 6 + 3i + j the edge pair
+ *              A_i x B_j)
+ * features() gives the device array (DY_ERR_INVALID_ARG while disabled). */
+int dy_narrowphase_set_features(dy_narrowphase_t np, int enable);
+int dy_narrowphase_features(dy_narrowphase_t np, const uint32_t **d_features);
 
 /* XpbdSolver(Stream&, n_iters, max_contacts, friction) (core/include/core/xpbd_solver.hpp:31-34) */
 int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float friction, dy_xpbd_t *out);

@@ -153,7 +153,16 @@ inline void box_axes(float qw, float qx, float qy, float qz, float E[3][3]) {
 // ----------------------------------------------------------------------------
 // Narrowphase primitives (narrowphase.cpp:46-306)
 // ----------------------------------------------------------------------------
-struct Hit { float px, py, pz, nx, ny, nz, depth; };
+// feat: the contact's feature word (kind | axes | incident-vertex mask | SAT
+// axis), defined in ref_cpu.h; the reference computes these quantities in
+// box_box and keeps only the contact (narrowphase.cpp:153-305).
+struct Hit { float px, py, pz, nx, ny, nz, depth; uint32_t feat; };
+
+constexpr uint32_t kFeatFaceA = 0, kFeatFaceB = 1, kFeatEdge = 2, kFeatSphSph = 3, kFeatSphBoxOut = 4,
+                   kFeatSphBoxIn = 5;
+inline uint32_t feat_word(uint32_t kind, uint32_t i, uint32_t j, uint32_t mask, uint32_t sat) {
+    return kind | (i << 3) | (j << 5) | (mask << 8) | (sat << 16);
+}
 
 bool sphere_vs_sphere(float ax, float ay, float az, float ra,
                       float bx, float by, float bz, float rb, Hit &h) {
@@ -169,6 +178,7 @@ bool sphere_vs_sphere(float ax, float ay, float az, float ra,
     h.px = 0.5f * ((ax - h.nx * ra) + (bx + h.nx * rb));
     h.py = 0.5f * ((ay - h.ny * ra) + (by + h.ny * rb));
     h.pz = 0.5f * ((az - h.nz * ra) + (bz + h.nz * rb));
+    h.feat = feat_word(kFeatSphSph, 0, 0, 0, 0);
     return true;
 }
 
@@ -192,6 +202,7 @@ bool sphere_vs_box(float sx, float sy, float sz, float r,
         h.nx = -(sg * E[f][0]); h.ny = -(sg * E[f][1]); h.nz = -(sg * E[f][2]);
         h.depth = r + pen[f];
         h.px = sx - h.nx * r; h.py = sy - h.ny * r; h.pz = sz - h.nz * r;
+        h.feat = feat_word(kFeatSphBoxIn, static_cast<uint32_t>(f), 0, 0, 0);
         return true;
     }
     if (d2 >= r * r) return false;
@@ -205,6 +216,7 @@ bool sphere_vs_box(float sx, float sy, float sz, float r,
     h.px = cx + cl[0] * E[0][0] + cl[1] * E[1][0] + cl[2] * E[2][0];
     h.py = cy + cl[0] * E[0][1] + cl[1] * E[1][1] + cl[2] * E[2][1];
     h.pz = cz + cl[0] * E[0][2] + cl[1] * E[1][2] + cl[2] * E[2][2];
+    h.feat = feat_word(kFeatSphBoxOut, 0, 0, 0, 0);
     return true;
 }
 
@@ -216,11 +228,12 @@ inline float box_radius(const float E[3][3], const float he[3], float lx, float 
 }
 
 // Box-box SAT + manifold.  Returns the number of manifold points (0..4); the
-// normal points from B towards A.
+// normal points from B towards A.  feat receives the feature word (SAT axis,
+// reference face or edge pair, incident vertices that passed the clip).
 int box_vs_box(float acx, float acy, float acz, const float A[3][3], const float ah[3],
                float bcx, float bcy, float bcz, const float B[3][3], const float bh[3],
                float opx[4], float opy[4], float opz[4], float &onx, float &ony, float &onz,
-               float od[4]) {
+               float od[4], uint32_t &feat) {
     const float kPar = 1e-5f;
     float C[3][3];
     float dx = bcx - acx, dy = bcy - acy, dz = bcz - acz;
@@ -260,6 +273,8 @@ int box_vs_box(float acx, float acy, float acz, const float A[3][3], const float
     }
     onx = bnx; ony = bny; onz = bnz;
     int n_out = 0;
+    const uint32_t sat_axis = btype == 0 ? bi : (btype == 1 ? 3 + bi : 6 + 3 * bi + bj);
+    uint32_t mask = 0;
 
     if (btype == 0 || btype == 1) {
         if (btype == 0) {
@@ -295,6 +310,7 @@ int box_vs_box(float acx, float acy, float acz, const float A[3][3], const float
             float q1 = ux * R[t1][0] + uy * R[t1][1] + uz * R[t1][2];
             if (std::fabs(q0) > rh[t0] + 1e-3f) continue;
             if (std::fabs(q1) > rh[t1] + 1e-3f) continue;
+            mask |= 1u << k;
             if (n_out < 4) {
                 opx[n_out] = vx - sd * dv * bnx;
                 opy[n_out] = vy - sd * dv * bny;
@@ -303,6 +319,7 @@ int box_vs_box(float acx, float acy, float acz, const float A[3][3], const float
                 ++n_out;
             }
         }
+        feat = feat_word(a_ref ? kFeatFaceA : kFeatFaceB, static_cast<uint32_t>(rf), 0, mask, sat_axis);
     } else {
         // Edge-edge: midpoint of the closest points of the two supporting edges.
         const int i = bi, j = bj;
@@ -335,6 +352,7 @@ int box_vs_box(float acx, float acy, float acz, const float A[3][3], const float
         opz[0] = 0.5f * (qaz + qbz);
         od[0] = best;
         n_out = 1;
+        feat = feat_word(kFeatEdge, static_cast<uint32_t>(i), static_cast<uint32_t>(j), 0, sat_axis);
     }
     return n_out;
 }
@@ -369,9 +387,10 @@ bool contact_for_pair(const rc_bodies &bv, const rc_shapes &sv, uint32_t ia, uin
         box_axes(bv.rot_w[ia], bv.rot_x[ia], bv.rot_y[ia], bv.rot_z[ia], A);
         box_axes(bv.rot_w[ib], bv.rot_x[ib], bv.rot_y[ib], bv.rot_z[ib], B);
         float px[4], py[4], pz[4], dd[4], nx, ny, nz;
+        uint32_t feat = 0;
         int m = box_vs_box(bv.pos_x[ia], bv.pos_y[ia], bv.pos_z[ia], A, ah,
                            bv.pos_x[ib], bv.pos_y[ib], bv.pos_z[ib], B, bh,
-                           px, py, pz, nx, ny, nz, dd);
+                           px, py, pz, nx, ny, nz, dd, feat);
         if (m == 0) return false;
         // One contact per pair: centroid of the manifold, mean depth.
         float sx = 0.f, sy = 0.f, sz = 0.f, sdp = 0.f;
@@ -380,6 +399,7 @@ bool contact_for_pair(const rc_bodies &bv, const rc_shapes &sv, uint32_t ia, uin
         h.px = sx * inv; h.py = sy * inv; h.pz = sz * inv;
         h.nx = nx; h.ny = ny; h.nz = nz;
         h.depth = sdp * inv;
+        h.feat = feat;
         return true;
     }
     return false;
@@ -428,11 +448,11 @@ inline void renormalise(rc_bodies &b, uint32_t i) {
 }
 
 struct ContactSoA {
-    std::vector<uint32_t> a, b;
+    std::vector<uint32_t> a, b, feat;
     std::vector<float> px, py, pz, nx, ny, nz, depth;
     void resize(size_t n) {
         a.resize(n); b.resize(n); px.resize(n); py.resize(n); pz.resize(n);
-        nx.resize(n); ny.resize(n); nz.resize(n); depth.resize(n);
+        nx.resize(n); ny.resize(n); nz.resize(n); depth.resize(n); feat.resize(n);
     }
 };
 
@@ -1210,6 +1230,7 @@ uint32_t rc_np_run(rc_world *w, const rc_pair *pairs, uint32_t n_pairs, const rc
         sc.px[slot] = h.px; sc.py[slot] = h.py; sc.pz[slot] = h.pz;
         sc.nx[slot] = h.nx; sc.ny[slot] = h.ny; sc.nz[slot] = h.nz;
         sc.depth[slot] = h.depth;
+        sc.feat[slot] = h.feat;
         w->ckeys[slot] = static_cast<uint64_t>(i) << 32;  // sub index 0
     }
     w->last_count = cnt;
@@ -1229,9 +1250,15 @@ uint32_t rc_np_run(rc_world *w, const rc_pair *pairs, uint32_t n_pairs, const rc
         st.px[k] = sc.px[src]; st.py[k] = sc.py[src]; st.pz[k] = sc.pz[src];
         st.nx[k] = sc.nx[src]; st.ny[k] = sc.ny[src]; st.nz[k] = sc.nz[src];
         st.depth[k] = sc.depth[src];
+        st.feat[k] = sc.feat[src];
     }
     w->n_contacts = nc;
     return nc;
+}
+
+void rc_np_features(const rc_world *w, uint32_t *out, uint32_t n) {
+    if (n > w->n_contacts) n = w->n_contacts;
+    std::copy(w->store.feat.begin(), w->store.feat.begin() + n, out);
 }
 
 void rc_np_contacts(const rc_world *w, rc_contacts *o) {
@@ -1537,8 +1564,8 @@ void rc_ray_query(const rc_world *w, uint32_t n, const rc_bodies *b, const rc_sh
     }
 }
 
-int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
-                    const float *pb, const float *qa, const float *qb, float *out) {
+int rc_pair_contact_feat(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                         const float *pb, const float *qa, const float *qb, float *out, uint32_t *feat) {
     float px[2] = {pa[0], pb[0]}, py[2] = {pa[1], pb[1]}, pz[2] = {pa[2], pb[2]};
     float rw[2] = {qa[0], qb[0]}, rx[2] = {qa[1], qb[1]}, ry[2] = {qa[2], qb[2]}, rz[2] = {qa[3], qb[3]};
     uint32_t shape[2] = {0, 1}, flags[2] = {0, 0};
@@ -1554,7 +1581,13 @@ int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, 
     out[0] = h.px; out[1] = h.py; out[2] = h.pz;
     out[3] = h.nx; out[4] = h.ny; out[5] = h.nz;
     out[6] = h.depth;
+    if (feat) *feat = h.feat;
     return 1;
+}
+
+int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                    const float *pb, const float *qa, const float *qb, float *out) {
+    return rc_pair_contact_feat(ta, tb, ha, hb, pa, pb, qa, qb, out, nullptr);
 }
 
 void rc_project_contact(const float *contact, float *a, float *b, float mu, float *lam, float dt) {

@@ -132,6 +132,21 @@ const rc_pair *rc_bp_pairs(const rc_world *w);
 uint32_t rc_np_run(rc_world *w, const rc_pair *pairs, uint32_t n_pairs,
                    const rc_bodies *b, const rc_shapes *s);
 void rc_np_contacts(const rc_world *w, rc_contacts *out);
+/* Feature words of the stored contacts (first n, in store order).  The word
+ * (no reference counterpart; narrowphase.cpp:153-305 computes its parts and
+ * keeps only the contact):
+ *   bits 0-2   kind: 0 box-box, face of A is the reference face; 1 box-box,
+ *              face of B; 2 box-box edge-edge; 3 sphere-sphere; 4 sphere-box,
+ *              centre outside (closest point); 5 sphere-box, centre inside
+ *   bits 3-4   reference-face axis (0, 1), A's edge axis (2), the face of least
+ *              penetration (5)
+ *   bits 5-6   B's edge axis (2)
+ *   bits 8-15  kinds 0/1: incident vertices k = 0..7 (signs of the half
+ *              extents by the bits of k) that passed the depth and both side
+ *              tests; the first four in k order form the manifold
+ *   bits 16-19 box-box: the SAT axis of least overlap before the reference-face
+ *              choice (0-2 A's faces, 3-5 B's faces, 6 + 3i + j edge i x j) */
+void rc_np_features(const rc_world *w, uint32_t *out, uint32_t n);
 /* Overwrite the contact store (for solver-only tests). */
 void rc_np_set_contacts(rc_world *w, uint32_t n, const uint32_t *body_a, const uint32_t *body_b,
                         const float *px, const float *py, const float *pz,
@@ -180,6 +195,8 @@ void rc_frame_joints(rc_world *w, rc_bodies *b, const rc_shapes *s, const rc_aab
 /* ---- per-element entry points (for function-level parity checks) ----- */
 /* Contact of one pair (narrowphase.cpp:920-1000): ha/hb half extents, pa/pb
  * centres, qa/qb quaternions wxyz.  out[7] = p xyz, n xyz, depth; returns 1 on hit. */
+int rc_pair_contact_feat(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
+                         const float *pb, const float *qa, const float *qb, float *out, uint32_t *feat);
 int rc_pair_contact(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa,
                     const float *pb, const float *qa, const float *qb, float *out);
 /* One solve_contact (xpbd_solver.cpp:413-521) on two bodies given as 20 floats

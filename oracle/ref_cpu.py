@@ -127,6 +127,9 @@ def lib():
                                       C.c_int, C.POINTER(RcFrameStats)]
         L.rc_pair_contact.restype = C.c_int
         L.rc_pair_contact.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
+        L.rc_pair_contact_feat.restype = C.c_int
+        L.rc_pair_contact_feat.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 8
+        L.rc_np_features.argtypes = [vp, vp, C.c_uint32]
         L.rc_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
         L.rc_leaf_extent.argtypes = [C.c_uint32, vp, vp, vp]
         L.rc_karras_node.argtypes = [vp, C.c_int, C.c_int, vp]
@@ -307,6 +310,15 @@ class World:
             out[f] = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).copy() if n else np.zeros(0, dt)
         return out
 
+    def features(self) -> np.ndarray:
+        """Feature words of the stored contacts (see ref_cpu.h)."""
+        c = RcContacts()
+        self.L.rc_np_contacts(self.h, C.byref(c))
+        out = np.zeros(c.n, np.uint32)
+        if c.n:
+            self.L.rc_np_features(self.h, out.ctypes.data, c.n)
+        return out
+
     def set_contacts(self, c: dict):
         n = len(c["body_a"])
         arrs = [np.ascontiguousarray(c[f], dt) for f, dt in (
@@ -357,6 +369,15 @@ class World:
 
 def integrate(bodies: Bodies, gravity, dt: float, n_substeps: int):
     lib().rc_integrate(C.byref(bodies.view()), gravity[0], gravity[1], gravity[2], dt, n_substeps)
+
+
+def pair_contact(ta: int, tb: int, ha, hb, pa, pb, qa, qb):
+    """One pair's contact: (hit, [px py pz nx ny nz depth], feature word)."""
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (ha, hb, pa, pb, qa, qb)]
+    out = np.zeros(7, np.float32)
+    feat = np.zeros(1, np.uint32)
+    hit = lib().rc_pair_contact_feat(ta, tb, *[a.ctypes.data for a in arrs], out.ctypes.data, feat.ctypes.data)
+    return bool(hit), out, int(feat[0])
 
 
 def set_threads(n: int):

@@ -41,6 +41,9 @@ CASES = [
     ("mixed_3000", lambda: scenes.random_mixed(3000, seed=11, extent=(12, 10, 12)), 0),
     ("mixed_3000_f30", lambda: scenes.random_mixed(3000, seed=12, extent=(12, 10, 12)), 30),
     ("mixed_12000", lambda: scenes.random_mixed(12000, seed=13, extent=(24, 16, 24)), 0),
+    # the 32 768-body workload at its stated size: the initial drop and frame 40
+    ("c2_32768_f0", lambda: scenes.random_mixed(32768, seed=20260002), 0),
+    ("c2_32768_f40", lambda: scenes.random_mixed(32768, seed=20260002), 40),
 ]
 
 
@@ -93,6 +96,8 @@ def test_narrowphase_bitexact(gpu, oracle, stage_inputs, case):
     nc_ref = w.narrowphase(pairs, ob, osh)
     ref = w.contacts()
 
+    feat_ref = w.features()
+
     s = gpu.Stream()
     db = _gpu_bodies(gpu, sc, s, ob.state())
     ds = gpu.DeviceShapes(sc, s)
@@ -104,6 +109,18 @@ def test_narrowphase_bitexact(gpu, oracle, stage_inputs, case):
     assert np_.last_count(s) == ref["last_count"]
     for f, _ in gpu.CONTACT_FIELDS:
         assert np.array_equal(bits(got[f]), bits(ref[f])), f
+    # the device-count overload over a larger launch bound, with feature words
+    cnt = gpu.DeviceArray.from_numpy(np.array([len(pairs)], np.uint32), s)
+    bound = max(len(pairs) + 1000, 2 * len(pairs))
+    dp2 = gpu.DeviceArray.from_numpy(np.concatenate([pairs.ravel(), np.zeros(2 * (bound - len(pairs)), np.uint32)]),
+                                     s)
+    np_.set_features(True)
+    np_.run_device_count(s, dp2.ptr, cnt.ptr, bound, db, ds)
+    got2 = np_.contacts(s)
+    assert got2["n"] == nc_ref
+    for f, _ in gpu.CONTACT_FIELDS:
+        assert np.array_equal(bits(got2[f]), bits(ref[f])), f
+    assert np.array_equal(np_.features(s), feat_ref)
 
 
 def _contacts_for(oracle, sc, ob, osh):
@@ -114,7 +131,7 @@ def _contacts_for(oracle, sc, ob, osh):
     return w
 
 
-@pytest.mark.parametrize("case", ["stress_f90", "stress_f140", "mixed_3000", "mixed_3000_f30"])
+@pytest.mark.parametrize("case", ["stress_f90", "stress_f140", "mixed_3000", "mixed_3000_f30", "c2_32768_f40"])
 @pytest.mark.parametrize("force_serial", [False, True])
 def test_solver_bitexact(gpu, oracle, stage_inputs, case, force_serial):
     sc, ob0, osh = stage_inputs[case]
@@ -147,6 +164,25 @@ def test_solver_bitexact(gpu, oracle, stage_inputs, case, force_serial):
     for f in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
               "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z"):
         assert np.array_equal(bits(got[f]), bits(ref[f])), f
+
+
+def test_c2_frames_match_oracle(gpu, oracle, stage_inputs):
+    """The 32 768-body workload, frames 40..43 through the whole frame step
+    (integrate, broadphase, device-count narrowphase, coloured solve): pair and
+    contact counts and the state hash equal the oracle's after every frame."""
+    sc, ob0, osh = stage_inputs["c2_32768_f40"]
+    ob = ob0.copy()
+    ow = oracle.World.for_scene(sc)
+    gw = gpu.World(sc)
+    gw.bodies.upload_state(ob0.state(), gw.s)
+    for f in range(3):
+        print(f"[c2 frames] frame {40 + f}", flush=True)
+        ost = ow.frame(ob, osh, sc)
+        gst = gw.step()
+        assert (gst["n_pairs"], gst["n_contacts"], gst["n_colors"]) == \
+            (ost["n_pairs"], ost["n_contacts"], ost["n_colors"]), f
+        assert gw.fnv() == ob.fnv(), f"frame {40 + f}"
+    assert gst["n_contacts"] > 5000
 
 
 def test_golden_box_hash_on_gpu(gpu):

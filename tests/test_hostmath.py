@@ -119,20 +119,27 @@ def _pair_cases(rng, n):
 
 
 def test_pair_contacts(hm, oracle):
+    """Contact values and feature words (kind, axes, incident-vertex mask,
+    SAT axis) of the kernels' arithmetic equal the oracle's bit for bit."""
     L = oracle.lib()
     hits = 0
+    kinds = set()
     a, b = np.zeros(7, F32), np.zeros(7, F32)
+    fa, fb = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
     for (ta, tb, ha, hb, pa, pb, qa, qb) in _pair_cases(RNG, 20000):
         args = [np.ascontiguousarray(v).ctypes.data for v in (ha, hb, pa, pb, qa, qb)]
         a[:] = 0
         b[:] = 0
-        ra = hm.hm_pair_contact(ta, tb, *args, a.ctypes.data)
-        rb = L.rc_pair_contact(ta, tb, *args, b.ctypes.data)
+        ra = hm.hm_pair_contact_feat(ta, tb, *args, a.ctypes.data, fa.ctypes.data)
+        rb = L.rc_pair_contact_feat(ta, tb, *args, b.ctypes.data, fb.ctypes.data)
         assert ra == rb
         if ra:
             hits += 1
             assert a.view(np.uint32).tolist() == b.view(np.uint32).tolist()
+            assert fa[0] == fb[0]
+            kinds.add(int(fa[0]) & 7)
     assert hits > 5000
+    assert kinds == {0, 1, 2, 3, 4, 5}
 
 
 def _rand_body(rng, dynamic=True):
