@@ -6,60 +6,74 @@
 // (narrowphase.cpp:1077-1127) reduces to an order-preserving compaction of the
 // pair list.  It is done in one pass: workgroups claim 256-pair tiles in
 // ascending order from a ticket, test one pair per lane, scan the hits of the
-// tile and find the tile's output offset by decoupled look-back over the
-// earlier tiles' published sums (Merrill & Garland, "Single-pass Parallel
-// Prefix Scan with Decoupled Look-back", 2016); each contact is then written
-// straight to its slot.  Contacts come out in pair order by construction; no
+// tile, publish the tile's count and sum the earlier tiles' published counts
+// for the tile's output offset (a single-pass scan after Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016, reading
+// every predecessor at once); each contact is then written straight to its
+// slot.  Contacts come out in pair order by construction; no
 // per-pair staging, no sort.  The launch is bounded by the device-held pair
 // count: tiles past it are never claimed.
+#include <cstdlib>
+
 #include "narrowphase_pass.hpp"
 #include "pair_contact.hpp"
 
 namespace dy {
 
 constexpr uint32_t kNpThreads = 256;   // pairs per tile
-constexpr uint32_t kNpMaxGrid = 2048;  // persistent workgroups at most (8 per CU on 256 CUs)
-// Tile status word: value in bits 0-31, state in bits 32-33 (0 not ready,
-// 1 the tile's own hit count, 2 the inclusive count of tiles 0..t).
-constexpr uint64_t kTileAgg = 1ull << 32, kTileInc = 2ull << 32;
+// Persistent workgroups at most: 2 per CU on 256 CUs (a sweep over 256, 512,
+// 1024 and 2048 on the 32 768-body frame put 512 first).
+constexpr uint32_t kNpMaxGrid = 512;
+// Tile count word: the run's epoch in bits 32-63, the tile's hit count in
+// bits 0-31; a word whose epoch is not the run's is not yet published (so the
+// array needs no clearing between runs: each run has a new epoch, and memory
+// from before the first run is zero, which no epoch takes).  The word carries
+// everything it publishes, so it is written and read with relaxed agent-scope
+// atomics: coherent across the XCDs' L2s by the memory model's
+// single-location coherence, with no release/acquire fence (on gfx950 an
+// agent-scope fence writes back or invalidates the whole XCD L2, which the
+// other workgroups are reading bodies from).
 constexpr uint32_t kNpSpinLimit = 1u << 24;
 
-// Exclusive prefix of tile t: sum of the hit counts of tiles 0..t-1, read by
-// wave 0 from the status words of the 64 preceding tiles at a time.
-__device__ __forceinline__ uint32_t np_lookback(uint64_t *status, uint32_t t, uint32_t *err) {
+__device__ __forceinline__ void np_publish(uint64_t *status, uint32_t t, uint32_t epoch, uint32_t count) {
+    __hip_atomic_store(&status[t], (static_cast<uint64_t>(epoch) << 32) | count, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of tile t, by wave 0: the sum of the counts of tiles
+// 0..t-1, all read at once (lane l reads tiles l, l + 64, ...), again until
+// every one of them has been published.  Every tile publishes its count as
+// soon as its pairs are tested, so this is about one round trip; the reads
+// total t words per tile.
+__device__ __forceinline__ uint32_t np_lookback(uint64_t *status, uint32_t t, uint32_t epoch, uint32_t *err) {
     const uint32_t lane = lane_id();
-    uint32_t prefix = 0;
-    int64_t base = static_cast<int64_t>(t) - 1;
-    uint32_t spins = 0;
-    while (base >= 0) {
-        const int64_t j = base - lane;
-        uint64_t w = j >= 0 ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kTileInc;
-        // every tile in the window must have published at least its own count
-        if (__ballot(w < kTileAgg) != 0) {
-            if (++spins > kNpSpinLimit) {
-                if (lane == 0) atomicOr(err, 1u);
-                return prefix;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
+    for (uint32_t spins = 0;; ++spins) {
+        uint32_t sum = 0;
+        bool waiting = false;
+#pragma unroll 8
+        for (uint32_t j = lane; j < t; j += 64) {
+            const uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            waiting = waiting || static_cast<uint32_t>(w >> 32) != epoch;
+            sum += static_cast<uint32_t>(w);
         }
-        const uint64_t inc = __ballot(w >= kTileInc);  // lanes whose tile holds an inclusive count
-        const uint32_t stop = inc ? static_cast<uint32_t>(__builtin_ctzll(inc)) : 64u;
-        uint32_t v = lane <= stop ? static_cast<uint32_t>(w) : 0u;
+        if (__ballot(waiting) == 0) {
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-        prefix += v;
-        if (stop < 64) break;
-        base -= 64;
+            for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+            return sum;
+        }
+        if (spins > kNpSpinLimit) {
+            if (lane == 0) atomicOr(err, 1u);
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
-    return prefix;
 }
 
 // Persistent pass: claim tile t, test its pairs, publish the tile's count,
 // look back for its offset, publish the inclusive count, write the contacts.
 __global__ __launch_bounds__(kNpThreads) void np_fused(const dy_contact_pair *__restrict__ pairs, uint32_t n_bound,
                                                        const uint32_t *__restrict__ d_n, dy_body_view bv,
-                                                       dy_shape_view sv, uint64_t *status, uint32_t *ticket,
+                                                       dy_shape_view sv, uint64_t *status, uint32_t epoch,
                                                        NpOut out) {
     __shared__ uint32_t wsum[17];
     __shared__ uint32_t s_tile, s_prefix;
@@ -67,7 +81,7 @@ __global__ __launch_bounds__(kNpThreads) void np_fused(const dy_contact_pair *__
     const uint32_t n_tiles = (n + kNpThreads - 1) / kNpThreads;
     const uint32_t cap = out.cv.capacity;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+        if (threadIdx.x == 0) s_tile = atomicAdd(&out.counts[kNpTicket], 1u);
         __syncthreads();
         const uint32_t t = s_tile;
         if (t >= n_tiles) break;  // uniform: every wave of the workgroup leaves here
@@ -83,18 +97,8 @@ __global__ __launch_bounds__(kNpThreads) void np_fused(const dy_contact_pair *__
         uint32_t total;
         const uint32_t local = block_exclusive_scan(hit, wsum, total);
         if (threadIdx.x < 64) {
-            uint32_t prefix = 0;
-            if (t == 0) {
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(&status[0], kTileInc | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(&status[t], kTileAgg | total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                prefix = np_lookback(status, t, &out.counts[3]);
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(&status[t], kTileInc | (prefix + total), __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (threadIdx.x == 0) np_publish(status, t, epoch, total);
+            const uint32_t prefix = t ? np_lookback(status, t, epoch, &out.counts[3]) : 0u;
             if (threadIdx.x == 0) {
                 s_prefix = prefix;
                 if (t == n_tiles - 1) {  // the last tile knows the true count
@@ -119,20 +123,24 @@ __global__ __launch_bounds__(kNpThreads) void np_fused(const dy_contact_pair *__
     }
 }
 
-// Persistent workgroups for a launch over at most n_bound pairs.
+// Persistent workgroups for a launch over at most n_bound pairs
+// (DYPHUR_NP_GRID overrides the cap of kNpMaxGrid).
 uint32_t np_grid(uint32_t n_bound) {
+    static const uint32_t cap = [] {
+        const char *env = std::getenv("DYPHUR_NP_GRID");
+        const int v = env ? std::atoi(env) : 0;
+        return v > 0 ? static_cast<uint32_t>(v) : kNpMaxGrid;
+    }();
     const uint32_t tiles = ceil_div(n_bound, kNpThreads);
-    return tiles < kNpMaxGrid ? tiles : kNpMaxGrid;
+    return tiles < cap ? tiles : cap;
 }
 
 int np_enqueue(hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound, const uint32_t *d_n,
-               const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t *ticket, NpOut out) {
+               const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t epoch, NpOut out) {
+    if (epoch == 0) return DY_ERR_INVALID_ARG;
     DY_TRY(hipMemsetAsync(out.counts, 0, kNpCounts * sizeof(uint32_t), st));
     if (n_bound == 0) return DY_OK;
-    const uint32_t tiles = ceil_div(n_bound, kNpThreads);
-    DY_TRY(hipMemsetAsync(status, 0, tiles * sizeof(uint64_t), st));
-    DY_TRY(hipMemsetAsync(ticket, 0, sizeof(uint32_t), st));
-    np_fused<<<np_grid(n_bound), kNpThreads, 0, st>>>(pairs, n_bound, d_n, bv, sv, status, ticket, out);
+    np_fused<<<np_grid(n_bound), kNpThreads, 0, st>>>(pairs, n_bound, d_n, bv, sv, status, epoch, out);
     DY_LAUNCH_CHECK();
     return DY_OK;
 }
@@ -149,24 +157,32 @@ struct dy_narrowphase_s {
     uint32_t *counts = nullptr;  // kNpCounts words
     uint32_t *feat = nullptr;    // feature words (dy_narrowphase_set_features), else null
     bool want_feat = false;
-    // tile status (grows with the launch bound) + ticket
+    // tile count words (grow with the launch bound; zeroed when allocated)
     DevBuf<uint64_t> status;
-    uint32_t *ticket = nullptr;
+    uint32_t epoch = 0;
 };
 
 namespace {
 int np_launch(dy_narrowphase_s *np, hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound,
               const uint32_t *d_n, const dy_body_view &bv, const dy_shape_view &sv) {
-    DY_TRY_STATUS(np->status.ensure(ceil_div(n_bound > 0 ? n_bound : 1u, kNpThreads)));
+    const size_t tiles = ceil_div(n_bound > 0 ? n_bound : 1u, kNpThreads);
+    if (np->status.cap < tiles) {
+        DY_TRY_STATUS(np->status.ensure(tiles));
+        DY_TRY(hipMemsetAsync(np->status.p, 0, np->status.cap * sizeof(uint64_t), st));
+    }
+    if (++np->epoch == 0) {  // after 2^32 - 1 runs: clear once and start again
+        DY_TRY(hipMemsetAsync(np->status.p, 0, np->status.cap * sizeof(uint64_t), st));
+        np->epoch = 1;
+    }
     const dy_contact_view cv{np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth,
                              np->counts, np->cap};
-    return np_enqueue(st, pairs, n_bound, d_n, bv, sv, np->status.p, np->ticket,
+    return np_enqueue(st, pairs, n_bound, d_n, bv, sv, np->status.p, np->epoch,
                       NpOut{cv, np->counts, np->want_feat ? np->feat : nullptr});
 }
 
 void np_free(dy_narrowphase_s *np) {
     void *ptrs[] = {np->body_a, np->body_b, np->px, np->py, np->pz, np->nx, np->ny, np->nz, np->depth,
-                    np->counts, np->feat, np->ticket};
+                    np->counts, np->feat};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     np->status.release();
@@ -190,7 +206,6 @@ int dy_narrowphase_create(dy_stream_t s, uint32_t max_contacts, dy_narrowphase_t
     for (auto p : f)
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(p), c * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&np->counts), kNpCounts * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&np->ticket), sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemsetAsync(np->counts, 0, kNpCounts * sizeof(uint32_t), hs(s));
     if (e != hipSuccess) {
         np_free(np);

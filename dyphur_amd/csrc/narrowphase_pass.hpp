@@ -6,8 +6,9 @@
 namespace dy {
 
 // counts[]: [0] stored (clamped to capacity), [1] true count, [2] an
-// unsupported shape was met, [3] a look-back wait gave up (internal error).
-constexpr uint32_t kNpCounts = 4;
+// unsupported shape was met, [3] a wait for an earlier tile's count gave up
+// (internal error), [4] the tile ticket.
+constexpr uint32_t kNpCounts = 5, kNpTicket = 4;
 
 struct NpOut {
     dy_contact_view cv;  // the store: arrays and capacity (cv.n is not read)
@@ -16,10 +17,10 @@ struct NpOut {
 };
 
 // Enqueues one pass over at most n_bound pairs (the first min(*d_n, n_bound)
-// when d_n is not null): clears counts, status and ticket, then tests the
-// pairs and writes the contacts in pair order.  status must hold
-// ceil(n_bound / 256) words, ticket one word.
+// when d_n is not null): clears counts, then tests the pairs and writes the
+// contacts in pair order.  status holds ceil(n_bound / 256) tile words, each
+// zero or tagged with an epoch other than this run's (epoch != 0).
 int np_enqueue(hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound, const uint32_t *d_n,
-               const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t *ticket, NpOut out);
+               const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t epoch, NpOut out);
 
 }  // namespace dy
