@@ -26,8 +26,8 @@ BUILD = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DYPHUR_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "primitives.hip", "broadphase.hip", "narrowphase.hip", "solver.hip",
-               "batch.hip", "ray_query.hip"]
+HIP_SOURCES = ["runtime.hip", "primitives.hip", "broadphase.hip", "narrowphase.hip", "narrowphase_noalloc.hip",
+               "solver.hip", "batch.hip", "ray_query.hip"]
 
 # -ffp-contract=off: parity with the reference requires every float op to round
 # on its own.  Packed-FP32 instructions are disabled: some of their forms have

@@ -157,6 +157,8 @@ _SIGS = {
     "dy_narrowphase_last_count": (C.c_int, [vp, vp, C.POINTER(u32)]),
     "dy_narrowphase_set_features": (C.c_int, [vp, C.c_int]),
     "dy_narrowphase_features": (C.c_int, [vp, C.POINTER(vp)]),
+    "dy_narrowphase_run_noalloc": (C.c_int, [vp, vp, vp, u32, C.POINTER(BodyView), C.POINTER(ShapeView),
+                                              C.POINTER(ContactView), vp, vp]),
     "dy_xpbd_create": (C.c_int, [vp, C.c_int, u32, f32, C.POINTER(vp)]),
     "dy_xpbd_destroy": (C.c_int, [vp]),
     "dy_xpbd_set_force_serial": (C.c_int, [vp, C.c_int]),

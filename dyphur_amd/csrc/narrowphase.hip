@@ -20,7 +20,7 @@
 
 namespace dy {
 
-constexpr uint32_t kNpThreads = 256;   // pairs per tile
+constexpr uint32_t kNpThreads = kNpTilePairs;
 // Persistent workgroups at most: 2 per CU on 256 CUs (a sweep over 256, 512,
 // 1024 and 2048 on the 32 768-body frame put 512 first).
 constexpr uint32_t kNpMaxGrid = 512;

@@ -8,7 +8,9 @@ namespace dy {
 // counts[]: [0] stored (clamped to capacity), [1] true count, [2] an
 // unsupported shape was met, [3] a wait for an earlier tile's count gave up
 // (internal error), [4] the tile ticket.
-constexpr uint32_t kNpCounts = 5, kNpTicket = 4;
+constexpr uint32_t kNpCounts = DY_NARROWPHASE_COUNT_WORDS, kNpTicket = 4;
+static_assert(kNpCounts == 5, "counts layout");
+constexpr uint32_t kNpTilePairs = 256;  // pairs per tile (one per lane of a workgroup)
 
 struct NpOut {
     dy_contact_view cv;  // the store: arrays and capacity (cv.n is not read)

@@ -289,6 +289,25 @@ int dy_narrowphase_last_count(dy_narrowphase_t np, dy_stream_t s, uint32_t *out)
  * features() gives the device array (DY_ERR_INVALID_ARG while disabled). */
 int dy_narrowphase_set_features(dy_narrowphase_t np, int enable);
 int dy_narrowphase_features(dy_narrowphase_t np, const uint32_t **d_features);
+/* Allocation-free narrowphase.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same pass and results
+ * as dy_narrowphase_run_device_count over the first min(*d_n_pairs, max_pairs)
+ * pairs (all max_pairs when d_n_pairs is NULL), written into the caller's
+ * store: store's arrays and capacity (store->n is not read), d_counts =
+ * DY_NARROWPHASE_COUNT_WORDS device words, on return [0] the stored count,
+ * [1] the true count, [2] nonzero if a hull or mesh shape was met, [3]
+ * nonzero on an internal error ([4] is working space); d_features (may be
+ * NULL) receives a feature word per stored contact.  A contact view with
+ * n = d_counts feeds dy_xpbd_solve.  Calls no hipMalloc/hipFree: its only
+ * working memory is a fixed 512 KiB device array per GPU, so max_pairs is at
+ * most 16 777 216 (larger: DY_ERR_INVALID_ARG).  Calls that overlap on one GPU
+ * (other streams, other threads) share that array and race; calls in one
+ * stream's order do not.  Asynchronous. */
+#define DY_NARROWPHASE_COUNT_WORDS 5
+int dy_narrowphase_run_noalloc(dy_stream_t s, const dy_contact_pair *d_pairs, const uint32_t *d_n_pairs,
+                               uint32_t max_pairs, const dy_body_view *bodies, const dy_shape_view *shapes,
+                               const dy_contact_view *store, uint32_t *d_counts, uint32_t *d_features);
 
 /* XpbdSolver(Stream&, n_iters, max_contacts, friction) (core/include/core/xpbd_solver.hpp:31-34) */
 int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float friction, dy_xpbd_t *out);
