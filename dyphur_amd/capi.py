@@ -11,8 +11,11 @@ BodyView / ShapeView.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -224,6 +227,52 @@ def check(status: int, what: str):
         raise DyError(status, what)
 
 
+# ---------------------------------------------------------------------------
+# Lifetimes: every object that owns a device resource (stream, event, graph,
+# stage object, batch, tape, allocation) is tracked, and release_all() closes
+# the live ones newest first.  It runs at interpreter exit (atexit, while the
+# library is still loaded), so the HIP runtime has nothing of ours left to
+# tear down inside exit(), where a profiler that intercepts the runtime may
+# already have finalised.
+# ---------------------------------------------------------------------------
+_live: "dict[int, weakref.ref]" = {}
+_next_id = itertools.count()
+
+
+def _track(obj) -> None:
+    k = next(_next_id)
+    _live[k] = weakref.ref(obj, lambda _r, k=k: _live.pop(k, None))
+
+
+def release_all() -> None:
+    """Closes every live stream, event, graph, stage object, batch, tape and
+    device allocation made through this module, newest first."""
+    for k in sorted(_live, reverse=True):
+        r = _live.pop(k, None)
+        o = r() if r is not None else None
+        if o is not None:
+            try:
+                o.close()
+            except Exception:
+                pass
+
+
+atexit.register(release_all)
+
+
+class _Owned:
+    """Base of the resource owners: close() releases once, __del__ closes."""
+
+    def close(self):  # pragma: no cover - overridden
+        pass
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_count() -> int:
     n = C.c_int(0)
     check(lib().dy_device_count(C.byref(n)), "dy_device_count")
@@ -233,32 +282,29 @@ def device_count() -> int:
 # ---------------------------------------------------------------------------
 # Streams, events, device memory
 # ---------------------------------------------------------------------------
-class Stream:
+class Stream(_Owned):
     def __init__(self):
         h = vp()
         check(lib().dy_stream_create(C.byref(h)), "dy_stream_create")
         self.h = h
+        _track(self)
 
     def wait(self):
         check(lib().dy_stream_synchronize(self.h), "dy_stream_synchronize")
 
     def close(self):
         if getattr(self, "h", None):
+            lib().dy_stream_synchronize(self.h)
             lib().dy_stream_destroy(self.h)
             self.h = None
 
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
-
-class Event:
+class Event(_Owned):
     def __init__(self):
         h = vp()
         check(lib().dy_event_create(C.byref(h)), "dy_event_create")
         self.h = h
+        _track(self)
 
     def record(self, s: Stream):
         check(lib().dy_event_record(self.h, s.h), "dy_event_record")
@@ -271,15 +317,13 @@ class Event:
         check(lib().dy_event_elapsed_ms(self.h, end.h, C.byref(ms)), "dy_event_elapsed_ms")
         return ms.value
 
-    def __del__(self):
-        try:
-            if getattr(self, "h", None):
-                lib().dy_event_destroy(self.h)
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_event_destroy(self.h)
+            self.h = None
 
 
-class DeviceArray:
+class DeviceArray(_Owned):
     """A device allocation with a dtype and length."""
 
     def __init__(self, n: int, dtype):
@@ -289,6 +333,7 @@ class DeviceArray:
         p = vp()
         check(lib().dy_malloc(C.byref(p), max(self.nbytes, 1)), "dy_malloc")
         self.ptr = p.value
+        _track(self)
 
     @classmethod
     def from_numpy(cls, a: np.ndarray, s: Stream) -> "DeviceArray":
@@ -315,11 +360,8 @@ class DeviceArray:
             lib().dy_free(vp(self.ptr))
             self.ptr = None
 
-    def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+    def close(self):
+        self.free()
 
 
 def download_ptr(ptr: int, n: int, dtype, s: Stream) -> np.ndarray:
@@ -466,11 +508,12 @@ def integrate(s: Stream, bodies: DeviceBodies, gravity, dt: float, n_substeps: i
     check(lib().dy_integrate(s.h, C.byref(v), gravity[0], gravity[1], gravity[2], dt, n_substeps), "dy_integrate")
 
 
-class Broadphase:
+class Broadphase(_Owned):
     def __init__(self, s: Stream, max_bodies: int, max_pairs: int):
         h = vp()
         check(lib().dy_broadphase_create(s.h, max_bodies, max_pairs, C.byref(h)), "dy_broadphase_create")
         self.h = h
+        _track(self)
         self.max_bodies, self.max_pairs = max_bodies, max_pairs
 
     def build_and_query(self, s: Stream, bodies: DeviceBodies, shapes: DeviceShapes, bounds):
@@ -519,20 +562,18 @@ class Broadphase:
                        aabb=download_ptr(v.aabb, 8 * (2 * n - 1), np.uint16, s).reshape(2 * n - 1, 8))
         return out
 
-    def __del__(self):
-        try:
-            if getattr(self, "h", None):
-                lib().dy_broadphase_destroy(self.h)
-                self.h = None
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_broadphase_destroy(self.h)
+            self.h = None
 
 
-class Narrowphase:
+class Narrowphase(_Owned):
     def __init__(self, s: Stream, max_contacts: int):
         h = vp()
         check(lib().dy_narrowphase_create(s.h, max_contacts, C.byref(h)), "dy_narrowphase_create")
         self.h = h
+        _track(self)
         self.max_contacts = max_contacts
 
     def run(self, s: Stream, pairs_ptr: int, n_pairs: int, bodies: DeviceBodies, shapes: DeviceShapes):
@@ -579,20 +620,18 @@ class Narrowphase:
             out[f] = download_ptr(getattr(v, f), n, dt, s)
         return out
 
-    def __del__(self):
-        try:
-            if getattr(self, "h", None):
-                lib().dy_narrowphase_destroy(self.h)
-                self.h = None
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_narrowphase_destroy(self.h)
+            self.h = None
 
 
-class XpbdSolver:
+class XpbdSolver(_Owned):
     def __init__(self, s: Stream, n_iters: int = 10, max_contacts: int = 16384, friction: float = 0.5):
         h = vp()
         check(lib().dy_xpbd_create(s.h, n_iters, max_contacts, friction, C.byref(h)), "dy_xpbd_create")
         self.h = h
+        _track(self)
         self.max_contacts = max_contacts
 
     def set_force_serial(self, v: bool):
@@ -622,13 +661,10 @@ class XpbdSolver:
         check(lib().dy_xpbd_download_classes(self.h, s.h, out.ctypes.data, n), "dy_xpbd_download_classes")
         return out
 
-    def __del__(self):
-        try:
-            if getattr(self, "h", None):
-                lib().dy_xpbd_destroy(self.h)
-                self.h = None
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_xpbd_destroy(self.h)
+            self.h = None
 
 
 class World:
@@ -710,12 +746,13 @@ def ray_query(s: Stream, bp: "Broadphase", n: int, bodies: "DeviceBodies", shape
     return out[:, 0].view(np.float32).copy(), out[:, 1].copy()
 
 
-class Graph:
+class Graph(_Owned):
     """Calls issued on stream s inside ``with Graph.capture(s) as g:`` are
     recorded into a HIP graph; ``g.launch(s)`` replays them."""
 
     def __init__(self):
         self.h = vp()
+        _track(self)
 
     @classmethod
     def capture(cls, s: Stream) -> "_Capture":
@@ -724,13 +761,10 @@ class Graph:
     def launch(self, s: Stream):
         check(lib().dy_graph_launch(self.h, s.h), "dy_graph_launch")
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().dy_graph_destroy(self.h)
-                self.h = vp()
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_graph_destroy(self.h)
+            self.h = vp()
 
 
 class _Capture:
@@ -752,7 +786,7 @@ BATCH_STAT_FIELDS = ("n_pairs", "n_contacts", "n_classes", "used_serial", "round
                      "substeps", "sum_pairs", "sum_contacts", "sum_dyn_endpoints", "solves", "grid_substeps")
 
 
-class Batch:
+class Batch(_Owned):
     """Many independent scenes stepped together (dy_batch_*): one launch runs
     every scene's frames, one workgroup per scene.  Body indices in pairs and
     contacts are scene-local, as each scene's own World would report them."""
@@ -778,6 +812,7 @@ class Batch:
         check(lib().dy_batch_create(self.s.h, len(self.scenes), self.body_offset.ctypes.data,
                                     self.max_pairs.ctypes.data, self.max_contacts.ctypes.data,
                                     int(self.n_iters), float(self.mu), C.byref(self.h)), "dy_batch_create")
+        _track(self)
         lay = BatchLayout()
         check(lib().dy_batch_layout_get(self.h, C.byref(lay)), "dy_batch_layout_get")
         self.layout = lay
@@ -870,20 +905,17 @@ class Batch:
     def scene_slice(self, i: int) -> slice:
         return slice(int(self.body_offset[i]), int(self.body_offset[i + 1]))
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().dy_batch_destroy(self.h)
-                self.h = vp()
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_batch_destroy(self.h)
+            self.h = vp()
 
 
 MUTABLE_FIELDS = ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y", "rot_z",
                   "vel_x", "vel_y", "vel_z", "ang_x", "ang_y", "ang_z")
 
 
-class Diff:
+class Diff(_Owned):
     """Reverse-mode derivatives of a Batch's frames (dy_diff_*): forward()
     steps the batch keeping checkpoints; backward() turns the adjoint of the
     final state into the adjoint of the initial state and of inv_mass."""
@@ -895,6 +927,7 @@ class Diff:
         self.n_substeps = n_substeps
         check(lib().dy_diff_create(batch.h, max_frames, checkpoint_every, n_substeps, tape_contacts_per_body,
                                    tape_bytes, C.byref(self.h)), "dy_diff_create")
+        _track(self)
         n = batch.n_bodies
         self.adj = {k: DeviceArray(n, np.float32) for k in MUTABLE_FIELDS + ("inv_mass",)}
         self.frames = 0
@@ -942,10 +975,7 @@ class Diff:
             return None
         return {k: self.adj[k].download(b.s) for k in self.adj}
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().dy_diff_destroy(self.h)
-                self.h = vp()
-        except Exception:
-            pass
+    def close(self):
+        if getattr(self, "h", None):
+            lib().dy_diff_destroy(self.h)
+            self.h = vp()
