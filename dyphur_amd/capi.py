@@ -797,8 +797,11 @@ class Batch(_Owned):
         self.scenes = list(scene_list)
         if any(getattr(sc, "joints", None) is not None for sc in self.scenes):
             raise ValueError("the batched form solves contacts only; step scenes with joints with World")
+        overridden = {"n_iters"} if n_iters is not None else set()
+        if friction is not None:
+            overridden.add("mu")
+        self.merged = _scenes.concat(self.scenes, uniform=[k for k in _scenes.RUN_PARAMS if k not in overridden])
         self.s = s or Stream()
-        self.merged = _scenes.concat(self.scenes)
         m = self.merged.meta
         self.body_offset = np.ascontiguousarray(m["body_offset"], np.uint32)
         self.max_pairs = np.ascontiguousarray(m["max_pairs"], np.uint32)

@@ -453,12 +453,24 @@ def diff_at_scale(n_scenes: int = 1024, seed: int = 20260005, median: float = 12
     return [diff_mixed(int(c), rng) for c in counts]
 
 
-def concat(scenes: list[Scene], name: str = "batch") -> Scene:
+RUN_PARAMS = ("dt", "n_substeps", "n_iters", "mu", "gravity")
+
+
+def concat(scenes: list[Scene], name: str = "batch", uniform=RUN_PARAMS) -> Scene:
     """Scenes laid end to end for the batched step: body arrays concatenated,
     each body's shape index offset into the concatenated catalog.  meta holds
-    body_offset (n_scenes + 1) and the per-scene pair / contact capacities."""
+    body_offset (n_scenes + 1) and the per-scene pair / contact capacities.
+    The batch steps every scene with one set of run parameters, so each
+    parameter named in `uniform` must be the same in every scene (ValueError
+    otherwise)."""
     if not scenes:
         raise ValueError("no scenes")
+    for key in uniform:
+        v0 = tuple(np.atleast_1d(getattr(scenes[0], key)).tolist())
+        for i, sc in enumerate(scenes[1:], 1):
+            if tuple(np.atleast_1d(getattr(sc, key)).tolist()) != v0:
+                raise ValueError(f"scene {i} has {key} = {getattr(sc, key)!r}, scene 0 has "
+                                 f"{getattr(scenes[0], key)!r}: a batch steps every scene with the same {key}")
     body_off = np.zeros(len(scenes) + 1, np.uint32)
     body_off[1:] = np.cumsum([sc.n for sc in scenes])
     shape_off = np.zeros(len(scenes) + 1, np.int64)

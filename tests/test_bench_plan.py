@@ -69,3 +69,21 @@ def test_kinetic_energy_closed_form():
     if dyn[1]:
         want += 0.5 * 3.0 * 1.0
     assert math.isclose(bench.kinetic_energy(st, sc), want, rel_tol=1e-6)
+
+
+@pytest.mark.parametrize("field,value", [("dt", 1.0 / 120.0), ("n_substeps", 2), ("n_iters", 4), ("mu", 0.2),
+                                         ("gravity", (0.0, -1.62, 0.0))])
+def test_batch_rejects_scenes_with_different_run_parameters(field, value):
+    """A batch steps all its scenes with one dt, substep count, iteration count,
+    friction and gravity: a list that disagrees is refused before any device
+    work (scenes.concat, which Batch calls first)."""
+    from dyphur_amd import capi, scenes
+    a, b = scenes.stress_blocks(2, 2, 2), scenes.stress_blocks(2, 3, 2)
+    setattr(b, field, value)
+    with pytest.raises(ValueError, match=field):
+        scenes.concat([a, b])
+    with pytest.raises(ValueError, match=field):
+        capi.Batch([a, b])
+    if field in ("n_iters", "mu"):  # Batch's explicit override applies one value to every scene
+        uniform = [k for k in scenes.RUN_PARAMS if k != field]
+        assert scenes.concat([a, b], uniform=uniform).n == a.n + b.n
