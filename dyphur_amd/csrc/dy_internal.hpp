@@ -8,6 +8,17 @@
 #include "../../include/dyphur_hip.h"
 #include "dy_math.hpp"
 
+// The one-XCD handoffs below (xcd_barrier, the dataflow sweep's body records
+// and versions) rely on gfx950's cache hierarchy: the vector L1 writes through
+// to the XCD's L2, a store is in the L2 once s_waitcnt vmcnt(0) has retired
+// it, and agent-scope (sc1) loads are served by the L2, so workgroups of one
+// XCD see each other's completed stores without cache maintenance.  The HIP
+// memory model does not promise this for plain stores, so the device code is
+// built for gfx950 only.
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(__gfx950__)
+#error "dyphur_amd's device code relies on gfx950's L1/L2 behaviour; build with --offload-arch=gfx950"
+#endif
+
 #define DY_TRY(expr)                                          \
     do {                                                      \
         hipError_t dy_e_ = (expr);                            \
