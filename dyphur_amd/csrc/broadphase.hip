@@ -46,6 +46,16 @@ struct alignas(16) TNode {
     uint32_t pad0, pad1;
 };
 
+// Walk record of internal node p, 48 B: both children's fp16 boxes, subtree
+// max body indices and node numbers, so a walk tests the children of the
+// node it loads and never loads a leaf.
+struct alignas(16) TPair {
+    uint32_t lb01, lb23, lb45, lmax;
+    uint32_t rb01, rb23, rb45, rmax;
+    int32_t left, right;
+    uint32_t pad0, pad1;
+};
+
 struct Box6 { float mn[3], mx[3]; };
 
 __device__ __forceinline__ Box6 unpack_box(uint32_t b01, uint32_t b23, uint32_t b45) {
@@ -81,6 +91,14 @@ __device__ __forceinline__ void merge_packed(const uint32_t a[3], const uint32_t
     o[0] = r[0] | (static_cast<uint32_t>(r[1]) << 16);
     o[1] = r[2] | (static_cast<uint32_t>(r[3]) << 16);
     o[2] = r[4] | (static_cast<uint32_t>(r[5]) << 16);
+}
+
+__device__ __forceinline__ void store_pair(TPair *tp, int par, const uint32_t lb[3], uint32_t lmax,
+                                           const uint32_t rb[3], uint32_t rmax, int l, int r) {
+    uint4 *w = reinterpret_cast<uint4 *>(&tp[par]);
+    w[0] = make_uint4(lb[0], lb[1], lb[2], lmax);
+    w[1] = make_uint4(rb[0], rb[1], rb[2], rmax);
+    w[2] = make_uint4(static_cast<uint32_t>(l), static_cast<uint32_t>(r), 0u, 0u);
 }
 
 __device__ __forceinline__ void store_node(dy_node_box *nb, TNode *tn, int node, const uint32_t p[3],
@@ -359,7 +377,7 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
                                                      const int32_t *__restrict__ left,
                                                      const int32_t *__restrict__ right,
                                                      const int2 *__restrict__ range, dy_node_box *nodebox,
-                                                     TNode *tn, uint32_t *cnt_top, uint32_t *entries,
+                                                     TNode *tn, TPair *tp, uint32_t *cnt_top, uint32_t *entries,
                                                      uint32_t *counters /* [0] entries, [1] done */) {
     __shared__ uint32_t lbox[2][kChunk][3];  // [0] leaves, [1] internal nodes
     __shared__ uint32_t lmax[2][kChunk];
@@ -417,6 +435,7 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
             merge_packed(cb[0], cb[1], m);
             const uint32_t mb = cm[0] > cm[1] ? cm[0] : cm[1];
             store_node(nodebox, tn, par, m, mb);
+            store_pair(tp, par, cb[0], cm[0], cb[1], cm[1], c[0], c[1]);
             lbox[1][par - base][0] = m[0]; lbox[1][par - base][1] = m[1]; lbox[1][par - base][2] = m[2];
             lmax[1][par - base] = mb;
             node = par;
@@ -450,6 +469,7 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
             merge_packed(a, b, m);
             const uint32_t ml = tn[l].maxbody, mr = tn[r].maxbody;
             store_node(nodebox, tn, par, m, ml > mr ? ml : mr);
+            store_pair(tp, par, a, ml, b, mr, l, r);
             node = par;
         }
     }
@@ -476,40 +496,71 @@ __device__ __forceinline__ uint32_t row_insert(uint32_t (&s)[kRow], uint32_t x) 
 }
 
 // Walk the tree for query box q; count partners with body index > thr and
-// keep the kRow smallest of them in s.  Returns the count.
-__device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, int n, const Box6 &q, uint32_t thr,
-                                         uint32_t (&s)[kRow], uint32_t *err, uint32_t *ovf = nullptr) {
+// keep the kRow smallest of them in s (the rest go to ovf when given).
+// Depth-first over internal nodes: a node is visited once its own box has
+// passed the test at its parent, and its record tests both children, so each
+// overlapping internal node is loaded once and no leaf is loaded.  The next
+// node stays in a register; a per-lane stack in LDS (stk: this lane's column
+// of a [kStack][kWalkThreads] array) holds the right children deferred while
+// the left subtree is walked, so LDS is touched only where both children
+// pass and where a subtree ends.  kStack entries hold any tree of at most
+// kStack levels; a deeper one flags err.
+constexpr uint32_t kWalkThreads = 64;
+constexpr uint32_t kStack = 68;
+__device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, const TPair *__restrict__ tp, int n,
+                                         const Box6 &q, uint32_t thr, uint32_t (&s)[kRow], uint32_t *stk,
+                                         uint32_t *err, uint32_t *ovf = nullptr) {
     uint32_t count = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kRow; ++j) s[j] = 0xffffffffu;
-    int node = 0;
-    // A depth-first walk visits each of the 2n-1 nodes at most once.
-    const int max_steps = 2 * n;
-    for (int steps = 0; node != kEnd; ++steps) {
-        if (steps >= max_steps || node < 0 || node >= 2 * n - 1) { atomicOr(err, 4u); break; }
-        const uint4 w0 = *reinterpret_cast<const uint4 *>(&tn[node]);
-        const int2 w1 = *reinterpret_cast<const int2 *>(&tn[node].child);
-        const Box6 b = unpack_box(w0.x, w0.y, w0.z);
-        if (w0.w > thr && overlaps(b, q)) {
-            if (node >= n - 1) {
-                const uint32_t ev = row_insert(s, static_cast<uint32_t>(w1.x));
-                if (ovf && ev != 0xffffffffu && count - kRow < kOvf) ovf[count - kRow] = ev;
-                ++count;
-                node = w1.y;
-            } else {
-                node = w1.x;
-            }
+    const int n_int = n - 1;
+    int x = -1;
+    {
+        const uint4 r = *reinterpret_cast<const uint4 *>(&tn[0]);
+        if (r.w > thr && overlaps(unpack_box(r.x, r.y, r.z), q)) x = 0;
+    }
+    uint32_t sp = 0;
+    while (x >= 0) {
+        if (x >= n_int) { atomicOr(err, 4u); break; }
+        const uint4 w0 = *reinterpret_cast<const uint4 *>(&tp[x]);
+        const uint4 w1 = *(reinterpret_cast<const uint4 *>(&tp[x]) + 1);
+        const int2 ch = *reinterpret_cast<const int2 *>(&tp[x].left);
+        const bool pl = w0.w > thr && overlaps(unpack_box(w0.x, w0.y, w0.z), q);
+        const bool pr = w1.w > thr && overlaps(unpack_box(w1.x, w1.y, w1.z), q);
+        const bool il = ch.x < n_int, ir = ch.y < n_int;
+        if (pl && !il) {
+            const uint32_t ev = row_insert(s, w0.w);
+            if (ovf && ev != 0xffffffffu && count - kRow < kOvf) ovf[count - kRow] = ev;
+            ++count;
+        }
+        if (pr && !ir) {
+            const uint32_t ev = row_insert(s, w1.w);
+            if (ovf && ev != 0xffffffffu && count - kRow < kOvf) ovf[count - kRow] = ev;
+            ++count;
+        }
+        const bool gl = pl && il, gr = pr && ir;
+        if (gl && gr) {  // defer the right subtree
+            if (sp >= kStack) { atomicOr(err, 4u); break; }
+            stk[(sp++) * kWalkThreads] = static_cast<uint32_t>(ch.y);
+            x = ch.x;
+        } else if (gl) {
+            x = ch.x;
+        } else if (gr) {
+            x = ch.y;
         } else {
-            node = w1.y;
+            x = sp > 0 ? static_cast<int>(stk[(--sp) * kWalkThreads]) : -1;
         }
     }
     return count;
 }
 
-__global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ tn, int n,
-                                                      uint32_t *__restrict__ counts, uint32_t *__restrict__ row,
-                                                      uint32_t *__restrict__ ovf, uint32_t *__restrict__ leaf_of,
-                                                      uint32_t *err, uint32_t xcd_map) {
+__global__ __launch_bounds__(kWalkThreads) void traverse_count(const TNode *__restrict__ tn,
+                                                               const TPair *__restrict__ tp, int n,
+                                                               uint32_t *__restrict__ counts,
+                                                               uint32_t *__restrict__ row, uint32_t *__restrict__ ovf,
+                                                               uint32_t *__restrict__ leaf_of, uint32_t *err,
+                                                               uint32_t xcd_map) {
+    __shared__ uint32_t stk[kStack * kWalkThreads];
     // With xcd_map the launch's workgroups, which the hardware deals out to
     // the 8 XCDs round-robin, take contiguous runs of leaves per XCD: walks of
     // neighbouring leaves share subtrees, so each XCD's L2 holds one region.
@@ -524,7 +575,7 @@ __global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ 
     const uint32_t a = static_cast<uint32_t>(leaf.child);
     const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     uint32_t s[kRow];
-    const uint32_t c = walk(tn, n, q, a, s, err, ovf + static_cast<size_t>(a) * kOvf);
+    const uint32_t c = walk(tn, tp, n, q, a, s, stk + threadIdx.x, err, ovf + static_cast<size_t>(a) * kOvf);
     counts[a] = c;
     leaf_of[a] = static_cast<uint32_t>(k);
     const uint32_t m = c < kRow ? c : kRow;
@@ -532,13 +583,15 @@ __global__ __launch_bounds__(256) void traverse_count(const TNode *__restrict__ 
         if (j < m) row[static_cast<size_t>(a) * kRow + j] = s[j];
 }
 
-__global__ __launch_bounds__(256) void emit_pairs(const TNode *__restrict__ tn, int n,
+__global__ __launch_bounds__(kWalkThreads) void emit_pairs(const TNode *__restrict__ tn,
+                                                           const TPair *__restrict__ tp, int n,
                                                   const uint32_t *__restrict__ counts,
                                                   const uint32_t *__restrict__ offs, const uint32_t *__restrict__ row,
                                                   const uint32_t *__restrict__ ovf,
                                                   const uint32_t *__restrict__ leaf_of,
                                                   dy_contact_pair *__restrict__ pairs, uint32_t cap,
                                                   uint32_t *err) {
+    __shared__ uint32_t stk[kStack * kWalkThreads];
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= static_cast<uint32_t>(n)) return;
     const uint32_t L = counts[a];
@@ -577,7 +630,7 @@ __global__ __launch_bounds__(256) void emit_pairs(const TNode *__restrict__ tn, 
     const TNode leaf = tn[n - 1 + leaf_of[a]];
     const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     while (done < L && off + done < cap) {
-        walk(tn, n, q, thr, s, err);
+        walk(tn, tp, n, q, thr, s, stk + threadIdx.x, err);
         const uint32_t c = (L - done) < kRow ? (L - done) : kRow;
         for (uint32_t j = 0; j < kRow; ++j)
             if (j < c && off + done + j < cap) pairs[off + done + j] = dy_contact_pair{a, s[j]};
@@ -605,13 +658,11 @@ struct dy_broadphase_s {
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
     uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
-    // traverse_count launch shape (emit_pairs uses the same workgroup size):
-    // 64-thread workgroups spread the walks over
-    // all CUs (256-thread ones left half of them idle at 32k bodies) and the
-    // XCD map keeps each XCD on one region of the tree; together 1.9 % off
-    // the 32k-body frame.  DYPHUR_TRAVERSE_THREADS / DYPHUR_TRAVERSE_XCD
-    // override them.
-    uint32_t trav_threads = 64;
+    TPair *tp = nullptr;        // walk records of the internal nodes
+    // traverse_count / emit_pairs run 64-thread workgroups (kWalkThreads):
+    // the walks spread over all CUs (256-thread ones left half of them idle at
+    // 32k bodies); the XCD map keeps each XCD on one region of the tree.
+    // DYPHUR_TRAVERSE_XCD=0 turns the map off.
     uint32_t trav_xcd = 1;
     // Kernel timing (dy_broadphase_set_timing): events between the launches
     // of one build, read back when the next build starts or on request.
@@ -623,7 +674,7 @@ struct dy_broadphase_s {
 namespace {
 void bp_free(dy_broadphase_s *b) {
     void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
-                    b->root, b->range, b->nodebox, b->tn, b->cnt_top, b->entries, b->counters,
+                    b->root, b->range, b->nodebox, b->tn, b->tp, b->cnt_top, b->entries, b->counters,
                     b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs,
                     b->sort_mat, b->sort_tot};
     for (void *p : ptrs)
@@ -671,7 +722,7 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     DY_A(b->code0, nb); DY_A(b->idx0, nb); DY_A(b->code1, nb); DY_A(b->idx1, nb);
     DY_A(b->morton64, nb);
     DY_A(b->left, nb); DY_A(b->right, nb); DY_A(b->parent, nodes); DY_A(b->root, 1);
-    DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes);
+    DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes); DY_A(b->tp, nb);
     DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 8);
     DY_A(b->sort_mat, 256 * ceil_div(static_cast<uint32_t>(nb), kSgTile)); DY_A(b->sort_tot, 256);
     DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->ovf, nb * kOvf); DY_A(b->leaf_of, nb);
@@ -695,10 +746,6 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
             if (const char *env = std::getenv("DYPHUR_SORT_BLOCKS")) want = std::atoi(env);
             b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > per_cu * cus ? per_cu * cus : want));
         }
-    }
-    if (const char *env = std::getenv("DYPHUR_TRAVERSE_THREADS")) {
-        const int t = std::atoi(env);
-        if (t == 64 || t == 128 || t == 256) b->trav_threads = static_cast<uint32_t>(t);
     }
     if (const char *env = std::getenv("DYPHUR_TRAVERSE_XCD")) b->trav_xcd = std::atoi(env) ? 1u : 0u;
     if (e == hipSuccess) e = hipMemsetAsync(b->root, 0, sizeof(int32_t), hs(s));
@@ -757,16 +804,16 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[2], st));
     leaf_refit<<<ceil_div(n, kChunk), kChunk, 0, st>>>(*bodies, *shapes, ni, bp->idx0, bp->parent, bp->left,
-                                                       bp->right, bp->range, bp->nodebox, bp->tn, bp->cnt_top,
+                                                       bp->right, bp->range, bp->nodebox, bp->tn, bp->tp, bp->cnt_top,
                                                        bp->entries, bp->counters);
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[3], st));
-    traverse_count<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(
-        bp->tn, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
+    traverse_count<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(
+        bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[4], st));
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
-    emit_pairs<<<ceil_div(n, bp->trav_threads), bp->trav_threads, 0, st>>>(bp->tn, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
+    emit_pairs<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(bp->tn, bp->tp, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
                                                  bp->pairs, bp->max_pairs, bp->counters + 2);
     DY_LAUNCH_CHECK();
     if (tm) {
