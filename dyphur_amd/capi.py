@@ -363,6 +363,12 @@ class DeviceArray(_Owned):
     def close(self):
         self.free()
 
+    def copy_from_device(self, src_ptr: int, s: Stream, count: int | None = None):
+        """Device-to-device copy of count elements (default all) from src_ptr."""
+        nb = (self.n if count is None else count) * self.dtype.itemsize
+        if nb:
+            check(lib().dy_memcpy_d2d_async(vp(self.ptr), vp(src_ptr), nb, s.h), "dy_memcpy_d2d_async")
+
 
 def download_ptr(ptr: int, n: int, dtype, s: Stream) -> np.ndarray:
     dtype = np.dtype(dtype)
@@ -960,16 +966,21 @@ class Diff(_Owned):
         v.n = self.batch.n_bodies
         return v
 
-    def backward(self, adj_final: dict, download: bool = True) -> dict | None:
+    def backward(self, adj_final: dict, download: bool = True, device_ptrs: bool = False) -> dict | None:
         """adj_final: host arrays (n,) for any of the 13 mutable fields (others
-        zero).  Returns the adjoint of the initial state's 13 fields and of
-        inv_mass (host arrays) unless download is False."""
+        zero), or with device_ptrs device addresses of float32 arrays (n,).
+        Returns the adjoint of the initial state's 13 fields and of inv_mass
+        (host arrays) unless download is False; self.adj holds them on the
+        device either way."""
         b = self.batch
         sc = b.scenes[0]
         g = sc.gravity
         n = b.n_bodies
         for k in MUTABLE_FIELDS:
-            self.adj[k].upload(np.ascontiguousarray(adj_final.get(k, np.zeros(n)), np.float32), b.s)
+            if device_ptrs and k in adj_final:
+                self.adj[k].copy_from_device(adj_final[k], b.s)
+            else:
+                self.adj[k].upload(np.ascontiguousarray(adj_final.get(k, np.zeros(n)), np.float32), b.s)
         v = self._adj_view()
         bv, sv = b.bodies.view(), b.shapes.view()
         check(lib().dy_diff_backward(self.h, b.s.h, C.byref(bv), C.byref(sv), g[0], g[1], g[2], sc.dt,
