@@ -116,7 +116,7 @@ template <class S, int N> DY_HD Dual<S, N> absval(const Dual<S, N> &a) { return 
 // dy_math.hpp's float functions)
 // ---------------------------------------------------------------------------
 template <class T> struct M3T { T m[3][3]; };
-template <class T> struct HitT { T px, py, pz, nx, ny, nz, depth; };
+template <class T> struct HitT { T px, py, pz, nx, ny, nz, depth; uint32_t feat; };
 
 // Element i of a 3-array and row i of a matrix for a run-time i, by selects:
 // no run-time indexing, so everything stays in registers.
@@ -163,6 +163,7 @@ DY_HD bool sphere_sphere_t(const T &ax, const T &ay, const T &az, S ra, const T 
     h.px = S(0.5f) * ((ax - h.nx * ra) + (bx + h.nx * rb));
     h.py = S(0.5f) * ((ay - h.ny * ra) + (by + h.ny * rb));
     h.pz = S(0.5f) * ((az - h.nz * ra) + (bz + h.nz * rb));
+    h.feat = feat_word(kFeatSphSph, 0, 0, 0, 0);
     return true;
 }
 
@@ -188,6 +189,7 @@ DY_HD bool sphere_box_t(const T &sx, const T &sy, const T &sz, S r, const T &cx,
         h.nx = -(sg * pickm(E, f, 0)); h.ny = -(sg * pickm(E, f, 1)); h.nz = -(sg * pickm(E, f, 2));
         h.depth = r + pf;
         h.px = sx - h.nx * r; h.py = sy - h.ny * r; h.pz = sz - h.nz * r;
+        h.feat = feat_word(kFeatSphBoxIn, static_cast<uint32_t>(f), 0, 0, 0);
         return true;
     }
     if (d2 >= r * r) return false;
@@ -201,6 +203,7 @@ DY_HD bool sphere_box_t(const T &sx, const T &sy, const T &sz, S r, const T &cx,
     h.px = cx + cl[0] * E.m[0][0] + cl[1] * E.m[1][0] + cl[2] * E.m[2][0];
     h.py = cy + cl[0] * E.m[0][1] + cl[1] * E.m[1][1] + cl[2] * E.m[2][1];
     h.pz = cz + cl[0] * E.m[0][2] + cl[1] * E.m[1][2] + cl[2] * E.m[2][2];
+    h.feat = feat_word(kFeatSphBoxOut, 0, 0, 0, 0);
     return true;
 }
 
@@ -288,6 +291,8 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
         if (!ok || s.type < 0) return false;
     }
     if (winner) *winner = s.type * 16 + s.i * 4 + s.j;
+    const uint32_t sat = static_cast<uint32_t>(s.type == 0 ? s.i : (s.type == 1 ? 3 + s.i : 6 + 3 * s.i + s.j));
+    uint32_t feat;
     const T nx = s.nx, ny = s.ny, nz = s.nz;
     T sx = T(S(0)), sy = T(S(0)), sz = T(S(0)), sd = T(S(0));
     int m = 0;
@@ -319,6 +324,7 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
         T r0[3], r1[3];
         #pragma unroll
         for (int c = 0; c < 3; ++c) { r0[c] = pickm(R, t0, c); r1[c] = pickm(R, t1, c); }
+        uint32_t mask = 0;
         #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const S s0 = (k & 1) ? S(1) : S(-1), s1 = (k & 2) ? S(1) : S(-1), s2 = (k & 4) ? S(1) : S(-1);
@@ -333,6 +339,7 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
             const T q1 = ux * r1[0] + uy * r1[1] + uz * r1[2];
             if (absval(q0) > rh_t0 + S(1e-3f)) continue;
             if (absval(q1) > rh_t1 + S(1e-3f)) continue;
+            mask |= 1u << k;
             if (m < 4) {
                 sx += vx - sdir * dv * nx;
                 sy += vy - sdir * dv * ny;
@@ -341,6 +348,7 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
                 ++m;
             }
         }
+        feat = feat_word(a_ref ? kFeatFaceA : kFeatFaceB, static_cast<uint32_t>(ref), 0, mask, sat);
     } else {
         const int i = s.i, j = s.j;
         const int ta0 = (i + 1) % 3, ta1 = (i + 2) % 3;
@@ -383,12 +391,14 @@ DY_HD bool box_box_t(const T &acx, const T &acy, const T &acz, const M3T<T> &A, 
         sz += S(0.5f) * (qaz + qbz);
         sd += s.best;
         m = 1;
+        feat = feat_word(kFeatEdge, static_cast<uint32_t>(i), static_cast<uint32_t>(j), 0, sat);
     }
     if (m == 0) return false;
     const S inv = S(1) / static_cast<S>(m);
     h.px = sx * inv; h.py = sy * inv; h.pz = sz * inv;
     h.nx = nx; h.ny = ny; h.nz = nz;
     h.depth = sd * inv;
+    h.feat = feat;
     return true;
 }
 
@@ -467,26 +477,264 @@ DY_HD bool contact_adj(uint32_t ta, uint32_t tb, const S ha[3], const S hb[3], c
     return true;
 }
 
-// Reverse mode for the box-box face contact (the usual case of resting
-// boxes), written out: the manifold's point and depth are means over the kept
-// incident vertices v_k = c_I + sum_m s_km h_m I_m, and the normal is a signed
-// row of one box's axes, so the adjoint is linear in the kept vertices.  The
-// float search and the manifold's decisions (winning axis, reference-face
-// swap, kept vertices) are recomputed with the forward's arithmetic.  Returns
-// false for any other contact kind (the caller falls back to contact_adj).
+// ---------------------------------------------------------------------------
+// Contact geometry in reverse mode, written out per contact kind
+// ---------------------------------------------------------------------------
+// The forward's feature word (dy_math.hpp feat_word: kind, face or edge
+// indices, passing incident vertices, winning separating axis) fixes every
+// branch the forward took, so the adjoints below search nothing: they
+// recompute only the quantities their reverse needs, with the forward's
+// arithmetic, then propagate the adjoints of (p xyz, n xyz, depth) to the two
+// poses.  Sign choices (clamps, |x|, which side of a face) are piecewise
+// constant and take the forward's branch; |x| at x == 0 takes +1, as the
+// dual numbers' absval does.
+
+// Sphere-sphere (dy_math.hpp sphere_sphere): n = d / |d| with d = a - b
+// (fixed (0, 1, 0) when |d| < 1e-8), depth = ra + rb - |d|,
+// p = (a + b) / 2 + (rb - ra) n / 2.
 template <class S>
-DY_HD bool box_face_adj(const S ha[3], const S hb[3], const S pa[3], const S qa[4], const S pb[3], const S qb[4],
-                        const S hbar[7], S ga[7], S gb[7]) {
+DY_HD void sphere_sphere_adj(S ra, S rb, const S pa[3], const S pb[3], const S hbar[7], S ga[7], S gb[7]) {
+    const S d[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
+    const S len = sq_root(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    S db[3] = {S(0), S(0), S(0)};
+    if (len > S(0)) {
+        const S inv = S(1) / len;
+        const S n[3] = {d[0] * inv, d[1] * inv, d[2] * inv};
+        const S half_dr = S(0.5f) * (rb - ra);
+        const S nb[3] = {hbar[3] + half_dr * hbar[0], hbar[4] + half_dr * hbar[1], hbar[5] + half_dr * hbar[2]};
+        const S nn = n[0] * nb[0] + n[1] * nb[1] + n[2] * nb[2];
+        const bool unit = !(len < S(1e-8f));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) db[c] = -hbar[6] * n[c] + (unit ? (nb[c] - n[c] * nn) * inv : S(0));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ga[c] += S(0.5f) * hbar[c] + db[c];
+        gb[c] += S(0.5f) * hbar[c] - db[c];
+    }
+}
+
+// Sphere (centre sc, radius r) against a box (centre bc, rotation qb, half
+// extents he), the normal pointing from the box to the sphere as
+// dy_math.hpp sphere_box returns it.  Outside: q = E (sc - bc), l = q -
+// clamp(q), n = -E^T l / |l|, depth = r - |l|, p = bc + E^T clamp(q).
+// Inside (face f of least penetration): n = -sg E_f, depth = r + he_f - |q_f|,
+// p = sc - r n.  gs[3] receives the sphere centre's adjoint, gb[7] the box's
+// centre and quaternion adjoints.
+template <class S>
+DY_HD void sphere_box_adj(uint32_t feat, S r, const S sc[3], const S bc[3], const S qb[4], const S he[3],
+                          const S hbar[7], S gs[3], S gb[7]) {
+    const M3T<S> E = axes_t(qb[0], qb[1], qb[2], qb[3], S(1));
+    const S d[3] = {sc[0] - bc[0], sc[1] - bc[1], sc[2] - bc[2]};
+    S q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q[a] = d[0] * E.m[a][0] + d[1] * E.m[a][1] + d[2] * E.m[a][2];
+    M3T<S> Eb{};
+    S qbar[3] = {S(0), S(0), S(0)};
+    S cb[3] = {hbar[0], hbar[1], hbar[2]};  // outside: p = bc + E^T clamp(q)
+    if ((feat & 7u) == kFeatSphBoxIn) {
+        const int f = static_cast<int>((feat >> 3) & 3u);
+        const S qf = pick3(q, f);
+        const S sg = qf >= S(0) ? S(1) : S(-1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gs[c] += hbar[c];
+            cb[c] = S(0);
+            const S nb = hbar[3 + c] - r * hbar[c];  // p = sc - r n
+#pragma unroll
+            for (int a = 0; a < 3; ++a) Eb.m[a][c] += a == f ? -sg * nb : S(0);
+        }
+        const S qfb = -hbar[6] * sg;
+        qbar[0] = f == 0 ? qfb : S(0);
+        qbar[1] = f == 1 ? qfb : S(0);
+        qbar[2] = f == 2 ? qfb : S(0);
+    } else {
+        S cl[3], l[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            cl[a] = q[a] < -he[a] ? -he[a] : (q[a] > he[a] ? he[a] : q[a]);
+            l[a] = q[a] - cl[a];
+        }
+        const S len = sq_root(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+        const S inv = S(1) / len;
+        S w[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) w[c] = l[0] * E.m[0][c] + l[1] * E.m[1][c] + l[2] * E.m[2][c];
+        // n = -w / |l|, depth = r - |l|
+        const S nw = hbar[3] * w[0] + hbar[4] * w[1] + hbar[5] * w[2];
+        const S lenb = nw * inv * inv - hbar[6];
+        S wb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wb[c] = -hbar[3 + c] * inv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const S clb = hbar[0] * E.m[a][0] + hbar[1] * E.m[a][1] + hbar[2] * E.m[a][2];
+            const S lb = l[a] * lenb * inv + wb[0] * E.m[a][0] + wb[1] * E.m[a][1] + wb[2] * E.m[a][2];
+            const bool free_axis = !(q[a] < -he[a]) && !(q[a] > he[a]);
+            qbar[a] = lb + (free_axis ? clb - lb : S(0));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Eb.m[a][c] += cl[a] * hbar[c] + l[a] * wb[c];
+        }
+    }
+    // q_a = (sc - bc) . E_a
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gs[c] += qbar[a] * E.m[a][c];
+            cb[c] -= qbar[a] * E.m[a][c];
+            Eb.m[a][c] += qbar[a] * d[c];
+        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gb[c] += cb[c];
+    S qbb[4] = {S(0), S(0), S(0), S(0)};
+    axes_adj(qb, Eb, qbb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gb[3 + k] += qbb[k];
+}
+
+// Box-box edge contact on axis l = A_i x B_j / |A_i x B_j| (dy_math.hpp
+// box_box, edge branch): n = sg l, depth = sum_m ah_m |A_m.l| + sum_m bh_m
+// |B_m.l| - |(cb - ca).l| (the m = i and m = j terms are identically zero
+// and have no derivative), p = the midpoint of the closest points of the two
+// support edges.
+template <class S>
+DY_HD void box_edge_adj(uint32_t feat, const S ha[3], const S hb[3], const S pa[3], const S qa[4], const S pb[3],
+                        const S qb[4], const S hbar[7], S ga[7], S gb[7]) {
+    const int i = static_cast<int>((feat >> 3) & 3u), j = static_cast<int>((feat >> 5) & 3u);
+    const M3T<S> A = axes_t(qa[0], qa[1], qa[2], qa[3], S(1));
+    const M3T<S> B = axes_t(qb[0], qb[1], qb[2], qb[3], S(1));
+    const int ta0 = (i + 1) % 3, ta1 = (i + 2) % 3, tb0 = (j + 1) % 3, tb1 = (j + 2) % 3;
+    S Ai[3], Bj[3], A0[3], A1[3], B0[3], B1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Ai[c] = pickm(A, i, c); A0[c] = pickm(A, ta0, c); A1[c] = pickm(A, ta1, c);
+        Bj[c] = pickm(B, j, c); B0[c] = pickm(B, tb0, c); B1[c] = pickm(B, tb1, c);
+    }
+    // forward: the axis, its sign and the support edges
+    const S cr[3] = {Ai[1] * Bj[2] - Ai[2] * Bj[1], Ai[2] * Bj[0] - Ai[0] * Bj[2], Ai[0] * Bj[1] - Ai[1] * Bj[0]};
+    const S il = S(1) / sq_root(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    const S l[3] = {cr[0] * il, cr[1] * il, cr[2] * il};
+    const S dv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+    const S e = dv[0] * l[0] + dv[1] * l[1] + dv[2] * l[2];
+    const S sg = e >= S(0) ? S(-1) : S(1);
+    const S n[3] = {sg * l[0], sg * l[1], sg * l[2]};
+    const S sa0 = (-n[0] * A0[0] - n[1] * A0[1] - n[2] * A0[2]) >= S(0) ? S(1) : S(-1);
+    const S sa1 = (-n[0] * A1[0] - n[1] * A1[1] - n[2] * A1[2]) >= S(0) ? S(1) : S(-1);
+    const S sb0 = (n[0] * B0[0] + n[1] * B0[1] + n[2] * B0[2]) >= S(0) ? S(1) : S(-1);
+    const S sb1 = (n[0] * B1[0] + n[1] * B1[1] + n[2] * B1[2]) >= S(0) ? S(1) : S(-1);
+    const S ka0 = sa0 * pick3(ha, ta0), ka1 = sa1 * pick3(ha, ta1);
+    const S kb0 = sb0 * pick3(hb, tb0), kb1 = sb1 * pick3(hb, tb1);
+    S ea[3], eb[3], w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ea[c] = pa[c] + ka0 * A0[c] + ka1 * A1[c];
+        eb[c] = pb[c] + kb0 * B0[c] + kb1 * B1[c];
+        w[c] = ea[c] - eb[c];
+    }
+    const S cc = Ai[0] * Bj[0] + Ai[1] * Bj[1] + Ai[2] * Bj[2];
+    const S den = S(1) - cc * cc;
+    const S d1 = w[0] * Ai[0] + w[1] * Ai[1] + w[2] * Ai[2];
+    const S d2 = w[0] * Bj[0] + w[1] * Bj[1] + w[2] * Bj[2];
+    const bool solve = absval(den) > S(1e-6f);
+    const S u = solve ? (cc * d2 - d1) / den : S(0);
+    const S v = solve ? (d2 - cc * d1) / den : S(0);
+    // reverse: p = (ea + u Ai + eb + v Bj) / 2
+    S Aib[3], Bjb[3], eab[3], ebb[3];
+    const S pbh[3] = {S(0.5f) * hbar[0], S(0.5f) * hbar[1], S(0.5f) * hbar[2]};
+    const S ub = pbh[0] * Ai[0] + pbh[1] * Ai[1] + pbh[2] * Ai[2];
+    const S vb = pbh[0] * Bj[0] + pbh[1] * Bj[1] + pbh[2] * Bj[2];
+    S ccb = S(0), d1b = S(0), d2b = S(0);
+    if (solve) {
+        const S iden = S(1) / den;
+        ccb = (ub * (d2 + S(2) * cc * u) + vb * (S(2) * cc * v - d1)) * iden;
+        d1b = (-ub - cc * vb) * iden;
+        d2b = (cc * ub + vb) * iden;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const S wb = d1b * Ai[c] + d2b * Bj[c];
+        Aib[c] = u * pbh[c] + d1b * w[c] + ccb * Bj[c];
+        Bjb[c] = v * pbh[c] + d2b * w[c] + ccb * Ai[c];
+        eab[c] = pbh[c] + wb;
+        ebb[c] = pbh[c] - wb;
+    }
+    M3T<S> Ab{}, Bb{};
+    // ea = ca + ka0 A_ta0 + ka1 A_ta1, eb likewise
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Ab.m[r][c] += (r == ta0 ? ka0 : (r == ta1 ? ka1 : S(0))) * eab[c];
+            Bb.m[r][c] += (r == tb0 ? kb0 : (r == tb1 ? kb1 : S(0))) * ebb[c];
+        }
+    // n = sg l; depth = rA + rB - |e|
+    const S deb = hbar[6];
+    const S se = e >= S(0) ? S(1) : S(-1);
+    S lb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lb[c] = sg * hbar[3 + c] - deb * se * dv[c];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const S tA = A.m[m][0] * l[0] + A.m[m][1] * l[1] + A.m[m][2] * l[2];
+        const S tB = B.m[m][0] * l[0] + B.m[m][1] * l[1] + B.m[m][2] * l[2];
+        const S kA = m == i ? S(0) : deb * ha[m] * (tA >= S(0) ? S(1) : S(-1));
+        const S kB = m == j ? S(0) : deb * hb[m] * (tB >= S(0) ? S(1) : S(-1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Ab.m[m][c] += kA * l[c];
+            Bb.m[m][c] += kB * l[c];
+            lb[c] += kA * A.m[m][c] + kB * B.m[m][c];
+        }
+    }
+    // l = cr / |cr|, cr = Ai x Bj
+    const S ll = l[0] * lb[0] + l[1] * lb[1] + l[2] * lb[2];
+    S crb[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) crb[c] = (lb[c] - l[c] * ll) * il;
+    Aib[0] += Bj[1] * crb[2] - Bj[2] * crb[1];
+    Aib[1] += Bj[2] * crb[0] - Bj[0] * crb[2];
+    Aib[2] += Bj[0] * crb[1] - Bj[1] * crb[0];
+    Bjb[0] += crb[1] * Ai[2] - crb[2] * Ai[1];
+    Bjb[1] += crb[2] * Ai[0] - crb[0] * Ai[2];
+    Bjb[2] += crb[0] * Ai[1] - crb[1] * Ai[0];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Ab.m[r][c] += r == i ? Aib[c] : S(0);
+            Bb.m[r][c] += r == j ? Bjb[c] : S(0);
+        }
+    // the depth's -|e| with e = (cb - ca).l
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const S dvb = -deb * se * l[c];
+        ga[c] += eab[c] - dvb;
+        gb[c] += ebb[c] + dvb;
+    }
+    S qab[4] = {S(0), S(0), S(0), S(0)}, qbb[4] = {S(0), S(0), S(0), S(0)};
+    axes_adj(qa, Ab, qab);
+    axes_adj(qb, Bb, qbb);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { ga[3 + k] += qab[k]; gb[3 + k] += qbb[k]; }
+}
+
+// Box-box face contact (the usual case of resting boxes): the manifold's
+// point and depth are means over the kept incident vertices v_k = c_I +
+// sum_m s_km h_m I_m, and the normal is a signed row of one box's axes, so
+// the adjoint is linear in the kept vertices.  The winning axis, the
+// reference face and the passing vertices come from the feature word; the
+// first four passing vertices in vertex order are the kept ones.
+template <class S>
+DY_HD void box_face_adj(uint32_t feat, const S ha[3], const S hb[3], const S pa[3], const S qa[4], const S pb[3],
+                        const S qb[4], const S hbar[7], S ga[7], S gb[7]) {
     const S one = S(1);
     const M3T<S> A = axes_t(qa[0], qa[1], qa[2], qa[3], one);
     const M3T<S> B = axes_t(qb[0], qb[1], qb[2], qb[3], one);
-    int axis = -1;
-    {
-        HitT<S> h;
-        if (!box_box_t(pa[0], pa[1], pa[2], A, ha, pb[0], pb[1], pb[2], B, hb, h, -1, &axis)) return false;
-    }
-    const int ty = axis >> 4, ai = (axis >> 2) & 3;
-    if (ty == 2) return false;
+    const uint32_t sat = (feat >> 16) & 0xffu;
+    const int ty = sat < 3u ? 0 : 1, ai = static_cast<int>(sat < 3u ? sat : sat - 3u);
+    const bool a_ref = (feat & 7u) == kFeatFaceA;
+    const int ref = static_cast<int>((feat >> 3) & 3u);
     // normal: sg * row ai of A (ty 0) or of B (ty 1), sign from the centre offset
     const S dx = pb[0] - pa[0], dy = pb[1] - pa[1], dz = pb[2] - pa[2];
     S l[3];
@@ -494,64 +742,35 @@ DY_HD bool box_face_adj(const S ha[3], const S hb[3], const S pa[3], const S qa[
     for (int c = 0; c < 3; ++c) l[c] = ty == 0 ? pickm(A, ai, c) : pickm(B, ai, c);
     const S sg = (dx * l[0] + dy * l[1] + dz * l[2]) >= S(0) ? S(-1) : S(1);
     const S n[3] = {sg * l[0], sg * l[1], sg * l[2]};
-    // reference-face swap (box_box_t)
-    int type = ty, ref = ai;
-    if (type == 0) {
-        const int ta0 = (ref + 1) % 3, ta1 = (ref + 2) % 3;
-        S cr[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            cr[j] = pickm(A, ref, 0) * B.m[j][0] + pickm(A, ref, 1) * B.m[j][1] + pickm(A, ref, 2) * B.m[j][2];
-        int jb = 0;
-        if (absval(cr[1]) > absval(cr[0])) jb = 1;
-        if (absval(cr[2]) > absval(pick3(cr, jb))) jb = 2;
-        const int tb0 = (jb + 1) % 3, tb1 = (jb + 2) % 3;
-        if (pick3(hb, tb0) * pick3(hb, tb1) > pick3(ha, ta0) * pick3(ha, ta1)) { type = 1; ref = jb; }
-    }
-    const bool a_ref = type == 0;
-    const M3T<S> R = pick_mat(a_ref, A, B);
     const M3T<S> I = pick_mat(a_ref, B, A);
-    S rh[3], ih[3], rc[3], ic[3];
+    S ih[3], rc[3], ic[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        rh[c] = a_ref ? ha[c] : hb[c];
         ih[c] = a_ref ? hb[c] : ha[c];
         rc[c] = a_ref ? pa[c] : pb[c];
         ic[c] = a_ref ? pb[c] : pa[c];
     }
-    const int t0 = (ref + 1) % 3, t1 = (ref + 2) % 3;
     const S ref_n = rc[0] * n[0] + rc[1] * n[1] + rc[2] * n[2];
     const S sdir = a_ref ? S(1) : S(-1);
-    const S rh_ref = pick3(rh, ref), rh_t0 = pick3(rh, t0), rh_t1 = pick3(rh, t1);
-    S r0[3], r1[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { r0[c] = pickm(R, t0, c); r1[c] = pickm(R, t1, c); }
-    // forward decisions: kept vertices (first four in k order) and their dv
+    const S rh_ref = a_ref ? pick3(ha, ref) : pick3(hb, ref);
+    // kept vertices: the first four passing ones, and their dv
     uint32_t kept = 0;
     S dvk[8];
     int m = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         dvk[k] = S(0);
+        if (!((feat >> (8 + k)) & 1u) || m >= 4) continue;
         const S s0 = (k & 1) ? S(1) : S(-1), s1 = (k & 2) ? S(1) : S(-1), s2 = (k & 4) ? S(1) : S(-1);
         const S vx = ic[0] + s0 * ih[0] * I.m[0][0] + s1 * ih[1] * I.m[1][0] + s2 * ih[2] * I.m[2][0];
         const S vy = ic[1] + s0 * ih[0] * I.m[0][1] + s1 * ih[1] * I.m[1][1] + s2 * ih[2] * I.m[2][1];
         const S vz = ic[2] + s0 * ih[0] * I.m[0][2] + s1 * ih[1] * I.m[1][2] + s2 * ih[2] * I.m[2][2];
         const S vn = vx * n[0] + vy * n[1] + vz * n[2];
-        const S dv = rh_ref + sdir * (vn - ref_n);
-        if (dv <= S(0)) continue;
-        const S ux = vx - rc[0], uy = vy - rc[1], uz = vz - rc[2];
-        const S q0 = ux * r0[0] + uy * r0[1] + uz * r0[2];
-        const S q1 = ux * r1[0] + uy * r1[1] + uz * r1[2];
-        if (absval(q0) > rh_t0 + S(1e-3f)) continue;
-        if (absval(q1) > rh_t1 + S(1e-3f)) continue;
-        if (m < 4) {
-            kept |= 1u << k;
-            dvk[k] = dv;
-            ++m;
-        }
+        kept |= 1u << k;
+        dvk[k] = rh_ref + sdir * (vn - ref_n);
+        ++m;
     }
-    if (m == 0) return false;
+    if (m == 0) return;
     // reverse: p = mean(v - sdir dv n), depth = mean(dv)
     const S inv = S(1) / static_cast<S>(m);
     const S sb[3] = {hbar[0] * inv, hbar[1] * inv, hbar[2] * inv};
@@ -604,7 +823,28 @@ DY_HD bool box_face_adj(const S ha[3], const S hb[3], const S pa[3], const S qa[
     axes_adj(qb, Bb, qbb);
 #pragma unroll
     for (int k = 0; k < 4; ++k) { ga[3 + k] += qab[k]; gb[3 + k] += qbb[k]; }
-    return true;
+}
+
+// Reverse mode of one contact's geometry (pair_contact_t's dispatch, the
+// sphere-box normal negated when the sphere is body A): accumulates the
+// adjoints of (pa xyz, qa wxyz) into ga[7] and of (pb, qb) into gb[7], given
+// the adjoints hbar[7] of (p xyz, n xyz, depth) and the forward's feature
+// word.
+template <class S>
+DY_HD void geom_adj(uint32_t ta, uint32_t tb, uint32_t feat, const S ha[3], const S hb[3], const S pa[3],
+                    const S qa[4], const S pb[3], const S qb[4], const S hbar[7], S ga[7], S gb[7]) {
+    if (ta == kSphere && tb == kSphere) {
+        sphere_sphere_adj(ha[0], hb[0], pa, pb, hbar, ga, gb);
+    } else if (ta == kSphere) {
+        const S hneg[7] = {hbar[0], hbar[1], hbar[2], -hbar[3], -hbar[4], -hbar[5], hbar[6]};
+        sphere_box_adj(feat, ha[0], pa, pb, qb, hb, hneg, ga, gb);
+    } else if (tb == kSphere) {
+        sphere_box_adj(feat, hb[0], pb, pa, qa, ha, hbar, gb, ga);
+    } else if ((feat & 7u) == kFeatEdge) {
+        box_edge_adj(feat, ha, hb, pa, qa, pb, qb, hbar, ga, gb);
+    } else {
+        box_face_adj(feat, ha, hb, pa, qa, pb, qb, hbar, ga, gb);
+    }
 }
 
 // ---------------------------------------------------------------------------

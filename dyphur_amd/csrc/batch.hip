@@ -104,19 +104,22 @@ struct SceneScratch {
     uint32_t *rnd, *cls, *order;        // [C]
     uint32_t *list;                     // [2C]
     uint32_t *ccnt, *coff;              // [C + 1]
+    uint32_t *feat;                     // [C] contact feature words (taped runs)
     uint32_t *bits;                     // [n * ceil(n / 32)] overlap rows
 };
 
 __host__ __device__ inline uint64_t scene_scratch_words(uint32_t n, uint32_t c) {
-    const uint64_t w = 6ull * n + 10ull * c + 2ull + static_cast<uint64_t>(n) * ((n + 31u) / 32u);
+    const uint64_t w = 6ull * n + 11ull * c + 2ull + static_cast<uint64_t>(n) * ((n + 31u) / 32u);
     return (w + 1ull) & ~1ull;
 }
 
 // Per-frame record of one scene for the reverse sweep (floats, each part
 // 16-byte aligned): frame-start state (13 fields x n, SoA), post-integrate
 // pose (7 x n), quaternions before each iteration's renormalisation
-// (iters x 4 x n), a header (contacts solved, classes), class offsets
-// (C + 1), contacts in class order (8 floats: p, n, depth; and the body pair),
+// (iters x 4 x n), a header (contacts solved, classes, error bits, the
+// iterations that projected a live contact: bit min(it, 31)), class offsets
+// (C + 1), contacts in class order (8 words: p, n, depth, the feature word;
+// and the body pair),
 // and each contact's inputs at each iteration (28 floats: both bodies' 13
 // mutable fields, lambda; written only for a live projection, see below),
 // and each contact's lambda at each iteration.
@@ -156,6 +159,7 @@ __device__ __forceinline__ SceneScratch carve(uint32_t *p, uint32_t n, uint32_t 
     s.list = p; p += 2 * c;
     s.ccnt = p; p += c + 1;
     s.coff = p; p += c + 1;
+    s.feat = p; p += c;
     s.bits = p;
     return s;
 }
@@ -909,6 +913,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                 cv.pos_x[slot] = hc.px; cv.pos_y[slot] = hc.py; cv.pos_z[slot] = hc.pz;
                 cv.norm_x[slot] = hc.nx; cv.norm_y[slot] = hc.ny; cv.norm_z[slot] = hc.nz;
                 cv.depth[slot] = hc.depth;
+                if constexpr (TAPE) S.feat[slot] = hc.feat;
             }
             run += tot;
         }
@@ -919,6 +924,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
 
         // 5. solve (xpbd_solver.cpp:409-733), contacts only.
         uint32_t serial = 1, rounds = 0, n_cls = 0, overflow = 0, err = 0, n_dyn_end = 0;
+        uint32_t live_its = 0;  // bit min(it, 31): this thread projected a live contact in iteration it
         if (nc > 0) {
             for (uint32_t x = r; x < n; x += NT) { S.deg[x] = 0; S.fill[x] = 0; S.used[x] = 0; }
             for (uint32_t k = r; k < nc; k += NT) {
@@ -1001,7 +1007,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                         reinterpret_cast<float4 *>(rec)[2 * j] =
                             make_float4(cv.pos_x[k], cv.pos_y[k], cv.pos_z[k], cv.norm_x[k]);
                         reinterpret_cast<float4 *>(rec)[2 * j + 1] =
-                            make_float4(cv.norm_y[k], cv.norm_z[k], cv.depth[k], 0.f);
+                            make_float4(cv.norm_y[k], cv.norm_z[k], cv.depth[k], __uint_as_float(S.feat[k]));
                         ab[2 * j] = cv.body_a[k];
                         ab[2 * j + 1] = cv.body_b[k];
                     }
@@ -1024,6 +1030,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                             // contact is live (its depth not yet covered by lambda: after its
                             // first projection lambda == depth for the rest of the substep)
                             if (!contact_live(ci.depth, l)) continue;
+                            live_its |= 1u << (it < 31 ? it : 31);
                             BodyState Ba = load_body(bv, a), Bb = load_body(bv, b);
                             if (taped) {
                                 float4 *pr = reinterpret_cast<float4 *>(
@@ -1058,6 +1065,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
             }
         }
         if (err) atomicOr(&tf[20], err);
+        if (live_its) atomicOr(&tf[22], live_its);
         tm.sync();
         mark(5);
         if constexpr (TAPE) {
@@ -1067,7 +1075,7 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                 hdr[0] = terr ? 0u : nc;
                 hdr[1] = terr ? 0u : n_cls;
                 hdr[2] = terr;
-                hdr[3] = 0u;
+                hdr[3] = tf[22];
             }
         }
         if (r == 0) {
@@ -1321,45 +1329,59 @@ __device__ void reverse_scene(const Team &tm, const BatchArgs &A, const DiffArgs
             const uint32_t *ab = reinterpret_cast<const uint32_t *>(fb + TL.ab);
             for (uint32_t j = r; j < 8u * nc; j += NT) acc_of(j >> 3)[j & 7u] = 0.f;
             tm.sync();
-            // phases (it, c) newest first; each lane's first contact of the
-            // next phase is loaded while this one computes
-            PreRec cur, nxt;
-            cur.j = ~0u;
-            if (iters > 0) pre_fetch(cur, fb, TL, co, C, iters - 1u, ncls - 1u, r);
-            for (int it = static_cast<int>(iters) - 1; it >= 0; --it) {
-                const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(it);
+            // Iterations newest first.  A projection changes nothing once its
+            // contact's lambda covers the depth (after the contact's first
+            // projection in a substep), so most iterations project nothing:
+            // hdr[3] bit min(it, 31) says iteration it projected a live
+            // contact.  The iterations down to the next live one need only
+            // their renormalisation adjoints, applied per body in one pass;
+            // then the live iteration's classes last to first, each lane's
+            // first contact of the next class loaded while this one computes.
+            const uint32_t live_its = hdr[3];
+            auto is_live = [&](int it) { return ((live_its >> (it < 31 ? it : 31)) & 1u) != 0u; };
+            for (int it = static_cast<int>(iters) - 1; it >= 0;) {
+                int lo = it;
+                while (lo > 0 && !is_live(lo)) --lo;
+                const bool live = is_live(lo);
+                PreRec cur, nxt;
+                cur.j = ~0u;
+                if (live) pre_fetch(cur, fb, TL, co, C, static_cast<uint32_t>(lo), ncls - 1u, r);
                 for (uint32_t i = r; i < n; i += NT) {
                     if (gv.inv_mass[i] == 0.f) continue;
-                    const float q[4] = {qr[i], qr[n + i], qr[2 * n + i], qr[3 * n + i]};
                     float qb[4] = {av.rot_w[i], av.rot_x[i], av.rot_y[i], av.rot_z[i]};
-                    renorm_adj(q, qb);
+                    for (int k = it; k >= lo; --k) {
+                        const float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(k);
+                        const float q[4] = {qr[i], qr[n + i], qr[2 * n + i], qr[3 * n + i]};
+                        renorm_adj(q, qb);
+                    }
                     av.rot_w[i] = qb[0]; av.rot_x[i] = qb[1]; av.rot_y[i] = qb[2]; av.rot_z[i] = qb[3];
                 }
                 tm.sync();
-                for (int c = static_cast<int>(ncls) - 1; c >= 0; --c) {
-                    const bool more = c > 0 || it > 0;
-                    if (more)
-                        pre_fetch(nxt, fb, TL, co, C, c > 0 ? static_cast<uint32_t>(it) : static_cast<uint32_t>(it - 1),
-                                  c > 0 ? static_cast<uint32_t>(c - 1) : ncls - 1u, r);
-                    const uint32_t end = co[c + 1];
-                    if (cur.j < end && cur.live)
-                        contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
-                                          reinterpret_cast<const float *>(cur.m), acc_of(cur.j));
-                    for (uint32_t j = cur.j + NT; j < end; j += NT) {  // classes larger than the team
-                        PreRec x;
-                        pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(it), static_cast<uint32_t>(c), r + (j - cur.j));
-                        if (x.live)
-                            contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1,
-                                              reinterpret_cast<const float *>(x.m), acc_of(j));
+                if (live) {
+                    for (int c = static_cast<int>(ncls) - 1; c >= 0; --c) {
+                        if (c > 0) pre_fetch(nxt, fb, TL, co, C, static_cast<uint32_t>(lo), static_cast<uint32_t>(c - 1), r);
+                        const uint32_t end = co[c + 1];
+                        if (cur.j < end && cur.live)
+                            contact_phase_adj(A, h, gv, av, cur.a, cur.b, cur.r0, cur.r1,
+                                              reinterpret_cast<const float *>(cur.m), acc_of(cur.j));
+                        for (uint32_t j = cur.j + NT; j < end; j += NT) {  // classes larger than the team
+                            PreRec x;
+                            pre_fetch(x, fb, TL, co, C, static_cast<uint32_t>(lo), static_cast<uint32_t>(c),
+                                      r + (j - cur.j));
+                            if (x.live)
+                                contact_phase_adj(A, h, gv, av, x.a, x.b, x.r0, x.r1,
+                                                  reinterpret_cast<const float *>(x.m), acc_of(j));
+                        }
+                        tm.sync();
+                        if (c > 0) cur = nxt;
                     }
-                    tm.sync();
-                    if (more) cur = nxt;
                 }
+                it = lo - 1;
             }
             // contact geometry: (p, n, depth) adjoints to the post-integrate
-            // poses, every contact in parallel (written-out face adjoint,
-            // dual-number replay for the other kinds), then added to the
-            // bodies class by class
+            // poses, every contact in parallel (written out per contact kind,
+            // the branches fixed by the forward's feature word), then added
+            // to the bodies class by class
             const float *po = fb + TL.po;
             for (uint32_t j = r; j < nc; j += NT) {
                 const uint32_t a = ab[2 * j], b = ab[2 * j + 1];
@@ -1374,9 +1396,9 @@ __device__ void reverse_scene(const Team &tm, const BatchArgs &A, const DiffArgs
                 float hbar[7];
                 const float *aj = acc_of(j);
                 for (int k = 0; k < 7; ++k) hbar[k] = aj[1 + k];
+                const uint32_t feat = __float_as_uint(reinterpret_cast<const float4 *>(fb + TL.rec)[2 * j + 1].w);
                 float ga[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gb2[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                const bool done = ta == kBox && tb == kBox && box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
-                if (!done) (void)contact_adj<float, 2>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
+                geom_adj(ta, tb, feat, ha, hb, pa, qa, pb, qb, hbar, ga, gb2);
                 for (int k = 0; k < 7; ++k) {
                     pg[14u * j + k] = ga[k];
                     pg[14u * j + 7 + k] = gb2[k];

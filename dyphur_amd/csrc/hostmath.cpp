@@ -190,9 +190,32 @@ int ha_contact_adj_d(uint32_t ta, uint32_t tb, const double *ha, const double *h
         default: return contact_adj<double, 14>(ta, tb, ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
     }
 }
-int ha_box_face_adj_d(const double *ha, const double *hb, const double *pa, const double *pb, const double *qa,
-                      const double *qb, const double *hbar, double *ga, double *gb) {
-    return box_face_adj(ha, hb, pa, qa, pb, qb, hbar, ga, gb) ? 1 : 0;
+// Feature word of one pair's contact through the templated replay (float or
+// double); returns 1 on hit.
+int ha_pair_feat_f(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
+                   const float *qa, const float *qb, uint32_t *feat) {
+    HitT<float> h{};
+    if (!pair_contact_t(ta, tb, ha, hb, pa, qa, pb, qb, h, -1, nullptr)) return 0;
+    *feat = h.feat;
+    return 1;
+}
+int ha_pair_feat_d(uint32_t ta, uint32_t tb, const double *ha, const double *hb, const double *pa,
+                   const double *pb, const double *qa, const double *qb, uint32_t *feat) {
+    HitT<double> h{};
+    if (!pair_contact_t(ta, tb, ha, hb, pa, qa, pb, qb, h, -1, nullptr)) return 0;
+    *feat = h.feat;
+    return 1;
+}
+// Written-out geometry adjoint for the contact kind the feature word names:
+// ga[7], gb[7] (+=) = J^T hbar.
+void ha_geom_adj_d(uint32_t ta, uint32_t tb, uint32_t feat, const double *ha, const double *hb, const double *pa,
+                   const double *pb, const double *qa, const double *qb, const double *hbar, double *ga,
+                   double *gb) {
+    geom_adj(ta, tb, feat, ha, hb, pa, qa, pb, qb, hbar, ga, gb);
+}
+void ha_geom_adj_f(uint32_t ta, uint32_t tb, uint32_t feat, const float *ha, const float *hb, const float *pa,
+                   const float *pb, const float *qa, const float *qb, const float *hbar, float *ga, float *gb) {
+    geom_adj(ta, tb, feat, ha, hb, pa, qa, pb, qb, hbar, ga, gb);
 }
 int ha_contact_adj_f(uint32_t ta, uint32_t tb, const float *ha, const float *hb, const float *pa, const float *pb,
                      const float *qa, const float *qb, const float *hbar, float *ga, float *gb) {

@@ -29,8 +29,16 @@ def hm():
     L.ha_pair_contact_d.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 8
     L.ha_contact_adj_d.restype = i
     L.ha_contact_adj_d.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 9 + [i]
-    L.ha_box_face_adj_d.restype = i
-    L.ha_box_face_adj_d.argtypes = [vp] * 9
+    L.ha_pair_feat_f.restype = i
+    L.ha_pair_feat_f.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
+    L.ha_pair_feat_d.restype = i
+    L.ha_pair_feat_d.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
+    L.ha_geom_adj_d.restype = None
+    L.ha_geom_adj_d.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 9
+    L.ha_geom_adj_f.restype = None
+    L.ha_geom_adj_f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 9
+    L.hm_pair_contact_feat.restype = i
+    L.hm_pair_contact_feat.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 8
     L.ha_contact_adj_f.restype = i
     L.ha_contact_adj_f.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 9
     L.ha_project_f.restype = i
@@ -269,6 +277,11 @@ def test_contact_geometry_adjoint_matches_central_differences(hm):
         assert np.allclose(got[1], got[14], rtol=1e-12, atol=1e-12) and np.allclose(got[7], got[14], rtol=1e-12,
                                                                                     atol=1e-12)
         _close(got[14], fd, 2e-6)
+        feat = C.c_uint32(0)
+        assert hm.ha_pair_feat_d(ta, tb, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), C.byref(feat))
+        ga, gb = np.zeros(7), np.zeros(7)
+        hm.ha_geom_adj_d(ta, tb, feat.value, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(ga), P(gb))
+        _close(np.concatenate([ga, gb]), fd, 2e-6)
         seen[kind] = seen.get(kind, 0) + 1
     # every dispatch branch and both box-box kinds (face of A / of B, edge) exercised
     assert {(0, 0, 0), (0, 0, 1), (1, 0, -1), (0, 1, -1), (1, 1, -1)} <= set(seen), seen
@@ -337,28 +350,99 @@ def test_host_forward_mode_matches_central_differences(hm):
     assert np.abs(sd - fd).max() <= 1e-4 * max(1.0, np.abs(fd).max())
 
 
-def test_box_face_adjoint_equals_dual_jacobian(hm):
-    """The written-out face-contact adjoint equals the dual-number Jacobian
-    on every box-box face contact (and declines edge contacts)."""
+FEAT_KINDS = {0: "face of A", 1: "face of B", 2: "edge", 3: "sphere-sphere", 4: "sphere-box outside",
+              5: "sphere-box inside"}
+
+
+def _pair_any(rng, it):
+    """Random touching candidates of every dispatch branch; every fifth
+    sphere-box pair puts the sphere's centre inside the box."""
+    ta, tb = TYPES[it % 4]
+    ha, hb, pa, pb, qa, qb = _pair(rng, ta, tb)
+    if (ta, tb) != (BOX, BOX) and ta != tb and (it // 4) % 5 == 0:
+        box_h, box_c, box_q = (hb, pb, qb) if ta == SPHERE else (ha, pa, qa)
+        w, x, y, z = box_q
+        E = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w)],
+                      [2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w)],
+                      [2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)]])
+        centre = box_c + E.T @ (rng.uniform(-0.9, 0.9, 3) * box_h)
+        if ta == SPHERE:
+            pa = centre
+        else:
+            pb = centre
+    return ta, tb, ha, hb, pa, pb, qa, qb
+
+
+def test_replay_feature_words_equal_the_forward(hm):
+    """The templated replay at float reports the forward's feature word (kind,
+    face or edge, passing vertices, winning axis) on every contact."""
+    rng = np.random.default_rng(14)
+    seen = set()
+    for it in range(6000):
+        ta, tb, *xs = _pair_any(rng, it)
+        ha, hb, pa, pb, qa, qb = (np.asarray(x, F32) for x in xs)
+        out = np.zeros(7, F32)
+        f1, f2 = C.c_uint32(0), C.c_uint32(0)
+        hit = hm.hm_pair_contact_feat(ta, tb, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(out), C.byref(f1))
+        assert hm.ha_pair_feat_f(ta, tb, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), C.byref(f2)) == hit
+        if hit:
+            assert f1.value == f2.value, (ta, tb, hex(f1.value), hex(f2.value))
+            seen.add(f1.value & 7)
+    assert seen == set(FEAT_KINDS), seen
+
+
+def test_geometry_adjoints_equal_dual_jacobian(hm):
+    """The written-out adjoint of every contact kind, its branches taken from
+    the feature word, equals the transposed dual-number Jacobian (float64)."""
     rng = np.random.default_rng(12)
-    faces = edges = 0
-    for _ in range(3000):
-        ha, hb, pa, pb, qa, qb = _pair(rng, BOX, BOX)
-        out = np.zeros(7)
-        w = C.c_int(-1)
-        if not hm.ha_pair_contact_d(BOX, BOX, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(out), C.byref(w)):
+    count = dict.fromkeys(FEAT_KINDS, 0)
+    for it in range(8000):
+        ta, tb, ha, hb, pa, pb, qa, qb = _pair_any(rng, it)
+        feat = C.c_uint32(0)
+        if not hm.ha_pair_feat_d(ta, tb, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), C.byref(feat)):
             continue
         hbar = rng.normal(size=7)
         g1a, g1b, g2a, g2b = np.zeros(7), np.zeros(7), np.zeros(7), np.zeros(7)
-        hm.ha_contact_adj_d(BOX, BOX, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g1a), P(g1b), 14)
-        ok = hm.ha_box_face_adj_d(P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g2a), P(g2b))
-        if (w.value >> 4) == 2:
-            assert not ok
-            edges += 1
+        assert hm.ha_contact_adj_d(ta, tb, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g1a), P(g1b), 14)
+        hm.ha_geom_adj_d(ta, tb, feat.value, P(ha), P(hb), P(pa), P(pb), P(qa), P(qb), P(hbar), P(g2a), P(g2b))
+        want, got = np.concatenate([g1a, g1b]), np.concatenate([g2a, g2b])
+        kind = feat.value & 7
+        assert np.abs(got - want).max() <= 1e-10 * max(1.0, np.abs(want).max()), (FEAT_KINDS[kind], got, want)
+        count[kind] += 1
+    print({FEAT_KINDS[k]: v for k, v in count.items()})
+    assert min(count.values()) >= 40, count
+
+
+def test_geometry_adjoints_in_float_track_float64(hm):
+    """At float (the kernels' precision) the written-out adjoints stay within
+    float rounding of the float64 Jacobian, no further from it than the
+    dual-number replay at float."""
+    rng = np.random.default_rng(15)
+    worst = {}
+    for it in range(6000):
+        ta, tb, *xs = _pair_any(rng, it)
+        f32 = [np.asarray(x, F32) for x in xs]
+        h32 = np.asarray(rng.normal(size=7), F32)
+        feat = C.c_uint32(0)
+        if not hm.ha_pair_feat_f(ta, tb, *(P(x) for x in f32), C.byref(feat)):
             continue
-        assert ok
-        faces += 1
-        want = np.concatenate([g1a, g1b])
-        got = np.concatenate([g2a, g2b])
-        assert np.abs(got - want).max() <= 1e-10 * max(1.0, np.abs(want).max()), (got, want)
-    assert faces > 500 and edges > 10
+        fd = C.c_uint32(0)
+        d64 = [x.astype(F64) for x in f32]
+        if not hm.ha_pair_feat_d(ta, tb, *(P(x) for x in d64), C.byref(fd)) or fd.value != feat.value:
+            continue  # float and double round to different branches here
+        ga, gb = np.zeros(7), np.zeros(7)
+        h64 = h32.astype(F64)
+        hm.ha_contact_adj_d(ta, tb, *(P(x) for x in d64), P(h64), P(ga), P(gb), 14)
+        want = np.concatenate([ga, gb])
+        fa, fb = np.zeros(7, F32), np.zeros(7, F32)
+        hm.ha_geom_adj_f(ta, tb, feat.value, *(P(x) for x in f32), P(h32), P(fa), P(fb))
+        da, db = np.zeros(7, F32), np.zeros(7, F32)
+        assert hm.ha_contact_adj_f(ta, tb, *(P(x) for x in f32), P(h32), P(da), P(db))
+        scale = max(1.0, np.abs(want).max())
+        e_new = np.abs(np.concatenate([fa, fb]) - want).max() / scale
+        e_dual = np.abs(np.concatenate([da, db]) - want).max() / scale
+        kind = feat.value & 7
+        worst[kind] = max(worst.get(kind, 0.0), e_new)
+        assert e_new <= max(1e-4, 4.0 * e_dual), (FEAT_KINDS[kind], e_new, e_dual)
+    print({FEAT_KINDS[k]: f"{v:.1e}" for k, v in worst.items()})
+    assert set(worst) == set(FEAT_KINDS), worst
