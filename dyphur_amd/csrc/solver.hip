@@ -71,8 +71,9 @@ enum : uint32_t {
     kMNcc = 11,          // contacts solved; constraint k >= kMNcc is joint k - kMNcc
     kMNj = 12,           // joints solved
     kMJBad = 13,         // a joint of this solve failed validation
+    kMLive = 14,         // a contact left live by its first iteration (sweep_flow)
     kMBarAssign = 16,    // 2 words: grid barrier of assign_lists
-    kMBarSweep = 18,     // 2 words: grid barrier of sweep_persistent
+    kMBarSweep = 18,     // 2 words: grid barrier of the sweeps (sweep_flow: after iteration 0)
     kMFcnt = 20,         // 3 words: frontier sizes
     kMRegSweep = 24,     // 2 words: check-in of sweep_xcd (participants, arrivals)
     kMRegAssign = 26,    // 2 words: check-in of assign_lists
@@ -948,6 +949,8 @@ __global__ __launch_bounds__(1024) void sweep_xcd(const uint32_t *__restrict__ b
 // the lanes that are ready run while the others keep waiting, so a class
 // boundary inside a wave cannot stall it.  All threads are co-resident
 // (cooperative launch) and every wait is bounded (kErrFlow aborts the sweep).
+// Without joints, one barrier after iteration 0 decides whether any contact
+// is still live; usually none is, and the sweep ends there.
 // Versions and body records move through the L2 of XCD kSweepXcc (agent-scope
 // loads, stores acknowledged before the version store), as in sweep_xcd.
 // ---------------------------------------------------------------------------
@@ -991,7 +994,18 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
     const uint32_t gstride = nblk * blockDim.x;
     const uint32_t lane = lane_id();
     const uint32_t wave0 = b * blockDim.x + (threadIdx.x & ~63u);
+    uint32_t gen = 0;
     for (uint32_t it = 0; it < n_iters; ++it) {
+        if (!JOINTS && it == 1) {
+            // A contact's lambda changes only in its own projection, which
+            // sets it to the depth (contact_live then stays false for the
+            // rest of the solve) unless project_contact declines to move the
+            // bodies (lambda unchanged).  With no contact left live after
+            // iteration 0 the later iterations change nothing: their
+            // renormalisations are applied by the final scatter.
+            if (!xcd_barrier(meta + kMBarSweep, gen, nblk, err, kErrBarrier)) return;
+            if (ld_agent(&meta[kMLive]) == 0u) return;
+        }
         for (uint32_t i0 = wave0; i0 < nc; i0 += gstride) {
             const uint32_t i = i0 + lane;
             bool todo = i < nc;
@@ -1024,6 +1038,7 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
                         lam[i] = l;
                         if (da) store_aos(dyn, ab.x, A, it);
                         if (db) store_aos(dyn, ab.y, B, it);
+                        if (!JOINTS && it == 0 && contact_live(p1.z, l)) atomicOr(&meta[kMLive], 1u);
                     }
                     if (ready) todo = false;
                 }
