@@ -833,6 +833,8 @@ def main():
                  "timed_contacts_mean": float(np.mean([t["n_contacts"] for t in timed])),
                  "run_pairs_mean": float(np.mean([t["n_pairs"] for t in stats])),
                  "run_contacts_mean": float(np.mean([t["n_contacts"] for t in stats])),
+                 "timed_classes_mean": float(np.mean([t["n_colors"] for t in timed])),
+                 "timed_assign_rounds_mean": float(np.mean([t["rounds"] for t in timed])),
                  "colors_max": int(max(t["n_colors"] for t in stats)),
                  "pairs_max": int(max(t["n_pairs"] for t in stats)),
                  "contacts_max": int(max(t["n_contacts"] for t in stats))},

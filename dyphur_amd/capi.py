@@ -725,7 +725,7 @@ class World:
         self.solver.solve(s, self.np.contacts_view(), self.bodies, sc.dt, self.joints)
         st = self.solver.stats()
         self.last = dict(n_pairs=n_pairs, n_contacts=st["n_constraints"] - self.n_joints, n_colors=st["n_classes"],
-                         used_serial=st["used_serial"], dyn_endpoints=st["dyn_endpoints"])
+                         used_serial=st["used_serial"], dyn_endpoints=st["dyn_endpoints"], rounds=st["rounds"])
         return self.last
 
     def state(self) -> dict:

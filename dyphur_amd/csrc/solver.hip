@@ -79,7 +79,11 @@ enum : uint32_t {
     kMRegAssign = 26,    // 2 words: check-in of assign_lists
     kMRegMotor = 28,     // 2 words: check-in of joint_motors_xcd
     kMBarMotor = 30,     // 2 words: barrier of joint_motors_xcd
-    kMWords = 32,
+    kMDeep = 32,         // a contact's chain is longer than 256 rounds (colour_flow)
+    kMQHead = 64,        // colour_flow's ready queue, one 128-byte line per counter:
+    kMQTail = 96,        //   positions claimed, entries pushed (after the first
+    kMQDone = 128,       //   frontier), contacts coloured
+    kMWords = 160,
 };
 constexpr uint32_t kErrBarrier = 2u, kErrCapacity = 4u, kErrAssign = 8u, kErrFlow = 16u, kErrJoint = 32u;
 
@@ -319,7 +323,8 @@ __global__ __launch_bounds__(kSolveThreads) void list_rank(const uint32_t *__res
 }
 
 // Contacts heading every list of a dynamic endpoint form the first frontier
-// (fr0, size in fcnt[0]).
+// (fr0, size in fcnt[0]); the dataflow colouring's queue (queue[0, nc)) is
+// cleared to all ones.
 __global__ __launch_bounds__(kSolveThreads) void list_link(Cons cs, const uint32_t *__restrict__ meta,
                                                            const float *__restrict__ im,
                                                            const uint32_t *__restrict__ off,
@@ -327,13 +332,15 @@ __global__ __launch_bounds__(kSolveThreads) void list_link(Cons cs, const uint32
                                                            const uint32_t *__restrict__ slot,
                                                            const uint32_t *__restrict__ crank,
                                                            uint4 *__restrict__ lrec, uint32_t *__restrict__ pending,
-                                                           uint32_t *__restrict__ frontier, uint32_t *fcount) {
+                                                           uint32_t *__restrict__ frontier, uint32_t *fcount,
+                                                           uint64_t *__restrict__ queue) {
     const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
     const uint32_t lane = lane_id();
     for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
         const uint32_t c = c0 + lane;
         const bool pred[1] = {c < nc && link_contact(c, cs, ncc, im, off, deg, slot, crank, lrec, pending)};
         const uint32_t val[1] = {c};
+        if (c < nc) queue[c] = ~0ull;
         wave_append<1>(pred, val, frontier, fcount);
     }
 }
@@ -380,11 +387,163 @@ struct AssignBufs {
     const uint32_t *adj, *own, *off, *deg;
     uint32_t *slot, *crank;
     uint4 *lrec;
+    uint64_t *fq;  // colour_flow's queue: (round << 32) | contact
     uint32_t *pending;
     uint64_t *used, *state;
     uint32_t *fr0, *fr1;
     uint32_t *meta, *hist, *boff, *bfill;
+    uint32_t nbody;
 };
+
+// Touches every 128-byte line of [p, p + bytes) once across the participants
+// (gtid of gstride), so that the XCD's L2 holds the lines before the
+// dependent accesses of colour_flow reach them.
+__device__ __forceinline__ void touch_lines(const void *p, size_t bytes, uint32_t gtid, uint32_t gstride) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (size_t o = static_cast<size_t>(gtid) * 128u; o < bytes; o += static_cast<size_t>(gstride) * 128u) {
+        const uint32_t v = *reinterpret_cast<const uint32_t *>(b + o);
+        asm volatile("" ::"v"(v));
+    }
+}
+
+// Dataflow colouring: the JP assignment without round barriers.  A contact
+// can be coloured once the contacts ahead of it on both its lists are: then
+// used[a] | used[b] holds exactly its higher-priority neighbours' colours, as
+// in its JP round, and its round (the longest decreasing-priority chain
+// ending at it) is one more than the larger of its list predecessors'
+// rounds.  A predecessor decrements a successor's pending count and adds its
+// own round above bit 8 in the same atomic, so the last of a contact's (at
+// most two) predecessors learns the other's round from the returned value.
+// That lane continues with the successor itself (its list record loaded
+// while the predecessor was coloured); a second ready successor is pushed to
+// a queue (positions fr0[0, n0) hold the first frontier, then queue[0, ..)
+// entries (round << 32 | contact) in push order), from which lanes with
+// nothing in hand claim positions, one atomic per wave.  A claimed position
+// is read until its entry appears; it may never appear once every contact
+// is coloured, which ends the pass.  Writers wait for their stores (masks,
+// state) before the decrements; data written during the kernel is read with
+// agent-scope loads (the one-XCD handoff of assign_lists' rounds).
+// Progress: a claimed position below the queue's fill level is always
+// filled, so while uncoloured contacts remain some lane holds a ready one.
+// False on a timeout (kErrFlow).
+__device__ bool colour_flow(const AssignBufs &B, uint32_t nc, uint32_t *lhist, uint32_t &my_max,
+                            uint32_t &my_done, uint32_t &my_round) {
+    uint32_t *meta = B.meta;
+    uint64_t *queue = B.fq;
+    const uint32_t lane = lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t n0 = ld_agent(&meta[kMFcnt]);
+    uint32_t cur = kNone, pos = kNone, round = 0, spins = 0, unpublished = 0;
+    uint4 rec = make_uint4(kNone, kNone, kNone, kNone);
+    bool have_rec = false;
+    for (;;) {
+        // lanes with nothing in hand claim queue positions
+        const bool need = cur == kNone && pos == kNone;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+            const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(nm));
+            uint32_t base = 0;
+            if (lane == lead) base = atomicAdd(&meta[kMQHead], static_cast<uint32_t>(__popcll(nm)));
+            base = __shfl(base, static_cast<int>(lead), 64);
+            if (need) pos = base + static_cast<uint32_t>(__popcll(nm & lt));
+        }
+        if (pos < nc) {
+            if (pos < n0) {
+                cur = B.fr0[pos];
+                round = 1u;
+                pos = kNone;
+                have_rec = false;
+            } else {
+                const uint64_t w = ld_agent(&queue[pos - n0]);
+                if (w != ~0ull) {
+                    cur = static_cast<uint32_t>(w);
+                    round = static_cast<uint32_t>(w >> 32);
+                    pos = kNone;
+                    have_rec = false;
+                }
+            }
+        }
+        // colour the contacts in hand
+        bool push = false;
+        uint64_t pushed = 0;
+        const bool work = cur != kNone;
+        if (work) {
+            if (!have_rec) rec = B.lrec[cur];
+            const uint64_t ua = rec.x != kNone ? ld_agent(&B.used[rec.x]) : 0ull;
+            const uint64_t ub = rec.y != kNone ? ld_agent(&B.used[rec.y]) : 0ull;
+            const uint4 none4 = make_uint4(kNone, kNone, kNone, kNone);
+            const uint4 rec_a = rec.z != kNone ? B.lrec[rec.z] : none4;  // the successors' records
+            const uint4 rec_b = rec.w != kNone ? B.lrec[rec.w] : none4;
+            const uint64_t u = ua | ub;
+            uint32_t cls;
+            if (u == ~0ull) {
+                atomicOr(&meta[kMOverflow], 1u);
+                cls = kMaxColours;
+            } else {
+                cls = static_cast<uint32_t>(__builtin_ctzll(~u));
+            }
+            const uint64_t bit = 1ull << (cls & 63u);
+            if (rec.x != kNone) B.used[rec.x] = ua | bit;
+            if (rec.y != kNone) B.used[rec.y] = ub | bit;
+            B.state[cur] = (static_cast<uint64_t>(round) << 32) | cls;
+            if (round > 256u) atomicOr(&meta[kMDeep], 1u);
+            wait_vmem();  // masks and state in the L2 before a successor can start
+            const uint32_t rr = round < 0xffffu ? round : 0xffffu;  // rounds past 256 only flag
+            const uint32_t add = (rr << 8) - 1u;
+            const uint32_t oa = rec.z != kNone ? atomicAdd(&B.pending[rec.z], add) : 0u;
+            const uint32_t ob = rec.w != kNone ? atomicAdd(&B.pending[rec.w], add) : 0u;
+            const bool ra = rec.z != kNone && (oa & 0xffu) == 1u, rb = rec.w != kNone && (ob & 0xffu) == 1u;
+            const uint32_t na = 1u + (rr > (oa >> 8) ? rr : (oa >> 8));
+            const uint32_t nb = 1u + (rr > (ob >> 8) ? rr : (ob >> 8));
+            my_max = cls > my_max ? cls : my_max;
+            my_round = round > my_round ? round : my_round;
+            ++my_done;
+            atomicAdd(&lhist[cls < kMaxColours ? cls : kMaxColours], 1u);
+            const uint32_t succ_b = rec.w;
+            if (ra) {
+                cur = rec.z;
+                rec = rec_a;
+                round = na;
+                have_rec = true;
+                push = rb;
+                pushed = (static_cast<uint64_t>(nb) << 32) | succ_b;
+            } else if (rb) {
+                cur = succ_b;
+                rec = rec_b;
+                round = nb;
+                have_rec = true;
+            } else {
+                cur = kNone;
+                have_rec = false;
+            }
+        }
+        // a second ready successor goes to the queue
+        const uint64_t pm = __ballot(push);
+        if (pm) {
+            const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(pm));
+            uint32_t base = 0;
+            if (lane == lead) base = atomicAdd(&meta[kMQTail], static_cast<uint32_t>(__popcll(pm)));
+            base = __shfl(base, static_cast<int>(lead), 64);
+            if (push) queue[base + static_cast<uint32_t>(__popcll(pm & lt))] = pushed;
+        }
+        unpublished += static_cast<uint32_t>(__popcll(__ballot(work)));
+        if (!__ballot(cur != kNone)) {
+            // the count is read only by idle waves: a wave adds its coloured
+            // contacts when it runs out of work
+            if (unpublished) {
+                if (lane == 0) atomicAdd(&meta[kMQDone], unpublished);
+                unpublished = 0;
+            }
+            if (ld_agent(&meta[kMQDone]) >= nc) return true;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kGridSpinLimit) {
+                if (lane == 0) atomicOr(&meta[kMErr], kErrFlow);
+                return false;
+            }
+            if ((spins & 255u) == 0u && (ld_agent(&meta[kMErr]) & kErrFlow)) return false;
+        }
+    }
+}
 
 // JP rounds over frontiers in one cooperative launch.  Round r assigns the
 // contacts that head the ordered lists of all their dynamic endpoints: on a
@@ -426,6 +585,18 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         const uint32_t max_rounds = serial ? nc + 1u : 256u;
         uint32_t *cur = B.fr0, *nxt = B.fr1;
         uint32_t ncur = ld_agent(&fcnt[0]), round = 1;
+        if (!serial) {  // the colouring as dataflow; rounds only for the reference order
+            touch_lines(B.used, sizeof(uint64_t) * B.nbody, gtid, gstride);
+            touch_lines(B.lrec, sizeof(uint4) * nc, gtid, gstride);
+            touch_lines(B.pending, sizeof(uint32_t) * nc, gtid, gstride);
+            touch_lines(B.fq, sizeof(uint64_t) * nc, gtid, gstride);
+            touch_lines(B.fr0, sizeof(uint32_t) * ld_agent(&fcnt[0]), gtid, gstride);
+            uint32_t my_round = 0;
+            (void)colour_flow(B, nc, lhist, my_max, my_done, my_round);
+            my_round = wave_max_u32(my_round);
+            if (lane == 0 && my_round) atomicMax(&meta[kMRounds], my_round);
+            ncur = 0;
+        }
         for (; ncur > 0; ++round) {
             if (round > max_rounds) {  // deeper than the reference allows (uniform)
                 exceeded = true;
@@ -454,6 +625,10 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         }
         rounds = round - 1u;
         xcd_barrier(bar, gen, nblk, err, kErrBarrier);  // overflow flags of the last round
+        if (!serial) {
+            rounds = ld_agent(&meta[kMRounds]);
+            exceeded = ld_agent(&meta[kMDeep]) != 0u;  // a timeout leaves contacts unassigned: kErrAssign
+        }
         const bool fallback = !serial && (exceeded || ld_agent(&meta[kMOverflow]) != 0u);
         if (!fallback) break;
         // The reference's serial path: contact order, classes = dependency
@@ -501,16 +676,25 @@ __global__ __launch_bounds__(1024) void assign_lists(AssignBufs B) {
         if (!ok) atomicOr(err, kErrAssign);
         meta[kMRounds] = rounds;
         meta[kMNcls] = ncls;
-        if (serial) {
-            B.boff[ncls] = ok ? running : 0u;
-        } else {
-            uint32_t run = 0;
-            for (uint32_t k = 0; k < ncls; ++k) {
-                B.boff[k] = run;
-                run += ld_agent(&B.hist[k]);
+        if (serial) B.boff[ncls] = ok ? running : 0u;
+    }
+    if (!serial && gtid < 64u) {
+        // class offsets: exclusive scan of the <= 65 class sizes by one wave
+        const uint32_t h0 = gtid < ncls ? ld_agent(&B.hist[gtid]) : 0u;
+        const uint32_t h1 = gtid + 64u < ncls ? ld_agent(&B.hist[gtid + 64u]) : 0u;
+        uint32_t s0 = h0, s1 = h1;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t0 = __shfl_up(s0, d, 64), t1 = __shfl_up(s1, d, 64);
+            if (lane >= static_cast<uint32_t>(d)) {
+                s0 += t0;
+                s1 += t1;
             }
-            B.boff[ncls] = run;
         }
+        const uint32_t tot0 = __shfl(s0, 63, 64), tot1 = __shfl(s1, 63, 64);
+        if (gtid < ncls) B.boff[gtid] = s0 - h0;
+        if (gtid + 64u < ncls) B.boff[gtid + 64u] = tot0 + s1 - h1;
+        if (gtid == 0) B.boff[ncls] = tot0 + tot1;
     }
     for (uint32_t k = gtid; k < ncls; k += gstride) B.bfill[k] = 0u;
 }
@@ -1174,6 +1358,7 @@ struct dy_xpbd_s {
     uint32_t *meta = nullptr;  // kMWords device words
     uint4 *lrec = nullptr, *rseq = nullptr;
     uint2 *rec_ab = nullptr;
+    uint64_t *fq = nullptr;
     float4 *rec_p0 = nullptr, *rec_p1 = nullptr;
     DevBuf<uint32_t> deg, off, fill, scan_tmp, ver;
     DevBuf<uint64_t> used;
@@ -1255,7 +1440,7 @@ int collect_timing(dy_xpbd_s *x) {
 void free_constraint_bufs(dy_xpbd_s *x) {
     void **ptrs[] = {(void **)&x->lam, (void **)&x->state, (void **)&x->adj, (void **)&x->own, (void **)&x->slot,
                      (void **)&x->crank, (void **)&x->pending, (void **)&x->fr0, (void **)&x->fr1,
-                     (void **)&x->lrec, (void **)&x->boff, (void **)&x->bfill, (void **)&x->hist,
+                     (void **)&x->lrec, (void **)&x->fq, (void **)&x->boff, (void **)&x->bfill, (void **)&x->hist,
                      (void **)&x->rec_id, (void **)&x->cpos, (void **)&x->rseq, (void **)&x->rec_ab,
                      (void **)&x->rec_p0, (void **)&x->rec_p1};
     for (void **p : ptrs)
@@ -1284,6 +1469,7 @@ hipError_t alloc_constraint_bufs(dy_xpbd_s *x, uint32_t n_con) {
     dalloc(x->fr0, c);
     dalloc(x->fr1, c);
     dalloc(x->lrec, c);
+    dalloc(x->fq, c);
     dalloc(x->boff, ncls);
     dalloc(x->bfill, ncls);
     dalloc(x->hist, ncls);
@@ -1424,7 +1610,7 @@ int assign_by_rounds(dy_xpbd_s *x, hipStream_t st, const Cons &cv, const float *
                                                                    x->slot, x->crank);
             DY_LAUNCH_CHECK();
             list_link<<<grid_for(nc), kSolveThreads, 0, st>>>(cv, x->meta, im, x->off.p, x->deg.p, x->slot, x->crank,
-                                                              x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
+                                                              x->lrec, x->pending, x->fr0, x->meta + kMFcnt, x->fq);
             DY_LAUNCH_CHECK();
             continue;
         }
@@ -1540,15 +1726,16 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
                                                               x->crank);
     DY_LAUNCH_CHECK();
     list_link<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, bv->inv_mass, x->off.p, x->deg.p, x->slot,
-                                                         x->crank, x->lrec, x->pending, x->fr0, x->meta + kMFcnt);
+                                                         x->crank, x->lrec, x->pending, x->fr0, x->meta + kMFcnt,
+                                                         x->fq);
     DY_LAUNCH_CHECK();
 
     const int J = nj ? 1 : 0;
     bool coop = x->path == 0 && !x->coop_refused && x->coop_assign_max > 0 && x->coop_sweep_max[J] > 0;
     if (x->timing) DY_TRY(hipEventRecord(x->ev[0], st));
     if (coop) {
-        AssignBufs B{cs, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->pending,
-                     x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill};
+        AssignBufs B{cs, bv->inv_mass, x->adj, x->own, x->off.p, x->deg.p, x->slot, x->crank, x->lrec, x->fq,
+                     x->pending, x->used.p, x->state, x->fr0, x->fr1, x->meta, x->hist, x->boff, x->bfill, nb};
         void *args[] = {&B};
         // Participants on one XCD out of a launch with as many on every XCD:
         // by default 16 workgroups of 256 threads per XCD on a 256-CU device,
