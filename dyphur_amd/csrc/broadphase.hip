@@ -12,13 +12,16 @@
 //                      counters, workgroup scope); the last workgroup to finish
 //                      (agent-scope release/acquire around a ticket) merges the
 //                      few nodes whose ranges cross chunks.
-//   4. traverse_count  one thread per leaf in Morton order, 64-thread
-//                      workgroups with contiguous leaf runs per XCD: stackless
-//                      walk, counting partners b > a and keeping the 16
-//                      smallest in registers (sorted).
+//   4. traverse_team   a team of 16 lanes per leaf (Morton order), 256-thread
+//                      workgroups with contiguous leaf runs per XCD: the team
+//                      expands up to 16 nodes of its LDS stack per step,
+//                      collects the partners b > a in LDS and ranks them into
+//                      the row (16 smallest) and overflow buffer, ascending.
+//                      (traverse_count: one lane per leaf, kept as a variant.)
 //   5. scan of counts  row offsets per body a; the total is the true pair count.
-//   6. emit_pairs      one thread per body: writes its row (a, b0 < b1 < ...);
-//                      rows longer than 16 re-walk the tree for the next 16.
+//   6. emit_coop       one wavefront per 64 bodies writes their contiguous
+//                      output range with one pair per lane per store; rows
+//                      longer than 16 + 128 re-walk the tree for the rest.
 // The output is the reference's pair set in ascending (a, b) order, so
 // sort_pairs has nothing left to do.
 #include <cstdlib>
@@ -507,6 +510,7 @@ __device__ __forceinline__ uint32_t row_insert(uint32_t (&s)[kRow], uint32_t x) 
 // kStack levels; a deeper one flags err.
 constexpr uint32_t kWalkThreads = 64;
 constexpr uint32_t kStack = 68;
+template <uint32_t STRIDE = kWalkThreads>
 __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, const TPair *__restrict__ tp, int n,
                                          const Box6 &q, uint32_t thr, uint32_t (&s)[kRow], uint32_t *stk,
                                          uint32_t *err, uint32_t *ovf = nullptr) {
@@ -541,14 +545,14 @@ __device__ __forceinline__ uint32_t walk(const TNode *__restrict__ tn, const TPa
         const bool gl = pl && il, gr = pr && ir;
         if (gl && gr) {  // defer the right subtree
             if (sp >= kStack) { atomicOr(err, 4u); break; }
-            stk[(sp++) * kWalkThreads] = static_cast<uint32_t>(ch.y);
+            stk[(sp++) * STRIDE] = static_cast<uint32_t>(ch.y);
             x = ch.x;
         } else if (gl) {
             x = ch.x;
         } else if (gr) {
             x = ch.y;
         } else {
-            x = sp > 0 ? static_cast<int>(stk[(--sp) * kWalkThreads]) : -1;
+            x = sp > 0 ? static_cast<int>(stk[(--sp) * STRIDE]) : -1;
         }
     }
     return count;
@@ -583,26 +587,158 @@ __global__ __launch_bounds__(kWalkThreads) void traverse_count(const TNode *__re
         if (j < m) row[static_cast<size_t>(a) * kRow + j] = s[j];
 }
 
-__global__ __launch_bounds__(kWalkThreads) void emit_pairs(const TNode *__restrict__ tn,
-                                                           const TPair *__restrict__ tp, int n,
-                                                  const uint32_t *__restrict__ counts,
-                                                  const uint32_t *__restrict__ offs, const uint32_t *__restrict__ row,
-                                                  const uint32_t *__restrict__ ovf,
-                                                  const uint32_t *__restrict__ leaf_of,
-                                                  dy_contact_pair *__restrict__ pairs, uint32_t cap,
-                                                  uint32_t *err) {
-    __shared__ uint32_t stk[kStack * kWalkThreads];
-    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= static_cast<uint32_t>(n)) return;
-    const uint32_t L = counts[a];
-    if (L == 0) return;
-    const uint32_t off = offs[a];
-    if (off >= cap) return;
-    const uint32_t m = L < kRow ? L : kRow;
-    for (uint32_t j = 0; j < m; ++j) {
-        if (off + j < cap) pairs[off + j] = dy_contact_pair{a, row[static_cast<size_t>(a) * kRow + j]};
+// Team-cooperative traversal: kTeam lanes of a wavefront walk one query
+// together.  The team keeps a shared stack of internal nodes whose boxes
+// passed the test at their parent (LDS); each step pops up to kTeam of them,
+// one per lane, loads their two-wide records and tests both children;
+// passing internal children are pushed and passing leaves (bodies above the
+// query's) appended to the team's partner list, at positions from team
+// ballots (prefix counts), so a step costs one dependent record load for
+// the whole team.  Any order finds the same partner set; at the end each
+// partner's rank among them (a count of smaller ones) places it: ranks below
+// kRow in the row, the rest in the overflow buffer, both in ascending order.
+// A query whose stack (stack_cap entries) or partner list would outgrow the
+// team's LDS is walked again by one lane with the per-lane walk (the team's
+// stack area as its stack).
+// A Karras tree's walk pushes each internal node at most once, so a team
+// takes at most n - 1 steps; more flags err.
+constexpr uint32_t kTeamBlock = 256;
+constexpr uint32_t kTeamStack = 128;    // >= kStack: the fallback walk uses it
+constexpr uint32_t kTeamPartners = kRow + kOvf;  // a longer row is completed by emit_tail's walks
+static_assert(kTeamStack >= kStack && kTeamPartners <= kRow + kOvf, "team sizes");
+
+template <uint32_t kTeam>
+__global__ __launch_bounds__(kTeamBlock) void traverse_team(const TNode *__restrict__ tn,
+                                                            const TPair *__restrict__ tp, int n,
+                                                            uint32_t *__restrict__ counts,
+                                                            uint32_t *__restrict__ row, uint32_t *__restrict__ ovf,
+                                                            uint32_t *__restrict__ leaf_of, uint32_t *err,
+                                                            uint32_t xcd_map, uint32_t stack_cap) {
+    static_assert(64u % kTeam == 0u, "teams inside a wavefront");
+    constexpr uint32_t kTeamsPerBlock = kTeamBlock / kTeam;
+    __shared__ uint32_t stk_all[kTeamsPerBlock][kTeamStack];
+    __shared__ uint32_t prt_all[kTeamsPerBlock][kTeamPartners];
+    // contiguous runs of leaves per XCD, as traverse_count
+    uint32_t g = blockIdx.x;
+    if (xcd_map) {
+        const uint32_t G = gridDim.x, per = G / kXcds, rem = G % kXcds, x = g % kXcds, slot = g / kXcds;
+        g = x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
     }
-    if (L <= kRow) return;
+    const uint32_t team = threadIdx.x / kTeam, tl = threadIdx.x % kTeam;
+    const int k = static_cast<int>(g * kTeamsPerBlock + team);
+    if (k >= n) return;  // whole teams
+    uint32_t *stk = stk_all[team], *prt = prt_all[team];
+    const TNode leaf = tn[n - 1 + k];
+    const uint32_t a = static_cast<uint32_t>(leaf.child);
+    const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
+    const uint32_t lane = lane_id();
+    const uint64_t tmask = ((1ull << kTeam) - 1ull) << (lane & ~(kTeam - 1u));
+    const uint64_t below = tmask & ((1ull << lane) - 1ull);
+    const int n_int = n - 1;
+    uint32_t sp = 0, np = 0;
+    bool full = false;
+    {
+        const uint4 r = *reinterpret_cast<const uint4 *>(&tn[0]);
+        if (r.w > a && overlaps(unpack_box(r.x, r.y, r.z), q)) {
+            if (tl == 0) stk[0] = 0u;
+            sp = 1;
+        }
+    }
+    uint32_t steps = 0;
+    while (sp > 0) {
+        if (++steps > static_cast<uint32_t>(n)) {
+            if (tl == 0) atomicOr(err, 4u);
+            return;
+        }
+        const uint32_t npop = sp < kTeam ? sp : kTeam;
+        const int x = tl < npop ? static_cast<int>(stk[sp - 1u - tl]) : -1;
+        sp -= npop;
+        bool gl = false, gr = false, hl = false, hr = false;
+        uint32_t cl = 0, cr = 0, vl = 0, vr = 0;
+        if (x >= 0) {
+            if (x >= n_int) {
+                atomicOr(err, 4u);
+            } else {
+                const uint4 w0 = *reinterpret_cast<const uint4 *>(&tp[x]);
+                const uint4 w1 = *(reinterpret_cast<const uint4 *>(&tp[x]) + 1);
+                const int2 ch = *reinterpret_cast<const int2 *>(&tp[x].left);
+                const bool pl = w0.w > a && overlaps(unpack_box(w0.x, w0.y, w0.z), q);
+                const bool pr = w1.w > a && overlaps(unpack_box(w1.x, w1.y, w1.z), q);
+                const bool il = ch.x < n_int, ir = ch.y < n_int;
+                gl = pl && il;
+                gr = pr && ir;
+                hl = pl && !il;
+                hr = pr && !ir;
+                cl = static_cast<uint32_t>(ch.x);
+                cr = static_cast<uint32_t>(ch.y);
+                vl = w0.w;
+                vr = w1.w;
+            }
+        }
+        const uint64_t bgl = __ballot(gl) & tmask, bgr = __ballot(gr) & tmask;
+        const uint64_t bhl = __ballot(hl) & tmask, bhr = __ballot(hr) & tmask;
+        const uint32_t ngl = static_cast<uint32_t>(__popcll(bgl)), ngr = static_cast<uint32_t>(__popcll(bgr));
+        const uint32_t nhl = static_cast<uint32_t>(__popcll(bhl)), nhr = static_cast<uint32_t>(__popcll(bhr));
+        if (sp + ngl + ngr > stack_cap || np + nhl + nhr > kTeamPartners) {
+            full = true;
+            break;
+        }
+        if (gl) stk[sp + static_cast<uint32_t>(__popcll(bgl & below))] = cl;
+        if (gr) stk[sp + ngl + static_cast<uint32_t>(__popcll(bgr & below))] = cr;
+        if (hl) prt[np + static_cast<uint32_t>(__popcll(bhl & below))] = vl;
+        if (hr) prt[np + nhl + static_cast<uint32_t>(__popcll(bhr & below))] = vr;
+        sp += ngl + ngr;
+        np += nhl + nhr;
+    }
+    if (full) {
+        uint32_t c = 0;
+        if (tl == 0) {
+            uint32_t s[kRow];
+            c = walk<1>(tn, tp, n, q, a, s, stk, err, ovf + static_cast<size_t>(a) * kOvf);
+            counts[a] = c;
+            leaf_of[a] = static_cast<uint32_t>(k);
+            const uint32_t m = c < kRow ? c : kRow;
+            for (uint32_t j = 0; j < kRow; ++j)
+                if (j < m) row[static_cast<size_t>(a) * kRow + j] = s[j];
+        }
+        // A row of at most kRow + kOvf partners is emitted straight from the
+        // overflow buffer, so the walk's overflow entries (in eviction
+        // order) are put in ascending order, ranked in the team's LDS.
+        c = __shfl(c, static_cast<int>(lane & ~(kTeam - 1u)), 64);
+        if (c > kRow && c <= kRow + kOvf) {
+            uint32_t *o = ovf + static_cast<size_t>(a) * kOvf;
+            const uint32_t m2 = c - kRow;
+            for (uint32_t i = tl; i < m2; i += kTeam) stk[i] = o[i];
+            for (uint32_t i = tl; i < m2; i += kTeam) {
+                const uint32_t v = stk[i];
+                uint32_t r = 0;
+                for (uint32_t j = 0; j < m2; ++j) r += stk[j] < v ? 1u : 0u;
+                o[r] = v;
+            }
+        }
+        return;
+    }
+    for (uint32_t j = tl; j < np; j += kTeam) {
+        const uint32_t v = prt[j];
+        uint32_t r = 0;
+        for (uint32_t i = 0; i < np; ++i) r += prt[i] < v ? 1u : 0u;
+        if (r < kRow) row[static_cast<size_t>(a) * kRow + r] = v;
+        else ovf[static_cast<size_t>(a) * kOvf + (r - kRow)] = v;
+    }
+    if (tl == 0) {
+        counts[a] = np;
+        leaf_of[a] = static_cast<uint32_t>(k);
+    }
+}
+
+// Entries kRow.. of a body's row (more than kRow partners): from the
+// overflow buffer, kRow at a time in ascending order, or, past kRow + kOvf,
+// by walks for the next kRow partners above the last one written.
+__device__ __forceinline__ void emit_tail(const TNode *__restrict__ tn, const TPair *__restrict__ tp, int n,
+                                          uint32_t a, uint32_t L, uint32_t off, const uint32_t *__restrict__ row,
+                                          const uint32_t *__restrict__ ovf, const uint32_t *__restrict__ leaf_of,
+                                          dy_contact_pair *__restrict__ pairs, uint32_t cap, uint32_t *stk,
+                                          uint32_t *err) {
     uint32_t thr = row[static_cast<size_t>(a) * kRow + kRow - 1];
     uint32_t done = kRow;
     uint32_t s[kRow];
@@ -630,13 +766,88 @@ __global__ __launch_bounds__(kWalkThreads) void emit_pairs(const TNode *__restri
     const TNode leaf = tn[n - 1 + leaf_of[a]];
     const Box6 q = unpack_box(leaf.b01, leaf.b23, leaf.b45);
     while (done < L && off + done < cap) {
-        walk(tn, tp, n, q, thr, s, stk + threadIdx.x, err);
+        walk(tn, tp, n, q, thr, s, stk, err);
         const uint32_t c = (L - done) < kRow ? (L - done) : kRow;
         for (uint32_t j = 0; j < kRow; ++j)
             if (j < c && off + done + j < cap) pairs[off + done + j] = dy_contact_pair{a, s[j]};
         done += c;
         thr = s[kRow - 1];
     }
+}
+
+// One thread per body: writes its row (a, b0 < b1 < ...).
+__global__ __launch_bounds__(kWalkThreads) void emit_pairs(const TNode *__restrict__ tn,
+                                                           const TPair *__restrict__ tp, int n,
+                                                  const uint32_t *__restrict__ counts,
+                                                  const uint32_t *__restrict__ offs, const uint32_t *__restrict__ row,
+                                                  const uint32_t *__restrict__ ovf,
+                                                  const uint32_t *__restrict__ leaf_of,
+                                                  dy_contact_pair *__restrict__ pairs, uint32_t cap,
+                                                  uint32_t *err) {
+    __shared__ uint32_t stk[kStack * kWalkThreads];
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= static_cast<uint32_t>(n)) return;
+    const uint32_t L = counts[a];
+    if (L == 0) return;
+    const uint32_t off = offs[a];
+    if (off >= cap) return;
+    const uint32_t m = L < kRow ? L : kRow;
+    for (uint32_t j = 0; j < m; ++j) {
+        if (off + j < cap) pairs[off + j] = dy_contact_pair{a, row[static_cast<size_t>(a) * kRow + j]};
+    }
+    if (L > kRow) emit_tail(tn, tp, n, a, L, off, row, ovf, leaf_of, pairs, cap, stk + threadIdx.x, err);
+}
+
+// One wavefront per 64 consecutive bodies, whose pairs are one contiguous
+// range of the output: the 64 rows (contiguous) are staged in LDS with
+// coalesced loads, then each lane writes every 64th pair of the range,
+// finding its body by a binary search over the rows' offsets, so each store
+// instruction writes 64 consecutive pairs.  traverse_team leaves the
+// overflow entries of a row in ascending order, so entries kRow .. kRow +
+// kOvf come straight from the overflow buffer; rows longer than that are
+// completed by the body's lane (emit_tail's walks).
+__global__ __launch_bounds__(kWalkThreads) void emit_coop(const TNode *__restrict__ tn,
+                                                          const TPair *__restrict__ tp, int n,
+                                                          const uint32_t *__restrict__ counts,
+                                                          const uint32_t *__restrict__ offs,
+                                                          const uint32_t *__restrict__ row,
+                                                          const uint32_t *__restrict__ ovf,
+                                                          const uint32_t *__restrict__ leaf_of,
+                                                          dy_contact_pair *__restrict__ pairs, uint32_t cap,
+                                                          uint32_t *err) {
+    __shared__ uint32_t stk[kStack * kWalkThreads];
+    __shared__ uint4 lrow[kWalkThreads * kRow / 4];
+    __shared__ uint32_t loff[kWalkThreads + 1];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t a0 = blockIdx.x * kWalkThreads;
+    const uint32_t nb = static_cast<uint32_t>(n) - a0 < kWalkThreads ? static_cast<uint32_t>(n) - a0 : kWalkThreads;
+    const uint32_t a = a0 + lane;
+    const bool valid = lane < nb;
+    const uint32_t L = valid ? counts[a] : 0u, off = valid ? offs[a] : 0u;
+    const uint4 *src = reinterpret_cast<const uint4 *>(row + static_cast<size_t>(a0) * kRow);
+    for (uint32_t t = lane; t < nb * (kRow / 4); t += kWalkThreads) lrow[t] = src[t];
+    const uint32_t base = __shfl(off, 0, 64);
+    const uint32_t total = __shfl(off + L, static_cast<int>(nb - 1u), 64) - base;
+    loff[lane] = valid ? off - base : total;
+    if (lane == 0) loff[kWalkThreads] = total;
+    __syncthreads();
+    const uint32_t *lr = reinterpret_cast<const uint32_t *>(lrow);
+    for (uint32_t e = lane; e < total; e += kWalkThreads) {
+        if (base + e >= cap) break;
+        uint32_t lo = 0, hi = nb;  // loff[lo] <= e < loff[hi]
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (loff[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t j = e - loff[lo];
+        const uint32_t b = a0 + lo;
+        if (j < kRow) pairs[base + e] = dy_contact_pair{b, lr[lo * kRow + j]};
+        else if (loff[lo + 1] - loff[lo] <= kRow + kOvf)
+            pairs[base + e] = dy_contact_pair{b, ovf[static_cast<size_t>(b) * kOvf + (j - kRow)]};
+    }
+    if (valid && L > kRow + kOvf && off < cap)
+        emit_tail(tn, tp, n, a, L, off, row, ovf, leaf_of, pairs, cap, stk + threadIdx.x, err);
 }
 
 }  // namespace dy
@@ -664,6 +875,9 @@ struct dy_broadphase_s {
     // 32k bodies); the XCD map keeps each XCD on one region of the tree.
     // DYPHUR_TRAVERSE_XCD=0 turns the map off.
     uint32_t trav_xcd = 1;
+    uint32_t trav_team = 16;  // lanes per query of the team traversal (4, 8, 16; 0: traverse_count)
+    uint32_t emit_coop = 1;  // wavefront-wide pair emission (DYPHUR_EMIT_COOP=0: emit_pairs)
+    uint32_t team_stack = kTeamStack;  // team stack entries before the per-lane fallback (DYPHUR_TEAM_STACK)
     // Kernel timing (dy_broadphase_set_timing): events between the launches
     // of one build, read back when the next build starts or on request.
     bool timing = false, ev_pending = false;
@@ -748,6 +962,15 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
         }
     }
     if (const char *env = std::getenv("DYPHUR_TRAVERSE_XCD")) b->trav_xcd = std::atoi(env) ? 1u : 0u;
+    if (const char *env = std::getenv("DYPHUR_EMIT_COOP")) b->emit_coop = std::atoi(env) ? 1u : 0u;
+    if (const char *env = std::getenv("DYPHUR_TEAM_STACK")) {
+        const int v = std::atoi(env);
+        b->team_stack = v >= 1 && v <= static_cast<int>(kTeamStack) ? static_cast<uint32_t>(v) : kTeamStack;
+    }
+    if (const char *env = std::getenv("DYPHUR_TRAVERSE_TEAM")) {
+        const int t = std::atoi(env);
+        b->trav_team = (t == 4 || t == 8 || t == 16) ? static_cast<uint32_t>(t) : 0u;
+    }
     if (e == hipSuccess) e = hipMemsetAsync(b->root, 0, sizeof(int32_t), hs(s));
     if (e != hipSuccess) {
         bp_free(b);
@@ -808,13 +1031,33 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
                                                        bp->entries, bp->counters);
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[3], st));
-    traverse_count<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(
-        bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
+    if (bp->trav_team == 4) {
+        traverse_team<4><<<ceil_div(n, kTeamBlock / 4), kTeamBlock, 0, st>>>(
+            bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd,
+            bp->team_stack);
+    } else if (bp->trav_team == 8) {
+        traverse_team<8><<<ceil_div(n, kTeamBlock / 8), kTeamBlock, 0, st>>>(
+            bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd,
+            bp->team_stack);
+    } else if (bp->trav_team == 16) {
+        traverse_team<16><<<ceil_div(n, kTeamBlock / 16), kTeamBlock, 0, st>>>(
+            bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd,
+            bp->team_stack);
+    } else {
+        traverse_count<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(
+            bp->tn, bp->tp, ni, bp->counts, bp->row, bp->ovf, bp->leaf_of, bp->counters + 2, bp->trav_xcd);
+    }
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[4], st));
     DY_TRY_STATUS(scan_exclusive(st, bp->counts, bp->offs, n, bp->count, bp->scan_tmp));
-    emit_pairs<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(bp->tn, bp->tp, ni, bp->counts, bp->offs, bp->row, bp->ovf, bp->leaf_of,
-                                                 bp->pairs, bp->max_pairs, bp->counters + 2);
+    if (bp->emit_coop && bp->trav_team)  // needs traverse_team's ordered overflow entries
+        emit_coop<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(bp->tn, bp->tp, ni, bp->counts, bp->offs,
+                                                                       bp->row, bp->ovf, bp->leaf_of, bp->pairs,
+                                                                       bp->max_pairs, bp->counters + 2);
+    else
+        emit_pairs<<<ceil_div(n, kWalkThreads), kWalkThreads, 0, st>>>(bp->tn, bp->tp, ni, bp->counts, bp->offs,
+                                                                        bp->row, bp->ovf, bp->leaf_of, bp->pairs,
+                                                                        bp->max_pairs, bp->counters + 2);
     DY_LAUNCH_CHECK();
     if (tm) {
         DY_TRY(hipEventRecord(bp->ev[5], st));

@@ -15,21 +15,33 @@ constexpr uint32_t kScanItems = 8;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;  // one workgroup pass
 constexpr size_t kScanSingleMax = 1u << 16;
 
-// One workgroup walks all tiles with a running carry.  `offset_in` (may be
-// null) is added to every output; `total` (may be null) receives the sum.
+// One workgroup scans up to kScanSingleMax values: every tile's values are
+// loaded before the first tile is scanned (one memory latency for the whole
+// input), then the tiles are scanned in order with a running carry.
+// `total` (may be null) receives the sum.
 __global__ __launch_bounds__(kScanThreads) void scan_one_block(const uint32_t *__restrict__ in,
                                                                uint32_t *__restrict__ out, uint32_t n,
                                                                uint32_t *__restrict__ total) {
+    constexpr uint32_t kMaxTiles = static_cast<uint32_t>(kScanSingleMax / kScanTile);
     __shared__ uint32_t tile[kScanTile];
     __shared__ uint32_t wsum[17];
     const uint32_t t = threadIdx.x;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += kScanTile) {
+    const uint32_t ntiles = (n + kScanTile - 1u) / kScanTile;
+    uint32_t in_v[kMaxTiles][kScanItems];
+#pragma unroll
+    for (uint32_t b = 0; b < kMaxTiles; ++b)
 #pragma unroll
         for (uint32_t j = 0; j < kScanItems; ++j) {
-            const uint32_t idx = base + j * kScanThreads + t;
-            tile[j * kScanThreads + t] = idx < n ? in[idx] : 0u;
+            const uint32_t idx = b * kScanTile + j * kScanThreads + t;
+            in_v[b][j] = idx < n ? in[idx] : 0u;
         }
+    uint32_t carry = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < kMaxTiles; ++b) {
+        if (b >= ntiles) break;
+        const uint32_t base = b * kScanTile;
+#pragma unroll
+        for (uint32_t j = 0; j < kScanItems; ++j) tile[j * kScanThreads + t] = in_v[b][j];
         __syncthreads();
         uint32_t v[kScanItems], sum = 0;
 #pragma unroll

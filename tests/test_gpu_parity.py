@@ -311,3 +311,37 @@ def test_morton_sort_workgroup_counts(gpu, oracle, stage_inputs, monkeypatch, so
     got = bp.bvh(s, sc.n)
     assert np.array_equal(got["morton"], ref["morton"])
     assert np.array_equal(got["sorted_idx"], ref["sorted_idx"])
+
+
+@pytest.mark.parametrize("variant", [{}, {"DYPHUR_EMIT_COOP": "0"}, {"DYPHUR_TRAVERSE_TEAM": "0"},
+                                     {"DYPHUR_TRAVERSE_TEAM": "16"}, {"DYPHUR_TEAM_STACK": "3"}])
+def test_broadphase_long_rows(gpu, oracle, monkeypatch, variant):
+    """Dense clusters, so rows run past the 16 kept in registers and past
+    16 + 128 (the team traversal's partner list; the emission's re-walks);
+    every traversal and emission variant gives the reference's pair list.  A
+    3-entry team stack sends most queries to the per-lane fallback walk,
+    whose overflow entries the team then orders."""
+    for k, v in variant.items():
+        monkeypatch.setenv(k, v)
+    for n_dyn, seed, ext in ((400, 7, (2.0, 2.0, 2.0)), (1500, 8, (6.0, 5.0, 6.0))):
+        sc = scenes.random_mixed(n_dyn, seed=seed, extent=ext)
+        sc.max_pairs = sc.n * sc.n
+        ob, osh = oracle.Bodies(sc), oracle.Shapes(sc)
+        w = oracle.World.for_scene(sc)
+        n_ref = w.build_and_query(ob, osh, sc.bounds)
+        w.sort_pairs(n_ref)
+        ref_pairs = w.pairs(n_ref)
+        rows = np.bincount(ref_pairs[:, 0] if ref_pairs.ndim == 2 else ref_pairs["a"], minlength=sc.n)
+        print(f"[long rows] {variant} n={sc.n} pairs={n_ref} longest row {rows.max()} "
+              f"rows 17-64: {int(((rows > 16) & (rows <= 64)).sum())} 65-144: {int(((rows > 64) & (rows <= 144)).sum())} "
+              f"> 144: {int((rows > 144).sum())}", flush=True)
+        s = gpu.Stream()
+        db = _gpu_bodies(gpu, sc, s, ob.state())
+        ds = gpu.DeviceShapes(sc, s)
+        bp = gpu.Broadphase(s, sc.n, sc.max_pairs)
+        bp.build_and_query(s, db, ds, sc.bounds)
+        n = bp.download_count(s)
+        assert n == n_ref
+        assert np.array_equal(bp.pairs(s, n), ref_pairs)
+        if n_dyn == 400:
+            assert rows.max() > 144 and ((rows > 64) & (rows <= 144)).any() and ((rows > 16) & (rows <= 64)).any()
