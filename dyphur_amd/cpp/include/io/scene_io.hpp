@@ -37,7 +37,7 @@ struct SceneFileDesc {
 inline void write_scene(const std::string &prefix, uint32_t n_bodies, const uint32_t *body_shape_idx,
                         const ShapeParams *shapes, uint32_t n_shapes) {
     std::ofstream f(prefix + ".scene", std::ios::binary);
-    if (!f) throw std::runtime_error("write_scene: cannot open " + prefix + ".scene");
+    if (!f) throw std::runtime_error("scene file not writable: " + prefix + ".scene");
     const uint32_t hdr[2] = {n_bodies, n_shapes};
     f.write(reinterpret_cast<const char *>(hdr), sizeof hdr);
     f.write(reinterpret_cast<const char *>(body_shape_idx), static_cast<std::streamsize>(n_bodies) * 4);
@@ -48,12 +48,12 @@ inline void write_scene(const std::string &prefix, uint32_t n_bodies, const uint
         f.write(reinterpret_cast<const char *>(h), 12);
         f.write(reinterpret_cast<const char *>(&pad), 4);
     }
-    if (!f) throw std::runtime_error("write_scene: write failed for " + prefix + ".scene");
+    if (!f) throw std::runtime_error("scene file incomplete after writing: " + prefix + ".scene");
 }
 
 inline SceneFileDesc read_scene(const std::string &prefix) {
     std::ifstream f(prefix + ".scene", std::ios::binary);
-    if (!f) throw std::runtime_error("read_scene: cannot open " + prefix + ".scene");
+    if (!f) throw std::runtime_error("scene file not readable: " + prefix + ".scene");
     SceneFileDesc d;
     uint32_t hdr[2] = {0, 0};
     f.read(reinterpret_cast<char *>(hdr), sizeof hdr);
@@ -73,13 +73,13 @@ inline SceneFileDesc read_scene(const std::string &prefix) {
         d.shapes[i].half_y = h[1];
         d.shapes[i].half_z = h[2];
     }
-    if (!f) throw std::runtime_error("read_scene: truncated " + prefix + ".scene");
+    if (!f) throw std::runtime_error("scene file shorter than its header says: " + prefix + ".scene");
     return d;
 }
 
 inline void write_meshmap(const std::string &prefix, const std::vector<std::pair<uint32_t, std::string>> &entries) {
     std::ofstream f(prefix + ".meshmap");
-    if (!f) throw std::runtime_error("write_meshmap: cannot open " + prefix + ".meshmap");
+    if (!f) throw std::runtime_error("mesh map not writable: " + prefix + ".meshmap");
     for (const auto &e : entries) f << e.first << " " << e.second << "\n";
 }
 
@@ -105,7 +105,7 @@ struct TrajHeader {
 
 inline TrajHeader read_traj_header(const std::string &prefix) {
     std::ifstream f(prefix + ".trajectory", std::ios::binary);
-    if (!f) throw std::runtime_error("read_traj_header: cannot open " + prefix + ".trajectory");
+    if (!f) throw std::runtime_error("trajectory not readable: " + prefix + ".trajectory");
     TrajHeader h{};
     f.read(reinterpret_cast<char *>(&h), sizeof h);
     return h;
@@ -117,7 +117,7 @@ class TrajectoryWriter {
 public:
     TrajectoryWriter(const std::string &prefix, uint32_t n_bodies, uint32_t n_frames)
         : f_(prefix + ".trajectory", std::ios::binary), n_(n_bodies), row_(7 * static_cast<size_t>(n_bodies)) {
-        if (!f_) throw std::runtime_error("TrajectoryWriter: cannot open " + prefix + ".trajectory");
+        if (!f_) throw std::runtime_error("trajectory not writable: " + prefix + ".trajectory");
         const uint32_t hdr[2] = {n_bodies, n_frames};
         f_.write(reinterpret_cast<const char *>(hdr), sizeof hdr);
     }
@@ -129,7 +129,7 @@ public:
             r[3] = qw[i]; r[4] = qx[i]; r[5] = qy[i]; r[6] = qz[i];
         }
         f_.write(reinterpret_cast<const char *>(row_.data()), static_cast<std::streamsize>(row_.size() * 4));
-        if (!f_) throw std::runtime_error("TrajectoryWriter: write failed");
+        if (!f_) throw std::runtime_error("trajectory frame not written in full");
     }
 
 private:
@@ -140,7 +140,7 @@ private:
 
 inline void write_golden(const std::string &prefix, uint64_t hash) {
     std::ofstream f(prefix + ".golden");
-    if (!f) throw std::runtime_error("write_golden: cannot open " + prefix + ".golden");
+    if (!f) throw std::runtime_error("golden file not writable: " + prefix + ".golden");
     char buf[32];
     std::snprintf(buf, sizeof buf, "%016" PRIx64 "\n", hash);
     f << buf;
@@ -154,7 +154,7 @@ struct RunMetrics {
 
 inline void write_metrics(const std::string &prefix, const RunMetrics &m) {
     std::ofstream f(prefix + ".metrics.json");
-    if (!f) throw std::runtime_error("write_metrics: cannot open " + prefix + ".metrics.json");
+    if (!f) throw std::runtime_error("metrics file not writable: " + prefix + ".metrics.json");
     char buf[512];
     std::snprintf(buf, sizeof buf,
                   "{\n  \"n_bodies\": %u,\n  \"n_frames\": %u,\n  \"simulation_seconds\": %.4f,\n"

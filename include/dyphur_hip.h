@@ -149,7 +149,7 @@ typedef struct dy_broadphase_timing {
     double sort_ms;           /* Morton keys + key sort */
     double karras_ms;         /* Karras tree */
     double refit_ms;          /* fp16 leaf boxes + bottom-up refit */
-    double traverse_ms;       /* per-leaf traversal, pair counts and sorted rows */
+    double traverse_ms;       /* traversal: pair counts and sorted rows */
     double emit_ms;           /* row-count scan + pair emission */
 } dy_broadphase_timing;
 
