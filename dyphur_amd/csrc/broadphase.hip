@@ -9,9 +9,10 @@
 //                      traversal stackless.
 //   3. leaf_refit      one workgroup per 1024 leaves: fp16 leaf boxes, then a
 //                      counter-gated bottom-up merge inside the chunk (LDS
-//                      counters, workgroup scope); the last workgroup to finish
-//                      (agent-scope release/acquire around a ticket) merges the
-//                      few nodes whose ranges cross chunks.
+//                      counters and the chunk's topology in LDS); refit_top,
+//                      one workgroup, then merges the few nodes whose ranges
+//                      cross chunks (the launch boundary publishes the
+//                      chunks' nodes to it).
 //   4. traverse_team   a team of 16 lanes per leaf (Morton order), 256-thread
 //                      workgroups with contiguous leaf runs per XCD: the team
 //                      expands up to 16 nodes of its LDS stack per step,
@@ -346,9 +347,16 @@ __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, 
 // ---------------------------------------------------------------------------
 // 2. Karras build + escape links
 // ---------------------------------------------------------------------------
+// Internal nodes whose leaf ranges cross a refit chunk ("top" nodes) are
+// listed for refit_top: top_slot[i] = the node's place in top_list, -1 for a
+// node inside one chunk, -2 past kTopMax (refit_top then climbs in global
+// memory).
+constexpr uint32_t kTopMax = 1024;
 __global__ void karras_build(const uint64_t *__restrict__ codes, int n, int32_t *__restrict__ left,
                              int32_t *__restrict__ right, int32_t *__restrict__ parent, int2 *__restrict__ range,
-                             TNode *__restrict__ tn, uint32_t *__restrict__ cnt_top, int32_t *__restrict__ root) {
+                             TNode *__restrict__ tn, uint32_t *__restrict__ cnt_top, int32_t *__restrict__ root,
+                             int32_t *__restrict__ top_slot, uint32_t *__restrict__ top_list,
+                             uint32_t *top_count) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n - 1) return;
     int lo, hi, g;
@@ -361,6 +369,16 @@ __global__ void karras_build(const uint64_t *__restrict__ codes, int n, int32_t 
     parent[r] = i;
     range[i] = make_int2(lo, hi);
     cnt_top[i] = 0u;
+    int32_t ts = -1;
+    if (lo / static_cast<int>(kChunk) != hi / static_cast<int>(kChunk)) {
+        const uint32_t slot = atomicAdd(top_count, 1u);
+        ts = -2;
+        if (slot < kTopMax) {
+            top_list[slot] = static_cast<uint32_t>(i);
+            ts = static_cast<int32_t>(slot);
+        }
+    }
+    top_slot[i] = ts;
     tn[i].child = l;
     tn[l].escape = r;
     tn[r].escape = karras_escape(codes, n, hi);
@@ -381,15 +399,26 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
                                                      const int32_t *__restrict__ right,
                                                      const int2 *__restrict__ range, dy_node_box *nodebox,
                                                      TNode *tn, TPair *tp, uint32_t *cnt_top, uint32_t *entries,
-                                                     uint32_t *counters /* [0] entries, [1] done */) {
+                                                     uint32_t *counters /* [0] entries */) {
     __shared__ uint32_t lbox[2][kChunk][3];  // [0] leaves, [1] internal nodes
     __shared__ uint32_t lmax[2][kChunk];
     __shared__ uint32_t lcnt[kChunk];
-    __shared__ uint32_t is_last;
+    // topology of the internal nodes base .. base + kChunk - 1 (a node whose
+    // range lies inside the chunk has its number inside it: a Karras node is
+    // one end of its own range)
+    __shared__ int32_t lpar[kChunk], lleft[kChunk], lright[kChunk];
+    __shared__ int2 lrange[kChunk];
     const int tid = static_cast<int>(threadIdx.x);
     const int base = static_cast<int>(blockIdx.x * kChunk);
     const int k = base + tid;
     lcnt[tid] = 0u;
+    if (k < n - 1) {
+        lpar[tid] = parent[k];
+        lleft[tid] = left[k];
+        lright[tid] = right[k];
+        lrange[tid] = range[k];
+    }
+    const int leaf_par = k < n ? parent[n - 1 + k] : -1;
     __syncthreads();
     if (k < n) {
         const uint32_t body = sorted_idx[k];
@@ -410,10 +439,11 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
         int node = leaf;
         for (int guard = 0;; ++guard) {
             if (guard >= kMaxHeight) { atomicOr(&counters[2], 1u); break; }
-            const int par = parent[node];
+            const int par = node == leaf ? leaf_par : lpar[node - base];
             if (par < 0) break;
-            const int2 rg = range[par];
-            if (rg.x < base || rg.y >= base + static_cast<int>(kChunk)) {
+            const bool inside = par >= base && par < base + static_cast<int>(kChunk) && par < n - 1;
+            const int2 rg = inside ? lrange[par - base] : make_int2(-1, -1);
+            if (!inside || rg.x < base || rg.y >= base + static_cast<int>(kChunk)) {
                 const uint32_t e = atomicAdd(&counters[0], 1u);
                 if (e < static_cast<uint32_t>(n)) entries[e] = static_cast<uint32_t>(node);
                 else atomicOr(&counters[2], 2u);
@@ -422,7 +452,7 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
             const uint32_t old = __hip_atomic_fetch_add(&lcnt[par - base], 1u, __ATOMIC_ACQ_REL,
                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u) break;
-            const int c[2] = {left[par], right[par]};
+            const int c[2] = {lleft[par - base], lright[par - base]};
             uint32_t cb[2][3], cm[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
@@ -444,17 +474,90 @@ __global__ __launch_bounds__(kChunk) void leaf_refit(dy_body_view bv, dy_shape_v
             node = par;
         }
     }
-    // Publish this chunk's nodes, then elect the last workgroup.
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t ticket = atomicAdd(&counters[1], 1u);
-        is_last = (ticket == gridDim.x - 1) ? 1u : 0u;
+}
+
+// The nodes whose ranges cross chunks, bottom-up from the chunk subtrees'
+// roots that leaf_refit listed (one workgroup; the kernel boundary makes
+// every chunk's nodes visible).  Children counters cnt_top at workgroup
+// scope: the second child to arrive merges and climbs on.
+__global__ __launch_bounds__(kChunk) void refit_top(int n, const int32_t *__restrict__ parent,
+                                                    const int32_t *__restrict__ left,
+                                                    const int32_t *__restrict__ right, dy_node_box *nodebox,
+                                                    TNode *tn, TPair *tp, uint32_t *cnt_top,
+                                                    const uint32_t *__restrict__ entries,
+                                                    const int32_t *__restrict__ top_slot,
+                                                    const uint32_t *__restrict__ top_list,
+                                                    uint32_t *counters /* [0] entries, [1] top nodes */) {
+    const int tid = static_cast<int>(threadIdx.x);
+    const uint32_t n_top = counters[1];
+    if (n_top <= kTopMax) {
+        // All top nodes in LDS: each one's two child boxes (a child inside a
+        // chunk: loaded from tn; a top child: filled in when it is merged),
+        // its parent's slot and side, and a count of its top children.
+        __shared__ uint32_t box[kTopMax][2][4];  // fp16 box words, max body
+        __shared__ int32_t tnid[kTopMax], tpar[kTopMax], tchild[kTopMax][2];
+        __shared__ uint32_t tneed[kTopMax], tcnt[kTopMax];
+        for (uint32_t sl = static_cast<uint32_t>(tid); sl < n_top; sl += kChunk) {
+            const int node = static_cast<int>(top_list[sl]);
+            tnid[sl] = node;
+            const int c[2] = {left[node], right[node]};
+            const int p = parent[node];
+            int32_t pp = -1;
+            if (p >= 0) pp = (top_slot[p] << 1) | (left[p] == node ? 0 : 1);  // a top node's parent is a top node
+            tpar[sl] = pp;
+            uint32_t need = 0;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                tchild[sl][side] = c[side];
+                const bool top_child = c[side] < n - 1 && top_slot[c[side]] >= 0;
+                if (top_child) {
+                    ++need;
+                } else {
+                    box[sl][side][0] = tn[c[side]].b01;
+                    box[sl][side][1] = tn[c[side]].b23;
+                    box[sl][side][2] = tn[c[side]].b45;
+                    box[sl][side][3] = tn[c[side]].maxbody;
+                }
+            }
+            tneed[sl] = need;
+            tcnt[sl] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t sl = static_cast<uint32_t>(tid); sl < n_top; sl += kChunk) {
+            if (tneed[sl] != 0u) continue;
+            int cur = static_cast<int>(sl);
+            for (int guard = 0;; ++guard) {
+                if (guard >= kMaxHeight) { atomicOr(&counters[2], 1u); break; }
+                const uint32_t a[3] = {box[cur][0][0], box[cur][0][1], box[cur][0][2]};
+                const uint32_t b[3] = {box[cur][1][0], box[cur][1][1], box[cur][1][2]};
+                const uint32_t ml = box[cur][0][3], mr = box[cur][1][3];
+                uint32_t m[3];
+                merge_packed(a, b, m);
+                const uint32_t mb = ml > mr ? ml : mr;
+                const int node = tnid[cur];
+                store_node(nodebox, tn, node, m, mb);
+                store_pair(tp, node, a, ml, b, mr, tchild[cur][0], tchild[cur][1]);
+                const int32_t pp = tpar[cur];
+                if (pp < 0) break;
+                const int ps = pp >> 1, side = pp & 1;
+                box[ps][side][0] = m[0];
+                box[ps][side][1] = m[1];
+                box[ps][side][2] = m[2];
+                box[ps][side][3] = mb;
+                const uint32_t old = __hip_atomic_fetch_add(&tcnt[ps], 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (old + 1u < tneed[ps]) break;
+                cur = ps;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            counters[0] = 0u;
+            counters[1] = 0u;
+        }
+        return;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    uint32_t n_entries = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t n_entries = counters[0];
     n_entries = n_entries < static_cast<uint32_t>(n) ? n_entries : static_cast<uint32_t>(n);
     for (uint32_t e = tid; e < n_entries; e += kChunk) {
         int node = static_cast<int>(entries[e]);
@@ -862,7 +965,8 @@ struct dy_broadphase_s {
     int2 *range = nullptr;
     dy_node_box *nodebox = nullptr;
     TNode *tn = nullptr;
-    uint32_t *cnt_top = nullptr, *entries = nullptr, *counters = nullptr;
+    uint32_t *cnt_top = nullptr, *entries = nullptr, *counters = nullptr, *top_list = nullptr;
+    int32_t *top_slot = nullptr;
     uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *ovf = nullptr, *leaf_of = nullptr;
     uint32_t *scan_tmp = nullptr;
     uint32_t *sort_mat = nullptr, *sort_tot = nullptr;
@@ -890,7 +994,7 @@ void bp_free(dy_broadphase_s *b) {
     void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
                     b->root, b->range, b->nodebox, b->tn, b->tp, b->cnt_top, b->entries, b->counters,
                     b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs,
-                    b->sort_mat, b->sort_tot};
+                    b->sort_mat, b->sort_tot, b->top_slot, b->top_list};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t &e : b->ev)
@@ -938,6 +1042,7 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     DY_A(b->left, nb); DY_A(b->right, nb); DY_A(b->parent, nodes); DY_A(b->root, 1);
     DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes); DY_A(b->tp, nb);
     DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 8);
+    DY_A(b->top_slot, nb); DY_A(b->top_list, kTopMax);
     DY_A(b->sort_mat, 256 * ceil_div(static_cast<uint32_t>(nb), kSgTile)); DY_A(b->sort_tot, 256);
     DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->ovf, nb * kOvf); DY_A(b->leaf_of, nb);
     DY_A(b->scan_tmp, scan_tmp_words(nb)); DY_A(b->pairs, max_pairs);
@@ -1023,12 +1128,16 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     }
     if (tm) DY_TRY(hipEventRecord(bp->ev[1], st));
     karras_build<<<ceil_div(n - 1, 256), 256, 0, st>>>(bp->morton64, ni, bp->left, bp->right, bp->parent,
-                                                       bp->range, bp->tn, bp->cnt_top, bp->root);
+                                                       bp->range, bp->tn, bp->cnt_top, bp->root, bp->top_slot,
+                                                       bp->top_list, bp->counters + 1);
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[2], st));
     leaf_refit<<<ceil_div(n, kChunk), kChunk, 0, st>>>(*bodies, *shapes, ni, bp->idx0, bp->parent, bp->left,
                                                        bp->right, bp->range, bp->nodebox, bp->tn, bp->tp, bp->cnt_top,
                                                        bp->entries, bp->counters);
+    DY_LAUNCH_CHECK();
+    refit_top<<<1, kChunk, 0, st>>>(ni, bp->parent, bp->left, bp->right, bp->nodebox, bp->tn, bp->tp, bp->cnt_top,
+                                    bp->entries, bp->top_slot, bp->top_list, bp->counters);
     DY_LAUNCH_CHECK();
     if (tm) DY_TRY(hipEventRecord(bp->ev[3], st));
     if (bp->trav_team == 4) {

@@ -345,3 +345,28 @@ def test_broadphase_long_rows(gpu, oracle, monkeypatch, variant):
         assert np.array_equal(bp.pairs(s, n), ref_pairs)
         if n_dyn == 400:
             assert rows.max() > 144 and ((rows > 64) & (rows <= 144)).any() and ((rows > 16) & (rows <= 64)).any()
+
+
+def test_broadphase_100k_bodies(gpu, oracle):
+    """100 001 bodies: more than 1 024 tree nodes cross the 1 024-leaf refit
+    chunks, so the top of the tree is merged by refit_top's global-memory
+    climb instead of its LDS form; tree, boxes and pairs equal the
+    reference's."""
+    sc = scenes.random_mixed(100000, seed=21, extent=(60.0, 40.0, 60.0))
+    ob, osh = oracle.Bodies(sc), oracle.Shapes(sc)
+    w = oracle.World.for_scene(sc)
+    n_ref = w.build_and_query(ob, osh, sc.bounds)
+    ref = w.bvh(sc.n)
+    w.sort_pairs(n_ref)
+    ref_pairs = w.pairs(n_ref)
+    s = gpu.Stream()
+    db = _gpu_bodies(gpu, sc, s, ob.state())
+    ds = gpu.DeviceShapes(sc, s)
+    bp = gpu.Broadphase(s, sc.n, sc.max_pairs)
+    bp.build_and_query(s, db, ds, sc.bounds)
+    n = bp.download_count(s)
+    got = bp.bvh(s, sc.n)
+    for k in ("morton", "sorted_idx", "left", "right", "parent", "aabb"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert n == n_ref
+    assert np.array_equal(bp.pairs(s, n), ref_pairs)
