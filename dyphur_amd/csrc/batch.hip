@@ -1016,7 +1016,28 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                 mark(4);
                 // I iterations: classes in order, then renormalise every body
                 // (xpbd_solver.cpp:578-586,700-733).
+                uint32_t still_live = 0;  // a contact left live by its iteration-0 projection
                 for (int it = 0; it < A.n_iters; ++it) {
+                    if (it > 0 && tf[19] == 0u) {
+                        // A contact's lambda changes only in its own projection,
+                        // which sets it to the depth unless project_contact
+                        // declines to move the bodies; with no contact left live
+                        // after iteration 0 the later iterations project nothing
+                        // and only their renormalisations remain.
+                        for (uint32_t i = r; i < n; i += NT) {
+                            float qw = bv.rot_w[i], qx = bv.rot_x[i], qy = bv.rot_y[i], qz = bv.rot_z[i];
+                            for (int k = it; k < A.n_iters; ++k) {
+                                if constexpr (TAPE) {
+                                    float *qr = fb + TL.qr + 4ull * n * static_cast<uint32_t>(k);
+                                    qr[i] = qw; qr[n + i] = qx; qr[2 * n + i] = qy; qr[3 * n + i] = qz;
+                                }
+                                renormalise(qw, qx, qy, qz);
+                            }
+                            bv.rot_w[i] = qw; bv.rot_x[i] = qx; bv.rot_y[i] = qy; bv.rot_z[i] = qz;
+                        }
+                        tm.sync();
+                        break;
+                    }
                     for (uint32_t c = 0; c < n_cls; ++c) {
                         const uint32_t beg = S.coff[c], end = S.coff[c + 1];
                         for (uint32_t j = beg + r; j < end; j += NT) {
@@ -1048,9 +1069,11 @@ __device__ void run_scene(const Team &tm, const BatchArgs &A, uint32_t s, float 
                                 if (Ba.im != 0.f) store_body(bv, a, Ba);
                                 if (Bb.im != 0.f) store_body(bv, b, Bb);
                             }
+                            if (it == 0 && contact_live(ci.depth, l)) still_live = 1u;
                         }
                         tm.sync();
                     }
+                    if (it == 0 && still_live) atomicOr(&tf[19], 1u);
                     for (uint32_t i = r; i < n; i += NT) {
                         float qw = bv.rot_w[i], qx = bv.rot_x[i], qy = bv.rot_y[i], qz = bv.rot_z[i];
                         if constexpr (TAPE) {

@@ -1245,13 +1245,15 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
     }
 }
 
-// Dataflow sweep with a record cursor per lane: a lane moves on to its next
-// record (stride = participating threads, iterations in order) as soon as its
-// current one is done, so the lanes of a wave do not wait for each other.
-// The earliest unfinished (iteration, record) is always at its lane's cursor
-// and ready, so the sweep always progresses.  Waves sleep only when none of
-// their lanes advanced; a wave with no progress for kGridSpinLimit polls
-// aborts the sweep (kErrFlow).
+// Dataflow sweep with a record cursor per lane: within an iteration a lane
+// moves on to its next record (stride = participating threads) as soon as
+// its current one is done, so the lanes of a wave do not wait for each
+// other.  The earliest unfinished record is always at its lane's cursor and
+// ready, so the sweep always progresses.  Records that change nothing only
+// advance their bodies' versions, and without joints one barrier after
+// iteration 0 ends the sweep when no contact is left live, as in
+// sweep_flow.  Waves sleep only when none of their lanes advanced; a wave
+// with no progress for kGridSpinLimit polls aborts the sweep (kErrFlow).
 template <bool JOINTS>
 __global__ __launch_bounds__(1024) void sweep_flow_lanes(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
                                                          float *__restrict__ lam, float4 *__restrict__ dyn,
@@ -1264,60 +1266,61 @@ __global__ __launch_bounds__(1024) void sweep_flow_lanes(uint32_t n_iters, CRec 
     const uint32_t nc = meta[kMNc];
     const uint32_t gstride = nblk * blockDim.x, first = b * blockDim.x + threadIdx.x;
     const uint32_t lane = lane_id();
-    uint32_t it = 0, i = first;
-    if (i >= nc) it = n_iters;  // nothing for this lane
-    bool loaded = false;
-    uint2 ab = make_uint2(0u, 0u);
-    uint4 sq = make_uint4(0u, 0u, 0u, 0u);
-    float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
-    float l = 0.f;
-    uint32_t ta = 0, tb = 0, idle = 0;
-    while (__ballot(it < n_iters)) {
-        bool done_now = false;
-        if (it < n_iters) {
-            if (!loaded) {
-                ab = rec.ab[i];
-                sq = rseq[i];
-                p0 = rec.p0[i];
-                p1 = rec.p1[i];
-                l = lam[i];
-                ta = it * sq.y + sq.x;
-                tb = it * sq.w + sq.z;
-                loaded = true;
-            }
-            const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
-                               (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
-            if (ready) {
-                uint32_t ra, rb;
-                BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
-                const bool da = A.im != 0.f, db = B.im != 0.f;
-                apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
-                lam[i] = l;
-                if (da) store_aos(dyn, ab.x, A, it);
-                if (db) store_aos(dyn, ab.y, B, it);
-                done_now = true;
-            }
+    uint32_t gen = 0;
+    for (uint32_t it = 0; it < n_iters; ++it) {
+        if (!JOINTS && it == 1) {
+            if (!xcd_barrier(meta + kMBarSweep, gen, nblk, err, kErrBarrier)) return;
+            if (ld_agent(&meta[kMLive]) == 0u) return;
         }
-        wait_vmem();  // body stores acknowledged by the L2 before the versions move
-        if (done_now) {
-            if (sq.y) __hip_atomic_store(&ver[ab.x], ta + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (sq.w) __hip_atomic_store(&ver[ab.y], tb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            loaded = false;
-            i += gstride;
-            if (i >= nc) {
-                i = first;
-                ++it;
-            }
+        uint32_t i = first, idle = 0, ta = 0, tb = 0;
+        uint2 ab = make_uint2(0u, 0u);
+        uint4 sq = make_uint4(0u, 0u, 0u, 0u);
+        float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+        float l = 0.f;
+        if (i < nc) {
+            ab = rec.ab[i]; sq = rseq[i]; p0 = rec.p0[i]; p1 = rec.p1[i]; l = lam[i];
+            ta = it * sq.y + sq.x; tb = it * sq.w + sq.z;
         }
-        if (__ballot(done_now)) {
-            idle = 0;
-        } else {
-            __builtin_amdgcn_s_sleep(1);
-            if (++idle > kGridSpinLimit) {
-                if (lane == 0) atomicOr(err, kErrFlow);
-                return;
+        while (__ballot(i < nc)) {
+            bool done_now = false;
+            if (i < nc) {
+                const bool ready = (sq.y == 0u || ld_agent(&ver[ab.x]) == ta) &&
+                                   (sq.w == 0u || ld_agent(&ver[ab.y]) == tb);
+                if (ready) {
+                    const bool live = (JOINTS && __float_as_uint(p1.w) == kRecJoint) || contact_live(p1.z, l);
+                    if (live) {
+                        uint32_t ra, rb;
+                        BodyState A = load_aos_agent(dyn, sta, ab.x, ra), B = load_aos_agent(dyn, sta, ab.y, rb);
+                        const bool da = A.im != 0.f, db = B.im != 0.f;
+                        apply_record<JOINTS>(p0, p1, A, B, ra, rb, it, l, mu, dt, jpk);
+                        lam[i] = l;
+                        if (da) store_aos(dyn, ab.x, A, it);
+                        if (db) store_aos(dyn, ab.y, B, it);
+                        if (!JOINTS && it == 0 && contact_live(p1.z, l)) atomicOr(&meta[kMLive], 1u);
+                    }
+                    done_now = true;
+                }
             }
-            if ((idle & 255u) == 0u && (l2_read(err) & kErrFlow)) return;
+            wait_vmem();  // body stores acknowledged by the L2 before the versions move
+            if (done_now) {
+                if (sq.y) __hip_atomic_store(&ver[ab.x], ta + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (sq.w) __hip_atomic_store(&ver[ab.y], tb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                i += gstride;
+                if (i < nc) {  // the next record, loaded now
+                    ab = rec.ab[i]; sq = rseq[i]; p0 = rec.p0[i]; p1 = rec.p1[i]; l = lam[i];
+                    ta = it * sq.y + sq.x; tb = it * sq.w + sq.z;
+                }
+            }
+            if (__ballot(done_now)) {
+                idle = 0;
+            } else {
+                __builtin_amdgcn_s_sleep(1);
+                if (++idle > kGridSpinLimit) {
+                    if (lane == 0) atomicOr(err, kErrFlow);
+                    return;
+                }
+                if ((idle & 255u) == 0u && (l2_read(err) & kErrFlow)) return;
+            }
         }
     }
 }
