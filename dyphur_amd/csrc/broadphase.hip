@@ -211,84 +211,104 @@ __global__ __launch_bounds__(kMsThreads) void morton_sort(dy_body_view bv, dy_aa
         morton64[i] = (static_cast<uint64_t>(code0[i]) << 32) | idx0[i];
 }
 
-// Multi-workgroup form of the same sort for larger n: one cooperative launch,
-// four 8-bit LSD passes over 1024-key tiles, each pass in three phases
-// separated by grid barriers:
-//   A. per-tile digit histograms into mat[digit * n_tiles + tile];
-//   B. per-digit exclusive scan of mat across tiles, row totals into tot;
-//   C. digit bases (scan of tot), then each tile scatters its keys stably:
-//      position = base[d] + mat[d][tile] + rank of the key among the tile's
-//      keys with digit d (wave ballots, waves in index order).
-// The last pass also writes the 64-bit keys (code << 32 | body).
+// Multi-workgroup form of the same sort for larger n: one cooperative launch
+// whose workgroups on one XCD do the work (the others check in and leave),
+// four 8-bit LSD passes over 1024-key tiles.  Workgroup b owns a contiguous
+// run of tiles.  Pass p:
+//   A. each workgroup counts the digits of its tiles' keys (pass p's input
+//      order) into hist[tile][digit] (pass 0: while computing the keys);
+//   B. after a barrier, every workgroup reads the counts of all tiles (digit
+//      d's keys in the tiles before its own and in total; a scan of the totals
+//      gives the digit bases), ranks its keys within each tile (wave ballots,
+//      waves in index order: stable) and scatters them;
+// and a barrier between the passes: seven barriers in all.  The participants
+// share the XCD's L2, so the barriers need no cache maintenance: whatever the
+// kernel itself wrote (keys, counts) is read with agent-scope loads, which
+// skip the vector L1 and are served by that L2.  The last pass also writes
+// the 64-bit keys (code << 32 | body).
 constexpr uint32_t kSgThreads = 256;
+constexpr uint32_t kSortXcc = 0;
 constexpr uint32_t kSgWaves = kSgThreads / 64;
 constexpr uint32_t kSgItems = 4;
 constexpr uint32_t kSgTile = kSgThreads * kSgItems;
 constexpr uint32_t kErrSortBarrier = 8u;
 
+__device__ __forceinline__ uint32_t sort_digit(uint32_t key, uint32_t pass) {
+    return (key >> (8u * pass)) & (pass == 3u ? 0x3fu : 0xffu);
+}
+
 __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, dy_aabb bd, uint32_t n,
                                                                uint32_t n_tiles, uint32_t *code0, uint32_t *idx0,
-                                                               uint32_t *code1, uint32_t *idx1, uint32_t *mat,
-                                                               uint32_t *tot, uint64_t *morton64, uint32_t *bar,
+                                                               uint32_t *code1, uint32_t *idx1, uint32_t *hist,
+                                                               uint64_t *morton64, uint32_t *bar, uint32_t *reg,
                                                                uint32_t *err) {
     __shared__ uint32_t lh[256];
     __shared__ uint32_t dbase[256];
     __shared__ uint32_t wcnt[kSgWaves][256];
     __shared__ uint32_t wsum[17];
-    uint32_t gen = 0;
+    uint32_t gen = 0, bidx = 0, nblk = 0;
+    if (!xcd_checkin(kSortXcc, reg, err, kErrSortBarrier, bidx, nblk)) return;
     const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t i = blockIdx.x * kSgThreads + t; i < n; i += gridDim.x * kSgThreads) {
-        const uint32_t qx = quantize10(bv.pos_x[i], bd.min_x, bd.max_x);
-        const uint32_t qy = quantize10(bv.pos_y[i], bd.min_y, bd.max_y);
-        const uint32_t qz = quantize10(bv.pos_z[i], bd.min_z, bd.max_z);
-        code0[i] = morton3(qx, qy, qz);
-        idx0[i] = i;
+    const uint32_t per = (n_tiles + nblk - 1u) / nblk;
+    const uint32_t t0 = bidx * per < n_tiles ? bidx * per : n_tiles;
+    const uint32_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        lh[t] = 0;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kSgItems; ++j) {
+            const uint32_t i = tile * kSgTile + j * kSgThreads + t;
+            if (i < n) {
+                const uint32_t qx = quantize10(bv.pos_x[i], bd.min_x, bd.max_x);
+                const uint32_t qy = quantize10(bv.pos_y[i], bd.min_y, bd.max_y);
+                const uint32_t qz = quantize10(bv.pos_z[i], bd.min_z, bd.max_z);
+                const uint32_t code = morton3(qx, qy, qz);
+                code0[i] = code;
+                idx0[i] = i;
+                atomicAdd(&lh[sort_digit(code, 0u)], 1u);
+            }
+        }
+        __syncthreads();
+        hist[static_cast<size_t>(tile) * 256u + t] = lh[t];
+        __syncthreads();
     }
-    grid_barrier(bar, gen, err, kErrSortBarrier);
+    if (!xcd_barrier(bar, gen, nblk, err, kErrSortBarrier)) return;
     uint32_t *sk = code0, *sv = idx0, *dk = code1, *dv = idx1;
     for (uint32_t pass = 0; pass < 4; ++pass) {
-        const uint32_t bit = 8u * pass;
-        const uint32_t mask = pass == 3 ? 0x3fu : 0xffu;
-        // A. tile histograms
-        for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-            lh[t] = 0;
-            __syncthreads();
+        uint32_t *h = hist;  // read in B before the barrier that ends the pass, rewritten in A after it
+        if (pass > 0) {  // the own tiles' counts
+            for (uint32_t tile = t0; tile < t1; ++tile) {
+                lh[t] = 0;
+                __syncthreads();
 #pragma unroll
-            for (uint32_t j = 0; j < kSgItems; ++j) {
-                const uint32_t idx = tile * kSgTile + j * kSgThreads + t;
-                if (idx < n) atomicAdd(&lh[(sk[idx] >> bit) & mask], 1u);
-            }
-            __syncthreads();
-            mat[t * n_tiles + tile] = lh[t];
-            __syncthreads();
-        }
-        grid_barrier(bar, gen, err, kErrSortBarrier);
-        // B. per-digit scan across tiles, one wavefront per digit (no
-        // workgroup barriers: the 256 digits spread over every wave of the grid)
-        for (uint32_t d = blockIdx.x * kSgWaves + wave; d < 256u; d += gridDim.x * kSgWaves) {
-            uint32_t carry = 0;
-            for (uint32_t c0 = 0; c0 < n_tiles; c0 += 64u) {
-                const bool in = c0 + lane < n_tiles;
-                const uint32_t v = in ? mat[d * n_tiles + c0 + lane] : 0u;
-                uint32_t x = v;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t y = __shfl_up(x, static_cast<unsigned>(o), 64);
-                    if (lane >= static_cast<uint32_t>(o)) x += y;
+                for (uint32_t j = 0; j < kSgItems; ++j) {
+                    const uint32_t idx = tile * kSgTile + j * kSgThreads + t;
+                    if (idx < n) atomicAdd(&lh[sort_digit(ld_agent(&sk[idx]), pass)], 1u);
                 }
-                if (in) mat[d * n_tiles + c0 + lane] = carry + x - v;
-                carry += __shfl(x, 63, 64);
+                __syncthreads();
+                h[static_cast<size_t>(tile) * 256u + t] = lh[t];
+                __syncthreads();
             }
-            if (lane == 0) tot[d] = carry;
+            if (!xcd_barrier(bar, gen, nblk, err, kErrSortBarrier)) return;
         }
-        grid_barrier(bar, gen, err, kErrSortBarrier);
-        // C. digit bases and stable scatter
+        // digit t: keys in the tiles before t0 and in all tiles
+        uint32_t before = 0, total = 0;
+        for (uint32_t tt = 0; tt < n_tiles; tt += 8u) {
+            uint32_t v[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) v[k] = tt + k < n_tiles ? ld_agent(&h[(tt + k) * 256u + t]) : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                total += v[k];
+                before += tt + k < t0 ? v[k] : 0u;
+            }
+        }
         {
-            uint32_t total = 0;
-            dbase[t] = block_exclusive_scan(tot[t], wsum, total);
+            uint32_t all = 0;
+            dbase[t] = block_exclusive_scan(total, wsum, all) + before;
         }
-        for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        for (uint32_t tile = t0; tile < t1; ++tile) {
             for (uint32_t i = t; i < kSgWaves * 256u; i += kSgThreads) (&wcnt[0][0])[i] = 0;
             __syncthreads();
             uint32_t kk[kSgItems], vv[kSgItems], dd[kSgItems], rr[kSgItems];
@@ -296,9 +316,9 @@ __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, 
             for (uint32_t j = 0; j < kSgItems; ++j) {
                 const uint32_t idx = tile * kSgTile + wave * (64 * kSgItems) + j * 64 + lane;
                 const bool valid = idx < n;
-                kk[j] = valid ? sk[idx] : 0u;
-                vv[j] = valid ? sv[idx] : 0u;
-                const uint32_t d = valid ? ((kk[j] >> bit) & mask) : 256u;
+                kk[j] = valid ? ld_agent(&sk[idx]) : 0u;
+                vv[j] = valid ? ld_agent(&sv[idx]) : 0u;
+                const uint32_t d = valid ? sort_digit(kk[j], pass) : 256u;
                 uint64_t peers = ~0ull;
 #pragma unroll
                 for (int b = 0; b < 9; ++b) {
@@ -314,31 +334,30 @@ __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, 
                 __builtin_amdgcn_wave_barrier();
             }
             __syncthreads();
-            {
-                uint32_t run = 0;
-                for (uint32_t w = 0; w < kSgWaves; ++w) {
-                    const uint32_t c = wcnt[w][t];
-                    wcnt[w][t] = run;
-                    run += c;
-                }
+            uint32_t in_tile = 0;  // this tile's keys with digit t
+            for (uint32_t w = 0; w < kSgWaves; ++w) {
+                const uint32_t c = wcnt[w][t];
+                wcnt[w][t] = in_tile;
+                in_tile += c;
             }
             __syncthreads();
 #pragma unroll
             for (uint32_t j = 0; j < kSgItems; ++j) {
                 if (dd[j] < 256u) {
-                    const uint32_t pos = dbase[dd[j]] + mat[dd[j] * n_tiles + tile] + wcnt[wave][dd[j]] + rr[j];
-                    if (pos >= n) {  // inconsistent histogram: flag instead of writing out of range
+                    const uint32_t pos = dbase[dd[j]] + wcnt[wave][dd[j]] + rr[j];
+                    if (pos >= n) {  // inconsistent counts: flag instead of writing out of range
                         atomicOr(err, kErrSortBarrier);
                         continue;
                     }
                     dk[pos] = kk[j];
                     dv[pos] = vv[j];
-                    if (pass == 3) morton64[pos] = (static_cast<uint64_t>(kk[j]) << 32) | vv[j];
+                    if (pass == 3u) morton64[pos] = (static_cast<uint64_t>(kk[j]) << 32) | vv[j];
                 }
             }
             __syncthreads();
+            dbase[t] += in_tile;
         }
-        if (pass < 3) grid_barrier(bar, gen, err, kErrSortBarrier);
+        if (pass < 3 && !xcd_barrier(bar, gen, nblk, err, kErrSortBarrier)) return;
         uint32_t *tk = sk; sk = dk; dk = tk;
         uint32_t *tv = sv; sv = dv; dv = tv;
     }
@@ -969,7 +988,7 @@ struct dy_broadphase_s {
     int32_t *top_slot = nullptr;
     uint32_t *counts = nullptr, *offs = nullptr, *row = nullptr, *ovf = nullptr, *leaf_of = nullptr;
     uint32_t *scan_tmp = nullptr;
-    uint32_t *sort_mat = nullptr, *sort_tot = nullptr;
+    uint32_t *sort_hist = nullptr;  // morton_sort_grid's digit counts, 256 per tile
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
     uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
@@ -994,7 +1013,7 @@ void bp_free(dy_broadphase_s *b) {
     void *ptrs[] = {b->code0, b->idx0, b->code1, b->idx1, b->morton64, b->left, b->right, b->parent,
                     b->root, b->range, b->nodebox, b->tn, b->tp, b->cnt_top, b->entries, b->counters,
                     b->counts, b->offs, b->row, b->ovf, b->leaf_of, b->scan_tmp, b->pairs,
-                    b->sort_mat, b->sort_tot, b->top_slot, b->top_list};
+                    b->sort_hist, b->top_slot, b->top_list};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t &e : b->ev)
@@ -1043,16 +1062,16 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     DY_A(b->range, nb); DY_A(b->nodebox, nodes); DY_A(b->tn, nodes); DY_A(b->tp, nb);
     DY_A(b->cnt_top, nb); DY_A(b->entries, nb); DY_A(b->counters, 8);
     DY_A(b->top_slot, nb); DY_A(b->top_list, kTopMax);
-    DY_A(b->sort_mat, 256 * ceil_div(static_cast<uint32_t>(nb), kSgTile)); DY_A(b->sort_tot, 256);
+    DY_A(b->sort_hist, 256 * ceil_div(static_cast<uint32_t>(nb), kSgTile));
     DY_A(b->counts, nb); DY_A(b->offs, nb); DY_A(b->row, nb * kRow); DY_A(b->ovf, nb * kOvf); DY_A(b->leaf_of, nb);
     DY_A(b->scan_tmp, scan_tmp_words(nb)); DY_A(b->pairs, max_pairs);
 #undef DY_A
     if (e == hipSuccess) b->count = b->counters + 3;
     if (e == hipSuccess) e = hipMemsetAsync(b->counters, 0, 8 * sizeof(uint32_t), hs(s));
     if (e == hipSuccess) {
-        // Workgroups of morton_sort_grid: at most one per 1024 keys, 32 by
-        // default (DYPHUR_SORT_BLOCKS overrides), and no more than can be
-        // co-resident for a cooperative launch.
+        // Participating workgroups of morton_sort_grid (on one XCD): 32 by
+        // default (DYPHUR_SORT_BLOCKS overrides), and no more than one XCD
+        // holds at once for a cooperative launch of eight times as many.
         int dev = 0, coop = 0, per_cu = 0, cus = 0;
         e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
@@ -1063,7 +1082,8 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
         if (e == hipSuccess && coop && per_cu > 0) {
             int want = 32;
             if (const char *env = std::getenv("DYPHUR_SORT_BLOCKS")) want = std::atoi(env);
-            b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > per_cu * cus ? per_cu * cus : want));
+            const int cap = per_cu * cus / 8;
+            b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > cap ? cap : want));
         }
     }
     if (const char *env = std::getenv("DYPHUR_TRAVERSE_XCD")) b->trav_xcd = std::atoi(env) ? 1u : 0u;
@@ -1101,7 +1121,7 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     if (n > bp->max_bodies || n >= kMortonMax) return DY_ERR_INVALID_ARG;
     hipStream_t st = hs(s);
     if (bp->timing) DY_TRY_STATUS(bp_collect_timing(bp));
-    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 4 * sizeof(uint32_t), st));
+    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 6 * sizeof(uint32_t), st));
     if (n < 2) return DY_OK;
     const bool tm = bp->timing;
     if (tm) DY_TRY(hipEventRecord(bp->ev[0], st));
@@ -1109,14 +1129,16 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     const uint32_t n_tiles = ceil_div(n, kSgTile);
     bool sorted = false;
     if (bp->sort_blocks > 0 && n_tiles > 4) {
-        const uint32_t blocks = bp->sort_blocks;  // all of them: the digit scans use every wave
+        // sort_blocks participants on one XCD out of a launch with as many on
+        // every XCD (8 on gfx950)
+        const uint32_t blocks = 8u * bp->sort_blocks;
         dy_body_view bv = *bodies;
         dy_aabb bd = *bounds;
         uint32_t nv = n, ntv = n_tiles;
-        uint32_t *c0 = bp->code0, *i0 = bp->idx0, *c1 = bp->code1, *i1 = bp->idx1, *mat = bp->sort_mat,
-                 *tot = bp->sort_tot, *bar = bp->counters + 4, *err = bp->counters + 2;
+        uint32_t *c0 = bp->code0, *i0 = bp->idx0, *c1 = bp->code1, *i1 = bp->idx1, *hist = bp->sort_hist,
+                 *bar = bp->counters + 4, *reg = bp->counters + 6, *err = bp->counters + 2;
         uint64_t *m64 = bp->morton64;
-        void *args[] = {&bv, &bd, &nv, &ntv, &c0, &i0, &c1, &i1, &mat, &tot, &m64, &bar, &err};
+        void *args[] = {&bv, &bd, &nv, &ntv, &c0, &i0, &c1, &i1, &hist, &m64, &bar, &reg, &err};
         sorted = try_coop_launch(reinterpret_cast<const void *>(morton_sort_grid), dim3(blocks), dim3(kSgThreads),
                                  args, st) == hipSuccess;
         if (!sorted) bp->sort_blocks = 0;  // refused: the single-workgroup sort from now on

@@ -292,10 +292,11 @@ def test_solver_launch_shapes_bitexact(gpu, oracle, stage_inputs, monkeypatch, s
         assert np.array_equal(bits(got[f]), bits(ref[f])), f
 
 
-@pytest.mark.parametrize("sort_blocks", ["0", "1", "7", "32"])
+@pytest.mark.parametrize("sort_blocks", ["0", "1", "7", "32", "64"])
 def test_morton_sort_workgroup_counts(gpu, oracle, stage_inputs, monkeypatch, sort_blocks):
     """The Morton sort gives the reference order with one workgroup (0) and
-    with the cooperative multi-workgroup form at any grid size."""
+    with the cooperative multi-workgroup form at any number of participating
+    workgroups (on one XCD)."""
     monkeypatch.setenv("DYPHUR_SORT_BLOCKS", sort_blocks)
     sc, ob, osh = stage_inputs["mixed_12000"]
     print(f"[morton sort] blocks={sort_blocks} n={sc.n}", flush=True)
