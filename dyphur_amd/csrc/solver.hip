@@ -183,14 +183,41 @@ __device__ __forceinline__ JointIn load_joint(const float4 *__restrict__ jpk, ui
     return r;
 }
 
+// Runs of equal keys over consecutive lanes (kNone: no key).  Contacts come
+// grouped by their first body (the broadphase lists each body's partners in
+// one row), so one atomic per run of body a replaces one per contact.
+struct WaveRun {
+    uint32_t start, len;  // first lane of this lane's run, its length
+    bool head;            // this lane starts the run
+};
+__device__ __forceinline__ WaveRun wave_run(uint32_t key) {
+    const uint32_t lane = lane_id();
+    const uint32_t prev = __shfl_up(key, 1u, 64);
+    const bool head = lane == 0 || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    const uint32_t start = 63u - static_cast<uint32_t>(__builtin_clzll(heads & le));
+    const uint64_t after = heads & ~le;
+    const uint32_t end = after ? static_cast<uint32_t>(__builtin_ctzll(after)) : 64u;
+    return WaveRun{start, end - start, head};
+}
+
+// One thread per constraint, every lane of a wave reaching the loop body
+// (wave-uniform bounds), for the run aggregation.
 __global__ void solve_degree(Cons cs, const uint32_t *__restrict__ meta, const float *__restrict__ im,
                              uint32_t *__restrict__ deg) {
     const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
-    for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
-        uint32_t a, b;
-        cons_ends(cs, c, ncc, a, b);
-        if (im[a] != 0.f) atomicAdd(&deg[a], 1u);
-        if (im[b] != 0.f) atomicAdd(&deg[b], 1u);
+    for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
+        const uint32_t c = c0 + lane_id();
+        uint32_t a = kNone, b = kNone;
+        if (c < nc) {
+            cons_ends(cs, c, ncc, a, b);
+            if (im[a] == 0.f) a = kNone;
+            if (im[b] == 0.f) b = kNone;
+        }
+        const WaveRun r = wave_run(a);
+        if (r.head && a != kNone) atomicAdd(&deg[a], r.len);
+        if (b != kNone) atomicAdd(&deg[b], 1u);
     }
 }
 
@@ -199,15 +226,24 @@ __global__ void solve_fill(Cons cs, const uint32_t *__restrict__ meta, const flo
                            const uint32_t *__restrict__ off, uint32_t *__restrict__ fill, uint32_t *__restrict__ adj,
                            uint32_t *__restrict__ own) {
     const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
-    for (uint32_t c = grid_tid(); c < nc; c += grid_stride()) {
-        uint32_t a, b;
-        cons_ends(cs, c, ncc, a, b);
-        if (im[a] != 0.f) {
-            const uint32_t p = off[a] + atomicAdd(&fill[a], 1u);
+    for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
+        const uint32_t c = c0 + lane_id();
+        uint32_t a = kNone, b = kNone;
+        if (c < nc) {
+            cons_ends(cs, c, ncc, a, b);
+            if (im[a] == 0.f) a = kNone;
+            if (im[b] == 0.f) b = kNone;
+        }
+        const WaveRun r = wave_run(a);
+        uint32_t base = 0;
+        if (r.head && a != kNone) base = atomicAdd(&fill[a], r.len);
+        base = __shfl(base, static_cast<int>(r.start), 64);
+        if (a != kNone) {
+            const uint32_t p = off[a] + base + (lane_id() - r.start);
             adj[p] = c << 1;
             own[p] = a;
         }
-        if (im[b] != 0.f) {
+        if (b != kNone) {
             const uint32_t p = off[b] + atomicAdd(&fill[b], 1u);
             adj[p] = (c << 1) | 1u;
             own[p] = b;
@@ -258,6 +294,30 @@ __device__ __forceinline__ void wave_append(const bool (&pred)[N], const uint32_
 #pragma unroll
     for (int t = 0; t < N; ++t)
         if (pred[t]) out[base + pre[t] + static_cast<uint32_t>(__popcll(m[t] & lt))] = val[t];
+}
+
+// Workgroup-aggregated append of one value per thread: one atomic per
+// workgroup (the counter is shared by the whole grid, so fewer arrivals on
+// its address).  Must be reached by every thread of the workgroup.
+__device__ __forceinline__ void block_append(bool pred, uint32_t val, uint32_t *out, uint32_t *count) {
+    __shared__ uint32_t s_cnt[kSolveThreads / 64 + 1];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t m = __ballot(pred);
+    if (lane == 0) s_cnt[wave] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t c = s_cnt[w];
+            s_cnt[w] = total;
+            total += c;
+        }
+        s_cnt[nw] = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (pred) out[s_cnt[nw] + s_cnt[wave] + static_cast<uint32_t>(__popcll(m & lt))] = val;
+    __syncthreads();
 }
 
 // Per-body contact lists.  Every dynamic endpoint (contact c, end e) has an
@@ -335,13 +395,11 @@ __global__ __launch_bounds__(kSolveThreads) void list_link(Cons cs, const uint32
                                                            uint32_t *__restrict__ frontier, uint32_t *fcount,
                                                            uint64_t *__restrict__ queue) {
     const uint32_t nc = meta[kMNc], ncc = meta[kMNcc];
-    const uint32_t lane = lane_id();
-    for (uint32_t c0 = grid_tid() & ~63u; c0 < nc; c0 += grid_stride()) {
-        const uint32_t c = c0 + lane;
-        const bool pred[1] = {c < nc && link_contact(c, cs, ncc, im, off, deg, slot, crank, lrec, pending)};
-        const uint32_t val[1] = {c};
+    for (uint32_t c0 = blockIdx.x * kSolveThreads; c0 < nc; c0 += gridDim.x * kSolveThreads) {
+        const uint32_t c = c0 + threadIdx.x;
+        const bool head = c < nc && link_contact(c, cs, ncc, im, off, deg, slot, crank, lrec, pending);
         if (c < nc) queue[c] = ~0ull;
-        wave_append<1>(pred, val, frontier, fcount);
+        block_append(head, c, frontier, fcount);
     }
 }
 
@@ -764,7 +822,7 @@ struct CRec {
     float4 *p0, *p1;
 };
 
-__global__ __launch_bounds__(kSolveThreads) void class_gather(Cons cs, uint32_t *meta,
+__global__ __launch_bounds__(1024) void class_gather(Cons cs, uint32_t *meta,
                                                               const uint64_t *__restrict__ state,
                                                               const uint32_t *__restrict__ boff,
                                                               uint32_t *__restrict__ bfill, CRec rec,
@@ -1759,8 +1817,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
 
     // Class-sorted constraint records.
     CRec rec{x->rec_ab, x->rec_p0, x->rec_p1};
-    class_gather<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, x->state, x->boff, x->bfill, rec, x->rec_id,
-                                                            x->cpos, x->rseq);
+    // workgroups of 1 024: one class-count atomic per class present per 1 024 constraints
+    class_gather<<<ceil_div(grid_for(ncap), 4u), 1024, 0, st>>>(cs, x->meta, x->state, x->boff, x->bfill, rec,
+                                                                 x->rec_id, x->cpos, x->rseq);
     DY_LAUNCH_CHECK();
     const int *flow_max = x->sweep_mode == 3 ? x->coop_lanes_max : x->coop_flow_max;
     const bool flow = coop && (x->sweep_mode == 2 || x->sweep_mode == 3) && flow_max[J] > 0;
