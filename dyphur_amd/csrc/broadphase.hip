@@ -1121,7 +1121,7 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     if (n > bp->max_bodies || n >= kMortonMax) return DY_ERR_INVALID_ARG;
     hipStream_t st = hs(s);
     if (bp->timing) DY_TRY_STATUS(bp_collect_timing(bp));
-    DY_TRY(hipMemsetAsync(bp->counters + 2, 0, 6 * sizeof(uint32_t), st));
+    DY_TRY_STATUS(clear_words(st, bp->counters + 2, 6));
     if (n < 2) return DY_OK;
     const bool tm = bp->timing;
     if (tm) DY_TRY(hipEventRecord(bp->ev[0], st));

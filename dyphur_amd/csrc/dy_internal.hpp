@@ -3,6 +3,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 #include <stdint.h>
 
 #include "../../include/dyphur_hip.h"
@@ -33,13 +35,29 @@
 
 #define DY_LAUNCH_CHECK() DY_TRY(hipGetLastError())
 
-// A cooperative launch that may be refused: another process that has made
-// cooperative launches on the device holds them (hipErrorIllegalState), or
-// the grid does not fit.  Nothing is enqueued then; the runtime's last error
-// is cleared so the refusal does not resurface at the next launch check, and
-// the caller takes its non-cooperative path.
+// Launch of a kernel whose workgroups wait for each other (grid barriers,
+// XCD check-ins, dataflow hand-offs).  Every caller sizes the grid to what the
+// device holds at once (occupancy x CUs), so on a device that runs only this
+// stream's work an ordinary launch makes the whole grid resident: the
+// stream's previous kernel has finished and the dispatcher places every
+// workgroup.  That is the default, because the runtime's cooperative path
+// leaves the device idle for about 10 us before and after each such kernel.
+// The guarantee itself needs the cooperative path: with DYPHUR_COOP_LAUNCH=1
+// (callers that run other work on the device at the same time) the launch is
+// cooperative, and may be refused: another process that has made cooperative
+// launches on the device holds them (hipErrorIllegalState), or the grid does
+// not fit.  Nothing is enqueued then; the runtime's last error is cleared so
+// the refusal does not resurface at the next launch check, and the caller
+// takes its non-cooperative path.  A grid kept from residency by other work
+// for longer than the kernels' spin limits ends with the kernel's error flag
+// set, not in a hang.
+inline bool coop_launch_required() {
+    const char *e = std::getenv("DYPHUR_COOP_LAUNCH");
+    return e && std::atoi(e) != 0;
+}
 inline hipError_t try_coop_launch(const void *fn, dim3 grid, dim3 block, void **args, hipStream_t st) {
-    const hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, st);
+    const hipError_t e = coop_launch_required() ? hipLaunchCooperativeKernel(fn, grid, block, args, 0, st)
+                                                : hipLaunchKernel(fn, grid, block, args, 0, st);
     if (e != hipSuccess) (void)hipGetLastError();
     return e;
 }
@@ -74,6 +92,10 @@ struct DevBuf {
 
 // Primitives implemented in primitives.hip, used by the stage translation units.
 // Exclusive scan of n u32 values; d_total receives the sum (may be null).
+// Zeroes n words with one small kernel (a hipMemsetAsync of a few words can
+// take two fill dispatches, and cannot be recorded into a graph).
+int clear_words(hipStream_t s, uint32_t *p, uint32_t n);
+
 // `tmp` must hold at least scan_tmp_words(n) u32.
 size_t scan_tmp_words(size_t n);
 int scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, uint32_t *d_total,

@@ -138,7 +138,7 @@ uint32_t np_grid(uint32_t n_bound) {
 int np_enqueue(hipStream_t st, const dy_contact_pair *pairs, uint32_t n_bound, const uint32_t *d_n,
                const dy_body_view &bv, const dy_shape_view &sv, uint64_t *status, uint32_t epoch, NpOut out) {
     if (epoch == 0) return DY_ERR_INVALID_ARG;
-    DY_TRY(hipMemsetAsync(out.counts, 0, kNpCounts * sizeof(uint32_t), st));
+    DY_TRY_STATUS(clear_words(st, out.counts, kNpCounts));
     if (n_bound == 0) return DY_OK;
     np_fused<<<np_grid(n_bound), kNpThreads, 0, st>>>(pairs, n_bound, d_n, bv, sv, status, epoch, out);
     DY_LAUNCH_CHECK();

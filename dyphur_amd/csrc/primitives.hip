@@ -114,6 +114,17 @@ __global__ __launch_bounds__(kMbThreads) void scan_tiles(const uint32_t *__restr
     }
 }
 
+__global__ __launch_bounds__(64) void clear_words_kernel(uint32_t *__restrict__ p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += 64u) p[i] = 0u;
+}
+
+int clear_words(hipStream_t s, uint32_t *p, uint32_t n) {
+    if (n == 0) return DY_OK;
+    clear_words_kernel<<<1, 64, 0, s>>>(p, n);
+    DY_LAUNCH_CHECK();
+    return DY_OK;
+}
+
 size_t scan_tmp_words(size_t n) {
     const size_t nb = (n + kMbTile - 1) / kMbTile;
     return 2 * nb + 2;

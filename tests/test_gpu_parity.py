@@ -226,6 +226,22 @@ def test_coloured_run_to_run_identical(gpu):
     assert hs[0] == hs[1]
 
 
+def test_cooperative_launch_path_identical(gpu, monkeypatch):
+    """The grid-synchronised kernels (Morton sort, class assignment, sweep)
+    give the same frames launched as ordinary launches (the default) and
+    through the runtime's cooperative path (DYPHUR_COOP_LAUNCH=1)."""
+    sc = scenes.random_mixed(12000, seed=11, extent=(24, 14, 24))
+    hs = []
+    for coop in ("0", "1"):
+        monkeypatch.setenv("DYPHUR_COOP_LAUNCH", coop)
+        print(f"[launch path] cooperative={coop}", flush=True)
+        w = gpu.World(sc)
+        for _ in range(30):
+            w.step()
+        hs.append(w.fnv())
+    assert hs[0] == hs[1]
+
+
 def test_colour_overflow_falls_back_to_serial(gpu, oracle):
     """301 contacts sharing one dynamic plate: JP runs out of its 64 colours and
     the reference solves in contact order; the GPU must take the same path."""
