@@ -95,16 +95,38 @@ __device__ __forceinline__ uint32_t grid_tid() { return blockIdx.x * blockDim.x 
 // Zeroes the per-solve state and publishes the constraint counts and the
 // path: the reference runs its constraints serially at <= 256 constraints
 // (contacts + joints) or when forced (xpbd_solver.cpp:611-619).
-__global__ void solve_prep(uint32_t nb, const uint32_t *__restrict__ ncp, uint32_t cap, uint32_t nj,
+// AoS body records: dyn = (p.xyz, q.w) (q.xyz, v.x) (v.yz, w.xy) (w.z, renorms, -, -);
+// sta = (inv_mass, Ixx, Iyy, Izz) (Ixy, Ixz, Iyz, -).
+__device__ __forceinline__ void gather_body(const dy_body_view &bv, uint32_t i, float4 *__restrict__ dyn,
+                                            float4 *__restrict__ sta) {
+    dyn[4 * i + 0] = make_float4(bv.pos_x[i], bv.pos_y[i], bv.pos_z[i], bv.rot_w[i]);
+    dyn[4 * i + 1] = make_float4(bv.rot_x[i], bv.rot_y[i], bv.rot_z[i], bv.vel_x[i]);
+    dyn[4 * i + 2] = make_float4(bv.vel_y[i], bv.vel_z[i], bv.ang_x[i], bv.ang_y[i]);
+    dyn[4 * i + 3] = make_float4(bv.ang_z[i], __uint_as_float(0u), 0.f, 0.f);
+    sta[2 * i + 0] = make_float4(bv.inv_mass[i], bv.iI_xx[i], bv.iI_yy[i], bv.iI_zz[i]);
+    sta[2 * i + 1] = make_float4(bv.iI_xy[i], bv.iI_xz[i], bv.iI_yz[i], 0.f);
+}
+
+// Per-solve reset (per-body counters and masks, per-constraint lambdas, the
+// metadata) and the bodies' AoS records.
+__global__ void solve_prep(dy_body_view bv, const uint32_t *__restrict__ ncp, uint32_t cap, uint32_t nj,
                            int force_serial, uint32_t *__restrict__ deg, uint32_t *__restrict__ fill,
                            uint64_t *__restrict__ used, uint32_t *__restrict__ ver, float *__restrict__ lam,
-                           uint32_t *__restrict__ hist, uint32_t *__restrict__ meta) {
+                           uint32_t *__restrict__ hist, uint32_t *__restrict__ meta, float4 *__restrict__ dyn,
+                           float4 *__restrict__ sta) {
+    const uint32_t nb = bv.n;
     const uint32_t n_raw = *ncp;
     const uint32_t nc = n_raw < cap ? n_raw : cap;
     const uint32_t ncon = nc + nj;
     const uint32_t top = nb > ncon ? nb : ncon;
     for (uint32_t i = grid_tid(); i < top; i += grid_stride()) {
-        if (i < nb) { deg[i] = 0; fill[i] = 0; used[i] = 0; ver[i] = 0; }
+        if (i < nb) {
+            deg[i] = 0;
+            fill[i] = 0;
+            used[i] = 0;
+            ver[i] = 0;
+            gather_body(bv, i, dyn, sta);
+        }
         if (i < ncon) lam[i] = 0.f;
     }
     const uint32_t i = grid_tid();
@@ -878,19 +900,6 @@ __global__ __launch_bounds__(1024) void class_gather(Cons cs, uint32_t *meta,
     }
 }
 
-// AoS body records: dyn = (p.xyz, q.w) (q.xyz, v.x) (v.yz, w.xy) (w.z, renorms, -, -);
-// sta = (inv_mass, Ixx, Iyy, Izz) (Ixy, Ixz, Iyz, -).
-__global__ void gather_bodies(dy_body_view bv, float4 *__restrict__ dyn, float4 *__restrict__ sta) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= bv.n) return;
-    dyn[4 * i + 0] = make_float4(bv.pos_x[i], bv.pos_y[i], bv.pos_z[i], bv.rot_w[i]);
-    dyn[4 * i + 1] = make_float4(bv.rot_x[i], bv.rot_y[i], bv.rot_z[i], bv.vel_x[i]);
-    dyn[4 * i + 2] = make_float4(bv.vel_y[i], bv.vel_z[i], bv.ang_x[i], bv.ang_y[i]);
-    dyn[4 * i + 3] = make_float4(bv.ang_z[i], __uint_as_float(0u), 0.f, 0.f);
-    sta[2 * i + 0] = make_float4(bv.inv_mass[i], bv.iI_xx[i], bv.iI_yy[i], bv.iI_zz[i]);
-    sta[2 * i + 1] = make_float4(bv.iI_xy[i], bv.iI_xz[i], bv.iI_yz[i], 0.f);
-}
-
 __device__ __forceinline__ BodyState load_aos(const float4 *__restrict__ dyn, const float4 *__restrict__ sta,
                                               uint32_t i, uint32_t &renorms) {
     const float4 d0 = dyn[4 * i], d1 = dyn[4 * i + 1], d2 = dyn[4 * i + 2], d3 = dyn[4 * i + 3];
@@ -1416,7 +1425,8 @@ struct dy_xpbd_s {
     uint32_t *adj = nullptr, *own = nullptr, *slot = nullptr, *crank = nullptr, *pending = nullptr;
     uint32_t *fr0 = nullptr, *fr1 = nullptr;
     uint32_t *boff = nullptr, *bfill = nullptr, *hist = nullptr, *rec_id = nullptr, *cpos = nullptr;
-    uint32_t *meta = nullptr;  // kMWords device words
+    uint32_t *meta = nullptr;  // kMWords device words, inside meta_block
+    uint8_t *meta_block = nullptr;
     uint4 *lrec = nullptr, *rseq = nullptr;
     uint2 *rec_ab = nullptr;
     uint64_t *fq = nullptr;
@@ -1567,11 +1577,58 @@ void xp_free(dy_xpbd_s *x) {
     if (x->ev_meta) (void)hipEventDestroy(x->ev_meta);
     if (x->h_meta) (void)hipHostFree(x->h_meta);
     free_constraint_bufs(x);
-    void *ptrs[] = {x->meta, x->jpk, x->jend};
+    void *ptrs[] = {x->meta_block, x->jpk, x->jend};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     x->deg.release(); x->off.release(); x->fill.release(); x->scan_tmp.release(); x->ver.release();
     x->used.release(); x->dyn.release(); x->sta.release();
+}
+
+// Placement of the metadata words.  Device atomics are served beyond the
+// XCD's L2 and their latency depends on the address: of two lines 8 KiB
+// apart, one is typically served about a third faster than the other from a
+// given XCD, and which one depends on the physical page.  The assignment's
+// queue counters (kMQHead, kMQTail, kMQDone) take an atomic per wave and
+// claim, so the metadata goes into whichever half of a 16 KiB block answers
+// faster from the assignment's XCD, measured once at creation by chains of
+// dependent atomics on each half's queue-counter line.
+constexpr uint32_t kMetaHalf = 8192u, kMetaBlock = 2u * kMetaHalf;
+constexpr uint32_t kMetaProbeChain = 64u;
+static_assert(kMWords * sizeof(uint32_t) <= kMetaHalf, "metadata fits half the block");
+
+__global__ void meta_place_probe(uint32_t *block, uint32_t *elect, uint64_t *ticks) {
+    if (threadIdx.x != 0 || xcc_id() != kSweepXcc) return;
+    if (atomicCAS(elect, 0u, 1u) != 0u) return;  // one workgroup on that XCD measures
+    for (uint32_t h = 0; h < 2u; ++h) {
+        uint32_t *p = block + h * (kMetaHalf / 4u) + kMQHead;
+        uint32_t v = 0;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        for (uint32_t i = 0; i < kMetaProbeChain; ++i) v = atomicAdd(p + (v >> 31), 1u);  // v < 2^31: dependent chain
+        ticks[h] = __builtin_amdgcn_s_memrealtime() - t0 + (v >> 31);
+    }
+}
+
+// Chooses x->meta inside x->meta_block (allocated, kMetaBlock bytes) and
+// zeroes the block.  Synchronous (creation only).
+hipError_t place_meta(dy_xpbd_s *x, hipStream_t st) {
+    uint32_t *scratch = nullptr;  // [0] election, [2..5] two 64-bit tick counts
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&scratch), 8 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(x->meta_block, 0, kMetaBlock, st);
+    if (e == hipSuccess) e = hipMemsetAsync(scratch, 0, 8 * sizeof(uint32_t), st);
+    uint64_t ticks[2] = {0, 0};
+    if (e == hipSuccess) {
+        meta_place_probe<<<8, 64, 0, st>>>(reinterpret_cast<uint32_t *>(x->meta_block), scratch,
+                                            reinterpret_cast<uint64_t *>(scratch + 2));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(ticks, scratch + 2, sizeof(ticks), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (scratch) (void)hipFree(scratch);
+    // the second half only when it measured clearly faster (both zero: no measurement)
+    const bool second = ticks[0] > 0 && ticks[1] > 0 && ticks[1] * 10u < ticks[0] * 9u;
+    x->meta = reinterpret_cast<uint32_t *>(x->meta_block + (second ? kMetaHalf : 0u));
+    if (e == hipSuccess) e = hipMemsetAsync(x->meta_block, 0, kMetaBlock, st);
+    return e;
 }
 
 int env_int(const char *name, int dflt) {
@@ -1702,10 +1759,10 @@ int dy_xpbd_create(dy_stream_t s, int n_iters, uint32_t max_contacts, float fric
     x->mu = friction;
     x->cap = max_contacts;
     hipError_t e = alloc_constraint_bufs(x, max_contacts);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta), kMWords * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x->meta_block), kMetaBlock);
+    if (e == hipSuccess) e = place_meta(x, hs(s));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&x->h_meta), kMWords * sizeof(uint32_t));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&x->ev_meta, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMemsetAsync(x->meta, 0, kMWords * sizeof(uint32_t), hs(s));
     if (e == hipSuccess) e = hipStreamSynchronize(hs(s));
     if (e != hipSuccess) {
         xp_free(x);
@@ -1765,8 +1822,9 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
     x->last_nb = nb;
     uint32_t top = nb > ncap ? nb : ncap;
     if (top < kMWords + kMaxColours + 1u) top = kMWords + kMaxColours + 1u;
-    solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(nb, cv->n, cap, nj, x->force_serial, x->deg.p, x->fill.p,
-                                                         x->used.p, x->ver.p, x->lam, x->hist, x->meta);
+    solve_prep<<<grid_for(top), kSolveThreads, 0, st>>>(*bv, cv->n, cap, nj, x->force_serial, x->deg.p, x->fill.p,
+                                                         x->used.p, x->ver.p, x->lam, x->hist, x->meta, x->dyn.p,
+                                                         x->sta.p);
     DY_LAUNCH_CHECK();
     if (nj) {
         joint_pack<<<ceil_div(nj, 256), 256, 0, st>>>(*jv, nb, x->jpk, x->jend, x->meta);
@@ -1780,8 +1838,6 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
     DY_TRY_STATUS(scan_exclusive(st, x->deg.p, x->off.p, nb, x->meta + kMDyn, x->scan_tmp.p));
     solve_fill<<<grid_for(ncap), kSolveThreads, 0, st>>>(cs, x->meta, bv->inv_mass, x->off.p, x->fill.p, x->adj,
                                                           x->own);
-    DY_LAUNCH_CHECK();
-    gather_bodies<<<ceil_div(nb, 256), 256, 0, st>>>(*bv, x->dyn.p, x->sta.p);
     DY_LAUNCH_CHECK();
     list_rank<<<grid_for(2u * ncap), kSolveThreads, 0, st>>>(x->adj, x->own, x->off.p, x->deg.p, x->meta, x->slot,
                                                               x->crank);
