@@ -357,9 +357,16 @@ __device__ __forceinline__ void rank_entry(uint32_t p, const uint32_t *__restric
     const uint32_t beg = off[x], n = deg[x];
     const uint32_t self = p - beg;
     uint32_t rank = 0;
-    for (uint32_t q = 0; q < n; ++q) {
-        const uint64_t kq = order_key(adj[beg + q] >> 1, serial);
-        rank += (kq > kc || (kq == kc && q < self)) ? 1u : 0u;  // ties (a == b) by position
+    for (uint32_t q0 = 0; q0 < n; q0 += 8u) {  // eight entries' loads in flight at a time
+        uint32_t e[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) e[k] = q0 + k < n ? adj[beg + q0 + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t q = q0 + k;
+            const uint64_t kq = order_key(e[k] >> 1, serial);
+            rank += (q < n && (kq > kc || (kq == kc && q < self))) ? 1u : 0u;  // ties (a == b) by position
+        }
     }
     slot[beg + rank] = c;
     crank[2u * c + (v & 1u)] = rank;
@@ -1225,7 +1232,16 @@ __global__ __launch_bounds__(kSolveThreads) void flow_seq(const uint32_t *__rest
         const uint32_t kc = cls_of(state[c]);
         const uint32_t beg = off[x], n = deg[x];
         uint32_t rank = 0;
-        for (uint32_t q = 0; q < n; ++q) rank += cls_of(state[adj[beg + q] >> 1]) < kc ? 1u : 0u;
+        for (uint32_t q0 = 0; q0 < n; q0 += 8u) {  // eight entries' loads in flight at a time
+            uint32_t e[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) e[k] = q0 + k < n ? adj[beg + q0 + k] : 0u;
+            uint64_t sk[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) sk[k] = q0 + k < n ? state[e[k] >> 1] : 0ull;
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) rank += (q0 + k < n && cls_of(sk[k]) < kc) ? 1u : 0u;
+        }
         uint32_t *r = reinterpret_cast<uint32_t *>(&rseq[cpos[c]]);
         r[2u * (v & 1u)] = rank;
         r[2u * (v & 1u) + 1u] = n;
