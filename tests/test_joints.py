@@ -504,13 +504,15 @@ print(w.fnv())
 
 
 @pytest.mark.gpu
-def test_gpu_second_process_takes_the_launch_per_round_path(gpu, tmp_path):
+def test_gpu_second_process_takes_the_launch_per_round_path(gpu, tmp_path, monkeypatch):
     """While this process holds the device's cooperative launches, another
     process's are refused; its solver then launches per round and per class
-    and gives the same bits."""
+    and gives the same bits.  Both processes ask for cooperative launches
+    (DYPHUR_COOP_LAUNCH=1; the default launches are ordinary ones)."""
     import os
     import subprocess
     import sys
+    monkeypatch.setenv("DYPHUR_COOP_LAUNCH", "1")
     w = gpu.World(scenes.joint_chains(40, 8, seed=7))
     for _ in range(8):
         w.step()
