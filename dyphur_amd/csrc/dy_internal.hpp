@@ -242,21 +242,24 @@ __device__ __forceinline__ bool xcd_barrier(uint32_t *bar, uint32_t &gen, uint32
     return s_ok != 0;
 }
 
-// Check-in of a cooperative launch that works on XCD `xcc` only: every
-// workgroup arrives on reg[1] (agent scope); the ones on that XCD take dense
-// indices from reg[0] and wait until all have arrived.  Returns true for a
-// participant, with its index and the participant count.
+// Check-in of a launch that works on XCD `xcc` only: every workgroup arrives
+// on reg[1] (agent scope); the ones on that XCD take dense indices from
+// reg[0] and wait until all have arrived.  Returns true for a participant,
+// with its index and the participant count.  The check-in publishes no data
+// (participants read what earlier kernels wrote, visible at the kernel
+// boundary, and hand data to each other through the XCD's L2), so its atomics
+// and polls are relaxed: no L2 write-back or invalidation.
 __device__ __forceinline__ bool xcd_checkin(uint32_t xcc, uint32_t *reg, uint32_t *err, uint32_t err_bit,
                                             uint32_t &idx, uint32_t &n) {
     __shared__ uint32_t s_idx, s_n;
     const bool mine = xcc_id() == xcc;
     if (threadIdx.x == 0) {
         s_idx = mine ? __hip_atomic_fetch_add(&reg[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        __hip_atomic_fetch_add(&reg[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&reg[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_n = 0;
         if (mine) {
             uint32_t spins = 0;
-            while (__hip_atomic_load(&reg[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gridDim.x) {
+            while (__hip_atomic_load(&reg[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gridDim.x) {
                 __builtin_amdgcn_s_sleep(2);
                 if (++spins > kGridSpinLimit) {
                     atomicOr(err, err_bit);

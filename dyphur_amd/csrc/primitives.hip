@@ -62,6 +62,53 @@ __global__ __launch_bounds__(kScanThreads) void scan_one_block(const uint32_t *_
     if (t == 0 && total) *total = carry;
 }
 
+// One workgroup, values staged in LDS (one pad word per 32, so that a
+// thread's contiguous run falls on distinct banks): thread t scans the run
+// [t*K, t*K + K) serially, one workgroup scan of the 1 024 run sums gives each
+// run's base — four workgroup barriers in all, against several per 8 192-value
+// tile in scan_one_block.  Up to kScanLdsMax values.
+constexpr uint32_t kScanLdsMax = kScanThreads * 36u;  // 36 864 values, 148.5 KiB with padding
+
+__global__ __launch_bounds__(kScanThreads) void scan_lds(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                         uint32_t n, uint32_t *__restrict__ total) {
+    __shared__ uint32_t buf[kScanLdsMax + kScanLdsMax / 32u];
+    __shared__ uint32_t wsum[17];
+    const uint32_t t = threadIdx.x;
+    const uint32_t k_run = (n + kScanThreads - 1u) / kScanThreads;  // values per thread (<= 36)
+    const uint32_t padded = k_run * kScanThreads;
+    for (uint32_t i0 = 0; i0 < padded; i0 += 8u * kScanThreads) {  // coalesced, eight loads in flight
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t i = i0 + k * kScanThreads + t;
+            v[k] = i < n ? in[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 8u; ++k) {
+            const uint32_t i = i0 + k * kScanThreads + t;
+            if (i < padded) buf[i + (i >> 5)] = v[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t r0 = t * k_run;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < k_run; ++k) {
+        const uint32_t i = r0 + k;
+        sum += buf[i + (i >> 5)];
+    }
+    uint32_t all = 0;
+    uint32_t run = block_exclusive_scan(sum, wsum, all);
+    for (uint32_t k = 0; k < k_run; ++k) {
+        const uint32_t i = r0 + k, j = i + (i >> 5);
+        const uint32_t v = buf[j];
+        buf[j] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < n; i += kScanThreads) out[i] = buf[i + (i >> 5)];
+    if (t == 0 && total) *total = all;
+}
+
 constexpr uint32_t kMbThreads = 256;
 constexpr uint32_t kMbItems = 8;
 constexpr uint32_t kMbTile = kMbThreads * kMbItems;
@@ -137,6 +184,11 @@ int scan_exclusive(hipStream_t s, const uint32_t *in, uint32_t *out, size_t n, u
         return DY_OK;
     }
     if (n > 0xffffffffu) return DY_ERR_INVALID_ARG;
+    if (n <= kScanLdsMax) {
+        scan_lds<<<1, kScanThreads, 0, s>>>(in, out, static_cast<uint32_t>(n), d_total);
+        DY_LAUNCH_CHECK();
+        return DY_OK;
+    }
     if (n <= kScanSingleMax) {
         scan_one_block<<<1, kScanThreads, 0, s>>>(in, out, static_cast<uint32_t>(n), d_total);
         DY_LAUNCH_CHECK();

@@ -226,6 +226,27 @@ def test_coloured_run_to_run_identical(gpu):
     assert hs[0] == hs[1]
 
 
+@pytest.mark.parametrize("n", [1, 63, 1024, 4097, 32769, 36864, 36865, 65536, 65537, 200003])
+def test_exclusive_scan_sizes(gpu, n):
+    """dy_scan_exclusive_u32 over every path (LDS-staged workgroup up to
+    36 864 values, tiled workgroup up to 65 536, three launches beyond): the
+    exclusive prefix sums and the total, exact (values small enough that no
+    sum wraps)."""
+    print(f"[scan] n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 64, size=n, dtype=np.uint32)
+    s = gpu.Stream()
+    din = gpu.DeviceArray.from_numpy(a, s)
+    dout = gpu.DeviceArray(n, np.uint32)
+    dtot = gpu.DeviceArray.from_numpy(np.array([0xDEADBEEF], np.uint32), s)
+    gpu.check(gpu.lib().dy_scan_exclusive_u32(s.h, gpu.vp(din.ptr), gpu.vp(dout.ptr), n, gpu.vp(dtot.ptr)),
+              "dy_scan_exclusive_u32")
+    got = dout.download(s)
+    ref = np.concatenate([[0], np.cumsum(a, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    assert np.array_equal(got, ref)
+    assert int(dtot.download(s)[0]) == int(a.sum(dtype=np.uint64))
+
+
 def test_cooperative_launch_path_identical(gpu, monkeypatch):
     """The grid-synchronised kernels (Morton sort, class assignment, sweep)
     give the same frames launched as ordinary launches (the default) and
