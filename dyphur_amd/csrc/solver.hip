@@ -1328,6 +1328,181 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
     }
 }
 
+// One contact projection by a pair of lanes (project_contact, dy_math.hpp):
+// the lane of side 0 holds body A, the lane of side 1 body B, each computes
+// its own body's terms and updates its own body.  The sums project_contact
+// forms left to right (w, the relative velocities, wt) are formed on both
+// lanes in the same order from exchanged terms, so every value is bit-equal
+// to project_contact's.  Every branch around an exchange depends on shared
+// values only, so the two lanes always take it together.
+__device__ __forceinline__ float pair_xchg(float v) { return __shfl_xor(v, 1, 64); }
+
+__device__ __forceinline__ void project_contact_pair(const ContactIn &c, BodyState &S, uint32_t side, float mu,
+                                                     float &lam, float dt) {
+    if (c.depth <= 0.f) return;
+    const float rem = c.depth - lam;
+    if (rem <= 1e-6f) return;
+    const float nx = c.nx, ny = c.ny, nz = c.nz;
+    const float ws = S.im, wo = pair_xchg(ws);
+    const float wa = side ? wo : ws, wb = side ? ws : wo;
+    const float rx = c.px - S.px, ry = c.py - S.py, rz = c.pz - S.pz;
+    const float cx = ry * nz - rz * ny, cy = rz * nx - rx * nz, cz = rx * ny - ry * nx;
+    float kx, ky, kz;
+    inv_inertia_world(S, cx, cy, cz, kx, ky, kz);
+    const float ang_s = cx * kx + cy * ky + cz * kz, ang_o = pair_xchg(ang_s);
+    const float wa_ang = side ? ang_o : ang_s, wb_ang = side ? ang_s : ang_o;
+    const float w = wa + wb + wa_ang + wb_ang;
+    if (w < 1e-10f) return;
+    const float dl = rem / w;
+    lam += rem;
+    if (side == 0u) {
+        if (wa != 0.f) { S.px += wa * dl * nx; S.py += wa * dl * ny; S.pz += wa * dl * nz; }
+        rotate_by(S, dl * kx, dl * ky, dl * kz);
+    } else {
+        if (wb != 0.f) { S.px -= wb * dl * nx; S.py -= wb * dl * ny; S.pz -= wb * dl * nz; }
+        rotate_by(S, -dl * kx, -dl * ky, -dl * kz);
+    }
+    {
+        const float vsx = S.vx + S.wy * rz - S.wz * ry;
+        const float vsy = S.vy + S.wz * rx - S.wx * rz;
+        const float vsz = S.vz + S.wx * ry - S.wy * rx;
+        const float vox = pair_xchg(vsx), voy = pair_xchg(vsy), voz = pair_xchg(vsz);
+        const float vax = side ? vox : vsx, vay = side ? voy : vsy, vaz = side ? voz : vsz;
+        const float vbx = side ? vsx : vox, vby = side ? vsy : voy, vbz = side ? vsz : voz;
+        const float qx = vax - vbx, qy = vay - vby, qz = vaz - vbz;
+        const float vn = qx * nx + qy * ny + qz * nz;
+        if (vn < 0.f) {
+            const float jn = -vn / w;
+            float gx, gy, gz;
+            inv_inertia_world(S, cx, cy, cz, gx, gy, gz);
+            if (ws != 0.f) {
+                if (side == 0u) {
+                    S.vx += wa * jn * nx; S.vy += wa * jn * ny; S.vz += wa * jn * nz;
+                    S.wx += jn * gx; S.wy += jn * gy; S.wz += jn * gz;
+                } else {
+                    S.vx -= wb * jn * nx; S.vy -= wb * jn * ny; S.vz -= wb * jn * nz;
+                    S.wx -= jn * gx; S.wy -= jn * gy; S.wz -= jn * gz;
+                }
+            }
+        }
+    }
+    if (mu > 0.f) {
+        const float vsx = S.vx + S.wy * rz - S.wz * ry;
+        const float vsy = S.vy + S.wz * rx - S.wx * rz;
+        const float vsz = S.vz + S.wx * ry - S.wy * rx;
+        const float vox = pair_xchg(vsx), voy = pair_xchg(vsy), voz = pair_xchg(vsz);
+        const float vax = side ? vox : vsx, vay = side ? voy : vsy, vaz = side ? voz : vsz;
+        const float vbx = side ? vsx : vox, vby = side ? vsy : voy, vbz = side ? vsz : voz;
+        const float ux = vax - vbx, uy = vay - vby, uz = vaz - vbz;
+        const float un = ux * nx + uy * ny + uz * nz;
+        const float tx0 = ux - un * nx, ty0 = uy - un * ny, tz0 = uz - un * nz;
+        const float t2 = tx0 * tx0 + ty0 * ty0 + tz0 * tz0;
+        if (t2 > 1e-12f) {
+            const float tl = sqrtf(t2);
+            const float tx = tx0 / tl, ty = ty0 / tl, tz = tz0 / tl;
+            const float dx = ry * tz - rz * ty, dy = rz * tx - rx * tz, dz = rx * ty - ry * tx;
+            float fx, fy, fz;
+            inv_inertia_world(S, dx, dy, dz, fx, fy, fz);
+            const float p0 = dx * fx, p1 = dy * fy, p2 = dz * fz;
+            const float o0 = pair_xchg(p0), o1 = pair_xchg(p1), o2 = pair_xchg(p2);
+            const float a0 = side ? o0 : p0, a1 = side ? o1 : p1, a2 = side ? o2 : p2;
+            const float b0 = side ? p0 : o0, b1 = side ? p1 : o1, b2 = side ? p2 : o2;
+            const float wt = wa + wb + a0 + a1 + a2 + b0 + b1 + b2;
+            if (wt > 1e-10f) {
+                const float jcap = lam / (dt * w);
+                float jt = tl / wt;
+                jt = fminf(jt, mu * jcap);
+                if (ws != 0.f) {
+                    if (side == 0u) {
+                        S.vx -= wa * jt * tx; S.vy -= wa * jt * ty; S.vz -= wa * jt * tz;
+                        S.wx -= jt * fx; S.wy -= jt * fy; S.wz -= jt * fz;
+                    } else {
+                        S.vx += wb * jt * tx; S.vy += wb * jt * ty; S.vz += wb * jt * tz;
+                        S.wx += jt * fx; S.wy += jt * fy; S.wz += jt * fz;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Contact-only dataflow sweep with a pair of lanes per record (lanes 2k and
+// 2k + 1 of a wave take record i0 + k, one body each, project_contact_pair):
+// each lane polls, loads, renormalises, stores and publishes its own body,
+// so a pass of the projection is about half as long.  Otherwise sweep_flow.
+__global__ __launch_bounds__(1024) void sweep_flow_pair(uint32_t n_iters, CRec rec, const uint4 *__restrict__ rseq,
+                                                        float *__restrict__ lam, float4 *__restrict__ dyn,
+                                                        const float4 *__restrict__ sta, uint32_t *ver, float mu,
+                                                        float dt, uint32_t *meta,
+                                                        const float4 *__restrict__ /* joints: none */) {
+    uint32_t *err = meta + kMErr;
+    uint32_t bidx = 0, nblk = 0;
+    if (!xcd_checkin(kSweepXcc, meta + kMRegSweep, err, kErrBarrier, bidx, nblk)) return;
+    if (meta[kMErr] & kErrAssign) return;
+    const uint32_t nc = meta[kMNc];
+    const uint32_t rstride = nblk * blockDim.x / 2u;  // records per round over all waves
+    const uint32_t lane = lane_id(), side = lane & 1u;
+    const uint32_t wave_rec0 = (bidx * blockDim.x + (threadIdx.x & ~63u)) / 2u;
+    uint32_t gen = 0;
+    for (uint32_t it = 0; it < n_iters; ++it) {
+        if (it == 1) {
+            if (!xcd_barrier(meta + kMBarSweep, gen, nblk, err, kErrBarrier)) return;
+            if (ld_agent(&meta[kMLive]) == 0u) return;
+        }
+        for (uint32_t r0 = wave_rec0; r0 < nc; r0 += rstride) {
+            const uint32_t i = r0 + (lane >> 1);
+            bool todo = i < nc;  // the same on both lanes of a pair
+            uint2 ab = make_uint2(0u, 0u);
+            uint4 sq = make_uint4(0u, 0u, 0u, 0u);
+            float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+            float l = 0.f;
+            if (todo) {
+                ab = rec.ab[i];
+                sq = rseq[i];
+                p0 = rec.p0[i];
+                p1 = rec.p1[i];
+                l = lam[i];
+            }
+            const uint32_t x = side ? ab.y : ab.x;             // this lane's body
+            const bool dyn_end = (side ? sq.w : sq.y) != 0u;    // a dynamic end: it waits and publishes
+            const uint32_t t = side ? it * sq.w + sq.z : it * sq.y + sq.x;
+            const bool live = contact_live(p1.z, l);
+            bool published = false;
+            uint32_t spins = 0;
+            while (__ballot(todo)) {
+                const bool mine = todo && (!dyn_end || ld_agent(&ver[x]) == t);
+                // the exchange runs on every lane: a lane inside a short-circuit would read an inactive partner
+                const int partner = __shfl_xor(mine ? 1 : 0, 1, 64);
+                const bool ready = mine && partner != 0;
+                if (ready && live) {
+                    uint32_t rs;
+                    BodyState S = load_aos_agent(dyn, sta, x, rs);
+                    if (S.im != 0.f) catch_up(S, rs, it);
+                    const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+                    project_contact_pair(ci, S, side, mu, l, dt);
+                    if (side == 0u) lam[i] = l;
+                    if (S.im != 0.f) store_aos(dyn, x, S, it);
+                    if (side == 0u && it == 0 && contact_live(p1.z, l)) atomicOr(&meta[kMLive], 1u);
+                }
+                if (ready) todo = false;
+                wait_vmem();  // body stores acknowledged by the L2 before the versions move
+                if (!todo && i < nc && !published) {
+                    if (dyn_end) __hip_atomic_store(&ver[x], t + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    published = true;
+                }
+                if (__ballot(todo)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kGridSpinLimit) {
+                        if (lane == 0) atomicOr(err, kErrFlow);
+                        return;
+                    }
+                    if ((spins & 255u) == 0u && (l2_read(err) & kErrFlow)) return;
+                }
+            }
+        }
+    }
+}
+
 // Dataflow sweep with a record cursor per lane: within an iteration a lane
 // moves on to its next record (stride = participating threads) as soon as
 // its current one is done, so the lanes of a wave do not wait for each
@@ -1460,7 +1635,9 @@ struct dy_xpbd_s {
     // Co-resident workgroup limits of the cooperative kernels, [0] contacts
     // only, [1] the variants that also solve joints.
     int coop_assign_max = 0, coop_sweep_max[2] = {0, 0}, coop_xcd_max[2] = {0, 0}, coop_flow_max[2] = {0, 0},
-        coop_lanes_max[2] = {0, 0}, coop_motor_max = 0, device_cus = 0;
+        coop_lanes_max[2] = {0, 0}, coop_pair_max = 0, coop_motor_max = 0, device_cus = 0;
+    int sweep_pair = 1;     // contact-only dataflow sweeps use sweep_flow_pair (DYPHUR_SWEEP_PAIR=0: sweep_flow)
+    int pair_threads = 512;  // its workgroup size (DYPHUR_SWEEP_PAIR_THREADS)
     // Constraint capacity (contacts + joints) of the constraint-indexed
     // buffers, and the joint capacity of jpk / jend.
     uint32_t ccap = 0, jcap = 0;
@@ -1652,12 +1829,13 @@ int env_int(const char *name, int dflt) {
     return e ? std::atoi(e) : dflt;
 }
 
-// Cooperative-launch limits; DYPHUR_{ASSIGN,SWEEP}_{BLOCKS,THREADS},
-// DYPHUR_SWEEP_MODE (2 = dataflow sweep, the default; 0 = class phases on one
-// XCD; 1 = class phases over the whole grid) and
-// DYPHUR_SOLVER_PATH
-// (1 = one launch per round / per class) override the launch shapes for
-// measurement.
+// Launch limits of the grid-synchronised kernels; DYPHUR_{ASSIGN,SWEEP}_
+// {BLOCKS,THREADS}, DYPHUR_SWEEP_MODE (2 = dataflow sweep, the default; 3 =
+// dataflow with a cursor per lane; 0 = class phases on one XCD; 1 = class
+// phases over the whole grid), DYPHUR_SWEEP_PAIR (0: contact-only dataflow
+// sweeps with one lane per record instead of a pair), DYPHUR_SWEEP_PAIR_THREADS
+// and DYPHUR_SOLVER_PATH (1 = one launch per round / per class) override the
+// launch shapes for measurement.
 int query_coop(dy_xpbd_s *x) {
     int dev = 0, coop = 0;
     DY_TRY(hipGetDevice(&dev));
@@ -1673,6 +1851,11 @@ int query_coop(dy_xpbd_s *x) {
     const int at = env_int("DYPHUR_ASSIGN_THREADS", 256), sw = env_int("DYPHUR_SWEEP_THREADS", 256);
     x->assign_threads = (at == 64 || at == 128 || at == 256 || at == 512 || at == 1024) ? at : 256;
     x->sweep_threads = (sw == 64 || sw == 128 || sw == 256 || sw == 512 || sw == 1024) ? sw : 256;
+    x->sweep_pair = env_int("DYPHUR_SWEEP_PAIR", 1) != 0 ? 1 : 0;
+    {
+        const int pt = env_int("DYPHUR_SWEEP_PAIR_THREADS", 512);
+        x->pair_threads = (pt == 128 || pt == 256 || pt == 512 || pt == 1024) ? pt : 512;
+    }
     if (!coop) return DY_OK;
     int per_cu = 0;
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(assign_lists),
@@ -1693,6 +1876,9 @@ int query_coop(dy_xpbd_s *x) {
     DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(joint_motors_xcd),
                                                         x->sweep_threads, 0));
     x->coop_motor_max = per_cu * x->device_cus;
+    DY_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(sweep_flow_pair),
+                                                        x->pair_threads, 0));
+    x->coop_pair_max = per_cu * x->device_cus;
     return DY_OK;
 }
 
@@ -1930,8 +2116,17 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
     if (x->timing) DY_TRY(hipEventRecord(x->ev[2], st));
     bool swept = iters == 0;
     if (!swept && coop && flow) {
-        uint32_t blocks = 8u * coop_blocks(x, x->sweep_blocks_override, x->sweep_threads, flow_max[J]);
-        if (blocks > static_cast<uint32_t>(flow_max[J])) blocks = static_cast<uint32_t>(flow_max[J]);
+        // contact-only: a pair of lanes per record, one workgroup of
+        // pair_threads per CU of the XCD by default (the measured best on the
+        // 32k-body frame: 32 x 512 against 16..48 x 256..1024)
+        const bool pair = x->sweep_mode == 2 && !J && x->sweep_pair && x->coop_pair_max > 0;
+        const int threads = pair ? x->pair_threads : x->sweep_threads;
+        const int cap = pair ? x->coop_pair_max : flow_max[J];
+        uint32_t blocks = 8u * coop_blocks(x, x->sweep_blocks_override > 0 ? x->sweep_blocks_override
+                                              : pair                        ? x->device_cus / 8
+                                                                            : 0,
+                                           threads, cap);
+        if (blocks > static_cast<uint32_t>(cap)) blocks = static_cast<uint32_t>(cap);
         uint32_t iv = iters;
         const uint4 *rseqv = x->rseq;
         float *lamv = x->lam;
@@ -1944,9 +2139,11 @@ int dy_xpbd_solve(dy_xpbd_t x, dy_stream_t s, const dy_contact_view *cv, const d
         if (x->sweep_mode == 3)
             fn = J ? reinterpret_cast<const void *>(sweep_flow_lanes<true>)
                    : reinterpret_cast<const void *>(sweep_flow_lanes<false>);
+        else if (pair)
+            fn = reinterpret_cast<const void *>(sweep_flow_pair);
         else
             fn = J ? reinterpret_cast<const void *>(sweep_flow<true>) : reinterpret_cast<const void *>(sweep_flow<false>);
-        swept = try_coop_launch(fn, dim3(blocks), dim3(static_cast<uint32_t>(x->sweep_threads)), args, st) == hipSuccess;
+        swept = try_coop_launch(fn, dim3(blocks), dim3(static_cast<uint32_t>(threads)), args, st) == hipSuccess;
     } else if (!swept && coop) {
         // sweep_xcd: P participants on one XCD out of a launch of P per XCD
         // (8 XCDs on gfx950); sweep_persistent: P workgroups over the chip.

@@ -197,9 +197,14 @@ def test_golden_box_hash_on_gpu(gpu):
     assert hs[0] == hs[1] == 0xCBCD209C3665C818
 
 
-@pytest.mark.parametrize("sweep_mode", ["2", "3", "0"])
+@pytest.mark.parametrize("sweep_mode", ["2", "2-single", "3", "0"])
 def test_stress_frames_match_oracle(gpu, oracle, monkeypatch, sweep_mode):
-    monkeypatch.setenv("DYPHUR_SWEEP_MODE", sweep_mode)
+    """150 frames of the stress scene against the oracle under every sweep:
+    the dataflow sweep with a pair of lanes per record (the default) and with
+    one lane per record, the per-lane-cursor form and the class phases."""
+    monkeypatch.setenv("DYPHUR_SWEEP_MODE", sweep_mode.split("-")[0])
+    if sweep_mode.endswith("-single"):
+        monkeypatch.setenv("DYPHUR_SWEEP_PAIR", "0")
     print(f"[stress 150 frames] sweep mode {sweep_mode}", flush=True)
     sc = scenes.stress_blocks()
     frames = 150
