@@ -4,6 +4,11 @@
 #include "adjoint.hpp"
 #include "joint_math.hpp"
 
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <random>
+#include <thread>
 #include <vector>
 
 using namespace dy;
@@ -452,3 +457,70 @@ void ha_sim_dir_f(int n, double *st, double *st_dot, const double *im, const dou
 }
 
 }  // extern "C"
+
+// project_contact_pair against project_contact: the two lanes run as two
+// threads that meet at every exchange (each writes its value, both read,
+// then both leave before either writes again), over n random contacts
+// (seeded; every third with a static second body, every fifth with a lambda
+// already accumulated).  Returns the number of contacts whose lambda or any
+// body word differs in its bits from project_contact's.
+namespace {
+struct LaneRendezvous {
+    std::mutex m;
+    std::condition_variable cv;
+    float val[2] = {0.f, 0.f};
+    int in1 = 0, in2 = 0;
+    long gen1 = 0, gen2 = 0;
+    float swap(uint32_t side, float v) {
+        std::unique_lock<std::mutex> lk(m);
+        val[side] = v;
+        long g = gen1;
+        if (++in1 == 2) { in1 = 0; ++gen1; cv.notify_all(); } else { cv.wait(lk, [&] { return gen1 != g; }); }
+        const float r = val[side ^ 1u];
+        g = gen2;
+        if (++in2 == 2) { in2 = 0; ++gen2; cv.notify_all(); } else { cv.wait(lk, [&] { return gen2 != g; }); }
+        return r;
+    }
+};
+uint32_t f_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+}  // namespace
+
+extern "C" int hm_pair_projection_mismatches(int n, uint32_t seed) {
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(-1.f, 1.f);
+    auto body = [&](BodyState &s, bool dyn) {
+        s.px = U(rng); s.py = U(rng); s.pz = U(rng);
+        const float q[4] = {U(rng), U(rng), U(rng), U(rng)};
+        const float qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        s.qw = q[0] / qn; s.qx = q[1] / qn; s.qy = q[2] / qn; s.qz = q[3] / qn;
+        s.vx = U(rng); s.vy = U(rng); s.vz = U(rng); s.wx = U(rng); s.wy = U(rng); s.wz = U(rng);
+        s.im = dyn ? 0.5f + 0.5f * U(rng) : 0.f;
+        s.ixx = dyn ? 1.f + 0.5f * U(rng) : 0.f; s.iyy = dyn ? 1.f + 0.5f * U(rng) : 0.f;
+        s.izz = dyn ? 1.f + 0.5f * U(rng) : 0.f; s.ixy = dyn ? 0.1f * U(rng) : 0.f;
+        s.ixz = dyn ? 0.1f * U(rng) : 0.f; s.iyz = dyn ? 0.1f * U(rng) : 0.f;
+    };
+    int bad = 0;
+    for (int t = 0; t < n; ++t) {
+        BodyState A{}, B{};
+        body(A, true);
+        body(B, t % 3 != 0);
+        const float nv[3] = {U(rng), U(rng), U(rng)};
+        const float nl = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const ContactIn c{U(rng), U(rng), U(rng), nv[0] / nl, nv[1] / nl, nv[2] / nl, 0.05f + 0.05f * U(rng)};
+        const float mu = 0.5f, dt = 1.f / 60.f, lam0 = (t % 5 == 0) ? 0.01f : 0.f;
+        BodyState A1 = A, B1 = B;
+        float l1 = lam0;
+        project_contact(c, A1, B1, mu, l1, dt);
+        BodyState A2 = A, B2 = B;
+        float la = lam0, lb = lam0;
+        LaneRendezvous X;
+        std::thread other([&] { project_contact_pair(c, B2, 1u, mu, lb, dt, [&](float v) { return X.swap(1u, v); }); });
+        project_contact_pair(c, A2, 0u, mu, la, dt, [&](float v) { return X.swap(0u, v); });
+        other.join();
+        bool ok = f_bits(la) == f_bits(l1) && f_bits(lb) == f_bits(l1);
+        const float *a1 = &A1.px, *a2 = &A2.px, *b1 = &B1.px, *b2 = &B2.px;
+        for (int k = 0; k < 13; ++k) ok = ok && f_bits(a1[k]) == f_bits(a2[k]) && f_bits(b1[k]) == f_bits(b2[k]);
+        bad += ok ? 0 : 1;
+    }
+    return bad;
+}

@@ -1328,104 +1328,6 @@ __global__ __launch_bounds__(1024) void sweep_flow(uint32_t n_iters, CRec rec, c
     }
 }
 
-// One contact projection by a pair of lanes (project_contact, dy_math.hpp):
-// the lane of side 0 holds body A, the lane of side 1 body B, each computes
-// its own body's terms and updates its own body.  The sums project_contact
-// forms left to right (w, the relative velocities, wt) are formed on both
-// lanes in the same order from exchanged terms, so every value is bit-equal
-// to project_contact's.  Every branch around an exchange depends on shared
-// values only, so the two lanes always take it together.
-__device__ __forceinline__ float pair_xchg(float v) { return __shfl_xor(v, 1, 64); }
-
-__device__ __forceinline__ void project_contact_pair(const ContactIn &c, BodyState &S, uint32_t side, float mu,
-                                                     float &lam, float dt) {
-    if (c.depth <= 0.f) return;
-    const float rem = c.depth - lam;
-    if (rem <= 1e-6f) return;
-    const float nx = c.nx, ny = c.ny, nz = c.nz;
-    const float ws = S.im, wo = pair_xchg(ws);
-    const float wa = side ? wo : ws, wb = side ? ws : wo;
-    const float rx = c.px - S.px, ry = c.py - S.py, rz = c.pz - S.pz;
-    const float cx = ry * nz - rz * ny, cy = rz * nx - rx * nz, cz = rx * ny - ry * nx;
-    float kx, ky, kz;
-    inv_inertia_world(S, cx, cy, cz, kx, ky, kz);
-    const float ang_s = cx * kx + cy * ky + cz * kz, ang_o = pair_xchg(ang_s);
-    const float wa_ang = side ? ang_o : ang_s, wb_ang = side ? ang_s : ang_o;
-    const float w = wa + wb + wa_ang + wb_ang;
-    if (w < 1e-10f) return;
-    const float dl = rem / w;
-    lam += rem;
-    if (side == 0u) {
-        if (wa != 0.f) { S.px += wa * dl * nx; S.py += wa * dl * ny; S.pz += wa * dl * nz; }
-        rotate_by(S, dl * kx, dl * ky, dl * kz);
-    } else {
-        if (wb != 0.f) { S.px -= wb * dl * nx; S.py -= wb * dl * ny; S.pz -= wb * dl * nz; }
-        rotate_by(S, -dl * kx, -dl * ky, -dl * kz);
-    }
-    {
-        const float vsx = S.vx + S.wy * rz - S.wz * ry;
-        const float vsy = S.vy + S.wz * rx - S.wx * rz;
-        const float vsz = S.vz + S.wx * ry - S.wy * rx;
-        const float vox = pair_xchg(vsx), voy = pair_xchg(vsy), voz = pair_xchg(vsz);
-        const float vax = side ? vox : vsx, vay = side ? voy : vsy, vaz = side ? voz : vsz;
-        const float vbx = side ? vsx : vox, vby = side ? vsy : voy, vbz = side ? vsz : voz;
-        const float qx = vax - vbx, qy = vay - vby, qz = vaz - vbz;
-        const float vn = qx * nx + qy * ny + qz * nz;
-        if (vn < 0.f) {
-            const float jn = -vn / w;
-            float gx, gy, gz;
-            inv_inertia_world(S, cx, cy, cz, gx, gy, gz);
-            if (ws != 0.f) {
-                if (side == 0u) {
-                    S.vx += wa * jn * nx; S.vy += wa * jn * ny; S.vz += wa * jn * nz;
-                    S.wx += jn * gx; S.wy += jn * gy; S.wz += jn * gz;
-                } else {
-                    S.vx -= wb * jn * nx; S.vy -= wb * jn * ny; S.vz -= wb * jn * nz;
-                    S.wx -= jn * gx; S.wy -= jn * gy; S.wz -= jn * gz;
-                }
-            }
-        }
-    }
-    if (mu > 0.f) {
-        const float vsx = S.vx + S.wy * rz - S.wz * ry;
-        const float vsy = S.vy + S.wz * rx - S.wx * rz;
-        const float vsz = S.vz + S.wx * ry - S.wy * rx;
-        const float vox = pair_xchg(vsx), voy = pair_xchg(vsy), voz = pair_xchg(vsz);
-        const float vax = side ? vox : vsx, vay = side ? voy : vsy, vaz = side ? voz : vsz;
-        const float vbx = side ? vsx : vox, vby = side ? vsy : voy, vbz = side ? vsz : voz;
-        const float ux = vax - vbx, uy = vay - vby, uz = vaz - vbz;
-        const float un = ux * nx + uy * ny + uz * nz;
-        const float tx0 = ux - un * nx, ty0 = uy - un * ny, tz0 = uz - un * nz;
-        const float t2 = tx0 * tx0 + ty0 * ty0 + tz0 * tz0;
-        if (t2 > 1e-12f) {
-            const float tl = sqrtf(t2);
-            const float tx = tx0 / tl, ty = ty0 / tl, tz = tz0 / tl;
-            const float dx = ry * tz - rz * ty, dy = rz * tx - rx * tz, dz = rx * ty - ry * tx;
-            float fx, fy, fz;
-            inv_inertia_world(S, dx, dy, dz, fx, fy, fz);
-            const float p0 = dx * fx, p1 = dy * fy, p2 = dz * fz;
-            const float o0 = pair_xchg(p0), o1 = pair_xchg(p1), o2 = pair_xchg(p2);
-            const float a0 = side ? o0 : p0, a1 = side ? o1 : p1, a2 = side ? o2 : p2;
-            const float b0 = side ? p0 : o0, b1 = side ? p1 : o1, b2 = side ? p2 : o2;
-            const float wt = wa + wb + a0 + a1 + a2 + b0 + b1 + b2;
-            if (wt > 1e-10f) {
-                const float jcap = lam / (dt * w);
-                float jt = tl / wt;
-                jt = fminf(jt, mu * jcap);
-                if (ws != 0.f) {
-                    if (side == 0u) {
-                        S.vx -= wa * jt * tx; S.vy -= wa * jt * ty; S.vz -= wa * jt * tz;
-                        S.wx -= jt * fx; S.wy -= jt * fy; S.wz -= jt * fz;
-                    } else {
-                        S.vx += wb * jt * tx; S.vy += wb * jt * ty; S.vz += wb * jt * tz;
-                        S.wx += jt * fx; S.wy += jt * fy; S.wz += jt * fz;
-                    }
-                }
-            }
-        }
-    }
-}
-
 // Contact-only dataflow sweep with a pair of lanes per record (lanes 2k and
 // 2k + 1 of a wave take record i0 + k, one body each, project_contact_pair):
 // each lane polls, loads, renormalises, stores and publishes its own body,
@@ -1479,7 +1381,7 @@ __global__ __launch_bounds__(1024) void sweep_flow_pair(uint32_t n_iters, CRec r
                     BodyState S = load_aos_agent(dyn, sta, x, rs);
                     if (S.im != 0.f) catch_up(S, rs, it);
                     const ContactIn ci{p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
-                    project_contact_pair(ci, S, side, mu, l, dt);
+                    project_contact_pair(ci, S, side, mu, l, dt, [](float v) { return __shfl_xor(v, 1, 64); });
                     if (side == 0u) lam[i] = l;
                     if (S.im != 0.f) store_aos(dyn, x, S, it);
                     if (side == 0u && it == 0 && contact_live(p1.z, l)) atomicOr(&meta[kMLive], 1u);

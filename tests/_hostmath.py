@@ -34,6 +34,8 @@ def lib():
         L.hm_pair_contact.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 7
         L.hm_pair_contact_feat.restype = C.c_int
         L.hm_pair_contact_feat.argtypes = [C.c_uint32, C.c_uint32] + [vp] * 8
+        L.hm_pair_projection_mismatches.restype = C.c_int
+        L.hm_pair_projection_mismatches.argtypes = [C.c_int, C.c_uint32]
         L.hm_project_contact.restype = C.c_int
         L.hm_project_contact.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float]
         L.hm_integrate_body.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]

@@ -197,3 +197,12 @@ def test_integrate_body(hm, oracle):
             ref = np.array([ob.arr[k][i] for k in ("pos_x", "pos_y", "pos_z", "rot_w", "rot_x", "rot_y",
                                                    "rot_z", "vel_x", "vel_y", "vel_z")], F32)
             assert kin[:10].view(np.uint32).tolist() == ref.view(np.uint32).tolist()
+
+
+def test_pair_projection_matches_project_contact(hm):
+    """The sweep's two-lane projection (project_contact_pair: each lane one
+    body, the shared sums formed in the same order from exchanged terms)
+    gives project_contact's lambda and both bodies bit for bit, on 3 000
+    random contacts (static second bodies and accumulated lambdas among
+    them); the two lanes run as two host threads meeting at every exchange."""
+    assert hm.hm_pair_projection_mismatches(3000, 20260007) == 0
