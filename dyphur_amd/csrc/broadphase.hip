@@ -228,6 +228,7 @@ __global__ __launch_bounds__(kMsThreads) void morton_sort(dy_body_view bv, dy_aa
 // the 64-bit keys (code << 32 | body).
 constexpr uint32_t kSgThreads = 256;
 constexpr uint32_t kSortXcc = 0;
+constexpr uint32_t kSortMaxBlocks = 64;  // default cap on participants (more pay more per barrier)
 constexpr uint32_t kSgWaves = kSgThreads / 64;
 constexpr uint32_t kSgItems = 4;
 constexpr uint32_t kSgTile = kSgThreads * kSgItems;
@@ -250,9 +251,9 @@ __global__ __launch_bounds__(kSgThreads) void morton_sort_grid(dy_body_view bv, 
     if (!xcd_checkin(kSortXcc, reg, err, kErrSortBarrier, bidx, nblk)) return;
     const uint32_t t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t per = (n_tiles + nblk - 1u) / nblk;
-    const uint32_t t0 = bidx * per < n_tiles ? bidx * per : n_tiles;
-    const uint32_t t1 = t0 + per < n_tiles ? t0 + per : n_tiles;
+    // balanced contiguous runs: tile counts differ by at most one
+    const uint32_t t0 = static_cast<uint32_t>(static_cast<uint64_t>(bidx) * n_tiles / nblk);
+    const uint32_t t1 = static_cast<uint32_t>(static_cast<uint64_t>(bidx + 1u) * n_tiles / nblk);
     for (uint32_t tile = t0; tile < t1; ++tile) {
         lh[t] = 0;
         __syncthreads();
@@ -991,7 +992,9 @@ struct dy_broadphase_s {
     uint32_t *sort_hist = nullptr;  // morton_sort_grid's digit counts, 256 per tile
     dy_contact_pair *pairs = nullptr;
     uint32_t *count = nullptr;  // = counters + 3
-    uint32_t sort_blocks = 0;   // co-resident workgroups for morton_sort_grid (0: single-workgroup sort)
+    uint32_t sort_blocks = 0;   // participants of morton_sort_grid per launch (0: single-workgroup sort)
+    uint32_t sort_cap = 0;      // the most one XCD holds at once
+    bool sort_fixed = false;    // DYPHUR_SORT_BLOCKS given: sort_blocks as set, not one per tile
     TPair *tp = nullptr;        // walk records of the internal nodes
     // traverse_count / emit_pairs run 64-thread workgroups (kWalkThreads):
     // the walks spread over all CUs (256-thread ones left half of them idle at
@@ -1069,9 +1072,10 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
     if (e == hipSuccess) b->count = b->counters + 3;
     if (e == hipSuccess) e = hipMemsetAsync(b->counters, 0, 8 * sizeof(uint32_t), hs(s));
     if (e == hipSuccess) {
-        // Participating workgroups of morton_sort_grid (on one XCD): 32 by
-        // default (DYPHUR_SORT_BLOCKS overrides), and no more than one XCD
-        // holds at once for a cooperative launch of eight times as many.
+        // Participating workgroups of morton_sort_grid (on one XCD): by
+        // default one per 1 024-key tile up to kSortMaxBlocks (DYPHUR_SORT_BLOCKS
+        // fixes the count), and no more than one XCD holds at once for a
+        // launch of eight times as many.
         int dev = 0, coop = 0, per_cu = 0, cus = 0;
         e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
@@ -1080,9 +1084,13 @@ int dy_broadphase_create(dy_stream_t s, uint32_t max_bodies, uint32_t max_pairs,
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(morton_sort_grid),
                                                             kSgThreads, 0);
         if (e == hipSuccess && coop && per_cu > 0) {
-            int want = 32;
-            if (const char *env = std::getenv("DYPHUR_SORT_BLOCKS")) want = std::atoi(env);
+            int want = static_cast<int>(kSortMaxBlocks);
+            if (const char *env = std::getenv("DYPHUR_SORT_BLOCKS")) {
+                want = std::atoi(env);
+                b->sort_fixed = true;
+            }
             const int cap = per_cu * cus / 8;
+            b->sort_cap = static_cast<uint32_t>(cap > 0 ? cap : 0);
             b->sort_blocks = static_cast<uint32_t>(want < 0 ? 0 : (want > cap ? cap : want));
         }
     }
@@ -1131,7 +1139,10 @@ int dy_broadphase_build_and_query(dy_broadphase_t bp, dy_stream_t s, const dy_bo
     if (bp->sort_blocks > 0 && n_tiles > 4) {
         // sort_blocks participants on one XCD out of a launch with as many on
         // every XCD (8 on gfx950)
-        const uint32_t blocks = 8u * bp->sort_blocks;
+        // one participant per tile (each sorts one tile per pass), fewer when
+        // the count is fixed or capped
+        const uint32_t per_xcd = bp->sort_fixed || n_tiles > bp->sort_blocks ? bp->sort_blocks : n_tiles;
+        const uint32_t blocks = 8u * per_xcd;
         dy_body_view bv = *bodies;
         dy_aabb bd = *bounds;
         uint32_t nv = n, ntv = n_tiles;
